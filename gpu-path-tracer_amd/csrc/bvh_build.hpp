@@ -1,0 +1,37 @@
+// Host BVH builder of the core: binned-SAH BVH2, collapsed into 128-byte BVH4
+// nodes.  Replaces HostBvh + DeviceBvh::BuildFlattenSync
+// (Core/Source/Graphics/Cuda/Bvh/HostBvh.cpp:51-100, DeviceBvh.cpp:7-52),
+// whose object-median BVH2 with 40-byte nodes is redesigned here for
+// 128-byte cache lines and four-wide box tests.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "pt_device.hpp"
+
+namespace pt {
+
+struct Bvh4Build {
+    std::vector<Bvh4Node> nodes;      // nodes[0] is the root
+    std::vector<uint32_t> order;      // order[k] = original prim of BVH slot k
+    uint32_t depth = 0;               // BVH4 levels below the root (root = 1)
+    uint32_t leaf_count = 0;
+    uint32_t max_leaf = 0;
+};
+
+struct BvhBuildOptions {
+    uint32_t max_leaf_size = 8;       // <= kMaxLeafSize
+    uint32_t bins = 16;
+    float traversal_cost = 1.0f;
+    float intersection_cost = 1.0f;
+    uint32_t threads = 0;             // 0 = hardware concurrency
+};
+
+// positions: [T][3][3] floats.  Throws std::runtime_error on bad input.
+Bvh4Build build_bvh4(const float* positions, uint32_t triangle_count, const BvhBuildOptions& options);
+
+// Padding that the reference adds around every node box (HostBvh.cpp:37-48).
+void pad_bounds(float lo[3], float hi[3]);
+
+}  // namespace pt

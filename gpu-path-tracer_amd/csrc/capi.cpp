@@ -1,0 +1,199 @@
+// extern "C" entry points of include/pt_core.h: argument checks, exception
+// barrier, and forwarding to pt::Renderer.
+#include <cstring>
+#include <new>
+
+#include "renderer.hpp"
+
+struct pt_renderer {
+    pt::Renderer impl;
+};
+
+namespace {
+template <class F>
+pt_status guarded(pt_renderer* r, F&& f) {
+    if (!r) return PT_ERROR_INVALID_ARGUMENT;
+    try {
+        return f(r->impl);
+    } catch (const std::bad_alloc&) {
+        return r->impl.fail(PT_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception& e) {
+        return r->impl.fail(PT_ERROR_INVALID_ARGUMENT, e.what());
+    }
+}
+}  // namespace
+
+extern "C" {
+
+const char* pt_version(void) { return "pt-core 0.1 (gfx950 wavefront)"; }
+
+pt_status pt_renderer_create(pt_renderer** out) {
+    if (!out) return PT_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    pt_renderer* r = new (std::nothrow) pt_renderer();
+    if (!r) return PT_ERROR_OUT_OF_MEMORY;
+    const pt_status s = r->impl.init();
+    if (s != PT_OK) {
+        delete r;
+        return s;
+    }
+    *out = r;
+    return PT_OK;
+}
+
+void pt_renderer_destroy(pt_renderer* r) { delete r; }
+
+const char* pt_get_error_message(const pt_renderer* r) { return r ? r->impl.error_message() : "null renderer"; }
+
+pt_status pt_initialize_rendering_buffers(pt_renderer* r, uint32_t width, uint32_t height) {
+    return guarded(r, [&](pt::Renderer& x) { return x.InitializeRenderingBuffers(width, height); });
+}
+
+pt_status pt_initialize_scene_buffers(pt_renderer* r, const pt_scene_desc* scene) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!scene) return x.fail(PT_ERROR_INVALID_ARGUMENT, "scene is NULL");
+        return x.InitializeSceneBuffers(*scene);
+    });
+}
+
+pt_status pt_clear(pt_renderer* r, uint8_t cr, uint8_t cg, uint8_t cb, uint8_t ca) {
+    return guarded(r, [&](pt::Renderer& x) { return x.Clear(cr, cg, cb, ca); });
+}
+
+pt_status pt_start_simulation(pt_renderer* r, const pt_render_settings* settings, const pt_camera* camera,
+                              const pt_motion_state* motion_states, uint32_t motion_state_count,
+                              const char* output_folder) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!settings || !camera) return x.fail(PT_ERROR_INVALID_ARGUMENT, "settings and camera must not be NULL");
+        return x.StartSimulation(*settings, *camera, motion_states, motion_state_count, output_folder);
+    });
+}
+
+pt_status pt_resume_simulation(pt_renderer* r, uint32_t start_frame) {
+    return guarded(r, [&](pt::Renderer& x) { return x.ResumeSimulation(start_frame); });
+}
+
+pt_status pt_stop_rendering(pt_renderer* r) {
+    return guarded(r, [&](pt::Renderer& x) { return x.StopRendering(); });
+}
+
+int pt_is_rendering(const pt_renderer* r) { return r && r->impl.IsRendering() ? 1 : 0; }
+
+pt_status pt_get_progress(const pt_renderer* r, pt_progress* out) {
+    if (!r || !out) return PT_ERROR_INVALID_ARGUMENT;
+    r->impl.GetProgress(*out);
+    return PT_OK;
+}
+
+pt_status pt_frame_copy(pt_renderer* r, uint32_t* rgba, size_t pixel_count) {
+    return guarded(r, [&](pt::Renderer& x) { return x.CopyFrame(rgba, pixel_count); });
+}
+
+pt_status pt_frame_copy_hdr(pt_renderer* r, float* rgba, size_t pixel_count) {
+    return guarded(r, [&](pt::Renderer& x) { return x.CopyFrameHdr(rgba, pixel_count); });
+}
+
+size_t pt_peek_last_error(const pt_renderer* r, char* buf, size_t capacity) {
+    return r ? r->impl.PeekLastError(buf, capacity) : 0;
+}
+
+size_t pt_consume_last_error(pt_renderer* r, char* buf, size_t capacity) {
+    return r ? r->impl.ConsumeLastError(buf, capacity) : 0;
+}
+
+pt_status pt_render_frame(pt_renderer* r, const pt_render_settings* settings, const pt_camera* camera,
+                          const pt_motion_state* motion_state, uint32_t frame_flags, pt_frame_stats* stats) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!settings || !camera || !motion_state) return x.fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
+        return x.RenderFrameSync(*settings, *camera, *motion_state, frame_flags, stats);
+    });
+}
+
+pt_status pt_trace_closest(pt_renderer* r, const pt_ray* rays, uint32_t count, pt_hit* hits) {
+    return guarded(r, [&](pt::Renderer& x) { return x.TraceBatch(rays, count, false, false, hits, nullptr, nullptr, nullptr); });
+}
+
+pt_status pt_trace_any(pt_renderer* r, const pt_ray* rays, uint32_t count, uint32_t* occluded) {
+    return guarded(r, [&](pt::Renderer& x) { return x.TraceBatch(rays, count, true, false, occluded, nullptr, nullptr, nullptr); });
+}
+
+pt_status pt_trace_closest_device(pt_renderer* r, const pt_ray* rays_device, uint32_t count, pt_hit* hits_device,
+                                  void* stream) {
+    return guarded(r, [&](pt::Renderer& x) {
+        return x.TraceBatch(rays_device, count, false, true, hits_device, nullptr, nullptr, (hipStream_t)stream);
+    });
+}
+
+pt_status pt_trace_any_device(pt_renderer* r, const pt_ray* rays_device, uint32_t count, uint32_t* occluded_device,
+                              void* stream) {
+    return guarded(r, [&](pt::Renderer& x) {
+        return x.TraceBatch(rays_device, count, true, true, occluded_device, nullptr, nullptr, (hipStream_t)stream);
+    });
+}
+
+pt_status pt_trace_closest_counts(pt_renderer* r, const pt_ray* rays, uint32_t count, pt_hit* hits,
+                                  uint32_t* nodes_visited, uint32_t* tris_tested) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!nodes_visited || !tris_tested) return x.fail(PT_ERROR_INVALID_ARGUMENT, "count arrays must not be NULL");
+        return x.TraceBatch(rays, count, false, false, hits, nodes_visited, tris_tested, nullptr);
+    });
+}
+
+pt_status pt_classify_hits(pt_renderer* r, const pt_ray* rays, uint32_t count, uint32_t* queue_of_ray,
+                           uint32_t* queue_counts) {
+    return guarded(r, [&](pt::Renderer& x) { return x.Classify(rays, count, queue_of_ray, queue_counts); });
+}
+
+pt_status pt_alias_sample(pt_renderer* r, const float* u, uint32_t count, uint32_t* index, float* pmf) {
+    return guarded(r, [&](pt::Renderer& x) { return x.AliasSample(u, count, index, pmf); });
+}
+
+pt_status pt_get_alias_table(pt_renderer* r, pt_alias_bin* bins, uint32_t capacity, uint32_t* count) {
+    return guarded(r, [&](pt::Renderer& x) { return x.GetAliasTable(bins, capacity, count); });
+}
+
+pt_status pt_get_bvh_info(pt_renderer* r, pt_bvh_info* info) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!info) return x.fail(PT_ERROR_INVALID_ARGUMENT, "info is NULL");
+        return x.GetBvhInfo(*info);
+    });
+}
+
+pt_status pt_get_bvh4(pt_renderer* r, void* nodes, uint32_t node_capacity, uint32_t* prim_order, uint32_t prim_capacity) {
+    return guarded(r, [&](pt::Renderer& x) { return x.GetBvh4(nodes, node_capacity, prim_order, prim_capacity); });
+}
+
+pt_status pt_get_tuning(pt_renderer* r, pt_tuning* tuning) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!tuning) return x.fail(PT_ERROR_INVALID_ARGUMENT, "tuning is NULL");
+        return x.GetTuning(*tuning);
+    });
+}
+
+pt_status pt_set_tuning(pt_renderer* r, const pt_tuning* tuning) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!tuning) return x.fail(PT_ERROR_INVALID_ARGUMENT, "tuning is NULL");
+        return x.SetTuning(*tuning);
+    });
+}
+
+pt_status pt_build_bvh4(const float* positions, uint32_t triangle_count, uint32_t max_leaf_size, void* nodes,
+                        uint32_t node_capacity, uint32_t* node_count, uint32_t* prim_order, uint32_t* depth) {
+    if ((triangle_count > 0 && !positions) || !node_count) return PT_ERROR_INVALID_ARGUMENT;
+    try {
+        pt::BvhBuildOptions o;
+        if (max_leaf_size) o.max_leaf_size = max_leaf_size;
+        const pt::Bvh4Build b = pt::build_bvh4(positions, triangle_count, o);
+        *node_count = (uint32_t)b.nodes.size();
+        if (depth) *depth = b.depth;
+        if (!nodes) return PT_OK;
+        if (node_capacity < b.nodes.size()) return PT_ERROR_INVALID_ARGUMENT;
+        std::memcpy(nodes, b.nodes.data(), b.nodes.size() * sizeof(pt::Bvh4Node));
+        if (prim_order) std::memcpy(prim_order, b.order.data(), b.order.size() * sizeof(uint32_t));
+        return PT_OK;
+    } catch (const std::exception&) {
+        return PT_ERROR_INVALID_ARGUMENT;
+    }
+}
+
+}  // extern "C"

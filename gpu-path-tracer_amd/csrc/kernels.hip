@@ -1,0 +1,1138 @@
+// Wavefront path-tracing kernels for gfx950 (CDNA4, wave64).
+//
+// Stages of one bounce (the loop of Renderer::RenderFrame,
+// Core/Source/Graphics/Cuda/PathTracing/Renderer.cpp:517-623):
+//   closest  - persistent BVH4 traversal with per-lane ray refill and an LDS
+//              stack; pushes each path slot into its material queue or regen
+//   shade    - one launch over the diffuse | ggx | mirror | emissive queues
+//   shadow   - persistent any-hit traversal of the NEE shadow rays
+//   generate - camera rays for recycled slots
+//   prepare  - one wave: queue bookkeeping, next ray count, ray statistics
+// All queue pushes are wave-aggregated: a ballot per target queue, one atomic
+// per wave and queue, lane offsets from mbcnt.
+#include <hip/hip_runtime.h>
+
+#include "launch.hpp"
+
+namespace pt {
+
+// ================================================================ wave helpers
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t mbcnt(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+
+// Every active lane calls; lanes with `pred` receive consecutive positions
+// reserved with one atomicAdd on `counter` for the whole wave.
+__device__ __forceinline__ uint32_t wave_reserve(unsigned int* counter, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    uint32_t pos = 0;
+    if (mask) {
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+        uint32_t base = 0;
+        if (lane_id() == leader) base = atomicAdd(counter, (unsigned int)__popcll(mask));
+        base = readlane_u32(base, leader);
+        pos = base + mbcnt(mask);
+    }
+    return pos;
+}
+
+// Multi-queue variant: `q` in [0, nq) or -1.  One ballot + atomic per distinct
+// queue present in the wave.
+__device__ __forceinline__ uint32_t wave_reserve_multi(unsigned int* counters, int q) {
+    uint32_t pos = 0;
+    uint64_t pending = __ballot(q >= 0);
+    while (pending) {
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+        const int lq = (int)readlane_u32((uint32_t)q, leader);
+        const uint64_t m = __ballot(q == lq);
+        uint32_t base = 0;
+        if (lane_id() == leader) base = atomicAdd(counters + lq, (unsigned int)__popcll(m));
+        base = readlane_u32(base, leader);
+        if (q == lq) pos = base + mbcnt(m);
+        pending &= ~m;
+    }
+    return pos;
+}
+
+// ================================================================ intersection
+
+// Moller-Trumbore with the reference's operation order and tests
+// (Intersect.cu:25-61).  Returns true if t lies in [tmin, +inf); the caller
+// applies its own upper bound.
+__device__ __forceinline__ bool intersect_triangle(const TriRecord* tri, V3 o, V3 d, float tmin,
+                                                   float& t, float& u, float& v, uint32_t& prim,
+                                                   uint32_t& queue) {
+    const float4* p = reinterpret_cast<const float4*>(tri);
+    const float4 a = p[0], b = p[1], c = p[2];
+    const V3 v0 = xyz(a), e1 = xyz(b), e2 = xyz(c);
+    prim = __float_as_uint(a.w);
+    queue = __float_as_uint(b.w);
+    const V3 pv = cross(d, e2);
+    const float det = dot(e1, pv);
+    if (fabsf(det) < 1e-7f) return false;
+    const float inv_det = 1.0f / det;
+    const V3 s = o - v0;
+    u = dot(s, pv) * inv_det;
+    if (u < 0.0f || u > 1.0f) return false;
+    const V3 qv = cross(s, e1);
+    v = dot(d, qv) * inv_det;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = dot(e2, qv) * inv_det;
+    return t >= tmin;
+}
+
+struct RayFrame {
+    V3 o, d;
+    V3 rd;      // 1/d with zero components replaced by +-1e-20
+    V3 ord;     // o * rd
+    float tmin, tmax;
+};
+
+__device__ __forceinline__ float safe_rcp(float x) {
+    const float ax = fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x;
+    return 1.0f / ax;
+}
+
+__device__ __forceinline__ void setup_ray(RayFrame& r, V3 o, V3 d, float tmin, float tmax) {
+    r.o = o;
+    r.d = d;
+    r.rd = v3(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+    r.ord = o * r.rd;
+    r.tmin = tmin;
+    r.tmax = tmax;
+}
+
+// Entry distance of the four children of a node, +inf for a miss.
+__device__ __forceinline__ void box4(const Bvh4Node* node, const RayFrame& r, float tmax, float t0[4]) {
+    const float4* p = reinterpret_cast<const float4*>(node);
+    const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
+    const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const float ax = fmaf(lxa[i], r.rd.x, -r.ord.x), bx = fmaf(hxa[i], r.rd.x, -r.ord.x);
+        const float ay = fmaf(lya[i], r.rd.y, -r.ord.y), by = fmaf(hya[i], r.rd.y, -r.ord.y);
+        const float az = fmaf(lza[i], r.rd.z, -r.ord.z), bz = fmaf(hza[i], r.rd.z, -r.ord.z);
+        const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), r.tmin));
+        const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
+        t0[i] = tn <= tf ? tn : __builtin_inff();
+    }
+}
+
+__device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
+    if (tb < ta) {
+        const float t = ta;
+        ta = tb;
+        tb = t;
+        const uint32_t x = ra;
+        ra = rb;
+        rb = x;
+    }
+}
+
+// ================================================================ persistent traversal
+//
+// One lane = one ray at a time.  When at least `refill` lanes of a wave are
+// idle, the wave claims that many rays with one atomic and the idle lanes start
+// on them, so the SIMD stays busy while long rays finish (dynamic fetch).
+// The stack lives in LDS as [entry][lane] (bank-conflict free) and spills to a
+// global area past `LDS` entries.  Each entry is (node ref, entry distance) so
+// popped subtrees that lie beyond the current hit are skipped without a fetch.
+
+template <int BLOCK, int LDS, bool ANY, bool COUNT, class Src>
+__device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev& sc, const TraceConfig& cfg) {
+    __shared__ uint2 stack[LDS][BLOCK];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t gtid = blockIdx.x * BLOCK + tid;
+    const uint32_t stride = gridDim.x * BLOCK;
+    const uint32_t cap = LDS + cfg.spill_depth;
+    const uint32_t n = src.count();
+    unsigned int* work = src.work();
+
+    bool active = false;
+    bool exhausted = false;
+    uint32_t item = 0, ref = 0, sp = 0;
+    RayFrame r;
+    float best_t = 0.0f, best_u = 0.0f, best_v = 0.0f;
+    uint32_t best_prim = kInvalid, best_queue = 0;
+    uint32_t nodes = 0, tris = 0;
+
+    while (true) {
+        if (!exhausted) {
+            const uint64_t idle = __ballot(!active);
+            const uint32_t nidle = (uint32_t)__popcll(idle);
+            if (nidle >= cfg.refill || nidle == 64u) {
+                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
+                uint32_t base = 0;
+                if (lane_id() == leader) base = atomicAdd(work, nidle);
+                base = readlane_u32(base, leader);
+                if (base + nidle >= n) exhausted = true;
+                if (!active) {
+                    const uint32_t my = base + mbcnt(idle);
+                    if (my < n) {
+                        V3 o, d;
+                        float tmin, tmax;
+                        item = src.load(my, o, d, tmin, tmax);
+                        setup_ray(r, o, d, tmin, tmax);
+                        best_prim = kInvalid;
+                        best_queue = 0;
+                        best_t = tmax;
+                        best_u = best_v = 0.0f;
+                        nodes = tris = 0;
+                        ref = 0;   // root node
+                        sp = 0;
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (!active) continue;
+
+        bool pop = false;
+        bool done = false;
+        if (!ref_is_leaf(ref)) {
+            float t0[4];
+            const Bvh4Node* node = sc.nodes + ref;
+            box4(node, r, r.tmax, t0);
+            const uint4 cref = reinterpret_cast<const uint4*>(node)[6];
+            uint32_t c[4] = {cref.x, cref.y, cref.z, cref.w};
+            if (COUNT) nodes++;
+            if (!ANY) {
+                // nearest child first: sort the four (distance, ref) pairs
+                cswap(t0[0], c[0], t0[1], c[1]);
+                cswap(t0[2], c[2], t0[3], c[3]);
+                cswap(t0[0], c[0], t0[2], c[2]);
+                cswap(t0[1], c[1], t0[3], c[3]);
+                cswap(t0[1], c[1], t0[2], c[2]);
+            }
+            uint32_t next = kInvalid;
+            float tnext = 0.0f;
+#pragma unroll
+            for (int i = 3; i >= 0; i--) {
+                if (t0[i] != __builtin_inff() && c[i] != kInvalid) {
+                    if (next != kInvalid) {
+                        // the previous candidate is farther: it goes on the stack
+                        if (sp < cap) {
+                            const uint2 e = make_uint2(next, __float_as_uint(tnext));
+                            if (sp < (uint32_t)LDS) stack[sp][tid] = e;
+                            else cfg.spill[(size_t)(sp - LDS) * stride + gtid] = e;
+                            sp++;
+                        } else {
+                            atomicOr(cfg.overflow, 1u);
+                        }
+                    }
+                    next = c[i];
+                    tnext = t0[i];
+                }
+            }
+            if (next == kInvalid) pop = true;
+            else ref = next;
+        } else {
+            const uint32_t first = leaf_first(ref);
+            const uint32_t cnt = leaf_count(ref);
+            if (COUNT) tris += cnt;
+            for (uint32_t k = 0; k < cnt; k++) {
+                float t, u, v;
+                uint32_t prim, queue;
+                if (intersect_triangle(sc.tris + first + k, r.o, r.d, r.tmin, t, u, v, prim, queue)) {
+                    if (ANY) {
+                        if (t < r.tmax) {
+                            done = true;
+                            best_prim = prim;
+                            break;
+                        }
+                    } else if (t < r.tmax || (t == r.tmax && best_prim != kInvalid && prim < best_prim)) {
+                        r.tmax = t;
+                        best_t = t;
+                        best_u = u;
+                        best_v = v;
+                        best_prim = prim;
+                        best_queue = queue;
+                    }
+                }
+            }
+            pop = !done;
+        }
+        if (pop) {
+            done = true;
+            while (sp > 0) {
+                sp--;
+                const uint2 e = sp < (uint32_t)LDS ? stack[sp][tid] : cfg.spill[(size_t)(sp - LDS) * stride + gtid];
+                if (ANY || __uint_as_float(e.y) <= r.tmax) {
+                    ref = e.x;
+                    done = false;
+                    break;
+                }
+            }
+        }
+        if (done) {
+            if (ANY) src.finish_any(item, best_prim != kInvalid, nodes, tris);
+            else src.finish_closest(item, best_prim, best_t, best_u, best_v, best_queue, nodes, tris);
+            active = false;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- ray sources
+
+struct WavefrontClosestSrc {
+    PathState p;
+    int cur;
+    bool instrument;
+    __device__ __forceinline__ uint32_t count() const { return p.ctr->ray[cur]; }
+    __device__ __forceinline__ unsigned int* work() const { return &p.ctr->work_closest; }
+    __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax) const {
+        const uint32_t slot = (cur ? p.rayq[1] : p.rayq[0])[i];
+        const float4 a = p.ray_o[slot], b = p.ray_d[slot];
+        o = xyz(a);
+        tmin = a.w;
+        d = xyz(b);
+        tmax = b.w;
+        return slot;
+    }
+    __device__ __forceinline__ void finish_closest(uint32_t slot, uint32_t prim, float t, float u, float v,
+                                                   uint32_t queue, uint32_t nodes, uint32_t tris) const {
+        const bool hit = prim != kInvalid;
+        if (hit) p.hit[slot] = make_float4(t, u, v, __uint_as_float(prim));
+        const int q = hit ? (int)queue : kQueueRegen;
+        const uint32_t pos = wave_reserve_multi(p.ctr->queue, q);
+        switch (q) {   // queue arrays are separate allocations
+            case 0: p.q[0][pos] = slot; break;
+            case 1: p.q[1][pos] = slot; break;
+            case 2: p.q[2][pos] = slot; break;
+            case 3: p.q[3][pos] = slot; break;
+            default: p.q[4][pos] = slot; break;
+        }
+        if (instrument) {
+            atomicAdd(&p.ctr->closest_nodes, (unsigned long long)nodes);
+            atomicAdd(&p.ctr->closest_tris, (unsigned long long)tris);
+        }
+    }
+    __device__ void finish_any(uint32_t, bool, uint32_t, uint32_t) const {}
+};
+
+struct WavefrontShadowSrc {
+    PathState p;
+    bool instrument;
+    __device__ __forceinline__ uint32_t count() const { return p.ctr->shadow; }
+    __device__ __forceinline__ unsigned int* work() const { return &p.ctr->work_shadow; }
+    __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax) const {
+        const uint32_t slot = p.shadowq[i];
+        const float4 a = p.sh_o[slot], b = p.sh_d[slot];
+        o = xyz(a);
+        tmax = a.w;
+        d = xyz(b);
+        tmin = kMinT;
+        return slot;
+    }
+    __device__ void finish_closest(uint32_t, uint32_t, float, float, float, uint32_t, uint32_t, uint32_t) const {}
+    __device__ __forceinline__ void finish_any(uint32_t slot, bool occluded, uint32_t nodes, uint32_t tris) const {
+        if (!occluded) {
+            const float4 L4 = p.sh_L[slot];
+            V3 L = xyz(L4);
+            // ClampRadiance(radiance, 20), Intersect.cu:125-134
+            const float l2 = dot(L, L);
+            const float max_len = 20.0f;
+            if (l2 > max_len * max_len) L = L * (max_len * (1.0f / sqrtf(l2)));
+            float* acc = reinterpret_cast<float*>(p.accum + __float_as_uint(L4.w));
+            atomicAdd(acc + 0, L.x);
+            atomicAdd(acc + 1, L.y);
+            atomicAdd(acc + 2, L.z);
+        }
+        if (instrument) {
+            atomicAdd(&p.ctr->shadow_nodes, (unsigned long long)nodes);
+            atomicAdd(&p.ctr->shadow_tris, (unsigned long long)tris);
+        }
+    }
+};
+
+struct BatchSrc {
+    const float4* rays;
+    float4* hits;
+    uint32_t* occluded;
+    uint32_t* nodes_out;
+    uint32_t* tris_out;
+    unsigned int* work_ctr;
+    uint32_t n;
+    __device__ __forceinline__ uint32_t count() const { return n; }
+    __device__ __forceinline__ unsigned int* work() const { return work_ctr; }
+    __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax) const {
+        const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
+        o = xyz(a);
+        tmin = a.w;
+        d = xyz(b);
+        tmax = b.w;
+        return i;
+    }
+    __device__ __forceinline__ void finish_closest(uint32_t i, uint32_t prim, float t, float u, float v,
+                                                   uint32_t, uint32_t nodes, uint32_t tris) const {
+        // pt_hit layout: prim, t, u, v
+        if (prim == kInvalid) hits[i] = make_float4(__uint_as_float(kInvalid), 0.0f, 0.0f, 0.0f);
+        else hits[i] = make_float4(__uint_as_float(prim), t, u, v);
+        if (nodes_out) nodes_out[i] = nodes;
+        if (tris_out) tris_out[i] = tris;
+    }
+    __device__ __forceinline__ void finish_any(uint32_t i, bool occ, uint32_t nodes, uint32_t tris) const {
+        occluded[i] = occ ? 1u : 0u;
+        if (nodes_out) nodes_out[i] = nodes;
+        if (tris_out) tris_out[i] = tris;
+    }
+};
+
+struct ClassifySrc {
+    const float4* rays;
+    uint32_t* queues[kQueueSlots];
+    unsigned int* queue_counts;
+    unsigned int* work_ctr;
+    uint32_t n;
+    __device__ __forceinline__ uint32_t count() const { return n; }
+    __device__ __forceinline__ unsigned int* work() const { return work_ctr; }
+    __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax) const {
+        const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
+        o = xyz(a);
+        tmin = a.w;
+        d = xyz(b);
+        tmax = b.w;
+        return i;
+    }
+    __device__ __forceinline__ void finish_closest(uint32_t i, uint32_t prim, float, float, float,
+                                                   uint32_t queue, uint32_t, uint32_t) const {
+        const int q = prim != kInvalid ? (int)queue : kQueueRegen;
+        const uint32_t pos = wave_reserve_multi(queue_counts, q);
+        switch (q) {
+            case 0: queues[0][pos] = i; break;
+            case 1: queues[1][pos] = i; break;
+            case 2: queues[2][pos] = i; break;
+            case 3: queues[3][pos] = i; break;
+            default: queues[4][pos] = i; break;
+        }
+    }
+    __device__ void finish_any(uint32_t, bool, uint32_t, uint32_t) const {}
+};
+
+template <int BLOCK, int LDS, bool ANY, bool COUNT, class Src>
+__global__ __launch_bounds__(BLOCK) void k_trace(Src src, SceneDev sc, TraceConfig cfg) {
+    trace_persistent<BLOCK, LDS, ANY, COUNT, Src>(src, sc, cfg);
+}
+
+// ================================================================ shading math
+
+struct Frame3 {
+    V3 t, b, n;
+    __device__ __forceinline__ V3 to_local(V3 d) const { return v3(dot(d, t), dot(d, b), dot(d, n)); }
+    __device__ __forceinline__ V3 from_local(V3 d) const { return d.x * t + d.y * b + d.z * n; }
+};
+
+// Math::SafeNormalize (Math.hpp:108-118) with 1/sqrt in place of rsqrt.
+__device__ __forceinline__ V3 safe_normalize(V3 v, V3 fallback) {
+    const float l2 = dot(v, v);
+    if (l2 > 1e-16f) return v * (1.0f / sqrtf(l2));
+    return fallback;
+}
+
+// Math::BuildFrame(normal) (Math.hpp:120-134)
+__device__ __forceinline__ Frame3 build_frame(V3 n) {
+    V3 t;
+    if (fabsf(n.x) > fabsf(n.z)) t = normalize(v3(n.y, -n.x, 0.0f));
+    else t = normalize(v3(0.0f, n.z, -n.y));
+    return Frame3{t, cross(n, t), n};
+}
+
+// Math::BuildFrame(normal, tangent, sign) (Math.hpp:136-148)
+__device__ __forceinline__ Frame3 build_frame(V3 n, V3 t, float sign) {
+    n = safe_normalize(n, v3(0.0f, 0.0f, 1.0f));
+    const Frame3 fb = build_frame(n);
+    t = t - n * dot(n, t);
+    t = safe_normalize(t, fb.t);
+    return Frame3{t, cross(n, t) * sign, n};
+}
+
+__device__ __forceinline__ V3 fix_normal(V3 gn, V3 d) { return dot(gn, d) > 0.0f ? -gn : gn; }
+
+__device__ __forceinline__ float luminance(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+__device__ __forceinline__ V3 cosine_hemisphere(float u0, float u1) {
+    const float phi = 2.0f * kPi * u0;
+    const float cos_t = sqrtf(u1);
+    const float sin_t = sqrtf(1.0f - u1);
+    float s, c;
+    sincosf(phi, &s, &c);
+    return v3(sin_t * c, sin_t * s, cos_t);
+}
+
+// Math.hpp:217-262 trig helpers on a local direction
+__device__ __forceinline__ float cos2_theta(V3 w) { return w.z * w.z; }
+__device__ __forceinline__ float sin2_theta(V3 w) { return fmaxf(0.0f, 1.0f - cos2_theta(w)); }
+__device__ __forceinline__ float sin_theta(V3 w) { return sqrtf(sin2_theta(w)); }
+__device__ __forceinline__ float tan2_theta(V3 w) { return sin2_theta(w) / cos2_theta(w); }
+__device__ __forceinline__ float cos_phi(V3 w) {
+    const float st = sin_theta(w);
+    return st == 0.0f ? 1.0f : clampf(w.x / st, -1.0f, 1.0f);
+}
+__device__ __forceinline__ float sin_phi(V3 w) {
+    const float st = sin_theta(w);
+    return st == 0.0f ? 0.0f : clampf(w.y / st, -1.0f, 1.0f);
+}
+
+// GGX terms, MaterialEval.cu:587-650
+__device__ __forceinline__ float ggx_lambda(V3 w, float alpha) {
+    const float t2 = tan2_theta(w);
+    if (t2 == __builtin_inff()) return 0.0f;
+    const float a2 = sqr(cos_phi(w) * alpha) + sqr(sin_phi(w) * alpha);
+    return (sqrtf(1.0f + a2 * t2) - 1.0f) * 0.5f;
+}
+__device__ __forceinline__ float ggx_g1(V3 w, float alpha) { return 1.0f / (1.0f + ggx_lambda(w, alpha)); }
+__device__ __forceinline__ float ggx_g(V3 wo, V3 wi, float alpha) {
+    return 1.0f / (1.0f + ggx_lambda(wo, alpha) + ggx_lambda(wi, alpha));
+}
+__device__ __forceinline__ float ggx_d(V3 wm, float alpha) {
+    const float t2 = tan2_theta(wm);
+    if (t2 == __builtin_inff()) return 0.0f;
+    const float cos4 = sqr(cos2_theta(wm));
+    if (cos4 < 1e-16f) return 0.0f;
+    const float e = t2 * (sqr(cos_phi(wm) / alpha) + sqr(sin_phi(wm) / alpha));
+    return 1.0f / (kPi * sqr(alpha) * cos4 * sqr(1.0f + e));
+}
+__device__ __forceinline__ float ggx_dv(V3 w, V3 wm, float alpha) {
+    return ggx_g1(w, alpha) / fabsf(w.z) * ggx_d(wm, alpha) * fabsf(dot(w, wm));
+}
+__device__ __forceinline__ V3 fresnel_schlick(V3 f0, float cos_h) {
+    const float k = powf(1.0f - cos_h, 5.0f);
+    return f0 + (v3(1.0f) - f0) * k;
+}
+// Visible-normal sampling (Heitz 2018), TrowbridgeSampleWm, MaterialEval.cu:594-610
+__device__ __forceinline__ V3 ggx_sample_wm(V3 w, float alpha, float u0, float u1) {
+    V3 wh = normalize(v3(alpha * w.x, alpha * w.y, w.z));
+    if (wh.z < 0.0f) wh = -wh;
+    const V3 t1 = (wh.z < 0.999f) ? normalize(cross(wh, v3(0.0f, 0.0f, 1.0f))) : v3(1.0f, 0.0f, 0.0f);
+    const V3 t2 = cross(wh, t1);
+    const float rr = sqrtf(u0);
+    const float theta = 2.0f * kPi * u1;
+    float s, c;
+    sincosf(theta, &s, &c);
+    float px = rr * c, py = rr * s;
+    const float h = sqrtf(1.0f - px * px);
+    const float lt = (1.0f + wh.z) * 0.5f;
+    py = h + lt * (py - h);
+    const float pz = sqrtf(fmaxf(0.0f, 1.0f - (px * px + py * py)));
+    const V3 nh = px * t1 + py * t2 + pz * wh;
+    return normalize(v3(alpha * nh.x, alpha * nh.y, fmaxf(1e-6f, nh.z)));
+}
+
+// ---------------------------------------------------------------- per-sample state
+
+struct Draws {
+    uint64_t state, inc;
+    __device__ __forceinline__ float next() {
+        const uint32_t x = pcg_output(state);
+        state = state * kPcgMult + inc;
+        return u01(x);
+    }
+};
+
+__device__ __forceinline__ Draws draws_at(const FrameParams* fp, uint64_t sample, uint32_t bounce) {
+    const PcgStream s = pcg_seed(fp->seed, sample);
+    const PcgJump j = fp->bounce_jump[bounce];
+    return Draws{j.mult * s.state + j.plus * s.inc, s.inc};
+}
+
+struct Hit {
+    V3 p, gn, n;
+    float4 tan;
+};
+
+__device__ __forceinline__ float4 lerp4(float b0, float4 a, float b1, float4 b, float b2, float4 c) {
+    return make_float4(b0 * a.x + b1 * b.x + b2 * c.x, b0 * a.y + b1 * b.y + b2 * c.y,
+                       b0 * a.z + b1 * b.z + b2 * c.z, b0 * a.w + b1 * b.w + b2 * c.w);
+}
+
+// LoadResolvedHit, MaterialEval.cu:54-87 (no UVs: materials are constant)
+__device__ __forceinline__ Hit resolve_hit(const ShadeRecord* rec, V3 o, V3 d, float t, float u, float v,
+                                           uint32_t& material) {
+    const float4* s = reinterpret_cast<const float4*>(rec);
+    const float4 n0 = s[0], n1 = s[1], n2 = s[2], t0 = s[3], t1 = s[4], t2 = s[5], g = s[6];
+    const float b0 = 1.0f - u - v, b1 = u, b2 = v;
+    Hit h;
+    h.p = o + d * t;
+    h.gn = fix_normal(xyz(g), d);
+    h.n = b0 * xyz(n0) + b1 * xyz(n1) + b2 * xyz(n2);
+    h.tan = lerp4(b0, t0, b1, t1, b2, t2);
+    material = __float_as_uint(g.w);
+    return h;
+}
+
+// GetShadingFrame, MaterialEval.cu:96-115, with the normal-map texel as a material constant.
+__device__ __forceinline__ Frame3 shading_frame(const Hit& h, V3 nts) {
+    const V3 bt = v3(h.tan.x, h.tan.y, h.tan.z);
+    const float sign = h.tan.w < 0.0f ? -1.0f : 1.0f;
+    const Frame3 base = build_frame(h.n, bt, sign);
+    const V3 sn = nts.x * base.t + nts.y * base.b + nts.z * base.n;
+    return build_frame(sn, bt, sign);
+}
+
+struct Continuation {
+    bool terminated;
+    uint32_t depth;
+    V3 thr;
+};
+
+// ComputePathContinuation, MaterialEval.cu:129-146
+__device__ __forceinline__ Continuation continue_path(uint32_t depth, V3 thr, uint32_t limit, uint32_t rr_start,
+                                                      V3 weight, float rr) {
+    const uint32_t next = depth + 1u;
+    if (next >= limit) return Continuation{true, next, v3(0.0f)};
+    const V3 nt = thr * weight;
+    if (next < rr_start) return Continuation{false, next, nt};
+    const float q = clampf(maxc(nt), 0.0f, 1.0f);
+    if (rr > q) return Continuation{true, next, v3(0.0f)};
+    return Continuation{false, next, nt / q};
+}
+
+struct LightSample {
+    V3 p, gn, wi;
+    float d2;
+    V3 emission;
+    float pdf;
+};
+
+// AliasTableView::Sample, AliasTableView.hpp:32-60
+__device__ __forceinline__ uint32_t alias_pick(const AliasBin* bins, uint32_t size, float u, float& pmf) {
+    const float scaled = u * (float)size;
+    const uint32_t bi = min((uint32_t)scaled, size - 1u);
+    const float up = fminf(scaled - (float)bi, kOneMinusEpsilon);
+    const AliasBin b = bins[bi];
+    if (up < b.q) {
+        pmf = b.p;
+        return bi;
+    }
+    pmf = bins[b.alias].p;
+    return b.alias;
+}
+
+// SampleLightTriangle, MaterialEval.cu:171-192
+__device__ __forceinline__ LightSample sample_light(const SceneDev& sc, float us, float x0, float x1, V3 p) {
+    float pmf;
+    const uint32_t li = alias_pick(sc.bins, sc.light_count, us, pmf);
+    const float4* L = reinterpret_cast<const float4*>(sc.lights + li);
+    const float4 a = L[0], e1 = L[1], e2 = L[2], g = L[3], em = L[4];
+    const float s0 = sqrtf(x0);
+    const float b1 = s0 * (1.0f - x1);
+    const float b2 = s0 * x1;
+    LightSample ls;
+    ls.p = xyz(a) + b1 * xyz(e1) + b2 * xyz(e2);
+    const V3 to = ls.p - p;
+    ls.d2 = dot(to, to);
+    ls.wi = normalize(to);
+    ls.gn = fix_normal(xyz(g), ls.wi);
+    ls.emission = xyz(em);
+    ls.pdf = pmf / a.w;
+    return ls;
+}
+
+struct ShadowOut {
+    bool emit;
+    V3 o, d, L;
+    float tmax;
+};
+
+__device__ __forceinline__ void make_shadow(ShadowOut& so, V3 p, V3 gn, V3 lp, V3 L) {
+    const V3 origin = p + gn * 1e-4f;
+    const V3 sv = lp - origin;
+    const float dist = length(sv);
+    so.emit = true;
+    so.o = origin;
+    so.d = sv / dist;
+    so.tmax = dist - 1e-4f;
+    so.L = L;
+}
+
+// ================================================================ kernels
+
+__global__ void k_frame_begin(PathState p) {
+    if (threadIdx.x == 0) {
+        Counters* c = p.ctr;
+        const FrameParams* fp = p.fp;
+        const uint64_t total = fp->total_samples;
+        const uint32_t gen = (uint32_t)(total < (uint64_t)p.capacity ? total : (uint64_t)p.capacity);
+        c->ray[0] = gen;
+        c->ray[1] = 0;
+        for (int i = 0; i < kQueueSlots; i++) c->queue[i] = 0;
+        c->shadow = 0;
+        c->work_closest = 0;
+        c->work_shadow = 0;
+        c->active = gen;
+        c->iterations = 0;
+        c->invalid_pixels = 0;
+        c->stack_overflow = 0;
+        c->launched = gen;
+        c->ext_rays = 0;
+        c->shadow_rays = 0;
+        c->closest_nodes = c->closest_tris = 0;
+        c->shadow_nodes = c->shadow_tris = 0;
+    }
+}
+
+__global__ void k_fill_accum(float4* accum, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        accum[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// InitializePathsKernel / RegeneratePathsKernel, PathGeneration.cu:56-102
+__global__ void k_generate(PathState p, int init, int nxt) {
+    const FrameParams* fp = p.fp;
+    Counters* c = p.ctr;
+    uint32_t n, base;
+    uint64_t launched;
+    if (init) {
+        n = c->ray[0];
+        base = 0;
+        launched = 0;
+    } else {
+        const uint64_t total = fp->total_samples;
+        launched = c->launched;
+        const uint64_t remaining = total - launched;
+        const uint32_t regen = c->queue[kQueueRegen];
+        n = (uint64_t)regen < remaining ? regen : (uint32_t)remaining;
+        base = c->ray[nxt];
+    }
+    const uint32_t W = fp->width, H = fp->height, spp = fp->spp, axis = fp->spp_axis;
+    const V3 co = v3(fp->cam.origin[0], fp->cam.origin[1], fp->cam.origin[2]);
+    const V3 llc = v3(fp->cam.llc[0], fp->cam.llc[1], fp->cam.llc[2]);
+    const V3 hor = v3(fp->cam.horizontal[0], fp->cam.horizontal[1], fp->cam.horizontal[2]);
+    const V3 ver = v3(fp->cam.vertical[0], fp->cam.vertical[1], fp->cam.vertical[2]);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = init ? i : p.q[kQueueRegen][i];
+        const uint64_t sample = launched + i;
+        const PcgStream s = pcg_seed(fp->seed, sample);
+        Draws dr{s.state, s.inc};
+        const float jx = dr.next();
+        const float jy = dr.next();
+        // ComputePathSample, PathGeneration.cu:15-33
+        const uint64_t pixel64 = sample / spp;
+        const uint32_t cell = (uint32_t)(sample - pixel64 * spp);
+        const uint32_t pixel = (uint32_t)pixel64;
+        const uint32_t y = pixel / W, x = pixel - y * W;
+        const uint32_t cy = cell / axis, cx = cell - cy * axis;
+        const float u = ((float)x + ((float)cx + jx) / (float)axis) / (float)W;
+        const float v = ((float)y + ((float)cy + jy) / (float)axis) / (float)H;
+        // GenerateCameraRay, PathGeneration.cu:35-43
+        const V3 d = normalize(llc + u * hor + v * ver - co);
+        p.ray_o[slot] = f4(co, kMinT);
+        p.ray_d[slot] = f4(d, kMaxT);
+        p.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        p.meta[slot] = make_uint4(pixel, kDeltaBit, (uint32_t)sample, (uint32_t)(sample >> 32));
+        (nxt ? p.rayq[1] : p.rayq[0])[base + i] = slot;
+    }
+}
+
+// PrepareIterationKernel (Iteration.cu:7-23) plus the host-side bookkeeping of
+// Renderer.cpp:573-619, done on the device so the loop needs no host sync.
+__global__ void k_prepare(PathState p, int cur) {
+    if (threadIdx.x == 0) {
+        Counters* c = p.ctr;
+        const int nxt = cur ^ 1;
+        const uint64_t total = p.fp->total_samples;
+        const uint64_t remaining = total - c->launched;
+        const uint32_t regen = c->queue[kQueueRegen];
+        const uint32_t nr = (uint64_t)regen < remaining ? regen : (uint32_t)remaining;
+        const uint32_t traced = c->ray[cur];
+        c->ext_rays += traced;
+        c->shadow_rays += c->shadow;
+        if (traced) c->iterations++;
+        c->ray[nxt] += nr;
+        c->launched += nr;
+        c->ray[cur] = 0;
+        for (int i = 0; i < kQueueSlots; i++) c->queue[i] = 0;
+        c->shadow = 0;
+        c->work_closest = 0;
+        c->work_shadow = 0;
+        c->active = c->ray[nxt];
+    }
+}
+
+// The four material stages of one bounce in one launch: element e of the
+// virtual range [diffuse | ggx | mirror | emissive] is queue entry e - offset.
+// Each wave walks 64 consecutive entries, so a wave shades one material except
+// at the three queue boundaries.
+__global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur) {
+    const FrameParams* fp = p.fp;
+    Counters* c = p.ctr;
+    const int nxt = cur ^ 1;
+    const uint32_t c0 = c->queue[0], c1 = c->queue[1], c2 = c->queue[2], c3 = c->queue[3];
+    const uint32_t e1 = c0, e2 = c0 + c1, e3 = e2 + c2, total = e3 + c3;
+    const uint32_t limit = fp->depth_limit, rr_start = fp->rr_start;
+    const bool nee = fp->use_nee != 0u && sc.light_count > 0u;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t lane = lane_id();
+
+    for (uint32_t wbase = wave * 64u; wbase < total; wbase += waves * 64u) {
+        const uint32_t e = wbase + lane;
+        const bool valid = e < total;
+        bool to_ray = false, to_regen = false;
+        ShadowOut so;
+        so.emit = false;
+        uint32_t slot = 0;
+        V3 add_L = v3(0.0f);
+        bool add = false;
+        uint32_t pixel = 0;
+        if (valid) {
+            int q;
+            uint32_t j;
+            if (e < e1) { q = 0; j = e; }
+            else if (e < e2) { q = 1; j = e - e1; }
+            else if (e < e3) { q = 2; j = e - e2; }
+            else { q = 3; j = e - e3; }
+            const uint32_t* qa = q == 0 ? p.q[0] : q == 1 ? p.q[1] : q == 2 ? p.q[2] : p.q[3];
+            slot = qa[j];
+            const uint4 m = p.meta[slot];
+            pixel = m.x;
+            const uint32_t depth = m.y & 0xFFu;
+            const bool delta = (m.y & kDeltaBit) != 0u;
+            const float4 th4 = p.thr[slot];
+            const V3 thr = xyz(th4);
+            const float4 hh = p.hit[slot];
+            const uint32_t prim = __float_as_uint(hh.w);
+            if (q == 3) {
+                // EvaluateEmissiveKernel, MaterialEval.cu:882-912
+                if (!fp->use_nee || delta) {
+                    const uint32_t mi = reinterpret_cast<const uint32_t*>(sc.shade + prim)[27];
+                    const MaterialRecord* mr = sc.materials + mi;
+                    add_L = thr * v3(mr->emission[0], mr->emission[1], mr->emission[2]);
+                    add = true;
+                }
+                to_regen = true;
+            } else {
+                const float4 o4 = p.ray_o[slot], d4 = p.ray_d[slot];
+                const V3 ro = xyz(o4), rdir = xyz(d4);
+                uint32_t mi;
+                const Hit h = resolve_hit(sc.shade + prim, ro, rdir, hh.x, hh.y, hh.z, mi);
+                const MaterialRecord* mr = sc.materials + mi;
+                const float4* mf = reinterpret_cast<const float4*>(mr);
+                const float4 m0 = mf[0], m1 = mf[1], m2 = mf[2], m4 = mf[4];
+                const V3 nts = xyz(m4);
+                const Frame3 fr = shading_frame(h, nts);
+                Draws dr = draws_at(fp, ((uint64_t)m.w << 32) | m.z, depth);
+                const float u0 = dr.next(), u1 = dr.next(), u2 = dr.next(), u3 = dr.next();
+                float l0 = 0.0f, l1 = 0.0f, l2 = 0.0f;
+                if (nee) {
+                    l0 = dr.next();
+                    l1 = dr.next();
+                    l2 = dr.next();
+                }
+                V3 wi = v3(0.0f), weight = v3(0.0f);
+                bool scatter = false;
+                bool new_delta = false;
+                if (q == 0) {
+                    // Diffuse, MaterialEval.cu:231-350
+                    const V3 albedo = xyz(m0);
+                    if (nee) {
+                        const LightSample ls = sample_light(sc, l0, l1, l2, h.p);
+                        if (!(ls.d2 < 1e-6f)) {
+                            const float ci = fmaxf(dot(fr.n, ls.wi), 0.0f);
+                            const float gci = fmaxf(dot(h.gn, ls.wi), 0.0f);
+                            if (!(ci <= 0.0f || gci <= 0.0f)) {
+                                const float cl = fmaxf(dot(ls.gn, v3(0.0f) - ls.wi), 0.0f);
+                                if (!(cl <= 0.0f)) {
+                                    const V3 brdf = albedo / kPi;
+                                    const float G = ci * cl / ls.d2;
+                                    const V3 L = ls.emission * thr * brdf * G / ls.pdf;
+                                    make_shadow(so, h.p, h.gn, ls.p, L);
+                                }
+                            }
+                        }
+                    }
+                    wi = fr.from_local(cosine_hemisphere(u0, u1));
+                    const float ci = fmaxf(dot(fr.n, wi), 0.0f);
+                    const float gci = fmaxf(dot(h.gn, wi), 0.0f);
+                    if (!(ci <= 0.0f || gci <= 0.0f)) {
+                        weight = albedo;
+                        scatter = true;
+                    }
+                } else if (q == 1) {
+                    // GGX, MaterialEval.cu:652-880
+                    const V3 base = xyz(m0);
+                    const float alpha = m1.w;
+                    const float metallic = m2.x;
+                    const float ior = m2.y;
+                    const float cur_ior = th4.w;
+                    const float df0 = sqr((cur_ior - ior) / (cur_ior + ior));
+                    const V3 f0 = lerp3(v3(df0), base, metallic);
+                    const V3 diffc = base * (1.0f - metallic);
+                    const V3 wo = fr.to_local(v3(0.0f) - rdir);
+                    if (nee) {
+                        const LightSample ls = sample_light(sc, l0, l1, l2, h.p);
+                        if (!(ls.d2 < 1e-6f || ls.pdf <= 0.0f)) {
+                            const float gci = dot(h.gn, ls.wi);
+                            const float cl = fmaxf(dot(ls.gn, v3(0.0f) - ls.wi), 0.0f);
+                            if (!(gci <= 0.0f) && !(cl <= 0.0f)) {
+                                const V3 wl = fr.to_local(ls.wi);
+                                const float ci = wl.z, co = wo.z;
+                                if (!(ci <= 0.0f || co <= 0.0f)) {
+                                    const V3 wm = normalize(wo + wl);
+                                    if (!(wm.z <= 0.0f)) {
+                                        const float owm = dot(wo, wm), iwm = dot(wl, wm);
+                                        if (!(owm <= 1e-6f || iwm <= 1e-6f)) {
+                                            const V3 F = fresnel_schlick(f0, clampf(owm, 0.0f, 1.0f));
+                                            const V3 fd = (v3(1.0f) - F) * diffc * kInvPi;
+                                            const V3 fs = F * ggx_d(wm, alpha) * ggx_g(wo, wl, alpha) / (4.0f * ci * co);
+                                            const V3 bsdf = fd + fs;
+                                            const float G = ci * cl / ls.d2;
+                                            const V3 L = ls.emission * thr * bsdf * G / ls.pdf;
+                                            make_shadow(so, h.p, h.gn, ls.p, L);
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    // EvaluateGgxIndirect, MaterialEval.cu:720-836
+                    if (wo.z > 0.0f) {
+                        const float dw = luminance(diffc), sw = luminance(f0);
+                        const float wsum = dw + sw;
+                        if (!(wsum <= 1e-6f)) {
+                            const float pd = dw / wsum, ps = sw / wsum;
+                            V3 wl;
+                            if (u2 < pd) wl = cosine_hemisphere(u0, u1);
+                            else wl = reflect(-wo, ggx_sample_wm(wo, alpha, u0, u1));
+                            const float ci = wl.z, co = wo.z;
+                            if (!(ci <= 0.0f)) {
+                                const V3 wm = normalize(wl + wo);
+                                if (!(wm.z <= 0.0f)) {
+                                    const float owm = dot(wo, wm);
+                                    if (!(owm <= 1e-6f)) {
+                                        const V3 F = fresnel_schlick(f0, clampf(owm, 0.0f, 1.0f));
+                                        const V3 fd = (v3(1.0f) - F) * diffc * kInvPi;
+                                        const V3 fs = F * ggx_d(wm, alpha) * ggx_g(wo, wl, alpha) / (4.0f * ci * co);
+                                        const float pdfd = ci * kInvPi;
+                                        const float pdfs = ggx_dv(wo, wm, alpha) / (4.0f * owm);
+                                        const float pdf = pd * pdfd + ps * pdfs;
+                                        if (!(pdf <= 1e-8f)) {
+                                            const V3 ww = fr.from_local(wl);
+                                            if (!(dot(ww, h.gn) <= 0.0f)) {
+                                                wi = ww;
+                                                weight = (fd + fs) * (ci / pdf);
+                                                scatter = true;
+                                            }
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                    }
+                } else {
+                    // Mirror, MaterialEval.cu:352-392
+                    wi = reflect(rdir, fr.n);
+                    weight = xyz(m1);
+                    scatter = true;
+                    new_delta = true;
+                }
+                if (scatter) {
+                    const Continuation k = continue_path(depth, thr, limit, rr_start, weight, u3);
+                    if (!k.terminated) {
+                        // SpawnScatteredRay, MaterialEval.cu:148-156
+                        const V3 no = ro + rdir * hh.x + h.gn * 1e-4f;
+                        p.ray_o[slot] = f4(no, kMinT);
+                        p.ray_d[slot] = f4(wi, kMaxT);
+                        p.thr[slot] = f4(k.thr, th4.w);
+                        p.meta[slot] = make_uint4(m.x, k.depth | (new_delta ? kDeltaBit : 0u), m.z, m.w);
+                        to_ray = true;
+                    } else {
+                        to_regen = true;
+                    }
+                } else {
+                    to_regen = true;
+                }
+            }
+        }
+        if (add) {
+            float* acc = reinterpret_cast<float*>(p.accum + pixel);
+            atomicAdd(acc + 0, add_L.x);
+            atomicAdd(acc + 1, add_L.y);
+            atomicAdd(acc + 2, add_L.z);
+        }
+        const uint32_t sh_pos = wave_reserve(&c->shadow, so.emit);
+        if (so.emit) {
+            p.sh_o[slot] = f4(so.o, so.tmax);
+            p.sh_d[slot] = f4(so.d, 0.0f);
+            p.sh_L[slot] = f4(so.L, __uint_as_float(pixel));
+            p.shadowq[sh_pos] = slot;
+        }
+        const uint32_t ray_pos = wave_reserve(&c->ray[nxt], to_ray);
+        if (to_ray) (nxt ? p.rayq[1] : p.rayq[0])[ray_pos] = slot;
+        const uint32_t rg_pos = wave_reserve(&c->queue[kQueueRegen], to_regen);
+        if (to_regen) p.q[kQueueRegen][rg_pos] = slot;
+    }
+}
+
+// PostprocessKernel, Image.cu:34-54
+__global__ void k_postprocess(PathState p) {
+    const FrameParams* fp = p.fp;
+    const float inv_spp = fp->inv_spp;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.pixels; i += gridDim.x * blockDim.x) {
+        const float4 a = p.accum[i];
+        const bool ok = isfinite(a.x) && a.x >= 0.0f && isfinite(a.y) && a.y >= 0.0f && isfinite(a.z) && a.z >= 0.0f;
+        if (!ok) atomicAdd(&p.ctr->invalid_pixels, 1u);
+        const float inv_gamma = 1.0f / 2.2f;
+        float c[3] = {a.x * inv_spp, a.y * inv_spp, a.z * inv_spp};
+        uint32_t out = 0xFF000000u;
+        for (int k = 0; k < 3; k++) {
+            const float t = c[k] / (c[k] + 1.0f);
+            const float g = powf(t, inv_gamma);
+            const uint32_t b = (uint32_t)(unsigned char)clampf(g * 255.0f, 0.0f, 255.0f);
+            out |= b << (8 * k);
+        }
+        p.framebuffer[i] = out;
+    }
+}
+
+__global__ void k_alias_sample(SceneDev sc, const float* u, uint32_t n, uint32_t* index, float* pmf) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float pm;
+        index[i] = alias_pick(sc.bins, sc.light_count, u[i], pm);
+        pmf[i] = pm;
+    }
+}
+
+__global__ void k_zero_u32(unsigned int* ptr, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ptr[i] = 0u;
+}
+
+// ================================================================ launchers
+
+namespace {
+
+uint32_t grid_for(uint32_t n, uint32_t block, uint32_t max_grid) {
+    uint64_t g = ((uint64_t)n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > max_grid) g = max_grid;
+    return (uint32_t)g;
+}
+
+template <int BLOCK, int LDS, bool ANY, bool COUNT, class Src>
+hipError_t run_trace(hipStream_t s, const Src& src, const SceneDev& sc, const TraceConfig& cfg) {
+    hipLaunchKernelGGL((k_trace<BLOCK, LDS, ANY, COUNT, Src>), dim3(cfg.blocks), dim3(BLOCK), 0, s, src, sc, cfg);
+    return hipGetLastError();
+}
+
+template <bool ANY, bool COUNT, class Src>
+hipError_t dispatch_trace(hipStream_t s, const Src& src, const SceneDev& sc, const TraceConfig& cfg) {
+    if (cfg.block == 64 && cfg.lds_stack == 16) return run_trace<64, 16, ANY, COUNT>(s, src, sc, cfg);
+    if (cfg.block == 128 && cfg.lds_stack == 16) return run_trace<128, 16, ANY, COUNT>(s, src, sc, cfg);
+    if (cfg.block == 128 && cfg.lds_stack == 8) return run_trace<128, 8, ANY, COUNT>(s, src, sc, cfg);
+    if (cfg.block == 256 && cfg.lds_stack == 8) return run_trace<256, 8, ANY, COUNT>(s, src, sc, cfg);
+    if (cfg.block == 256 && cfg.lds_stack == 16) return run_trace<256, 16, ANY, COUNT>(s, src, sc, cfg);
+    return hipErrorInvalidValue;
+}
+
+template <int BLOCK, int LDS, bool ANY, bool COUNT, class Src>
+int occ_one() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<BLOCK, LDS, ANY, COUNT, Src>, BLOCK, 0) != hipSuccess) return 0;
+    return nb;
+}
+
+template <bool ANY, bool COUNT, class Src>
+int occ(const TraceConfig& cfg) {
+    if (cfg.block == 64 && cfg.lds_stack == 16) return occ_one<64, 16, ANY, COUNT, Src>();
+    if (cfg.block == 128 && cfg.lds_stack == 16) return occ_one<128, 16, ANY, COUNT, Src>();
+    if (cfg.block == 128 && cfg.lds_stack == 8) return occ_one<128, 8, ANY, COUNT, Src>();
+    if (cfg.block == 256 && cfg.lds_stack == 8) return occ_one<256, 8, ANY, COUNT, Src>();
+    if (cfg.block == 256 && cfg.lds_stack == 16) return occ_one<256, 16, ANY, COUNT, Src>();
+    return 0;
+}
+
+}  // namespace
+
+bool trace_config_supported(uint32_t block, uint32_t lds_stack) {
+    return (block == 64 && lds_stack == 16) || (block == 128 && (lds_stack == 16 || lds_stack == 8)) ||
+           (block == 256 && (lds_stack == 8 || lds_stack == 16));
+}
+
+int trace_occupancy(const TraceConfig& cfg, bool any, bool count) {
+    if (any) return count ? occ<true, true, WavefrontShadowSrc>(cfg) : occ<true, false, WavefrontShadowSrc>(cfg);
+    return count ? occ<false, true, WavefrontClosestSrc>(cfg) : occ<false, false, WavefrontClosestSrc>(cfg);
+}
+
+hipError_t launch_frame_begin(hipStream_t s, const PathState& p) {
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_accum(hipStream_t s, const PathState& p) {
+    hipLaunchKernelGGL(k_fill_accum, dim3(grid_for(p.pixels, 256, 4096)), dim3(256), 0, s, p.accum, p.pixels);
+    return hipGetLastError();
+}
+
+hipError_t launch_generate(hipStream_t s, const PathState& p, bool init, int nxt, uint32_t grid, uint32_t block) {
+    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(block), 0, s, p, init ? 1 : 0, nxt);
+    return hipGetLastError();
+}
+
+hipError_t launch_closest(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool count,
+                          const TraceConfig& cfg) {
+    WavefrontClosestSrc src{p, cur, count};
+    return count ? dispatch_trace<false, true>(s, src, sc, cfg) : dispatch_trace<false, false>(s, src, sc, cfg);
+}
+
+hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, uint32_t grid, uint32_t block) {
+    if (block != 256) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, p, sc, cur);
+    return hipGetLastError();
+}
+
+hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, bool count, const TraceConfig& cfg) {
+    WavefrontShadowSrc src{p, count};
+    return count ? dispatch_trace<true, true>(s, src, sc, cfg) : dispatch_trace<true, false>(s, src, sc, cfg);
+}
+
+hipError_t launch_prepare(hipStream_t s, const PathState& p, int cur) {
+    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(64), 0, s, p, cur);
+    return hipGetLastError();
+}
+
+hipError_t launch_postprocess(hipStream_t s, const PathState& p) {
+    hipLaunchKernelGGL(k_postprocess, dim3(grid_for(p.pixels, 256, 4096)), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_batch(hipStream_t s, const SceneDev& sc, const float4* rays, uint32_t n, float4* hits,
+                              uint32_t* occluded, uint32_t* nodes, uint32_t* tris, unsigned int* work, bool any,
+                              const TraceConfig& cfg) {
+    BatchSrc src{rays, hits, occluded, nodes, tris, work, n};
+    const bool count = nodes != nullptr || tris != nullptr;
+    if (any) return count ? dispatch_trace<true, true>(s, src, sc, cfg) : dispatch_trace<true, false>(s, src, sc, cfg);
+    return count ? dispatch_trace<false, true>(s, src, sc, cfg) : dispatch_trace<false, false>(s, src, sc, cfg);
+}
+
+hipError_t launch_classify(hipStream_t s, const SceneDev& sc, const float4* rays, uint32_t n, uint32_t* const* queues,
+                           unsigned int* queue_counts, unsigned int* work, const TraceConfig& cfg) {
+    ClassifySrc src;
+    src.rays = rays;
+    for (int i = 0; i < kQueueSlots; i++) src.queues[i] = queues[i];
+    src.queue_counts = queue_counts;
+    src.work_ctr = work;
+    src.n = n;
+    return dispatch_trace<false, false>(s, src, sc, cfg);
+}
+
+hipError_t launch_alias_sample(hipStream_t s, const SceneDev& sc, const float* u, uint32_t n, uint32_t* index, float* pmf) {
+    hipLaunchKernelGGL(k_alias_sample, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, sc, u, n, index, pmf);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(hipStream_t s, unsigned int* ptr, uint32_t n) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, ptr, n);
+    return hipGetLastError();
+}
+
+}  // namespace pt

@@ -1,0 +1,82 @@
+// Kernel launchers of the wavefront core (host side), the counterpart of the
+// reference's free-function launcher layer
+// (Core/Include/Core/Graphics/Cuda/PathTracing/Kernels/Launchers.hpp:18-146).
+// Views are POD structs of device pointers passed by value.
+#pragma once
+
+#include "pt_device.hpp"
+
+namespace pt {
+
+// Queue slots: 0..3 material queues (diffuse, ggx, mirror, emissive), 4 regen.
+constexpr int kQueueRegen = 4;
+constexpr int kQueueSlots = 5;
+
+struct SceneDev {
+    const Bvh4Node* nodes;
+    const TriRecord* tris;          // BVH order
+    const ShadeRecord* shade;       // original order
+    const MaterialRecord* materials;
+    const LightRecord* lights;
+    const AliasBin* bins;
+    uint32_t light_count;
+    uint32_t tri_count;
+};
+
+struct PathState {                  // path pool, SoA by slot
+    float4* ray_o;                  // origin, tMin
+    float4* ray_d;                  // direction, tMax
+    float4* hit;                    // t, u, v, prim (bits)
+    float4* thr;                    // throughput, medium ior
+    uint4* meta;                    // pixel, depth|delta, sample lo, sample hi
+    float4* sh_o;                   // shadow origin, tMax
+    float4* sh_d;                   // shadow direction
+    float4* sh_L;                   // shadow radiance, pixel (bits)
+    uint32_t* rayq[2];
+    uint32_t* q[kQueueSlots];
+    uint32_t* shadowq;
+    Counters* ctr;
+    const FrameParams* fp;
+    float4* accum;
+    uint32_t* framebuffer;          // RGBA8
+    uint32_t capacity;
+    uint32_t pixels;
+};
+
+struct TraceConfig {
+    uint32_t block = 128;           // threads per block
+    uint32_t lds_stack = 16;        // stack entries per lane held in LDS
+    uint32_t blocks = 0;            // persistent grid size
+    uint32_t refill = 16;           // idle lanes that trigger a fetch of new rays
+    uint32_t spill_depth = 0;       // stack entries per lane in the global spill area
+    uint2* spill = nullptr;         // [spill_depth][blocks*block]
+    unsigned int* overflow = nullptr; // set when a push would exceed lds_stack+spill_depth
+};
+
+// Blocks of `block` threads that fit on one CU for the traversal kernels.
+int trace_occupancy(const TraceConfig& cfg, bool any, bool count);
+bool trace_config_supported(uint32_t block, uint32_t lds_stack);
+
+hipError_t launch_frame_begin(hipStream_t s, const PathState& p);
+hipError_t launch_fill_accum(hipStream_t s, const PathState& p);
+hipError_t launch_generate(hipStream_t s, const PathState& p, bool init, int nxt, uint32_t grid, uint32_t block);
+hipError_t launch_closest(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool count, const TraceConfig& cfg);
+hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, uint32_t grid, uint32_t block);
+hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, bool count, const TraceConfig& cfg);
+hipError_t launch_prepare(hipStream_t s, const PathState& p, int cur);
+hipError_t launch_postprocess(hipStream_t s, const PathState& p);
+
+// Batch hooks.  rays: n x 2 float4 (origin|tMin, direction|tMax).
+// Closest: hits[i] = (t, u, v, prim bits); any: occluded[i].  nodes/tris may be null.
+hipError_t launch_trace_batch(hipStream_t s, const SceneDev& sc, const float4* rays, uint32_t n,
+                              float4* hits, uint32_t* occluded, uint32_t* nodes, uint32_t* tris,
+                              unsigned int* work, bool any, const TraceConfig& cfg);
+// Closest hit + wave-aggregated push of the ray index into 5 queues.
+hipError_t launch_classify(hipStream_t s, const SceneDev& sc, const float4* rays, uint32_t n,
+                           uint32_t* const* queues, unsigned int* queue_counts, unsigned int* work,
+                           const TraceConfig& cfg);
+hipError_t launch_alias_sample(hipStream_t s, const SceneDev& sc, const float* u, uint32_t n,
+                               uint32_t* index, float* pmf);
+hipError_t launch_zero_u32(hipStream_t s, unsigned int* p, uint32_t n);
+
+}  // namespace pt

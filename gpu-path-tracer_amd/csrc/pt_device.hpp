@@ -1,0 +1,242 @@
+// Shared host/device definitions of the wavefront core: record layouts in HBM,
+// float3 arithmetic, the counter-addressable PCG32 sample streams, and the
+// per-sample math (camera, frames, BSDFs) that the kernels run.
+//
+// Every translation unit is compiled with -ffp-contract=off and IEEE division
+// and square root, so a float expression written here rounds the same way on
+// the CPU and on gfx950.  That is what makes hit ids and queue membership
+// reproducible bit for bit by an independent CPU implementation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PT_HD __host__ __device__ __forceinline__
+
+namespace pt {
+
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr float kMinT = 1e-4f;          // PathTracerDefaults.hpp:13
+constexpr float kMaxT = 1e34f;          // PathTracerDefaults.hpp:14
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 1.0f / 3.14159265358979323846f;
+constexpr float kOneMinusEpsilon = 0x1.fffffep-1f;   // Math.hpp:11
+constexpr uint32_t kMaxDepthLimit = 100;             // PathTracerDefaults.hpp:18
+constexpr uint32_t kQueueCount = 4;                  // diffuse, ggx, mirror, emissive
+
+// ---------------------------------------------------------------- float3 math
+struct V3 {
+    float x, y, z;
+};
+
+PT_HD V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+PT_HD V3 v3(float s) { return V3{s, s, s}; }
+PT_HD V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+PT_HD V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+PT_HD V3 operator-(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+PT_HD V3 operator*(V3 a, V3 b) { return V3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+PT_HD V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+PT_HD V3 operator*(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+PT_HD V3 operator/(V3 a, float s) { return V3{a.x / s, a.y / s, a.z / s}; }
+PT_HD V3 operator/(V3 a, V3 b) { return V3{a.x / b.x, a.y / b.y, a.z / b.z}; }
+PT_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+PT_HD V3 cross(V3 a, V3 b) {
+    return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+PT_HD float length(V3 a) { return sqrtf(dot(a, a)); }
+// One definition of normalize on both sides: v * (1/sqrt(v.v)), IEEE ops.
+PT_HD V3 normalize(V3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+PT_HD V3 reflect(V3 i, V3 n) { return i - 2.0f * n * dot(n, i); }
+PT_HD float maxc(V3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+PT_HD float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+PT_HD V3 lerp3(V3 a, V3 b, float t) { return a + t * (b - a); }
+PT_HD float sqr(float v) { return v * v; }
+
+PT_HD V3 xyz(float4 a) { return V3{a.x, a.y, a.z}; }
+PT_HD float4 f4(V3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// ---------------------------------------------------------------- records in HBM
+
+// BVH4 node: one 128-byte line.  Six 4-wide coordinate rows so one lane's box
+// test is 24 independent slab terms, then four child references.  A reference
+// with bit 31 clear is a node index; with bit 31 set it is a leaf:
+// bits 4..30 = first triangle (BVH order), bits 0..3 = count - 1.
+struct alignas(128) Bvh4Node {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    uint32_t child[4];
+    uint32_t info[4];   // per-child triangle count (leaf) or 0; diagnostics only
+};
+static_assert(sizeof(Bvh4Node) == 128, "BVH4 node must fill one cache line");
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+PT_HD bool ref_is_leaf(uint32_t r) { return (r & kLeafBit) != 0u; }
+PT_HD uint32_t leaf_first(uint32_t r) { return (r & 0x7FFFFFFFu) >> 4; }
+PT_HD uint32_t leaf_count(uint32_t r) { return (r & 15u) + 1u; }
+PT_HD uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
+    return kLeafBit | (first << 4) | (count - 1u);
+}
+constexpr uint32_t kMaxLeafSize = 16;
+constexpr uint32_t kMaxLeafFirst = (1u << 27) - 1u;
+
+// Intersection record, 48 B, in BVH order (TriangleIntersection, Triangle.hpp:8-17):
+//   v0.xyz | original prim id,  e1.xyz | queue id,  e2.xyz | material id
+struct alignas(16) TriRecord {
+    float v0[3];
+    uint32_t prim;
+    float e1[3];
+    uint32_t queue;
+    float e2[3];
+    uint32_t material;
+};
+static_assert(sizeof(TriRecord) == 48, "triangle record is 48 B");
+
+// Shading record, 112 B, by original prim id (TriangleShading, Triangle.hpp:19-31,
+// without the UVs: every material here is constant).
+struct alignas(16) ShadeRecord {
+    float n[3][4];      // corner normals (w unused)
+    float t[3][4];      // corner tangents, w = handedness
+    float gn[3];        // geometric normal normalize(cross(e1, e2))
+    uint32_t material;
+};
+static_assert(sizeof(ShadeRecord) == 112, "shading record is 112 B");
+
+// Constant material, 80 B.
+struct alignas(16) MaterialRecord {
+    float color[3];
+    uint32_t queue;
+    float specular[3];
+    float alpha;        // max(roughness^2, 1e-3), MaterialEval.cu:860
+    float metallic;
+    float ior;
+    uint32_t model;
+    float pad0;
+    float emission[3];
+    float pad1;
+    float nts[3];       // normalize(normal_ts), the decoded normal-map constant
+    float pad2;
+};
+static_assert(sizeof(MaterialRecord) == 80, "material record is 80 B");
+
+// Emissive triangle, 80 B (LightTriangle, Triangle.hpp:40-58, constant emission).
+struct alignas(16) LightRecord {
+    float v0[3];
+    float area;
+    float e1[3];
+    uint32_t prim;      // original prim id (LightTriangle.index)
+    float e2[3];
+    float pad0;
+    float gn[3];
+    float pad1;
+    float emission[3];
+    float pad2;
+};
+static_assert(sizeof(LightRecord) == 80, "light record is 80 B");
+
+struct AliasBin {
+    float q, p;
+    uint32_t alias;
+    uint32_t value;
+};
+
+// Per-path flags in meta.y: bits 0..7 depth, bit 31 = last scatter was delta.
+constexpr uint32_t kDeltaBit = 0x80000000u;
+
+// ---------------------------------------------------------------- PCG32
+// PCG-XSH-RR 64/32 (O'Neill 2014, pcg-random.org).  A sample owns the stream
+// `sample index`; draw n of that sample is the n-th output of
+// pcg32_srandom(seed, sample).  Jump-ahead is Brown's arbitrary-stride LCG
+// advance (1994), so any (sample, draw) pair is addressable without state.
+constexpr uint64_t kPcgMult = 6364136223846793005ull;
+
+PT_HD uint32_t pcg_output(uint64_t old) {
+    uint32_t xorshifted = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xorshifted >> rot) | (xorshifted << ((0u - rot) & 31u));
+}
+
+struct PcgStream {
+    uint64_t state;
+    uint64_t inc;
+};
+
+PT_HD PcgStream pcg_seed(uint64_t initstate, uint64_t initseq) {
+    PcgStream s;
+    s.state = 0u;
+    s.inc = (initseq << 1u) | 1u;
+    s.state = s.state * kPcgMult + s.inc;
+    s.state += initstate;
+    s.state = s.state * kPcgMult + s.inc;
+    return s;
+}
+
+// Affine map of an n-step advance with unit increment: state' = mult*state + plus*inc.
+struct PcgJump {
+    uint64_t mult;
+    uint64_t plus;
+};
+
+PT_HD PcgJump pcg_jump(uint64_t delta) {
+    uint64_t cur_mult = kPcgMult, cur_plus = 1u, acc_mult = 1u, acc_plus = 0u;
+    while (delta > 0u) {
+        if (delta & 1u) {
+            acc_mult *= cur_mult;
+            acc_plus = acc_plus * cur_mult + cur_plus;
+        }
+        cur_plus = (cur_mult + 1u) * cur_plus;
+        cur_mult *= cur_mult;
+        delta >>= 1u;
+    }
+    return PcgJump{acc_mult, acc_plus};
+}
+
+PT_HD float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+
+// Draw layout of one sample: draws 0,1 are the camera jitter; bounce b owns
+// draws 2+8b .. 9+8b: slot 0,1 direction, 2 lobe, 3 Russian roulette,
+// 4 light pick, 5,6 point on the light, 7 spare.
+constexpr uint32_t kDrawsPerBounce = 8;
+PT_HD uint64_t bounce_draw_base(uint32_t bounce) { return 2u + (uint64_t)kDrawsPerBounce * bounce; }
+
+// ---------------------------------------------------------------- camera
+struct CameraBasis {           // DeviceCamera, DeviceCamera.hpp:5-11
+    float origin[3];
+    float llc[3];
+    float horizontal[3];
+    float vertical[3];
+};
+
+// Frame constants in device memory, read by every kernel of the bounce loop so
+// that one captured graph serves every frame.
+struct FrameParams {
+    CameraBasis cam;
+    uint32_t width, height, spp, spp_axis;
+    uint32_t depth_limit, use_nee, rr_start, pad0;
+    uint64_t seed;
+    uint64_t total_samples;
+    float inv_spp;
+    uint32_t pad1[3];
+    PcgJump bounce_jump[kMaxDepthLimit + 1];   // advance to draw 2+8b
+};
+
+// Counters of the wavefront, one block of device memory.
+struct Counters {
+    uint32_t ray[2];
+    uint32_t queue[kQueueCount + 1];   // diffuse, ggx, mirror, emissive, regen
+    uint32_t shadow;
+    uint32_t work_closest;
+    uint32_t work_shadow;
+    uint32_t active;            // ray count of the next iteration
+    uint32_t iterations;
+    uint32_t invalid_pixels;
+    uint32_t stack_overflow;
+    uint32_t pad[2];
+    unsigned long long launched;
+    unsigned long long ext_rays;
+    unsigned long long shadow_rays;
+    unsigned long long closest_nodes;
+    unsigned long long closest_tris;
+    unsigned long long shadow_nodes;
+    unsigned long long shadow_tris;
+};
+
+}  // namespace pt

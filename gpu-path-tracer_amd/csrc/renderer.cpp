@@ -1,0 +1,852 @@
+#include "renderer.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <string>
+
+namespace pt {
+
+// PathTracerDefaults.hpp:9-35
+constexpr uint32_t kDefaultPool = 1u << 20;
+constexpr uint32_t kMinWidth = 1, kMaxWidth = 7680, kMinHeight = 1, kMaxHeight = 4320;
+constexpr uint32_t kMinSpp = 1, kMaxSpp = 10000000;
+constexpr uint32_t kMinDepth = 1, kMaxDepth = 100;
+constexpr uint32_t kDefaultRrStart = 3;
+
+hipError_t DeviceBuffer::allocate(size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (ptr_ && bytes_ >= bytes) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&ptr_, bytes);
+    if (e != hipSuccess) {
+        ptr_ = nullptr;
+        return e;
+    }
+    bytes_ = bytes;
+    return hipSuccess;
+}
+
+void DeviceBuffer::release() {
+    if (ptr_) (void)hipFree(ptr_);
+    ptr_ = nullptr;
+    bytes_ = 0;
+}
+
+Renderer::Renderer() {
+    tuning_.trace_block = 128;
+    tuning_.trace_blocks_per_cu = 0;   // 0 = occupancy
+    tuning_.shade_block = 256;
+    tuning_.pool_size = kDefaultPool;
+    tuning_.use_graph = 1;
+}
+
+Renderer::~Renderer() {
+    (void)StopRendering();
+    invalidate_graph();
+    for (auto& e : ring_events_)
+        if (e) (void)hipEventDestroy(e);
+    if (h_fp_) (void)hipHostFree(h_fp_);
+    if (h_ctr_ring_) (void)hipHostFree(h_ctr_ring_);
+    if (h_frame_staging_) (void)hipHostFree(h_frame_staging_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+pt_status Renderer::check_hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return PT_OK;
+    const pt_status s = (e == hipErrorOutOfMemory) ? PT_ERROR_OUT_OF_MEMORY : PT_ERROR_DEVICE;
+    return fail(s, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define PT_TRY_HIP(expr, what)                              \
+    do {                                                    \
+        pt_status _s = check_hip((expr), (what));           \
+        if (_s != PT_OK) return _s;                         \
+    } while (0)
+
+#define PT_TRY(expr)                                        \
+    do {                                                    \
+        pt_status _s = (expr);                              \
+        if (_s != PT_OK) return _s;                         \
+    } while (0)
+
+pt_status Renderer::init() {
+    int count = 0;
+    PT_TRY_HIP(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (count < 1) return fail(PT_ERROR_DEVICE, "no HIP device is visible");
+    PT_TRY_HIP(hipGetDevice(&device_), "hipGetDevice");
+    PT_TRY_HIP(hipDeviceGetAttribute(&cu_count_, hipDeviceAttributeMultiprocessorCount, device_), "CU count");
+    PT_TRY_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    PT_TRY_HIP(hipHostMalloc((void**)&h_fp_, sizeof(FrameParams), hipHostMallocDefault), "hipHostMalloc");
+    PT_TRY_HIP(hipHostMalloc((void**)&h_ctr_ring_, sizeof(Counters) * kRing, hipHostMallocDefault), "hipHostMalloc");
+    for (auto& e : ring_events_) PT_TRY_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    PT_TRY_HIP(d_ctr_.allocate(sizeof(Counters)), "counters");
+    PT_TRY_HIP(d_fp_.allocate(sizeof(FrameParams)), "frame params");
+    PT_TRY_HIP(d_work_.allocate(64), "work counter");
+    PT_TRY_HIP(d_overflow_.allocate(64), "overflow flag");
+    PT_TRY_HIP(hipMemsetAsync(d_ctr_.as<void>(), 0, sizeof(Counters), stream_), "memset");
+    PT_TRY_HIP(hipMemsetAsync(d_overflow_.as<void>(), 0, 64, stream_), "memset");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "sync");
+    return PT_OK;
+}
+
+void Renderer::invalidate_graph() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+}
+
+PathState Renderer::path_state() const {
+    PathState p{};
+    p.ray_o = d_ray_o_.as<float4>();
+    p.ray_d = d_ray_d_.as<float4>();
+    p.hit = d_hit_.as<float4>();
+    p.thr = d_thr_.as<float4>();
+    p.meta = d_meta_.as<uint4>();
+    p.sh_o = d_sh_o_.as<float4>();
+    p.sh_d = d_sh_d_.as<float4>();
+    p.sh_L = d_sh_L_.as<float4>();
+    p.rayq[0] = d_rayq_[0].as<uint32_t>();
+    p.rayq[1] = d_rayq_[1].as<uint32_t>();
+    for (int i = 0; i < kQueueSlots; i++) p.q[i] = d_q_[i].as<uint32_t>();
+    p.shadowq = d_shadowq_.as<uint32_t>();
+    p.ctr = d_ctr_.as<Counters>();
+    p.fp = d_fp_.as<FrameParams>();
+    p.accum = d_accum_.as<float4>();
+    p.framebuffer = d_frame_.as<uint32_t>();
+    p.capacity = pool_capacity_;
+    p.pixels = width_ * height_;
+    return p;
+}
+
+SceneDev Renderer::scene_dev() const {
+    SceneDev s{};
+    s.nodes = d_nodes_.as<Bvh4Node>();
+    s.tris = d_tris_.as<TriRecord>();
+    s.shade = d_shade_.as<ShadeRecord>();
+    s.materials = d_materials_.as<MaterialRecord>();
+    s.lights = d_lights_.as<LightRecord>();
+    s.bins = d_bins_.as<AliasBin>();
+    s.light_count = (uint32_t)host_.lights.size();
+    s.tri_count = host_.tri_count;
+    return s;
+}
+
+pt_status Renderer::ensure_pool() {
+    const uint32_t P = tuning_.pool_size;
+    if (P == pool_capacity_ && d_ray_o_.bytes() > 0) return PT_OK;
+    invalidate_graph();
+    const size_t f4 = sizeof(float4) * (size_t)P, u4 = sizeof(uint4) * (size_t)P, u1 = sizeof(uint32_t) * (size_t)P;
+    DeviceBuffer* f4s[] = {&d_ray_o_, &d_ray_d_, &d_hit_, &d_thr_, &d_sh_o_, &d_sh_d_, &d_sh_L_};
+    for (DeviceBuffer* b : f4s) {
+        b->release();
+        PT_TRY_HIP(b->allocate(f4), "path pool allocation");
+    }
+    d_meta_.release();
+    PT_TRY_HIP(d_meta_.allocate(u4), "path pool allocation");
+    for (auto& b : d_rayq_) {
+        b.release();
+        PT_TRY_HIP(b.allocate(u1), "ray queue allocation");
+    }
+    for (auto& b : d_q_) {
+        b.release();
+        PT_TRY_HIP(b.allocate(u1), "material queue allocation");
+    }
+    d_shadowq_.release();
+    PT_TRY_HIP(d_shadowq_.allocate(u1), "shadow queue allocation");
+    pool_capacity_ = P;
+    return PT_OK;
+}
+
+pt_status Renderer::ensure_pixels(uint32_t w, uint32_t h) {
+    if (w == width_ && h == height_ && d_accum_.bytes() > 0) return PT_OK;
+    invalidate_graph();
+    const size_t n = (size_t)w * h;
+    d_accum_.release();
+    d_frame_.release();
+    PT_TRY_HIP(d_accum_.allocate(n * sizeof(float4)), "accumulation buffer");
+    PT_TRY_HIP(d_frame_.allocate(n * sizeof(uint32_t)), "framebuffer");
+    if (h_frame_staging_pixels_ < n) {
+        if (h_frame_staging_) (void)hipHostFree(h_frame_staging_);
+        h_frame_staging_ = nullptr;
+        PT_TRY_HIP(hipHostMalloc((void**)&h_frame_staging_, n * sizeof(uint32_t), hipHostMallocDefault), "pinned framebuffer");
+        h_frame_staging_pixels_ = n;
+    }
+    PT_TRY_HIP(hipMemsetAsync(d_accum_.as<void>(), 0, n * sizeof(float4), stream_), "memset");
+    PT_TRY_HIP(hipMemsetAsync(d_frame_.as<void>(), 0, n * sizeof(uint32_t), stream_), "memset");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "sync");
+    {
+        std::scoped_lock lock(frame_mutex_);
+        frame_.assign(n, 0u);
+    }
+    width_ = w;
+    height_ = h;
+    return PT_OK;
+}
+
+pt_status Renderer::configure_trace() {
+    invalidate_graph();
+    TraceConfig cfg;
+    cfg.block = tuning_.trace_block;
+    cfg.lds_stack = cfg.block >= 256 ? 8 : 16;
+    if (!trace_config_supported(cfg.block, cfg.lds_stack))
+        return fail(PT_ERROR_INVALID_ARGUMENT, "unsupported traversal block size " + std::to_string(cfg.block));
+    int occ_closest = trace_occupancy(cfg, false, false);
+    int occ_shadow = trace_occupancy(cfg, true, false);
+    int occ = std::max(1, std::min(occ_closest, occ_shadow));
+    if (tuning_.trace_blocks_per_cu) occ = (int)tuning_.trace_blocks_per_cu;
+    cfg.blocks = (uint32_t)(occ * std::max(1, cu_count_));
+    // worst-case stack: a BVH4 level pushes at most three siblings
+    const uint32_t depth = has_scene_ ? host_.bvh.depth : 1u;
+    stack_total_ = 3u * depth + 2u;
+    cfg.spill_depth = stack_total_ > cfg.lds_stack ? stack_total_ - cfg.lds_stack : 0u;
+    const size_t spill_bytes = (size_t)cfg.spill_depth * cfg.blocks * cfg.block * sizeof(uint2);
+    PT_TRY_HIP(d_spill_.allocate(spill_bytes), "traversal stack spill area");
+    cfg.spill = d_spill_.as<uint2>();
+    cfg.overflow = d_overflow_.as<unsigned int>();
+    trace_ = cfg;
+    return PT_OK;
+}
+
+pt_status Renderer::InitializeRenderingBuffers(uint32_t width, uint32_t height) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    width = std::min(std::max(width, kMinWidth), kMaxWidth);
+    height = std::min(std::max(height, kMinHeight), kMaxHeight);
+    PT_TRY(ensure_pool());
+    PT_TRY(ensure_pixels(width, height));
+    return PT_OK;
+}
+
+pt_status Renderer::InitializeSceneBuffers(const pt_scene_desc& desc) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    BvhBuildOptions bo;
+    HostScene hs;
+    SceneError err{PT_OK, ""};
+    if (!build_host_scene(desc, bo, hs, err)) return fail(err.status, err.message);
+    invalidate_graph();
+    has_scene_ = false;
+    host_ = std::move(hs);
+    auto upload = [&](DeviceBuffer& b, const void* src, size_t bytes, const char* what) -> pt_status {
+        b.release();
+        PT_TRY_HIP(b.allocate(bytes), what);
+        if (bytes) PT_TRY_HIP(hipMemcpyAsync(b.as<void>(), src, bytes, hipMemcpyHostToDevice, stream_), what);
+        return PT_OK;
+    };
+    PT_TRY(upload(d_nodes_, host_.bvh.nodes.data(), host_.bvh.nodes.size() * sizeof(Bvh4Node), "BVH nodes"));
+    PT_TRY(upload(d_tris_, host_.tris.data(), host_.tris.size() * sizeof(TriRecord), "triangle records"));
+    PT_TRY(upload(d_shade_, host_.shade.data(), host_.shade.size() * sizeof(ShadeRecord), "shading records"));
+    PT_TRY(upload(d_materials_, host_.materials.data(), host_.materials.size() * sizeof(MaterialRecord), "materials"));
+    PT_TRY(upload(d_lights_, host_.lights.data(), host_.lights.size() * sizeof(LightRecord), "lights"));
+    PT_TRY(upload(d_bins_, host_.bins.data(), host_.bins.size() * sizeof(AliasBin), "alias table"));
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "scene upload");
+    has_scene_ = true;
+    PT_TRY(configure_trace());
+    return PT_OK;
+}
+
+pt_status Renderer::Clear(uint8_t r, uint8_t g, uint8_t b, uint8_t a) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Cannot clear renderer while rendering");
+    const uint32_t c = (uint32_t)r | ((uint32_t)g << 8) | ((uint32_t)b << 16) | ((uint32_t)a << 24);
+    std::scoped_lock lock(frame_mutex_);
+    std::fill(frame_.begin(), frame_.end(), c);
+    return PT_OK;
+}
+
+pt_status Renderer::launch_iteration(int cur, bool count) {
+    const PathState p = path_state();
+    const SceneDev sc = scene_dev();
+    const uint32_t grid = (uint32_t)std::max(1, cu_count_) * 8u;
+    PT_TRY_HIP(launch_closest(stream_, p, sc, cur, count, trace_), "closest-hit launch");
+    PT_TRY_HIP(launch_shade(stream_, p, sc, cur, grid, 256), "shade launch");
+    PT_TRY_HIP(launch_shadow(stream_, p, sc, count, trace_), "shadow launch");
+    PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
+    PT_TRY_HIP(launch_prepare(stream_, p, cur), "prepare launch");
+    return PT_OK;
+}
+
+pt_status Renderer::ensure_graph() {
+    if (graph_exec_) return PT_OK;
+    PT_TRY_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "graph capture");
+    pt_status s = launch_iteration(0, false);
+    if (s == PT_OK) s = launch_iteration(1, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(stream_, &g);
+    if (s != PT_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return s;
+    }
+    PT_TRY_HIP(e, "graph capture");
+    graph_ = g;
+    PT_TRY_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "graph instantiate");
+    return PT_OK;
+}
+
+namespace {
+struct ClampedSettings {
+    uint32_t width, height, spp, axis, depth, rr;
+};
+ClampedSettings clamp_settings(const pt_render_settings& s) {
+    ClampedSettings c;
+    c.width = std::min(std::max(s.width, kMinWidth), kMaxWidth);
+    c.height = std::min(std::max(s.height, kMinHeight), kMaxHeight);
+    const uint32_t spp = std::min(std::max(s.samples_per_pixel, kMinSpp), kMaxSpp);
+    c.depth = std::min(std::max(s.path_depth, kMinDepth), kMaxDepth);
+    c.axis = (uint32_t)std::sqrt((double)spp);   // Renderer.cpp:366-367
+    while ((uint64_t)(c.axis + 1) * (c.axis + 1) <= spp) c.axis++;
+    while ((uint64_t)c.axis * c.axis > spp) c.axis--;
+    c.spp = c.axis * c.axis;
+    c.rr = s.rr_start_depth ? s.rr_start_depth : kDefaultRrStart;
+    return c;
+}
+}  // namespace
+
+pt_status Renderer::render_frame(const pt_render_settings& s, const pt_camera& camera, const pt_motion_state& state,
+                                 uint32_t flags, pt_frame_stats* stats, const std::atomic<bool>* stop) {
+    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    const ClampedSettings c = clamp_settings(s);
+    PT_TRY(ensure_pool());
+    PT_TRY(ensure_pixels(c.width, c.height));
+
+    FrameParams& fp = *h_fp_;
+    std::memset(&fp, 0, sizeof(fp));
+    fp.cam = make_camera_basis(camera, (float)c.width / (float)c.height, state);
+    fp.width = c.width;
+    fp.height = c.height;
+    fp.spp = c.spp;
+    fp.spp_axis = c.axis;
+    fp.depth_limit = c.depth;
+    fp.use_nee = s.use_nee ? 1u : 0u;
+    fp.rr_start = c.rr;
+    fp.seed = s.seed;
+    fp.total_samples = (uint64_t)c.spp * c.width * c.height;
+    fp.inv_spp = 1.0f / (float)c.spp;
+    for (uint32_t b = 0; b <= kMaxDepthLimit; b++) fp.bounce_jump[b] = pcg_jump(bounce_draw_base(b));
+    last_spp_ = c.spp;
+
+    const bool profile = (flags & PT_FRAME_PROFILE) != 0;
+    const bool count = (flags & PT_FRAME_INSTRUMENT) != 0;
+    const bool use_graph = tuning_.use_graph && !profile && !count && !(flags & PT_FRAME_NO_GRAPH);
+    if (use_graph) PT_TRY(ensure_graph());
+
+    hipEvent_t t_begin = nullptr, t_end = nullptr;
+    PT_TRY_HIP(hipEventCreate(&t_begin), "event");
+    PT_TRY_HIP(hipEventCreate(&t_end), "event");
+    struct EventGuard {
+        hipEvent_t a, b;
+        ~EventGuard() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } guard{t_begin, t_end};
+
+    const PathState p = path_state();
+    const SceneDev sc = scene_dev();
+    const uint32_t grid = (uint32_t)std::max(1, cu_count_) * 8u;
+    PT_TRY_HIP(hipMemcpyAsync(d_fp_.as<void>(), h_fp_, sizeof(FrameParams), hipMemcpyHostToDevice, stream_), "frame params");
+    PT_TRY_HIP(hipEventRecord(t_begin, stream_), "event");
+    PT_TRY_HIP(launch_frame_begin(stream_, p), "frame begin");
+    PT_TRY_HIP(launch_fill_accum(stream_, p), "clear accumulation");
+    PT_TRY_HIP(launch_generate(stream_, p, true, 0, grid, 256), "initialize paths");
+
+    // Every path ends within `depth` iterations and slots are refilled at once,
+    // so the loop needs at most (ceil(total/P)+1)*depth iterations.
+    const uint64_t waves = (fp.total_samples + pool_capacity_ - 1) / pool_capacity_ + 1;
+    const uint64_t max_iters = 2 * waves * (uint64_t)(c.depth + 1) + 16;
+
+    double closest_ms = 0, shadow_ms = 0, shade_ms = 0, regen_ms = 0;
+    uint32_t closest_launches = 0;
+    if (profile) {
+        hipEvent_t ev[6];
+        for (auto& e : ev) PT_TRY_HIP(hipEventCreate(&e), "event");
+        struct Guard6 {
+            hipEvent_t* e;
+            ~Guard6() {
+                for (int i = 0; i < 6; i++) (void)hipEventDestroy(e[i]);
+            }
+        } g6{ev};
+        int cur = 0;
+        for (uint64_t it = 0;; it++) {
+            if (it > max_iters) return fail(PT_ERROR_DEVICE, "wavefront did not terminate");
+            PT_TRY_HIP(hipEventRecord(ev[0], stream_), "event");
+            PT_TRY_HIP(launch_closest(stream_, p, sc, cur, false, trace_), "closest-hit launch");
+            PT_TRY_HIP(hipEventRecord(ev[1], stream_), "event");
+            PT_TRY_HIP(launch_shade(stream_, p, sc, cur, grid, 256), "shade launch");
+            PT_TRY_HIP(hipEventRecord(ev[2], stream_), "event");
+            PT_TRY_HIP(launch_shadow(stream_, p, sc, false, trace_), "shadow launch");
+            PT_TRY_HIP(hipEventRecord(ev[3], stream_), "event");
+            PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
+            PT_TRY_HIP(hipEventRecord(ev[4], stream_), "event");
+            PT_TRY_HIP(launch_prepare(stream_, p, cur), "prepare launch");
+            PT_TRY_HIP(hipMemcpyAsync(h_ctr_ring_, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream_), "counters");
+            PT_TRY_HIP(hipEventRecord(ev[5], stream_), "event");
+            PT_TRY_HIP(hipEventSynchronize(ev[5]), "sync");
+            float ms[4];
+            for (int k = 0; k < 4; k++) PT_TRY_HIP(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]), "elapsed");
+            closest_ms += ms[0];
+            shade_ms += ms[1];
+            shadow_ms += ms[2];
+            regen_ms += ms[3];
+            closest_launches++;
+            done_samples_.store(h_ctr_ring_[0].launched, std::memory_order_relaxed);
+            cur ^= 1;
+            if (h_ctr_ring_[0].active == 0) break;
+            if (stop && stop->load()) return PT_OK;
+        }
+    } else {
+        const int lag = 2;
+        int cur = 0;
+        for (uint64_t k = 0;; k++) {
+            if (k * (use_graph ? 2 : 1) > max_iters) return fail(PT_ERROR_DEVICE, "wavefront did not terminate");
+            if (use_graph) {
+                PT_TRY_HIP(hipGraphLaunch(graph_exec_, stream_), "graph launch");
+            } else {
+                PT_TRY(launch_iteration(cur, count));
+                cur ^= 1;
+            }
+            const int slot = (int)(k % kRing);
+            PT_TRY_HIP(hipMemcpyAsync(h_ctr_ring_ + slot, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream_), "counters");
+            PT_TRY_HIP(hipEventRecord(ring_events_[slot], stream_), "event");
+            if (k >= (uint64_t)lag) {
+                const int old = (int)((k - lag) % kRing);
+                PT_TRY_HIP(hipEventSynchronize(ring_events_[old]), "sync");
+                done_samples_.store(h_ctr_ring_[old].launched, std::memory_order_relaxed);
+                if (h_ctr_ring_[old].active == 0) break;
+            }
+            if (stop && stop->load()) {
+                PT_TRY_HIP(hipStreamSynchronize(stream_), "sync");
+                return PT_OK;
+            }
+        }
+    }
+    PT_TRY_HIP(launch_postprocess(stream_, p), "postprocess");
+    PT_TRY_HIP(hipEventRecord(t_end, stream_), "event");
+    PT_TRY_HIP(hipMemcpyAsync(h_ctr_ring_, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream_), "counters");
+    const size_t npix = (size_t)c.width * c.height;
+    PT_TRY_HIP(hipMemcpyAsync(h_frame_staging_, p.framebuffer, npix * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "framebuffer");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "frame sync");
+    {
+        std::scoped_lock lock(frame_mutex_);
+        frame_.assign(h_frame_staging_, h_frame_staging_ + npix);
+    }
+    const Counters& ctr = h_ctr_ring_[0];
+    if (ctr.stack_overflow) return fail(PT_ERROR_DEVICE, "traversal stack overflow");
+    float total_ms = 0;
+    PT_TRY_HIP(hipEventElapsedTime(&total_ms, t_begin, t_end), "elapsed");
+    done_samples_.store(ctr.launched, std::memory_order_relaxed);
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->render_ms = total_ms;
+        stats->extension_rays = ctr.ext_rays;
+        stats->shadow_rays = ctr.shadow_rays;
+        stats->samples = ctr.launched;
+        stats->iterations = ctr.iterations;
+        stats->closest_launches = profile ? closest_launches : ctr.iterations;
+        stats->closest_ms = closest_ms;
+        stats->shadow_ms = shadow_ms;
+        stats->shade_ms = shade_ms;
+        stats->regen_ms = regen_ms;
+        stats->closest_nodes = ctr.closest_nodes;
+        stats->closest_tris = ctr.closest_tris;
+        stats->shadow_nodes = ctr.shadow_nodes;
+        stats->shadow_tris = ctr.shadow_tris;
+    }
+    if (ctr.invalid_pixels) return fail(PT_ERROR_DEVICE, std::to_string(ctr.invalid_pixels) + " pixels hold non-finite or negative radiance");
+    return PT_OK;
+}
+
+pt_status Renderer::RenderFrameSync(const pt_render_settings& s, const pt_camera& camera, const pt_motion_state& state,
+                                    uint32_t flags, pt_frame_stats* stats) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    return render_frame(s, camera, state, flags, stats, nullptr);
+}
+
+// ---------------------------------------------------------------- async rendering
+
+pt_status Renderer::StartSimulation(const pt_render_settings& s, const pt_camera& camera, const pt_motion_state* states,
+                                    uint32_t n, const char* output_folder) {
+    if (is_rendering_.load()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (n > 0 && !states) return fail(PT_ERROR_INVALID_ARGUMENT, "motion states are NULL");
+    if (render_thread_.joinable()) render_thread_.join();
+    {
+        std::scoped_lock lock(error_mutex_);
+        parked_error_.reset();
+    }
+    const ClampedSettings c = clamp_settings(s);
+    PT_TRY(ensure_pool());
+    PT_TRY(ensure_pixels(c.width, c.height));
+    settings_ = s;
+    camera_ = camera;
+    motion_.assign(states, states + n);
+    output_folder_ = output_folder ? output_folder : "";
+    total_frames_ = n;
+    total_samples_ = (uint64_t)c.spp * c.width * c.height;
+    done_frames_.store(0);
+    done_samples_.store(0);
+    stop_requested_.store(false);
+    is_rendering_.store(true);
+    try {
+        render_thread_ = std::thread([this]() { simulation_loop(0); });
+    } catch (...) {
+        is_rendering_.store(false);
+        return fail(PT_ERROR_INVALID_STATE, "Failed to start render thread");
+    }
+    return PT_OK;
+}
+
+pt_status Renderer::ResumeSimulation(uint32_t start_frame) {
+    if (is_rendering_.load()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    if (start_frame >= motion_.size())
+        return fail(PT_ERROR_OUT_OF_RANGE, "Start frame is out of range - nothing to render");
+    if (render_thread_.joinable()) render_thread_.join();
+    done_frames_.store(start_frame);
+    done_samples_.store(0);
+    stop_requested_.store(false);
+    is_rendering_.store(true);
+    try {
+        render_thread_ = std::thread([this, start_frame]() { simulation_loop(start_frame); });
+    } catch (...) {
+        is_rendering_.store(false);
+        return fail(PT_ERROR_INVALID_STATE, "Failed to start render thread");
+    }
+    return PT_OK;
+}
+
+pt_status Renderer::StopRendering() {
+    if (render_thread_.joinable()) {
+        stop_requested_.store(true);
+        render_thread_.join();
+    }
+    if (stream_) {
+        hipError_t e = hipStreamSynchronize(stream_);
+        if (e != hipSuccess) {
+            is_rendering_.store(false);
+            return check_hip(e, "StopRendering");
+        }
+    }
+    is_rendering_.store(false);
+    return PT_OK;
+}
+
+void Renderer::simulation_loop(uint32_t start_frame) {
+    for (uint32_t f = start_frame; f < total_frames_; f++) {
+        pt_status s = render_frame(settings_, camera_, motion_[f], 0, nullptr, &stop_requested_);
+        if (s == PT_OK && !stop_requested_.load() && !output_folder_.empty()) {
+            const int digits = (int)std::max<size_t>(4, std::to_string(total_frames_ ? total_frames_ - 1 : 0).size());
+            char name[64];
+            std::snprintf(name, sizeof(name), "frame_%0*u.png", digits, done_frames_.load());
+            s = save_frame_png((std::filesystem::path(output_folder_) / name).string());
+        }
+        if (s != PT_OK) {
+            (void)hipStreamSynchronize(stream_);
+            std::scoped_lock lock(error_mutex_);
+            parked_error_ = last_error_;
+            done_samples_.store(0);
+            break;
+        }
+        if (stop_requested_.load()) break;
+        done_frames_.fetch_add(1);
+    }
+    is_rendering_.store(false);
+}
+
+void Renderer::GetProgress(pt_progress& p) const {
+    std::memset(&p, 0, sizeof(p));
+    p.is_rendering = is_rendering_.load() ? 1u : 0u;
+    p.done_frames = done_frames_.load();
+    p.total_frames = total_frames_;
+    p.width = width_;
+    p.height = height_;
+    p.samples_per_pixel = last_spp_;
+    p.samples_per_pixel_axis = (uint32_t)std::lround(std::sqrt((double)last_spp_));
+    p.path_depth = std::min(std::max(settings_.path_depth, kMinDepth), kMaxDepth);
+    p.done_samples = done_samples_.load();
+    p.total_samples = total_samples_;
+}
+
+pt_status Renderer::CopyFrame(uint32_t* rgba, size_t n) {
+    std::scoped_lock lock(frame_mutex_);
+    if (!rgba || n < frame_.size()) return fail(PT_ERROR_INVALID_ARGUMENT, "frame buffer too small");
+    std::memcpy(rgba, frame_.data(), frame_.size() * sizeof(uint32_t));
+    return PT_OK;
+}
+
+pt_status Renderer::CopyFrameHdr(float* rgba, size_t n) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "HDR readback needs a finished render");
+    const size_t npix = (size_t)width_ * height_;
+    if (!rgba || n < npix) return fail(PT_ERROR_INVALID_ARGUMENT, "HDR buffer too small");
+    PT_TRY_HIP(hipMemcpyAsync(rgba, d_accum_.as<void>(), npix * sizeof(float4), hipMemcpyDeviceToHost, stream_), "HDR readback");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "HDR readback");
+    const float inv = 1.0f / (float)last_spp_;
+    for (size_t i = 0; i < 4 * npix; i++) rgba[i] *= inv;
+    return PT_OK;
+}
+
+static size_t copy_message(const std::string& m, char* buf, size_t cap) {
+    if (buf && cap) {
+        const size_t k = std::min(cap - 1, m.size());
+        std::memcpy(buf, m.data(), k);
+        buf[k] = 0;
+    }
+    return m.size();
+}
+
+size_t Renderer::PeekLastError(char* buf, size_t cap) const {
+    std::scoped_lock lock(error_mutex_);
+    if (!parked_error_) return 0;
+    return copy_message(*parked_error_, buf, cap);
+}
+
+size_t Renderer::ConsumeLastError(char* buf, size_t cap) {
+    std::scoped_lock lock(error_mutex_);
+    if (!parked_error_) return 0;
+    const size_t k = copy_message(*parked_error_, buf, cap);
+    parked_error_.reset();
+    return k;
+}
+
+// ---------------------------------------------------------------- PNG (stored deflate)
+
+namespace {
+uint32_t crc32_update(uint32_t crc, const uint8_t* p, size_t n) {
+    static uint32_t table[256];
+    static bool init = false;
+    if (!init) {
+        for (uint32_t i = 0; i < 256; i++) {
+            uint32_t c = i;
+            for (int k = 0; k < 8; k++) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+            table[i] = c;
+        }
+        init = true;
+    }
+    for (size_t i = 0; i < n; i++) crc = table[(crc ^ p[i]) & 0xFFu] ^ (crc >> 8);
+    return crc;
+}
+void put_be32(std::vector<uint8_t>& v, uint32_t x) {
+    v.push_back((uint8_t)(x >> 24));
+    v.push_back((uint8_t)(x >> 16));
+    v.push_back((uint8_t)(x >> 8));
+    v.push_back((uint8_t)x);
+}
+void chunk(std::vector<uint8_t>& out, const char* type, const std::vector<uint8_t>& data) {
+    put_be32(out, (uint32_t)data.size());
+    std::vector<uint8_t> td(type, type + 4);
+    td.insert(td.end(), data.begin(), data.end());
+    out.insert(out.end(), td.begin(), td.end());
+    put_be32(out, crc32_update(0xFFFFFFFFu, td.data(), td.size()) ^ 0xFFFFFFFFu);
+}
+}  // namespace
+
+pt_status Renderer::save_frame_png(const std::string& path) {
+    std::vector<uint32_t> px;
+    {
+        std::scoped_lock lock(frame_mutex_);
+        px = frame_;
+    }
+    const uint32_t W = width_, H = height_;
+    std::vector<uint8_t> raw;
+    raw.reserve((size_t)H * (1 + 4 * (size_t)W));
+    for (uint32_t row = 0; row < H; row++) {
+        const uint32_t y = H - 1 - row;   // row 0 of the image is the top: flip (Renderer.cpp:672)
+        raw.push_back(0);
+        for (uint32_t x = 0; x < W; x++) {
+            const uint32_t c = px[(size_t)y * W + x];
+            raw.push_back((uint8_t)c);
+            raw.push_back((uint8_t)(c >> 8));
+            raw.push_back((uint8_t)(c >> 16));
+            raw.push_back((uint8_t)(c >> 24));
+        }
+    }
+    std::vector<uint8_t> z = {0x78, 0x01};
+    uint32_t a = 1, b = 0;
+    for (uint8_t v : raw) {
+        a = (a + v) % 65521u;
+        b = (b + a) % 65521u;
+    }
+    size_t off = 0;
+    do {
+        const size_t len = std::min<size_t>(65535, raw.size() - off);
+        const bool last = off + len == raw.size();
+        z.push_back(last ? 1 : 0);
+        z.push_back((uint8_t)len);
+        z.push_back((uint8_t)(len >> 8));
+        z.push_back((uint8_t)~len);
+        z.push_back((uint8_t)(~len >> 8));
+        z.insert(z.end(), raw.begin() + off, raw.begin() + off + len);
+        off += len;
+    } while (off < raw.size());
+    put_be32(z, (b << 16) | a);
+    std::vector<uint8_t> png = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
+    std::vector<uint8_t> ihdr;
+    put_be32(ihdr, W);
+    put_be32(ihdr, H);
+    ihdr.insert(ihdr.end(), {8, 6, 0, 0, 0});
+    chunk(png, "IHDR", ihdr);
+    chunk(png, "IDAT", z);
+    chunk(png, "IEND", {});
+    std::error_code ec;
+    std::filesystem::create_directories(std::filesystem::path(path).parent_path(), ec);
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return fail(PT_ERROR_IO, "cannot open " + path);
+    const size_t wrote = std::fwrite(png.data(), 1, png.size(), f);
+    std::fclose(f);
+    if (wrote != png.size()) return fail(PT_ERROR_IO, "short write to " + path);
+    return PT_OK;
+}
+
+// ---------------------------------------------------------------- hooks
+
+pt_status Renderer::TraceBatch(const pt_ray* rays, uint32_t n, bool any, bool device, void* out, uint32_t* nodes,
+                               uint32_t* tris, hipStream_t stream) {
+    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    if (n == 0) return PT_OK;
+    if (!rays || !out) return fail(PT_ERROR_INVALID_ARGUMENT, "rays and output must not be NULL");
+    hipStream_t s = stream ? stream : stream_;
+    const size_t in_bytes = (size_t)n * sizeof(pt_ray);
+    const size_t out_elem = any ? sizeof(uint32_t) : sizeof(pt_hit);
+    const float4* d_in;
+    void* d_out;
+    if (device) {
+        d_in = reinterpret_cast<const float4*>(rays);
+        d_out = out;
+    } else {
+        PT_TRY_HIP(d_hook_in_.allocate(in_bytes), "hook input");
+        PT_TRY_HIP(d_hook_out_.allocate((size_t)n * out_elem), "hook output");
+        PT_TRY_HIP(hipMemcpyAsync(d_hook_in_.as<void>(), rays, in_bytes, hipMemcpyHostToDevice, s), "hook input");
+        d_in = d_hook_in_.as<float4>();
+        d_out = d_hook_out_.as<void>();
+    }
+    uint32_t* d_nodes = nullptr;
+    uint32_t* d_tris = nullptr;
+    if (nodes || tris) {
+        PT_TRY_HIP(d_hook_aux_.allocate((size_t)n * 2 * sizeof(uint32_t)), "hook counts");
+        d_nodes = d_hook_aux_.as<uint32_t>();
+        d_tris = d_nodes + n;
+    }
+    PT_TRY_HIP(launch_zero_u32(s, d_work_.as<unsigned int>(), 1), "zero");
+    PT_TRY_HIP(launch_trace_batch(s, scene_dev(), d_in, n, any ? nullptr : static_cast<float4*>(d_out),
+                                  any ? static_cast<uint32_t*>(d_out) : nullptr, d_nodes, d_tris,
+                                  d_work_.as<unsigned int>(), any, trace_),
+               "trace launch");
+    if (!device) PT_TRY_HIP(hipMemcpyAsync(out, d_out, (size_t)n * out_elem, hipMemcpyDeviceToHost, s), "hook output");
+    if (nodes) PT_TRY_HIP(hipMemcpyAsync(nodes, d_nodes, (size_t)n * 4, hipMemcpyDeviceToHost, s), "hook counts");
+    if (tris) PT_TRY_HIP(hipMemcpyAsync(tris, d_tris, (size_t)n * 4, hipMemcpyDeviceToHost, s), "hook counts");
+    if (!device || nodes || tris) PT_TRY_HIP(hipStreamSynchronize(s), "hook sync");
+    return PT_OK;
+}
+
+pt_status Renderer::Classify(const pt_ray* rays, uint32_t n, uint32_t* queue_of_ray, uint32_t* counts) {
+    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    if (!rays || !queue_of_ray || !counts) return fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
+    for (int q = 0; q < kQueueSlots; q++) counts[q] = 0;
+    if (n == 0) return PT_OK;
+    PT_TRY_HIP(d_hook_in_.allocate((size_t)n * sizeof(pt_ray)), "hook input");
+    PT_TRY_HIP(d_hook_q_.allocate((size_t)n * kQueueSlots * sizeof(uint32_t) + 64), "hook queues");
+    PT_TRY_HIP(hipMemcpyAsync(d_hook_in_.as<void>(), rays, (size_t)n * sizeof(pt_ray), hipMemcpyHostToDevice, stream_), "hook input");
+    uint32_t* qbase = d_hook_q_.as<uint32_t>();
+    uint32_t* queues[kQueueSlots];
+    for (int q = 0; q < kQueueSlots; q++) queues[q] = qbase + (size_t)q * n;
+    unsigned int* qcounts = reinterpret_cast<unsigned int*>(qbase + (size_t)kQueueSlots * n);
+    PT_TRY_HIP(launch_zero_u32(stream_, d_work_.as<unsigned int>(), 1), "zero");
+    PT_TRY_HIP(launch_zero_u32(stream_, qcounts, kQueueSlots), "zero");
+    PT_TRY_HIP(launch_classify(stream_, scene_dev(), d_hook_in_.as<float4>(), n, queues, qcounts, d_work_.as<unsigned int>(), trace_),
+               "classify launch");
+    std::vector<uint32_t> host((size_t)n * kQueueSlots + 16);
+    PT_TRY_HIP(hipMemcpyAsync(host.data(), qbase, ((size_t)n * kQueueSlots + kQueueSlots) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "hook queues");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "hook sync");
+    std::vector<uint8_t> seen(n, 0);
+    for (uint32_t i = 0; i < n; i++) queue_of_ray[i] = PT_INVALID_INDEX;
+    uint64_t total = 0;
+    for (int q = 0; q < kQueueSlots; q++) {
+        const uint32_t cnt = host[(size_t)kQueueSlots * n + q];
+        counts[q] = cnt;
+        total += cnt;
+        if (cnt > n) return fail(PT_ERROR_DEVICE, "queue count exceeds the batch");
+        for (uint32_t k = 0; k < cnt; k++) {
+            const uint32_t ray = host[(size_t)q * n + k];
+            if (ray >= n || seen[ray]) return fail(PT_ERROR_DEVICE, "queue holds a ray twice or out of range");
+            seen[ray] = 1;
+            queue_of_ray[ray] = (uint32_t)q;
+        }
+    }
+    if (total != n) return fail(PT_ERROR_DEVICE, "queues do not partition the batch");
+    return PT_OK;
+}
+
+pt_status Renderer::AliasSample(const float* u, uint32_t n, uint32_t* index, float* pmf) {
+    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (host_.lights.empty()) return fail(PT_ERROR_INVALID_STATE, "scene has no emissive triangles");
+    if (n == 0) return PT_OK;
+    if (!u || !index || !pmf) return fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
+    PT_TRY_HIP(d_hook_in_.allocate((size_t)n * 4), "hook input");
+    PT_TRY_HIP(d_hook_out_.allocate((size_t)n * 8), "hook output");
+    PT_TRY_HIP(hipMemcpyAsync(d_hook_in_.as<void>(), u, (size_t)n * 4, hipMemcpyHostToDevice, stream_), "hook input");
+    uint32_t* d_idx = d_hook_out_.as<uint32_t>();
+    float* d_pmf = reinterpret_cast<float*>(d_idx + n);
+    PT_TRY_HIP(launch_alias_sample(stream_, scene_dev(), d_hook_in_.as<float>(), n, d_idx, d_pmf), "alias launch");
+    PT_TRY_HIP(hipMemcpyAsync(index, d_idx, (size_t)n * 4, hipMemcpyDeviceToHost, stream_), "hook output");
+    PT_TRY_HIP(hipMemcpyAsync(pmf, d_pmf, (size_t)n * 4, hipMemcpyDeviceToHost, stream_), "hook output");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "hook sync");
+    return PT_OK;
+}
+
+pt_status Renderer::GetAliasTable(pt_alias_bin* bins, uint32_t cap, uint32_t* count) {
+    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (count) *count = (uint32_t)host_.bins.size();
+    if (!bins) return PT_OK;
+    if (cap < host_.bins.size()) return fail(PT_ERROR_INVALID_ARGUMENT, "capacity too small");
+    for (size_t i = 0; i < host_.bins.size(); i++)
+        bins[i] = pt_alias_bin{host_.bins[i].q, host_.bins[i].p, host_.bins[i].alias, host_.bins[i].value};
+    return PT_OK;
+}
+
+pt_status Renderer::GetBvhInfo(pt_bvh_info& info) {
+    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    info.node_count = (uint32_t)host_.bvh.nodes.size();
+    info.leaf_count = host_.bvh.leaf_count;
+    info.max_depth = host_.bvh.depth;
+    info.max_leaf_size = host_.bvh.max_leaf;
+    info.triangle_count = host_.tri_count;
+    info.light_count = (uint32_t)host_.lights.size();
+    info.stack_lds = trace_.lds_stack;
+    info.stack_total = trace_.lds_stack + trace_.spill_depth;
+    return PT_OK;
+}
+
+pt_status Renderer::GetBvh4(void* nodes, uint32_t node_cap, uint32_t* order, uint32_t order_cap) {
+    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (nodes) {
+        if (node_cap < host_.bvh.nodes.size()) return fail(PT_ERROR_INVALID_ARGUMENT, "node capacity too small");
+        std::memcpy(nodes, host_.bvh.nodes.data(), host_.bvh.nodes.size() * sizeof(Bvh4Node));
+    }
+    if (order) {
+        if (order_cap < host_.bvh.order.size()) return fail(PT_ERROR_INVALID_ARGUMENT, "order capacity too small");
+        std::memcpy(order, host_.bvh.order.data(), host_.bvh.order.size() * sizeof(uint32_t));
+    }
+    return PT_OK;
+}
+
+pt_status Renderer::GetTuning(pt_tuning& t) const {
+    t = tuning_;
+    return PT_OK;
+}
+
+pt_status Renderer::SetTuning(const pt_tuning& t) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    if (!trace_config_supported(t.trace_block, t.trace_block >= 256 ? 8 : 16))
+        return fail(PT_ERROR_INVALID_ARGUMENT, "trace_block must be 64, 128 or 256");
+    if (t.shade_block != 256) return fail(PT_ERROR_INVALID_ARGUMENT, "shade_block must be 256");
+    if (t.pool_size < 1024 || t.pool_size > (1u << 26)) return fail(PT_ERROR_INVALID_ARGUMENT, "pool_size must be in [1024, 2^26]");
+    tuning_ = t;
+    invalidate_graph();
+    if (d_ray_o_.bytes()) PT_TRY(ensure_pool());
+    if (has_scene_) PT_TRY(configure_trace());
+    return PT_OK;
+}
+
+}  // namespace pt

@@ -1,0 +1,362 @@
+"""ctypes binding of libptcore.so (include/pt_core.h) and a Python mirror of
+the reference Renderer's interface
+(Core/Include/Core/Graphics/Cuda/PathTracing/Renderer.hpp:42-189).
+
+The library is the product: there is no fallback.  Loading fails loudly if
+libptcore.so has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libptcore.so")
+
+PT_OK = 0
+STATUS_NAMES = {0: "PT_OK", 1: "PT_ERROR_INVALID_ARGUMENT", 2: "PT_ERROR_INVALID_STATE", 3: "PT_ERROR_OUT_OF_RANGE",
+                4: "PT_ERROR_DEVICE", 5: "PT_ERROR_OUT_OF_MEMORY", 6: "PT_ERROR_UNSUPPORTED", 7: "PT_ERROR_IO"}
+QUEUE_DIFFUSE, QUEUE_GGX, QUEUE_MIRROR, QUEUE_EMISSIVE, QUEUE_REGEN = range(5)
+INVALID = 0xFFFFFFFF
+FRAME_DEFAULT, FRAME_PROFILE, FRAME_INSTRUMENT, FRAME_NO_GRAPH = 0, 1, 2, 4
+
+
+class PtError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("triangle_count", C.c_uint32), ("positions", C.c_void_p), ("normals", C.c_void_p),
+                ("tangents", C.c_void_p), ("material_ids", C.c_void_p), ("material_count", C.c_uint32),
+                ("materials", C.c_void_p), ("light_weights", C.c_void_p)]
+
+
+class RenderSettings(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("samples_per_pixel", C.c_uint32),
+                ("path_depth", C.c_uint32), ("use_nee", C.c_uint32), ("rr_start_depth", C.c_uint32),
+                ("seed", C.c_uint64)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("fov_y", C.c_float), ("near_plane", C.c_float), ("far_plane", C.c_float)]
+
+
+class MotionState(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("rotation", C.c_float * 4)]
+
+
+class Progress(C.Structure):
+    _fields_ = [("is_rendering", C.c_uint32), ("done_frames", C.c_uint32), ("total_frames", C.c_uint32),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("samples_per_pixel", C.c_uint32),
+                ("samples_per_pixel_axis", C.c_uint32), ("path_depth", C.c_uint32),
+                ("done_samples", C.c_uint64), ("total_samples", C.c_uint64)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("render_ms", C.c_double), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("samples", C.c_uint64), ("iterations", C.c_uint32), ("closest_launches", C.c_uint32),
+                ("closest_ms", C.c_double), ("shadow_ms", C.c_double), ("shade_ms", C.c_double),
+                ("regen_ms", C.c_double), ("closest_nodes", C.c_uint64), ("closest_tris", C.c_uint64),
+                ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class AliasBin(C.Structure):
+    _fields_ = [("q", C.c_float), ("p", C.c_float), ("alias", C.c_uint32), ("value", C.c_uint32)]
+
+
+ALIAS_DTYPE = np.dtype([("q", "<f4"), ("p", "<f4"), ("alias", "<u4"), ("value", "<u4")])
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("node_count", C.c_uint32), ("leaf_count", C.c_uint32), ("max_depth", C.c_uint32),
+                ("max_leaf_size", C.c_uint32), ("triangle_count", C.c_uint32), ("light_count", C.c_uint32),
+                ("stack_lds", C.c_uint32), ("stack_total", C.c_uint32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("trace_block", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("shade_block", C.c_uint32),
+                ("pool_size", C.c_uint32), ("use_graph", C.c_uint32)]
+
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+# name: (restype, argtypes); every symbol of include/pt_core.h
+SIGNATURES = {
+    "pt_version": (C.c_char_p, []),
+    "pt_renderer_create": (C.c_int, [C.POINTER(_P)]),
+    "pt_renderer_destroy": (None, [_P]),
+    "pt_get_error_message": (C.c_char_p, [_P]),
+    "pt_initialize_rendering_buffers": (C.c_int, [_P, _U32, _U32]),
+    "pt_initialize_scene_buffers": (C.c_int, [_P, C.POINTER(SceneDesc)]),
+    "pt_clear": (C.c_int, [_P, C.c_uint8, C.c_uint8, C.c_uint8, C.c_uint8]),
+    "pt_start_simulation": (C.c_int, [_P, C.POINTER(RenderSettings), C.POINTER(Camera), C.POINTER(MotionState), _U32,
+                                      C.c_char_p]),
+    "pt_resume_simulation": (C.c_int, [_P, _U32]),
+    "pt_stop_rendering": (C.c_int, [_P]),
+    "pt_is_rendering": (C.c_int, [_P]),
+    "pt_get_progress": (C.c_int, [_P, C.POINTER(Progress)]),
+    "pt_frame_copy": (C.c_int, [_P, _P, C.c_size_t]),
+    "pt_frame_copy_hdr": (C.c_int, [_P, _P, C.c_size_t]),
+    "pt_peek_last_error": (C.c_size_t, [_P, C.c_char_p, C.c_size_t]),
+    "pt_consume_last_error": (C.c_size_t, [_P, C.c_char_p, C.c_size_t]),
+    "pt_render_frame": (C.c_int, [_P, C.POINTER(RenderSettings), C.POINTER(Camera), C.POINTER(MotionState), _U32,
+                                  C.POINTER(FrameStats)]),
+    "pt_trace_closest": (C.c_int, [_P, _P, _U32, _P]),
+    "pt_trace_any": (C.c_int, [_P, _P, _U32, _P]),
+    "pt_trace_closest_device": (C.c_int, [_P, _P, _U32, _P, _P]),
+    "pt_trace_any_device": (C.c_int, [_P, _P, _U32, _P, _P]),
+    "pt_trace_closest_counts": (C.c_int, [_P, _P, _U32, _P, _P, _P]),
+    "pt_classify_hits": (C.c_int, [_P, _P, _U32, _P, _P]),
+    "pt_alias_sample": (C.c_int, [_P, _P, _U32, _P, _P]),
+    "pt_get_alias_table": (C.c_int, [_P, _P, _U32, C.POINTER(_U32)]),
+    "pt_get_bvh_info": (C.c_int, [_P, C.POINTER(BvhInfo)]),
+    "pt_get_bvh4": (C.c_int, [_P, _P, _U32, _P, _U32]),
+    "pt_build_bvh4": (C.c_int, [_P, _U32, _U32, _P, _U32, C.POINTER(_U32), _P, C.POINTER(_U32)]),
+    "pt_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "pt_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libptcore.so and bind every declared entry point."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python gpu-path-tracer_amd/build.py`")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _settings(d):
+    s = RenderSettings()
+    s.width = d.get("width", 256)
+    s.height = d.get("height", 256)
+    s.samples_per_pixel = d.get("samples_per_pixel", 16)
+    s.path_depth = d.get("path_depth", 8)
+    s.use_nee = d.get("use_nee", 1)
+    s.rr_start_depth = d.get("rr_start_depth", 3)
+    s.seed = d.get("seed", 1)
+    return s
+
+
+def _motion(view):
+    m = MotionState()
+    m.position[:] = [float(c) for c in view.position]
+    m.rotation[:] = [float(c) for c in view.rotation]
+    return m
+
+
+class Renderer:
+    """Python mirror of Core::Graphics::Cuda::Renderer over the C ABI.  Method
+    names follow the reference; each failure raises PtError with the status
+    and the library's message (the reference returns std::unexpected(Error))."""
+
+    def __init__(self):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.pt_renderer_create(C.byref(h))
+        if st != PT_OK:
+            raise PtError(st, "pt_renderer_create failed (is a GPU visible?)")
+        self.h = h
+        self._keep = None
+
+    def close(self):
+        if self.h:
+            self.lib.pt_renderer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st):
+        if st != PT_OK:
+            msg = self.lib.pt_get_error_message(self.h)
+            raise PtError(st, msg.decode() if msg else "")
+
+    # ---- boundary (Renderer.hpp:114-143)
+    def InitializeRenderingBuffers(self, width, height):
+        self._check(self.lib.pt_initialize_rendering_buffers(self.h, width, height))
+
+    def InitializeSceneBuffers(self, scene):
+        arrs = dict(
+            positions=np.ascontiguousarray(scene.positions, np.float32),
+            normals=np.ascontiguousarray(scene.normals, np.float32),
+            tangents=None if scene.tangents is None else np.ascontiguousarray(scene.tangents, np.float32),
+            material_ids=np.ascontiguousarray(scene.material_ids, np.uint32),
+            materials=np.ascontiguousarray(scene.materials),
+            light_weights=None if scene.light_weights is None else np.ascontiguousarray(scene.light_weights, np.float32),
+        )
+        d = SceneDesc(scene.triangle_count, _ptr(arrs["positions"]), _ptr(arrs["normals"]), _ptr(arrs["tangents"]),
+                      _ptr(arrs["material_ids"]), len(arrs["materials"]), _ptr(arrs["materials"]),
+                      _ptr(arrs["light_weights"]))
+        self._check(self.lib.pt_initialize_scene_buffers(self.h, C.byref(d)))
+
+    def Clear(self, r, g, b, a):
+        self._check(self.lib.pt_clear(self.h, r, g, b, a))
+
+    def StartSimulation(self, settings, view, motion_states, output_folder=None):
+        s = _settings(settings)
+        cam = Camera(view.fov_y, 0.1, 1000.0)
+        arr = (MotionState * len(motion_states))(*[_motion(v) for v in motion_states])
+        self._keep = arr
+        out = output_folder.encode() if output_folder else None
+        self._check(self.lib.pt_start_simulation(self.h, C.byref(s), C.byref(cam), arr, len(motion_states), out))
+
+    def ResumeSimulation(self, start_frame):
+        self._check(self.lib.pt_resume_simulation(self.h, start_frame))
+
+    def StopRendering(self):
+        self._check(self.lib.pt_stop_rendering(self.h))
+
+    def IsRendering(self):
+        return bool(self.lib.pt_is_rendering(self.h))
+
+    def GetProgress(self):
+        p = Progress()
+        self._check(self.lib.pt_get_progress(self.h, C.byref(p)))
+        return {k: getattr(p, k) for k, _ in p._fields_}
+
+    def GetFrame(self, width, height):
+        out = np.zeros(width * height, np.uint32)
+        self._check(self.lib.pt_frame_copy(self.h, _ptr(out), out.size))
+        return out.reshape(height, width)
+
+    def GetFrameHdr(self, width, height):
+        out = np.zeros((height, width, 4), np.float32)
+        self._check(self.lib.pt_frame_copy_hdr(self.h, _ptr(out), width * height))
+        return out
+
+    def PeekLastError(self):
+        buf = C.create_string_buffer(4096)
+        n = self.lib.pt_peek_last_error(self.h, buf, 4096)
+        return buf.value.decode() if n else None
+
+    def ConsumeLastError(self):
+        buf = C.create_string_buffer(4096)
+        n = self.lib.pt_consume_last_error(self.h, buf, 4096)
+        return buf.value.decode() if n else None
+
+    # ---- synchronous frame
+    def RenderFrame(self, settings, view, flags=FRAME_DEFAULT):
+        s = _settings(settings)
+        cam = Camera(view.fov_y, 0.1, 1000.0)
+        m = _motion(view)
+        st = FrameStats()
+        self._check(self.lib.pt_render_frame(self.h, C.byref(s), C.byref(cam), C.byref(m), flags, C.byref(st)))
+        return st.as_dict()
+
+    # ---- kernel-level hooks
+    def trace_closest(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        hits = np.zeros((len(rays), 4), np.uint32)
+        self._check(self.lib.pt_trace_closest(self.h, _ptr(rays), len(rays), _ptr(hits)))
+        return hits
+
+    def trace_any(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        occ = np.zeros(len(rays), np.uint32)
+        self._check(self.lib.pt_trace_any(self.h, _ptr(rays), len(rays), _ptr(occ)))
+        return occ
+
+    def trace_closest_counts(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        hits = np.zeros((len(rays), 4), np.uint32)
+        nodes = np.zeros(len(rays), np.uint32)
+        tris = np.zeros(len(rays), np.uint32)
+        self._check(self.lib.pt_trace_closest_counts(self.h, _ptr(rays), len(rays), _ptr(hits), _ptr(nodes),
+                                                     _ptr(tris)))
+        return hits, nodes, tris
+
+    def classify(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        q = np.zeros(len(rays), np.uint32)
+        counts = np.zeros(5, np.uint32)
+        self._check(self.lib.pt_classify_hits(self.h, _ptr(rays), len(rays), _ptr(q), _ptr(counts)))
+        return q, counts
+
+    def alias_sample(self, u):
+        u = np.ascontiguousarray(u, np.float32)
+        idx = np.zeros(len(u), np.uint32)
+        pmf = np.zeros(len(u), np.float32)
+        self._check(self.lib.pt_alias_sample(self.h, _ptr(u), len(u), _ptr(idx), _ptr(pmf)))
+        return idx, pmf
+
+    def alias_table(self):
+        n = C.c_uint32()
+        self._check(self.lib.pt_get_alias_table(self.h, None, 0, C.byref(n)))
+        bins = np.zeros(n.value, ALIAS_DTYPE)
+        if n.value:
+            self._check(self.lib.pt_get_alias_table(self.h, _ptr(bins), n.value, C.byref(n)))
+        return bins
+
+    def bvh_info(self):
+        b = BvhInfo()
+        self._check(self.lib.pt_get_bvh_info(self.h, C.byref(b)))
+        return {k: getattr(b, k) for k, _ in b._fields_}
+
+    def bvh4(self):
+        info = self.bvh_info()
+        nodes = np.zeros((info["node_count"], 32), np.uint32)
+        order = np.zeros(info["triangle_count"], np.uint32)
+        self._check(self.lib.pt_get_bvh4(self.h, _ptr(nodes), info["node_count"], _ptr(order), len(order)))
+        return nodes, order
+
+    def get_tuning(self):
+        t = Tuning()
+        self._check(self.lib.pt_get_tuning(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def set_tuning(self, **kw):
+        cur = self.get_tuning()
+        cur.update(kw)
+        t = Tuning(**cur)
+        self._check(self.lib.pt_set_tuning(self.h, C.byref(t)))
+
+
+def build_bvh4(positions, max_leaf_size=0):
+    """Host-only BVH4 build: returns (nodes (N, 32) uint32 rows of 128 B, prim order, depth)."""
+    lib = load_library()
+    P = np.ascontiguousarray(positions, np.float32)
+    n, depth = C.c_uint32(), C.c_uint32()
+    st = lib.pt_build_bvh4(_ptr(P), len(P), max_leaf_size, None, 0, C.byref(n), None, C.byref(depth))
+    if st != PT_OK:
+        raise PtError(st, "pt_build_bvh4 failed")
+    nodes = np.zeros((n.value, 32), np.uint32)
+    order = np.zeros(len(P), np.uint32)
+    st = lib.pt_build_bvh4(_ptr(P), len(P), max_leaf_size, _ptr(nodes), n.value, C.byref(n), _ptr(order),
+                           C.byref(depth))
+    if st != PT_OK:
+        raise PtError(st, "pt_build_bvh4 failed")
+    return nodes, order, depth.value
+
+
+def hits_as_fields(hits):
+    """(n, 4) uint32 pt_hit rows -> prim (u32), t, u, v (f32)."""
+    h = np.ascontiguousarray(hits, np.uint32)
+    return h[:, 0].copy(), h[:, 1].view(np.float32).copy(), h[:, 2].view(np.float32).copy(), h[:, 3].view(np.float32).copy()

@@ -1,0 +1,290 @@
+/*
+ * pt_core.h - C ABI of the MI355X wavefront path-tracing core.
+ *
+ * This is the drop-in boundary for the reference renderer's scene-upload and
+ * render entry points (Core/Include/Core/Graphics/Cuda/PathTracing/Renderer.hpp:114-143)
+ * and, one level down, for its kernel launchers
+ * (Core/Include/Core/Graphics/Cuda/PathTracing/Kernels/Launchers.hpp:18-146).
+ *
+ * Conventions
+ *  - Every call that can fail returns a pt_status; PT_OK is 0.  The message of
+ *    the last failed call on a handle is returned by pt_get_error_message().
+ *    This replaces std::expected<void, Core::Utils::Error> (Core/Include/Core/Utils/Error.hpp:11-123).
+ *  - Errors raised on the render thread are parked and read with
+ *    pt_peek_last_error()/pt_consume_last_error(), as Renderer::PeekLastError /
+ *    ConsumeLastError do (Renderer.cpp:308-320).
+ *  - Plain pointers and sizes only.  Host pointers unless a name ends in
+ *    _device, in which case the pointer is device memory of the renderer's GPU
+ *    and `stream` is a hipStream_t (NULL = the renderer's own stream).
+ *  - Calls are not thread-safe, except pt_frame_copy*, pt_get_progress,
+ *    pt_is_rendering and the error accessors, as in the reference
+ *    (Renderer.hpp:95-99,131-143).
+ */
+#ifndef PT_CORE_H
+#define PT_CORE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pt_renderer pt_renderer;
+
+typedef enum pt_status {
+    PT_OK = 0,
+    PT_ERROR_INVALID_ARGUMENT = 1,
+    PT_ERROR_INVALID_STATE = 2,   /* e.g. "Renderer is already rendering" (Renderer.cpp:347-348) */
+    PT_ERROR_OUT_OF_RANGE = 3,    /* e.g. ResumeSimulation start frame (Renderer.cpp:406-407) */
+    PT_ERROR_DEVICE = 4,          /* a HIP runtime call failed */
+    PT_ERROR_OUT_OF_MEMORY = 5,
+    PT_ERROR_UNSUPPORTED = 6,     /* shading model outside this core (Normal, Phong) */
+    PT_ERROR_IO = 7               /* frame could not be saved */
+} pt_status;
+
+/* Values of GlobalShadingModel (Core/Include/Core/Graphics/Cuda/PathTracing/Material.hpp:8-17). */
+enum {
+    PT_MODEL_NORMAL = 0,
+    PT_MODEL_DIFFUSE = 1,
+    PT_MODEL_MIRROR = 2,
+    PT_MODEL_PHONG = 3,
+    PT_MODEL_GGX = 4,
+    PT_MODEL_EMISSIVE = 5,
+    PT_MODEL_COUNT = 6
+};
+
+/* Material queues of the wavefront, in the order the shade launch visits them,
+ * plus the regeneration queue for misses (MaterialEvalQueueViewsProvider.hpp:9-24,
+ * RegenQueueView.hpp:10-85). */
+enum {
+    PT_QUEUE_DIFFUSE = 0,
+    PT_QUEUE_GGX = 1,
+    PT_QUEUE_MIRROR = 2,
+    PT_QUEUE_EMISSIVE = 3,
+    PT_QUEUE_REGEN = 4,
+    PT_QUEUE_COUNT = 5
+};
+
+#define PT_INVALID_INDEX 0xFFFFFFFFu
+
+/* Constant (untextured) material.  Replaces the reference's texture-handle
+ * Material (Material.hpp:73-87): every channel is the fp32 value the 1x1
+ * texture would return. */
+typedef struct pt_material {
+    uint32_t model;        /* PT_MODEL_* */
+    float color[3];        /* albedo / base color, linear */
+    float specular[3];     /* mirror reflectance (the reference default is 0) */
+    float roughness;       /* rma.x */
+    float metallic;        /* rma.y */
+    float ior;             /* 1.0 default */
+    float emission[3];     /* radiance */
+    float normal_ts[3];    /* constant tangent-space normal, (2*texel-1); (0,0,1) = flat */
+} pt_material;
+
+/* Flattened world-space triangle soup.  Replaces the (entt::registry,
+ * Assets::Storage) pair of InitializeSceneBuffers (Renderer.cpp:292-306):
+ * the caller has already applied BuildTriangles' transforms (Renderer.cpp:159-213). */
+typedef struct pt_scene_desc {
+    uint32_t triangle_count;
+    const float* positions;       /* [T][3 corners][xyz] */
+    const float* normals;         /* [T][3][xyz], world space, unit length */
+    const float* tangents;        /* [T][3][xyzw], w = handedness sign; NULL = derive from normals */
+    const uint32_t* material_ids; /* [T] index into materials */
+    uint32_t material_count;
+    const pt_material* materials;
+    const float* light_weights;   /* [T] alias-table weight of each emissive triangle;
+                                     NULL = triangle area, as LightSampler.cpp:16-18 */
+} pt_scene_desc;
+
+/* Settings of StartSimulation (Renderer.hpp:118-126) and PathTracerDefaults.hpp:9-35. */
+typedef struct pt_render_settings {
+    uint32_t width;
+    uint32_t height;
+    uint32_t samples_per_pixel;  /* rounded down to a square, Renderer.cpp:366-367 */
+    uint32_t path_depth;         /* clamped to [1,100] */
+    uint32_t use_nee;            /* next-event estimation on/off */
+    uint32_t rr_start_depth;     /* 0 = default 3 */
+    uint64_t seed;               /* PCG32 seed of the sample streams */
+} pt_render_settings;
+
+/* Ecs::Camera (Core/Include/Core/Graphics/Ecs/Camera.hpp): vertical fov in radians. */
+typedef struct pt_camera {
+    float fov_y;
+    float near_plane;
+    float far_plane;
+} pt_camera;
+
+/* Capture::MotionState (Core/Include/Core/Capture/Sample.hpp:6-10). */
+typedef struct pt_motion_state {
+    float position[3];
+    float rotation[4];   /* unit quaternion, x y z w */
+} pt_motion_state;
+
+typedef struct pt_progress {
+    uint32_t is_rendering;
+    uint32_t done_frames;
+    uint32_t total_frames;
+    uint32_t width;
+    uint32_t height;
+    uint32_t samples_per_pixel;
+    uint32_t samples_per_pixel_axis;
+    uint32_t path_depth;
+    uint64_t done_samples;
+    uint64_t total_samples;
+} pt_progress;
+
+/* Device-side counters of one rendered frame. */
+typedef struct pt_frame_stats {
+    double render_ms;            /* hipEvent time of the render loop (no upload, no readback) */
+    uint64_t extension_rays;     /* rays fed to closest-hit traversal */
+    uint64_t shadow_rays;        /* rays fed to any-hit traversal */
+    uint64_t samples;            /* camera samples generated */
+    uint32_t iterations;         /* wavefront iterations */
+    uint32_t closest_launches;   /* closest-hit launches that had work */
+    double closest_ms;           /* summed hipEvent time of closest-hit launches (profiled frames only) */
+    double shadow_ms;            /* summed time of any-hit launches (profiled frames only) */
+    double shade_ms;             /* summed time of shade launches (profiled frames only) */
+    double regen_ms;             /* summed time of regeneration launches (profiled frames only) */
+    uint64_t closest_nodes;      /* BVH nodes visited by closest-hit (instrumented frames only) */
+    uint64_t closest_tris;       /* triangles tested by closest-hit (instrumented frames only) */
+    uint64_t shadow_nodes;
+    uint64_t shadow_tris;
+} pt_frame_stats;
+
+/* Frame-mode flags for pt_render_frame. */
+enum {
+    PT_FRAME_DEFAULT = 0,
+    PT_FRAME_PROFILE = 1,       /* time each stage with hipEvents (no graph replay) */
+    PT_FRAME_INSTRUMENT = 2,    /* count nodes/triangles visited (instrumented traversal) */
+    PT_FRAME_NO_GRAPH = 4       /* launch stage by stage instead of replaying the hipGraph */
+};
+
+/* Test/bench ray record: 32 bytes. */
+typedef struct pt_ray {
+    float origin[3];
+    float t_min;
+    float direction[3];
+    float t_max;
+} pt_ray;
+
+/* Closest-hit record: original (pre-BVH) primitive index, PT_INVALID_INDEX on a miss. */
+typedef struct pt_hit {
+    uint32_t prim;
+    float t;
+    float u;
+    float v;
+} pt_hit;
+
+/* Alias-table bin (AliasTableView.hpp:8-14): q, p, alias, value. */
+typedef struct pt_alias_bin {
+    float q;
+    float p;
+    uint32_t alias;
+    uint32_t value;
+} pt_alias_bin;
+
+typedef struct pt_bvh_info {
+    uint32_t node_count;      /* 128-byte BVH4 nodes */
+    uint32_t leaf_count;
+    uint32_t max_depth;       /* BVH4 levels */
+    uint32_t max_leaf_size;
+    uint32_t triangle_count;
+    uint32_t light_count;
+    uint32_t stack_lds;       /* traversal-stack entries held in LDS per lane */
+    uint32_t stack_total;     /* worst-case stack entries (LDS + spill) */
+} pt_bvh_info;
+
+/* Kernel launch parameters picked by the autotuner. */
+typedef struct pt_tuning {
+    uint32_t trace_block;        /* threads per block of the traversal kernels (64..512) */
+    uint32_t trace_blocks_per_cu;
+    uint32_t shade_block;
+    uint32_t pool_size;          /* path slots (queue capacity) */
+    uint32_t use_graph;          /* replay the bounce loop as a hipGraph */
+} pt_tuning;
+
+/* ---- lifetime (Renderer ctor/dtor, Renderer.hpp:106-112) ---- */
+pt_status pt_renderer_create(pt_renderer** out);
+void pt_renderer_destroy(pt_renderer* renderer);
+const char* pt_get_error_message(const pt_renderer* renderer);
+const char* pt_version(void);
+
+/* ---- scene upload and rendering buffers ---- */
+/* Renderer::InitializeRenderingBuffers, Renderer.cpp:265-290 */
+pt_status pt_initialize_rendering_buffers(pt_renderer* renderer, uint32_t width, uint32_t height);
+/* Renderer::InitializeSceneBuffers, Renderer.cpp:292-306 */
+pt_status pt_initialize_scene_buffers(pt_renderer* renderer, const pt_scene_desc* scene);
+/* Renderer::Clear, Renderer.cpp:322-328 */
+pt_status pt_clear(pt_renderer* renderer, uint8_t r, uint8_t g, uint8_t b, uint8_t a);
+
+/* ---- asynchronous rendering (render thread) ---- */
+/* Renderer::StartSimulation, Renderer.cpp:330-399.  output_folder may be NULL (no PNGs). */
+pt_status pt_start_simulation(pt_renderer* renderer, const pt_render_settings* settings,
+                              const pt_camera* camera, const pt_motion_state* motion_states,
+                              uint32_t motion_state_count, const char* output_folder);
+/* Renderer::ResumeSimulation, Renderer.cpp:401-427 */
+pt_status pt_resume_simulation(pt_renderer* renderer, uint32_t start_frame);
+/* Renderer::StopRendering, Renderer.cpp:429-440 */
+pt_status pt_stop_rendering(pt_renderer* renderer);
+/* Renderer::IsRendering and the progress getters, Renderer.hpp:131-140 */
+int pt_is_rendering(const pt_renderer* renderer);
+pt_status pt_get_progress(const pt_renderer* renderer, pt_progress* out);
+/* FrameView::Lock()->GetData(), Renderer.hpp:45-104: RGBA8 pixels, row 0 = bottom. */
+pt_status pt_frame_copy(pt_renderer* renderer, uint32_t* rgba, size_t pixel_count);
+/* accumulation / spp as float RGBA (needed for relMSE; no reference counterpart). */
+pt_status pt_frame_copy_hdr(pt_renderer* renderer, float* rgba, size_t pixel_count);
+/* Renderer::PeekLastError / ConsumeLastError, Renderer.cpp:308-320.
+ * Copy the parked render-thread error into buf; returns its length, 0 if none. */
+size_t pt_peek_last_error(const pt_renderer* renderer, char* buf, size_t capacity);
+size_t pt_consume_last_error(pt_renderer* renderer, char* buf, size_t capacity);
+
+/* ---- synchronous frame (bench and tests): one RenderFrame, Renderer.cpp:468-647 ---- */
+pt_status pt_render_frame(pt_renderer* renderer, const pt_render_settings* settings,
+                          const pt_camera* camera, const pt_motion_state* motion_state,
+                          uint32_t frame_flags, pt_frame_stats* stats);
+
+/* ---- kernel-level hooks on caller-supplied batches ---- */
+/* IntersectRaysWithSceneKernel (Intersect.cu:63-123) on a ray batch. */
+pt_status pt_trace_closest(pt_renderer* renderer, const pt_ray* rays, uint32_t count, pt_hit* hits);
+/* IntersectShadowRaysWithSceneKernel's occlusion test (Intersect.cu:136-179). occluded[i] = 0/1. */
+pt_status pt_trace_any(pt_renderer* renderer, const pt_ray* rays, uint32_t count, uint32_t* occluded);
+pt_status pt_trace_closest_device(pt_renderer* renderer, const pt_ray* rays_device, uint32_t count,
+                                  pt_hit* hits_device, void* stream);
+pt_status pt_trace_any_device(pt_renderer* renderer, const pt_ray* rays_device, uint32_t count,
+                              uint32_t* occluded_device, void* stream);
+/* Per-ray visit counts of the closest-hit traversal (the algorithmic-byte model). */
+pt_status pt_trace_closest_counts(pt_renderer* renderer, const pt_ray* rays, uint32_t count,
+                                  pt_hit* hits, uint32_t* nodes_visited, uint32_t* tris_tested);
+/* Queue membership of the closest-hit push (Intersect.cu:115-122): queue_of_ray[i] is a
+ * PT_QUEUE_* value; queue_counts[PT_QUEUE_COUNT] receives the queue sizes. */
+pt_status pt_classify_hits(pt_renderer* renderer, const pt_ray* rays, uint32_t count,
+                           uint32_t* queue_of_ray, uint32_t* queue_counts);
+/* AliasTableView::Sample (AliasTableView.hpp:32-60) of the light table on the GPU. */
+pt_status pt_alias_sample(pt_renderer* renderer, const float* u, uint32_t count,
+                          uint32_t* index, float* pmf);
+/* Host copy of the light alias table built at scene upload (AliasTable.hpp:22-84). */
+pt_status pt_get_alias_table(pt_renderer* renderer, pt_alias_bin* bins, uint32_t capacity,
+                             uint32_t* count);
+pt_status pt_get_bvh_info(pt_renderer* renderer, pt_bvh_info* info);
+/* Raw BVH4 nodes (128 B each) and triangle order, for external traversal checks. */
+pt_status pt_get_bvh4(pt_renderer* renderer, void* nodes, uint32_t node_capacity,
+                      uint32_t* prim_order, uint32_t prim_capacity);
+
+/* Host-only BVH build of the core (no GPU needed): the 128-byte BVH4 nodes and
+ * BVH-order -> original prim permutation that InitializeSceneBuffers uploads.
+ * Replaces HostBvh + DeviceBvh::BuildFlattenSync (HostBvh.cpp:51-100, DeviceBvh.cpp:7-52).
+ * With nodes == NULL only *node_count and *depth are written. */
+pt_status pt_build_bvh4(const float* positions, uint32_t triangle_count, uint32_t max_leaf_size,
+                        void* nodes, uint32_t node_capacity, uint32_t* node_count,
+                        uint32_t* prim_order, uint32_t* depth);
+
+/* ---- tuning ---- */
+pt_status pt_get_tuning(pt_renderer* renderer, pt_tuning* tuning);
+pt_status pt_set_tuning(pt_renderer* renderer, const pt_tuning* tuning);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PT_CORE_H */

@@ -1,0 +1,171 @@
+"""ctypes binding of the CPU oracle (test infrastructure only: imported by
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+BIN_DTYPE = np.dtype([("q", "<f4"), ("p", "<f4"), ("alias", "<u4"), ("value", "<u4")])
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} missing: run `python oracle/build.py`")
+        L = C.CDLL(LIB)
+        sig = {
+            "orc_scene_create": (_P, [_U32, _P, _P, _P, _P, _U32, _P, _P]),
+            "orc_scene_destroy": (None, [_P]),
+            "orc_scene_info": (None, [_P, _P]),
+            "orc_scene_bvh": (None, [_P, _P, _P]),
+            "orc_trace_closest": (None, [_P, _P, _U32, _P, C.c_int, _U32, _P, _P]),
+            "orc_trace_any": (None, [_P, _P, _U32, _P, C.c_int, _U32]),
+            "orc_classify": (None, [_P, _P, _U32, _P, _U32]),
+            "orc_bvh4_closest": (None, [_P, _P, _P, _P, _U32, _P, _P, _P, _P, _U32]),
+            "orc_alias_build": (None, [_P, _U32, _P]),
+            "orc_alias_sample": (None, [_P, _U32, _P, _U32, _P, _P]),
+            "orc_scene_alias": (None, [_P, _P]),
+            "orc_pcg32": (None, [C.c_uint64, C.c_uint64, C.c_uint64, _U32, _P]),
+            "orc_camera_basis": (None, [C.c_float, C.c_float, _P, _P, _P]),
+            "orc_render": (None, [_P, _P, C.c_uint64, _P, _U32, _U32, _U32, _P, _P]),
+            "orc_tonemap": (None, [_P, _U32, _P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def default_threads():
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+class OracleScene:
+    """CPU restatement of the reference scene upload + traversal + shading."""
+
+    def __init__(self, scene):
+        self._arrs = [np.ascontiguousarray(scene.positions, np.float32),
+                      np.ascontiguousarray(scene.normals, np.float32),
+                      None if scene.tangents is None else np.ascontiguousarray(scene.tangents, np.float32),
+                      np.ascontiguousarray(scene.material_ids, np.uint32),
+                      np.ascontiguousarray(scene.materials),
+                      None if scene.light_weights is None else np.ascontiguousarray(scene.light_weights, np.float32)]
+        P, N, T, M, mats, W = self._arrs
+        self.h = lib().orc_scene_create(scene.triangle_count, _p(P), _p(N), _p(T), _p(M), len(mats), _p(mats), _p(W))
+        info = np.zeros(6, np.uint32)
+        lib().orc_scene_info(self.h, _p(info))
+        self.info = dict(nodes=int(info[0]), depth=int(info[1]), max_depth=int(info[2]), leaf_size=int(info[3]),
+                         lights=int(info[4]), triangles=int(info[5]))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def closest(self, rays, brute=False, threads=None, counts=False):
+        rays = np.ascontiguousarray(rays, np.float32)
+        hits = np.zeros((len(rays), 4), np.uint32)
+        nv = np.zeros(len(rays), np.uint32) if counts else None
+        nt = np.zeros(len(rays), np.uint32) if counts else None
+        lib().orc_trace_closest(self.h, _p(rays), len(rays), _p(hits), 1 if brute else 0,
+                                threads or default_threads(), _p(nv), _p(nt))
+        return (hits, nv, nt) if counts else hits
+
+    def any(self, rays, brute=False, threads=None):
+        rays = np.ascontiguousarray(rays, np.float32)
+        occ = np.zeros(len(rays), np.uint32)
+        lib().orc_trace_any(self.h, _p(rays), len(rays), _p(occ), 1 if brute else 0, threads or default_threads())
+        return occ
+
+    def classify(self, rays, threads=None):
+        rays = np.ascontiguousarray(rays, np.float32)
+        q = np.zeros(len(rays), np.uint32)
+        lib().orc_classify(self.h, _p(rays), len(rays), _p(q), threads or default_threads())
+        return q
+
+    def bvh4_closest(self, nodes, order, rays, threads=None):
+        """Walk the core's BVH4 in the GPU kernel's order: hits and per-ray
+        node / triangle visit counts and peak stack depth."""
+        nodes = np.ascontiguousarray(nodes, np.uint32)
+        order = np.ascontiguousarray(order, np.uint32)
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = len(rays)
+        hits = np.zeros((n, 4), np.uint32)
+        nv, nt, ms = (np.zeros(n, np.uint32) for _ in range(3))
+        lib().orc_bvh4_closest(self.h, _p(nodes), _p(order), _p(rays), n, _p(hits), _p(nv), _p(nt), _p(ms),
+                               threads or default_threads())
+        return hits, nv, nt, ms
+
+    def alias_table(self):
+        bins = np.zeros(self.info["lights"], BIN_DTYPE)
+        lib().orc_scene_alias(self.h, _p(bins))
+        return bins
+
+    def bvh(self):
+        nodes = np.zeros((self.info["nodes"], 10), np.float32)
+        order = np.zeros(self.info["triangles"], np.uint32)
+        lib().orc_scene_bvh(self.h, _p(nodes), _p(order))
+        return nodes, order
+
+    def render(self, settings, view, pixel_begin=0, pixel_end=None, threads=None):
+        """HDR (accumulated radiance / spp) of pixels [pixel_begin, pixel_end)."""
+        w, h = settings["width"], settings["height"]
+        pixel_end = w * h if pixel_end is None else pixel_end
+        s = np.array([w, h, settings["samples_per_pixel"], settings["path_depth"], settings.get("use_nee", 1),
+                      settings.get("rr_start_depth", 3)], np.uint32)
+        cam = camera_basis(view, w / h)
+        out = np.zeros((pixel_end - pixel_begin, 4), np.float32)
+        cnt = np.zeros(2, np.uint64)
+        lib().orc_render(self.h, _p(s), settings.get("seed", 1), _p(cam), pixel_begin, pixel_end,
+                         threads or default_threads(), _p(out), _p(cnt))
+        return out, int(cnt[0]), int(cnt[1])
+
+
+def camera_basis(view, aspect):
+    pos = np.asarray(view.position, np.float32)
+    q = np.asarray(view.rotation, np.float32)
+    out = np.zeros(12, np.float32)
+    lib().orc_camera_basis(C.c_float(view.fov_y), C.c_float(aspect), _p(pos), _p(q), _p(out))
+    return out
+
+
+def alias_build(weights):
+    w = np.ascontiguousarray(weights, np.float32)
+    bins = np.zeros(len(w), BIN_DTYPE)
+    lib().orc_alias_build(_p(w), len(w), _p(bins))
+    return bins
+
+
+def alias_sample(bins, u):
+    bins = np.ascontiguousarray(bins)
+    u = np.ascontiguousarray(u, np.float32)
+    idx = np.zeros(len(u), np.uint32)
+    pmf = np.zeros(len(u), np.float32)
+    lib().orc_alias_sample(_p(bins), len(bins), _p(u), len(u), _p(idx), _p(pmf))
+    return idx, pmf
+
+
+def pcg32(initstate, initseq, n, advance=0):
+    out = np.zeros(n, np.uint32)
+    lib().orc_pcg32(initstate, initseq, advance, n, _p(out))
+    return out
+
+
+def tonemap(hdr):
+    hdr = np.ascontiguousarray(hdr, np.float32).reshape(-1, 4)
+    out = np.zeros(len(hdr), np.uint32)
+    lib().orc_tonemap(_p(hdr), len(hdr), _p(out))
+    return out

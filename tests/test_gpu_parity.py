@@ -1,0 +1,239 @@
+"""Parity of the HIP core against the CPU oracle, through the C ABI."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import ptcore
+import scenes
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def c1(gpu_renderer, cornell):
+    gpu_renderer.InitializeRenderingBuffers(64, 64)
+    gpu_renderer.InitializeSceneBuffers(cornell)
+    return gpu_renderer
+
+
+def test_c1_closest_hit_bit_exact_1m_rays(c1, cornell_oracle):
+    """1,048,576 seeded rays: (prim, t, u, v) equal the oracle's (t, prim) minimum bit for bit."""
+    rays = scenes.random_rays(1 << 20, (-1, 0, -1), (1, 2, 1), 101)
+    got = c1.trace_closest(rays)
+    want = cornell_oracle.closest(rays, brute=True)
+    assert np.array_equal(got, want)
+    ref = cornell_oracle.closest(rays)            # reference traversal order
+    diff = (got != ref).any(1)
+    assert np.array_equal(got[diff, 1], ref[diff, 1])   # only exact ties in t differ
+
+
+def test_c1_any_hit_matches(c1, cornell, cornell_oracle):
+    rays = scenes.random_rays(1 << 18, (-1, 0, -1), (1, 2, 1), 102)
+    rays[:, 7] = np.random.default_rng(1).uniform(0.01, 3.0, len(rays)).astype(np.float32)
+    assert np.array_equal(c1.trace_any(rays), cornell_oracle.any(rays))
+    sh = scenes.shadow_rays(cornell, 262144, 103)
+    assert np.array_equal(c1.trace_any(sh), cornell_oracle.any(sh))
+
+
+def test_c1_queue_membership(c1, cornell_oracle):
+    rays = scenes.random_rays(1 << 18, (-1, 0, -1), (1, 2, 1), 104)
+    q, counts = c1.classify(rays)
+    want = cornell_oracle.classify(rays)
+    assert np.array_equal(q, want)
+    assert np.array_equal(counts, np.bincount(want, minlength=5))
+
+
+def test_c1_alias_table_and_picks(c1, cornell_oracle):
+    bins = c1.alias_table()
+    want = cornell_oracle.alias_table()
+    assert bins.tobytes() == want.tobytes()
+    import oracle
+    u = np.random.default_rng(5).random(1 << 20).astype(np.float32)
+    u[:4] = [0.0, 0.5, 0.99999994, 0.25]
+    idx, pmf = c1.alias_sample(u)
+    oidx, opmf = oracle.alias_sample(want, u)
+    assert np.array_equal(idx, oidx) and np.array_equal(pmf, opmf)
+
+
+def test_golden_fixtures(c1):
+    path = os.path.join(GOLDEN, "c1_hits_4096.npz")
+    g = np.load(path)
+    assert np.array_equal(c1.trace_closest(g["rays"]), g["hits"])
+    assert np.array_equal(c1.trace_any(g["shadow_rays"]), g["occluded"])
+    q, _ = c1.classify(g["rays"])
+    assert np.array_equal(q, g["queue"])
+
+
+def test_empty_batch_and_miss_rays(c1):
+    assert c1.trace_closest(np.zeros((0, 8), np.float32)).shape == (0, 4)
+    rays = scenes.random_rays(1000, (5, 5, 5), (6, 6, 6), 1)
+    rays[:, 4:7] = np.abs(rays[:, 4:7]) + 0.1   # pointing away from the room
+    h = c1.trace_closest(rays)
+    assert (h[:, 0] == ptcore.INVALID).all()
+    q, counts = c1.classify(rays)
+    assert (q == ptcore.QUEUE_REGEN).all() and counts[4] == 1000
+
+
+def _relmse(a, b, eps=1e-2):
+    return float(np.mean((a - b) ** 2 / (b ** 2 + eps)))
+
+
+def test_c1_render_matches_oracle(c1, cornell, cornell_oracle):
+    st = dict(width=64, height=64, samples_per_pixel=256, path_depth=8, use_nee=1, rr_start_depth=3, seed=1)
+    stats = c1.RenderFrame(st, cornell.view)
+    hdr = c1.GetFrameHdr(64, 64).reshape(-1, 4)
+    ref, ext, sh = cornell_oracle.render(st, cornell.view)
+    # same sample streams: ray counts agree closely, images agree to rounding
+    assert abs(stats["extension_rays"] - ext) / ext < 0.01
+    assert abs(stats["shadow_rays"] - sh) / sh < 0.01
+    rm = _relmse(hdr[:, :3], ref[:, :3])
+    assert rm < 1e-3, rm
+    mg, mo = hdr[:, :3].mean(), ref[:, :3].mean()
+    se = math.sqrt(ref[:, :3].mean(1).var() / len(ref))
+    assert abs(mg - mo) < 3 * se
+
+
+def test_c1_render_graph_equals_stage_by_stage(c1, cornell):
+    st = dict(width=32, height=32, samples_per_pixel=16, path_depth=8, use_nee=1, seed=2)
+    s1 = c1.RenderFrame(st, cornell.view)
+    a = c1.GetFrameHdr(32, 32)
+    s2 = c1.RenderFrame(st, cornell.view, ptcore.FRAME_NO_GRAPH)
+    b = c1.GetFrameHdr(32, 32)
+    s3 = c1.RenderFrame(st, cornell.view, ptcore.FRAME_PROFILE)
+    c = c1.GetFrameHdr(32, 32)
+    for s in (s2, s3):
+        assert s["extension_rays"] == s1["extension_rays"] and s["shadow_rays"] == s1["shadow_rays"]
+    assert np.allclose(a, b, rtol=1e-4, atol=1e-5) and np.allclose(a, c, rtol=1e-4, atol=1e-5)
+    assert s1["samples"] == 32 * 32 * 16
+
+
+def test_c1_small_pool_regeneration(c1, cornell):
+    """A pool smaller than the sample count forces path regeneration."""
+    st = dict(width=32, height=32, samples_per_pixel=16, path_depth=8, use_nee=1, seed=2)
+    big = c1.RenderFrame(st, cornell.view)
+    a = c1.GetFrameHdr(32, 32)
+    tun = c1.get_tuning()
+    try:
+        c1.set_tuning(pool_size=1024)
+        small = c1.RenderFrame(st, cornell.view)
+        b = c1.GetFrameHdr(32, 32)
+    finally:
+        c1.set_tuning(**tun)
+    assert small["samples"] == big["samples"] == 32 * 32 * 16
+    assert small["extension_rays"] == big["extension_rays"]
+    assert small["iterations"] > big["iterations"]
+    assert np.allclose(a, b, rtol=1e-4, atol=1e-5)
+
+
+def test_async_simulation_lifecycle(c1, cornell, tmp_path):
+    st = dict(width=32, height=32, samples_per_pixel=4, path_depth=4, use_nee=1, seed=1)
+    views = [cornell.view, cornell.view]
+    c1.StartSimulation(st, cornell.view, views, str(tmp_path))
+    with pytest.raises(ptcore.PtError) as e:
+        c1.StartSimulation(st, cornell.view, views, str(tmp_path))
+    assert e.value.status == 2 or not c1.IsRendering()
+    import time
+    t0 = time.time()
+    while c1.IsRendering() and time.time() - t0 < 60:
+        time.sleep(0.01)
+    c1.StopRendering()
+    p = c1.GetProgress()
+    assert p["done_frames"] == 2 and p["total_frames"] == 2
+    assert c1.PeekLastError() is None
+    pngs = sorted(os.listdir(tmp_path))
+    assert pngs == ["frame_0000.png", "frame_0001.png"]
+    assert open(tmp_path / pngs[0], "rb").read(8) == b"\x89PNG\r\n\x1a\n"
+    with pytest.raises(ptcore.PtError) as e:
+        c1.ResumeSimulation(5)
+    assert e.value.status == 3
+    c1.ResumeSimulation(1)
+    c1.StopRendering()
+    frame = c1.GetFrame(32, 32)
+    assert (frame >> 24 == 255).all()
+
+
+def test_unsupported_material_is_rejected(gpu_renderer, cornell):
+    s = scenes.cornell_box()
+    mats = s.materials.copy()
+    mats[0]["model"] = scenes.MODEL_PHONG
+    s.materials = mats
+    with pytest.raises(ptcore.PtError) as e:
+        gpu_renderer.InitializeSceneBuffers(s)
+    assert e.value.status == 6
+    gpu_renderer.InitializeSceneBuffers(cornell)   # restore
+
+
+@pytest.fixture(scope="module")
+def c2_pair(oracle_lib):
+    s = scenes.production_scene()
+    r = ptcore.Renderer()
+    r.InitializeRenderingBuffers(64, 64)
+    r.InitializeSceneBuffers(s)
+    o = oracle_lib.OracleScene(s)
+    yield s, r, o
+    r.close()
+
+
+def test_c2_closest_hit_16m_rays(c2_pair):
+    """Production scene (983,042 triangles), 16,777,216 seeded rays: a subset is
+    bit-exact against the oracle; every hit is re-verified with the same
+    float32 arithmetic; the any-hit answer agrees with closest-hit everywhere."""
+    s, r, o = c2_pair
+    rays = scenes.random_rays(1 << 24, (-10, -10, -10), (110, 110, 110), 201)
+    hits = r.trace_closest(rays)
+    sub = np.random.default_rng(0).choice(len(rays), 65536, replace=False)
+    ref = o.closest(rays[sub])
+    diff = (hits[sub] != ref).any(1)
+    assert np.array_equal(hits[sub][diff, 1], ref[diff, 1])
+    assert diff.mean() < 1e-3
+    prim, t, u, v = ptcore.hits_as_fields(hits)
+    hit = prim != ptcore.INVALID
+    assert hit.mean() > 0.01
+    # recompute Moller-Trumbore in float32 for the reported triangle (no contraction in numpy)
+    R = rays[hit]
+    P = s.positions[prim[hit]]
+    o3, d = R[:, 0:3], R[:, 4:7]
+    v0 = P[:, 0]
+    e1 = P[:, 1] - P[:, 0]
+    e2 = P[:, 2] - P[:, 0]
+
+    def cross(a, b):
+        return np.stack([a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1], a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2],
+                         a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]], 1)
+
+    def dot(a, b):
+        return (a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]) + a[:, 2] * b[:, 2]
+
+    pv = cross(d, e2)
+    inv = np.float32(1.0) / dot(e1, pv)
+    sv = o3 - v0
+    qv = cross(sv, e1)
+    assert np.array_equal(dot(sv, pv) * inv, u[hit])
+    assert np.array_equal(dot(d, qv) * inv, v[hit])
+    assert np.array_equal(dot(e2, qv) * inv, t[hit])
+    probe = rays.copy()
+    probe[:, 7] = np.where(hit, t * np.float32(1.0001), np.float32(1e34))
+    assert np.array_equal(r.trace_any(probe).astype(bool), hit)
+
+
+def test_c2_queue_membership_and_alias(c2_pair, oracle_lib):
+    s, r, o = c2_pair
+    rays = scenes.random_rays(1 << 16, (-10, -10, -10), (110, 110, 110), 202)
+    q, counts = r.classify(rays)
+    hits = o.closest(rays)
+    # membership from the reference traversal's hit (ties between coplanar
+    # triangles of one sphere share a material queue)
+    model = s.materials["model"][s.material_ids[np.where(hits[:, 0] == ptcore.INVALID, 0, hits[:, 0])]]
+    qmap = {scenes.MODEL_DIFFUSE: 0, scenes.MODEL_GGX: 1, scenes.MODEL_MIRROR: 2, scenes.MODEL_EMISSIVE: 3}
+    want = np.array([qmap[m] for m in model], np.uint32)
+    want[hits[:, 0] == ptcore.INVALID] = 4
+    assert (q == want).mean() > 0.999
+    bins = r.alias_table()
+    assert bins.tobytes() == o.alias_table().tobytes()
+    u = np.random.default_rng(9).random(1 << 20).astype(np.float32)
+    idx, pmf = r.alias_sample(u)
+    oi, op = oracle_lib.alias_sample(bins, u)
+    assert np.array_equal(idx, oi) and np.array_equal(pmf, op)
