@@ -40,14 +40,15 @@ def cpu_baseline(scene, settings, view, target_s):
     log(f"cpu baseline: oracle BVH built in {time.time() - t0:.1f}s, {threads} threads")
     w, h = settings["width"], settings["height"]
     centre = (h // 2) * w + w // 2
-    n = 64
-    while True:
-        t0 = time.perf_counter()
-        _, ext, sh = o.render(settings, view, centre, centre + n, threads=threads)
-        dt = time.perf_counter() - t0
-        if dt > target_s / 4 or n >= w * h // 2:
-            break
-        n = min(w * h // 2, int(n * max(2.0, target_s / 2 / max(dt, 1e-3))))
+    # calibrate on a small band, then time one band sized to about target_s
+    n = 256
+    t0 = time.perf_counter()
+    o.render(settings, view, centre, centre + n, threads=threads)
+    dt = time.perf_counter() - t0
+    n = int(max(256, min(w * h // 2, n * target_s / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    _, ext, sh = o.render(settings, view, centre - n // 2, centre - n // 2 + n, threads=threads)
+    dt = time.perf_counter() - t0
     return {"value": round((ext + sh) / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{n} pixels at the image centre of the same frame, {settings['samples_per_pixel']} spp, "
                       f"{ext} extension + {sh} shadow rays in {dt:.1f}s"}

@@ -61,6 +61,22 @@ __device__ __forceinline__ uint32_t wave_reserve_multi(unsigned int* counters, i
     return pos;
 }
 
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Adds per-lane instrumentation totals with one atomic per wave.
+__device__ __forceinline__ void add_counts(unsigned long long* nodes_ctr, unsigned long long* tris_ctr, uint32_t nodes,
+                                           uint32_t tris) {
+    const uint32_t n = wave_sum(nodes), t = wave_sum(tris);
+    if (lane_id() == 0) {
+        atomicAdd(nodes_ctr, (unsigned long long)n);
+        atomicAdd(tris_ctr, (unsigned long long)t);
+    }
+}
+
 // ================================================================ intersection
 
 // Moller-Trumbore with the reference's operation order and tests
@@ -159,25 +175,36 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
 
     bool active = false;
     bool exhausted = false;
+    uint32_t chunk_next = 0, chunk_end = 0;   // wave-uniform range of ray indices owned by this wave
     uint32_t item = 0, ref = 0, sp = 0;
     RayFrame r;
     float best_t = 0.0f, best_u = 0.0f, best_v = 0.0f;
     uint32_t best_prim = kInvalid, best_queue = 0;
-    uint32_t nodes = 0, tris = 0;
+    uint32_t nodes = 0, tris = 0, acc_nodes = 0, acc_tris = 0;
 
     while (true) {
         if (!exhausted) {
             const uint64_t idle = __ballot(!active);
             const uint32_t nidle = (uint32_t)__popcll(idle);
             if (nidle >= cfg.refill || nidle == 64u) {
-                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
-                uint32_t base = 0;
-                if (lane_id() == leader) base = atomicAdd(work, nidle);
-                base = readlane_u32(base, leader);
-                if (base + nidle >= n) exhausted = true;
-                if (!active) {
-                    const uint32_t my = base + mbcnt(idle);
-                    if (my < n) {
+                if (chunk_next >= chunk_end) {
+                    // one atomic claims `chunk` rays for the whole wave
+                    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
+                    uint32_t base = 0;
+                    if (lane_id() == leader) base = atomicAdd(work, cfg.chunk);
+                    base = readlane_u32(base, leader);
+                    if (base >= n) {
+                        exhausted = true;
+                    } else {
+                        chunk_next = base;
+                        chunk_end = min(base + cfg.chunk, n);
+                    }
+                }
+                if (chunk_next < chunk_end) {
+                    const uint32_t avail = chunk_end - chunk_next;
+                    const uint32_t lane_rank = mbcnt(idle);
+                    if (!active && lane_rank < avail) {
+                        const uint32_t my = chunk_next + lane_rank;
                         V3 o, d;
                         float tmin, tmax;
                         item = src.load(my, o, d, tmin, tmax);
@@ -191,6 +218,7 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                         sp = 0;
                         active = true;
                     }
+                    chunk_next += min(nidle, avail);
                 }
             }
         }
@@ -277,9 +305,12 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
         if (done) {
             if (ANY) src.finish_any(item, best_prim != kInvalid, nodes, tris);
             else src.finish_closest(item, best_prim, best_t, best_u, best_v, best_queue, nodes, tris);
+            acc_nodes += nodes;
+            acc_tris += tris;
             active = false;
         }
     }
+    if (COUNT) src.flush_counts(acc_nodes, acc_tris);
 }
 
 // ---------------------------------------------------------------- ray sources
@@ -287,7 +318,6 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
 struct WavefrontClosestSrc {
     PathState p;
     int cur;
-    bool instrument;
     __device__ __forceinline__ uint32_t count() const { return p.ctr->ray[cur]; }
     __device__ __forceinline__ unsigned int* work() const { return &p.ctr->work_closest; }
     __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax) const {
@@ -297,32 +327,27 @@ struct WavefrontClosestSrc {
         tmin = a.w;
         d = xyz(b);
         tmax = b.w;
-        return slot;
+        return i;
     }
-    __device__ __forceinline__ void finish_closest(uint32_t slot, uint32_t prim, float t, float u, float v,
-                                                   uint32_t queue, uint32_t nodes, uint32_t tris) const {
+    // Writes the hit to the path slot and the target queue of ray-queue entry
+    // i; the binning pass scatters the slots into the queues afterwards.
+    __device__ __forceinline__ void finish_closest(uint32_t i, uint32_t prim, float t, float u, float v,
+                                                   uint32_t queue, uint32_t, uint32_t) const {
         const bool hit = prim != kInvalid;
-        if (hit) p.hit[slot] = make_float4(t, u, v, __uint_as_float(prim));
-        const int q = hit ? (int)queue : kQueueRegen;
-        const uint32_t pos = wave_reserve_multi(p.ctr->queue, q);
-        switch (q) {   // queue arrays are separate allocations
-            case 0: p.q[0][pos] = slot; break;
-            case 1: p.q[1][pos] = slot; break;
-            case 2: p.q[2][pos] = slot; break;
-            case 3: p.q[3][pos] = slot; break;
-            default: p.q[4][pos] = slot; break;
+        if (hit) {
+            const uint32_t slot = (cur ? p.rayq[1] : p.rayq[0])[i];
+            p.hit[slot] = make_float4(t, u, v, __uint_as_float(prim));
         }
-        if (instrument) {
-            atomicAdd(&p.ctr->closest_nodes, (unsigned long long)nodes);
-            atomicAdd(&p.ctr->closest_tris, (unsigned long long)tris);
-        }
+        p.qid[i] = (uint8_t)(hit ? queue : (uint32_t)kQueueRegen);
     }
     __device__ void finish_any(uint32_t, bool, uint32_t, uint32_t) const {}
+    __device__ __forceinline__ void flush_counts(uint32_t nodes, uint32_t tris) const {
+        add_counts(&p.ctr->closest_nodes, &p.ctr->closest_tris, nodes, tris);
+    }
 };
 
 struct WavefrontShadowSrc {
     PathState p;
-    bool instrument;
     __device__ __forceinline__ uint32_t count() const { return p.ctr->shadow; }
     __device__ __forceinline__ unsigned int* work() const { return &p.ctr->work_shadow; }
     __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax) const {
@@ -348,10 +373,9 @@ struct WavefrontShadowSrc {
             atomicAdd(acc + 1, L.y);
             atomicAdd(acc + 2, L.z);
         }
-        if (instrument) {
-            atomicAdd(&p.ctr->shadow_nodes, (unsigned long long)nodes);
-            atomicAdd(&p.ctr->shadow_tris, (unsigned long long)tris);
-        }
+    }
+    __device__ __forceinline__ void flush_counts(uint32_t nodes, uint32_t tris) const {
+        add_counts(&p.ctr->shadow_nodes, &p.ctr->shadow_tris, nodes, tris);
     }
 };
 
@@ -363,6 +387,7 @@ struct BatchSrc {
     uint32_t* tris_out;
     unsigned int* work_ctr;
     uint32_t n;
+    uint8_t* qid;                   // optional: target queue of each ray
     __device__ __forceinline__ uint32_t count() const { return n; }
     __device__ __forceinline__ unsigned int* work() const { return work_ctr; }
     __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax) const {
@@ -374,49 +399,22 @@ struct BatchSrc {
         return i;
     }
     __device__ __forceinline__ void finish_closest(uint32_t i, uint32_t prim, float t, float u, float v,
-                                                   uint32_t, uint32_t nodes, uint32_t tris) const {
+                                                   uint32_t queue, uint32_t nodes, uint32_t tris) const {
         // pt_hit layout: prim, t, u, v
-        if (prim == kInvalid) hits[i] = make_float4(__uint_as_float(kInvalid), 0.0f, 0.0f, 0.0f);
-        else hits[i] = make_float4(__uint_as_float(prim), t, u, v);
+        if (hits) {
+            if (prim == kInvalid) hits[i] = make_float4(__uint_as_float(kInvalid), 0.0f, 0.0f, 0.0f);
+            else hits[i] = make_float4(__uint_as_float(prim), t, u, v);
+        }
+        if (qid) qid[i] = (uint8_t)(prim != kInvalid ? queue : (uint32_t)kQueueRegen);
         if (nodes_out) nodes_out[i] = nodes;
         if (tris_out) tris_out[i] = tris;
     }
+    __device__ void flush_counts(uint32_t, uint32_t) const {}
     __device__ __forceinline__ void finish_any(uint32_t i, bool occ, uint32_t nodes, uint32_t tris) const {
         occluded[i] = occ ? 1u : 0u;
         if (nodes_out) nodes_out[i] = nodes;
         if (tris_out) tris_out[i] = tris;
     }
-};
-
-struct ClassifySrc {
-    const float4* rays;
-    uint32_t* queues[kQueueSlots];
-    unsigned int* queue_counts;
-    unsigned int* work_ctr;
-    uint32_t n;
-    __device__ __forceinline__ uint32_t count() const { return n; }
-    __device__ __forceinline__ unsigned int* work() const { return work_ctr; }
-    __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax) const {
-        const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
-        o = xyz(a);
-        tmin = a.w;
-        d = xyz(b);
-        tmax = b.w;
-        return i;
-    }
-    __device__ __forceinline__ void finish_closest(uint32_t i, uint32_t prim, float, float, float,
-                                                   uint32_t queue, uint32_t, uint32_t) const {
-        const int q = prim != kInvalid ? (int)queue : kQueueRegen;
-        const uint32_t pos = wave_reserve_multi(queue_counts, q);
-        switch (q) {
-            case 0: queues[0][pos] = i; break;
-            case 1: queues[1][pos] = i; break;
-            case 2: queues[2][pos] = i; break;
-            case 3: queues[3][pos] = i; break;
-            default: queues[4][pos] = i; break;
-        }
-    }
-    __device__ void finish_any(uint32_t, bool, uint32_t, uint32_t) const {}
 };
 
 template <int BLOCK, int LDS, bool ANY, bool COUNT, class Src>
@@ -759,6 +757,118 @@ __global__ void k_prepare(PathState p, int cur) {
     }
 }
 
+// Scatters entries [0, n) into the five queues by their queue id.  Each block
+// owns one contiguous range: a counting pass, one atomicAdd per queue per
+// block, then a scatter pass in which every wave writes at its own running
+// offsets.  Five global atomics per block in place of one per finishing ray.
+constexpr int kBinBlock = 256;
+constexpr int kBinWaves = kBinBlock / 64;
+
+template <class SlotOf>
+__device__ __forceinline__ void bin_entries(const uint8_t* qid, uint32_t n, const SlotOf& slot_of,
+                                            uint32_t* const* queues, unsigned int* counters) {
+    __shared__ uint32_t s_cnt[kQueueSlots][kBinWaves];
+    const uint32_t per = (((n + gridDim.x - 1) / gridDim.x) + 63u) & ~63u;
+    const uint32_t b0 = blockIdx.x * per;
+    const uint32_t b1 = min(b0 + per, n);
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    uint32_t cnt[kQueueSlots] = {0, 0, 0, 0, 0};
+    for (uint32_t base = b0; base < b1; base += kBinBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const int q = i < b1 ? (int)qid[i] : -1;
+#pragma unroll
+        for (int k = 0; k < kQueueSlots; k++) cnt[k] += (uint32_t)__popcll(__ballot(q == k));
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < kQueueSlots; k++) s_cnt[k][wave] = cnt[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)kQueueSlots) {
+        const int k = (int)threadIdx.x;
+        uint32_t total = 0;
+        for (int w = 0; w < kBinWaves; w++) total += s_cnt[k][w];
+        uint32_t run = total ? atomicAdd(counters + k, total) : 0u;
+        for (int w = 0; w < kBinWaves; w++) {
+            const uint32_t c = s_cnt[k][w];
+            s_cnt[k][w] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    uint32_t off[kQueueSlots];
+#pragma unroll
+    for (int k = 0; k < kQueueSlots; k++) off[k] = s_cnt[k][wave];
+    for (uint32_t base = b0; base < b1; base += kBinBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const int q = i < b1 ? (int)qid[i] : -1;
+        const uint32_t slot = q >= 0 ? slot_of(i) : 0u;
+#pragma unroll
+        for (int k = 0; k < kQueueSlots; k++) {
+            const uint64_t m = __ballot(q == k);
+            if (q == k) queues[k][off[k] + mbcnt(m)] = slot;
+            off[k] += (uint32_t)__popcll(m);
+        }
+    }
+}
+
+struct RayQueueSlot {
+    const uint32_t* rayq;
+    __device__ __forceinline__ uint32_t operator()(uint32_t i) const { return rayq[i]; }
+};
+struct IdentitySlot {
+    __device__ __forceinline__ uint32_t operator()(uint32_t i) const { return i; }
+};
+
+__global__ __launch_bounds__(kBinBlock) void k_bin(PathState p, int cur) {
+    const uint32_t n = p.ctr->ray[cur];
+    RayQueueSlot so{cur ? p.rayq[1] : p.rayq[0]};
+    uint32_t* queues[kQueueSlots] = {p.q[0], p.q[1], p.q[2], p.q[3], p.q[4]};
+    bin_entries(p.qid, n, so, queues, p.ctr->queue);
+}
+
+struct QueueSet {
+    uint32_t* q[kQueueSlots];
+};
+
+__global__ __launch_bounds__(kBinBlock) void k_bin_batch(const uint8_t* qid, uint32_t n, QueueSet qs,
+                                                          unsigned int* counters) {
+    bin_entries(qid, n, IdentitySlot{}, qs.q, counters);
+}
+
+// Block-wide reservation for three output queues: one atomicAdd per queue per
+// block.  Every thread of the block must call it.
+__device__ __forceinline__ void block_reserve3(const bool pred[3], unsigned int* c0, unsigned int* c1,
+                                               unsigned int* c2, uint32_t pos[3]) {
+    __shared__ uint32_t s_off[3][4];
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t waves = blockDim.x >> 6;
+    uint32_t pre[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const uint64_t m = __ballot(pred[k]);
+        pre[k] = mbcnt(m);
+        if (lane == 0) s_off[k][wave] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3u) {
+        const int k = (int)threadIdx.x;
+        unsigned int* ctr = k == 0 ? c0 : k == 1 ? c1 : c2;
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < waves; w++) total += s_off[k][w];
+        uint32_t run = total ? atomicAdd(ctr, total) : 0u;
+        for (uint32_t w = 0; w < waves; w++) {
+            const uint32_t c = s_off[k][w];
+            s_off[k][w] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 3; k++) pos[k] = s_off[k][wave] + pre[k];
+    __syncthreads();
+}
+
 // The four material stages of one bounce in one launch: element e of the
 // virtual range [diffuse | ggx | mirror | emissive] is queue entry e - offset.
 // Each wave walks 64 consecutive entries, so a wave shades one material except
@@ -771,12 +881,9 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
     const uint32_t e1 = c0, e2 = c0 + c1, e3 = e2 + c2, total = e3 + c3;
     const uint32_t limit = fp->depth_limit, rr_start = fp->rr_start;
     const bool nee = fp->use_nee != 0u && sc.light_count > 0u;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t lane = lane_id();
 
-    for (uint32_t wbase = wave * 64u; wbase < total; wbase += waves * 64u) {
-        const uint32_t e = wbase + lane;
+    for (uint32_t bbase = blockIdx.x * 256u; bbase < total; bbase += gridDim.x * 256u) {
+        const uint32_t e = bbase + threadIdx.x;
         const bool valid = e < total;
         bool to_ray = false, to_regen = false;
         ShadowOut so;
@@ -960,17 +1067,17 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
             atomicAdd(acc + 1, add_L.y);
             atomicAdd(acc + 2, add_L.z);
         }
-        const uint32_t sh_pos = wave_reserve(&c->shadow, so.emit);
+        const bool pred[3] = {to_ray, so.emit, to_regen};
+        uint32_t pos[3];
+        block_reserve3(pred, &c->ray[nxt], &c->shadow, &c->queue[kQueueRegen], pos);
         if (so.emit) {
             p.sh_o[slot] = f4(so.o, so.tmax);
             p.sh_d[slot] = f4(so.d, 0.0f);
             p.sh_L[slot] = f4(so.L, __uint_as_float(pixel));
-            p.shadowq[sh_pos] = slot;
+            p.shadowq[pos[1]] = slot;
         }
-        const uint32_t ray_pos = wave_reserve(&c->ray[nxt], to_ray);
-        if (to_ray) (nxt ? p.rayq[1] : p.rayq[0])[ray_pos] = slot;
-        const uint32_t rg_pos = wave_reserve(&c->queue[kQueueRegen], to_regen);
-        if (to_regen) p.q[kQueueRegen][rg_pos] = slot;
+        if (to_ray) (nxt ? p.rayq[1] : p.rayq[0])[pos[0]] = slot;
+        if (to_regen) p.q[kQueueRegen][pos[2]] = slot;
     }
 }
 
@@ -1080,8 +1187,13 @@ hipError_t launch_generate(hipStream_t s, const PathState& p, bool init, int nxt
 
 hipError_t launch_closest(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool count,
                           const TraceConfig& cfg) {
-    WavefrontClosestSrc src{p, cur, count};
+    WavefrontClosestSrc src{p, cur};
     return count ? dispatch_trace<false, true>(s, src, sc, cfg) : dispatch_trace<false, false>(s, src, sc, cfg);
+}
+
+hipError_t launch_bin(hipStream_t s, const PathState& p, int cur, uint32_t grid) {
+    hipLaunchKernelGGL(k_bin, dim3(grid), dim3(kBinBlock), 0, s, p, cur);
+    return hipGetLastError();
 }
 
 hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, uint32_t grid, uint32_t block) {
@@ -1091,7 +1203,7 @@ hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, i
 }
 
 hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, bool count, const TraceConfig& cfg) {
-    WavefrontShadowSrc src{p, count};
+    WavefrontShadowSrc src{p};
     return count ? dispatch_trace<true, true>(s, src, sc, cfg) : dispatch_trace<true, false>(s, src, sc, cfg);
 }
 
@@ -1108,21 +1220,22 @@ hipError_t launch_postprocess(hipStream_t s, const PathState& p) {
 hipError_t launch_trace_batch(hipStream_t s, const SceneDev& sc, const float4* rays, uint32_t n, float4* hits,
                               uint32_t* occluded, uint32_t* nodes, uint32_t* tris, unsigned int* work, bool any,
                               const TraceConfig& cfg) {
-    BatchSrc src{rays, hits, occluded, nodes, tris, work, n};
+    BatchSrc src{rays, hits, occluded, nodes, tris, work, n, nullptr};
     const bool count = nodes != nullptr || tris != nullptr;
     if (any) return count ? dispatch_trace<true, true>(s, src, sc, cfg) : dispatch_trace<true, false>(s, src, sc, cfg);
     return count ? dispatch_trace<false, true>(s, src, sc, cfg) : dispatch_trace<false, false>(s, src, sc, cfg);
 }
 
-hipError_t launch_classify(hipStream_t s, const SceneDev& sc, const float4* rays, uint32_t n, uint32_t* const* queues,
-                           unsigned int* queue_counts, unsigned int* work, const TraceConfig& cfg) {
-    ClassifySrc src;
-    src.rays = rays;
-    for (int i = 0; i < kQueueSlots; i++) src.queues[i] = queues[i];
-    src.queue_counts = queue_counts;
-    src.work_ctr = work;
-    src.n = n;
-    return dispatch_trace<false, false>(s, src, sc, cfg);
+hipError_t launch_classify(hipStream_t s, const SceneDev& sc, const float4* rays, uint32_t n, uint8_t* qid,
+                           uint32_t* const* queues, unsigned int* queue_counts, unsigned int* work,
+                           const TraceConfig& cfg, uint32_t bin_grid) {
+    BatchSrc src{rays, nullptr, nullptr, nullptr, nullptr, work, n, qid};
+    hipError_t e = dispatch_trace<false, false>(s, src, sc, cfg);
+    if (e != hipSuccess) return e;
+    QueueSet qs;
+    for (int i = 0; i < kQueueSlots; i++) qs.q[i] = queues[i];
+    hipLaunchKernelGGL(k_bin_batch, dim3(bin_grid), dim3(kBinBlock), 0, s, qid, n, qs, queue_counts);
+    return hipGetLastError();
 }
 
 hipError_t launch_alias_sample(hipStream_t s, const SceneDev& sc, const float* u, uint32_t n, uint32_t* index, float* pmf) {

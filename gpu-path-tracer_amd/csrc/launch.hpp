@@ -35,6 +35,7 @@ struct PathState {                  // path pool, SoA by slot
     uint32_t* rayq[2];
     uint32_t* q[kQueueSlots];
     uint32_t* shadowq;
+    uint8_t* qid;                   // queue id of each ray-queue entry (closest-hit -> binning)
     Counters* ctr;
     const FrameParams* fp;
     float4* accum;
@@ -47,7 +48,8 @@ struct TraceConfig {
     uint32_t block = 128;           // threads per block
     uint32_t lds_stack = 16;        // stack entries per lane held in LDS
     uint32_t blocks = 0;            // persistent grid size
-    uint32_t refill = 16;           // idle lanes that trigger a fetch of new rays
+    uint32_t refill = 16;           // idle lanes that trigger a refill from the wave's chunk
+    uint32_t chunk = 256;           // rays claimed per wave with one atomic
     uint32_t spill_depth = 0;       // stack entries per lane in the global spill area
     uint2* spill = nullptr;         // [spill_depth][blocks*block]
     unsigned int* overflow = nullptr; // set when a push would exceed lds_stack+spill_depth
@@ -61,20 +63,21 @@ hipError_t launch_frame_begin(hipStream_t s, const PathState& p);
 hipError_t launch_fill_accum(hipStream_t s, const PathState& p);
 hipError_t launch_generate(hipStream_t s, const PathState& p, bool init, int nxt, uint32_t grid, uint32_t block);
 hipError_t launch_closest(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool count, const TraceConfig& cfg);
+hipError_t launch_bin(hipStream_t s, const PathState& p, int cur, uint32_t grid);
 hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, uint32_t grid, uint32_t block);
 hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, bool count, const TraceConfig& cfg);
 hipError_t launch_prepare(hipStream_t s, const PathState& p, int cur);
 hipError_t launch_postprocess(hipStream_t s, const PathState& p);
 
 // Batch hooks.  rays: n x 2 float4 (origin|tMin, direction|tMax).
-// Closest: hits[i] = (t, u, v, prim bits); any: occluded[i].  nodes/tris may be null.
+// Closest: hits[i] in pt_hit layout (prim bits, t, u, v); any: occluded[i].  nodes/tris may be null.
 hipError_t launch_trace_batch(hipStream_t s, const SceneDev& sc, const float4* rays, uint32_t n,
                               float4* hits, uint32_t* occluded, uint32_t* nodes, uint32_t* tris,
                               unsigned int* work, bool any, const TraceConfig& cfg);
-// Closest hit + wave-aggregated push of the ray index into 5 queues.
-hipError_t launch_classify(hipStream_t s, const SceneDev& sc, const float4* rays, uint32_t n,
+// Closest hit + the binning pass of the wavefront: ray indices scattered into 5 queues.
+hipError_t launch_classify(hipStream_t s, const SceneDev& sc, const float4* rays, uint32_t n, uint8_t* qid,
                            uint32_t* const* queues, unsigned int* queue_counts, unsigned int* work,
-                           const TraceConfig& cfg);
+                           const TraceConfig& cfg, uint32_t bin_grid);
 hipError_t launch_alias_sample(hipStream_t s, const SceneDev& sc, const float* u, uint32_t n,
                                uint32_t* index, float* pmf);
 hipError_t launch_zero_u32(hipStream_t s, unsigned int* p, uint32_t n);

@@ -113,6 +113,7 @@ PathState Renderer::path_state() const {
     p.rayq[1] = d_rayq_[1].as<uint32_t>();
     for (int i = 0; i < kQueueSlots; i++) p.q[i] = d_q_[i].as<uint32_t>();
     p.shadowq = d_shadowq_.as<uint32_t>();
+    p.qid = d_qid_.as<uint8_t>();
     p.ctr = d_ctr_.as<Counters>();
     p.fp = d_fp_.as<FrameParams>();
     p.accum = d_accum_.as<float4>();
@@ -157,6 +158,8 @@ pt_status Renderer::ensure_pool() {
     }
     d_shadowq_.release();
     PT_TRY_HIP(d_shadowq_.allocate(u1), "shadow queue allocation");
+    d_qid_.release();
+    PT_TRY_HIP(d_qid_.allocate((size_t)P), "queue id allocation");
     pool_capacity_ = P;
     return PT_OK;
 }
@@ -260,6 +263,7 @@ pt_status Renderer::launch_iteration(int cur, bool count) {
     const SceneDev sc = scene_dev();
     const uint32_t grid = (uint32_t)std::max(1, cu_count_) * 8u;
     PT_TRY_HIP(launch_closest(stream_, p, sc, cur, count, trace_), "closest-hit launch");
+    PT_TRY_HIP(launch_bin(stream_, p, cur, bin_grid()), "binning launch");
     PT_TRY_HIP(launch_shade(stream_, p, sc, cur, grid, 256), "shade launch");
     PT_TRY_HIP(launch_shadow(stream_, p, sc, count, trace_), "shadow launch");
     PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
@@ -356,15 +360,15 @@ pt_status Renderer::render_frame(const pt_render_settings& s, const pt_camera& c
     const uint64_t waves = (fp.total_samples + pool_capacity_ - 1) / pool_capacity_ + 1;
     const uint64_t max_iters = 2 * waves * (uint64_t)(c.depth + 1) + 16;
 
-    double closest_ms = 0, shadow_ms = 0, shade_ms = 0, regen_ms = 0;
+    double closest_ms = 0, shadow_ms = 0, shade_ms = 0, regen_ms = 0, bin_ms = 0;
     uint32_t closest_launches = 0;
     if (profile) {
-        hipEvent_t ev[6];
+        hipEvent_t ev[7];
         for (auto& e : ev) PT_TRY_HIP(hipEventCreate(&e), "event");
         struct Guard6 {
             hipEvent_t* e;
             ~Guard6() {
-                for (int i = 0; i < 6; i++) (void)hipEventDestroy(e[i]);
+                for (int i = 0; i < 7; i++) (void)hipEventDestroy(e[i]);
             }
         } g6{ev};
         int cur = 0;
@@ -373,22 +377,25 @@ pt_status Renderer::render_frame(const pt_render_settings& s, const pt_camera& c
             PT_TRY_HIP(hipEventRecord(ev[0], stream_), "event");
             PT_TRY_HIP(launch_closest(stream_, p, sc, cur, false, trace_), "closest-hit launch");
             PT_TRY_HIP(hipEventRecord(ev[1], stream_), "event");
-            PT_TRY_HIP(launch_shade(stream_, p, sc, cur, grid, 256), "shade launch");
+            PT_TRY_HIP(launch_bin(stream_, p, cur, bin_grid()), "binning launch");
             PT_TRY_HIP(hipEventRecord(ev[2], stream_), "event");
-            PT_TRY_HIP(launch_shadow(stream_, p, sc, false, trace_), "shadow launch");
+            PT_TRY_HIP(launch_shade(stream_, p, sc, cur, grid, 256), "shade launch");
             PT_TRY_HIP(hipEventRecord(ev[3], stream_), "event");
-            PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
+            PT_TRY_HIP(launch_shadow(stream_, p, sc, false, trace_), "shadow launch");
             PT_TRY_HIP(hipEventRecord(ev[4], stream_), "event");
+            PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
+            PT_TRY_HIP(hipEventRecord(ev[5], stream_), "event");
             PT_TRY_HIP(launch_prepare(stream_, p, cur), "prepare launch");
             PT_TRY_HIP(hipMemcpyAsync(h_ctr_ring_, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream_), "counters");
-            PT_TRY_HIP(hipEventRecord(ev[5], stream_), "event");
-            PT_TRY_HIP(hipEventSynchronize(ev[5]), "sync");
-            float ms[4];
-            for (int k = 0; k < 4; k++) PT_TRY_HIP(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]), "elapsed");
+            PT_TRY_HIP(hipEventRecord(ev[6], stream_), "event");
+            PT_TRY_HIP(hipEventSynchronize(ev[6]), "sync");
+            float ms[5];
+            for (int k = 0; k < 5; k++) PT_TRY_HIP(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]), "elapsed");
             closest_ms += ms[0];
-            shade_ms += ms[1];
-            shadow_ms += ms[2];
-            regen_ms += ms[3];
+            bin_ms += ms[1];
+            shade_ms += ms[2];
+            shadow_ms += ms[3];
+            regen_ms += ms[4];
             closest_launches++;
             done_samples_.store(h_ctr_ring_[0].launched, std::memory_order_relaxed);
             cur ^= 1;
@@ -448,6 +455,7 @@ pt_status Renderer::render_frame(const pt_render_settings& s, const pt_camera& c
         stats->shadow_ms = shadow_ms;
         stats->shade_ms = shade_ms;
         stats->regen_ms = regen_ms;
+        stats->bin_ms = bin_ms;
         stats->closest_nodes = ctr.closest_nodes;
         stats->closest_tris = ctr.closest_tris;
         stats->shadow_nodes = ctr.shadow_nodes;
@@ -747,6 +755,7 @@ pt_status Renderer::Classify(const pt_ray* rays, uint32_t n, uint32_t* queue_of_
     if (n == 0) return PT_OK;
     PT_TRY_HIP(d_hook_in_.allocate((size_t)n * sizeof(pt_ray)), "hook input");
     PT_TRY_HIP(d_hook_q_.allocate((size_t)n * kQueueSlots * sizeof(uint32_t) + 64), "hook queues");
+    PT_TRY_HIP(d_hook_aux_.allocate((size_t)n), "hook queue ids");
     PT_TRY_HIP(hipMemcpyAsync(d_hook_in_.as<void>(), rays, (size_t)n * sizeof(pt_ray), hipMemcpyHostToDevice, stream_), "hook input");
     uint32_t* qbase = d_hook_q_.as<uint32_t>();
     uint32_t* queues[kQueueSlots];
@@ -754,7 +763,8 @@ pt_status Renderer::Classify(const pt_ray* rays, uint32_t n, uint32_t* queue_of_
     unsigned int* qcounts = reinterpret_cast<unsigned int*>(qbase + (size_t)kQueueSlots * n);
     PT_TRY_HIP(launch_zero_u32(stream_, d_work_.as<unsigned int>(), 1), "zero");
     PT_TRY_HIP(launch_zero_u32(stream_, qcounts, kQueueSlots), "zero");
-    PT_TRY_HIP(launch_classify(stream_, scene_dev(), d_hook_in_.as<float4>(), n, queues, qcounts, d_work_.as<unsigned int>(), trace_),
+    PT_TRY_HIP(launch_classify(stream_, scene_dev(), d_hook_in_.as<float4>(), n, d_hook_aux_.as<uint8_t>(), queues,
+                               qcounts, d_work_.as<unsigned int>(), trace_, bin_grid()),
                "classify launch");
     std::vector<uint32_t> host((size_t)n * kQueueSlots + 16);
     PT_TRY_HIP(hipMemcpyAsync(host.data(), qbase, ((size_t)n * kQueueSlots + kQueueSlots) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "hook queues");

@@ -3,6 +3,7 @@
 // (Core/Include/Core/Graphics/Cuda/PathTracing/Renderer.hpp:42-189).
 #pragma once
 
+#include <algorithm>
 #include <atomic>
 #include <condition_variable>
 #include <cstdint>
@@ -93,6 +94,7 @@ private:
     void simulation_loop(uint32_t start_frame);
     void invalidate_graph();
     PathState path_state() const;
+    uint32_t bin_grid() const { return (uint32_t)std::max(1, cu_count_) * 4u; }
     SceneDev scene_dev() const;
 
     int device_ = 0;
@@ -109,7 +111,7 @@ private:
     pt_tuning tuning_{};
     uint32_t pool_capacity_ = 0;
     DeviceBuffer d_ray_o_, d_ray_d_, d_hit_, d_thr_, d_meta_, d_sh_o_, d_sh_d_, d_sh_L_;
-    DeviceBuffer d_rayq_[2], d_q_[kQueueSlots], d_shadowq_;
+    DeviceBuffer d_rayq_[2], d_q_[kQueueSlots], d_shadowq_, d_qid_;
     DeviceBuffer d_ctr_, d_fp_, d_work_, d_overflow_;
     FrameParams* h_fp_ = nullptr;           // pinned staging
     Counters* h_ctr_ring_ = nullptr;        // pinned, kRing entries

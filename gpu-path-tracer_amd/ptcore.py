@@ -59,7 +59,7 @@ class FrameStats(C.Structure):
                 ("samples", C.c_uint64), ("iterations", C.c_uint32), ("closest_launches", C.c_uint32),
                 ("closest_ms", C.c_double), ("shadow_ms", C.c_double), ("shade_ms", C.c_double),
                 ("regen_ms", C.c_double), ("closest_nodes", C.c_uint64), ("closest_tris", C.c_uint64),
-                ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64)]
+                ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64), ("bin_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -150,6 +150,7 @@ typedef struct pt_frame_stats {
     uint64_t closest_tris;       /* triangles tested by closest-hit (instrumented frames only) */
     uint64_t shadow_nodes;
     uint64_t shadow_tris;
+    double bin_ms;               /* summed time of the queue-binning launches (profiled frames only) */
 } pt_frame_stats;
 
 /* Frame-mode flags for pt_render_frame. */
