@@ -20,9 +20,9 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-path-tracer_amd"))
 
 METRIC = "Mrays/s (extension+shadow) on 1 MI355X at equal relMSE; p50/p99 request latency"
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes of one closest-hit ray: queue index + ray (4 + 32) in,
-# hit record + queue push (16 + 4) out, 128 B per BVH4 node, 48 B per triangle
-RAY_IN_B, HIT_OUT_B, NODE_B, TRI_B = 36, 20, 128, 48
+# algorithmic bytes of one closest-hit ray: ray record (origin|tMin, direction|tMax)
+# 32 B in, hit record + queue id 17 B out, 128 B per BVH4 node, 48 B per triangle
+RAY_IN_B, HIT_OUT_B, NODE_B, TRI_B = 32, 17, 128, 48
 
 
 def log(msg):
@@ -118,9 +118,13 @@ def main():
     bytes_per_launch = bytes_per_ray * prof["extension_rays"] / launches
     avg_launch_ms = prof["closest_ms"] / launches
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    sh_nodes = inst["shadow_nodes"] / max(1, inst["shadow_rays"])
+    sh_tris = inst["shadow_tris"] / max(1, inst["shadow_rays"])
+    log(f"shadow rays: {sh_nodes:.2f} nodes + {sh_tris:.2f} tris per ray, "
+        f"{inst['shadow_rays'] / max(1, launches):.0f} rays per launch")
     log(f"closest-hit: {launches} launches, avg {avg_launch_ms:.3f} ms, {nodes_per_ray:.2f} nodes + "
         f"{tris_per_ray:.2f} tris per ray = {bytes_per_ray:.0f} B/ray; shade {prof['shade_ms']:.1f} ms, "
-        f"shadow {prof['shadow_ms']:.1f} ms, regen {prof['regen_ms']:.1f} ms, closest {prof['closest_ms']:.1f} ms")
+        f"shadow {prof['shadow_ms']:.1f} ms, regen {prof['regen_ms']:.1f} ms, bin {prof['bin_ms']:.1f} ms, closest {prof['closest_ms']:.1f} ms")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_trace<closest>", "bytes_per_ray": round(bytes_per_ray, 1),

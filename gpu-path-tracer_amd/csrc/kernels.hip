@@ -2,14 +2,19 @@
 //
 // Stages of one bounce (the loop of Renderer::RenderFrame,
 // Core/Source/Graphics/Cuda/PathTracing/Renderer.cpp:517-623):
-//   closest  - persistent BVH4 traversal with per-lane ray refill and an LDS
-//              stack; pushes each path slot into its material queue or regen
-//   shade    - one launch over the diffuse | ggx | mirror | emissive queues
-//   shadow   - persistent any-hit traversal of the NEE shadow rays
-//   generate - camera rays for recycled slots
-//   prepare  - one wave: queue bookkeeping, next ray count, ray statistics
-// All queue pushes are wave-aggregated: a ballot per target queue, one atomic
-// per wave and queue, lane offsets from mbcnt.
+//   closest  - persistent BVH4 traversal with per-lane ray refill from
+//              per-wave chunks and an LDS stack; writes each entry's hit and
+//              target queue by queue position
+//   bin      - scatters queue positions into the diffuse / ggx / mirror /
+//              emissive queues (five atomics per block)
+//   shade    - one launch over the four material queues; continuing paths are
+//              written compacted into the next ray queue, NEE shadow rays
+//              into the shadow queue
+//   shadow   - persistent any-hit traversal of the shadow queue
+//   generate - camera paths for the free tail of the next ray queue
+//   prepare  - one thread: counters, next ray count, ray statistics
+// Queue space is reserved per block or per wave (ballot + mbcnt), never per
+// lane: same-address atomics are expensive across the eight XCDs.
 #include <hip/hip_runtime.h>
 
 #include "launch.hpp"
@@ -172,6 +177,10 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
     const uint32_t cap = LDS + cfg.spill_depth;
     const uint32_t n = src.count();
     unsigned int* work = src.work();
+    // chunk size from the ray count: about four chunks per wave of the grid,
+    // one ray per lane at least, at most cfg.chunk
+    const uint32_t waves = stride >> 6;
+    const uint32_t chunk = min(max((n / (4u * waves) + 63u) & ~63u, 64u), max(cfg.chunk, 64u));
 
     bool active = false;
     bool exhausted = false;
@@ -188,16 +197,16 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
             const uint32_t nidle = (uint32_t)__popcll(idle);
             if (nidle >= cfg.refill || nidle == 64u) {
                 if (chunk_next >= chunk_end) {
-                    // one atomic claims `chunk` rays for the whole wave
+                    // one atomic claims a chunk of rays for the whole wave
                     const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
                     uint32_t base = 0;
-                    if (lane_id() == leader) base = atomicAdd(work, cfg.chunk);
+                    if (lane_id() == leader) base = atomicAdd(work, chunk);
                     base = readlane_u32(base, leader);
                     if (base >= n) {
                         exhausted = true;
                     } else {
                         chunk_next = base;
-                        chunk_end = min(base + cfg.chunk, n);
+                        chunk_end = min(base + chunk, n);
                     }
                 }
                 if (chunk_next < chunk_end) {
@@ -320,24 +329,22 @@ struct WavefrontClosestSrc {
     int cur;
     __device__ __forceinline__ uint32_t count() const { return p.ctr->ray[cur]; }
     __device__ __forceinline__ unsigned int* work() const { return &p.ctr->work_closest; }
+    // rays are stored by queue position: consecutive lanes read consecutive records
     __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax) const {
-        const uint32_t slot = (cur ? p.rayq[1] : p.rayq[0])[i];
-        const float4 a = p.ray_o[slot], b = p.ray_d[slot];
+        const float4 a = (cur ? p.ray_o[1] : p.ray_o[0])[i];
+        const float4 b = (cur ? p.ray_d[1] : p.ray_d[0])[i];
         o = xyz(a);
         tmin = a.w;
         d = xyz(b);
         tmax = b.w;
         return i;
     }
-    // Writes the hit to the path slot and the target queue of ray-queue entry
-    // i; the binning pass scatters the slots into the queues afterwards.
+    // Hit and target queue of ray-queue entry i; the binning pass scatters
+    // the entries into the material queues afterwards.
     __device__ __forceinline__ void finish_closest(uint32_t i, uint32_t prim, float t, float u, float v,
                                                    uint32_t queue, uint32_t, uint32_t) const {
         const bool hit = prim != kInvalid;
-        if (hit) {
-            const uint32_t slot = (cur ? p.rayq[1] : p.rayq[0])[i];
-            p.hit[slot] = make_float4(t, u, v, __uint_as_float(prim));
-        }
+        if (hit) p.hit[i] = make_float4(t, u, v, __uint_as_float(prim));
         p.qid[i] = (uint8_t)(hit ? queue : (uint32_t)kQueueRegen);
     }
     __device__ void finish_any(uint32_t, bool, uint32_t, uint32_t) const {}
@@ -351,18 +358,17 @@ struct WavefrontShadowSrc {
     __device__ __forceinline__ uint32_t count() const { return p.ctr->shadow; }
     __device__ __forceinline__ unsigned int* work() const { return &p.ctr->work_shadow; }
     __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax) const {
-        const uint32_t slot = p.shadowq[i];
-        const float4 a = p.sh_o[slot], b = p.sh_d[slot];
+        const float4 a = p.sh_o[i], b = p.sh_d[i];
         o = xyz(a);
         tmax = a.w;
         d = xyz(b);
         tmin = kMinT;
-        return slot;
+        return i;
     }
     __device__ void finish_closest(uint32_t, uint32_t, float, float, float, uint32_t, uint32_t, uint32_t) const {}
-    __device__ __forceinline__ void finish_any(uint32_t slot, bool occluded, uint32_t nodes, uint32_t tris) const {
+    __device__ __forceinline__ void finish_any(uint32_t i, bool occluded, uint32_t, uint32_t) const {
         if (!occluded) {
-            const float4 L4 = p.sh_L[slot];
+            const float4 L4 = p.sh_L[i];
             V3 L = xyz(L4);
             // ClampRadiance(radiance, 20), Intersect.cu:125-134
             const float l2 = dot(L, L);
@@ -684,7 +690,9 @@ __global__ void k_fill_accum(float4* accum, uint32_t n) {
         accum[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// InitializePathsKernel / RegeneratePathsKernel, PathGeneration.cu:56-102
+// InitializePathsKernel / RegeneratePathsKernel, PathGeneration.cu:56-102.
+// New paths go to the free tail of the next ray queue: positions
+// [ray[nxt], ray[nxt] + n) with n = min(free positions, samples left).
 __global__ void k_generate(PathState p, int init, int nxt) {
     const FrameParams* fp = p.fp;
     Counters* c = p.ctr;
@@ -695,21 +703,23 @@ __global__ void k_generate(PathState p, int init, int nxt) {
         base = 0;
         launched = 0;
     } else {
-        const uint64_t total = fp->total_samples;
         launched = c->launched;
-        const uint64_t remaining = total - launched;
-        const uint32_t regen = c->queue[kQueueRegen];
-        n = (uint64_t)regen < remaining ? regen : (uint32_t)remaining;
+        const uint64_t remaining = fp->total_samples - launched;
         base = c->ray[nxt];
+        const uint32_t free_slots = p.capacity - base;
+        n = (uint64_t)free_slots < remaining ? free_slots : (uint32_t)remaining;
     }
     const uint32_t W = fp->width, H = fp->height, spp = fp->spp, axis = fp->spp_axis;
     const V3 co = v3(fp->cam.origin[0], fp->cam.origin[1], fp->cam.origin[2]);
     const V3 llc = v3(fp->cam.llc[0], fp->cam.llc[1], fp->cam.llc[2]);
     const V3 hor = v3(fp->cam.horizontal[0], fp->cam.horizontal[1], fp->cam.horizontal[2]);
     const V3 ver = v3(fp->cam.vertical[0], fp->cam.vertical[1], fp->cam.vertical[2]);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t slot = init ? i : p.q[kQueueRegen][i];
-        const uint64_t sample = launched + i;
+    float4* ro = nxt ? p.ray_o[1] : p.ray_o[0];
+    float4* rd = nxt ? p.ray_d[1] : p.ray_d[0];
+    float4* th = nxt ? p.thr[1] : p.thr[0];
+    uint4* me = nxt ? p.meta[1] : p.meta[0];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint64_t sample = launched + k;
         const PcgStream s = pcg_seed(fp->seed, sample);
         Draws dr{s.state, s.inc};
         const float jx = dr.next();
@@ -724,11 +734,11 @@ __global__ void k_generate(PathState p, int init, int nxt) {
         const float v = ((float)y + ((float)cy + jy) / (float)axis) / (float)H;
         // GenerateCameraRay, PathGeneration.cu:35-43
         const V3 d = normalize(llc + u * hor + v * ver - co);
-        p.ray_o[slot] = f4(co, kMinT);
-        p.ray_d[slot] = f4(d, kMaxT);
-        p.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-        p.meta[slot] = make_uint4(pixel, kDeltaBit, (uint32_t)sample, (uint32_t)(sample >> 32));
-        (nxt ? p.rayq[1] : p.rayq[0])[base + i] = slot;
+        const uint32_t pos = base + k;
+        ro[pos] = f4(co, kMinT);
+        rd[pos] = f4(d, kMaxT);
+        th[pos] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        me[pos] = make_uint4(pixel, kDeltaBit, (uint32_t)sample, (uint32_t)(sample >> 32));
     }
 }
 
@@ -738,10 +748,9 @@ __global__ void k_prepare(PathState p, int cur) {
     if (threadIdx.x == 0) {
         Counters* c = p.ctr;
         const int nxt = cur ^ 1;
-        const uint64_t total = p.fp->total_samples;
-        const uint64_t remaining = total - c->launched;
-        const uint32_t regen = c->queue[kQueueRegen];
-        const uint32_t nr = (uint64_t)regen < remaining ? regen : (uint32_t)remaining;
+        const uint64_t remaining = p.fp->total_samples - c->launched;
+        const uint32_t free_slots = p.capacity - c->ray[nxt];
+        const uint32_t nr = (uint64_t)free_slots < remaining ? free_slots : (uint32_t)remaining;
         const uint32_t traced = c->ray[cur];
         c->ext_rays += traced;
         c->shadow_rays += c->shadow;
@@ -806,25 +815,20 @@ __device__ __forceinline__ void bin_entries(const uint8_t* qid, uint32_t n, cons
 #pragma unroll
         for (int k = 0; k < kQueueSlots; k++) {
             const uint64_t m = __ballot(q == k);
-            if (q == k) queues[k][off[k] + mbcnt(m)] = slot;
+            if (q == k && queues[k]) queues[k][off[k] + mbcnt(m)] = slot;
             off[k] += (uint32_t)__popcll(m);
         }
     }
 }
 
-struct RayQueueSlot {
-    const uint32_t* rayq;
-    __device__ __forceinline__ uint32_t operator()(uint32_t i) const { return rayq[i]; }
-};
 struct IdentitySlot {
     __device__ __forceinline__ uint32_t operator()(uint32_t i) const { return i; }
 };
 
 __global__ __launch_bounds__(kBinBlock) void k_bin(PathState p, int cur) {
     const uint32_t n = p.ctr->ray[cur];
-    RayQueueSlot so{cur ? p.rayq[1] : p.rayq[0]};
-    uint32_t* queues[kQueueSlots] = {p.q[0], p.q[1], p.q[2], p.q[3], p.q[4]};
-    bin_entries(p.qid, n, so, queues, p.ctr->queue);
+    uint32_t* queues[kQueueSlots] = {p.q[0], p.q[1], p.q[2], p.q[3], nullptr};
+    bin_entries(p.qid, n, IdentitySlot{}, queues, p.ctr->queue);
 }
 
 struct QueueSet {
@@ -836,24 +840,22 @@ __global__ __launch_bounds__(kBinBlock) void k_bin_batch(const uint8_t* qid, uin
     bin_entries(qid, n, IdentitySlot{}, qs.q, counters);
 }
 
-// Block-wide reservation for three output queues: one atomicAdd per queue per
+// Block-wide reservation for two output queues: one atomicAdd per queue per
 // block.  Every thread of the block must call it.
-__device__ __forceinline__ void block_reserve3(const bool pred[3], unsigned int* c0, unsigned int* c1,
-                                               unsigned int* c2, uint32_t pos[3]) {
-    __shared__ uint32_t s_off[3][4];
+__device__ __forceinline__ void block_reserve2(bool pred0, bool pred1, unsigned int* c0, unsigned int* c1,
+                                               uint32_t& pos0, uint32_t& pos1) {
+    __shared__ uint32_t s_off[2][4];
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
     const uint32_t waves = blockDim.x >> 6;
-    uint32_t pre[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const uint64_t m = __ballot(pred[k]);
-        pre[k] = mbcnt(m);
-        if (lane == 0) s_off[k][wave] = (uint32_t)__popcll(m);
+    const uint64_t m0 = __ballot(pred0), m1 = __ballot(pred1);
+    if (lane == 0) {
+        s_off[0][wave] = (uint32_t)__popcll(m0);
+        s_off[1][wave] = (uint32_t)__popcll(m1);
     }
     __syncthreads();
-    if (threadIdx.x < 3u) {
+    if (threadIdx.x < 2u) {
         const int k = (int)threadIdx.x;
-        unsigned int* ctr = k == 0 ? c0 : k == 1 ? c1 : c2;
+        unsigned int* ctr = k == 0 ? c0 : c1;
         uint32_t total = 0;
         for (uint32_t w = 0; w < waves; w++) total += s_off[k][w];
         uint32_t run = total ? atomicAdd(ctr, total) : 0u;
@@ -864,8 +866,8 @@ __device__ __forceinline__ void block_reserve3(const bool pred[3], unsigned int*
         }
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 3; k++) pos[k] = s_off[k][wave] + pre[k];
+    pos0 = s_off[0][wave] + mbcnt(m0);
+    pos1 = s_off[1][wave] + mbcnt(m1);
     __syncthreads();
 }
 
@@ -885,10 +887,13 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
     for (uint32_t bbase = blockIdx.x * 256u; bbase < total; bbase += gridDim.x * 256u) {
         const uint32_t e = bbase + threadIdx.x;
         const bool valid = e < total;
-        bool to_ray = false, to_regen = false;
+        bool to_ray = false;
         ShadowOut so;
         so.emit = false;
-        uint32_t slot = 0;
+        V3 new_o = v3(0.0f), new_d = v3(0.0f), new_thr = v3(0.0f);
+        uint32_t new_flags = 0;
+        uint4 m = make_uint4(0, 0, 0, 0);
+        float cur_ior = 1.0f;
         V3 add_L = v3(0.0f);
         bool add = false;
         uint32_t pixel = 0;
@@ -900,14 +905,15 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
             else if (e < e3) { q = 2; j = e - e2; }
             else { q = 3; j = e - e3; }
             const uint32_t* qa = q == 0 ? p.q[0] : q == 1 ? p.q[1] : q == 2 ? p.q[2] : p.q[3];
-            slot = qa[j];
-            const uint4 m = p.meta[slot];
+            const uint32_t i = qa[j];
+            m = (cur ? p.meta[1] : p.meta[0])[i];
             pixel = m.x;
             const uint32_t depth = m.y & 0xFFu;
             const bool delta = (m.y & kDeltaBit) != 0u;
-            const float4 th4 = p.thr[slot];
+            const float4 th4 = (cur ? p.thr[1] : p.thr[0])[i];
             const V3 thr = xyz(th4);
-            const float4 hh = p.hit[slot];
+            cur_ior = th4.w;
+            const float4 hh = p.hit[i];
             const uint32_t prim = __float_as_uint(hh.w);
             if (q == 3) {
                 // EvaluateEmissiveKernel, MaterialEval.cu:882-912
@@ -917,9 +923,9 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
                     add_L = thr * v3(mr->emission[0], mr->emission[1], mr->emission[2]);
                     add = true;
                 }
-                to_regen = true;
             } else {
-                const float4 o4 = p.ray_o[slot], d4 = p.ray_d[slot];
+                const float4 o4 = (cur ? p.ray_o[1] : p.ray_o[0])[i];
+                const float4 d4 = (cur ? p.ray_d[1] : p.ray_d[0])[i];
                 const V3 ro = xyz(o4), rdir = xyz(d4);
                 uint32_t mi;
                 const Hit h = resolve_hit(sc.shade + prim, ro, rdir, hh.x, hh.y, hh.z, mi);
@@ -971,7 +977,6 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
                     const float alpha = m1.w;
                     const float metallic = m2.x;
                     const float ior = m2.y;
-                    const float cur_ior = th4.w;
                     const float df0 = sqr((cur_ior - ior) / (cur_ior + ior));
                     const V3 f0 = lerp3(v3(df0), base, metallic);
                     const V3 diffc = base * (1.0f - metallic);
@@ -1047,17 +1052,12 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
                     const Continuation k = continue_path(depth, thr, limit, rr_start, weight, u3);
                     if (!k.terminated) {
                         // SpawnScatteredRay, MaterialEval.cu:148-156
-                        const V3 no = ro + rdir * hh.x + h.gn * 1e-4f;
-                        p.ray_o[slot] = f4(no, kMinT);
-                        p.ray_d[slot] = f4(wi, kMaxT);
-                        p.thr[slot] = f4(k.thr, th4.w);
-                        p.meta[slot] = make_uint4(m.x, k.depth | (new_delta ? kDeltaBit : 0u), m.z, m.w);
+                        new_o = ro + rdir * hh.x + h.gn * 1e-4f;
+                        new_d = wi;
+                        new_thr = k.thr;
+                        new_flags = k.depth | (new_delta ? kDeltaBit : 0u);
                         to_ray = true;
-                    } else {
-                        to_regen = true;
                     }
-                } else {
-                    to_regen = true;
                 }
             }
         }
@@ -1067,17 +1067,19 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
             atomicAdd(acc + 1, add_L.y);
             atomicAdd(acc + 2, add_L.z);
         }
-        const bool pred[3] = {to_ray, so.emit, to_regen};
-        uint32_t pos[3];
-        block_reserve3(pred, &c->ray[nxt], &c->shadow, &c->queue[kQueueRegen], pos);
+        uint32_t ray_pos, sh_pos;
+        block_reserve2(to_ray, so.emit, &c->ray[nxt], &c->shadow, ray_pos, sh_pos);
         if (so.emit) {
-            p.sh_o[slot] = f4(so.o, so.tmax);
-            p.sh_d[slot] = f4(so.d, 0.0f);
-            p.sh_L[slot] = f4(so.L, __uint_as_float(pixel));
-            p.shadowq[pos[1]] = slot;
+            p.sh_o[sh_pos] = f4(so.o, so.tmax);
+            p.sh_d[sh_pos] = f4(so.d, 0.0f);
+            p.sh_L[sh_pos] = f4(so.L, __uint_as_float(pixel));
         }
-        if (to_ray) (nxt ? p.rayq[1] : p.rayq[0])[pos[0]] = slot;
-        if (to_regen) p.q[kQueueRegen][pos[2]] = slot;
+        if (to_ray) {
+            (nxt ? p.ray_o[1] : p.ray_o[0])[ray_pos] = f4(new_o, kMinT);
+            (nxt ? p.ray_d[1] : p.ray_d[0])[ray_pos] = f4(new_d, kMaxT);
+            (nxt ? p.thr[1] : p.thr[0])[ray_pos] = f4(new_thr, cur_ior);
+            (nxt ? p.meta[1] : p.meta[0])[ray_pos] = make_uint4(m.x, new_flags, m.z, m.w);
+        }
     }
 }
 

@@ -23,19 +23,20 @@ struct SceneDev {
     uint32_t tri_count;
 };
 
-struct PathState {                  // path pool, SoA by slot
-    float4* ray_o;                  // origin, tMin
-    float4* ray_d;                  // direction, tMax
-    float4* hit;                    // t, u, v, prim (bits)
-    float4* thr;                    // throughput, medium ior
-    uint4* meta;                    // pixel, depth|delta, sample lo, sample hi
-    float4* sh_o;                   // shadow origin, tMax
-    float4* sh_d;                   // shadow direction
-    float4* sh_L;                   // shadow radiance, pixel (bits)
-    uint32_t* rayq[2];
-    uint32_t* q[kQueueSlots];
-    uint32_t* shadowq;
-    uint8_t* qid;                   // queue id of each ray-queue entry (closest-hit -> binning)
+// Path state, compacted every bounce: ray queue k holds, at position i, the
+// full state of one live path.  Shading writes each continuing path into the
+// next queue; new camera paths fill the free tail.  No slot indirection.
+struct PathState {
+    float4* ray_o[2];               // origin, tMin
+    float4* ray_d[2];               // direction, tMax
+    float4* thr[2];                 // throughput, medium ior
+    uint4* meta[2];                 // pixel, depth|delta, sample lo, sample hi
+    float4* hit;                    // t, u, v, prim (bits) of current-queue entry i
+    uint8_t* qid;                   // target queue of current-queue entry i
+    uint32_t* q[kQueueSlots];       // material queues: positions in the current ray queue
+    float4* sh_o;                   // shadow queue: origin, tMax
+    float4* sh_d;                   // direction
+    float4* sh_L;                   // radiance, pixel (bits)
     Counters* ctr;
     const FrameParams* fp;
     float4* accum;

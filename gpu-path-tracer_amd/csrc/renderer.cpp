@@ -9,8 +9,8 @@
 
 namespace pt {
 
-// PathTracerDefaults.hpp:9-35
-constexpr uint32_t kDefaultPool = 1u << 20;
+// Limits of PathTracerDefaults.hpp:9-35
+constexpr uint32_t kDefaultPool = 1u << 22;   // path slots; the reference uses 2^20 (PathTracerDefaults.hpp:9)
 constexpr uint32_t kMinWidth = 1, kMaxWidth = 7680, kMinHeight = 1, kMaxHeight = 4320;
 constexpr uint32_t kMinSpp = 1, kMaxSpp = 10000000;
 constexpr uint32_t kMinDepth = 1, kMaxDepth = 100;
@@ -101,19 +101,18 @@ void Renderer::invalidate_graph() {
 
 PathState Renderer::path_state() const {
     PathState p{};
-    p.ray_o = d_ray_o_.as<float4>();
-    p.ray_d = d_ray_d_.as<float4>();
+    for (int k = 0; k < 2; k++) {
+        p.ray_o[k] = d_ray_o_[k].as<float4>();
+        p.ray_d[k] = d_ray_d_[k].as<float4>();
+        p.thr[k] = d_thr_[k].as<float4>();
+        p.meta[k] = d_meta_[k].as<uint4>();
+    }
     p.hit = d_hit_.as<float4>();
-    p.thr = d_thr_.as<float4>();
-    p.meta = d_meta_.as<uint4>();
+    p.qid = d_qid_.as<uint8_t>();
+    for (int i = 0; i < kQueueSlots; i++) p.q[i] = d_q_[i].as<uint32_t>();
     p.sh_o = d_sh_o_.as<float4>();
     p.sh_d = d_sh_d_.as<float4>();
     p.sh_L = d_sh_L_.as<float4>();
-    p.rayq[0] = d_rayq_[0].as<uint32_t>();
-    p.rayq[1] = d_rayq_[1].as<uint32_t>();
-    for (int i = 0; i < kQueueSlots; i++) p.q[i] = d_q_[i].as<uint32_t>();
-    p.shadowq = d_shadowq_.as<uint32_t>();
-    p.qid = d_qid_.as<uint8_t>();
     p.ctr = d_ctr_.as<Counters>();
     p.fp = d_fp_.as<FrameParams>();
     p.accum = d_accum_.as<float4>();
@@ -138,26 +137,19 @@ SceneDev Renderer::scene_dev() const {
 
 pt_status Renderer::ensure_pool() {
     const uint32_t P = tuning_.pool_size;
-    if (P == pool_capacity_ && d_ray_o_.bytes() > 0) return PT_OK;
+    if (P == pool_capacity_ && d_hit_.bytes() > 0) return PT_OK;
     invalidate_graph();
-    const size_t f4 = sizeof(float4) * (size_t)P, u4 = sizeof(uint4) * (size_t)P, u1 = sizeof(uint32_t) * (size_t)P;
-    DeviceBuffer* f4s[] = {&d_ray_o_, &d_ray_d_, &d_hit_, &d_thr_, &d_sh_o_, &d_sh_d_, &d_sh_L_};
+    const size_t f4 = sizeof(float4) * (size_t)P, u1 = sizeof(uint32_t) * (size_t)P;
+    DeviceBuffer* f4s[] = {&d_ray_o_[0], &d_ray_o_[1], &d_ray_d_[0], &d_ray_d_[1], &d_thr_[0], &d_thr_[1],
+                           &d_meta_[0], &d_meta_[1], &d_hit_, &d_sh_o_, &d_sh_d_, &d_sh_L_};
     for (DeviceBuffer* b : f4s) {
         b->release();
-        PT_TRY_HIP(b->allocate(f4), "path pool allocation");
+        PT_TRY_HIP(b->allocate(f4), "path state allocation");
     }
-    d_meta_.release();
-    PT_TRY_HIP(d_meta_.allocate(u4), "path pool allocation");
-    for (auto& b : d_rayq_) {
-        b.release();
-        PT_TRY_HIP(b.allocate(u1), "ray queue allocation");
+    for (int i = 0; i < kQueueSlots; i++) {
+        d_q_[i].release();
+        if (i < kQueueCount) PT_TRY_HIP(d_q_[i].allocate(u1), "material queue allocation");
     }
-    for (auto& b : d_q_) {
-        b.release();
-        PT_TRY_HIP(b.allocate(u1), "material queue allocation");
-    }
-    d_shadowq_.release();
-    PT_TRY_HIP(d_shadowq_.allocate(u1), "shadow queue allocation");
     d_qid_.release();
     PT_TRY_HIP(d_qid_.allocate((size_t)P), "queue id allocation");
     pool_capacity_ = P;
@@ -854,7 +846,7 @@ pt_status Renderer::SetTuning(const pt_tuning& t) {
     if (t.pool_size < 1024 || t.pool_size > (1u << 26)) return fail(PT_ERROR_INVALID_ARGUMENT, "pool_size must be in [1024, 2^26]");
     tuning_ = t;
     invalidate_graph();
-    if (d_ray_o_.bytes()) PT_TRY(ensure_pool());
+    if (d_hit_.bytes()) PT_TRY(ensure_pool());
     if (has_scene_) PT_TRY(configure_trace());
     return PT_OK;
 }

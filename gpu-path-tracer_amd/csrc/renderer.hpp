@@ -110,8 +110,8 @@ private:
     // path pool and queues
     pt_tuning tuning_{};
     uint32_t pool_capacity_ = 0;
-    DeviceBuffer d_ray_o_, d_ray_d_, d_hit_, d_thr_, d_meta_, d_sh_o_, d_sh_d_, d_sh_L_;
-    DeviceBuffer d_rayq_[2], d_q_[kQueueSlots], d_shadowq_, d_qid_;
+    DeviceBuffer d_ray_o_[2], d_ray_d_[2], d_thr_[2], d_meta_[2];
+    DeviceBuffer d_hit_, d_sh_o_, d_sh_d_, d_sh_L_, d_q_[kQueueSlots], d_qid_;
     DeviceBuffer d_ctr_, d_fp_, d_work_, d_overflow_;
     FrameParams* h_fp_ = nullptr;           // pinned staging
     Counters* h_ctr_ring_ = nullptr;        // pinned, kRing entries
