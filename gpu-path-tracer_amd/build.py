@@ -23,7 +23,7 @@ SOURCES = ["kernels.hip", "bvh_build.cpp", "scene.cpp", "renderer.cpp", "capi.cp
 # way on the CPU and the GPU.  The packed-fp32 feature is switched off for all
 # device code.
 COMMON = [
-    "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+    "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function", "-fvisibility=hidden",
     f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{os.path.join(HERE, '..', 'include')}",

@@ -26,6 +26,11 @@
 #include <stddef.h>
 #include <stdint.h>
 
+/* The library is built with hidden visibility: only these entry points are exported. */
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility push(default)
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -286,6 +291,10 @@ pt_status pt_set_tuning(pt_renderer* renderer, const pt_tuning* tuning);
 
 #ifdef __cplusplus
 }
+#endif
+
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility pop
 #endif
 
 #endif /* PT_CORE_H */

@@ -57,3 +57,28 @@ def test_create_without_gpu_reports_an_error_status():
     st = lib.pt_renderer_create(ctypes.byref(h))
     assert st == 4   # PT_ERROR_DEVICE: no HIP device is visible
     assert not h.value
+
+
+def test_cpp_facade_compiles_and_links(tmp_path):
+    """The header-only C++ facade (reference method names) builds against the C ABI."""
+    import subprocess
+    src = tmp_path / "use_facade.cpp"
+    src.write_text(r'''
+#include <pt/Renderer.hpp>
+#include <cstdio>
+int main() {
+    pt::Renderer r;
+    if (!r.Created()) { std::printf("no device: %s\n", r.Created().message.c_str()); return 0; }
+    auto ok = r.InitializeRenderingBuffers(16, 16);
+    std::printf("%d %u\n", (int)ok.status, r.GetPixelGridWidth());
+    return 0;
+}
+''')
+    inc = os.path.join(ROOT, "gpu-path-tracer_amd", "include")
+    libdir = os.path.dirname(ptcore.LIB_PATH)
+    exe = tmp_path / "use_facade"
+    r = subprocess.run(["g++", "-std=c++17", "-I", inc, str(src), "-o", str(exe), f"-L{libdir}", "-lptcore",
+                        f"-Wl,-rpath,{libdir}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
