@@ -30,10 +30,11 @@ COMMON = [
 ]
 
 
-def _compile(src, verbose):
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+def _compile(src, verbose, build_dir=BUILD, defines=()):
+    obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
     path = os.path.join(CSRC, src)
-    cmd = [HIPCC] + COMMON + (["-x", "hip"] if src.endswith(".hip") else []) + ["-c", path, "-o", obj]
+    cmd = [HIPCC] + COMMON + [f"-D{d}" for d in defines] + (["-x", "hip"] if src.endswith(".hip") else []) + \
+        ["-c", path, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     t0 = time.time()
@@ -44,14 +45,17 @@ def _compile(src, verbose):
     return obj, time.time() - t0, warn
 
 
-def build(jobs=None, verbose=False):
+def build(jobs=None, verbose=False, out=LIB, defines=()):
+    """Compile every source and link `out`; `defines` are extra -D macros
+    (variant builds for A/B timing go to their own build directory)."""
     if not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
-    os.makedirs(BUILD, exist_ok=True)
+    build_dir = BUILD if out == LIB else out + ".build"
+    os.makedirs(build_dir, exist_ok=True)
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 4)
     objs = []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = {ex.submit(_compile, s, verbose): s for s in SOURCES}
+        futs = {ex.submit(_compile, s, verbose, build_dir, defines): s for s in SOURCES}
         for f in cf.as_completed(futs):
             obj, dt, warn = f.result()
             objs.append(obj)
@@ -59,22 +63,24 @@ def build(jobs=None, verbose=False):
                 print(f"  {os.path.basename(obj)}: {dt:.1f}s", flush=True)
             if warn and verbose:
                 print(warn, flush=True)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + sorted(objs) + ["-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--out", default=LIB)
+    ap.add_argument("-D", dest="defines", action="append", default=[])
     a = ap.parse_args()
     try:
-        print(build(a.jobs, a.verbose))
+        print(build(a.jobs, a.verbose, os.path.abspath(a.out), a.defines))
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

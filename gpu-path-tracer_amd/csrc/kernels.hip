@@ -84,19 +84,36 @@ __device__ __forceinline__ void add_counts(unsigned long long* nodes_ctr, unsign
 
 // ================================================================ intersection
 
+#ifndef PT_MT_BRANCHFREE
+#define PT_MT_BRANCHFREE 1
+#endif
+#ifndef PT_WHILE_WHILE
+#define PT_WHILE_WHILE 0   // if-if measured faster on gfx950 (one step per outer iteration)
+#endif
+
 // Moller-Trumbore with the reference's operation order and tests
-// (Intersect.cu:25-61).  Returns true if t lies in [tmin, +inf); the caller
-// applies its own upper bound.
-__device__ __forceinline__ bool intersect_triangle(const TriRecord* tri, V3 o, V3 d, float tmin,
-                                                   float& t, float& u, float& v, uint32_t& prim,
-                                                   uint32_t& queue) {
-    const float4* p = reinterpret_cast<const float4*>(tri);
-    const float4 a = p[0], b = p[1], c = p[2];
+// (Intersect.cu:25-61).  The values of accepted hits are the same float
+// operations as the reference's.  Returns true if t lies in [tmin, +inf); the
+// caller applies its own upper bound.  The branch-free form evaluates every
+// term and forms one accept mask at the end; the other exits at the first
+// failed test as the reference does.
+__device__ __forceinline__ bool intersect_triangle(const float4 a, const float4 b, const float4 c, V3 o, V3 d,
+                                                   float tmin, float& t, float& u, float& v) {
     const V3 v0 = xyz(a), e1 = xyz(b), e2 = xyz(c);
-    prim = __float_as_uint(a.w);
-    queue = __float_as_uint(b.w);
     const V3 pv = cross(d, e2);
     const float det = dot(e1, pv);
+#if PT_MT_BRANCHFREE
+    const float inv_det = 1.0f / det;
+    const V3 s = o - v0;
+    u = dot(s, pv) * inv_det;
+    const V3 qv = cross(s, e1);
+    v = dot(d, qv) * inv_det;
+    t = dot(e2, qv) * inv_det;
+    const bool ok_det = !(fabsf(det) < 1e-7f);
+    const bool ok_u = !(u < 0.0f || u > 1.0f);
+    const bool ok_v = !(v < 0.0f || u + v > 1.0f);
+    return ok_det & ok_u & ok_v & (t >= tmin);
+#else
     if (fabsf(det) < 1e-7f) return false;
     const float inv_det = 1.0f / det;
     const V3 s = o - v0;
@@ -107,6 +124,14 @@ __device__ __forceinline__ bool intersect_triangle(const TriRecord* tri, V3 o, V
     if (v < 0.0f || u + v > 1.0f) return false;
     t = dot(e2, qv) * inv_det;
     return t >= tmin;
+#endif
+}
+
+__device__ __forceinline__ void load_tri(const TriRecord* tri, float4& a, float4& b, float4& c) {
+    const float4* p = reinterpret_cast<const float4*>(tri);
+    a = p[0];
+    b = p[1];
+    c = p[2];
 }
 
 struct RayFrame {
@@ -182,7 +207,8 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
     const uint32_t waves = stride >> 6;
     const uint32_t chunk = min(max((n / (4u * waves) + 63u) & ~63u, 64u), max(cfg.chunk, 64u));
 
-    bool active = false;
+    bool active = false;      // lane holds a ray
+    bool fin = false;         // its traversal has ended
     bool exhausted = false;
     uint32_t chunk_next = 0, chunk_end = 0;   // wave-uniform range of ray indices owned by this wave
     uint32_t item = 0, ref = 0, sp = 0;
@@ -191,7 +217,47 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
     uint32_t best_prim = kInvalid, best_queue = 0;
     uint32_t nodes = 0, tris = 0, acc_nodes = 0, acc_tris = 0;
 
+    // next stack entry whose subtree can still hold a closer hit; fin when empty
+    auto pop = [&]() {
+        while (sp > 0) {
+            sp--;
+            const uint2 e = sp < (uint32_t)LDS ? stack[sp][tid] : cfg.spill[(size_t)(sp - LDS) * stride + gtid];
+            if (ANY || __uint_as_float(e.y) <= r.tmax) {
+                ref = e.x;
+                return;
+            }
+        }
+        fin = true;
+    };
+    auto push = [&](uint32_t node_ref, float t) {
+        if (sp < cap) {
+            const uint2 e = make_uint2(node_ref, __float_as_uint(t));
+            if (sp < (uint32_t)LDS) stack[sp][tid] = e;
+            else cfg.spill[(size_t)(sp - LDS) * stride + gtid] = e;
+            sp++;
+        } else {
+            atomicOr(cfg.overflow, 1u);
+        }
+    };
+    auto test_tri = [&](const float4& a, const float4& b, const float4& c) {
+        float t, u, v;
+        if (intersect_triangle(a, b, c, r.o, r.d, r.tmin, t, u, v)) {
+            const uint32_t prim = __float_as_uint(a.w);
+            if (ANY) {
+                if (t < r.tmax) best_prim = prim;
+            } else if (t < r.tmax || (t == r.tmax && best_prim != kInvalid && prim < best_prim)) {
+                r.tmax = t;
+                best_t = t;
+                best_u = u;
+                best_v = v;
+                best_prim = prim;
+                best_queue = __float_as_uint(b.w);
+            }
+        }
+    };
+
     while (true) {
+        // ---- refill idle lanes from the wave's chunk
         if (!exhausted) {
             const uint64_t idle = __ballot(!active);
             const uint32_t nidle = (uint32_t)__popcll(idle);
@@ -213,10 +279,9 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                     const uint32_t avail = chunk_end - chunk_next;
                     const uint32_t lane_rank = mbcnt(idle);
                     if (!active && lane_rank < avail) {
-                        const uint32_t my = chunk_next + lane_rank;
                         V3 o, d;
                         float tmin, tmax;
-                        item = src.load(my, o, d, tmin, tmax);
+                        item = src.load(chunk_next + lane_rank, o, d, tmin, tmax);
                         setup_ray(r, o, d, tmin, tmax);
                         best_prim = kInvalid;
                         best_queue = 0;
@@ -225,6 +290,7 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                         nodes = tris = 0;
                         ref = 0;   // root node
                         sp = 0;
+                        fin = false;
                         active = true;
                     }
                     chunk_next += min(nidle, avail);
@@ -232,86 +298,67 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
             }
         }
         if (__ballot(active) == 0) break;
-        if (!active) continue;
 
-        bool pop = false;
-        bool done = false;
-        if (!ref_is_leaf(ref)) {
-            float t0[4];
-            const Bvh4Node* node = sc.nodes + ref;
-            box4(node, r, r.tmax, t0);
-            const uint4 cref = reinterpret_cast<const uint4*>(node)[6];
-            uint32_t c[4] = {cref.x, cref.y, cref.z, cref.w};
-            if (COUNT) nodes++;
-            if (!ANY) {
-                // nearest child first: sort the four (distance, ref) pairs
-                cswap(t0[0], c[0], t0[1], c[1]);
-                cswap(t0[2], c[2], t0[3], c[3]);
-                cswap(t0[0], c[0], t0[2], c[2]);
-                cswap(t0[1], c[1], t0[3], c[3]);
-                cswap(t0[1], c[1], t0[2], c[2]);
-            }
-            uint32_t next = kInvalid;
-            float tnext = 0.0f;
+        // ---- inner nodes: descend until every lane sits on a leaf or is done
+        // (while-while); or one node step per outer iteration (if-if)
+        while (true) {
+            const bool step = active && !fin && !ref_is_leaf(ref);
+            if (__ballot(step) == 0) break;
+            if (step) {
+                float t0[4];
+                const Bvh4Node* node = sc.nodes + ref;
+                box4(node, r, r.tmax, t0);
+                const uint4 cref = reinterpret_cast<const uint4*>(node)[6];
+                uint32_t c[4] = {cref.x, cref.y, cref.z, cref.w};
+                if (COUNT) nodes++;
+                if (!ANY) {
+                    // nearest child first: sort the four (distance, ref) pairs
+                    cswap(t0[0], c[0], t0[1], c[1]);
+                    cswap(t0[2], c[2], t0[3], c[3]);
+                    cswap(t0[0], c[0], t0[2], c[2]);
+                    cswap(t0[1], c[1], t0[3], c[3]);
+                    cswap(t0[1], c[1], t0[2], c[2]);
+                }
+                uint32_t next = kInvalid;
+                float tnext = 0.0f;
 #pragma unroll
-            for (int i = 3; i >= 0; i--) {
-                if (t0[i] != __builtin_inff() && c[i] != kInvalid) {
-                    if (next != kInvalid) {
-                        // the previous candidate is farther: it goes on the stack
-                        if (sp < cap) {
-                            const uint2 e = make_uint2(next, __float_as_uint(tnext));
-                            if (sp < (uint32_t)LDS) stack[sp][tid] = e;
-                            else cfg.spill[(size_t)(sp - LDS) * stride + gtid] = e;
-                            sp++;
-                        } else {
-                            atomicOr(cfg.overflow, 1u);
-                        }
-                    }
-                    next = c[i];
-                    tnext = t0[i];
-                }
-            }
-            if (next == kInvalid) pop = true;
-            else ref = next;
-        } else {
-            const uint32_t first = leaf_first(ref);
-            const uint32_t cnt = leaf_count(ref);
-            if (COUNT) tris += cnt;
-            for (uint32_t k = 0; k < cnt; k++) {
-                float t, u, v;
-                uint32_t prim, queue;
-                if (intersect_triangle(sc.tris + first + k, r.o, r.d, r.tmin, t, u, v, prim, queue)) {
-                    if (ANY) {
-                        if (t < r.tmax) {
-                            done = true;
-                            best_prim = prim;
-                            break;
-                        }
-                    } else if (t < r.tmax || (t == r.tmax && best_prim != kInvalid && prim < best_prim)) {
-                        r.tmax = t;
-                        best_t = t;
-                        best_u = u;
-                        best_v = v;
-                        best_prim = prim;
-                        best_queue = queue;
+                for (int i = 3; i >= 0; i--) {
+                    if (t0[i] != __builtin_inff() && c[i] != kInvalid) {
+                        if (next != kInvalid) push(next, tnext);   // the previous candidate is farther
+                        next = c[i];
+                        tnext = t0[i];
                     }
                 }
+                if (next == kInvalid) pop();
+                else ref = next;
             }
-            pop = !done;
+            if (!PT_WHILE_WHILE) break;
         }
-        if (pop) {
-            done = true;
-            while (sp > 0) {
-                sp--;
-                const uint2 e = sp < (uint32_t)LDS ? stack[sp][tid] : cfg.spill[(size_t)(sp - LDS) * stride + gtid];
-                if (ANY || __uint_as_float(e.y) <= r.tmax) {
-                    ref = e.x;
-                    done = false;
-                    break;
+
+        // ---- leaves: test triangles two at a time, then continue from the stack
+        while (true) {
+            const bool step = active && !fin && ref_is_leaf(ref);
+            if (__ballot(step) == 0) break;
+            if (step) {
+                const uint32_t first = leaf_first(ref);
+                const uint32_t cnt = leaf_count(ref);
+                if (COUNT) tris += cnt;
+                for (uint32_t k = 0; k < cnt; k += 2) {
+                    float4 a0, b0, c0, a1, b1, c1;
+                    load_tri(sc.tris + first + k, a0, b0, c0);
+                    const bool two = k + 1 < cnt;
+                    if (two) load_tri(sc.tris + first + k + 1, a1, b1, c1);
+                    test_tri(a0, b0, c0);
+                    if (two) test_tri(a1, b1, c1);
                 }
+                if (ANY && best_prim != kInvalid) fin = true;
+                else pop();
             }
+            if (!PT_WHILE_WHILE) break;
         }
-        if (done) {
+
+        // ---- finished rays write their result
+        if (active && fin) {
             if (ANY) src.finish_any(item, best_prim != kInvalid, nodes, tris);
             else src.finish_closest(item, best_prim, best_t, best_u, best_v, best_queue, nodes, tris);
             acc_nodes += nodes;

@@ -10,7 +10,7 @@
 namespace pt {
 
 // Limits of PathTracerDefaults.hpp:9-35
-constexpr uint32_t kDefaultPool = 1u << 22;   // path slots; the reference uses 2^20 (PathTracerDefaults.hpp:9)
+constexpr uint32_t kDefaultPool = 1u << 24;   // path slots; the reference uses 2^20 (PathTracerDefaults.hpp:9)
 constexpr uint32_t kMinWidth = 1, kMaxWidth = 7680, kMinHeight = 1, kMaxHeight = 4320;
 constexpr uint32_t kMinSpp = 1, kMaxSpp = 10000000;
 constexpr uint32_t kMinDepth = 1, kMaxDepth = 100;
@@ -36,11 +36,13 @@ void DeviceBuffer::release() {
 }
 
 Renderer::Renderer() {
-    tuning_.trace_block = 128;
+    tuning_.trace_block = 256;
     tuning_.trace_blocks_per_cu = 0;   // 0 = occupancy
     tuning_.shade_block = 256;
     tuning_.pool_size = kDefaultPool;
     tuning_.use_graph = 1;
+    tuning_.bvh_max_leaf = 8;
+    tuning_.bvh_traversal_cost = 1.0f;
 }
 
 Renderer::~Renderer() {
@@ -218,6 +220,8 @@ pt_status Renderer::InitializeRenderingBuffers(uint32_t width, uint32_t height) 
 pt_status Renderer::InitializeSceneBuffers(const pt_scene_desc& desc) {
     if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
     BvhBuildOptions bo;
+    bo.max_leaf_size = tuning_.bvh_max_leaf;
+    bo.traversal_cost = tuning_.bvh_traversal_cost;
     HostScene hs;
     SceneError err{PT_OK, ""};
     if (!build_host_scene(desc, bo, hs, err)) return fail(err.status, err.message);
@@ -844,6 +848,9 @@ pt_status Renderer::SetTuning(const pt_tuning& t) {
         return fail(PT_ERROR_INVALID_ARGUMENT, "trace_block must be 64, 128 or 256");
     if (t.shade_block != 256) return fail(PT_ERROR_INVALID_ARGUMENT, "shade_block must be 256");
     if (t.pool_size < 1024 || t.pool_size > (1u << 26)) return fail(PT_ERROR_INVALID_ARGUMENT, "pool_size must be in [1024, 2^26]");
+    if (t.bvh_max_leaf < 1 || t.bvh_max_leaf > kMaxLeafSize) return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_max_leaf must be in [1, 16]");
+    if (!(t.bvh_traversal_cost > 0.0f) || t.bvh_traversal_cost > 100.0f)
+        return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_traversal_cost must be in (0, 100]");
     tuning_ = t;
     invalidate_graph();
     if (d_hit_.bytes()) PT_TRY(ensure_pool());

@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libptcore.so")
+# PTCORE_LIB selects another build of the same library (variant timing)
+LIB_PATH = os.environ.get("PTCORE_LIB", os.path.join(HERE, "libptcore.so"))
 
 PT_OK = 0
 STATUS_NAMES = {0: "PT_OK", 1: "PT_ERROR_INVALID_ARGUMENT", 2: "PT_ERROR_INVALID_STATE", 3: "PT_ERROR_OUT_OF_RANGE",
@@ -80,7 +81,8 @@ class BvhInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("trace_block", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("shade_block", C.c_uint32),
-                ("pool_size", C.c_uint32), ("use_graph", C.c_uint32)]
+                ("pool_size", C.c_uint32), ("use_graph", C.c_uint32), ("bvh_max_leaf", C.c_uint32),
+                ("bvh_traversal_cost", C.c_float)]
 
 
 _P = C.c_void_p

@@ -201,13 +201,15 @@ typedef struct pt_bvh_info {
     uint32_t stack_total;     /* worst-case stack entries (LDS + spill) */
 } pt_bvh_info;
 
-/* Kernel launch parameters picked by the autotuner. */
+/* Kernel launch and build parameters picked by the autotuner. */
 typedef struct pt_tuning {
-    uint32_t trace_block;        /* threads per block of the traversal kernels (64..512) */
-    uint32_t trace_blocks_per_cu;
-    uint32_t shade_block;
+    uint32_t trace_block;        /* threads per block of the traversal kernels: 64, 128 or 256 */
+    uint32_t trace_blocks_per_cu;/* 0 = as many as fit */
+    uint32_t shade_block;        /* 256 */
     uint32_t pool_size;          /* path slots (queue capacity) */
     uint32_t use_graph;          /* replay the bounce loop as a hipGraph */
+    uint32_t bvh_max_leaf;       /* 1..16 triangles; applied at the next scene upload */
+    float bvh_traversal_cost;    /* SAH cost of a node visit relative to a triangle test; next upload */
 } pt_tuning;
 
 /* ---- lifetime (Renderer ctor/dtor, Renderer.hpp:106-112) ---- */
