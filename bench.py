@@ -54,6 +54,32 @@ def cpu_baseline(scene, settings, view, target_s):
                       f"{ext} extension + {sh} shadow rays in {dt:.1f}s"}
 
 
+def calibrate_copy():
+    """1 GiB device-to-device copy, three times: the counters of a run under
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE give the scale factor of each
+    counter against a known byte count."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "repeats": 3}), flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per closest-hit launch from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE and WRITE_SIZE scaled by the copy
+    calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_closest_production.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,7 +90,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--trace-block", type=int, default=0)
     ap.add_argument("--pool", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (scale reference for FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (samples would shard by "
               "pixel range with one reduction of the accumulation buffer); run with --gpus 1", file=sys.stderr)
@@ -125,8 +156,11 @@ def main():
     log(f"closest-hit: {launches} launches, avg {avg_launch_ms:.3f} ms, {nodes_per_ray:.2f} nodes + "
         f"{tris_per_ray:.2f} tris per ray = {bytes_per_ray:.0f} B/ray; shade {prof['shade_ms']:.1f} ms, "
         f"shadow {prof['shadow_ms']:.1f} ms, regen {prof['regen_ms']:.1f} ms, bin {prof['bin_ms']:.1f} ms, closest {prof['closest_ms']:.1f} ms")
+    traffic = measured_traffic()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
+                "traffic_detail": traffic,
                 "kernel": "k_trace<closest>", "bytes_per_ray": round(bytes_per_ray, 1),
                 "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3),
                 "avg_launch_ms": round(avg_launch_ms, 4), "rays_per_launch": round(prof["extension_rays"] / launches, 1)}

@@ -10,7 +10,7 @@
 namespace pt {
 
 // Limits of PathTracerDefaults.hpp:9-35
-constexpr uint32_t kDefaultPool = 1u << 24;   // path slots; the reference uses 2^20 (PathTracerDefaults.hpp:9)
+constexpr uint32_t kDefaultPool = 1u << 25;   // path slots; the reference uses 2^20 (PathTracerDefaults.hpp:9)
 constexpr uint32_t kMinWidth = 1, kMaxWidth = 7680, kMinHeight = 1, kMaxHeight = 4320;
 constexpr uint32_t kMinSpp = 1, kMaxSpp = 10000000;
 constexpr uint32_t kMinDepth = 1, kMaxDepth = 100;
