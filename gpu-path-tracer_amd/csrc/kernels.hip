@@ -209,8 +209,14 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
 
     bool active = false;      // lane holds a ray
     bool fin = false;         // its traversal has ended
-    bool exhausted = false;
-    uint32_t chunk_next = 0, chunk_end = 0;   // wave-uniform range of ray indices owned by this wave
+    // Wave-uniform range of ray indices owned by this wave.  The first chunk of
+    // every wave is assigned statically (no atomic: a launch would otherwise
+    // start with one same-address atomic per wave); later chunks come from the
+    // counter, offset past the statically assigned range.
+    const uint32_t wave_id = gtid >> 6;
+    const uint32_t static_end = waves * chunk;
+    uint32_t chunk_next = min(wave_id * chunk, n), chunk_end = min(wave_id * chunk + chunk, n);
+    bool exhausted = chunk_next >= n;
     uint32_t item = 0, ref = 0, sp = 0;
     RayFrame r;
     float best_t = 0.0f, best_u = 0.0f, best_v = 0.0f;
@@ -268,6 +274,7 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                     uint32_t base = 0;
                     if (lane_id() == leader) base = atomicAdd(work, chunk);
                     base = readlane_u32(base, leader);
+                    base = static_end > n ? n : base + static_end;   // static_end >= n: nothing left
                     if (base >= n) {
                         exhausted = true;
                     } else {

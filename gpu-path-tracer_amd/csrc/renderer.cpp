@@ -43,6 +43,7 @@ Renderer::Renderer() {
     tuning_.use_graph = 1;
     tuning_.bvh_max_leaf = 8;
     tuning_.bvh_traversal_cost = 1.0f;
+    tuning_.trace_refill = 16;
 }
 
 Renderer::~Renderer() {
@@ -189,6 +190,7 @@ pt_status Renderer::configure_trace() {
     TraceConfig cfg;
     cfg.block = tuning_.trace_block;
     cfg.lds_stack = cfg.block >= 256 ? 8 : 16;
+    cfg.refill = tuning_.trace_refill;
     if (!trace_config_supported(cfg.block, cfg.lds_stack))
         return fail(PT_ERROR_INVALID_ARGUMENT, "unsupported traversal block size " + std::to_string(cfg.block));
     int occ_closest = trace_occupancy(cfg, false, false);
@@ -848,6 +850,7 @@ pt_status Renderer::SetTuning(const pt_tuning& t) {
         return fail(PT_ERROR_INVALID_ARGUMENT, "trace_block must be 64, 128 or 256");
     if (t.shade_block != 256) return fail(PT_ERROR_INVALID_ARGUMENT, "shade_block must be 256");
     if (t.pool_size < 1024 || t.pool_size > (1u << 26)) return fail(PT_ERROR_INVALID_ARGUMENT, "pool_size must be in [1024, 2^26]");
+    if (t.trace_refill < 1 || t.trace_refill > 64) return fail(PT_ERROR_INVALID_ARGUMENT, "trace_refill must be in [1, 64]");
     if (t.bvh_max_leaf < 1 || t.bvh_max_leaf > kMaxLeafSize) return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_max_leaf must be in [1, 16]");
     if (!(t.bvh_traversal_cost > 0.0f) || t.bvh_traversal_cost > 100.0f)
         return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_traversal_cost must be in (0, 100]");

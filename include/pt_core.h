@@ -210,6 +210,7 @@ typedef struct pt_tuning {
     uint32_t use_graph;          /* replay the bounce loop as a hipGraph */
     uint32_t bvh_max_leaf;       /* 1..16 triangles; applied at the next scene upload */
     float bvh_traversal_cost;    /* SAH cost of a node visit relative to a triangle test; next upload */
+    uint32_t trace_refill;       /* idle lanes of a wave that trigger fetching new rays (1..64) */
 } pt_tuning;
 
 /* ---- lifetime (Renderer ctor/dtor, Renderer.hpp:106-112) ---- */
