@@ -109,6 +109,22 @@ pt_status pt_render_frame(pt_renderer* r, const pt_render_settings* settings, co
     });
 }
 
+pt_status pt_add_scene(pt_renderer* r, const pt_scene_desc* scene, uint32_t* scene_id) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!scene || !scene_id) return x.fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
+        return x.AddScene(*scene, *scene_id);
+    });
+}
+
+pt_status pt_remove_scene(pt_renderer* r, uint32_t scene_id) {
+    return guarded(r, [&](pt::Renderer& x) { return x.RemoveScene(scene_id); });
+}
+
+pt_status pt_render_batch(pt_renderer* r, const pt_view* views, uint32_t view_count, float* const* hdr,
+                          uint32_t* const* rgba8, uint32_t frame_flags, pt_frame_stats* stats) {
+    return guarded(r, [&](pt::Renderer& x) { return x.RenderBatch(views, view_count, hdr, rgba8, frame_flags, stats); });
+}
+
 pt_status pt_trace_closest(pt_renderer* r, const pt_ray* rays, uint32_t count, pt_hit* hits) {
     return guarded(r, [&](pt::Renderer& x) { return x.TraceBatch(rays, count, false, false, hits, nullptr, nullptr, nullptr); });
 }

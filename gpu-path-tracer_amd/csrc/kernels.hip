@@ -193,8 +193,9 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32
 // global area past `LDS` entries.  Each entry is (node ref, entry distance) so
 // popped subtrees that lie beyond the current hit are skipped without a fetch.
 
-template <int BLOCK, int LDS, bool ANY, bool COUNT, class Src>
-__device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev& sc, const TraceConfig& cfg) {
+template <int BLOCK, int LDS, bool ANY, bool COUNT, bool MULTI, class Src>
+__device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev& sc, const SceneDev* scenes,
+                                                 const TraceConfig& cfg) {
     __shared__ uint2 stack[LDS][BLOCK];
     const uint32_t tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
@@ -218,6 +219,8 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
     uint32_t chunk_next = min(wave_id * chunk, n), chunk_end = min(wave_id * chunk + chunk, n);
     bool exhausted = chunk_next >= n;
     uint32_t item = 0, ref = 0, sp = 0;
+    const Bvh4Node* nodes_base = sc.nodes;   // per lane when MULTI: the ray's scene
+    const TriRecord* tris_base = sc.tris;
     RayFrame r;
     float best_t = 0.0f, best_u = 0.0f, best_v = 0.0f;
     uint32_t best_prim = kInvalid, best_queue = 0;
@@ -288,7 +291,12 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                     if (!active && lane_rank < avail) {
                         V3 o, d;
                         float tmin, tmax;
-                        item = src.load(chunk_next + lane_rank, o, d, tmin, tmax);
+                        uint32_t scene = 0;
+                        item = src.load(chunk_next + lane_rank, o, d, tmin, tmax, scene);
+                        if (MULTI) {
+                            nodes_base = scenes[scene].nodes;
+                            tris_base = scenes[scene].tris;
+                        }
                         setup_ray(r, o, d, tmin, tmax);
                         best_prim = kInvalid;
                         best_queue = 0;
@@ -313,7 +321,7 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
             if (__ballot(step) == 0) break;
             if (step) {
                 float t0[4];
-                const Bvh4Node* node = sc.nodes + ref;
+                const Bvh4Node* node = nodes_base + ref;
                 box4(node, r, r.tmax, t0);
                 const uint4 cref = reinterpret_cast<const uint4*>(node)[6];
                 uint32_t c[4] = {cref.x, cref.y, cref.z, cref.w};
@@ -352,9 +360,9 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                 if (COUNT) tris += cnt;
                 for (uint32_t k = 0; k < cnt; k += 2) {
                     float4 a0, b0, c0, a1, b1, c1;
-                    load_tri(sc.tris + first + k, a0, b0, c0);
+                    load_tri(tris_base + first + k, a0, b0, c0);
                     const bool two = k + 1 < cnt;
-                    if (two) load_tri(sc.tris + first + k + 1, a1, b1, c1);
+                    if (two) load_tri(tris_base + first + k + 1, a1, b1, c1);
                     test_tri(a0, b0, c0);
                     if (two) test_tri(a1, b1, c1);
                 }
@@ -384,11 +392,13 @@ struct WavefrontClosestSrc {
     __device__ __forceinline__ uint32_t count() const { return p.ctr->ray[cur]; }
     __device__ __forceinline__ unsigned int* work() const { return &p.ctr->work_closest; }
     // rays are stored by queue position: consecutive lanes read consecutive records
-    __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax) const {
+    // ray_o.w holds the scene id; extension rays start at kMinT (PathTracerDefaults.hpp:13)
+    __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax, uint32_t& scene) const {
         const float4 a = (cur ? p.ray_o[1] : p.ray_o[0])[i];
         const float4 b = (cur ? p.ray_d[1] : p.ray_d[0])[i];
         o = xyz(a);
-        tmin = a.w;
+        scene = __float_as_uint(a.w);
+        tmin = kMinT;
         d = xyz(b);
         tmax = b.w;
         return i;
@@ -411,11 +421,12 @@ struct WavefrontShadowSrc {
     PathState p;
     __device__ __forceinline__ uint32_t count() const { return p.ctr->shadow; }
     __device__ __forceinline__ unsigned int* work() const { return &p.ctr->work_shadow; }
-    __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax) const {
+    __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax, uint32_t& scene) const {
         const float4 a = p.sh_o[i], b = p.sh_d[i];
         o = xyz(a);
         tmax = a.w;
         d = xyz(b);
+        scene = __float_as_uint(b.w);
         tmin = kMinT;
         return i;
     }
@@ -450,7 +461,8 @@ struct BatchSrc {
     uint8_t* qid;                   // optional: target queue of each ray
     __device__ __forceinline__ uint32_t count() const { return n; }
     __device__ __forceinline__ unsigned int* work() const { return work_ctr; }
-    __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax) const {
+    __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax, uint32_t& scene) const {
+        scene = 0;
         const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
         o = xyz(a);
         tmin = a.w;
@@ -477,9 +489,9 @@ struct BatchSrc {
     }
 };
 
-template <int BLOCK, int LDS, bool ANY, bool COUNT, class Src>
-__global__ __launch_bounds__(BLOCK) void k_trace(Src src, SceneDev sc, TraceConfig cfg) {
-    trace_persistent<BLOCK, LDS, ANY, COUNT, Src>(src, sc, cfg);
+template <int BLOCK, int LDS, bool ANY, bool COUNT, bool MULTI, class Src>
+__global__ __launch_bounds__(BLOCK) void k_trace(Src src, SceneDev sc, const SceneDev* scenes, TraceConfig cfg) {
+    trace_persistent<BLOCK, LDS, ANY, COUNT, MULTI, Src>(src, sc, scenes, cfg);
 }
 
 // ================================================================ shading math
@@ -597,8 +609,8 @@ struct Draws {
     }
 };
 
-__device__ __forceinline__ Draws draws_at(const FrameParams* fp, uint64_t sample, uint32_t bounce) {
-    const PcgStream s = pcg_seed(fp->seed, sample);
+__device__ __forceinline__ Draws draws_at(const FrameParams* fp, uint64_t seed, uint64_t sample, uint32_t bounce) {
+    const PcgStream s = pcg_seed(seed, sample);
     const PcgJump j = fp->bounce_jump[bounce];
     return Draws{j.mult * s.state + j.plus * s.inc, s.inc};
 }
@@ -739,14 +751,27 @@ __global__ void k_frame_begin(PathState p) {
     }
 }
 
-__global__ void k_fill_accum(float4* accum, uint32_t n) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+__global__ void k_fill_accum(float4* accum, uint32_t begin, uint32_t end) {
+    for (uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x)
         accum[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Ring slot of the view owning global sample g: the last live view whose
+// sample_begin <= g (virtual indices in [lo, hi), sample_begin increasing).
+__device__ __forceinline__ uint32_t view_of_sample(const ViewRecord* views, uint32_t lo, uint32_t hi, uint64_t g) {
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (views[mid & (kViewRing - 1u)].sample_begin <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo & (kViewRing - 1u);
 }
 
 // InitializePathsKernel / RegeneratePathsKernel, PathGeneration.cu:56-102.
 // New paths go to the free tail of the next ray queue: positions
 // [ray[nxt], ray[nxt] + n) with n = min(free positions, samples left).
+// Global sample g belongs to the view in ring slot v = view_of_sample(g); its
+// local index g - sample_begin addresses the view's pixels and sample streams.
 __global__ void k_generate(PathState p, int init, int nxt) {
     const FrameParams* fp = p.fp;
     Counters* c = p.ctr;
@@ -763,18 +788,18 @@ __global__ void k_generate(PathState p, int init, int nxt) {
         const uint32_t free_slots = p.capacity - base;
         n = (uint64_t)free_slots < remaining ? free_slots : (uint32_t)remaining;
     }
-    const uint32_t W = fp->width, H = fp->height, spp = fp->spp, axis = fp->spp_axis;
-    const V3 co = v3(fp->cam.origin[0], fp->cam.origin[1], fp->cam.origin[2]);
-    const V3 llc = v3(fp->cam.llc[0], fp->cam.llc[1], fp->cam.llc[2]);
-    const V3 hor = v3(fp->cam.horizontal[0], fp->cam.horizontal[1], fp->cam.horizontal[2]);
-    const V3 ver = v3(fp->cam.vertical[0], fp->cam.vertical[1], fp->cam.vertical[2]);
+    const uint32_t view_lo = fp->view_lo, view_hi = fp->view_hi;
     float4* ro = nxt ? p.ray_o[1] : p.ray_o[0];
     float4* rd = nxt ? p.ray_d[1] : p.ray_d[0];
     float4* th = nxt ? p.thr[1] : p.thr[0];
     uint4* me = nxt ? p.meta[1] : p.meta[0];
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const uint64_t sample = launched + k;
-        const PcgStream s = pcg_seed(fp->seed, sample);
+        const uint64_t g = launched + k;
+        const uint32_t v = view_of_sample(p.views, view_lo, view_hi, g);
+        const ViewRecord* vr = p.views + v;
+        const uint64_t sample = g - vr->sample_begin;
+        const uint32_t W = vr->width, H = vr->height, spp = vr->spp, axis = vr->spp_axis;
+        const PcgStream s = pcg_seed(vr->seed, sample);
         Draws dr{s.state, s.inc};
         const float jx = dr.next();
         const float jy = dr.next();
@@ -785,14 +810,18 @@ __global__ void k_generate(PathState p, int init, int nxt) {
         const uint32_t y = pixel / W, x = pixel - y * W;
         const uint32_t cy = cell / axis, cx = cell - cy * axis;
         const float u = ((float)x + ((float)cx + jx) / (float)axis) / (float)W;
-        const float v = ((float)y + ((float)cy + jy) / (float)axis) / (float)H;
+        const float vv = ((float)y + ((float)cy + jy) / (float)axis) / (float)H;
         // GenerateCameraRay, PathGeneration.cu:35-43
-        const V3 d = normalize(llc + u * hor + v * ver - co);
+        const V3 co = v3(vr->cam.origin[0], vr->cam.origin[1], vr->cam.origin[2]);
+        const V3 llc = v3(vr->cam.llc[0], vr->cam.llc[1], vr->cam.llc[2]);
+        const V3 hor = v3(vr->cam.horizontal[0], vr->cam.horizontal[1], vr->cam.horizontal[2]);
+        const V3 ver = v3(vr->cam.vertical[0], vr->cam.vertical[1], vr->cam.vertical[2]);
+        const V3 d = normalize(llc + u * hor + vv * ver - co);
         const uint32_t pos = base + k;
-        ro[pos] = f4(co, kMinT);
+        ro[pos] = f4(co, __uint_as_float(vr->scene));
         rd[pos] = f4(d, kMaxT);
         th[pos] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-        me[pos] = make_uint4(pixel, kDeltaBit, (uint32_t)sample, (uint32_t)(sample >> 32));
+        me[pos] = make_uint4(pixel, kDeltaBit | (v << 8), (uint32_t)sample, (uint32_t)(sample >> 32));
     }
 }
 
@@ -928,15 +957,16 @@ __device__ __forceinline__ void block_reserve2(bool pred0, bool pred1, unsigned 
 // The four material stages of one bounce in one launch: element e of the
 // virtual range [diffuse | ggx | mirror | emissive] is queue entry e - offset.
 // Each wave walks 64 consecutive entries, so a wave shades one material except
-// at the three queue boundaries.
-__global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur) {
+// at the three queue boundaries.  MULTI: paths of several views (per-lane view
+// and scene records); otherwise every path belongs to view 0 of scene `sc1`,
+// whose records are wave-uniform loads.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cur) {
     const FrameParams* fp = p.fp;
     Counters* c = p.ctr;
     const int nxt = cur ^ 1;
     const uint32_t c0 = c->queue[0], c1 = c->queue[1], c2 = c->queue[2], c3 = c->queue[3];
     const uint32_t e1 = c0, e2 = c0 + c1, e3 = e2 + c2, total = e3 + c3;
-    const uint32_t limit = fp->depth_limit, rr_start = fp->rr_start;
-    const bool nee = fp->use_nee != 0u && sc.light_count > 0u;
 
     for (uint32_t bbase = blockIdx.x * 256u; bbase < total; bbase += gridDim.x * 256u) {
         const uint32_t e = bbase + threadIdx.x;
@@ -948,6 +978,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
         uint32_t new_flags = 0;
         uint4 m = make_uint4(0, 0, 0, 0);
         float cur_ior = 1.0f;
+        float scene_bits = 0.0f;
         V3 add_L = v3(0.0f);
         bool add = false;
         uint32_t pixel = 0;
@@ -961,9 +992,16 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
             const uint32_t* qa = q == 0 ? p.q[0] : q == 1 ? p.q[1] : q == 2 ? p.q[2] : p.q[3];
             const uint32_t i = qa[j];
             m = (cur ? p.meta[1] : p.meta[0])[i];
-            pixel = m.x;
-            const uint32_t depth = m.y & 0xFFu;
+            const uint32_t depth = flags_depth(m.y);
             const bool delta = (m.y & kDeltaBit) != 0u;
+            const ViewRecord* vr = p.views + (MULTI ? flags_view(m.y) : 0u);
+            const uint32_t scene = vr->scene;
+            const SceneDev sc = MULTI ? p.scenes[scene] : sc1;
+            const uint32_t limit = vr->depth_limit, rr_start = vr->rr_start;
+            const bool view_nee = vr->use_nee != 0u;
+            const bool nee = view_nee && sc.light_count > 0u;
+            pixel = vr->accum_offset + m.x;
+            scene_bits = __uint_as_float(scene);
             const float4 th4 = (cur ? p.thr[1] : p.thr[0])[i];
             const V3 thr = xyz(th4);
             cur_ior = th4.w;
@@ -971,7 +1009,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
             const uint32_t prim = __float_as_uint(hh.w);
             if (q == 3) {
                 // EvaluateEmissiveKernel, MaterialEval.cu:882-912
-                if (!fp->use_nee || delta) {
+                if (!view_nee || delta) {
                     const uint32_t mi = reinterpret_cast<const uint32_t*>(sc.shade + prim)[27];
                     const MaterialRecord* mr = sc.materials + mi;
                     add_L = thr * v3(mr->emission[0], mr->emission[1], mr->emission[2]);
@@ -988,7 +1026,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
                 const float4 m0 = mf[0], m1 = mf[1], m2 = mf[2], m4 = mf[4];
                 const V3 nts = xyz(m4);
                 const Frame3 fr = shading_frame(h, nts);
-                Draws dr = draws_at(fp, ((uint64_t)m.w << 32) | m.z, depth);
+                Draws dr = draws_at(fp, vr->seed, ((uint64_t)m.w << 32) | m.z, depth);
                 const float u0 = dr.next(), u1 = dr.next(), u2 = dr.next(), u3 = dr.next();
                 float l0 = 0.0f, l1 = 0.0f, l2 = 0.0f;
                 if (nee) {
@@ -1109,7 +1147,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
                         new_o = ro + rdir * hh.x + h.gn * 1e-4f;
                         new_d = wi;
                         new_thr = k.thr;
-                        new_flags = k.depth | (new_delta ? kDeltaBit : 0u);
+                        new_flags = k.depth | (m.y & 0x00FFFF00u) | (new_delta ? kDeltaBit : 0u);
                         to_ray = true;
                     }
                 }
@@ -1125,11 +1163,11 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
         block_reserve2(to_ray, so.emit, &c->ray[nxt], &c->shadow, ray_pos, sh_pos);
         if (so.emit) {
             p.sh_o[sh_pos] = f4(so.o, so.tmax);
-            p.sh_d[sh_pos] = f4(so.d, 0.0f);
+            p.sh_d[sh_pos] = f4(so.d, scene_bits);
             p.sh_L[sh_pos] = f4(so.L, __uint_as_float(pixel));
         }
         if (to_ray) {
-            (nxt ? p.ray_o[1] : p.ray_o[0])[ray_pos] = f4(new_o, kMinT);
+            (nxt ? p.ray_o[1] : p.ray_o[0])[ray_pos] = f4(new_o, scene_bits);
             (nxt ? p.ray_d[1] : p.ray_d[0])[ray_pos] = f4(new_d, kMaxT);
             (nxt ? p.thr[1] : p.thr[0])[ray_pos] = f4(new_thr, cur_ior);
             (nxt ? p.meta[1] : p.meta[0])[ray_pos] = make_uint4(m.x, new_flags, m.z, m.w);
@@ -1137,11 +1175,22 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc, int cur
     }
 }
 
-// PostprocessKernel, Image.cu:34-54
-__global__ void k_postprocess(PathState p) {
-    const FrameParams* fp = p.fp;
-    const float inv_spp = fp->inv_spp;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.pixels; i += gridDim.x * blockDim.x) {
+// Ring slot of the view owning accumulation pixel i among the views with
+// virtual index in [lo, hi), whose accum_offset increases with the index.
+__device__ __forceinline__ uint32_t view_of_pixel(const ViewRecord* views, uint32_t lo, uint32_t hi, uint32_t i) {
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (views[mid & (kViewRing - 1u)].accum_offset <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo & (kViewRing - 1u);
+}
+
+// PostprocessKernel, Image.cu:34-54, over pixels [begin, end) of the views
+// with virtual index in [lo, hi).
+__global__ void k_postprocess(PathState p, uint32_t lo, uint32_t hi, uint32_t begin, uint32_t end) {
+    for (uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x) {
+        const float inv_spp = p.views[view_of_pixel(p.views, lo, hi, i)].inv_spp;
         const float4 a = p.accum[i];
         const bool ok = isfinite(a.x) && a.x >= 0.0f && isfinite(a.y) && a.y >= 0.0f && isfinite(a.z) && a.z >= 0.0f;
         if (!ok) atomicAdd(&p.ctr->invalid_pixels, 1u);
@@ -1166,6 +1215,16 @@ __global__ void k_alias_sample(SceneDev sc, const float* u, uint32_t n, uint32_t
     }
 }
 
+// Writes words handed over by value: a stream-ordered update of small device
+// tables (views, scene table, frame constants) with no host staging buffer.
+struct WordPayload {
+    uint32_t w[kWritePayloadWords];
+};
+
+__global__ void k_write_words(uint32_t* dst, WordPayload src, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src.w[i];
+}
+
 __global__ void k_zero_u32(unsigned int* ptr, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ptr[i] = 0u;
 }
@@ -1181,26 +1240,27 @@ uint32_t grid_for(uint32_t n, uint32_t block, uint32_t max_grid) {
     return (uint32_t)g;
 }
 
-template <int BLOCK, int LDS, bool ANY, bool COUNT, class Src>
-hipError_t run_trace(hipStream_t s, const Src& src, const SceneDev& sc, const TraceConfig& cfg) {
-    hipLaunchKernelGGL((k_trace<BLOCK, LDS, ANY, COUNT, Src>), dim3(cfg.blocks), dim3(BLOCK), 0, s, src, sc, cfg);
+template <int BLOCK, int LDS, bool ANY, bool COUNT, bool MULTI, class Src>
+hipError_t run_trace(hipStream_t s, const Src& src, const SceneDev& sc, const SceneDev* scenes, const TraceConfig& cfg) {
+    hipLaunchKernelGGL((k_trace<BLOCK, LDS, ANY, COUNT, MULTI, Src>), dim3(cfg.blocks), dim3(BLOCK), 0, s, src, sc,
+                       scenes, cfg);
     return hipGetLastError();
 }
 
-template <bool ANY, bool COUNT, class Src>
-hipError_t dispatch_trace(hipStream_t s, const Src& src, const SceneDev& sc, const TraceConfig& cfg) {
-    if (cfg.block == 64 && cfg.lds_stack == 16) return run_trace<64, 16, ANY, COUNT>(s, src, sc, cfg);
-    if (cfg.block == 128 && cfg.lds_stack == 16) return run_trace<128, 16, ANY, COUNT>(s, src, sc, cfg);
-    if (cfg.block == 128 && cfg.lds_stack == 8) return run_trace<128, 8, ANY, COUNT>(s, src, sc, cfg);
-    if (cfg.block == 256 && cfg.lds_stack == 8) return run_trace<256, 8, ANY, COUNT>(s, src, sc, cfg);
-    if (cfg.block == 256 && cfg.lds_stack == 16) return run_trace<256, 16, ANY, COUNT>(s, src, sc, cfg);
+template <bool ANY, bool COUNT, bool MULTI, class Src>
+hipError_t dispatch_trace(hipStream_t s, const Src& src, const SceneDev& sc, const SceneDev* scenes,
+                          const TraceConfig& cfg) {
+    if (cfg.block == 64 && cfg.lds_stack == 16) return run_trace<64, 16, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
+    if (cfg.block == 128 && cfg.lds_stack == 16) return run_trace<128, 16, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
+    if (cfg.block == 256 && cfg.lds_stack == 8) return run_trace<256, 8, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
     return hipErrorInvalidValue;
 }
 
 template <int BLOCK, int LDS, bool ANY, bool COUNT, class Src>
 int occ_one() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<BLOCK, LDS, ANY, COUNT, Src>, BLOCK, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<BLOCK, LDS, ANY, COUNT, false, Src>, BLOCK, 0) != hipSuccess)
+        return 0;
     return nb;
 }
 
@@ -1208,17 +1268,14 @@ template <bool ANY, bool COUNT, class Src>
 int occ(const TraceConfig& cfg) {
     if (cfg.block == 64 && cfg.lds_stack == 16) return occ_one<64, 16, ANY, COUNT, Src>();
     if (cfg.block == 128 && cfg.lds_stack == 16) return occ_one<128, 16, ANY, COUNT, Src>();
-    if (cfg.block == 128 && cfg.lds_stack == 8) return occ_one<128, 8, ANY, COUNT, Src>();
     if (cfg.block == 256 && cfg.lds_stack == 8) return occ_one<256, 8, ANY, COUNT, Src>();
-    if (cfg.block == 256 && cfg.lds_stack == 16) return occ_one<256, 16, ANY, COUNT, Src>();
     return 0;
 }
 
 }  // namespace
 
 bool trace_config_supported(uint32_t block, uint32_t lds_stack) {
-    return (block == 64 && lds_stack == 16) || (block == 128 && (lds_stack == 16 || lds_stack == 8)) ||
-           (block == 256 && (lds_stack == 8 || lds_stack == 16));
+    return (block == 64 && lds_stack == 16) || (block == 128 && lds_stack == 16) || (block == 256 && lds_stack == 8);
 }
 
 int trace_occupancy(const TraceConfig& cfg, bool any, bool count) {
@@ -1231,8 +1288,9 @@ hipError_t launch_frame_begin(hipStream_t s, const PathState& p) {
     return hipGetLastError();
 }
 
-hipError_t launch_fill_accum(hipStream_t s, const PathState& p) {
-    hipLaunchKernelGGL(k_fill_accum, dim3(grid_for(p.pixels, 256, 4096)), dim3(256), 0, s, p.accum, p.pixels);
+hipError_t launch_fill_accum(hipStream_t s, const PathState& p, uint32_t begin, uint32_t end) {
+    if (end <= begin) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_accum, dim3(grid_for(end - begin, 256, 4096)), dim3(256), 0, s, p.accum, begin, end);
     return hipGetLastError();
 }
 
@@ -1241,10 +1299,14 @@ hipError_t launch_generate(hipStream_t s, const PathState& p, bool init, int nxt
     return hipGetLastError();
 }
 
-hipError_t launch_closest(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool count,
+hipError_t launch_closest(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool count, bool multi,
                           const TraceConfig& cfg) {
     WavefrontClosestSrc src{p, cur};
-    return count ? dispatch_trace<false, true>(s, src, sc, cfg) : dispatch_trace<false, false>(s, src, sc, cfg);
+    if (multi)
+        return count ? dispatch_trace<false, true, true>(s, src, sc, p.scenes, cfg)
+                     : dispatch_trace<false, false, true>(s, src, sc, p.scenes, cfg);
+    return count ? dispatch_trace<false, true, false>(s, src, sc, p.scenes, cfg)
+                 : dispatch_trace<false, false, false>(s, src, sc, p.scenes, cfg);
 }
 
 hipError_t launch_bin(hipStream_t s, const PathState& p, int cur, uint32_t grid) {
@@ -1252,15 +1314,22 @@ hipError_t launch_bin(hipStream_t s, const PathState& p, int cur, uint32_t grid)
     return hipGetLastError();
 }
 
-hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, uint32_t grid, uint32_t block) {
+hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool multi, uint32_t grid,
+                        uint32_t block) {
     if (block != 256) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, p, sc, cur);
+    if (multi) hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, s, p, sc, cur);
+    else hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(256), 0, s, p, sc, cur);
     return hipGetLastError();
 }
 
-hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, bool count, const TraceConfig& cfg) {
+hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, bool count, bool multi,
+                         const TraceConfig& cfg) {
     WavefrontShadowSrc src{p};
-    return count ? dispatch_trace<true, true>(s, src, sc, cfg) : dispatch_trace<true, false>(s, src, sc, cfg);
+    if (multi)
+        return count ? dispatch_trace<true, true, true>(s, src, sc, p.scenes, cfg)
+                     : dispatch_trace<true, false, true>(s, src, sc, p.scenes, cfg);
+    return count ? dispatch_trace<true, true, false>(s, src, sc, p.scenes, cfg)
+                 : dispatch_trace<true, false, false>(s, src, sc, p.scenes, cfg);
 }
 
 hipError_t launch_prepare(hipStream_t s, const PathState& p, int cur) {
@@ -1268,8 +1337,10 @@ hipError_t launch_prepare(hipStream_t s, const PathState& p, int cur) {
     return hipGetLastError();
 }
 
-hipError_t launch_postprocess(hipStream_t s, const PathState& p) {
-    hipLaunchKernelGGL(k_postprocess, dim3(grid_for(p.pixels, 256, 4096)), dim3(256), 0, s, p);
+hipError_t launch_postprocess(hipStream_t s, const PathState& p, uint32_t lo, uint32_t hi, uint32_t begin,
+                              uint32_t end) {
+    if (end <= begin || hi <= lo) return hipSuccess;
+    hipLaunchKernelGGL(k_postprocess, dim3(grid_for(end - begin, 256, 4096)), dim3(256), 0, s, p, lo, hi, begin, end);
     return hipGetLastError();
 }
 
@@ -1278,15 +1349,18 @@ hipError_t launch_trace_batch(hipStream_t s, const SceneDev& sc, const float4* r
                               const TraceConfig& cfg) {
     BatchSrc src{rays, hits, occluded, nodes, tris, work, n, nullptr};
     const bool count = nodes != nullptr || tris != nullptr;
-    if (any) return count ? dispatch_trace<true, true>(s, src, sc, cfg) : dispatch_trace<true, false>(s, src, sc, cfg);
-    return count ? dispatch_trace<false, true>(s, src, sc, cfg) : dispatch_trace<false, false>(s, src, sc, cfg);
+    if (any)
+        return count ? dispatch_trace<true, true, false>(s, src, sc, nullptr, cfg)
+                     : dispatch_trace<true, false, false>(s, src, sc, nullptr, cfg);
+    return count ? dispatch_trace<false, true, false>(s, src, sc, nullptr, cfg)
+                 : dispatch_trace<false, false, false>(s, src, sc, nullptr, cfg);
 }
 
 hipError_t launch_classify(hipStream_t s, const SceneDev& sc, const float4* rays, uint32_t n, uint8_t* qid,
                            uint32_t* const* queues, unsigned int* queue_counts, unsigned int* work,
                            const TraceConfig& cfg, uint32_t bin_grid) {
     BatchSrc src{rays, nullptr, nullptr, nullptr, nullptr, work, n, qid};
-    hipError_t e = dispatch_trace<false, false>(s, src, sc, cfg);
+    hipError_t e = dispatch_trace<false, false, false>(s, src, sc, nullptr, cfg);
     if (e != hipSuccess) return e;
     QueueSet qs;
     for (int i = 0; i < kQueueSlots; i++) qs.q[i] = queues[i];
@@ -1297,6 +1371,25 @@ hipError_t launch_classify(hipStream_t s, const SceneDev& sc, const float4* rays
 hipError_t launch_alias_sample(hipStream_t s, const SceneDev& sc, const float* u, uint32_t n, uint32_t* index, float* pmf) {
     hipLaunchKernelGGL(k_alias_sample, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, sc, u, n, index, pmf);
     return hipGetLastError();
+}
+
+hipError_t launch_write_words(hipStream_t s, void* dst, const void* src, size_t bytes) {
+    if (bytes % 4 != 0) return hipErrorInvalidValue;
+    const uint32_t* w = static_cast<const uint32_t*>(src);
+    uint32_t* d = static_cast<uint32_t*>(dst);
+    size_t left = bytes / 4;
+    while (left > 0) {
+        WordPayload pl;
+        const uint32_t n = (uint32_t)(left < kWritePayloadWords ? left : kWritePayloadWords);
+        for (uint32_t i = 0; i < n; i++) pl.w[i] = w[i];
+        hipLaunchKernelGGL(k_write_words, dim3(1), dim3(256), 0, s, d, pl, n);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        w += n;
+        d += n;
+        left -= n;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_zero_u32(hipStream_t s, unsigned int* ptr, uint32_t n) {

@@ -27,22 +27,24 @@ struct SceneDev {
 // full state of one live path.  Shading writes each continuing path into the
 // next queue; new camera paths fill the free tail.  No slot indirection.
 struct PathState {
-    float4* ray_o[2];               // origin, tMin
+    float4* ray_o[2];               // origin, scene id (bits); tMin is kMinT
     float4* ray_d[2];               // direction, tMax
     float4* thr[2];                 // throughput, medium ior
-    uint4* meta[2];                 // pixel, depth|delta, sample lo, sample hi
+    uint4* meta[2];                 // view pixel, flags (depth|view slot|delta), local sample lo, hi
     float4* hit;                    // t, u, v, prim (bits) of current-queue entry i
     uint8_t* qid;                   // target queue of current-queue entry i
     uint32_t* q[kQueueSlots];       // material queues: positions in the current ray queue
     float4* sh_o;                   // shadow queue: origin, tMax
-    float4* sh_d;                   // direction
-    float4* sh_L;                   // radiance, pixel (bits)
+    float4* sh_d;                   // direction, scene id (bits)
+    float4* sh_L;                   // radiance, accumulation pixel (bits)
     Counters* ctr;
     const FrameParams* fp;
-    float4* accum;
-    uint32_t* framebuffer;          // RGBA8
-    uint32_t capacity;
-    uint32_t pixels;
+    const ViewRecord* views;        // view ring, kViewRing records
+    const SceneDev* scenes;         // resident scenes by id, kMaxScenes records
+    float4* accum;                  // accumulation pixels of all live views
+    uint32_t* framebuffer;          // RGBA8, same indexing as accum
+    uint32_t capacity;              // path slots
+    uint32_t pixels;                // accumulation pixels allocated
 };
 
 struct TraceConfig {
@@ -61,14 +63,22 @@ int trace_occupancy(const TraceConfig& cfg, bool any, bool count);
 bool trace_config_supported(uint32_t block, uint32_t lds_stack);
 
 hipError_t launch_frame_begin(hipStream_t s, const PathState& p);
-hipError_t launch_fill_accum(hipStream_t s, const PathState& p);
+// Zeroes accumulation pixels [begin, end).
+hipError_t launch_fill_accum(hipStream_t s, const PathState& p, uint32_t begin, uint32_t end);
 hipError_t launch_generate(hipStream_t s, const PathState& p, bool init, int nxt, uint32_t grid, uint32_t block);
-hipError_t launch_closest(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool count, const TraceConfig& cfg);
+// `multi`: rays of several scenes (per-ray node/triangle pointers from p.scenes);
+// otherwise every ray belongs to scene `sc`.
+hipError_t launch_closest(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool count, bool multi,
+                          const TraceConfig& cfg);
 hipError_t launch_bin(hipStream_t s, const PathState& p, int cur, uint32_t grid);
-hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, uint32_t grid, uint32_t block);
-hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, bool count, const TraceConfig& cfg);
+hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool multi, uint32_t grid,
+                        uint32_t block);
+hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, bool count, bool multi,
+                         const TraceConfig& cfg);
 hipError_t launch_prepare(hipStream_t s, const PathState& p, int cur);
-hipError_t launch_postprocess(hipStream_t s, const PathState& p);
+// Tone-maps pixels [begin, end) of the views with virtual index in [lo, hi).
+hipError_t launch_postprocess(hipStream_t s, const PathState& p, uint32_t lo, uint32_t hi, uint32_t begin,
+                              uint32_t end);
 
 // Batch hooks.  rays: n x 2 float4 (origin|tMin, direction|tMax).
 // Closest: hits[i] in pt_hit layout (prim bits, t, u, v); any: occluded[i].  nodes/tris may be null.
@@ -81,6 +91,10 @@ hipError_t launch_classify(hipStream_t s, const SceneDev& sc, const float4* rays
                            const TraceConfig& cfg, uint32_t bin_grid);
 hipError_t launch_alias_sample(hipStream_t s, const SceneDev& sc, const float* u, uint32_t n,
                                uint32_t* index, float* pmf);
+// Stream-ordered copy of `bytes` (a multiple of 4) from host memory to device
+// memory, passed as kernel arguments: the host buffer may be reused at once.
+constexpr uint32_t kWritePayloadWords = 896;
+hipError_t launch_write_words(hipStream_t s, void* dst, const void* src, size_t bytes);
 hipError_t launch_zero_u32(hipStream_t s, unsigned int* p, uint32_t n);
 
 }  // namespace pt

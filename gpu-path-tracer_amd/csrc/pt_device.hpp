@@ -138,8 +138,11 @@ struct AliasBin {
     uint32_t value;
 };
 
-// Per-path flags in meta.y: bits 0..7 depth, bit 31 = last scatter was delta.
+// Per-path flags in meta.y: bits 0..7 depth, bits 8..23 view id, bit 31 = last
+// scatter was delta.
 constexpr uint32_t kDeltaBit = 0x80000000u;
+PT_HD uint32_t flags_depth(uint32_t f) { return f & 0xFFu; }
+PT_HD uint32_t flags_view(uint32_t f) { return (f >> 8) & 0xFFFFu; }
 
 // ---------------------------------------------------------------- PCG32
 // PCG-XSH-RR 64/32 (O'Neill 2014, pcg-random.org).  A sample owns the stream
@@ -205,16 +208,37 @@ struct CameraBasis {           // DeviceCamera, DeviceCamera.hpp:5-11
     float vertical[3];
 };
 
-// Frame constants in device memory, read by every kernel of the bounce loop so
-// that one captured graph serves every frame.
-struct FrameParams {
+// One camera view of one resident scene: a frame of the single-frame API, or
+// one element of a batch or of the serving stream.  Views live in a device
+// ring of kViewRing records addressed by a monotonically growing virtual
+// index; the ring slot (virtual & (kViewRing-1)) is what a path carries.  View
+// v owns the global samples [sample_begin, sample_begin + sample_count) and
+// pixels [accum_offset, accum_offset + w*h) of the accumulation buffer.  Its
+// sample streams are keyed by (seed, local sample index), so a view renders
+// the same alone, in a batch or interleaved with other requests.
+struct ViewRecord {
     CameraBasis cam;
     uint32_t width, height, spp, spp_axis;
-    uint32_t depth_limit, use_nee, rr_start, pad0;
+    uint32_t depth_limit, use_nee, rr_start, scene;
     uint64_t seed;
-    uint64_t total_samples;
+    uint64_t sample_begin;
+    uint64_t sample_count;
+    uint32_t accum_offset;
     float inv_spp;
-    uint32_t pad1[3];
+};
+static_assert(sizeof(ViewRecord) == 112, "view record is 112 B");
+constexpr uint32_t kViewRing = 4096;       // ring slot lives in bits 8..23 of the path flags
+constexpr uint32_t kMaxScenes = 64;
+
+// Wavefront constants in device memory, read by every kernel of the bounce
+// loop so that one captured graph serves every frame.  Views with virtual
+// index in [view_lo, view_hi) are live; views[view_lo] starts at or before the
+// next sample to launch, so the generator finds a sample's view by binary
+// search over sample_begin in that window.  total_samples is the global end of
+// the samples admitted so far (it grows while a server runs).
+struct FrameParams {
+    uint64_t total_samples;
+    uint32_t view_lo, view_hi;
     PcgJump bounce_jump[kMaxDepthLimit + 1];   // advance to draw 2+8b
 };
 

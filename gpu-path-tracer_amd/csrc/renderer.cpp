@@ -89,6 +89,10 @@ pt_status Renderer::init() {
     PT_TRY_HIP(d_fp_.allocate(sizeof(FrameParams)), "frame params");
     PT_TRY_HIP(d_work_.allocate(64), "work counter");
     PT_TRY_HIP(d_overflow_.allocate(64), "overflow flag");
+    PT_TRY_HIP(d_scene_table_.allocate(sizeof(SceneDev) * kMaxScenes), "scene table");
+    PT_TRY_HIP(d_views_.allocate(sizeof(ViewRecord) * kViewRing), "view ring");
+    PT_TRY_HIP(hipMemsetAsync(d_scene_table_.as<void>(), 0, sizeof(SceneDev) * kMaxScenes, stream_), "memset");
+    PT_TRY_HIP(hipMemsetAsync(d_views_.as<void>(), 0, sizeof(ViewRecord) * kViewRing, stream_), "memset");
     PT_TRY_HIP(hipMemsetAsync(d_ctr_.as<void>(), 0, sizeof(Counters), stream_), "memset");
     PT_TRY_HIP(hipMemsetAsync(d_overflow_.as<void>(), 0, 64, stream_), "memset");
     PT_TRY_HIP(hipStreamSynchronize(stream_), "sync");
@@ -96,10 +100,11 @@ pt_status Renderer::init() {
 }
 
 void Renderer::invalidate_graph() {
-    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
-    graph_exec_ = nullptr;
-    graph_ = nullptr;
+    for (GraphEntry& g : graphs_) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    graphs_.clear();
 }
 
 PathState Renderer::path_state() const {
@@ -118,23 +123,25 @@ PathState Renderer::path_state() const {
     p.sh_L = d_sh_L_.as<float4>();
     p.ctr = d_ctr_.as<Counters>();
     p.fp = d_fp_.as<FrameParams>();
+    p.views = d_views_.as<ViewRecord>();
+    p.scenes = d_scene_table_.as<SceneDev>();
     p.accum = d_accum_.as<float4>();
     p.framebuffer = d_frame_.as<uint32_t>();
     p.capacity = pool_capacity_;
-    p.pixels = width_ * height_;
+    p.pixels = accum_capacity_;
     return p;
 }
 
-SceneDev Renderer::scene_dev() const {
+SceneDev ResidentScene::dev() const {
     SceneDev s{};
-    s.nodes = d_nodes_.as<Bvh4Node>();
-    s.tris = d_tris_.as<TriRecord>();
-    s.shade = d_shade_.as<ShadeRecord>();
-    s.materials = d_materials_.as<MaterialRecord>();
-    s.lights = d_lights_.as<LightRecord>();
-    s.bins = d_bins_.as<AliasBin>();
-    s.light_count = (uint32_t)host_.lights.size();
-    s.tri_count = host_.tri_count;
+    s.nodes = nodes.as<Bvh4Node>();
+    s.tris = tris.as<TriRecord>();
+    s.shade = shade.as<ShadeRecord>();
+    s.materials = materials.as<MaterialRecord>();
+    s.lights = lights.as<LightRecord>();
+    s.bins = bins.as<AliasBin>();
+    s.light_count = (uint32_t)host.lights.size();
+    s.tri_count = host.tri_count;
     return s;
 }
 
@@ -159,30 +166,39 @@ pt_status Renderer::ensure_pool() {
     return PT_OK;
 }
 
-pt_status Renderer::ensure_pixels(uint32_t w, uint32_t h) {
-    if (w == width_ && h == height_ && d_accum_.bytes() > 0) return PT_OK;
+// Accumulation and tone-mapped pixels for `pixels` pixels over all views of a
+// run.  Grows only; the contents are cleared by each run.
+pt_status Renderer::ensure_accum(uint32_t pixels) {
+    if (pixels <= accum_capacity_ && d_accum_.bytes() > 0) return PT_OK;
     invalidate_graph();
-    const size_t n = (size_t)w * h;
+    hdr_valid_ = false;
+    const size_t n = std::max<size_t>(pixels, 1);
     d_accum_.release();
     d_frame_.release();
+    accum_capacity_ = 0;
     PT_TRY_HIP(d_accum_.allocate(n * sizeof(float4)), "accumulation buffer");
     PT_TRY_HIP(d_frame_.allocate(n * sizeof(uint32_t)), "framebuffer");
     if (h_frame_staging_pixels_ < n) {
         if (h_frame_staging_) (void)hipHostFree(h_frame_staging_);
         h_frame_staging_ = nullptr;
+        h_frame_staging_pixels_ = 0;
         PT_TRY_HIP(hipHostMalloc((void**)&h_frame_staging_, n * sizeof(uint32_t), hipHostMallocDefault), "pinned framebuffer");
         h_frame_staging_pixels_ = n;
     }
     PT_TRY_HIP(hipMemsetAsync(d_accum_.as<void>(), 0, n * sizeof(float4), stream_), "memset");
     PT_TRY_HIP(hipMemsetAsync(d_frame_.as<void>(), 0, n * sizeof(uint32_t), stream_), "memset");
     PT_TRY_HIP(hipStreamSynchronize(stream_), "sync");
-    {
-        std::scoped_lock lock(frame_mutex_);
-        frame_.assign(n, 0u);
-    }
+    accum_capacity_ = (uint32_t)n;
+    return PT_OK;
+}
+
+void Renderer::set_frame_size(uint32_t w, uint32_t h) {
+    if (w == width_ && h == height_) return;
+    std::scoped_lock lock(frame_mutex_);
+    frame_.assign((size_t)w * h, 0u);
     width_ = w;
     height_ = h;
-    return PT_OK;
+    hdr_valid_ = false;
 }
 
 pt_status Renderer::configure_trace() {
@@ -198,8 +214,11 @@ pt_status Renderer::configure_trace() {
     int occ = std::max(1, std::min(occ_closest, occ_shadow));
     if (tuning_.trace_blocks_per_cu) occ = (int)tuning_.trace_blocks_per_cu;
     cfg.blocks = (uint32_t)(occ * std::max(1, cu_count_));
-    // worst-case stack: a BVH4 level pushes at most three siblings
-    const uint32_t depth = has_scene_ ? host_.bvh.depth : 1u;
+    // worst-case stack over every resident scene: a BVH4 level pushes at most
+    // three siblings
+    uint32_t depth = 1;
+    for (const auto& sc : scenes_)
+        if (sc) depth = std::max(depth, sc->host.bvh.depth);
     stack_total_ = 3u * depth + 2u;
     cfg.spill_depth = stack_total_ > cfg.lds_stack ? stack_total_ - cfg.lds_stack : 0u;
     const size_t spill_bytes = (size_t)cfg.spill_depth * cfg.blocks * cfg.block * sizeof(uint2);
@@ -215,37 +234,65 @@ pt_status Renderer::InitializeRenderingBuffers(uint32_t width, uint32_t height) 
     width = std::min(std::max(width, kMinWidth), kMaxWidth);
     height = std::min(std::max(height, kMinHeight), kMaxHeight);
     PT_TRY(ensure_pool());
-    PT_TRY(ensure_pixels(width, height));
+    PT_TRY(ensure_accum(width * height));
+    set_frame_size(width, height);
     return PT_OK;
 }
 
-pt_status Renderer::InitializeSceneBuffers(const pt_scene_desc& desc) {
-    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+// Builds the host scene (validation, BVH, light table), uploads its records
+// and publishes them in slot `id` of the device scene table.
+pt_status Renderer::upload_scene(uint32_t id, const pt_scene_desc& desc) {
     BvhBuildOptions bo;
     bo.max_leaf_size = tuning_.bvh_max_leaf;
     bo.traversal_cost = tuning_.bvh_traversal_cost;
-    HostScene hs;
+    auto rs = std::make_unique<ResidentScene>();
     SceneError err{PT_OK, ""};
-    if (!build_host_scene(desc, bo, hs, err)) return fail(err.status, err.message);
-    invalidate_graph();
-    has_scene_ = false;
-    host_ = std::move(hs);
+    if (!build_host_scene(desc, bo, rs->host, err)) return fail(err.status, err.message);
+    const HostScene& h = rs->host;
     auto upload = [&](DeviceBuffer& b, const void* src, size_t bytes, const char* what) -> pt_status {
-        b.release();
         PT_TRY_HIP(b.allocate(bytes), what);
         if (bytes) PT_TRY_HIP(hipMemcpyAsync(b.as<void>(), src, bytes, hipMemcpyHostToDevice, stream_), what);
         return PT_OK;
     };
-    PT_TRY(upload(d_nodes_, host_.bvh.nodes.data(), host_.bvh.nodes.size() * sizeof(Bvh4Node), "BVH nodes"));
-    PT_TRY(upload(d_tris_, host_.tris.data(), host_.tris.size() * sizeof(TriRecord), "triangle records"));
-    PT_TRY(upload(d_shade_, host_.shade.data(), host_.shade.size() * sizeof(ShadeRecord), "shading records"));
-    PT_TRY(upload(d_materials_, host_.materials.data(), host_.materials.size() * sizeof(MaterialRecord), "materials"));
-    PT_TRY(upload(d_lights_, host_.lights.data(), host_.lights.size() * sizeof(LightRecord), "lights"));
-    PT_TRY(upload(d_bins_, host_.bins.data(), host_.bins.size() * sizeof(AliasBin), "alias table"));
+    PT_TRY(upload(rs->nodes, h.bvh.nodes.data(), h.bvh.nodes.size() * sizeof(Bvh4Node), "BVH nodes"));
+    PT_TRY(upload(rs->tris, h.tris.data(), h.tris.size() * sizeof(TriRecord), "triangle records"));
+    PT_TRY(upload(rs->shade, h.shade.data(), h.shade.size() * sizeof(ShadeRecord), "shading records"));
+    PT_TRY(upload(rs->materials, h.materials.data(), h.materials.size() * sizeof(MaterialRecord), "materials"));
+    PT_TRY(upload(rs->lights, h.lights.data(), h.lights.size() * sizeof(LightRecord), "lights"));
+    PT_TRY(upload(rs->bins, h.bins.data(), h.bins.size() * sizeof(AliasBin), "alias table"));
+    const SceneDev d = rs->dev();
+    PT_TRY_HIP(launch_write_words(stream_, d_scene_table_.as<SceneDev>() + id, &d, sizeof(SceneDev)), "scene table");
     PT_TRY_HIP(hipStreamSynchronize(stream_), "scene upload");
-    has_scene_ = true;
-    PT_TRY(configure_trace());
-    return PT_OK;
+    invalidate_graph();
+    scenes_[id] = std::move(rs);
+    return configure_trace();
+}
+
+pt_status Renderer::InitializeSceneBuffers(const pt_scene_desc& desc) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    return upload_scene(0, desc);
+}
+
+pt_status Renderer::AddScene(const pt_scene_desc& desc, uint32_t& id) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    for (uint32_t k = 1; k < kMaxScenes; k++) {
+        if (scenes_[k]) continue;
+        PT_TRY(upload_scene(k, desc));
+        id = k;
+        return PT_OK;
+    }
+    return fail(PT_ERROR_OUT_OF_RANGE, "all " + std::to_string(kMaxScenes - 1) + " scene slots are in use");
+}
+
+pt_status Renderer::RemoveScene(uint32_t id) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    if (id >= kMaxScenes || !scenes_[id]) return fail(PT_ERROR_INVALID_ARGUMENT, "no resident scene " + std::to_string(id));
+    invalidate_graph();
+    const SceneDev zero{};
+    PT_TRY_HIP(launch_write_words(stream_, d_scene_table_.as<SceneDev>() + id, &zero, sizeof(SceneDev)), "scene table");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "scene table");
+    scenes_[id].reset();
+    return configure_trace();
 }
 
 pt_status Renderer::Clear(uint8_t r, uint8_t g, uint8_t b, uint8_t a) {
@@ -256,24 +303,29 @@ pt_status Renderer::Clear(uint8_t r, uint8_t g, uint8_t b, uint8_t a) {
     return PT_OK;
 }
 
-pt_status Renderer::launch_iteration(int cur, bool count) {
+pt_status Renderer::launch_iteration(int cur, bool count, const LoopMode& mode) {
     const PathState p = path_state();
-    const SceneDev sc = scene_dev();
+    const SceneDev sc = scene_dev(mode.scene);
     const uint32_t grid = (uint32_t)std::max(1, cu_count_) * 8u;
-    PT_TRY_HIP(launch_closest(stream_, p, sc, cur, count, trace_), "closest-hit launch");
+    PT_TRY_HIP(launch_closest(stream_, p, sc, cur, count, mode.trace_multi, trace_), "closest-hit launch");
     PT_TRY_HIP(launch_bin(stream_, p, cur, bin_grid()), "binning launch");
-    PT_TRY_HIP(launch_shade(stream_, p, sc, cur, grid, 256), "shade launch");
-    PT_TRY_HIP(launch_shadow(stream_, p, sc, count, trace_), "shadow launch");
+    PT_TRY_HIP(launch_shade(stream_, p, sc, cur, mode.shade_multi, grid, 256), "shade launch");
+    PT_TRY_HIP(launch_shadow(stream_, p, sc, count, mode.trace_multi, trace_), "shadow launch");
     PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
     PT_TRY_HIP(launch_prepare(stream_, p, cur), "prepare launch");
     return PT_OK;
 }
 
-pt_status Renderer::ensure_graph() {
-    if (graph_exec_) return PT_OK;
+pt_status Renderer::ensure_graph(const LoopMode& mode, hipGraphExec_t& exec) {
+    const int key = mode.key();
+    for (const GraphEntry& g : graphs_)
+        if (g.key == key) {
+            exec = g.exec;
+            return PT_OK;
+        }
     PT_TRY_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "graph capture");
-    pt_status s = launch_iteration(0, false);
-    if (s == PT_OK) s = launch_iteration(1, false);
+    pt_status s = launch_iteration(0, false, mode);
+    if (s == PT_OK) s = launch_iteration(1, false, mode);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(stream_, &g);
     if (s != PT_OK) {
@@ -281,8 +333,14 @@ pt_status Renderer::ensure_graph() {
         return s;
     }
     PT_TRY_HIP(e, "graph capture");
-    graph_ = g;
-    PT_TRY_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "graph instantiate");
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return check_hip(e, "graph instantiate");
+    }
+    graphs_.push_back(GraphEntry{key, g, ge});
+    exec = ge;
     return PT_OK;
 }
 
@@ -305,33 +363,62 @@ ClampedSettings clamp_settings(const pt_render_settings& s) {
 }
 }  // namespace
 
-pt_status Renderer::render_frame(const pt_render_settings& s, const pt_camera& camera, const pt_motion_state& state,
-                                 uint32_t flags, pt_frame_stats* stats, const std::atomic<bool>* stop) {
-    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
-    const ClampedSettings c = clamp_settings(s);
+// View record of `v` with settings clamped as in Renderer.cpp:354-372; the
+// caller fills sample_begin and accum_offset.
+pt_status Renderer::make_view(const pt_view& v, ViewRecord& rec, uint32_t& depth) {
+    if (v.scene >= kMaxScenes || !scenes_[v.scene])
+        return fail(PT_ERROR_INVALID_ARGUMENT, "view names scene " + std::to_string(v.scene) + ", which is not resident");
+    const ClampedSettings c = clamp_settings(v.settings);
+    std::memset(&rec, 0, sizeof(rec));
+    rec.cam = make_camera_basis(v.camera, (float)c.width / (float)c.height, v.state);
+    rec.width = c.width;
+    rec.height = c.height;
+    rec.spp = c.spp;
+    rec.spp_axis = c.axis;
+    rec.depth_limit = c.depth;
+    rec.use_nee = v.settings.use_nee ? 1u : 0u;
+    rec.rr_start = c.rr;
+    rec.scene = v.scene;
+    rec.seed = v.settings.seed;
+    rec.sample_count = (uint64_t)c.spp * c.width * c.height;
+    rec.inv_spp = 1.0f / (float)c.spp;
+    depth = c.depth;
+    return PT_OK;
+}
+
+// Renders `views` (sample_begin and accum_offset consecutive from 0) in one
+// wavefront: every kernel of the bounce loop draws paths of all views from
+// the one pool.  Leaves the tone-mapped pixels in d_frame_ and the radiance
+// sums in d_accum_.
+pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max_depth, uint32_t flags,
+                              pt_frame_stats* stats, const std::atomic<bool>* stop) {
+    const uint32_t n = (uint32_t)views.size();
+    if (n == 0 || n > kViewRing) return fail(PT_ERROR_INVALID_ARGUMENT, "a batch holds 1 to " + std::to_string(kViewRing) + " views");
+    const ViewRecord& last = views.back();
+    const uint64_t total_samples = last.sample_begin + last.sample_count;
+    const uint64_t total_pixels64 = (uint64_t)last.accum_offset + (uint64_t)last.width * last.height;
+    if (total_pixels64 > 0x7FFFFFFFull) return fail(PT_ERROR_OUT_OF_RANGE, "the batch holds more than 2^31 pixels");
+    const uint32_t total_pixels = (uint32_t)total_pixels64;
     PT_TRY(ensure_pool());
-    PT_TRY(ensure_pixels(c.width, c.height));
+    PT_TRY(ensure_accum(total_pixels));
+
+    LoopMode mode;
+    mode.scene = views[0].scene;
+    mode.shade_multi = n > 1;
+    for (const ViewRecord& v : views) mode.trace_multi = mode.trace_multi || v.scene != mode.scene;
 
     FrameParams& fp = *h_fp_;
     std::memset(&fp, 0, sizeof(fp));
-    fp.cam = make_camera_basis(camera, (float)c.width / (float)c.height, state);
-    fp.width = c.width;
-    fp.height = c.height;
-    fp.spp = c.spp;
-    fp.spp_axis = c.axis;
-    fp.depth_limit = c.depth;
-    fp.use_nee = s.use_nee ? 1u : 0u;
-    fp.rr_start = c.rr;
-    fp.seed = s.seed;
-    fp.total_samples = (uint64_t)c.spp * c.width * c.height;
-    fp.inv_spp = 1.0f / (float)c.spp;
+    fp.total_samples = total_samples;
+    fp.view_lo = 0;
+    fp.view_hi = n;
     for (uint32_t b = 0; b <= kMaxDepthLimit; b++) fp.bounce_jump[b] = pcg_jump(bounce_draw_base(b));
-    last_spp_ = c.spp;
 
     const bool profile = (flags & PT_FRAME_PROFILE) != 0;
     const bool count = (flags & PT_FRAME_INSTRUMENT) != 0;
     const bool use_graph = tuning_.use_graph && !profile && !count && !(flags & PT_FRAME_NO_GRAPH);
-    if (use_graph) PT_TRY(ensure_graph());
+    hipGraphExec_t graph_exec = nullptr;
+    if (use_graph) PT_TRY(ensure_graph(mode, graph_exec));
 
     hipEvent_t t_begin = nullptr, t_end = nullptr;
     PT_TRY_HIP(hipEventCreate(&t_begin), "event");
@@ -345,41 +432,42 @@ pt_status Renderer::render_frame(const pt_render_settings& s, const pt_camera& c
     } guard{t_begin, t_end};
 
     const PathState p = path_state();
-    const SceneDev sc = scene_dev();
+    const SceneDev sc = scene_dev(mode.scene);
     const uint32_t grid = (uint32_t)std::max(1, cu_count_) * 8u;
-    PT_TRY_HIP(hipMemcpyAsync(d_fp_.as<void>(), h_fp_, sizeof(FrameParams), hipMemcpyHostToDevice, stream_), "frame params");
+    PT_TRY_HIP(launch_write_words(stream_, d_views_.as<void>(), views.data(), n * sizeof(ViewRecord)), "view records");
+    PT_TRY_HIP(launch_write_words(stream_, d_fp_.as<void>(), h_fp_, sizeof(FrameParams)), "frame params");
     PT_TRY_HIP(hipEventRecord(t_begin, stream_), "event");
     PT_TRY_HIP(launch_frame_begin(stream_, p), "frame begin");
-    PT_TRY_HIP(launch_fill_accum(stream_, p), "clear accumulation");
+    PT_TRY_HIP(launch_fill_accum(stream_, p, 0, total_pixels), "clear accumulation");
     PT_TRY_HIP(launch_generate(stream_, p, true, 0, grid, 256), "initialize paths");
 
     // Every path ends within `depth` iterations and slots are refilled at once,
     // so the loop needs at most (ceil(total/P)+1)*depth iterations.
-    const uint64_t waves = (fp.total_samples + pool_capacity_ - 1) / pool_capacity_ + 1;
-    const uint64_t max_iters = 2 * waves * (uint64_t)(c.depth + 1) + 16;
+    const uint64_t waves = (total_samples + pool_capacity_ - 1) / pool_capacity_ + 1;
+    const uint64_t max_iters = 2 * waves * (uint64_t)(max_depth + 1) + 16;
 
     double closest_ms = 0, shadow_ms = 0, shade_ms = 0, regen_ms = 0, bin_ms = 0;
     uint32_t closest_launches = 0;
     if (profile) {
         hipEvent_t ev[7];
         for (auto& e : ev) PT_TRY_HIP(hipEventCreate(&e), "event");
-        struct Guard6 {
+        struct Guard7 {
             hipEvent_t* e;
-            ~Guard6() {
+            ~Guard7() {
                 for (int i = 0; i < 7; i++) (void)hipEventDestroy(e[i]);
             }
-        } g6{ev};
+        } g7{ev};
         int cur = 0;
         for (uint64_t it = 0;; it++) {
             if (it > max_iters) return fail(PT_ERROR_DEVICE, "wavefront did not terminate");
             PT_TRY_HIP(hipEventRecord(ev[0], stream_), "event");
-            PT_TRY_HIP(launch_closest(stream_, p, sc, cur, false, trace_), "closest-hit launch");
+            PT_TRY_HIP(launch_closest(stream_, p, sc, cur, false, mode.trace_multi, trace_), "closest-hit launch");
             PT_TRY_HIP(hipEventRecord(ev[1], stream_), "event");
             PT_TRY_HIP(launch_bin(stream_, p, cur, bin_grid()), "binning launch");
             PT_TRY_HIP(hipEventRecord(ev[2], stream_), "event");
-            PT_TRY_HIP(launch_shade(stream_, p, sc, cur, grid, 256), "shade launch");
+            PT_TRY_HIP(launch_shade(stream_, p, sc, cur, mode.shade_multi, grid, 256), "shade launch");
             PT_TRY_HIP(hipEventRecord(ev[3], stream_), "event");
-            PT_TRY_HIP(launch_shadow(stream_, p, sc, false, trace_), "shadow launch");
+            PT_TRY_HIP(launch_shadow(stream_, p, sc, false, mode.trace_multi, trace_), "shadow launch");
             PT_TRY_HIP(hipEventRecord(ev[4], stream_), "event");
             PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
             PT_TRY_HIP(hipEventRecord(ev[5], stream_), "event");
@@ -406,9 +494,9 @@ pt_status Renderer::render_frame(const pt_render_settings& s, const pt_camera& c
         for (uint64_t k = 0;; k++) {
             if (k * (use_graph ? 2 : 1) > max_iters) return fail(PT_ERROR_DEVICE, "wavefront did not terminate");
             if (use_graph) {
-                PT_TRY_HIP(hipGraphLaunch(graph_exec_, stream_), "graph launch");
+                PT_TRY_HIP(hipGraphLaunch(graph_exec, stream_), "graph launch");
             } else {
-                PT_TRY(launch_iteration(cur, count));
+                PT_TRY(launch_iteration(cur, count, mode));
                 cur ^= 1;
             }
             const int slot = (int)(k % kRing);
@@ -426,16 +514,10 @@ pt_status Renderer::render_frame(const pt_render_settings& s, const pt_camera& c
             }
         }
     }
-    PT_TRY_HIP(launch_postprocess(stream_, p), "postprocess");
+    PT_TRY_HIP(launch_postprocess(stream_, p, 0, n, 0, total_pixels), "postprocess");
     PT_TRY_HIP(hipEventRecord(t_end, stream_), "event");
     PT_TRY_HIP(hipMemcpyAsync(h_ctr_ring_, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream_), "counters");
-    const size_t npix = (size_t)c.width * c.height;
-    PT_TRY_HIP(hipMemcpyAsync(h_frame_staging_, p.framebuffer, npix * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "framebuffer");
     PT_TRY_HIP(hipStreamSynchronize(stream_), "frame sync");
-    {
-        std::scoped_lock lock(frame_mutex_);
-        frame_.assign(h_frame_staging_, h_frame_staging_ + npix);
-    }
     const Counters& ctr = h_ctr_ring_[0];
     if (ctr.stack_overflow) return fail(PT_ERROR_DEVICE, "traversal stack overflow");
     float total_ms = 0;
@@ -463,6 +545,68 @@ pt_status Renderer::render_frame(const pt_render_settings& s, const pt_camera& c
     return PT_OK;
 }
 
+pt_status Renderer::render_frame(const pt_render_settings& s, const pt_camera& camera, const pt_motion_state& state,
+                                 uint32_t flags, pt_frame_stats* stats, const std::atomic<bool>* stop) {
+    if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    pt_view v{};
+    v.scene = 0;
+    v.settings = s;
+    v.camera = camera;
+    v.state = state;
+    std::vector<ViewRecord> views(1);
+    uint32_t depth = 0;
+    PT_TRY(make_view(v, views[0], depth));
+    set_frame_size(views[0].width, views[0].height);
+    last_spp_ = views[0].spp;
+    hdr_valid_ = false;
+    PT_TRY(run_views(views, depth, flags, stats, stop));
+    if (stop && stop->load()) return PT_OK;
+    const size_t npix = (size_t)width_ * height_;
+    PT_TRY_HIP(hipMemcpyAsync(h_frame_staging_, d_frame_.as<void>(), npix * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "framebuffer");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "frame sync");
+    {
+        std::scoped_lock lock(frame_mutex_);
+        frame_.assign(h_frame_staging_, h_frame_staging_ + npix);
+    }
+    hdr_valid_ = true;
+    return PT_OK;
+}
+
+pt_status Renderer::RenderBatch(const pt_view* views, uint32_t n, float* const* hdr, uint32_t* const* rgba,
+                                uint32_t flags, pt_frame_stats* stats) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    if (!views || n == 0) return fail(PT_ERROR_INVALID_ARGUMENT, "a batch needs at least one view");
+    if (n > kViewRing) return fail(PT_ERROR_INVALID_ARGUMENT, "a batch holds at most " + std::to_string(kViewRing) + " views");
+    std::vector<ViewRecord> recs(n);
+    uint64_t samples = 0, pixels = 0;
+    uint32_t max_depth = 1;
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t depth = 0;
+        PT_TRY(make_view(views[i], recs[i], depth));
+        recs[i].sample_begin = samples;
+        recs[i].accum_offset = (uint32_t)std::min<uint64_t>(pixels, 0xFFFFFFFFull);
+        samples += recs[i].sample_count;
+        pixels += (uint64_t)recs[i].width * recs[i].height;
+        max_depth = std::max(max_depth, depth);
+    }
+    if (pixels > 0x7FFFFFFFull) return fail(PT_ERROR_OUT_OF_RANGE, "the batch holds more than 2^31 pixels");
+    hdr_valid_ = false;
+    PT_TRY(run_views(recs, max_depth, flags, stats, nullptr));
+    for (uint32_t i = 0; i < n; i++) {
+        const size_t np = (size_t)recs[i].width * recs[i].height;
+        if (rgba && rgba[i])
+            PT_TRY_HIP(hipMemcpy(rgba[i], d_frame_.as<uint32_t>() + recs[i].accum_offset, np * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost), "batch framebuffer");
+        if (hdr && hdr[i]) {
+            PT_TRY_HIP(hipMemcpy(hdr[i], d_accum_.as<float4>() + recs[i].accum_offset, np * sizeof(float4),
+                                 hipMemcpyDeviceToHost), "batch HDR");
+            const float inv = recs[i].inv_spp;
+            for (size_t k = 0; k < 4 * np; k++) hdr[i][k] *= inv;
+        }
+    }
+    return PT_OK;
+}
+
 pt_status Renderer::RenderFrameSync(const pt_render_settings& s, const pt_camera& camera, const pt_motion_state& state,
                                     uint32_t flags, pt_frame_stats* stats) {
     if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
@@ -474,7 +618,7 @@ pt_status Renderer::RenderFrameSync(const pt_render_settings& s, const pt_camera
 pt_status Renderer::StartSimulation(const pt_render_settings& s, const pt_camera& camera, const pt_motion_state* states,
                                     uint32_t n, const char* output_folder) {
     if (is_rendering_.load()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
-    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
     if (n > 0 && !states) return fail(PT_ERROR_INVALID_ARGUMENT, "motion states are NULL");
     if (render_thread_.joinable()) render_thread_.join();
     {
@@ -483,7 +627,8 @@ pt_status Renderer::StartSimulation(const pt_render_settings& s, const pt_camera
     }
     const ClampedSettings c = clamp_settings(s);
     PT_TRY(ensure_pool());
-    PT_TRY(ensure_pixels(c.width, c.height));
+    PT_TRY(ensure_accum(c.width * c.height));
+    set_frame_size(c.width, c.height);
     settings_ = s;
     camera_ = camera;
     motion_.assign(states, states + n);
@@ -582,6 +727,7 @@ pt_status Renderer::CopyFrame(uint32_t* rgba, size_t n) {
 
 pt_status Renderer::CopyFrameHdr(float* rgba, size_t n) {
     if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "HDR readback needs a finished render");
+    if (!hdr_valid_) return fail(PT_ERROR_INVALID_STATE, "no finished single frame to read back");
     const size_t npix = (size_t)width_ * height_;
     if (!rgba || n < npix) return fail(PT_ERROR_INVALID_ARGUMENT, "HDR buffer too small");
     PT_TRY_HIP(hipMemcpyAsync(rgba, d_accum_.as<void>(), npix * sizeof(float4), hipMemcpyDeviceToHost, stream_), "HDR readback");
@@ -707,7 +853,7 @@ pt_status Renderer::save_frame_png(const std::string& path) {
 
 pt_status Renderer::TraceBatch(const pt_ray* rays, uint32_t n, bool any, bool device, void* out, uint32_t* nodes,
                                uint32_t* tris, hipStream_t stream) {
-    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
     if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
     if (n == 0) return PT_OK;
     if (!rays || !out) return fail(PT_ERROR_INVALID_ARGUMENT, "rays and output must not be NULL");
@@ -746,7 +892,7 @@ pt_status Renderer::TraceBatch(const pt_ray* rays, uint32_t n, bool any, bool de
 }
 
 pt_status Renderer::Classify(const pt_ray* rays, uint32_t n, uint32_t* queue_of_ray, uint32_t* counts) {
-    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
     if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
     if (!rays || !queue_of_ray || !counts) return fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
     for (int q = 0; q < kQueueSlots; q++) counts[q] = 0;
@@ -787,8 +933,8 @@ pt_status Renderer::Classify(const pt_ray* rays, uint32_t n, uint32_t* queue_of_
 }
 
 pt_status Renderer::AliasSample(const float* u, uint32_t n, uint32_t* index, float* pmf) {
-    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
-    if (host_.lights.empty()) return fail(PT_ERROR_INVALID_STATE, "scene has no emissive triangles");
+    if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (host0().lights.empty()) return fail(PT_ERROR_INVALID_STATE, "scene has no emissive triangles");
     if (n == 0) return PT_OK;
     if (!u || !index || !pmf) return fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
     PT_TRY_HIP(d_hook_in_.allocate((size_t)n * 4), "hook input");
@@ -804,37 +950,37 @@ pt_status Renderer::AliasSample(const float* u, uint32_t n, uint32_t* index, flo
 }
 
 pt_status Renderer::GetAliasTable(pt_alias_bin* bins, uint32_t cap, uint32_t* count) {
-    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
-    if (count) *count = (uint32_t)host_.bins.size();
+    if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (count) *count = (uint32_t)host0().bins.size();
     if (!bins) return PT_OK;
-    if (cap < host_.bins.size()) return fail(PT_ERROR_INVALID_ARGUMENT, "capacity too small");
-    for (size_t i = 0; i < host_.bins.size(); i++)
-        bins[i] = pt_alias_bin{host_.bins[i].q, host_.bins[i].p, host_.bins[i].alias, host_.bins[i].value};
+    if (cap < host0().bins.size()) return fail(PT_ERROR_INVALID_ARGUMENT, "capacity too small");
+    for (size_t i = 0; i < host0().bins.size(); i++)
+        bins[i] = pt_alias_bin{host0().bins[i].q, host0().bins[i].p, host0().bins[i].alias, host0().bins[i].value};
     return PT_OK;
 }
 
 pt_status Renderer::GetBvhInfo(pt_bvh_info& info) {
-    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
-    info.node_count = (uint32_t)host_.bvh.nodes.size();
-    info.leaf_count = host_.bvh.leaf_count;
-    info.max_depth = host_.bvh.depth;
-    info.max_leaf_size = host_.bvh.max_leaf;
-    info.triangle_count = host_.tri_count;
-    info.light_count = (uint32_t)host_.lights.size();
+    if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    info.node_count = (uint32_t)host0().bvh.nodes.size();
+    info.leaf_count = host0().bvh.leaf_count;
+    info.max_depth = host0().bvh.depth;
+    info.max_leaf_size = host0().bvh.max_leaf;
+    info.triangle_count = host0().tri_count;
+    info.light_count = (uint32_t)host0().lights.size();
     info.stack_lds = trace_.lds_stack;
     info.stack_total = trace_.lds_stack + trace_.spill_depth;
     return PT_OK;
 }
 
 pt_status Renderer::GetBvh4(void* nodes, uint32_t node_cap, uint32_t* order, uint32_t order_cap) {
-    if (!has_scene_) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
     if (nodes) {
-        if (node_cap < host_.bvh.nodes.size()) return fail(PT_ERROR_INVALID_ARGUMENT, "node capacity too small");
-        std::memcpy(nodes, host_.bvh.nodes.data(), host_.bvh.nodes.size() * sizeof(Bvh4Node));
+        if (node_cap < host0().bvh.nodes.size()) return fail(PT_ERROR_INVALID_ARGUMENT, "node capacity too small");
+        std::memcpy(nodes, host0().bvh.nodes.data(), host0().bvh.nodes.size() * sizeof(Bvh4Node));
     }
     if (order) {
-        if (order_cap < host_.bvh.order.size()) return fail(PT_ERROR_INVALID_ARGUMENT, "order capacity too small");
-        std::memcpy(order, host_.bvh.order.data(), host_.bvh.order.size() * sizeof(uint32_t));
+        if (order_cap < host0().bvh.order.size()) return fail(PT_ERROR_INVALID_ARGUMENT, "order capacity too small");
+        std::memcpy(order, host0().bvh.order.data(), host0().bvh.order.size() * sizeof(uint32_t));
     }
     return PT_OK;
 }
@@ -857,7 +1003,7 @@ pt_status Renderer::SetTuning(const pt_tuning& t) {
     tuning_ = t;
     invalidate_graph();
     if (d_hit_.bytes()) PT_TRY(ensure_pool());
-    if (has_scene_) PT_TRY(configure_trace());
+    if (has_scene()) PT_TRY(configure_trace());
     return PT_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <atomic>
 #include <condition_variable>
 #include <cstdint>
+#include <memory>
 #include <mutex>
 #include <optional>
 #include <string>
@@ -34,6 +35,14 @@ private:
     size_t bytes_ = 0;
 };
 
+// A scene resident on the device: host copy (for the hooks and the BVH
+// queries) and its device records.
+struct ResidentScene {
+    HostScene host;
+    DeviceBuffer nodes, tris, shade, materials, lights, bins;
+    SceneDev dev() const;
+};
+
 class Renderer {
 public:
     Renderer();
@@ -59,6 +68,12 @@ public:
     pt_status RenderFrameSync(const pt_render_settings& s, const pt_camera& camera, const pt_motion_state& state,
                               uint32_t flags, pt_frame_stats* stats);
 
+    // resident scenes and batches of views
+    pt_status AddScene(const pt_scene_desc& scene, uint32_t& id);
+    pt_status RemoveScene(uint32_t id);
+    pt_status RenderBatch(const pt_view* views, uint32_t n, float* const* hdr, uint32_t* const* rgba, uint32_t flags,
+                          pt_frame_stats* stats);
+
     // hooks
     pt_status TraceBatch(const pt_ray* rays, uint32_t n, bool any, bool device, void* out, uint32_t* nodes,
                          uint32_t* tris, hipStream_t stream);
@@ -77,35 +92,51 @@ public:
     }
 
 private:
-    struct FrameSetup {
-        uint32_t width, height, spp, axis, depth;
-        bool nee;
-        uint64_t total;
+    // Which kernel instances a set of views needs: per-ray scene lookups when
+    // the views span several scenes, per-path view lookups when there are
+    // several views.  One captured graph per mode.
+    struct LoopMode {
+        bool trace_multi = false;
+        bool shade_multi = false;
+        uint32_t scene = 0;          // the one scene when !trace_multi
+        int key() const { return trace_multi ? -1 - (int)shade_multi : (int)(scene * 2u + (shade_multi ? 1u : 0u)); }
+    };
+    struct GraphEntry {
+        int key;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
     };
     pt_status check_hip(hipError_t e, const char* what);
     pt_status ensure_pool();
-    pt_status ensure_pixels(uint32_t width, uint32_t height);
+    pt_status ensure_accum(uint32_t pixels);
+    void set_frame_size(uint32_t width, uint32_t height);
     pt_status configure_trace();
-    pt_status ensure_graph();
+    pt_status upload_scene(uint32_t id, const pt_scene_desc& desc);
+    pt_status make_view(const pt_view& v, ViewRecord& rec, uint32_t& depth);
+    pt_status ensure_graph(const LoopMode& mode, hipGraphExec_t& exec);
+    pt_status run_views(const std::vector<ViewRecord>& views, uint32_t max_depth, uint32_t flags, pt_frame_stats* stats,
+                        const std::atomic<bool>* stop);
     pt_status render_frame(const pt_render_settings& s, const pt_camera& camera, const pt_motion_state& state,
                            uint32_t flags, pt_frame_stats* stats, const std::atomic<bool>* stop);
-    pt_status launch_iteration(int cur, bool count);
+    pt_status launch_iteration(int cur, bool count, const LoopMode& mode);
     pt_status save_frame_png(const std::string& path);
     void simulation_loop(uint32_t start_frame);
     void invalidate_graph();
     PathState path_state() const;
     uint32_t bin_grid() const { return (uint32_t)std::max(1, cu_count_) * 4u; }
-    SceneDev scene_dev() const;
+    bool has_scene() const { return scenes_[0] != nullptr; }
+    const HostScene& host0() const { return scenes_[0]->host; }
+    SceneDev scene_dev(uint32_t id = 0) const { return scenes_[id]->dev(); }
 
     int device_ = 0;
     int cu_count_ = 0;
     hipStream_t stream_ = nullptr;
     std::string last_error_;
 
-    // scene
-    bool has_scene_ = false;
-    HostScene host_;
-    DeviceBuffer d_nodes_, d_tris_, d_shade_, d_materials_, d_lights_, d_bins_;
+    // resident scenes; id 0 is the scene of InitializeSceneBuffers
+    std::unique_ptr<ResidentScene> scenes_[kMaxScenes];
+    DeviceBuffer d_scene_table_;            // kMaxScenes SceneDev records
+    DeviceBuffer d_views_;                  // kViewRing ViewRecords
 
     // path pool and queues
     pt_tuning tuning_{};
@@ -118,8 +149,11 @@ private:
     static constexpr int kRing = 4;
     hipEvent_t ring_events_[kRing] = {};
 
-    // pixels
+    // pixels: accumulation capacity for all views of a run; width_ x height_
+    // is the frame of the single-frame API
     uint32_t width_ = 0, height_ = 0;
+    uint32_t accum_capacity_ = 0;
+    bool hdr_valid_ = false;                // accum[0, w*h) holds the last single frame
     DeviceBuffer d_accum_, d_frame_;
     uint32_t* h_frame_staging_ = nullptr;   // pinned
     size_t h_frame_staging_pixels_ = 0;
@@ -132,9 +166,8 @@ private:
     DeviceBuffer d_spill_;
     uint32_t stack_total_ = 0;
 
-    // graph of two bounce iterations (ping and pong)
-    hipGraphExec_t graph_exec_ = nullptr;
-    hipGraph_t graph_ = nullptr;
+    // graphs of two bounce iterations (ping and pong), one per loop mode
+    std::vector<GraphEntry> graphs_;
 
     // hook scratch
     DeviceBuffer d_hook_in_, d_hook_out_, d_hook_aux_, d_hook_q_;

@@ -48,6 +48,11 @@ class MotionState(C.Structure):
     _fields_ = [("position", C.c_float * 3), ("rotation", C.c_float * 4)]
 
 
+class View(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("reserved", C.c_uint32), ("settings", RenderSettings), ("camera", Camera),
+                ("state", MotionState)]
+
+
 class Progress(C.Structure):
     _fields_ = [("is_rendering", C.c_uint32), ("done_frames", C.c_uint32), ("total_frames", C.c_uint32),
                 ("width", C.c_uint32), ("height", C.c_uint32), ("samples_per_pixel", C.c_uint32),
@@ -108,6 +113,10 @@ SIGNATURES = {
     "pt_consume_last_error": (C.c_size_t, [_P, C.c_char_p, C.c_size_t]),
     "pt_render_frame": (C.c_int, [_P, C.POINTER(RenderSettings), C.POINTER(Camera), C.POINTER(MotionState), _U32,
                                   C.POINTER(FrameStats)]),
+    "pt_add_scene": (C.c_int, [_P, C.POINTER(SceneDesc), C.POINTER(_U32)]),
+    "pt_remove_scene": (C.c_int, [_P, _U32]),
+    "pt_render_batch": (C.c_int, [_P, C.POINTER(View), _U32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), _U32,
+                                  C.POINTER(FrameStats)]),
     "pt_trace_closest": (C.c_int, [_P, _P, _U32, _P]),
     "pt_trace_any": (C.c_int, [_P, _P, _U32, _P]),
     "pt_trace_closest_device": (C.c_int, [_P, _P, _U32, _P, _P]),
@@ -165,6 +174,31 @@ def _motion(view):
     return m
 
 
+def _scene_desc(scene):
+    """pt_scene_desc of a scenes.Scene, and the arrays it points into."""
+    arrs = dict(
+        positions=np.ascontiguousarray(scene.positions, np.float32),
+        normals=np.ascontiguousarray(scene.normals, np.float32),
+        tangents=None if scene.tangents is None else np.ascontiguousarray(scene.tangents, np.float32),
+        material_ids=np.ascontiguousarray(scene.material_ids, np.uint32),
+        materials=np.ascontiguousarray(scene.materials),
+        light_weights=None if scene.light_weights is None else np.ascontiguousarray(scene.light_weights, np.float32),
+    )
+    d = SceneDesc(scene.triangle_count, _ptr(arrs["positions"]), _ptr(arrs["normals"]), _ptr(arrs["tangents"]),
+                  _ptr(arrs["material_ids"]), len(arrs["materials"]), _ptr(arrs["materials"]),
+                  _ptr(arrs["light_weights"]))
+    return d, arrs
+
+
+def _view(scene_id, settings, view):
+    v = View()
+    v.scene = scene_id
+    v.settings = _settings(settings)
+    v.camera = Camera(view.fov_y, 0.1, 1000.0)
+    v.state = _motion(view)
+    return v
+
+
 class Renderer:
     """Python mirror of Core::Graphics::Cuda::Renderer over the C ABI.  Method
     names follow the reference; each failure raises PtError with the status
@@ -206,17 +240,7 @@ class Renderer:
         self._check(self.lib.pt_initialize_rendering_buffers(self.h, width, height))
 
     def InitializeSceneBuffers(self, scene):
-        arrs = dict(
-            positions=np.ascontiguousarray(scene.positions, np.float32),
-            normals=np.ascontiguousarray(scene.normals, np.float32),
-            tangents=None if scene.tangents is None else np.ascontiguousarray(scene.tangents, np.float32),
-            material_ids=np.ascontiguousarray(scene.material_ids, np.uint32),
-            materials=np.ascontiguousarray(scene.materials),
-            light_weights=None if scene.light_weights is None else np.ascontiguousarray(scene.light_weights, np.float32),
-        )
-        d = SceneDesc(scene.triangle_count, _ptr(arrs["positions"]), _ptr(arrs["normals"]), _ptr(arrs["tangents"]),
-                      _ptr(arrs["material_ids"]), len(arrs["materials"]), _ptr(arrs["materials"]),
-                      _ptr(arrs["light_weights"]))
+        d, _arrs = _scene_desc(scene)
         self._check(self.lib.pt_initialize_scene_buffers(self.h, C.byref(d)))
 
     def Clear(self, r, g, b, a):
@@ -272,6 +296,34 @@ class Renderer:
         st = FrameStats()
         self._check(self.lib.pt_render_frame(self.h, C.byref(s), C.byref(cam), C.byref(m), flags, C.byref(st)))
         return st.as_dict()
+
+    # ---- resident scenes and batched views
+    def AddScene(self, scene):
+        """Uploads a further scene; returns its id (1..63)."""
+        d, _arrs = _scene_desc(scene)
+        sid = C.c_uint32()
+        self._check(self.lib.pt_add_scene(self.h, C.byref(d), C.byref(sid)))
+        return sid.value
+
+    def RemoveScene(self, scene_id):
+        self._check(self.lib.pt_remove_scene(self.h, scene_id))
+
+    def RenderBatch(self, items, flags=FRAME_DEFAULT, rgba=False):
+        """items: [(scene_id, settings dict, scenes.View)], rendered in one
+        wavefront.  Returns ([hdr (h, w, 4) per view], stats), plus the RGBA8
+        images (h, w) when rgba=True."""
+        n = len(items)
+        views = (View * n)(*[_view(sid, st, v) for sid, st, v in items])
+        hdr, ldr = [], []
+        for i in range(n):
+            w, h = views[i].settings.width, views[i].settings.height
+            hdr.append(np.zeros((h, w, 4), np.float32))
+            ldr.append(np.zeros((h, w), np.uint32) if rgba else None)
+        hp = (C.c_void_p * n)(*[a.ctypes.data for a in hdr])
+        lp = (C.c_void_p * n)(*[a.ctypes.data if a is not None else None for a in ldr])
+        st = FrameStats()
+        self._check(self.lib.pt_render_batch(self.h, views, n, hp, lp, flags, C.byref(st)))
+        return (hdr, st.as_dict(), ldr) if rgba else (hdr, st.as_dict())
 
     # ---- kernel-level hooks
     def trace_closest(self, rays):

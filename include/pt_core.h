@@ -158,7 +158,18 @@ typedef struct pt_frame_stats {
     double bin_ms;               /* summed time of the queue-binning launches (profiled frames only) */
 } pt_frame_stats;
 
-/* Frame-mode flags for pt_render_frame. */
+/* One camera view of a resident scene: a frame of RenderFrame (Renderer.cpp:468-647)
+ * as one element of a batch.  scene 0 is the scene of pt_initialize_scene_buffers;
+ * pt_add_scene returns the others. */
+typedef struct pt_view {
+    uint32_t scene;
+    uint32_t reserved;
+    pt_render_settings settings;   /* width, height, spp, depth, NEE, RR start, seed */
+    pt_camera camera;
+    pt_motion_state state;
+} pt_view;
+
+/* Frame-mode flags for pt_render_frame and pt_render_batch. */
 enum {
     PT_FRAME_DEFAULT = 0,
     PT_FRAME_PROFILE = 1,       /* time each stage with hipEvents (no graph replay) */
@@ -252,6 +263,22 @@ size_t pt_consume_last_error(pt_renderer* renderer, char* buf, size_t capacity);
 pt_status pt_render_frame(pt_renderer* renderer, const pt_render_settings* settings,
                           const pt_camera* camera, const pt_motion_state* motion_state,
                           uint32_t frame_flags, pt_frame_stats* stats);
+
+/* ---- resident scenes and batched views (no reference counterpart: the reference
+ * renders one scene and one view per RenderFrame, Renderer.cpp:468-647) ---- */
+/* Uploads a further scene (same validation as pt_initialize_scene_buffers) and
+ * returns its id in 1..63. */
+pt_status pt_add_scene(pt_renderer* renderer, const pt_scene_desc* scene, uint32_t* scene_id);
+/* Frees a resident scene (id 0 included). */
+pt_status pt_remove_scene(pt_renderer* renderer, uint32_t scene_id);
+/* Renders view_count (1..4096) views, of any resident scenes, in one wavefront: the
+ * bounce-loop kernels take paths of every view from one path pool.  Each view's
+ * image equals the image it gets rendered alone.  hdr[i] (w*h float RGBA,
+ * radiance / spp) and rgba8[i] (w*h RGBA8, row 0 = bottom) receive view i; either
+ * array, or any entry, may be NULL.  stats aggregate the whole batch. */
+pt_status pt_render_batch(pt_renderer* renderer, const pt_view* views, uint32_t view_count,
+                          float* const* hdr, uint32_t* const* rgba8, uint32_t frame_flags,
+                          pt_frame_stats* stats);
 
 /* ---- kernel-level hooks on caller-supplied batches ---- */
 /* IntersectRaysWithSceneKernel (Intersect.cu:63-123) on a ray batch. */
