@@ -40,11 +40,16 @@ def cpu_baseline(scene, settings, view, target_s):
     log(f"cpu baseline: oracle BVH built in {time.time() - t0:.1f}s, {threads} threads")
     w, h = settings["width"], settings["height"]
     centre = (h // 2) * w + w // 2
-    # calibrate on a small band, then time one band sized to about target_s
+    # calibrate on growing bands until one takes 0.5 s (thread start-up no
+    # longer dominates), then time one band sized to about target_s
     n = 256
-    t0 = time.perf_counter()
-    o.render(settings, view, centre, centre + n, threads=threads)
-    dt = time.perf_counter() - t0
+    while True:
+        t0 = time.perf_counter()
+        o.render(settings, view, centre - n // 2, centre - n // 2 + n, threads=threads)
+        dt = time.perf_counter() - t0
+        if dt >= 0.5 or n >= w * h // 2:
+            break
+        n = min(w * h // 2, n * 4)
     n = int(max(256, min(w * h // 2, n * target_s / max(dt, 1e-3))))
     t0 = time.perf_counter()
     _, ext, sh = o.render(settings, view, centre - n // 2, centre - n // 2 + n, threads=threads)

@@ -125,6 +125,39 @@ pt_status pt_render_batch(pt_renderer* r, const pt_view* views, uint32_t view_co
     return guarded(r, [&](pt::Renderer& x) { return x.RenderBatch(views, view_count, hdr, rgba8, frame_flags, stats); });
 }
 
+pt_status pt_server_start(pt_renderer* r, const pt_server_config* config) {
+    return guarded(r, [&](pt::Renderer& x) {
+        pt_server_config c{};
+        if (config) c = *config;
+        return x.ServerStart(c);
+    });
+}
+
+pt_status pt_server_submit(pt_renderer* r, const pt_view* view, float* hdr, uint32_t* rgba8, uint64_t* request_id) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!view || !request_id) return x.fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
+        return x.ServerSubmit(*view, hdr, rgba8, *request_id);
+    });
+}
+
+pt_status pt_server_wait(pt_renderer* r, uint64_t request_id, uint32_t timeout_ms, pt_request_info* info) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!info) return x.fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
+        return x.ServerWait(request_id, timeout_ms, *info);
+    });
+}
+
+pt_status pt_server_get_stats(pt_renderer* r, pt_server_stats* stats) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!stats) return x.fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
+        return x.ServerStats(*stats);
+    });
+}
+
+pt_status pt_server_stop(pt_renderer* r, pt_server_stats* stats) {
+    return guarded(r, [&](pt::Renderer& x) { return x.ServerStop(stats); });
+}
+
 pt_status pt_trace_closest(pt_renderer* r, const pt_ray* rays, uint32_t count, pt_hit* hits) {
     return guarded(r, [&](pt::Renderer& x) { return x.TraceBatch(rays, count, false, false, hits, nullptr, nullptr, nullptr); });
 }

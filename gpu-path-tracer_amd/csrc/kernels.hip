@@ -770,8 +770,9 @@ __device__ __forceinline__ uint32_t view_of_sample(const ViewRecord* views, uint
 // InitializePathsKernel / RegeneratePathsKernel, PathGeneration.cu:56-102.
 // New paths go to the free tail of the next ray queue: positions
 // [ray[nxt], ray[nxt] + n) with n = min(free positions, samples left).
-// Global sample g belongs to the view in ring slot v = view_of_sample(g); its
-// local index g - sample_begin addresses the view's pixels and sample streams.
+// Global sample g belongs to the view record in ring slot v = view_of_sample(g);
+// its local index g - sample_begin + sample_offset addresses the view's pixels
+// and sample streams.
 __global__ void k_generate(PathState p, int init, int nxt) {
     const FrameParams* fp = p.fp;
     Counters* c = p.ctr;
@@ -797,7 +798,7 @@ __global__ void k_generate(PathState p, int init, int nxt) {
         const uint64_t g = launched + k;
         const uint32_t v = view_of_sample(p.views, view_lo, view_hi, g);
         const ViewRecord* vr = p.views + v;
-        const uint64_t sample = g - vr->sample_begin;
+        const uint64_t sample = g - vr->sample_begin + vr->sample_offset;
         const uint32_t W = vr->width, H = vr->height, spp = vr->spp, axis = vr->spp_axis;
         const PcgStream s = pcg_seed(vr->seed, sample);
         Draws dr{s.state, s.inc};
@@ -1187,8 +1188,9 @@ __device__ __forceinline__ uint32_t view_of_pixel(const ViewRecord* views, uint3
 }
 
 // PostprocessKernel, Image.cu:34-54, over pixels [begin, end) of the views
-// with virtual index in [lo, hi).
-__global__ void k_postprocess(PathState p, uint32_t lo, uint32_t hi, uint32_t begin, uint32_t end) {
+// with virtual index in [lo, hi).  With `scale`, the accumulation pixels are
+// replaced by radiance / spp, ready for a plain copy to the caller.
+__global__ void k_postprocess(PathState p, uint32_t lo, uint32_t hi, uint32_t begin, uint32_t end, uint32_t scale) {
     for (uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x) {
         const float inv_spp = p.views[view_of_pixel(p.views, lo, hi, i)].inv_spp;
         const float4 a = p.accum[i];
@@ -1204,6 +1206,7 @@ __global__ void k_postprocess(PathState p, uint32_t lo, uint32_t hi, uint32_t be
             out |= b << (8 * k);
         }
         p.framebuffer[i] = out;
+        if (scale) p.accum[i] = make_float4(c[0], c[1], c[2], a.w * inv_spp);
     }
 }
 
@@ -1338,9 +1341,9 @@ hipError_t launch_prepare(hipStream_t s, const PathState& p, int cur) {
 }
 
 hipError_t launch_postprocess(hipStream_t s, const PathState& p, uint32_t lo, uint32_t hi, uint32_t begin,
-                              uint32_t end) {
+                              uint32_t end, bool scale) {
     if (end <= begin || hi <= lo) return hipSuccess;
-    hipLaunchKernelGGL(k_postprocess, dim3(grid_for(end - begin, 256, 4096)), dim3(256), 0, s, p, lo, hi, begin, end);
+    hipLaunchKernelGGL(k_postprocess, dim3(grid_for(end - begin, 256, 4096)), dim3(256), 0, s, p, lo, hi, begin, end, scale ? 1u : 0u);
     return hipGetLastError();
 }
 

@@ -76,9 +76,10 @@ hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, i
 hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, bool count, bool multi,
                          const TraceConfig& cfg);
 hipError_t launch_prepare(hipStream_t s, const PathState& p, int cur);
-// Tone-maps pixels [begin, end) of the views with virtual index in [lo, hi).
+// Tone-maps pixels [begin, end) of the views with virtual index in [lo, hi);
+// with `scale` also divides the accumulation pixels by spp in place.
 hipError_t launch_postprocess(hipStream_t s, const PathState& p, uint32_t lo, uint32_t hi, uint32_t begin,
-                              uint32_t end);
+                              uint32_t end, bool scale = false);
 
 // Batch hooks.  rays: n x 2 float4 (origin|tMin, direction|tMax).
 // Closest: hits[i] in pt_hit layout (prim bits, t, u, v); any: occluded[i].  nodes/tris may be null.

@@ -211,11 +211,13 @@ struct CameraBasis {           // DeviceCamera, DeviceCamera.hpp:5-11
 // One camera view of one resident scene: a frame of the single-frame API, or
 // one element of a batch or of the serving stream.  Views live in a device
 // ring of kViewRing records addressed by a monotonically growing virtual
-// index; the ring slot (virtual & (kViewRing-1)) is what a path carries.  View
-// v owns the global samples [sample_begin, sample_begin + sample_count) and
-// pixels [accum_offset, accum_offset + w*h) of the accumulation buffer.  Its
-// sample streams are keyed by (seed, local sample index), so a view renders
-// the same alone, in a batch or interleaved with other requests.
+// index; the ring slot (virtual & (kViewRing-1)) is what a path carries.  A
+// record may be one chunk of a view: it owns the global samples
+// [sample_begin, sample_begin + sample_count), which are the view's local
+// samples [sample_offset, sample_offset + sample_count), and the view's pixels
+// [accum_offset, accum_offset + w*h) of the accumulation buffer.  Sample
+// streams are keyed by (seed, local sample index), so a view renders the same
+// alone, in a batch or interleaved with other requests.
 struct ViewRecord {
     CameraBasis cam;
     uint32_t width, height, spp, spp_axis;
@@ -223,10 +225,11 @@ struct ViewRecord {
     uint64_t seed;
     uint64_t sample_begin;
     uint64_t sample_count;
+    uint64_t sample_offset;
     uint32_t accum_offset;
     float inv_spp;
 };
-static_assert(sizeof(ViewRecord) == 112, "view record is 112 B");
+static_assert(sizeof(ViewRecord) == 120, "view record is 120 B");
 constexpr uint32_t kViewRing = 4096;       // ring slot lives in bits 8..23 of the path flags
 constexpr uint32_t kMaxScenes = 64;
 

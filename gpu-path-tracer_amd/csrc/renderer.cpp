@@ -1,4 +1,5 @@
 #include "renderer.hpp"
+#include "server.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -47,6 +48,7 @@ Renderer::Renderer() {
 }
 
 Renderer::~Renderer() {
+    if (server_) (void)ServerStop(nullptr);
     (void)StopRendering();
     invalidate_graph();
     for (auto& e : ring_events_)
@@ -63,17 +65,6 @@ pt_status Renderer::check_hip(hipError_t e, const char* what) {
     return fail(s, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define PT_TRY_HIP(expr, what)                              \
-    do {                                                    \
-        pt_status _s = check_hip((expr), (what));           \
-        if (_s != PT_OK) return _s;                         \
-    } while (0)
-
-#define PT_TRY(expr)                                        \
-    do {                                                    \
-        pt_status _s = (expr);                              \
-        if (_s != PT_OK) return _s;                         \
-    } while (0)
 
 pt_status Renderer::init() {
     int count = 0;
@@ -667,6 +658,7 @@ pt_status Renderer::ResumeSimulation(uint32_t start_frame) {
 }
 
 pt_status Renderer::StopRendering() {
+    if (server_) return fail(PT_ERROR_INVALID_STATE, "the server is running: stop it with pt_server_stop");
     if (render_thread_.joinable()) {
         stop_requested_.store(true);
         render_thread_.join();

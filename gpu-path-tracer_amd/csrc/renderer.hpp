@@ -35,6 +35,8 @@ private:
     size_t bytes_ = 0;
 };
 
+struct ServerState;   // server.hpp
+
 // A scene resident on the device: host copy (for the hooks and the BVH
 // queries) and its device records.
 struct ResidentScene {
@@ -74,6 +76,13 @@ public:
     pt_status RenderBatch(const pt_view* views, uint32_t n, float* const* hdr, uint32_t* const* rgba, uint32_t flags,
                           pt_frame_stats* stats);
 
+    // serving (server.cpp)
+    pt_status ServerStart(const pt_server_config& cfg);
+    pt_status ServerSubmit(const pt_view& view, float* hdr, uint32_t* rgba, uint64_t& id);
+    pt_status ServerWait(uint64_t id, uint32_t timeout_ms, pt_request_info& info);
+    pt_status ServerStats(pt_server_stats& stats);
+    pt_status ServerStop(pt_server_stats* stats);
+
     // hooks
     pt_status TraceBatch(const pt_ray* rays, uint32_t n, bool any, bool device, void* out, uint32_t* nodes,
                          uint32_t* tris, hipStream_t stream);
@@ -86,7 +95,9 @@ public:
     pt_status SetTuning(const pt_tuning& t);
 
     const char* error_message() const { return last_error_.c_str(); }
+    // Records the message of a failing call (the server threads fail too).
     pt_status fail(pt_status s, const std::string& msg) {
+        std::scoped_lock lock(message_mutex_);
         last_error_ = msg;
         return s;
     }
@@ -119,6 +130,9 @@ private:
     pt_status render_frame(const pt_render_settings& s, const pt_camera& camera, const pt_motion_state& state,
                            uint32_t flags, pt_frame_stats* stats, const std::atomic<bool>* stop);
     pt_status launch_iteration(int cur, bool count, const LoopMode& mode);
+    void server_loop();
+    void readback_loop();
+    pt_status server_step();
     pt_status save_frame_png(const std::string& path);
     void simulation_loop(uint32_t start_frame);
     void invalidate_graph();
@@ -132,6 +146,7 @@ private:
     int cu_count_ = 0;
     hipStream_t stream_ = nullptr;
     std::string last_error_;
+    std::mutex message_mutex_;
 
     // resident scenes; id 0 is the scene of InitializeSceneBuffers
     std::unique_ptr<ResidentScene> scenes_[kMaxScenes];
@@ -186,6 +201,22 @@ private:
     std::string output_folder_;
     mutable std::mutex error_mutex_;
     std::optional<std::string> parked_error_;
+
+    std::unique_ptr<ServerState> server_;
+    LoopMode server_mode_;
+    hipGraphExec_t server_graph_ = nullptr;
 };
+
+#define PT_TRY_HIP(expr, what)                              \
+    do {                                                    \
+        pt_status _s = check_hip((expr), (what));           \
+        if (_s != PT_OK) return _s;                         \
+    } while (0)
+
+#define PT_TRY(expr)                                        \
+    do {                                                    \
+        pt_status _s = (expr);                              \
+        if (_s != PT_OK) return _s;                         \
+    } while (0)
 
 }  // namespace pt

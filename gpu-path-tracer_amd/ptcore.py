@@ -19,6 +19,7 @@ STATUS_NAMES = {0: "PT_OK", 1: "PT_ERROR_INVALID_ARGUMENT", 2: "PT_ERROR_INVALID
                 4: "PT_ERROR_DEVICE", 5: "PT_ERROR_OUT_OF_MEMORY", 6: "PT_ERROR_UNSUPPORTED", 7: "PT_ERROR_IO"}
 QUEUE_DIFFUSE, QUEUE_GGX, QUEUE_MIRROR, QUEUE_EMISSIVE, QUEUE_REGEN = range(5)
 INVALID = 0xFFFFFFFF
+REQUEST_QUEUED, REQUEST_RUNNING, REQUEST_DONE, REQUEST_FAILED = 0, 1, 2, 3
 FRAME_DEFAULT, FRAME_PROFILE, FRAME_INSTRUMENT, FRAME_NO_GRAPH = 0, 1, 2, 4
 
 
@@ -51,6 +52,22 @@ class MotionState(C.Structure):
 class View(C.Structure):
     _fields_ = [("scene", C.c_uint32), ("reserved", C.c_uint32), ("settings", RenderSettings), ("camera", Camera),
                 ("state", MotionState)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("accum_budget_bytes", C.c_uint64), ("window_samples", C.c_uint64), ("chunk_samples", C.c_uint64),
+                ("max_requests", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("state", C.c_uint32), ("scene", C.c_uint32), ("samples", C.c_uint64),
+                ("submit_ms", C.c_double), ("admit_ms", C.c_double), ("done_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests_submitted", C.c_uint64), ("requests_done", C.c_uint64), ("requests_failed", C.c_uint64),
+                ("samples", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("iterations", C.c_uint64), ("wall_ms", C.c_double), ("busy_ms", C.c_double)]
 
 
 class Progress(C.Structure):
@@ -117,6 +134,11 @@ SIGNATURES = {
     "pt_remove_scene": (C.c_int, [_P, _U32]),
     "pt_render_batch": (C.c_int, [_P, C.POINTER(View), _U32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), _U32,
                                   C.POINTER(FrameStats)]),
+    "pt_server_start": (C.c_int, [_P, C.POINTER(ServerConfig)]),
+    "pt_server_submit": (C.c_int, [_P, C.POINTER(View), _P, _P, C.POINTER(C.c_uint64)]),
+    "pt_server_wait": (C.c_int, [_P, C.c_uint64, _U32, C.POINTER(RequestInfo)]),
+    "pt_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "pt_server_stop": (C.c_int, [_P, C.POINTER(ServerStats)]),
     "pt_trace_closest": (C.c_int, [_P, _P, _U32, _P]),
     "pt_trace_any": (C.c_int, [_P, _P, _U32, _P]),
     "pt_trace_closest_device": (C.c_int, [_P, _P, _U32, _P, _P]),
@@ -324,6 +346,43 @@ class Renderer:
         st = FrameStats()
         self._check(self.lib.pt_render_batch(self.h, views, n, hp, lp, flags, C.byref(st)))
         return (hdr, st.as_dict(), ldr) if rgba else (hdr, st.as_dict())
+
+    # ---- serving
+    def ServerStart(self, accum_budget_bytes=0, window_samples=0, chunk_samples=0, max_requests=0):
+        cfg = ServerConfig(accum_budget_bytes, window_samples, chunk_samples, max_requests, 0)
+        self._check(self.lib.pt_server_start(self.h, C.byref(cfg)))
+        self._requests = {}
+
+    def ServerSubmit(self, scene_id, settings, view, hdr=True, rgba=False):
+        """Queues one view; returns its request id.  The result arrays are
+        held here until ServerResult takes them."""
+        v = _view(scene_id, settings, view)
+        w, h = v.settings.width, v.settings.height
+        out_hdr = np.zeros((h, w, 4), np.float32) if hdr else None
+        out_rgba = np.zeros((h, w), np.uint32) if rgba else None
+        rid = C.c_uint64()
+        self._check(self.lib.pt_server_submit(self.h, C.byref(v), _ptr(out_hdr), _ptr(out_rgba), C.byref(rid)))
+        self._requests[rid.value] = (out_hdr, out_rgba)
+        return rid.value
+
+    def ServerWait(self, request_id, timeout_ms=120000):
+        info = RequestInfo()
+        self._check(self.lib.pt_server_wait(self.h, request_id, timeout_ms, C.byref(info)))
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    def ServerResult(self, request_id):
+        """(hdr, rgba) arrays of a finished request."""
+        return self._requests.pop(request_id)
+
+    def ServerStats(self):
+        st = ServerStats()
+        self._check(self.lib.pt_server_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def ServerStop(self):
+        st = ServerStats()
+        self._check(self.lib.pt_server_stop(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
 
     # ---- kernel-level hooks
     def trace_closest(self, rays):

@@ -280,6 +280,60 @@ pt_status pt_render_batch(pt_renderer* renderer, const pt_view* views, uint32_t 
                           float* const* hdr, uint32_t* const* rgba8, uint32_t frame_flags,
                           pt_frame_stats* stats);
 
+/* ---- serving (no reference counterpart: the reference serves one request at a time
+ * through StartSimulation/RenderFrame, Renderer.cpp:330-399,468-647) ----
+ * A server thread keeps every resident scene on the device and runs one continuous
+ * wavefront.  Submitted views are admitted while their pixels fit the memory budget;
+ * admitted requests share the path pool and the material and shadow queues, their
+ * samples scheduled in chunks, fewest remaining samples first.  A request's result
+ * equals the view rendered alone.  Scenes cannot change while the server runs; the
+ * other render calls return PT_ERROR_INVALID_STATE meanwhile. */
+typedef struct pt_server_config {
+    uint64_t accum_budget_bytes;  /* device memory for admitted requests' pixels, 20 B per
+                                     pixel; 0 = 2 GiB.  A request larger than it is rejected. */
+    uint64_t window_samples;      /* samples queued on the device ahead of the launch front;
+                                     0 = twice the path pool */
+    uint64_t chunk_samples;       /* largest piece of one request scheduled at once; 0 = pool/4 */
+    uint32_t max_requests;        /* requests admitted at once, 1..2048; 0 = 1024 */
+    uint32_t reserved;
+} pt_server_config;
+
+enum { PT_REQUEST_QUEUED = 0, PT_REQUEST_RUNNING = 1, PT_REQUEST_DONE = 2, PT_REQUEST_FAILED = 3 };
+
+typedef struct pt_request_info {
+    uint64_t id;
+    uint32_t state;               /* PT_REQUEST_* */
+    uint32_t scene;
+    uint64_t samples;
+    double submit_ms;             /* server clock: ms since pt_server_start */
+    double admit_ms;              /* pixels allocated, first chunk may be scheduled */
+    double done_ms;               /* results copied to the caller's buffers */
+} pt_request_info;
+
+typedef struct pt_server_stats {
+    uint64_t requests_submitted;
+    uint64_t requests_done;
+    uint64_t requests_failed;
+    uint64_t samples;             /* camera samples of finished requests */
+    uint64_t extension_rays;      /* rays traced by the server's wavefront so far */
+    uint64_t shadow_rays;
+    uint64_t iterations;          /* bounce iterations run */
+    double wall_ms;               /* server start to now (or to stop) */
+    double busy_ms;               /* time with at least one admitted request */
+} pt_server_stats;
+
+pt_status pt_server_start(pt_renderer* renderer, const pt_server_config* config);
+/* Queues one view; hdr (w*h float RGBA, radiance / spp) and rgba8 (w*h RGBA8, row 0 =
+ * bottom) may be NULL and must stay valid until the request is done. */
+pt_status pt_server_submit(pt_renderer* renderer, const pt_view* view, float* hdr, uint32_t* rgba8,
+                           uint64_t* request_id);
+/* Waits up to timeout_ms for the request to finish; info->state tells whether it did. */
+pt_status pt_server_wait(pt_renderer* renderer, uint64_t request_id, uint32_t timeout_ms,
+                         pt_request_info* info);
+pt_status pt_server_get_stats(pt_renderer* renderer, pt_server_stats* stats);
+/* Finishes every submitted request, then stops the server thread. */
+pt_status pt_server_stop(pt_renderer* renderer, pt_server_stats* stats);
+
 /* ---- kernel-level hooks on caller-supplied batches ---- */
 /* IntersectRaysWithSceneKernel (Intersect.cu:63-123) on a ray batch. */
 pt_status pt_trace_closest(pt_renderer* renderer, const pt_ray* rays, uint32_t count, pt_hit* hits);

@@ -82,3 +82,38 @@ int main() {
     assert r.returncode == 0, r.stderr
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stderr
+
+
+# C struct of the header -> the ctypes mirror in the binding (or numpy dtype)
+_LAYOUTS = {
+    "pt_scene_desc": ptcore.SceneDesc, "pt_render_settings": ptcore.RenderSettings, "pt_camera": ptcore.Camera,
+    "pt_motion_state": ptcore.MotionState, "pt_view": ptcore.View, "pt_progress": ptcore.Progress,
+    "pt_frame_stats": ptcore.FrameStats, "pt_alias_bin": ptcore.AliasBin, "pt_bvh_info": ptcore.BvhInfo,
+    "pt_tuning": ptcore.Tuning, "pt_server_config": ptcore.ServerConfig, "pt_request_info": ptcore.RequestInfo,
+    "pt_server_stats": ptcore.ServerStats,
+}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Size and every field offset of the ctypes mirrors equal the C layout
+    that gcc gives the header's structs."""
+    import scenes
+    lines = []
+    for cname, cls in _LAYOUTS.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append('printf("pt_material size %zu\\n", sizeof(pt_material));')
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "pt_core.h"\nint main(void) {\n'
+                   + "\n".join(lines) + "\nreturn 0;\n}\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, cls in _LAYOUTS.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)
+    assert got[("pt_material", "size")] == scenes.MATERIAL_DTYPE.itemsize
