@@ -181,7 +181,8 @@ uint32_t leaf_ref_checked(const TNode* n, Bvh4Build& out) {
     return make_leaf_ref(n->begin, n->count);
 }
 
-// Emits the BVH4 node that replaces BVH2 node `n` (internal) in DFS preorder.
+// Emits the node that replaces BVH2 node `n` (internal) in DFS preorder: up to
+// out.width children, by expanding the largest-area internal child.
 uint32_t emit(const TNode* n, Bvh4Build& out, uint32_t level) {
     const uint32_t index = (uint32_t)out.nodes.size();
     out.nodes.emplace_back();
@@ -190,7 +191,7 @@ uint32_t emit(const TNode* n, Bvh4Build& out, uint32_t level) {
 
     const TNode* ch[4] = {n->left.get(), n->right.get(), nullptr, nullptr};
     int count = 2;
-    while (count < 4) {
+    while (count < (int)out.width) {
         int pick = -1;
         float best = -1.0f;
         for (int i = 0; i < count; i++) {
@@ -221,6 +222,25 @@ uint32_t emit(const TNode* n, Bvh4Build& out, uint32_t level) {
 
 }  // namespace
 
+std::vector<Bvh2Node> to_bvh2(const Bvh4Build& b) {
+    std::vector<Bvh2Node> out(b.nodes.size());
+    for (size_t i = 0; i < b.nodes.size(); i++) {
+        const Bvh4Node& s = b.nodes[i];
+        Bvh2Node& d = out[i];
+        for (int k = 0; k < 2; k++) {
+            d.lox[k] = s.lox[k];
+            d.hix[k] = s.hix[k];
+            d.loy[k] = s.loy[k];
+            d.hiy[k] = s.hiy[k];
+            d.loz[k] = s.loz[k];
+            d.hiz[k] = s.hiz[k];
+            d.child[k] = s.child[k];
+            d.info[k] = s.info[k];
+        }
+    }
+    return out;
+}
+
 void pad_bounds(float lo[3], float hi[3]) {
     const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
     const float largest = fmaxf(ex, fmaxf(ey, ez));
@@ -235,6 +255,7 @@ Bvh4Build build_bvh4(const float* positions, uint32_t T, const BvhBuildOptions& 
     if (options.max_leaf_size < 1 || options.max_leaf_size > kMaxLeafSize)
         throw std::runtime_error("max_leaf_size must be in [1, 16]");
     if (options.bins < 2 || options.bins > 256) throw std::runtime_error("bins must be in [2, 256]");
+    if (options.width != 2 && options.width != 4) throw std::runtime_error("BVH node width must be 2 or 4");
     if ((uint64_t)T > (uint64_t)kMaxLeafFirst) throw std::runtime_error("scene exceeds 2^27 triangles");
 
     Builder b;
@@ -260,6 +281,7 @@ Bvh4Build build_bvh4(const float* positions, uint32_t T, const BvhBuildOptions& 
     }
 
     Bvh4Build out;
+    out.width = options.width;
     if (T == 0) {
         out.nodes.emplace_back();
         clear_node(out.nodes[0]);

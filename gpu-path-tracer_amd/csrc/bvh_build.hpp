@@ -13,12 +13,16 @@
 namespace pt {
 
 struct Bvh4Build {
-    std::vector<Bvh4Node> nodes;      // nodes[0] is the root
+    std::vector<Bvh4Node> nodes;      // nodes[0] is the root; slots >= width stay empty
     std::vector<uint32_t> order;      // order[k] = original prim of BVH slot k
     uint32_t depth = 0;               // BVH4 levels below the root (root = 1)
     uint32_t leaf_count = 0;
     uint32_t max_leaf = 0;
+    uint32_t width = 4;               // children per node: 4, or 2 (no collapse)
 };
+
+// The nodes of a width-2 build in the 64-byte layout.
+std::vector<Bvh2Node> to_bvh2(const Bvh4Build& b);
 
 struct BvhBuildOptions {
     uint32_t max_leaf_size = 8;       // <= kMaxLeafSize
@@ -26,6 +30,7 @@ struct BvhBuildOptions {
     float traversal_cost = 1.0f;
     float intersection_cost = 1.0f;
     uint32_t threads = 0;             // 0 = hardware concurrency
+    uint32_t width = 4;               // 4: collapse the BVH2 into BVH4 nodes; 2: keep it
 };
 
 // positions: [T][3][3] floats.  Throws std::runtime_error on bad input.

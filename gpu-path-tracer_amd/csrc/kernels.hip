@@ -173,6 +173,47 @@ __device__ __forceinline__ void box4(const Bvh4Node* node, const RayFrame& r, fl
     }
 }
 
+// Entry distances and references of the children of a node of width W.
+template <int W>
+struct NodeOf;
+template <>
+struct NodeOf<4> {
+    using type = Bvh4Node;
+};
+template <>
+struct NodeOf<2> {
+    using type = Bvh2Node;
+};
+
+__device__ __forceinline__ void children(const Bvh4Node* node, const RayFrame& r, float t0[4], uint32_t c[4]) {
+    box4(node, r, r.tmax, t0);
+    const uint4 cref = reinterpret_cast<const uint4*>(node)[6];
+    c[0] = cref.x;
+    c[1] = cref.y;
+    c[2] = cref.z;
+    c[3] = cref.w;
+}
+
+__device__ __forceinline__ void children(const Bvh2Node* node, const RayFrame& r, float t0[2], uint32_t c[2]) {
+    const float4* p = reinterpret_cast<const float4*>(node);
+    const float4 x = p[0], y = p[1], z = p[2];
+    const uint4 cref = reinterpret_cast<const uint4*>(node)[3];
+    const float lx[2] = {x.x, x.y}, hx[2] = {x.z, x.w};
+    const float ly[2] = {y.x, y.y}, hy[2] = {y.z, y.w};
+    const float lz[2] = {z.x, z.y}, hz[2] = {z.z, z.w};
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const float ax = fmaf(lx[i], r.rd.x, -r.ord.x), bx = fmaf(hx[i], r.rd.x, -r.ord.x);
+        const float ay = fmaf(ly[i], r.rd.y, -r.ord.y), by = fmaf(hy[i], r.rd.y, -r.ord.y);
+        const float az = fmaf(lz[i], r.rd.z, -r.ord.z), bz = fmaf(hz[i], r.rd.z, -r.ord.z);
+        const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), r.tmin));
+        const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), r.tmax));
+        t0[i] = tn <= tf ? tn : __builtin_inff();
+    }
+    c[0] = cref.x;
+    c[1] = cref.y;
+}
+
 __device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
     if (tb < ta) {
         const float t = ta;
@@ -193,9 +234,10 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32
 // global area past `LDS` entries.  Each entry is (node ref, entry distance) so
 // popped subtrees that lie beyond the current hit are skipped without a fetch.
 
-template <int BLOCK, int LDS, bool ANY, bool COUNT, bool MULTI, class Src>
+template <int BLOCK, int LDS, int W, bool ANY, bool COUNT, bool MULTI, class Src>
 __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev& sc, const SceneDev* scenes,
                                                  const TraceConfig& cfg) {
+    using Node = typename NodeOf<W>::type;
     __shared__ uint2 stack[LDS][BLOCK];
     const uint32_t tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
@@ -219,7 +261,7 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
     uint32_t chunk_next = min(wave_id * chunk, n), chunk_end = min(wave_id * chunk + chunk, n);
     bool exhausted = chunk_next >= n;
     uint32_t item = 0, ref = 0, sp = 0;
-    const Bvh4Node* nodes_base = sc.nodes;   // per lane when MULTI: the ray's scene
+    const Node* nodes_base = static_cast<const Node*>(sc.nodes);   // per lane when MULTI: the ray's scene
     const TriRecord* tris_base = sc.tris;
     RayFrame r;
     float best_t = 0.0f, best_u = 0.0f, best_v = 0.0f;
@@ -294,7 +336,7 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                         uint32_t scene = 0;
                         item = src.load(chunk_next + lane_rank, o, d, tmin, tmax, scene);
                         if (MULTI) {
-                            nodes_base = scenes[scene].nodes;
+                            nodes_base = static_cast<const Node*>(scenes[scene].nodes);
                             tris_base = scenes[scene].tris;
                         }
                         setup_ray(r, o, d, tmin, tmax);
@@ -320,24 +362,26 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
             const bool step = active && !fin && !ref_is_leaf(ref);
             if (__ballot(step) == 0) break;
             if (step) {
-                float t0[4];
-                const Bvh4Node* node = nodes_base + ref;
-                box4(node, r, r.tmax, t0);
-                const uint4 cref = reinterpret_cast<const uint4*>(node)[6];
-                uint32_t c[4] = {cref.x, cref.y, cref.z, cref.w};
+                float t0[W];
+                uint32_t c[W];
+                children(nodes_base + ref, r, t0, c);
                 if (COUNT) nodes++;
                 if (!ANY) {
-                    // nearest child first: sort the four (distance, ref) pairs
-                    cswap(t0[0], c[0], t0[1], c[1]);
-                    cswap(t0[2], c[2], t0[3], c[3]);
-                    cswap(t0[0], c[0], t0[2], c[2]);
-                    cswap(t0[1], c[1], t0[3], c[3]);
-                    cswap(t0[1], c[1], t0[2], c[2]);
+                    // nearest child first: sort the (distance, ref) pairs
+                    if (W == 4) {
+                        cswap(t0[0], c[0], t0[1], c[1]);
+                        cswap(t0[W - 2], c[W - 2], t0[W - 1], c[W - 1]);
+                        cswap(t0[0], c[0], t0[W - 2], c[W - 2]);
+                        cswap(t0[1], c[1], t0[W - 1], c[W - 1]);
+                        cswap(t0[1], c[1], t0[W - 2], c[W - 2]);
+                    } else {
+                        cswap(t0[0], c[0], t0[1], c[1]);
+                    }
                 }
                 uint32_t next = kInvalid;
                 float tnext = 0.0f;
 #pragma unroll
-                for (int i = 3; i >= 0; i--) {
+                for (int i = W - 1; i >= 0; i--) {
                     if (t0[i] != __builtin_inff() && c[i] != kInvalid) {
                         if (next != kInvalid) push(next, tnext);   // the previous candidate is farther
                         next = c[i];
@@ -489,9 +533,9 @@ struct BatchSrc {
     }
 };
 
-template <int BLOCK, int LDS, bool ANY, bool COUNT, bool MULTI, class Src>
+template <int BLOCK, int LDS, int W, bool ANY, bool COUNT, bool MULTI, class Src>
 __global__ __launch_bounds__(BLOCK) void k_trace(Src src, SceneDev sc, const SceneDev* scenes, TraceConfig cfg) {
-    trace_persistent<BLOCK, LDS, ANY, COUNT, MULTI, Src>(src, sc, scenes, cfg);
+    trace_persistent<BLOCK, LDS, W, ANY, COUNT, MULTI, Src>(src, sc, scenes, cfg);
 }
 
 // ================================================================ shading math
@@ -1243,26 +1287,37 @@ uint32_t grid_for(uint32_t n, uint32_t block, uint32_t max_grid) {
     return (uint32_t)g;
 }
 
-template <int BLOCK, int LDS, bool ANY, bool COUNT, bool MULTI, class Src>
+template <int BLOCK, int LDS, int W, bool ANY, bool COUNT, bool MULTI, class Src>
 hipError_t run_trace(hipStream_t s, const Src& src, const SceneDev& sc, const SceneDev* scenes, const TraceConfig& cfg) {
-    hipLaunchKernelGGL((k_trace<BLOCK, LDS, ANY, COUNT, MULTI, Src>), dim3(cfg.blocks), dim3(BLOCK), 0, s, src, sc,
+    hipLaunchKernelGGL((k_trace<BLOCK, LDS, W, ANY, COUNT, MULTI, Src>), dim3(cfg.blocks), dim3(BLOCK), 0, s, src, sc,
                        scenes, cfg);
     return hipGetLastError();
 }
 
+template <int W, bool ANY, bool COUNT, bool MULTI, class Src>
+hipError_t dispatch_block(hipStream_t s, const Src& src, const SceneDev& sc, const SceneDev* scenes,
+                          const TraceConfig& cfg) {
+    if (cfg.block == 64 && cfg.lds_stack == 16) return run_trace<64, 16, W, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
+    if (cfg.block == 128 && cfg.lds_stack == 16) return run_trace<128, 16, W, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
+    if (cfg.block == 256 && cfg.lds_stack == 8) return run_trace<256, 8, W, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
+    if (cfg.block == 512 && cfg.lds_stack == 8) return run_trace<512, 8, W, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
+    return hipErrorInvalidValue;
+}
+
+// Node width from the scene record, block size from the configuration.
 template <bool ANY, bool COUNT, bool MULTI, class Src>
 hipError_t dispatch_trace(hipStream_t s, const Src& src, const SceneDev& sc, const SceneDev* scenes,
                           const TraceConfig& cfg) {
-    if (cfg.block == 64 && cfg.lds_stack == 16) return run_trace<64, 16, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
-    if (cfg.block == 128 && cfg.lds_stack == 16) return run_trace<128, 16, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
-    if (cfg.block == 256 && cfg.lds_stack == 8) return run_trace<256, 8, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
+    if (sc.node_width == 4) return dispatch_block<4, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
+    if (sc.node_width == 2) return dispatch_block<2, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
     return hipErrorInvalidValue;
 }
 
 template <int BLOCK, int LDS, bool ANY, bool COUNT, class Src>
 int occ_one() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<BLOCK, LDS, ANY, COUNT, false, Src>, BLOCK, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<BLOCK, LDS, 4, ANY, COUNT, false, Src>, BLOCK, 0) !=
+        hipSuccess)
         return 0;
     return nb;
 }
@@ -1272,13 +1327,15 @@ int occ(const TraceConfig& cfg) {
     if (cfg.block == 64 && cfg.lds_stack == 16) return occ_one<64, 16, ANY, COUNT, Src>();
     if (cfg.block == 128 && cfg.lds_stack == 16) return occ_one<128, 16, ANY, COUNT, Src>();
     if (cfg.block == 256 && cfg.lds_stack == 8) return occ_one<256, 8, ANY, COUNT, Src>();
+    if (cfg.block == 512 && cfg.lds_stack == 8) return occ_one<512, 8, ANY, COUNT, Src>();
     return 0;
 }
 
 }  // namespace
 
 bool trace_config_supported(uint32_t block, uint32_t lds_stack) {
-    return (block == 64 && lds_stack == 16) || (block == 128 && lds_stack == 16) || (block == 256 && lds_stack == 8);
+    return (block == 64 && lds_stack == 16) || (block == 128 && lds_stack == 16) || (block == 256 && lds_stack == 8) ||
+           (block == 512 && lds_stack == 8);
 }
 
 int trace_occupancy(const TraceConfig& cfg, bool any, bool count) {

@@ -13,7 +13,7 @@ constexpr int kQueueRegen = 4;
 constexpr int kQueueSlots = 5;
 
 struct SceneDev {
-    const Bvh4Node* nodes;
+    const void* nodes;              // Bvh4Node[] or Bvh2Node[] by node_width
     const TriRecord* tris;          // BVH order
     const ShadeRecord* shade;       // original order
     const MaterialRecord* materials;
@@ -21,6 +21,8 @@ struct SceneDev {
     const AliasBin* bins;
     uint32_t light_count;
     uint32_t tri_count;
+    uint32_t node_width;            // 4 or 2
+    uint32_t pad;
 };
 
 // Path state, compacted every bounce: ray queue k holds, at position i, the
@@ -66,8 +68,9 @@ hipError_t launch_frame_begin(hipStream_t s, const PathState& p);
 // Zeroes accumulation pixels [begin, end).
 hipError_t launch_fill_accum(hipStream_t s, const PathState& p, uint32_t begin, uint32_t end);
 hipError_t launch_generate(hipStream_t s, const PathState& p, bool init, int nxt, uint32_t grid, uint32_t block);
-// `multi`: rays of several scenes (per-ray node/triangle pointers from p.scenes);
-// otherwise every ray belongs to scene `sc`.
+// `multi`: rays of several scenes (per-ray node/triangle pointers from p.scenes,
+// all scenes of node width sc.node_width); otherwise every ray belongs to scene
+// `sc`.
 hipError_t launch_closest(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool count, bool multi,
                           const TraceConfig& cfg);
 hipError_t launch_bin(hipStream_t s, const PathState& p, int cur, uint32_t grid);

@@ -68,6 +68,15 @@ struct alignas(128) Bvh4Node {
 };
 static_assert(sizeof(Bvh4Node) == 128, "BVH4 node must fill one cache line");
 
+// BVH2 node: 64 bytes, half a line, the same references and empty-slot rule.
+// Rows are (lo0, lo1, hi0, hi1) per axis, so one float4 load serves one axis.
+struct alignas(64) Bvh2Node {
+    float lox[2], hix[2], loy[2], hiy[2], loz[2], hiz[2];
+    uint32_t child[2];
+    uint32_t info[2];
+};
+static_assert(sizeof(Bvh2Node) == 64, "BVH2 node is 64 bytes");
+
 constexpr uint32_t kLeafBit = 0x80000000u;
 PT_HD bool ref_is_leaf(uint32_t r) { return (r & kLeafBit) != 0u; }
 PT_HD uint32_t leaf_first(uint32_t r) { return (r & 0x7FFFFFFFu) >> 4; }

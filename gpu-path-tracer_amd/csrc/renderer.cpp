@@ -45,6 +45,7 @@ Renderer::Renderer() {
     tuning_.bvh_max_leaf = 8;
     tuning_.bvh_traversal_cost = 1.0f;
     tuning_.trace_refill = 16;
+    tuning_.bvh_width = 4;
 }
 
 Renderer::~Renderer() {
@@ -125,7 +126,7 @@ PathState Renderer::path_state() const {
 
 SceneDev ResidentScene::dev() const {
     SceneDev s{};
-    s.nodes = nodes.as<Bvh4Node>();
+    s.nodes = nodes.as<void>();
     s.tris = tris.as<TriRecord>();
     s.shade = shade.as<ShadeRecord>();
     s.materials = materials.as<MaterialRecord>();
@@ -133,6 +134,7 @@ SceneDev ResidentScene::dev() const {
     s.bins = bins.as<AliasBin>();
     s.light_count = (uint32_t)host.lights.size();
     s.tri_count = host.tri_count;
+    s.node_width = host.bvh.width;
     return s;
 }
 
@@ -205,12 +207,11 @@ pt_status Renderer::configure_trace() {
     int occ = std::max(1, std::min(occ_closest, occ_shadow));
     if (tuning_.trace_blocks_per_cu) occ = (int)tuning_.trace_blocks_per_cu;
     cfg.blocks = (uint32_t)(occ * std::max(1, cu_count_));
-    // worst-case stack over every resident scene: a BVH4 level pushes at most
-    // three siblings
-    uint32_t depth = 1;
+    // worst-case stack over every resident scene: a level of a width-W BVH
+    // pushes at most W-1 siblings
+    stack_total_ = 3u + 2u;
     for (const auto& sc : scenes_)
-        if (sc) depth = std::max(depth, sc->host.bvh.depth);
-    stack_total_ = 3u * depth + 2u;
+        if (sc) stack_total_ = std::max(stack_total_, (sc->host.bvh.width - 1u) * sc->host.bvh.depth + 2u);
     cfg.spill_depth = stack_total_ > cfg.lds_stack ? stack_total_ - cfg.lds_stack : 0u;
     const size_t spill_bytes = (size_t)cfg.spill_depth * cfg.blocks * cfg.block * sizeof(uint2);
     PT_TRY_HIP(d_spill_.allocate(spill_bytes), "traversal stack spill area");
@@ -236,6 +237,7 @@ pt_status Renderer::upload_scene(uint32_t id, const pt_scene_desc& desc) {
     BvhBuildOptions bo;
     bo.max_leaf_size = tuning_.bvh_max_leaf;
     bo.traversal_cost = tuning_.bvh_traversal_cost;
+    bo.width = tuning_.bvh_width;
     auto rs = std::make_unique<ResidentScene>();
     SceneError err{PT_OK, ""};
     if (!build_host_scene(desc, bo, rs->host, err)) return fail(err.status, err.message);
@@ -245,7 +247,13 @@ pt_status Renderer::upload_scene(uint32_t id, const pt_scene_desc& desc) {
         if (bytes) PT_TRY_HIP(hipMemcpyAsync(b.as<void>(), src, bytes, hipMemcpyHostToDevice, stream_), what);
         return PT_OK;
     };
-    PT_TRY(upload(rs->nodes, h.bvh.nodes.data(), h.bvh.nodes.size() * sizeof(Bvh4Node), "BVH nodes"));
+    if (h.bvh.width == 2) {
+        const std::vector<Bvh2Node> n2 = to_bvh2(h.bvh);
+        PT_TRY(upload(rs->nodes, n2.data(), n2.size() * sizeof(Bvh2Node), "BVH nodes"));
+        PT_TRY_HIP(hipStreamSynchronize(stream_), "BVH nodes");
+    } else {
+        PT_TRY(upload(rs->nodes, h.bvh.nodes.data(), h.bvh.nodes.size() * sizeof(Bvh4Node), "BVH nodes"));
+    }
     PT_TRY(upload(rs->tris, h.tris.data(), h.tris.size() * sizeof(TriRecord), "triangle records"));
     PT_TRY(upload(rs->shade, h.shade.data(), h.shade.size() * sizeof(ShadeRecord), "shading records"));
     PT_TRY(upload(rs->materials, h.materials.data(), h.materials.size() * sizeof(MaterialRecord), "materials"));
@@ -396,7 +404,11 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
     LoopMode mode;
     mode.scene = views[0].scene;
     mode.shade_multi = n > 1;
-    for (const ViewRecord& v : views) mode.trace_multi = mode.trace_multi || v.scene != mode.scene;
+    for (const ViewRecord& v : views) {
+        mode.trace_multi = mode.trace_multi || v.scene != mode.scene;
+        if (scenes_[v.scene]->host.bvh.width != scenes_[mode.scene]->host.bvh.width)
+            return fail(PT_ERROR_INVALID_ARGUMENT, "the scenes of a batch must share one BVH node width");
+    }
 
     FrameParams& fp = *h_fp_;
     std::memset(&fp, 0, sizeof(fp));
@@ -961,6 +973,7 @@ pt_status Renderer::GetBvhInfo(pt_bvh_info& info) {
     info.light_count = (uint32_t)host0().lights.size();
     info.stack_lds = trace_.lds_stack;
     info.stack_total = trace_.lds_stack + trace_.spill_depth;
+    info.node_width = host0().bvh.width;
     return PT_OK;
 }
 
@@ -985,7 +998,8 @@ pt_status Renderer::GetTuning(pt_tuning& t) const {
 pt_status Renderer::SetTuning(const pt_tuning& t) {
     if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
     if (!trace_config_supported(t.trace_block, t.trace_block >= 256 ? 8 : 16))
-        return fail(PT_ERROR_INVALID_ARGUMENT, "trace_block must be 64, 128 or 256");
+        return fail(PT_ERROR_INVALID_ARGUMENT, "trace_block must be 64, 128, 256 or 512");
+    if (t.bvh_width != 2 && t.bvh_width != 4) return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_width must be 2 or 4");
     if (t.shade_block != 256) return fail(PT_ERROR_INVALID_ARGUMENT, "shade_block must be 256");
     if (t.pool_size < 1024 || t.pool_size > (1u << 26)) return fail(PT_ERROR_INVALID_ARGUMENT, "pool_size must be in [1024, 2^26]");
     if (t.trace_refill < 1 || t.trace_refill > 64) return fail(PT_ERROR_INVALID_ARGUMENT, "trace_refill must be in [1, 64]");

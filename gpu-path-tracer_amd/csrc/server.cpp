@@ -41,6 +41,9 @@ pt_status Renderer::ServerStart(const pt_server_config& cfg_in) {
     for (uint32_t k = 0; k < kMaxScenes && first_scene == kMaxScenes; k++)
         if (scenes_[k]) first_scene = k;
     if (first_scene == kMaxScenes) return fail(PT_ERROR_INVALID_STATE, "no resident scene");
+    for (uint32_t k = 0; k < kMaxScenes; k++)
+        if (scenes_[k] && scenes_[k]->host.bvh.width != scenes_[first_scene]->host.bvh.width)
+            return fail(PT_ERROR_INVALID_STATE, "the resident scenes must share one BVH node width");
     pt_server_config cfg = cfg_in;
     if (cfg.max_requests == 0) cfg.max_requests = kDefaultMaxRequests;
     if (cfg.max_requests > kMaxRequests)

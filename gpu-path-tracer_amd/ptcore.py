@@ -98,13 +98,13 @@ ALIAS_DTYPE = np.dtype([("q", "<f4"), ("p", "<f4"), ("alias", "<u4"), ("value", 
 class BvhInfo(C.Structure):
     _fields_ = [("node_count", C.c_uint32), ("leaf_count", C.c_uint32), ("max_depth", C.c_uint32),
                 ("max_leaf_size", C.c_uint32), ("triangle_count", C.c_uint32), ("light_count", C.c_uint32),
-                ("stack_lds", C.c_uint32), ("stack_total", C.c_uint32)]
+                ("stack_lds", C.c_uint32), ("stack_total", C.c_uint32), ("node_width", C.c_uint32)]
 
 
 class Tuning(C.Structure):
     _fields_ = [("trace_block", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("shade_block", C.c_uint32),
                 ("pool_size", C.c_uint32), ("use_graph", C.c_uint32), ("bvh_max_leaf", C.c_uint32),
-                ("bvh_traversal_cost", C.c_float), ("trace_refill", C.c_uint32)]
+                ("bvh_traversal_cost", C.c_float), ("trace_refill", C.c_uint32), ("bvh_width", C.c_uint32)]
 
 
 _P = C.c_void_p

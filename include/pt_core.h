@@ -210,11 +210,12 @@ typedef struct pt_bvh_info {
     uint32_t light_count;
     uint32_t stack_lds;       /* traversal-stack entries held in LDS per lane */
     uint32_t stack_total;     /* worst-case stack entries (LDS + spill) */
+    uint32_t node_width;      /* 4 or 2 children per node */
 } pt_bvh_info;
 
 /* Kernel launch and build parameters picked by the autotuner. */
 typedef struct pt_tuning {
-    uint32_t trace_block;        /* threads per block of the traversal kernels: 64, 128 or 256 */
+    uint32_t trace_block;        /* threads per block of the traversal kernels: 64, 128, 256 or 512 */
     uint32_t trace_blocks_per_cu;/* 0 = as many as fit */
     uint32_t shade_block;        /* 256 */
     uint32_t pool_size;          /* path slots (queue capacity) */
@@ -222,6 +223,7 @@ typedef struct pt_tuning {
     uint32_t bvh_max_leaf;       /* 1..16 triangles; applied at the next scene upload */
     float bvh_traversal_cost;    /* SAH cost of a node visit relative to a triangle test; next upload */
     uint32_t trace_refill;       /* idle lanes of a wave that trigger fetching new rays (1..64) */
+    uint32_t bvh_width;          /* children per BVH node: 4 (128-byte nodes) or 2 (64-byte); next upload */
 } pt_tuning;
 
 /* ---- lifetime (Renderer ctor/dtor, Renderer.hpp:106-112) ---- */
