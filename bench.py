@@ -85,12 +85,193 @@ def measured_traffic():
         return json.load(f)
 
 
+def _rays(st):
+    return st["extension_rays"] + st["shadow_rays"]
+
+
+def sequential_then_batched(r, items, steps, log_prefix):
+    """Device time of rendering `items` one view at a time (plain rendering,
+    the reference's one-request model) and all in one batch.  Returns
+    (sequential Mrays/s, batched Mrays/s, batched ms, rays, max relative
+    difference of a few views batched vs alone)."""
+    import numpy as np
+    alone, seq_ms, seq_rays = [], 0.0, 0
+    for it in items:                              # warm-up: allocation, graphs
+        r.RenderBatch([it])
+    for k, it in enumerate(items):
+        (hdr,), st = r.RenderBatch([it])
+        seq_ms += st["render_ms"]
+        seq_rays += _rays(st)
+        if k < 6:
+            alone.append(hdr)
+    r.RenderBatch(items)
+    bms, brays = [], 0
+    for _ in range(steps):
+        hdrs, st = r.RenderBatch(items)
+        bms.append(st["render_ms"])
+        brays = _rays(st)
+    bms.sort()
+    b_ms = bms[len(bms) // 2]
+    diff = max(float(np.max(np.abs(h - a) / (np.abs(a) + 1e-3))) for h, a in zip(hdrs, alone))
+    log(f"{log_prefix}: sequential {seq_ms:.1f} ms ({seq_rays} rays), batched {b_ms:.1f} ms ({brays} rays), "
+        f"max rel diff batched vs alone {diff:.2e}")
+    return seq_rays / seq_ms / 1e3, brays / b_ms / 1e3, b_ms, brays, diff
+
+
+def serve_run(r, requests, rate_hz, seed):
+    """Submits `requests` at Poisson arrival times of mean rate `rate_hz`
+    (all at once when rate_hz is None) to a fresh server; returns latencies
+    (ms), server stats and the wall time of the run."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    r.ServerStart()
+    t0 = time.perf_counter()
+    at = 0.0
+    ids = []
+    for sid, st, v in requests:
+        if rate_hz:
+            at += rng.exponential(1.0 / rate_hz)
+            delay = at - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+        ids.append(r.ServerSubmit(sid, st, v, hdr=False))
+    infos = [r.ServerWait(i, 600000) for i in ids]
+    stats = r.ServerStop()
+    bad = [i for i in infos if i["state"] != 2]
+    if bad:
+        raise RuntimeError(f"{len(bad)} requests did not finish: {bad[:2]}")
+    lat = np.array([i["done_ms"] - i["submit_ms"] for i in infos])
+    return lat, stats
+
+
+def serving_summary(r, requests, rates, seed):
+    """Latency percentiles and throughput at each arrival rate, plus the
+    saturated run (all requests at once) and plain sequential rendering of
+    the same requests."""
+    import numpy as np
+    seq_ms, seq_rays = 0.0, 0
+    for it in requests[:2]:
+        r.RenderBatch([it])
+    for it in requests:
+        _, st = r.RenderBatch([it])
+        seq_ms += st["render_ms"]
+        seq_rays += _rays(st)
+    seq = seq_rays / seq_ms / 1e3
+    serve_run(r, requests[:4], None, seed)      # warm-up: pool, arena, graph
+    lat, stats = serve_run(r, requests, None, seed)
+    sat = _rays(stats) / stats["busy_ms"] / 1e3
+    log(f"serving saturated: {len(requests)} requests, {stats['busy_ms']:.0f} ms busy, {sat:.1f} Mrays/s "
+        f"(sequential plain {seq:.1f} Mrays/s, {seq_ms:.0f} ms); latency p50 {np.percentile(lat, 50):.0f} "
+        f"p99 {np.percentile(lat, 99):.0f} ms")
+    out = {"requests": len(requests), "sequential_mrays_s": round(seq, 1), "saturated_mrays_s": round(sat, 1),
+           "speedup_vs_sequential": round(sat / seq, 3), "saturated_p50_ms": round(float(np.percentile(lat, 50)), 1),
+           "saturated_p99_ms": round(float(np.percentile(lat, 99)), 1), "rates": []}
+    sat_rate = len(requests) / (stats["busy_ms"] / 1e3)
+    for frac in rates:
+        rate = frac * sat_rate
+        lat, stats = serve_run(r, requests, rate, seed + 1)
+        mr = _rays(stats) / stats["wall_ms"] / 1e3
+        p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+        log(f"serving at {rate:.2f} req/s ({frac:.0%} of saturation): p50 {p50:.0f} ms, p99 {p99:.0f} ms, "
+            f"{mr:.1f} Mrays/s over the run")
+        out["rates"].append({"rate_hz": round(rate, 3), "load": frac, "p50_ms": round(p50, 1), "p99_ms": round(p99, 1),
+                             "mrays_s": round(mr, 1)})
+    return out
+
+
+def serving_workload(scales, count, seed, resolutions=None, spp=None):
+    """Resident config-1 rooms of the given triangle counts and `count`
+    seeded requests over them."""
+    import numpy as np
+    import scenes
+    rooms = [scenes.cornell_scaled(t) for t in scales]
+    rng = np.random.default_rng(seed)
+    kw = {}
+    if resolutions:
+        kw["resolutions"] = resolutions
+    if spp:
+        kw["spp"] = spp
+    reqs = [scenes.serve_request(rng, len(rooms), **kw) for _ in range(count)]
+    return rooms, reqs
+
+
+def upload_all(r, scene_list):
+    ids = [0]
+    r.InitializeSceneBuffers(scene_list[0])
+    for s in scene_list[1:]:
+        ids.append(r.AddScene(s))
+    return ids
+
+
+def run_batch(a):
+    """Config 3: 32 scenes, 1-4 views each, in one wavefront vs one view at a time."""
+    import ptcore
+    import scenes
+    t0 = time.time()
+    sc, views = scenes.batch_workload()
+    log(f"batch workload: {len(sc)} scenes, {sum(s.triangle_count for s in sc)} triangles, {len(views)} views, "
+        f"built in {time.time() - t0:.1f}s")
+    r = ptcore.Renderer()
+    if a.tuning:
+        import autotune
+        log(f"tuning {autotune.apply(r, a.tuning)}")
+    t0 = time.time()
+    ids = upload_all(r, sc)
+    log(f"scenes uploaded in {time.time() - t0:.1f}s")
+    items = [(ids[i], st, v) for i, st, v in views]
+    seq, bat, b_ms, brays, diff = sequential_then_batched(r, items, a.steps, "batch")
+    out = {"metric": "Mrays/s (extension+shadow), 84 views of 32 scenes in one batched wavefront",
+           "value": round(bat, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps, "warmup": 1,
+           "ms_per_step": round(b_ms, 3), "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "batch: 32 scenes 10k-2M triangles (soups, sphere clusters), 84 views "
+                                  "256x256-1920x1080, 4-64 spp, depth 8, NEE",
+                      "triangles": sum(s.triangle_count for s in sc), "views": len(views), "rays": brays,
+                      "tuning": r.get_tuning()},
+           "sequential_mrays_s": round(seq, 2), "speedup_vs_sequential": round(bat / seq, 3),
+           "max_rel_diff_batched_vs_alone": diff}
+    print(json.dumps(out), flush=True)
+    r.close()
+
+
+def run_serve(a):
+    """Config 4: 8 resident rooms of 1-20M triangles (scaled by --serve-scale),
+    Poisson requests of 512^2-3840x2160 and 4-256 spp at swept rates."""
+    import numpy as np
+    import ptcore
+    sizes = np.exp(np.linspace(np.log(1e6), np.log(20e6), 8)) * a.serve_scale
+    t0 = time.time()
+    rooms, reqs = serving_workload([int(x) for x in sizes], a.serve_requests, 4)
+    log(f"serving workload: {[s.triangle_count for s in rooms]} triangles, {len(reqs)} requests, "
+        f"built in {time.time() - t0:.1f}s")
+    r = ptcore.Renderer()
+    t0 = time.time()
+    upload_all(r, rooms)
+    log(f"resident set uploaded in {time.time() - t0:.1f}s")
+    del rooms
+    res = serving_summary(r, reqs, (0.5, 0.8), 11)
+    top = res["rates"][-1]
+    out = {"metric": "Mrays/s (extension+shadow) of the serving loop; p50/p99 request latency",
+           "value": res["saturated_mrays_s"], "unit": "Mrays/s", "n_gpus": 1, "steps": len(reqs), "warmup": 4,
+           "ms_per_step": None, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"serve: 8 resident rooms of {int(sizes[0])}-{int(sizes[-1])} triangles, "
+                                  f"{len(reqs)} Poisson requests 512x512-3840x2160, 4-256 spp, depth 8"},
+           "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"], "serving": res}
+    print(json.dumps(out), flush=True)
+    r.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=["production", "cornell"], default="production")
+    ap.add_argument("--config", choices=["production", "cornell", "batch", "serve"], default="production")
+    ap.add_argument("--tuning", default=None, help="JSON written by autotune.py")
+    ap.add_argument("--serve-scale", type=float, default=1.0, help="scale of the serving rooms' triangle counts")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving section of the production line")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--trace-block", type=int, default=0)
@@ -106,6 +287,12 @@ def main():
               "pixel range with one reduction of the accumulation buffer); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if a.config == "batch":
+        run_batch(a)
+        return
+    if a.config == "serve":
+        run_serve(a)
+        return
     import ptcore
     import scenes
 
@@ -170,6 +357,17 @@ def main():
                 "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3),
                 "avg_launch_ms": round(avg_launch_ms, 4), "rays_per_launch": round(prof["extension_rays"] / launches, 1)}
 
+    serving = None
+    if not a.no_serving and a.config == "production":
+        # the metric's latency half: a compact serving run over two resident
+        # config-1 rooms, requests up to 1920x1080 and 64 spp
+        rooms, reqs = serving_workload([250_000, 1_000_000], 40, 4, resolutions=scenes.SERVE_RESOLUTIONS[:3],
+                                       spp=scenes.SERVE_SPP[:5])
+        rs = ptcore.Renderer()
+        upload_all(rs, rooms)
+        serving = serving_summary(rs, reqs, (0.7,), 21)
+        rs.close()
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(scene, settings, view, a.cpu_seconds)
@@ -186,6 +384,11 @@ def main():
                    "rays_per_frame": rays // a.steps, "device_ms_per_frame": round(dev_ms / a.steps, 3)},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if serving:
+        out["p50_ms"] = serving["rates"][0]["p50_ms"]
+        out["p99_ms"] = serving["rates"][0]["p99_ms"]
+        out["serving"] = dict(serving, workload="2 resident config-1 rooms (250k, 1M triangles), 40 Poisson "
+                                                 "requests 512x512-1920x1080, 4-64 spp, depth 8")
     print(json.dumps(out), flush=True)
     r.close()
 

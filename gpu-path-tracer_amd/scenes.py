@@ -105,8 +105,35 @@ def _quad(p0, p1, p2, p3, normal):
     return P, N, T
 
 
-def _box(lo, hi):
-    """12 triangles of an axis-aligned box, normals pointing out."""
+def _grid_quad(p0, p1, p2, p3, normal, k):
+    """The quad (p0, p1, p2, p3) of _quad cut into k x k cells, 2 k^2 triangles
+    with the same orientation (cell (i, j) spans p0 + (i, j)/k of the edges
+    p1 - p0 and p3 - p0; the quads here are parallelograms)."""
+    if k == 1:
+        return _quad(p0, p1, p2, p3, normal)
+    a = np.asarray(p0, np.float64)
+    e1 = np.asarray(p1, np.float64) - a
+    e2 = np.asarray(p3, np.float64) - a
+    i, j = np.meshgrid(np.arange(k), np.arange(k), indexing="ij")
+    i, j = i.ravel()[:, None], j.ravel()[:, None]
+    c00 = a + e1 * (i / k) + e2 * (j / k)
+    c10 = a + e1 * ((i + 1) / k) + e2 * (j / k)
+    c11 = a + e1 * ((i + 1) / k) + e2 * ((j + 1) / k)
+    c01 = a + e1 * (i / k) + e2 * ((j + 1) / k)
+    P = np.empty((2 * k * k, 3, 3), np.float32)
+    P[0::2, 0], P[0::2, 1], P[0::2, 2] = c00, c10, c11
+    P[1::2, 0], P[1::2, 1], P[1::2, 2] = c00, c11, c01
+    n = np.asarray(normal, np.float32)
+    N = np.broadcast_to(n, P.shape).copy()
+    t = e1 / np.linalg.norm(e1)
+    T = np.zeros((P.shape[0], 3, 4), np.float32)
+    T[..., :3] = t
+    T[..., 3] = 1.0
+    return P, N, T
+
+
+def _box(lo, hi, k=1):
+    """12 k^2 triangles of an axis-aligned box, normals pointing out."""
     x0, y0, z0 = lo
     x1, y1, z1 = hi
     faces = [
@@ -117,14 +144,17 @@ def _box(lo, hi):
         ((x0, y1, z1), (x1, y1, z1), (x1, y1, z0), (x0, y1, z0), (0, 1, 0)),
         ((x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1), (0, -1, 0)),
     ]
-    parts = [_quad(*f) for f in faces]
+    parts = [_grid_quad(*f, k) for f in faces]
     return [np.concatenate([p[i] for p in parts]) for i in range(3)]
 
 
-def cornell_box():
+def cornell_box(tessellation=1):
     """Config 1: box room of 10 wall triangles (floor, ceiling, back, left, right;
     open toward the camera), two axis-aligned boxes (24 triangles) and an
-    emissive ceiling quad (2 triangles, radiance 15).  36 triangles."""
+    emissive ceiling quad (2 triangles, radiance 15).  36 triangles.
+    tessellation k cuts every quad into k x k cells: the same surfaces in
+    36 k^2 triangles (the config-1 generator scaled)."""
+    k = int(tessellation)
     white, red, green = 0, 1, 2
     mats = np.stack([
         material(MODEL_DIFFUSE, color=(0.73, 0.73, 0.73)),
@@ -138,25 +168,30 @@ def cornell_box():
         P.append(parts[0]); N.append(parts[1]); T.append(parts[2])
         M.append(np.full(parts[0].shape[0], mid, np.uint32))
 
-    add(_quad((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1), (0, 1, 0)), white)      # floor
-    add(_quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1), (0, -1, 0)), white)     # ceiling
-    add(_quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1), (0, 0, 1)), white)    # back
-    add(_quad((-1, 0, 1), (-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (1, 0, 0)), red)      # left
-    add(_quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1), (-1, 0, 0)), green)      # right
-    add(_box((0.1, 0.0, -0.1), (0.7, 0.6, 0.5)), white)                               # short box
-    add(_box((-0.7, 0.0, -0.7), (-0.1, 1.2, -0.1)), white)                            # tall box
-    add(_quad((-0.25, 1.999, -0.25), (0.25, 1.999, -0.25), (0.25, 1.999, 0.25), (-0.25, 1.999, 0.25),
-              (0, -1, 0)), 3)                                                          # light
+    add(_grid_quad((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1), (0, 1, 0), k), white)      # floor
+    add(_grid_quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1), (0, -1, 0), k), white)     # ceiling
+    add(_grid_quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1), (0, 0, 1), k), white)    # back
+    add(_grid_quad((-1, 0, 1), (-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (1, 0, 0), k), red)      # left
+    add(_grid_quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1), (-1, 0, 0), k), green)      # right
+    add(_box((0.1, 0.0, -0.1), (0.7, 0.6, 0.5), k), white)                                   # short box
+    add(_box((-0.7, 0.0, -0.7), (-0.1, 1.2, -0.1), k), white)                                # tall box
+    add(_grid_quad((-0.25, 1.999, -0.25), (0.25, 1.999, -0.25), (0.25, 1.999, 0.25), (-0.25, 1.999, 0.25),
+                   (0, -1, 0), k), 3)                                                          # light
     scene = Scene(
-        name="cornell_box_36",
+        name=f"cornell_box_{36 * k * k}",
         positions=np.concatenate(P), normals=np.concatenate(N), tangents=np.concatenate(T),
         material_ids=np.concatenate(M), materials=mats,
         view=View(fov_y=math.radians(40.0), position=(0.0, 1.0, 3.8), rotation=(0.0, 0.0, 0.0, 1.0)),
         settings=dict(width=256, height=256, samples_per_pixel=16, path_depth=8, use_nee=1,
                       rr_start_depth=3, seed=1),
     )
-    assert scene.triangle_count == 36
+    assert scene.triangle_count == 36 * k * k
     return scene
+
+
+def cornell_scaled(triangles):
+    """The config-1 room tessellated to about `triangles` triangles."""
+    return cornell_box(max(1, int(round(math.sqrt(triangles / 36.0)))))
 
 
 _ICO_CACHE = {}
@@ -321,3 +356,121 @@ def shadow_rays(scene, n, seed):
     rays[:, 4:7] = v / dist[:, None]
     rays[:, 7] = dist - 1e-4
     return rays
+
+
+# ---------------------------------------------------------------- batching workload
+
+_C1_COLORS = ((0.73, 0.73, 0.73), (0.65, 0.05, 0.05), (0.12, 0.45, 0.15))
+
+
+def _flat_frames(P):
+    """Per-corner geometric normals and edge tangents of a triangle soup."""
+    e1 = (P[:, 1] - P[:, 0]).astype(np.float64)
+    e2 = (P[:, 2] - P[:, 0]).astype(np.float64)
+    n = np.cross(e1, e2)
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    t = e1 / np.maximum(np.linalg.norm(e1, axis=1, keepdims=True), 1e-30)
+    N = np.repeat(n[:, None, :], 3, axis=1).astype(np.float32)
+    T = np.ones(P.shape[:2] + (4,), np.float32)
+    T[..., :3] = np.repeat(t[:, None, :], 3, axis=1)
+    return N, T
+
+
+def batch_scene(kind, triangles, seed, extent=100.0):
+    """One scene of the batching workload: a random triangle soup ("soup") or a
+    cluster of tessellated spheres ("spheres") of about `triangles` triangles in
+    an extent^3 box, with the config-1 material mix (Lambertian white, red and
+    green in 3:1:1) and one emissive quad of radiance 15 above the box."""
+    rng = np.random.default_rng(seed)
+    if kind == "soup":
+        n = int(triangles)
+        c = rng.uniform(0.0, extent, size=(n, 1, 3))
+        size = 2.5 * extent / n ** (1.0 / 3.0)
+        P = (c + rng.normal(0.0, size, size=(n, 3, 3))).astype(np.float32)
+        N, T = _flat_frames(P)
+        groups = np.arange(n) // 64
+    elif kind == "spheres":
+        sub = 2
+        while 20 * 4 ** (sub + 1) * 4 <= triangles and sub < 7:
+            sub += 1
+        unit = icosphere(sub)
+        per = unit.shape[0]
+        count = max(1, int(round(triangles / per)))
+        centers = rng.uniform(0.0, extent, size=(count, 3))
+        radii = rng.uniform(0.02, 0.08, size=count) * extent
+        P = (unit[None] * radii[:, None, None, None] + centers[:, None, None, :]).reshape(-1, 3, 3).astype(np.float32)
+        N = np.broadcast_to(unit[None], (count,) + unit.shape).reshape(-1, 3, 3).astype(np.float32)
+        Tn = _sphere_tangents(N)
+        T = np.concatenate([Tn, np.ones(Tn.shape[:-1] + (1,))], axis=-1).astype(np.float32)
+        groups = np.arange(P.shape[0]) // per
+    else:
+        raise ValueError(kind)
+    mats = [material(MODEL_DIFFUSE, color=col) for col in _C1_COLORS]
+    mats.append(material(MODEL_EMISSIVE, color=(1.0, 1.0, 1.0), emission=(15.0, 15.0, 15.0)))
+    pick = rng.choice(3, size=int(groups.max()) + 1, p=[0.6, 0.2, 0.2]).astype(np.uint32)
+    mid = pick[groups]
+    top = float(P[..., 1].max()) + 0.1 * extent
+    lo, hi = 0.2 * extent, 0.8 * extent
+    lq = _quad((lo, top, lo), (hi, top, lo), (hi, top, hi), (lo, top, hi), (0, -1, 0))
+    positions = np.concatenate([P, lq[0]]).astype(np.float32)
+    normals = np.concatenate([N, lq[1]]).astype(np.float32)
+    tangents = np.concatenate([T, lq[2]]).astype(np.float32)
+    material_ids = np.concatenate([mid, np.full(2, 3, np.uint32)]).astype(np.uint32)
+    cam = (0.5 * extent, 0.6 * extent, -0.9 * extent)
+    target = (0.5 * extent, 0.4 * extent, 0.5 * extent)
+    return Scene(
+        name=f"{kind}_{positions.shape[0]}", positions=positions, normals=normals, tangents=tangents,
+        material_ids=material_ids, materials=np.stack(mats),
+        view=View(fov_y=math.radians(45.0), position=cam, rotation=look_at_quaternion(cam, target)),
+        settings=dict(width=512, height=512, samples_per_pixel=16, path_depth=8, use_nee=1, rr_start_depth=3, seed=1),
+    )
+
+
+BATCH_RESOLUTIONS = ((256, 256), (512, 512), (640, 480), (1280, 720), (1920, 1080))
+BATCH_SPP = (4, 9, 16, 36, 64)
+
+
+def batch_workload(count=32, seed=3, tri_range=(10_000, 2_000_000), extent=100.0):
+    """Config 3: `count` scenes with log-uniform triangle counts in tri_range,
+    soups and sphere clusters alternating, each seen from 1-4 cameras at
+    resolutions 256^2..1920x1080 and 4..64 spp.  Returns (scenes, views) with
+    views = [(scene index, settings, View)]."""
+    rng = np.random.default_rng(seed)
+    tris = np.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1]), count)).astype(np.int64)
+    out_scenes, views = [], []
+    for i, t in enumerate(tris):
+        kind = "soup" if i % 2 == 0 else "spheres"
+        out_scenes.append(batch_scene(kind, int(t), seed * 1000 + i, extent))
+        for _ in range(int(rng.integers(1, 5))):
+            w, h = BATCH_RESOLUTIONS[int(rng.integers(len(BATCH_RESOLUTIONS)))]
+            spp = BATCH_SPP[int(rng.integers(len(BATCH_SPP)))]
+            ang = rng.uniform(0.0, 2.0 * math.pi)
+            pos = (0.5 * extent + 1.4 * extent * math.cos(ang), rng.uniform(0.3, 0.9) * extent,
+                   0.5 * extent + 1.4 * extent * math.sin(ang))
+            target = (0.5 * extent, 0.4 * extent, 0.5 * extent)
+            v = View(fov_y=math.radians(45.0), position=pos, rotation=look_at_quaternion(pos, target))
+            st = dict(width=w, height=h, samples_per_pixel=spp, path_depth=8, use_nee=1, rr_start_depth=3,
+                      seed=int(rng.integers(1, 1 << 31)))
+            views.append((i, st, v))
+    return out_scenes, views
+
+
+# ---------------------------------------------------------------- serving workload
+
+SERVE_RESOLUTIONS = ((512, 512), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160))
+SERVE_SPP = (4, 9, 16, 36, 64, 121, 256)
+
+
+def serve_request(rng, scene_count, resolutions=SERVE_RESOLUTIONS, spp=SERVE_SPP, depth=8):
+    """One request of config 4: a random resident scene, a camera inside the
+    config-1 room, a resolution and an spp (square counts: the reference rounds
+    spp down to a square)."""
+    sid = int(rng.integers(scene_count))
+    w, h = resolutions[int(rng.integers(len(resolutions)))]
+    s = spp[int(rng.integers(len(spp)))]
+    pos = (rng.uniform(-0.6, 0.6), rng.uniform(0.6, 1.6), rng.uniform(1.5, 3.8))
+    target = (rng.uniform(-0.5, 0.5), rng.uniform(0.3, 1.2), rng.uniform(-1.0, 0.0))
+    v = View(fov_y=math.radians(40.0), position=pos, rotation=look_at_quaternion(pos, target))
+    st = dict(width=w, height=h, samples_per_pixel=s, path_depth=depth, use_nee=1, rr_start_depth=3,
+              seed=int(rng.integers(1, 1 << 31)))
+    return sid, st, v
