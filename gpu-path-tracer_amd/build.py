@@ -17,7 +17,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["kernels.hip", "bvh_build.cpp", "scene.cpp", "renderer.cpp", "server.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "guide.hip", "bvh_build.cpp", "scene.cpp", "renderer.cpp", "server.cpp", "guide_host.cpp",
+           "capi.cpp"]
 
 # -ffp-contract=off: no FMA contraction, so float expressions round the same
 # way on the CPU and the GPU.  The packed-fp32 feature is switched off for all

@@ -158,6 +158,50 @@ pt_status pt_server_stop(pt_renderer* r, pt_server_stats* stats) {
     return guarded(r, [&](pt::Renderer& x) { return x.ServerStop(stats); });
 }
 
+pt_status pt_render_guided(pt_renderer* r, const pt_render_settings* settings, const pt_camera* camera,
+                           const pt_motion_state* motion_state, const pt_guide_settings* guide,
+                           pt_guide_stats* guide_stats, pt_frame_stats* stats) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!settings || !camera || !motion_state || !guide) return x.fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
+        return x.RenderGuided(*settings, *camera, *motion_state, *guide, guide_stats, stats);
+    });
+}
+
+pt_status pt_guide_reset(pt_renderer* r, uint64_t seed) {
+    return guarded(r, [&](pt::Renderer& x) { return x.GuideReset(seed); });
+}
+
+pt_status pt_guide_get_weights(pt_renderer* r, float* weights, uint32_t count) {
+    return guarded(r, [&](pt::Renderer& x) { return x.GuideGetWeights(weights, count); });
+}
+
+pt_status pt_guide_set_weights(pt_renderer* r, const float* weights, uint32_t count) {
+    return guarded(r, [&](pt::Renderer& x) { return x.GuideSetWeights(weights, count); });
+}
+
+pt_status pt_guide_set_box(pt_renderer* r, const float lo[3], const float hi[3]) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!lo || !hi) return x.fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
+        return x.GuideSetBox(lo, hi);
+    });
+}
+
+pt_status pt_guide_forward(pt_renderer* r, const float* inputs, uint32_t count, float* outputs) {
+    return guarded(r, [&](pt::Renderer& x) { return x.GuideForward(inputs, count, outputs); });
+}
+
+pt_status pt_guide_gradient(pt_renderer* r, const pt_guide_record* records, uint32_t count, float weight_max,
+                            float* gradient, float* loss) {
+    return guarded(r, [&](pt::Renderer& x) { return x.GuideGradient(records, count, weight_max, gradient, loss); });
+}
+
+pt_status pt_guide_train(pt_renderer* r, const pt_guide_record* records, uint32_t count, uint32_t steps,
+                         uint32_t batch, float learning_rate, float weight_max, float* losses) {
+    return guarded(r, [&](pt::Renderer& x) {
+        return x.GuideTrain(records, count, steps, batch, learning_rate, weight_max, losses);
+    });
+}
+
 pt_status pt_trace_closest(pt_renderer* r, const pt_ray* rays, uint32_t count, pt_hit* hits) {
     return guarded(r, [&](pt::Renderer& x) { return x.TraceBatch(rays, count, false, false, hits, nullptr, nullptr, nullptr); });
 }

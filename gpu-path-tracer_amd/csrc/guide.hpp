@@ -1,0 +1,199 @@
+// Learned path guiding: a small MLP maps a shading point (position,
+// normal, outgoing direction) to a mixture of 8 von Mises-Fisher lobes over
+// incident directions.  The renderer samples it with one-sample MIS against the
+// BSDF and trains it online from the radiance its own paths carry.
+//
+// The reference has no path guiding; its BSDF sampling routines
+// (MaterialEval.cu:270-309 diffuse, :720-836 GGX) are what the guide is
+// combined with.  Published methods followed here:
+//  * mixture-of-vMF output of an MLP and its KL-divergence training loss:
+//    Dong et al. 2023, "Neural Parametric Mixtures for Path Guiding";
+//    the loss gradient -(F/p) grad log q of Mueller et al. 2019, "Neural
+//    Importance Sampling";
+//  * vMF sampling in the numerically stable form of W. Jakob 2012,
+//    "Numerically stable sampling of the von Mises Fisher distribution on S^2";
+//  * Adam: Kingma and Ba 2015.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pt_device.hpp"
+
+namespace pt {
+
+constexpr int kGuideDim = 64;          // width of every layer (inputs padded)
+constexpr int kGuideLayers = 4;        // 3 hidden layers + output layer
+constexpr int kGuideLobes = 8;
+constexpr int kGuideOut = 5 * kGuideLobes;   // per lobe: mu (3, unnormalised), log kappa, logit
+constexpr int kGuideOctaves = 6;       // sin/cos(2^k pi x), k < 6, per position axis
+constexpr int kGuideFeatures = 3 + 3 * 2 * kGuideOctaves + 3 + 3;   // 45
+constexpr int kGuideLayerParams = kGuideDim * kGuideDim + kGuideDim; // W [out][in] then bias
+constexpr int kGuideParams = kGuideLayers * kGuideLayerParams;       // 16,640
+constexpr float kGuideLogKappaMin = -2.0f, kGuideLogKappaMax = 7.0f;
+
+// Mixture parameters of one shading point: 8 lobes of (mu xyz unit, kappa, pi).
+struct GuideMixture {
+    float mu[kGuideLobes][3];
+    float kappa[kGuideLobes];
+    float pi[kGuideLobes];
+};
+
+// Training record written at a scatter vertex: the vertex, the sampled
+// direction, the path throughput after the vertex, and the incident radiance
+// gathered later along that direction (rgb, summed by the record chain walk).
+struct alignas(16) GuideRecord {
+    float p[3];
+    uint32_t prev;          // previous record of the same path, or kInvalid
+    float n[3];
+    float pdf;              // density the direction was sampled with
+    float wo[3];
+    float cos_i;            // |n . wi|
+    float wi[3];
+    float pad;
+    float t_after[3];       // path throughput after this vertex
+    float pad2;
+    float li[3];            // incident radiance estimate along wi
+    float pad3;
+};
+static_assert(sizeof(GuideRecord) == 96, "guide record is 96 B");
+
+// Scene box used to normalise positions, and the MIS settings.
+struct GuideConfig {
+    float lo[3];
+    float guide_fraction;   // probability of sampling the guide (alpha)
+    float inv_extent[3];
+    uint32_t enabled;       // 0: plain BSDF sampling
+};
+
+// ---------------------------------------------------------------- features
+
+// Input features of a shading point: normalised position, its sin/cos
+// frequency encoding, the normal and the outgoing direction (45 values; the
+// rest of the 64 inputs are zero).
+PT_HD void guide_features(const GuideConfig& g, V3 p, V3 n, V3 wo, float f[kGuideFeatures]) {
+    const float q[3] = {(p.x - g.lo[0]) * g.inv_extent[0], (p.y - g.lo[1]) * g.inv_extent[1],
+                        (p.z - g.lo[2]) * g.inv_extent[2]};
+    int k = 0;
+    for (int a = 0; a < 3; a++) f[k++] = q[a];
+    for (int a = 0; a < 3; a++) {
+        float scale = kPi;
+        for (int o = 0; o < kGuideOctaves; o++) {
+            f[k++] = sinf(scale * q[a]);
+            f[k++] = cosf(scale * q[a]);
+            scale *= 2.0f;
+        }
+    }
+    f[k++] = n.x;
+    f[k++] = n.y;
+    f[k++] = n.z;
+    f[k++] = wo.x;
+    f[k++] = wo.y;
+    f[k++] = wo.z;
+}
+
+// ---------------------------------------------------------------- vMF mixture
+
+// log of the vMF density kappa/(2 pi (1 - e^{-2 kappa})) e^{kappa (mu.w - 1)}.
+PT_HD float vmf_log_pdf(float kappa, float mu_dot_w) {
+    return logf(kappa) - logf(2.0f * kPi) - log1pf(-expf(-2.0f * kappa)) + kappa * (mu_dot_w - 1.0f);
+}
+
+// Mixture parameters from the 40 raw network outputs.
+PT_HD void guide_mixture(const float o[kGuideOut], GuideMixture& m) {
+    float mx = -1e30f;
+    for (int j = 0; j < kGuideLobes; j++) mx = fmaxf(mx, o[5 * j + 4]);
+    float sum = 0.0f;
+    for (int j = 0; j < kGuideLobes; j++) {
+        const float ux = o[5 * j], uy = o[5 * j + 1], uz = o[5 * j + 2];
+        const float len = sqrtf(ux * ux + uy * uy + uz * uz);
+        const float inv = 1.0f / fmaxf(len, 1e-12f);
+        m.mu[j][0] = len > 1e-12f ? ux * inv : 0.0f;
+        m.mu[j][1] = len > 1e-12f ? uy * inv : 0.0f;
+        m.mu[j][2] = len > 1e-12f ? uz * inv : 1.0f;
+        m.kappa[j] = expf(clampf(o[5 * j + 3], kGuideLogKappaMin, kGuideLogKappaMax));
+        m.pi[j] = expf(o[5 * j + 4] - mx);
+        sum += m.pi[j];
+    }
+    const float inv_sum = 1.0f / sum;
+    for (int j = 0; j < kGuideLobes; j++) m.pi[j] *= inv_sum;
+}
+
+PT_HD float guide_pdf(const GuideMixture& m, V3 w) {
+    float q = 0.0f;
+    for (int j = 0; j < kGuideLobes; j++) {
+        const float d = m.mu[j][0] * w.x + m.mu[j][1] * w.y + m.mu[j][2] * w.z;
+        q += m.pi[j] * expf(vmf_log_pdf(m.kappa[j], d));
+    }
+    return q;
+}
+
+// Orthonormal basis around a unit vector (Duff et al. 2017, "Building an
+// Orthonormal Basis, Revisited").
+PT_HD void guide_basis(V3 n, V3& t, V3& b) {
+    const float s = copysignf(1.0f, n.z);
+    const float a = -1.0f / (s + n.z);
+    const float c = n.x * n.y * a;
+    t = v3(1.0f + s * n.x * n.x * a, s * c, -s * n.x);
+    b = v3(c, s + n.y * n.y * a, -n.y);
+}
+
+// Direction from the mixture: u0 picks the lobe, (u1, u2) the direction
+// within it (Jakob 2012: cos = 1 + log(u + (1-u) e^{-2 kappa}) / kappa).
+PT_HD V3 guide_sample(const GuideMixture& m, float u0, float u1, float u2) {
+    int j = 0;
+    float acc = m.pi[0];
+    while (j < kGuideLobes - 1 && u0 >= acc) {
+        j++;
+        acc += m.pi[j];
+    }
+    const float k = m.kappa[j];
+    const float u = fmaxf(u1, 1e-7f);
+    const float c = clampf(1.0f + logf(u + (1.0f - u) * expf(-2.0f * k)) / k, -1.0f, 1.0f);
+    const float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+    const float phi = 2.0f * kPi * u2;
+    const V3 mu = v3(m.mu[j][0], m.mu[j][1], m.mu[j][2]);
+    V3 t, b;
+    guide_basis(mu, t, b);
+    return normalize(mu * c + t * (s * cosf(phi)) + b * (s * sinf(phi)));
+}
+
+// d(loss)/d(raw outputs) of one record for loss = -weight * log q(wi).
+// Returns log q(wi).
+PT_HD float guide_loss_grad(const float o[kGuideOut], V3 wi, float weight, float g[kGuideOut]) {
+    GuideMixture m;
+    guide_mixture(o, m);
+    float lp[kGuideLobes];
+    float mxl = -1e30f;
+    for (int j = 0; j < kGuideLobes; j++) {
+        const float d = m.mu[j][0] * wi.x + m.mu[j][1] * wi.y + m.mu[j][2] * wi.z;
+        lp[j] = logf(fmaxf(m.pi[j], 1e-30f)) + vmf_log_pdf(m.kappa[j], d);
+        mxl = fmaxf(mxl, lp[j]);
+    }
+    float sum = 0.0f;
+    for (int j = 0; j < kGuideLobes; j++) sum += expf(lp[j] - mxl);
+    const float logq = mxl + logf(sum);
+    for (int j = 0; j < kGuideLobes; j++) {
+        const float r = expf(lp[j] - logq);            // responsibility of lobe j
+        // logits: d log q / d logit_j = r_j - pi_j
+        g[5 * j + 4] = -weight * (r - m.pi[j]);
+        // kappa = exp(s) (clamped): d log vMF / d kappa = 1/kappa - 2e^{-2k}/(1-e^{-2k}) + mu.w - 1
+        const float s = o[5 * j + 3];
+        const float k = m.kappa[j];
+        const float d = m.mu[j][0] * wi.x + m.mu[j][1] * wi.y + m.mu[j][2] * wi.z;
+        const float e = expf(-2.0f * k);
+        const float dk = 1.0f / k - 2.0f * e / (1.0f - e) + d - 1.0f;
+        const bool free_k = s > kGuideLogKappaMin && s < kGuideLogKappaMax;
+        g[5 * j + 3] = free_k ? -weight * r * dk * k : 0.0f;
+        // mu = u/|u|: d log vMF / d u = kappa (I - mu mu^T) wi / |u|
+        const float ux = o[5 * j], uy = o[5 * j + 1], uz = o[5 * j + 2];
+        const float len = sqrtf(ux * ux + uy * uy + uz * uz);
+        const float sc = len > 1e-12f ? -weight * r * k / len : 0.0f;
+        g[5 * j + 0] = sc * (wi.x - m.mu[j][0] * d);
+        g[5 * j + 1] = sc * (wi.y - m.mu[j][1] * d);
+        g[5 * j + 2] = sc * (wi.z - m.mu[j][2] * d);
+    }
+    return logq;
+}
+
+}  // namespace pt

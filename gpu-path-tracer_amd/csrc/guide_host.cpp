@@ -1,0 +1,352 @@
+// Host side of learned path guiding: network state, test hooks and the
+// multi-pass guided render with online training between passes.
+#include <cmath>
+#include <cstring>
+#include <random>
+
+#include "renderer.hpp"
+
+namespace pt {
+
+static_assert(sizeof(pt_guide_record) == sizeof(GuideRecord), "pt_guide_record mirrors GuideRecord");
+static_assert(PT_GUIDE_PARAMS == kGuideParams, "parameter count");
+static_assert(PT_GUIDE_INPUTS == kGuideDim && PT_GUIDE_OUTPUTS == kGuideDim, "layer width");
+
+namespace {
+constexpr uint32_t kDefaultRecordCapacity = 1u << 22;
+}
+
+// Guide box from the bounds of scene 0 (the union of the root's child boxes),
+// unless the caller set one.
+void Renderer::guide_box_from_scene() {
+    if (g_box_set_ || !has_scene()) return;
+    const Bvh4Node& root = host0().bvh.nodes[0];
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int c = 0; c < 4; c++) {
+        if (root.child[c] == kInvalid) continue;
+        lo[0] = std::min(lo[0], root.lox[c]);
+        lo[1] = std::min(lo[1], root.loy[c]);
+        lo[2] = std::min(lo[2], root.loz[c]);
+        hi[0] = std::max(hi[0], root.hix[c]);
+        hi[1] = std::max(hi[1], root.hiy[c]);
+        hi[2] = std::max(hi[2], root.hiz[c]);
+    }
+    GuideConfig c = g_cfg_;
+    for (int a = 0; a < 3; a++) {
+        if (!(hi[a] > lo[a])) {
+            lo[a] = 0.0f;
+            hi[a] = 1.0f;
+        }
+        c.lo[a] = lo[a];
+        c.inv_extent[a] = 1.0f / (hi[a] - lo[a]);
+    }
+    if (std::memcmp(&c, &g_cfg_, sizeof(c)) != 0) {
+        g_cfg_ = c;
+        invalidate_graph();
+    }
+}
+
+GuideDev Renderer::guide_dev() const {
+    GuideDev g{};
+    g.w32 = g_w32_.as<float>();
+    g.w = g_w_.as<uint16_t>();
+    g.wt = g_wt_.as<uint16_t>();
+    g.grad = g_grad_.as<float>();
+    g.m = g_m_.as<float>();
+    g.v = g_v_.as<float>();
+    g.mix = g_mix_.as<GuideMixture>();
+    g.recs = g_recs_.as<GuideRecord>();
+    g.rec_count = g_rec_count_.as<unsigned int>();
+    g.rec_capacity = g_rec_capacity_;
+    g.head[0] = g_head_[0].as<uint32_t>();
+    g.head[1] = g_head_[1].as<uint32_t>();
+    g.sh_head = g_sh_head_.as<uint32_t>();
+    g.cfg = g_cfg_;
+    return g;
+}
+
+pt_status Renderer::ensure_guide() {
+    if (g_w32_.bytes() == 0) {
+        PT_TRY_HIP(g_w32_.allocate(sizeof(float) * kGuideParams), "guide weights");
+        PT_TRY_HIP(g_w_.allocate(sizeof(uint16_t) * kGuideLayers * kGuideDim * kGuideDim), "guide weights");
+        PT_TRY_HIP(g_wt_.allocate(sizeof(uint16_t) * kGuideLayers * kGuideDim * kGuideDim), "guide weights");
+        PT_TRY_HIP(g_grad_.allocate(sizeof(float) * kGuideParams), "guide gradient");
+        PT_TRY_HIP(g_m_.allocate(sizeof(float) * kGuideParams), "guide Adam state");
+        PT_TRY_HIP(g_v_.allocate(sizeof(float) * kGuideParams), "guide Adam state");
+        if (!g_box_set_) {
+            for (int a = 0; a < 3; a++) {
+                g_cfg_.lo[a] = 0.0f;
+                g_cfg_.inv_extent[a] = 1.0f;
+            }
+        }
+    }
+    guide_box_from_scene();
+    if (!g_init_) PT_TRY(GuideReset(1));
+    return PT_OK;
+}
+
+pt_status Renderer::ensure_guide_pool(uint32_t cap) {
+    if (g_pool_ != pool_capacity_ || g_mix_.bytes() == 0) {
+        invalidate_graph();
+        g_mix_.release();
+        g_head_[0].release();
+        g_head_[1].release();
+        g_sh_head_.release();
+        PT_TRY_HIP(g_mix_.allocate(sizeof(GuideMixture) * (size_t)pool_capacity_), "guide mixtures");
+        PT_TRY_HIP(g_head_[0].allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");
+        PT_TRY_HIP(g_head_[1].allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");
+        PT_TRY_HIP(g_sh_head_.allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");
+        g_pool_ = pool_capacity_;
+    }
+    if (cap != g_rec_capacity_ || g_recs_.bytes() == 0) {
+        invalidate_graph();
+        g_recs_.release();
+        PT_TRY_HIP(g_recs_.allocate(sizeof(GuideRecord) * (size_t)cap), "guide records");
+        g_rec_capacity_ = cap;
+    }
+    if (g_rec_count_.bytes() == 0) PT_TRY_HIP(g_rec_count_.allocate(64), "guide record count");
+    return PT_OK;
+}
+
+// He-uniform hidden layers (bound sqrt(6 / fan_in)), a 10x smaller output
+// layer, zero biases except log kappa = 1 (broad initial lobes).
+pt_status Renderer::GuideReset(uint64_t seed) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    g_init_ = true;   // ensure_guide below must not recurse into a reset
+    PT_TRY(ensure_guide());
+    std::vector<float> w(kGuideParams, 0.0f);
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> uni(-1.0, 1.0);
+    for (int l = 0; l < kGuideLayers; l++) {
+        const double fan_in = l == 0 ? (double)kGuideFeatures : (double)kGuideDim;
+        double bound = std::sqrt(6.0 / fan_in);
+        if (l == kGuideLayers - 1) bound *= 0.1;
+        float* W = w.data() + (size_t)l * kGuideLayerParams;
+        for (int o = 0; o < kGuideDim; o++)
+            for (int i = 0; i < kGuideDim; i++)
+                W[o * kGuideDim + i] = (l == 0 && i >= kGuideFeatures) ? 0.0f : (float)(bound * uni(rng));
+        if (l == kGuideLayers - 1)
+            for (int j = 0; j < kGuideLobes; j++) W[kGuideDim * kGuideDim + 5 * j + 3] = 1.0f;
+    }
+    return GuideSetWeights(w.data(), kGuideParams);
+}
+
+pt_status Renderer::GuideGetWeights(float* w, uint32_t n) {
+    if (!w || n < (uint32_t)kGuideParams) return fail(PT_ERROR_INVALID_ARGUMENT, "weights buffer too small");
+    PT_TRY(ensure_guide());
+    PT_TRY_HIP(hipMemcpy(w, g_w32_.as<void>(), sizeof(float) * kGuideParams, hipMemcpyDeviceToHost), "guide weights");
+    return PT_OK;
+}
+
+pt_status Renderer::GuideSetWeights(const float* w, uint32_t n) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    if (!w || n != (uint32_t)kGuideParams)
+        return fail(PT_ERROR_INVALID_ARGUMENT, "expected " + std::to_string(kGuideParams) + " weights");
+    for (uint32_t k = 0; k < n; k++)
+        if (!std::isfinite(w[k])) return fail(PT_ERROR_INVALID_ARGUMENT, "non-finite weight");
+    PT_TRY(ensure_guide());
+    PT_TRY_HIP(hipMemcpyAsync(g_w32_.as<void>(), w, sizeof(float) * kGuideParams, hipMemcpyHostToDevice, stream_),
+               "guide weights");
+    PT_TRY_HIP(hipMemsetAsync(g_grad_.as<void>(), 0, sizeof(float) * kGuideParams, stream_), "guide gradient");
+    PT_TRY_HIP(hipMemsetAsync(g_m_.as<void>(), 0, sizeof(float) * kGuideParams, stream_), "guide Adam state");
+    PT_TRY_HIP(hipMemsetAsync(g_v_.as<void>(), 0, sizeof(float) * kGuideParams, stream_), "guide Adam state");
+    PT_TRY_HIP(launch_guide_sync(stream_, guide_dev()), "guide weights");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "guide weights");
+    g_step_ = 0;
+    g_init_ = true;
+    return PT_OK;
+}
+
+pt_status Renderer::GuideSetBox(const float lo[3], const float hi[3]) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    for (int a = 0; a < 3; a++) {
+        if (!(hi[a] > lo[a]) || !std::isfinite(lo[a]) || !std::isfinite(hi[a]))
+            return fail(PT_ERROR_INVALID_ARGUMENT, "the guide box needs lo < hi on every axis");
+        g_cfg_.lo[a] = lo[a];
+        g_cfg_.inv_extent[a] = 1.0f / (hi[a] - lo[a]);
+    }
+    g_box_set_ = true;
+    invalidate_graph();
+    return PT_OK;
+}
+
+pt_status Renderer::GuideForward(const float* in, uint32_t n, float* out) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    if (n == 0) return PT_OK;
+    if (!in || !out) return fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
+    PT_TRY(ensure_guide());
+    const size_t bytes = (size_t)n * kGuideDim * sizeof(float);
+    PT_TRY_HIP(d_hook_in_.allocate(bytes), "hook input");
+    PT_TRY_HIP(d_hook_out_.allocate(bytes), "hook output");
+    PT_TRY_HIP(hipMemcpyAsync(d_hook_in_.as<void>(), in, bytes, hipMemcpyHostToDevice, stream_), "hook input");
+    PT_TRY_HIP(launch_guide_forward(stream_, guide_dev(), d_hook_in_.as<float>(), n, d_hook_out_.as<float>()), "guide forward");
+    PT_TRY_HIP(hipMemcpyAsync(out, d_hook_out_.as<void>(), bytes, hipMemcpyDeviceToHost, stream_), "hook output");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "hook sync");
+    return PT_OK;
+}
+
+pt_status Renderer::GuideGradient(const pt_guide_record* recs, uint32_t n, float w_max, float* grad, float* loss) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    if (!recs || !grad || !loss) return fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
+    PT_TRY(ensure_guide());
+    PT_TRY_HIP(g_loss_.allocate(sizeof(float) * 16), "guide loss");
+    PT_TRY_HIP(d_hook_in_.allocate(sizeof(GuideRecord) * (size_t)std::max(n, 1u)), "hook input");
+    PT_TRY_HIP(hipMemcpyAsync(d_hook_in_.as<void>(), recs, sizeof(GuideRecord) * (size_t)n, hipMemcpyHostToDevice, stream_), "hook input");
+    PT_TRY_HIP(hipMemsetAsync(g_grad_.as<void>(), 0, sizeof(float) * kGuideParams, stream_), "guide gradient");
+    PT_TRY_HIP(hipMemsetAsync(g_loss_.as<void>(), 0, sizeof(float), stream_), "guide loss");
+    PT_TRY_HIP(launch_guide_train(stream_, guide_dev(), d_hook_in_.as<GuideRecord>(), n, n, 0, w_max, g_loss_.as<float>()),
+               "guide gradient");
+    PT_TRY_HIP(hipMemcpyAsync(grad, g_grad_.as<void>(), sizeof(float) * kGuideParams, hipMemcpyDeviceToHost, stream_), "guide gradient");
+    PT_TRY_HIP(hipMemcpyAsync(loss, g_loss_.as<void>(), sizeof(float), hipMemcpyDeviceToHost, stream_), "guide loss");
+    PT_TRY_HIP(hipMemsetAsync(g_grad_.as<void>(), 0, sizeof(float) * kGuideParams, stream_), "guide gradient");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "hook sync");
+    return PT_OK;
+}
+
+pt_status Renderer::GuideTrain(const pt_guide_record* recs, uint32_t n, uint32_t steps, uint32_t batch, float lr,
+                               float w_max, float* losses) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    if (!recs || n == 0) return fail(PT_ERROR_INVALID_ARGUMENT, "no records");
+    if (batch == 0 || batch % 64 != 0) return fail(PT_ERROR_INVALID_ARGUMENT, "batch must be a positive multiple of 64");
+    if (!(lr > 0.0f)) return fail(PT_ERROR_INVALID_ARGUMENT, "learning rate must be positive");
+    PT_TRY(ensure_guide());
+    PT_TRY_HIP(g_loss_.allocate(sizeof(float) * std::max<size_t>(steps, 16)), "guide loss");
+    PT_TRY_HIP(d_hook_in_.allocate(sizeof(GuideRecord) * (size_t)n), "hook input");
+    PT_TRY_HIP(hipMemcpyAsync(d_hook_in_.as<void>(), recs, sizeof(GuideRecord) * (size_t)n, hipMemcpyHostToDevice, stream_), "hook input");
+    PT_TRY_HIP(hipMemsetAsync(g_loss_.as<void>(), 0, sizeof(float) * std::max<size_t>(steps, 16), stream_), "guide loss");
+    const GuideDev g = guide_dev();
+    for (uint32_t s = 0; s < steps; s++) {
+        g_step_++;
+        const uint32_t seed = (g_step_ * 2654435761u) | 1u;
+        PT_TRY_HIP(launch_guide_train(stream_, g, d_hook_in_.as<GuideRecord>(), n, batch, seed, w_max, g_loss_.as<float>() + s),
+                   "guide training");
+        PT_TRY_HIP(launch_guide_adam(stream_, g, lr, g_step_, 1.0f / (float)batch), "guide Adam step");
+    }
+    if (losses && steps) {
+        PT_TRY_HIP(hipMemcpyAsync(losses, g_loss_.as<void>(), sizeof(float) * steps, hipMemcpyDeviceToHost, stream_), "guide loss");
+    }
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "guide training");
+    if (losses)
+        for (uint32_t s = 0; s < steps; s++) losses[s] /= (float)batch;
+    return PT_OK;
+}
+
+// ---------------------------------------------------------------- guided frame
+
+pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& camera, const pt_motion_state& state,
+                                 const pt_guide_settings& gs, pt_guide_stats* gstats, pt_frame_stats* stats) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (gs.passes < 1) return fail(PT_ERROR_INVALID_ARGUMENT, "at least one pass");
+    if (!(gs.guide_fraction >= 0.0f && gs.guide_fraction <= 1.0f))
+        return fail(PT_ERROR_INVALID_ARGUMENT, "guide_fraction must be in [0, 1]");
+    const bool learn = gs.guide_fraction > 0.0f;
+    if (learn && gs.train_steps > 0 && (gs.batch == 0 || gs.batch % 64 != 0))
+        return fail(PT_ERROR_INVALID_ARGUMENT, "batch must be a positive multiple of 64");
+    if (learn && gs.train_steps > 0 && !(gs.learning_rate > 0.0f))
+        return fail(PT_ERROR_INVALID_ARGUMENT, "learning rate must be positive");
+
+    // one view per pass, the samples per pixel split evenly
+    pt_view v{};
+    v.scene = 0;
+    v.settings = s;
+    v.settings.samples_per_pixel = std::max(1u, s.samples_per_pixel / gs.passes);
+    v.camera = camera;
+    v.state = state;
+    std::vector<ViewRecord> views(1);
+    uint32_t depth = 0;
+    PT_TRY(make_view(v, views[0], depth));
+    const uint32_t spp_pass = views[0].spp;
+    views[0].inv_spp = 1.0f / (float)(spp_pass * gs.passes);
+    set_frame_size(views[0].width, views[0].height);
+    last_spp_ = spp_pass * gs.passes;
+    hdr_valid_ = false;
+
+    PT_TRY(ensure_pool());
+    if (learn) {
+        guide_box_from_scene();
+        PT_TRY(ensure_guide());
+        if (gs.reset) PT_TRY(GuideReset(gs.seed));
+        PT_TRY(ensure_guide_pool(gs.record_capacity ? gs.record_capacity : kDefaultRecordCapacity));
+        PT_TRY_HIP(g_loss_.allocate(sizeof(float) * std::max<size_t>(gs.train_steps, 16)), "guide loss");
+    }
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    PT_TRY_HIP(hipEventCreate(&t0), "event");
+    PT_TRY_HIP(hipEventCreate(&t1), "event");
+    struct EventGuard {
+        hipEvent_t a, b;
+        ~EventGuard() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } guard{t0, t1};
+
+    pt_guide_stats gst{};
+    pt_frame_stats total{};
+    bool trained = learn && g_step_ > 0 && !gs.reset;
+    float first_loss = 0.0f, last_loss = 0.0f;
+    for (uint32_t pass = 0; pass < gs.passes; pass++) {
+        views[0].seed = s.seed + 0x9E3779B97F4A7C15ull * pass;
+        RunOptions opt;
+        opt.clear = pass == 0;
+        opt.guided = learn;
+        opt.guide_fraction = trained ? gs.guide_fraction : 0.0f;
+        if (learn) PT_TRY_HIP(launch_zero_u32(stream_, g_rec_count_.as<unsigned int>(), 1), "guide record count");
+        pt_frame_stats ps{};
+        PT_TRY(run_views(views, depth, PT_FRAME_DEFAULT, &ps, nullptr, opt));
+        total.render_ms += ps.render_ms;
+        total.extension_rays += ps.extension_rays;
+        total.shadow_rays += ps.shadow_rays;
+        total.samples += ps.samples;
+        total.iterations += ps.iterations;
+        gst.render_ms += ps.render_ms;
+        if (!learn || pass + 1 == gs.passes || gs.train_steps == 0) continue;
+
+        // train on this pass's records
+        uint32_t count = 0;
+        PT_TRY_HIP(hipMemcpy(&count, g_rec_count_.as<void>(), sizeof(uint32_t), hipMemcpyDeviceToHost), "guide record count");
+        count = std::min(count, g_rec_capacity_);
+        gst.records = count;
+        if (count == 0) continue;
+        const GuideDev g = guide_dev();
+        PT_TRY_HIP(hipMemsetAsync(g_loss_.as<void>(), 0, sizeof(float) * gs.train_steps, stream_), "guide loss");
+        PT_TRY_HIP(hipEventRecord(t0, stream_), "event");
+        for (uint32_t k = 0; k < gs.train_steps; k++) {
+            g_step_++;
+            const uint32_t seed = (uint32_t)(gs.seed * 0x9E3779B1u) ^ (g_step_ * 2654435761u) ^ 1u;
+            PT_TRY_HIP(launch_guide_train(stream_, g, g.recs, count, gs.batch, seed | 1u,
+                                          gs.weight_max > 0.0f ? gs.weight_max : 1e30f, g_loss_.as<float>() + k),
+                       "guide training");
+            PT_TRY_HIP(launch_guide_adam(stream_, g, gs.learning_rate, g_step_, 1.0f / (float)gs.batch), "guide Adam step");
+        }
+        PT_TRY_HIP(hipEventRecord(t1, stream_), "event");
+        std::vector<float> losses(gs.train_steps);
+        PT_TRY_HIP(hipMemcpyAsync(losses.data(), g_loss_.as<void>(), sizeof(float) * gs.train_steps, hipMemcpyDeviceToHost, stream_), "guide loss");
+        PT_TRY_HIP(hipStreamSynchronize(stream_), "guide training");
+        float ms = 0.0f;
+        PT_TRY_HIP(hipEventElapsedTime(&ms, t0, t1), "elapsed");
+        gst.train_ms += ms;
+        total.render_ms += ms;
+        if (gst.steps == 0) first_loss = losses[0] / (float)gs.batch;
+        last_loss = losses[gs.train_steps - 1] / (float)gs.batch;
+        gst.steps += gs.train_steps;
+        trained = true;
+    }
+    gst.passes = gs.passes;
+    gst.loss_first = first_loss;
+    gst.loss_last = last_loss;
+
+    const size_t npix = (size_t)width_ * height_;
+    PT_TRY_HIP(hipMemcpyAsync(h_frame_staging_, d_frame_.as<void>(), npix * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "framebuffer");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "frame sync");
+    {
+        std::scoped_lock lock(frame_mutex_);
+        frame_.assign(h_frame_staging_, h_frame_staging_ + npix);
+    }
+    hdr_valid_ = true;
+    if (stats) *stats = total;
+    if (gstats) *gstats = gst;
+    return PT_OK;
+}
+
+}  // namespace pt

@@ -461,6 +461,23 @@ struct WavefrontClosestSrc {
     }
 };
 
+// Adds the contribution L that a path made at a later vertex to every guide
+// record of the path: record k receives L / (path throughput after vertex k),
+// the radiance that arrived at vertex k along its sampled direction.
+__device__ __forceinline__ void guide_credit(GuideRecord* recs, uint32_t head, V3 L) {
+    uint32_t r = head;
+    for (uint32_t n = 0; r != kInvalid && n <= kMaxDepthLimit; n++) {
+        GuideRecord& g = recs[r];
+        const float4 t = *reinterpret_cast<const float4*>(g.t_after);
+        float4 li = *reinterpret_cast<const float4*>(g.li);
+        li.x += t.x > 0.0f ? L.x / t.x : 0.0f;
+        li.y += t.y > 0.0f ? L.y / t.y : 0.0f;
+        li.z += t.z > 0.0f ? L.z / t.z : 0.0f;
+        *reinterpret_cast<float4*>(g.li) = li;
+        r = g.prev;
+    }
+}
+
 struct WavefrontShadowSrc {
     PathState p;
     __device__ __forceinline__ uint32_t count() const { return p.ctr->shadow; }
@@ -487,6 +504,7 @@ struct WavefrontShadowSrc {
             atomicAdd(acc + 0, L.x);
             atomicAdd(acc + 1, L.y);
             atomicAdd(acc + 2, L.z);
+            if (p.guide.sh_head) guide_credit(p.guide.recs, p.guide.sh_head[i], L);
         }
     }
     __device__ __forceinline__ void flush_counts(uint32_t nodes, uint32_t tris) const {
@@ -867,6 +885,7 @@ __global__ void k_generate(PathState p, int init, int nxt) {
         rd[pos] = f4(d, kMaxT);
         th[pos] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
         me[pos] = make_uint4(pixel, kDeltaBit | (v << 8), (uint32_t)sample, (uint32_t)(sample >> 32));
+        if (p.guide.head[0]) (nxt ? p.guide.head[1] : p.guide.head[0])[pos] = kInvalid;
     }
 }
 
@@ -1004,8 +1023,11 @@ __device__ __forceinline__ void block_reserve2(bool pred0, bool pred1, unsigned 
 // Each wave walks 64 consecutive entries, so a wave shades one material except
 // at the three queue boundaries.  MULTI: paths of several views (per-lane view
 // and scene records); otherwise every path belongs to view 0 of scene `sc1`,
-// whose records are wave-uniform loads.
-template <bool MULTI>
+// whose records are wave-uniform loads.  GUIDED: diffuse and GGX directions by
+// one-sample MIS of the guide mixture (probability cfg.guide_fraction) and the
+// BSDF, each continuing diffuse/GGX vertex writes a training record, and
+// radiance reaching the pixel is credited to the path's records.
+template <bool MULTI, bool GUIDED>
 __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cur) {
     const FrameParams* fp = p.fp;
     Counters* c = p.ctr;
@@ -1024,6 +1046,9 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
         uint4 m = make_uint4(0, 0, 0, 0);
         float cur_ior = 1.0f;
         float scene_bits = 0.0f;
+        uint32_t head = kInvalid;          // newest guide record of the path
+        bool want_rec = false;
+        GuideRecord rec;
         V3 add_L = v3(0.0f);
         bool add = false;
         uint32_t pixel = 0;
@@ -1052,6 +1077,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
             cur_ior = th4.w;
             const float4 hh = p.hit[i];
             const uint32_t prim = __float_as_uint(hh.w);
+            if (GUIDED) head = (cur ? p.guide.head[1] : p.guide.head[0])[i];
             if (q == 3) {
                 // EvaluateEmissiveKernel, MaterialEval.cu:882-912
                 if (!view_nee || delta) {
@@ -1059,6 +1085,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                     const MaterialRecord* mr = sc.materials + mi;
                     add_L = thr * v3(mr->emission[0], mr->emission[1], mr->emission[2]);
                     add = true;
+                    if (GUIDED) guide_credit(p.guide.recs, head, add_L);
                 }
             } else {
                 const float4 o4 = (cur ? p.ray_o[1] : p.ray_o[0])[i];
@@ -1072,6 +1099,18 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                 const V3 nts = xyz(m4);
                 const Frame3 fr = shading_frame(h, nts);
                 Draws dr = draws_at(fp, vr->seed, ((uint64_t)m.w << 32) | m.z, depth);
+                // guide: draw slot 7 picks guide or BSDF; the mixture of this entry
+                const float gfrac = GUIDED ? p.guide.cfg.guide_fraction : 0.0f;
+                float u7 = 1.0f;
+                GuideMixture gm;
+                if (GUIDED && gfrac > 0.0f && q < 2) {
+                    Draws d7 = dr;
+                    for (int k = 0; k < 7; k++) d7.state = d7.state * kPcgMult + d7.inc;
+                    u7 = d7.next();
+                    gm = p.guide.mix[e];
+                }
+                const bool by_guide = GUIDED && gfrac > 0.0f && u7 < gfrac;
+                float pdf_used = 0.0f;
                 const float u0 = dr.next(), u1 = dr.next(), u2 = dr.next(), u3 = dr.next();
                 float l0 = 0.0f, l1 = 0.0f, l2 = 0.0f;
                 if (nee) {
@@ -1101,12 +1140,23 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                             }
                         }
                     }
-                    wi = fr.from_local(cosine_hemisphere(u0, u1));
+                    wi = by_guide ? guide_sample(gm, u2, u0, u1) : fr.from_local(cosine_hemisphere(u0, u1));
                     const float ci = fmaxf(dot(fr.n, wi), 0.0f);
                     const float gci = fmaxf(dot(h.gn, wi), 0.0f);
                     if (!(ci <= 0.0f || gci <= 0.0f)) {
-                        weight = albedo;
-                        scatter = true;
+                        if (GUIDED && gfrac > 0.0f) {
+                            // one-sample MIS: f cos / (a q + (1-a) cos/pi)
+                            const float pdf = gfrac * guide_pdf(gm, wi) + (1.0f - gfrac) * ci * kInvPi;
+                            if (pdf > 1e-8f) {
+                                weight = albedo * (ci * kInvPi / pdf);
+                                pdf_used = pdf;
+                                scatter = true;
+                            }
+                        } else {
+                            weight = albedo;
+                            pdf_used = ci * kInvPi;
+                            scatter = true;
+                        }
                     }
                 } else if (q == 1) {
                     // GGX, MaterialEval.cu:652-880
@@ -1151,7 +1201,8 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                         if (!(wsum <= 1e-6f)) {
                             const float pd = dw / wsum, ps = sw / wsum;
                             V3 wl;
-                            if (u2 < pd) wl = cosine_hemisphere(u0, u1);
+                            if (by_guide) wl = fr.to_local(guide_sample(gm, u2, u0, u1));
+                            else if (u2 < pd) wl = cosine_hemisphere(u0, u1);
                             else wl = reflect(-wo, ggx_sample_wm(wo, alpha, u0, u1));
                             const float ci = wl.z, co = wo.z;
                             if (!(ci <= 0.0f)) {
@@ -1164,12 +1215,14 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                                         const V3 fs = F * ggx_d(wm, alpha) * ggx_g(wo, wl, alpha) / (4.0f * ci * co);
                                         const float pdfd = ci * kInvPi;
                                         const float pdfs = ggx_dv(wo, wm, alpha) / (4.0f * owm);
-                                        const float pdf = pd * pdfd + ps * pdfs;
+                                        const V3 ww = fr.from_local(wl);
+                                        float pdf = pd * pdfd + ps * pdfs;
+                                        if (GUIDED && gfrac > 0.0f) pdf = gfrac * guide_pdf(gm, ww) + (1.0f - gfrac) * pdf;
                                         if (!(pdf <= 1e-8f)) {
-                                            const V3 ww = fr.from_local(wl);
                                             if (!(dot(ww, h.gn) <= 0.0f)) {
                                                 wi = ww;
                                                 weight = (fd + fs) * (ci / pdf);
+                                                pdf_used = pdf;
                                                 scatter = true;
                                             }
                                         }
@@ -1194,6 +1247,22 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                         new_thr = k.thr;
                         new_flags = k.depth | (m.y & 0x00FFFF00u) | (new_delta ? kDeltaBit : 0u);
                         to_ray = true;
+                        if (GUIDED && q < 2) {
+                            want_rec = true;
+                            const V3 wo3 = v3(0.0f) - rdir;
+                            rec.p[0] = h.p.x, rec.p[1] = h.p.y, rec.p[2] = h.p.z;
+                            rec.prev = head;
+                            rec.n[0] = h.gn.x, rec.n[1] = h.gn.y, rec.n[2] = h.gn.z;
+                            rec.pdf = pdf_used;
+                            rec.wo[0] = wo3.x, rec.wo[1] = wo3.y, rec.wo[2] = wo3.z;
+                            rec.cos_i = fabsf(dot(fr.n, wi));
+                            rec.wi[0] = wi.x, rec.wi[1] = wi.y, rec.wi[2] = wi.z;
+                            rec.pad = 0.0f;
+                            rec.t_after[0] = k.thr.x, rec.t_after[1] = k.thr.y, rec.t_after[2] = k.thr.z;
+                            rec.pad2 = 0.0f;
+                            rec.li[0] = rec.li[1] = rec.li[2] = 0.0f;
+                            rec.pad3 = 0.0f;
+                        }
                     }
                 }
             }
@@ -1206,6 +1275,26 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
         }
         uint32_t ray_pos, sh_pos;
         block_reserve2(to_ray, so.emit, &c->ray[nxt], &c->shadow, ray_pos, sh_pos);
+        uint32_t new_head = head;
+        if (GUIDED) {
+            // wave-aggregated slot in the record buffer
+            const uint64_t mask = __ballot(want_rec);
+            uint32_t base = 0;
+            if (mask) {
+                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+                if (lane_id() == leader) base = atomicAdd(p.guide.rec_count, (uint32_t)__popcll(mask));
+                base = readlane_u32(base, leader);
+            }
+            if (want_rec) {
+                const uint32_t ridx = base + mbcnt(mask);
+                if (ridx < p.guide.rec_capacity) {
+                    p.guide.recs[ridx] = rec;
+                    new_head = ridx;
+                }
+            }
+            if (so.emit) p.guide.sh_head[sh_pos] = head;
+            if (to_ray) (nxt ? p.guide.head[1] : p.guide.head[0])[ray_pos] = new_head;
+        }
         if (so.emit) {
             p.sh_o[sh_pos] = f4(so.o, so.tmax);
             p.sh_d[sh_pos] = f4(so.d, scene_bits);
@@ -1374,11 +1463,12 @@ hipError_t launch_bin(hipStream_t s, const PathState& p, int cur, uint32_t grid)
     return hipGetLastError();
 }
 
-hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool multi, uint32_t grid,
-                        uint32_t block) {
-    if (block != 256) return hipErrorInvalidValue;
-    if (multi) hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, s, p, sc, cur);
-    else hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(256), 0, s, p, sc, cur);
+hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool multi, bool guided,
+                        uint32_t grid, uint32_t block) {
+    if (block != 256 || (multi && guided)) return hipErrorInvalidValue;
+    if (guided) hipLaunchKernelGGL((k_shade<false, true>), dim3(grid), dim3(256), 0, s, p, sc, cur);
+    else if (multi) hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(256), 0, s, p, sc, cur);
+    else hipLaunchKernelGGL((k_shade<false, false>), dim3(grid), dim3(256), 0, s, p, sc, cur);
     return hipGetLastError();
 }
 

@@ -4,6 +4,7 @@
 // Views are POD structs of device pointers passed by value.
 #pragma once
 
+#include "guide.hpp"
 #include "pt_device.hpp"
 
 namespace pt {
@@ -23,6 +24,23 @@ struct SceneDev {
     uint32_t tri_count;
     uint32_t node_width;            // 4 or 2
     uint32_t pad;
+};
+
+// Device state of the guide network and of its training records.
+struct GuideDev {
+    float* w32;                     // fp32 masters: [layer] { W[out][in], b[out] }
+    uint16_t* w;                    // bf16 [layer][out][in]
+    uint16_t* wt;                   // bf16 [layer][in][out]
+    float* grad;
+    float* m;                       // Adam moments
+    float* v;
+    GuideMixture* mix;              // per diffuse/GGX entry of the current bounce ([diffuse | ggx] order)
+    GuideRecord* recs;              // training records of the current pass
+    unsigned int* rec_count;
+    uint32_t rec_capacity;
+    uint32_t* head[2];              // per path position: newest record of the path, kInvalid if none
+    uint32_t* sh_head;              // per shadow entry: newest record before its vertex
+    GuideConfig cfg;
 };
 
 // Path state, compacted every bounce: ray queue k holds, at position i, the
@@ -47,6 +65,7 @@ struct PathState {
     uint32_t* framebuffer;          // RGBA8, same indexing as accum
     uint32_t capacity;              // path slots
     uint32_t pixels;                // accumulation pixels allocated
+    GuideDev guide;                 // head[0] == nullptr: no guide records
 };
 
 struct TraceConfig {
@@ -74,8 +93,10 @@ hipError_t launch_generate(hipStream_t s, const PathState& p, bool init, int nxt
 hipError_t launch_closest(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool count, bool multi,
                           const TraceConfig& cfg);
 hipError_t launch_bin(hipStream_t s, const PathState& p, int cur, uint32_t grid);
-hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool multi, uint32_t grid,
-                        uint32_t block);
+// `guided`: one-sample MIS of the guide mixture and the BSDF, with training
+// records (single view only).
+hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool multi, bool guided,
+                        uint32_t grid, uint32_t block);
 hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, bool count, bool multi,
                          const TraceConfig& cfg);
 hipError_t launch_prepare(hipStream_t s, const PathState& p, int cur);
@@ -95,6 +116,14 @@ hipError_t launch_classify(hipStream_t s, const SceneDev& sc, const float4* rays
                            const TraceConfig& cfg, uint32_t bin_grid);
 hipError_t launch_alias_sample(hipStream_t s, const SceneDev& sc, const float* u, uint32_t n,
                                uint32_t* index, float* pmf);
+// Guide network (guide.hip).
+hipError_t launch_guide_sync(hipStream_t s, const GuideDev& g);
+hipError_t launch_guide_forward(hipStream_t s, const GuideDev& g, const float* features, uint32_t n, float* out);
+hipError_t launch_guide_infer(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, uint32_t grid);
+hipError_t launch_guide_train(hipStream_t s, const GuideDev& g, const GuideRecord* recs, uint32_t count,
+                              uint32_t batch, uint32_t seed, float w_max, float* loss);
+hipError_t launch_guide_adam(hipStream_t s, const GuideDev& g, float lr, uint32_t step, float grad_scale);
+
 // Stream-ordered copy of `bytes` (a multiple of 4) from host memory to device
 // memory, passed as kernel arguments: the host buffer may be reused at once.
 constexpr uint32_t kWritePayloadWords = 896;

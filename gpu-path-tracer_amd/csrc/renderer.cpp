@@ -99,8 +99,13 @@ void Renderer::invalidate_graph() {
     graphs_.clear();
 }
 
-PathState Renderer::path_state() const {
+PathState Renderer::path_state(const LoopMode* mode) const {
     PathState p{};
+    if (mode && mode->guided) {
+        p.guide = guide_dev();
+        p.guide.cfg.guide_fraction = mode->guide_fraction;
+        p.guide.cfg.enabled = 1;
+    }
     for (int k = 0; k < 2; k++) {
         p.ray_o[k] = d_ray_o_[k].as<float4>();
         p.ray_d[k] = d_ray_d_[k].as<float4>();
@@ -303,12 +308,14 @@ pt_status Renderer::Clear(uint8_t r, uint8_t g, uint8_t b, uint8_t a) {
 }
 
 pt_status Renderer::launch_iteration(int cur, bool count, const LoopMode& mode) {
-    const PathState p = path_state();
+    const PathState p = path_state(&mode);
     const SceneDev sc = scene_dev(mode.scene);
     const uint32_t grid = (uint32_t)std::max(1, cu_count_) * 8u;
     PT_TRY_HIP(launch_closest(stream_, p, sc, cur, count, mode.trace_multi, trace_), "closest-hit launch");
     PT_TRY_HIP(launch_bin(stream_, p, cur, bin_grid()), "binning launch");
-    PT_TRY_HIP(launch_shade(stream_, p, sc, cur, mode.shade_multi, grid, 256), "shade launch");
+    if (mode.guided && mode.guide_fraction > 0.0f)
+        PT_TRY_HIP(launch_guide_infer(stream_, p, sc, cur, (uint32_t)std::max(1, cu_count_) * 16u), "guide inference");
+    PT_TRY_HIP(launch_shade(stream_, p, sc, cur, mode.shade_multi, mode.guided, grid, 256), "shade launch");
     PT_TRY_HIP(launch_shadow(stream_, p, sc, count, mode.trace_multi, trace_), "shadow launch");
     PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
     PT_TRY_HIP(launch_prepare(stream_, p, cur), "prepare launch");
@@ -390,7 +397,7 @@ pt_status Renderer::make_view(const pt_view& v, ViewRecord& rec, uint32_t& depth
 // the one pool.  Leaves the tone-mapped pixels in d_frame_ and the radiance
 // sums in d_accum_.
 pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max_depth, uint32_t flags,
-                              pt_frame_stats* stats, const std::atomic<bool>* stop) {
+                              pt_frame_stats* stats, const std::atomic<bool>* stop, const RunOptions& opt) {
     const uint32_t n = (uint32_t)views.size();
     if (n == 0 || n > kViewRing) return fail(PT_ERROR_INVALID_ARGUMENT, "a batch holds 1 to " + std::to_string(kViewRing) + " views");
     const ViewRecord& last = views.back();
@@ -409,6 +416,10 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
         if (scenes_[v.scene]->host.bvh.width != scenes_[mode.scene]->host.bvh.width)
             return fail(PT_ERROR_INVALID_ARGUMENT, "the scenes of a batch must share one BVH node width");
     }
+    mode.guided = opt.guided;
+    mode.guide_fraction = opt.guided ? opt.guide_fraction : 0.0f;
+    if (mode.guided && (mode.trace_multi || mode.shade_multi))
+        return fail(PT_ERROR_INVALID_ARGUMENT, "guided rendering takes one view");
 
     FrameParams& fp = *h_fp_;
     std::memset(&fp, 0, sizeof(fp));
@@ -434,14 +445,14 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
         }
     } guard{t_begin, t_end};
 
-    const PathState p = path_state();
+    const PathState p = path_state(&mode);
     const SceneDev sc = scene_dev(mode.scene);
     const uint32_t grid = (uint32_t)std::max(1, cu_count_) * 8u;
     PT_TRY_HIP(launch_write_words(stream_, d_views_.as<void>(), views.data(), n * sizeof(ViewRecord)), "view records");
     PT_TRY_HIP(launch_write_words(stream_, d_fp_.as<void>(), h_fp_, sizeof(FrameParams)), "frame params");
     PT_TRY_HIP(hipEventRecord(t_begin, stream_), "event");
     PT_TRY_HIP(launch_frame_begin(stream_, p), "frame begin");
-    PT_TRY_HIP(launch_fill_accum(stream_, p, 0, total_pixels), "clear accumulation");
+    if (opt.clear) PT_TRY_HIP(launch_fill_accum(stream_, p, 0, total_pixels), "clear accumulation");
     PT_TRY_HIP(launch_generate(stream_, p, true, 0, grid, 256), "initialize paths");
 
     // Every path ends within `depth` iterations and slots are refilled at once,
@@ -468,7 +479,9 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
             PT_TRY_HIP(hipEventRecord(ev[1], stream_), "event");
             PT_TRY_HIP(launch_bin(stream_, p, cur, bin_grid()), "binning launch");
             PT_TRY_HIP(hipEventRecord(ev[2], stream_), "event");
-            PT_TRY_HIP(launch_shade(stream_, p, sc, cur, mode.shade_multi, grid, 256), "shade launch");
+            if (mode.guided && mode.guide_fraction > 0.0f)
+                PT_TRY_HIP(launch_guide_infer(stream_, p, sc, cur, (uint32_t)std::max(1, cu_count_) * 16u), "guide inference");
+            PT_TRY_HIP(launch_shade(stream_, p, sc, cur, mode.shade_multi, mode.guided, grid, 256), "shade launch");
             PT_TRY_HIP(hipEventRecord(ev[3], stream_), "event");
             PT_TRY_HIP(launch_shadow(stream_, p, sc, false, mode.trace_multi, trace_), "shadow launch");
             PT_TRY_HIP(hipEventRecord(ev[4], stream_), "event");

@@ -37,6 +37,13 @@ private:
 
 struct ServerState;   // server.hpp
 
+// Options of one wavefront run over a set of views.
+struct RunOptions {
+    bool clear = true;           // zero the accumulation pixels first
+    bool guided = false;         // guide MIS and training records (one view)
+    float guide_fraction = 0.0f;
+};
+
 // A scene resident on the device: host copy (for the hooks and the BVH
 // queries) and its device records.
 struct ResidentScene {
@@ -76,6 +83,18 @@ public:
     pt_status RenderBatch(const pt_view* views, uint32_t n, float* const* hdr, uint32_t* const* rgba, uint32_t flags,
                           pt_frame_stats* stats);
 
+    // learned path guiding (guide_host.cpp)
+    pt_status GuideReset(uint64_t seed);
+    pt_status GuideGetWeights(float* w, uint32_t n);
+    pt_status GuideSetWeights(const float* w, uint32_t n);
+    pt_status GuideSetBox(const float lo[3], const float hi[3]);
+    pt_status GuideForward(const float* features, uint32_t n, float* out);
+    pt_status GuideGradient(const pt_guide_record* recs, uint32_t n, float w_max, float* grad, float* loss);
+    pt_status GuideTrain(const pt_guide_record* recs, uint32_t n, uint32_t steps, uint32_t batch, float lr, float w_max,
+                         float* losses);
+    pt_status RenderGuided(const pt_render_settings& s, const pt_camera& camera, const pt_motion_state& state,
+                           const pt_guide_settings& g, pt_guide_stats* gstats, pt_frame_stats* stats);
+
     // serving (server.cpp)
     pt_status ServerStart(const pt_server_config& cfg);
     pt_status ServerSubmit(const pt_view& view, float* hdr, uint32_t* rgba, uint64_t& id);
@@ -110,8 +129,14 @@ private:
         bool trace_multi = false;
         bool shade_multi = false;
         uint32_t scene = 0;          // the one scene when !trace_multi
-        int key() const { return trace_multi ? -1 - (int)shade_multi : (int)(scene * 2u + (shade_multi ? 1u : 0u)); }
+        bool guided = false;         // guide MIS and training records (one view)
+        float guide_fraction = 0.0f; // baked into the captured kernels' arguments
+        int key() const {
+            if (guided) return (1 << 30) | (int)(scene << 12) | (int)(guide_fraction * 1000.0f + 0.5f);
+            return trace_multi ? -1 - (int)shade_multi : (int)(scene * 2u + (shade_multi ? 1u : 0u));
+        }
     };
+
     struct GraphEntry {
         int key;
         hipGraph_t graph;
@@ -126,7 +151,7 @@ private:
     pt_status make_view(const pt_view& v, ViewRecord& rec, uint32_t& depth);
     pt_status ensure_graph(const LoopMode& mode, hipGraphExec_t& exec);
     pt_status run_views(const std::vector<ViewRecord>& views, uint32_t max_depth, uint32_t flags, pt_frame_stats* stats,
-                        const std::atomic<bool>* stop);
+                        const std::atomic<bool>* stop, const RunOptions& opt = RunOptions());
     pt_status render_frame(const pt_render_settings& s, const pt_camera& camera, const pt_motion_state& state,
                            uint32_t flags, pt_frame_stats* stats, const std::atomic<bool>* stop);
     pt_status launch_iteration(int cur, bool count, const LoopMode& mode);
@@ -136,7 +161,11 @@ private:
     pt_status save_frame_png(const std::string& path);
     void simulation_loop(uint32_t start_frame);
     void invalidate_graph();
-    PathState path_state() const;
+    PathState path_state(const LoopMode* mode = nullptr) const;
+    pt_status ensure_guide();
+    pt_status ensure_guide_pool(uint32_t record_capacity);
+    GuideDev guide_dev() const;
+    void guide_box_from_scene();
     uint32_t bin_grid() const { return (uint32_t)std::max(1, cu_count_) * 4u; }
     bool has_scene() const { return scenes_[0] != nullptr; }
     const HostScene& host0() const { return scenes_[0]->host; }
@@ -201,6 +230,16 @@ private:
     std::string output_folder_;
     mutable std::mutex error_mutex_;
     std::optional<std::string> parked_error_;
+
+    // guide network and its training records
+    DeviceBuffer g_w32_, g_w_, g_wt_, g_grad_, g_m_, g_v_, g_loss_;
+    DeviceBuffer g_mix_, g_recs_, g_rec_count_, g_head_[2], g_sh_head_;
+    uint32_t g_rec_capacity_ = 0;
+    uint32_t g_pool_ = 0;            // path slots the per-path guide buffers cover
+    uint32_t g_step_ = 0;            // Adam steps taken
+    bool g_init_ = false;
+    bool g_box_set_ = false;         // box given by GuideSetBox (else scene 0's bounds)
+    GuideConfig g_cfg_{};
 
     std::unique_ptr<ServerState> server_;
     LoopMode server_mode_;

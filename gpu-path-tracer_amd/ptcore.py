@@ -70,6 +70,25 @@ class ServerStats(C.Structure):
                 ("iterations", C.c_uint64), ("wall_ms", C.c_double), ("busy_ms", C.c_double)]
 
 
+class GuideSettings(C.Structure):
+    _fields_ = [("passes", C.c_uint32), ("train_steps", C.c_uint32), ("batch", C.c_uint32),
+                ("learning_rate", C.c_float), ("guide_fraction", C.c_float), ("weight_max", C.c_float),
+                ("record_capacity", C.c_uint32), ("reset", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class GuideStats(C.Structure):
+    _fields_ = [("render_ms", C.c_double), ("train_ms", C.c_double), ("records", C.c_uint64),
+                ("loss_first", C.c_float), ("loss_last", C.c_float), ("steps", C.c_uint32), ("passes", C.c_uint32)]
+
+
+GUIDE_PARAMS = 16640
+GUIDE_DIM = 64
+GUIDE_RECORD_DTYPE = np.dtype([("position", "<f4", 3), ("prev", "<u4"), ("normal", "<f4", 3), ("pdf", "<f4"),
+                               ("wo", "<f4", 3), ("cos_i", "<f4"), ("wi", "<f4", 3), ("pad0", "<f4"),
+                               ("throughput", "<f4", 3), ("pad1", "<f4"), ("radiance", "<f4", 3), ("pad2", "<f4")])
+assert GUIDE_RECORD_DTYPE.itemsize == 96
+
+
 class Progress(C.Structure):
     _fields_ = [("is_rendering", C.c_uint32), ("done_frames", C.c_uint32), ("total_frames", C.c_uint32),
                 ("width", C.c_uint32), ("height", C.c_uint32), ("samples_per_pixel", C.c_uint32),
@@ -139,6 +158,15 @@ SIGNATURES = {
     "pt_server_wait": (C.c_int, [_P, C.c_uint64, _U32, C.POINTER(RequestInfo)]),
     "pt_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
     "pt_server_stop": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "pt_render_guided": (C.c_int, [_P, C.POINTER(RenderSettings), C.POINTER(Camera), C.POINTER(MotionState),
+                                   C.POINTER(GuideSettings), C.POINTER(GuideStats), C.POINTER(FrameStats)]),
+    "pt_guide_reset": (C.c_int, [_P, C.c_uint64]),
+    "pt_guide_get_weights": (C.c_int, [_P, _P, _U32]),
+    "pt_guide_set_weights": (C.c_int, [_P, _P, _U32]),
+    "pt_guide_set_box": (C.c_int, [_P, _P, _P]),
+    "pt_guide_forward": (C.c_int, [_P, _P, _U32, _P]),
+    "pt_guide_gradient": (C.c_int, [_P, _P, _U32, C.c_float, _P, _P]),
+    "pt_guide_train": (C.c_int, [_P, _P, _U32, _U32, _U32, C.c_float, C.c_float, _P]),
     "pt_trace_closest": (C.c_int, [_P, _P, _U32, _P]),
     "pt_trace_any": (C.c_int, [_P, _P, _U32, _P]),
     "pt_trace_closest_device": (C.c_int, [_P, _P, _U32, _P, _P]),
@@ -346,6 +374,58 @@ class Renderer:
         st = FrameStats()
         self._check(self.lib.pt_render_batch(self.h, views, n, hp, lp, flags, C.byref(st)))
         return (hdr, st.as_dict(), ldr) if rgba else (hdr, st.as_dict())
+
+    # ---- learned path guiding
+    def RenderGuided(self, settings, view, passes=4, train_steps=32, batch=16384, learning_rate=1e-3,
+                     guide_fraction=0.5, weight_max=1e4, record_capacity=0, reset=True, seed=1):
+        """Renders a frame in `passes` passes with the guide trained between
+        them; returns (frame stats, guide stats)."""
+        s = _settings(settings)
+        cam = Camera(view.fov_y, 0.1, 1000.0)
+        m = _motion(view)
+        g = GuideSettings(passes, train_steps, batch, learning_rate, guide_fraction, weight_max, record_capacity,
+                          1 if reset else 0, seed)
+        gst, st = GuideStats(), FrameStats()
+        self._check(self.lib.pt_render_guided(self.h, C.byref(s), C.byref(cam), C.byref(m), C.byref(g), C.byref(gst),
+                                              C.byref(st)))
+        return st.as_dict(), {k: getattr(gst, k) for k, _ in gst._fields_}
+
+    def guide_reset(self, seed=1):
+        self._check(self.lib.pt_guide_reset(self.h, seed))
+
+    def guide_weights(self):
+        w = np.zeros(GUIDE_PARAMS, np.float32)
+        self._check(self.lib.pt_guide_get_weights(self.h, _ptr(w), GUIDE_PARAMS))
+        return w
+
+    def guide_set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._check(self.lib.pt_guide_set_weights(self.h, _ptr(w), len(w)))
+
+    def guide_set_box(self, lo, hi):
+        lo = np.ascontiguousarray(lo, np.float32)
+        hi = np.ascontiguousarray(hi, np.float32)
+        self._check(self.lib.pt_guide_set_box(self.h, _ptr(lo), _ptr(hi)))
+
+    def guide_forward(self, inputs):
+        x = np.ascontiguousarray(inputs, np.float32)
+        out = np.zeros((len(x), GUIDE_DIM), np.float32)
+        self._check(self.lib.pt_guide_forward(self.h, _ptr(x), len(x), _ptr(out)))
+        return out
+
+    def guide_gradient(self, records, weight_max=1e30):
+        r = np.ascontiguousarray(records, GUIDE_RECORD_DTYPE)
+        g = np.zeros(GUIDE_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(self.lib.pt_guide_gradient(self.h, _ptr(r), len(r), weight_max, _ptr(g), _ptr(loss)))
+        return g, float(loss[0])
+
+    def guide_train(self, records, steps, batch, learning_rate=1e-3, weight_max=1e30):
+        r = np.ascontiguousarray(records, GUIDE_RECORD_DTYPE)
+        losses = np.zeros(steps, np.float32)
+        self._check(self.lib.pt_guide_train(self.h, _ptr(r), len(r), steps, batch, learning_rate, weight_max,
+                                            _ptr(losses)))
+        return losses
 
     # ---- serving
     def ServerStart(self, accum_budget_bytes=0, window_samples=0, chunk_samples=0, max_requests=0):

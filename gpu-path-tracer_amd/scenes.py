@@ -474,3 +474,68 @@ def serve_request(rng, scene_count, resolutions=SERVE_RESOLUTIONS, spp=SERVE_SPP
     st = dict(width=w, height=h, samples_per_pixel=s, path_depth=depth, use_nee=1, rr_start_depth=3,
               seed=int(rng.integers(1, 1 << 31)))
     return sid, st, v
+
+
+# ---------------------------------------------------------------- guiding workload
+
+def guiding_room(triangles=200_000, seed=5):
+    """Config 5: two rooms split by a 10 cm wall at z = 0 with a 0.5 m gap
+    (x in [1.0, 1.5], full height).  The only light is 4 emissive triangles
+    (two 0.4 m quads, radiance 50) under the ceiling of the back room, out of
+    sight of the camera in the front room.  Walls Lambertian (config-1 white,
+    red, green), two GGX boxes in the front room; every quad but the light is
+    tessellated so the scene holds about `triangles` triangles."""
+    quads = []   # (p0, p1, p2, p3, normal, material)
+    white, red, green, ggx, light = 0, 1, 2, 3, 4
+    X0, X1, Y1, Z0, Z1 = -2.0, 2.0, 2.5, -4.0, 3.0
+    # outer box, normals inward
+    quads += [((X0, 0, Z0), (X0, 0, Z1), (X1, 0, Z1), (X1, 0, Z0), (0, 1, 0), white),        # floor
+              ((X0, Y1, Z0), (X1, Y1, Z0), (X1, Y1, Z1), (X0, Y1, Z1), (0, -1, 0), white),    # ceiling
+              ((X0, 0, Z0), (X1, 0, Z0), (X1, Y1, Z0), (X0, Y1, Z0), (0, 0, 1), white),       # back
+              ((X1, 0, Z1), (X0, 0, Z1), (X0, Y1, Z1), (X1, Y1, Z1), (0, 0, -1), white),      # front
+              ((X0, 0, Z1), (X0, 0, Z0), (X0, Y1, Z0), (X0, Y1, Z1), (1, 0, 0), red),         # left
+              ((X1, 0, Z0), (X1, 0, Z1), (X1, Y1, Z1), (X1, Y1, Z0), (-1, 0, 0), green)]      # right
+
+    def slab(xa, xb, za, zb):
+        """Faces of the wall piece [xa, xb] x [0, Y1] x [za, zb] facing the rooms and the gap."""
+        return [((xa, 0, zb), (xb, 0, zb), (xb, Y1, zb), (xa, Y1, zb), (0, 0, 1), white),
+                ((xb, 0, za), (xa, 0, za), (xa, Y1, za), (xb, Y1, za), (0, 0, -1), white),
+                ((xa, 0, za), (xa, 0, zb), (xa, Y1, zb), (xa, Y1, za), (-1, 0, 0), white),
+                ((xb, 0, zb), (xb, 0, za), (xb, Y1, za), (xb, Y1, zb), (1, 0, 0), white)]
+
+    quads += slab(X0, 1.0, -0.05, 0.05)[:2] + [slab(X0, 1.0, -0.05, 0.05)[3]]
+    quads += slab(1.5, X1, -0.05, 0.05)[:3]
+    s = 0.4
+    lights = [((-1.4, Y1 - 0.01, -3.3), (-1.4 + s, Y1 - 0.01, -3.3), (-1.4 + s, Y1 - 0.01, -3.3 + s),
+               (-1.4, Y1 - 0.01, -3.3 + s), (0, -1, 0), light),
+              ((-0.8, Y1 - 0.01, -3.3), (-0.8 + s, Y1 - 0.01, -3.3), (-0.8 + s, Y1 - 0.01, -3.3 + s),
+               (-0.8, Y1 - 0.01, -3.3 + s), (0, -1, 0), light)]
+    boxes = [((-1.2, 0.0, 0.8), (-0.4, 0.9, 1.6)), ((0.3, 0.0, 0.4), (0.9, 1.4, 1.0))]
+    box_faces = 6 * len(boxes)
+    k = max(1, int(round(math.sqrt((triangles - 4) / (2.0 * (len(quads) + box_faces))))))
+    mats = np.stack([
+        material(MODEL_DIFFUSE, color=_C1_COLORS[0]), material(MODEL_DIFFUSE, color=_C1_COLORS[1]),
+        material(MODEL_DIFFUSE, color=_C1_COLORS[2]),
+        material(MODEL_GGX, color=(0.9, 0.8, 0.6), roughness=0.3, metallic=0.0, ior=1.5),
+        material(MODEL_EMISSIVE, color=(1.0, 1.0, 1.0), emission=(50.0, 50.0, 50.0)),
+    ])
+    P, N, T, M = [], [], [], []
+    for *pts, nrm, mid in quads:
+        g = _grid_quad(*pts, nrm, k)
+        P.append(g[0]); N.append(g[1]); T.append(g[2]); M.append(np.full(len(g[0]), mid, np.uint32))
+    for lo, hi in boxes:
+        b = _box(lo, hi, k)
+        P.append(b[0]); N.append(b[1]); T.append(b[2]); M.append(np.full(len(b[0]), ggx, np.uint32))
+    for *pts, nrm, mid in lights:
+        g = _quad(*pts, nrm)
+        P.append(g[0]); N.append(g[1]); T.append(g[2]); M.append(np.full(2, light, np.uint32))
+    cam = (-0.2, 1.4, 2.8)
+    target = (0.6, 1.0, 0.0)
+    return Scene(
+        name=f"guiding_room_{sum(len(p) for p in P)}",
+        positions=np.concatenate(P).astype(np.float32), normals=np.concatenate(N).astype(np.float32),
+        tangents=np.concatenate(T).astype(np.float32), material_ids=np.concatenate(M), materials=mats,
+        view=View(fov_y=math.radians(55.0), position=cam, rotation=look_at_quaternion(cam, target)),
+        settings=dict(width=1280, height=720, samples_per_pixel=64, path_depth=8, use_nee=1, rr_start_depth=3,
+                      seed=1),
+    )

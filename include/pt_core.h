@@ -336,6 +336,77 @@ pt_status pt_server_get_stats(pt_renderer* renderer, pt_server_stats* stats);
 /* Finishes every submitted request, then stops the server thread. */
 pt_status pt_server_stop(pt_renderer* renderer, pt_server_stats* stats);
 
+/* ---- learned path guiding (no reference counterpart: the reference samples
+ * directions from the BSDF only, MaterialEval.cu:270-309,720-836) ----
+ * An MLP of 3 hidden layers x 64 (BF16 weights, FP32 accumulation) maps a shading
+ * point's position (with a sin/cos frequency encoding), normal and outgoing
+ * direction to a mixture of 8 von Mises-Fisher lobes.  Diffuse and GGX vertices
+ * sample it by one-sample MIS with the BSDF, so the estimator stays unbiased, and
+ * the network is trained on the GPU with Adam from the radiance the paths carry. */
+enum { PT_GUIDE_INPUTS = 64, PT_GUIDE_OUTPUTS = 64, PT_GUIDE_PARAMS = 16640 };
+
+/* Training record of one scatter vertex (96 bytes). */
+typedef struct pt_guide_record {
+    float position[3];
+    uint32_t prev;            /* previous record of the same path (0xFFFFFFFF: none) */
+    float normal[3];          /* geometric normal facing the outgoing direction */
+    float pdf;                /* density the direction was sampled with */
+    float wo[3];              /* outgoing direction (toward the previous vertex) */
+    float cos_i;              /* |shading normal . wi| */
+    float wi[3];              /* sampled incident direction */
+    float pad0;
+    float throughput[3];      /* path throughput after this vertex */
+    float pad1;
+    float radiance[3];        /* incident radiance gathered along wi */
+    float pad2;
+} pt_guide_record;
+
+typedef struct pt_guide_settings {
+    uint32_t passes;          /* render passes; the samples per pixel are split evenly among them */
+    uint32_t train_steps;     /* Adam steps after each pass but the last */
+    uint32_t batch;           /* records per Adam step (multiple of 64) */
+    float learning_rate;
+    float guide_fraction;     /* probability of sampling the guide; 0 = plain BSDF sampling */
+    float weight_max;         /* clamp of a record's loss weight luminance(L) cos / pdf */
+    uint32_t record_capacity; /* training records kept per pass; 0 = 2^22 */
+    uint32_t reset;           /* re-initialise the network before the first pass */
+    uint64_t seed;            /* network initialisation and batch selection */
+} pt_guide_settings;
+
+typedef struct pt_guide_stats {
+    double render_ms;         /* device time of the passes */
+    double train_ms;          /* device time of the training steps */
+    uint64_t records;         /* training records written by the last trained pass */
+    float loss_first;         /* batch loss of the first and the last training step */
+    float loss_last;
+    uint32_t steps;           /* Adam steps taken in this call */
+    uint32_t passes;
+} pt_guide_stats;
+
+/* Renders one frame in `passes` passes (settings.samples_per_pixel / passes each);
+ * the network is trained between passes.  The frame is read with pt_frame_copy and
+ * pt_frame_copy_hdr; stats->render_ms includes the training time. */
+pt_status pt_render_guided(pt_renderer* renderer, const pt_render_settings* settings,
+                           const pt_camera* camera, const pt_motion_state* motion_state,
+                           const pt_guide_settings* guide, pt_guide_stats* guide_stats,
+                           pt_frame_stats* stats);
+/* Network parameters, fp32: per layer W[64 out][64 in] then b[64]. */
+pt_status pt_guide_reset(pt_renderer* renderer, uint64_t seed);
+pt_status pt_guide_get_weights(pt_renderer* renderer, float* weights, uint32_t count);
+pt_status pt_guide_set_weights(pt_renderer* renderer, const float* weights, uint32_t count);
+/* Box that positions are normalised to (default: the bounds of scene 0). */
+pt_status pt_guide_set_box(pt_renderer* renderer, const float lo[3], const float hi[3]);
+/* Raw network outputs [n][64] of inputs [n][64] (the MFMA forward pass). */
+pt_status pt_guide_forward(pt_renderer* renderer, const float* inputs, uint32_t count, float* outputs);
+/* Gradient (PT_GUIDE_PARAMS floats) of sum_i -w_i log q(wi_i) over the records, and
+ * the loss, without changing the network. */
+pt_status pt_guide_gradient(pt_renderer* renderer, const pt_guide_record* records, uint32_t count,
+                            float weight_max, float* gradient, float* loss);
+/* `steps` Adam steps on batches drawn from the records; losses[steps] may be NULL. */
+pt_status pt_guide_train(pt_renderer* renderer, const pt_guide_record* records, uint32_t count,
+                         uint32_t steps, uint32_t batch, float learning_rate, float weight_max,
+                         float* losses);
+
 /* ---- kernel-level hooks on caller-supplied batches ---- */
 /* IntersectRaysWithSceneKernel (Intersect.cu:63-123) on a ray batch. */
 pt_status pt_trace_closest(pt_renderer* renderer, const pt_ray* rays, uint32_t count, pt_hit* hits);

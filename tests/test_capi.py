@@ -90,7 +90,8 @@ _LAYOUTS = {
     "pt_motion_state": ptcore.MotionState, "pt_view": ptcore.View, "pt_progress": ptcore.Progress,
     "pt_frame_stats": ptcore.FrameStats, "pt_alias_bin": ptcore.AliasBin, "pt_bvh_info": ptcore.BvhInfo,
     "pt_tuning": ptcore.Tuning, "pt_server_config": ptcore.ServerConfig, "pt_request_info": ptcore.RequestInfo,
-    "pt_server_stats": ptcore.ServerStats,
+    "pt_server_stats": ptcore.ServerStats, "pt_guide_settings": ptcore.GuideSettings,
+    "pt_guide_stats": ptcore.GuideStats,
 }
 
 
@@ -104,6 +105,7 @@ def test_struct_layouts_match_header(tmp_path):
         for f, _ in cls._fields_:
             lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
     lines.append('printf("pt_material size %zu\\n", sizeof(pt_material));')
+    lines.append('printf("pt_guide_record size %zu\\n", sizeof(pt_guide_record));')
     src = tmp_path / "layout.c"
     src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "pt_core.h"\nint main(void) {\n'
                    + "\n".join(lines) + "\nreturn 0;\n}\n")
@@ -117,3 +119,4 @@ def test_struct_layouts_match_header(tmp_path):
         for f, _ in cls._fields_:
             assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)
     assert got[("pt_material", "size")] == scenes.MATERIAL_DTYPE.itemsize
+    assert got[("pt_guide_record", "size")] == ptcore.GUIDE_RECORD_DTYPE.itemsize

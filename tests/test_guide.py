@@ -1,0 +1,134 @@
+"""Learned path guiding on the GPU: the BF16 MFMA forward and backward passes
+against the CPU oracle, online training on a known distribution, and the
+guided renderer (unbiased against plain BSDF sampling, lower error at equal
+samples)."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import ptcore
+import scenes
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import guide_oracle as G  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _unit(v):
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def _records(n, seed):
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n, ptcore.GUIDE_RECORD_DTYPE)
+    r["position"] = rng.uniform(0.0, 1.0, (n, 3))
+    r["normal"] = _unit(rng.normal(size=(n, 3)))
+    r["wo"] = _unit(rng.normal(size=(n, 3)))
+    r["wi"] = _unit(rng.normal(size=(n, 3)))
+    r["radiance"] = rng.uniform(0.0, 2.0, (n, 3))
+    r["pdf"] = rng.uniform(0.05, 1.0, n)
+    r["cos_i"] = rng.uniform(0.0, 1.0, n)
+    r["prev"] = 0xFFFFFFFF
+    return r
+
+
+def test_guide_forward_matches_oracle(gpu_renderer):
+    w = G.init_weights(1)
+    gpu_renderer.guide_set_weights(w)
+    assert np.array_equal(gpu_renderer.guide_weights(), w)
+    x = np.random.default_rng(2).normal(0.0, 1.0, (300, 64)).astype(np.float32)   # 300: a partial last tile
+    got = gpu_renderer.guide_forward(x)
+    want, _ = G.forward(w, x, round=True)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    assert err < 1e-2, err
+    # rows are independent samples: permuting the inputs permutes the outputs
+    perm = np.random.default_rng(3).permutation(300)
+    assert np.allclose(gpu_renderer.guide_forward(x[perm]), got[perm], rtol=1e-6, atol=1e-6)
+
+
+def test_guide_gradient_matches_oracle(gpu_renderer):
+    w = G.init_weights(4)
+    gpu_renderer.guide_set_weights(w)
+    gpu_renderer.guide_set_box([0.0, 0.0, 0.0], [1.0, 1.0, 1.0])
+    recs = _records(640, 5)
+    g, loss = gpu_renderer.guide_gradient(recs)
+    gref, lref = G.record_gradient(w, recs, np.zeros(3), np.ones(3), round=True)
+    assert abs(loss - lref) <= 2e-3 * abs(lref), (loss, lref)
+    for l in range(G.LAYERS):
+        base = l * G.LAYER_PARAMS
+        for part in (slice(base, base + 4096), slice(base + 4096, base + G.LAYER_PARAMS)):
+            rel = np.linalg.norm(g[part] - gref[part]) / np.linalg.norm(gref[part])
+            assert rel < 3e-2, (l, part, rel)
+    assert np.array_equal(gpu_renderer.guide_weights(), w)      # the gradient leaves the weights alone
+
+
+def test_guide_learns_a_directional_distribution(gpu_renderer):
+    """Incident directions drawn from vMF(mu(p), 20) with mu depending on the
+    position: training must raise the held-out log-likelihood far above the
+    uniform density's log(1/4pi)."""
+    rng = np.random.default_rng(8)
+
+    def make(n):
+        r = np.zeros(n, ptcore.GUIDE_RECORD_DTYPE)
+        p = rng.uniform(0.0, 1.0, (n, 3))
+        mu = _unit(np.stack([p[:, 0] - 0.5, np.ones(n), p[:, 2] - 0.5], 1))
+        r["position"] = p
+        r["normal"] = [0.0, 1.0, 0.0]
+        r["wo"] = [0.0, 1.0, 0.0]
+        r["wi"] = G.vmf_sample(mu, 20.0, rng.random(n), rng.random(n))
+        r["radiance"] = 1.0
+        r["pdf"] = 1.0 / (4.0 * np.pi)
+        r["cos_i"] = 1.0
+        return r
+
+    train, held = make(1 << 16), make(4096)
+    gpu_renderer.guide_reset(3)
+    gpu_renderer.guide_set_box([0.0, 0.0, 0.0], [1.0, 1.0, 1.0])
+    losses = gpu_renderer.guide_train(train, steps=400, batch=4096, learning_rate=3e-3)
+    assert losses[-20:].mean() < losses[:20].mean()
+    x = G.features(np.zeros(3), np.ones(3), held["position"], held["normal"], held["wo"]).astype(np.float32)
+    lq = G.log_q(gpu_renderer.guide_forward(x), held["wi"])
+    assert lq.mean() > math.log(1.0 / (4.0 * math.pi)) + 2.0, lq.mean()
+
+
+@pytest.fixture(scope="module")
+def room_renderer():
+    room = scenes.guiding_room(20_000)
+    r = ptcore.Renderer()
+    r.InitializeRenderingBuffers(64, 36)
+    r.InitializeSceneBuffers(room)
+    yield r, room
+    r.close()
+
+
+def _relmse(a, b):
+    return float(np.mean((a - b) ** 2 / (b ** 2 + 1e-2)))
+
+
+def test_guided_render_is_unbiased_and_gains(room_renderer):
+    r, room = room_renderer
+    w, h = 64, 36
+    base = dict(width=w, height=h, path_depth=8, use_nee=1, rr_start_depth=3)
+    r.RenderGuided(dict(base, samples_per_pixel=4096, seed=99), room.view, passes=1, guide_fraction=0.0)
+    ref = r.GetFrameHdr(w, h)[..., :3]
+    st = dict(base, samples_per_pixel=256, seed=7)
+    sp, _ = r.RenderGuided(st, room.view, passes=4, guide_fraction=0.0)
+    plain = r.GetFrameHdr(w, h)[..., :3]
+    sg, gs = r.RenderGuided(st, room.view, passes=4, train_steps=64, batch=8192, learning_rate=3e-3,
+                            guide_fraction=0.5, reset=True, seed=1)
+    guided = r.GetFrameHdr(w, h)[..., :3]
+    assert sp["samples"] == sg["samples"] == w * h * 256
+    assert gs["steps"] == 3 * 64 and gs["records"] > 0 and math.isfinite(gs["loss_last"])
+    assert np.isfinite(guided).all() and (guided >= 0).all()
+    # unbiased: the image mean agrees with the reference's within 3 standard
+    # errors of the Monte Carlo noise (the spread of per-pixel differences)
+    for img in (plain, guided):
+        d = (img - ref).mean(-1).ravel()
+        assert abs(d.mean()) < 3 * d.std() / math.sqrt(d.size) + 1e-6, (d.mean(), d.std())
+    # the trained guide lowers the error at equal samples
+    e_plain, e_guided = _relmse(plain, ref), _relmse(guided, ref)
+    assert e_guided < 0.9 * e_plain, (e_guided, e_plain)
