@@ -136,35 +136,124 @@ __device__ __forceinline__ const float* bias_of(const GuideNet& n, int l) {
     return n.w32 + l * kGuideLayerParams + kGuideDim * kGuideDim;
 }
 
-// ---------------------------------------------------------------- forward on given features
+// ---------------------------------------------------------------- inference forward
+//
+// Inference computes each layer transposed, D = W . X^T (A = W rows from
+// global memory, B = X rows from LDS), so that lane l holds outputs
+// (16 mt + 4 (l >> 4) + j) of sample 16 nt + (l & 15): four consecutive
+// features of one sample, stored with one 8-byte LDS write.  The two bf16
+// ping-pong tiles double as the fp32 output tile (17 KB of LDS per wave).
 
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void mm_wxt(const uint16_t* x, const uint16_t* w, f32x4 acc[4][4]) {
+    const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++) {
+        bf16x4 wf[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) wf[mt] = glb4(w, 16 * mt + r, 16 * kt + 4 * kq);
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            const bf16x4 xf = row4(x, 16 * nt + r, 16 * kt + 4 * kq);
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wf[mt], xf, acc[mt][nt], 0, 0, 0);
+        }
+    }
+}
+
+__device__ __forceinline__ void infer_hidden(const uint16_t* in, uint16_t* out, const uint16_t* w, const float* bias) {
+    f32x4 acc[4][4];
+    mm_wxt(in, w, acc);
+    const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        const int o = 16 * mt + 4 * kq;
+        const float4 b = *reinterpret_cast<const float4*>(bias + o);
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            u16x4 v;
+            v[0] = to_bf16(fmaxf(acc[mt][nt][0] + b.x, 0.0f));
+            v[1] = to_bf16(fmaxf(acc[mt][nt][1] + b.y, 0.0f));
+            v[2] = to_bf16(fmaxf(acc[mt][nt][2] + b.z, 0.0f));
+            v[3] = to_bf16(fmaxf(acc[mt][nt][3] + b.w, 0.0f));
+            *reinterpret_cast<u16x4*>(out + (16 * nt + r) * kLd + o) = v;
+        }
+    }
+}
+
+constexpr int kLdInfer = 68;            // fp32 row stride of the inference output tile
+
+// Four layers on the tile in buf (bf16 [64][kLd] features in the first half);
+// leaves the fp32 outputs as [sample][kLdInfer] over the whole buffer.
+__device__ __forceinline__ void infer_forward(uint16_t* buf, const GuideNet& net) {
+    uint16_t* xa = buf;
+    uint16_t* xb = buf + 64 * kLd;
+    __syncthreads();
+    infer_hidden(xa, xb, net.w, bias_of(net, 0));
+    __syncthreads();
+    infer_hidden(xb, xa, net.w + 1 * kGuideDim * kGuideDim, bias_of(net, 1));
+    __syncthreads();
+    infer_hidden(xa, xb, net.w + 2 * kGuideDim * kGuideDim, bias_of(net, 2));
+    __syncthreads();
+    f32x4 acc[4][4];
+    mm_wxt(xb, net.w + 3 * kGuideDim * kGuideDim, acc);
+    __syncthreads();   // every read of xb is done: the outputs overwrite both tiles
+    float* o32 = reinterpret_cast<float*>(buf);
+    const float* bias = bias_of(net, 3);
+    const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        const int o = 16 * mt + 4 * kq;
+        const float4 b = *reinterpret_cast<const float4*>(bias + o);
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+            *reinterpret_cast<float4*>(o32 + (16 * nt + r) * kLdInfer + o) =
+                make_float4(acc[mt][nt][0] + b.x, acc[mt][nt][1] + b.y, acc[mt][nt][2] + b.z, acc[mt][nt][3] + b.w);
+    }
+    __syncthreads();
+}
+
+// One bf16 feature row (zeros past the features) with 8-byte stores.
+__device__ __forceinline__ void store_row(uint16_t* x, int s, const float f[kGuideFeatures], bool valid) {
+#pragma unroll
+    for (int i = 0; i < kGuideDim; i += 4) {
+        u16x4 v;
+#pragma unroll
+        for (int c = 0; c < 4; c++) v[c] = to_bf16(valid && i + c < kGuideFeatures ? f[i + c] : 0.0f);
+        *reinterpret_cast<u16x4*>(x + s * kLd + i) = v;
+    }
+}
+
+constexpr int kInferLds = 2 * 64 * kLd;   // uint16 elements: two bf16 tiles = one fp32 [64][68] tile
+static_assert(kInferLds * 2 == 64 * kLdInfer * 4, "the output tile fits the two feature tiles");
+
+// Raw outputs of given inputs (hook for tests).
 __global__ __launch_bounds__(64) void k_guide_forward(GuideNet net, const float* features, uint32_t n, float* out) {
-    __shared__ uint16_t xa[64 * kLd], xb[64 * kLd];
-    __shared__ float o32[64 * kLdOut];
+    __shared__ __attribute__((aligned(16))) uint16_t buf[kInferLds];
     const int s = threadIdx.x;
     const uint32_t idx = blockIdx.x * 64u + (uint32_t)s;
     const bool valid = idx < n;
-    for (int i = 0; i < kGuideDim; i++) xa[s * kLd + i] = to_bf16(valid ? features[(size_t)idx * kGuideDim + i] : 0.0f);
-    __syncthreads();
-    layer_hidden(xa, xb, net.w, bias_of(net, 0));
-    __syncthreads();
-    layer_hidden(xb, xa, net.w + 1 * kGuideDim * kGuideDim, bias_of(net, 1));
-    __syncthreads();
-    layer_hidden(xa, xb, net.w + 2 * kGuideDim * kGuideDim, bias_of(net, 2));
-    __syncthreads();
-    layer_output(xb, o32, net.w + 3 * kGuideDim * kGuideDim, bias_of(net, 3));
-    __syncthreads();
+    float f[kGuideFeatures];
+    for (int i = 0; i < kGuideFeatures; i++) f[i] = valid ? features[(size_t)idx * kGuideDim + i] : 0.0f;
+    store_row(buf, s, f, valid);
+    // inputs past the features are part of the hook's contract too
+    for (int i = kGuideFeatures; i < kGuideDim; i++) buf[s * kLd + i] = to_bf16(valid ? features[(size_t)idx * kGuideDim + i] : 0.0f);
+    infer_forward(buf, net);
+    const float* o32 = reinterpret_cast<const float*>(buf);
     if (valid)
-        for (int o = 0; o < kGuideDim; o++) out[(size_t)idx * kGuideDim + o] = o32[s * kLdOut + o];
+        for (int o = 0; o < kGuideDim; o++) out[(size_t)idx * kGuideDim + o] = o32[s * kLdInfer + o];
 }
-
-// ---------------------------------------------------------------- inference in the wavefront
 
 // Mixture of every diffuse and GGX queue entry of the current bounce, in the
 // shade kernel's virtual order [diffuse | ggx].
 __global__ __launch_bounds__(64) void k_guide_infer(PathState p, SceneDev sc, GuideNet net, int cur) {
-    __shared__ uint16_t xa[64 * kLd], xb[64 * kLd];
-    __shared__ float o32[64 * kLdOut];
+    __shared__ __attribute__((aligned(16))) uint16_t buf[kInferLds];
     const uint32_t c0 = p.ctr->queue[0], c1 = p.ctr->queue[1];
     const uint32_t total = c0 + c1;
     const GuideConfig g = p.guide.cfg;
@@ -172,7 +261,9 @@ __global__ __launch_bounds__(64) void k_guide_infer(PathState p, SceneDev sc, Gu
     for (uint32_t base = blockIdx.x * 64u; base < total; base += gridDim.x * 64u) {
         const uint32_t e = base + (uint32_t)s;
         const bool valid = e < total;
-        float f[kGuideFeatures];
+        uint16_t* row = buf + s * kLd;
+#pragma unroll
+        for (int k = 0; k < kGuideDim; k += 4) *reinterpret_cast<u16x4*>(row + k) = u16x4{0, 0, 0, 0};
         if (valid) {
             const uint32_t i = e < c0 ? p.q[0][e] : p.q[1][e - c0];
             const float4 o4 = (cur ? p.ray_o[1] : p.ray_o[0])[i];
@@ -183,24 +274,33 @@ __global__ __launch_bounds__(64) void k_guide_infer(PathState p, SceneDev sc, Gu
             const float* gn = sc.shade[__float_as_uint(hh.w)].gn;
             V3 n = v3(gn[0], gn[1], gn[2]);
             if (dot(n, d) > 0.0f) n = -n;
-            guide_features(g, x, n, -d, f);
+            guide_features_each(g, x, n, -d, [&](int k, float v) { row[k] = to_bf16(v); });
         }
-        store_features(xa, s, f, valid);
-        __syncthreads();
-        layer_hidden(xa, xb, net.w, bias_of(net, 0));
-        __syncthreads();
-        layer_hidden(xb, xa, net.w + 1 * kGuideDim * kGuideDim, bias_of(net, 1));
-        __syncthreads();
-        layer_hidden(xa, xb, net.w + 2 * kGuideDim * kGuideDim, bias_of(net, 2));
-        __syncthreads();
-        layer_output(xb, o32, net.w + 3 * kGuideDim * kGuideDim, bias_of(net, 3));
-        __syncthreads();
+        // re-read the weights from cache every tile: hoisting them out of the
+        // loop takes 128 registers and leaves one wave per SIMD
+        uint32_t opaque = 0;
+        asm volatile("" : "+s"(opaque));
+        GuideNet tile_net = net;
+        tile_net.w += opaque;
+        tile_net.w32 += opaque;
+        infer_forward(buf, tile_net);
         if (valid) {
-            float o[kGuideOut];
-            for (int k = 0; k < kGuideOut; k++) o[k] = o32[s * kLdOut + k];
-            GuideMixture m;
-            guide_mixture(o, m);
-            p.guide.mix[e] = m;
+            // lobe by lobe from the output row straight to the mixture record
+            const float* o = reinterpret_cast<const float*>(buf) + s * kLdInfer;
+            float mx, inv_sum;
+            guide_softmax(o, mx, inv_sum);
+            float* m = p.guide.mix + e;
+            const size_t st = p.guide.mix_stride;
+            for (int j = 0; j < kGuideLobes; j++) {
+                float mu[3], kappa, pi, c;
+                guide_lobe(o + 5 * j, mx, inv_sum, mu, kappa, pi, c);
+                m[(3 * j + 0) * st] = mu[0];
+                m[(3 * j + 1) * st] = mu[1];
+                m[(3 * j + 2) * st] = mu[2];
+                m[(24 + j) * st] = kappa;
+                m[(32 + j) * st] = pi;
+                m[(40 + j) * st] = c;
+            }
         }
         __syncthreads();
     }

@@ -32,11 +32,14 @@ constexpr int kGuideLayerParams = kGuideDim * kGuideDim + kGuideDim; // W [out][
 constexpr int kGuideParams = kGuideLayers * kGuideLayerParams;       // 16,640
 constexpr float kGuideLogKappaMin = -2.0f, kGuideLogKappaMax = 7.0f;
 
-// Mixture parameters of one shading point: 8 lobes of (mu xyz unit, kappa, pi).
+// Mixture parameters of one shading point: 8 lobes of (mu xyz unit, kappa,
+// pi) and c = log(pi kappa / (2 pi (1 - e^{-2 kappa}))), so that the density
+// is sum_j exp(c_j + kappa_j (mu_j . w - 1)).
 struct GuideMixture {
     float mu[kGuideLobes][3];
     float kappa[kGuideLobes];
     float pi[kGuideLobes];
+    float c[kGuideLobes];
 };
 
 // Training record written at a scatter vertex: the vertex, the sampled
@@ -68,28 +71,37 @@ struct GuideConfig {
 
 // ---------------------------------------------------------------- features
 
-// Input features of a shading point: normalised position, its sin/cos
-// frequency encoding, the normal and the outgoing direction (45 values; the
-// rest of the 64 inputs are zero).
-PT_HD void guide_features(const GuideConfig& g, V3 p, V3 n, V3 wo, float f[kGuideFeatures]) {
+// Input features of a shading point, handed to put(index, value): the
+// normalised position, its sin/cos frequency encoding, the normal and the
+// outgoing direction (45 values; the rest of the 64 inputs are zero).
+template <class Put>
+PT_HD void guide_features_each(const GuideConfig& g, V3 p, V3 n, V3 wo, Put put) {
     const float q[3] = {(p.x - g.lo[0]) * g.inv_extent[0], (p.y - g.lo[1]) * g.inv_extent[1],
                         (p.z - g.lo[2]) * g.inv_extent[2]};
     int k = 0;
-    for (int a = 0; a < 3; a++) f[k++] = q[a];
+    for (int a = 0; a < 3; a++) put(k++, q[a]);
     for (int a = 0; a < 3; a++) {
-        float scale = kPi;
+        // octave 0 directly, the others by doubling: sin 2x = 2 sin x cos x,
+        // cos 2x = 1 - 2 sin^2 x
+        float sn = sinf(kPi * q[a]), cs = cosf(kPi * q[a]);
         for (int o = 0; o < kGuideOctaves; o++) {
-            f[k++] = sinf(scale * q[a]);
-            f[k++] = cosf(scale * q[a]);
-            scale *= 2.0f;
+            put(k++, sn);
+            put(k++, cs);
+            const float s2 = 2.0f * sn * cs, c2 = 1.0f - 2.0f * sn * sn;
+            sn = s2;
+            cs = c2;
         }
     }
-    f[k++] = n.x;
-    f[k++] = n.y;
-    f[k++] = n.z;
-    f[k++] = wo.x;
-    f[k++] = wo.y;
-    f[k++] = wo.z;
+    put(k++, n.x);
+    put(k++, n.y);
+    put(k++, n.z);
+    put(k++, wo.x);
+    put(k++, wo.y);
+    put(k++, wo.z);
+}
+
+PT_HD void guide_features(const GuideConfig& g, V3 p, V3 n, V3 wo, float f[kGuideFeatures]) {
+    guide_features_each(g, p, n, wo, [&](int k, float v) { f[k] = v; });
 }
 
 // ---------------------------------------------------------------- vMF mixture
@@ -99,33 +111,66 @@ PT_HD float vmf_log_pdf(float kappa, float mu_dot_w) {
     return logf(kappa) - logf(2.0f * kPi) - log1pf(-expf(-2.0f * kappa)) + kappa * (mu_dot_w - 1.0f);
 }
 
-// Mixture parameters from the 40 raw network outputs.
-PT_HD void guide_mixture(const float o[kGuideOut], GuideMixture& m) {
-    float mx = -1e30f;
+// Softmax shift and normaliser of the 8 lobe logits o[5j+4].
+PT_HD void guide_softmax(const float* o, float& mx, float& inv_sum) {
+    mx = -1e30f;
     for (int j = 0; j < kGuideLobes; j++) mx = fmaxf(mx, o[5 * j + 4]);
     float sum = 0.0f;
-    for (int j = 0; j < kGuideLobes; j++) {
-        const float ux = o[5 * j], uy = o[5 * j + 1], uz = o[5 * j + 2];
-        const float len = sqrtf(ux * ux + uy * uy + uz * uz);
-        const float inv = 1.0f / fmaxf(len, 1e-12f);
-        m.mu[j][0] = len > 1e-12f ? ux * inv : 0.0f;
-        m.mu[j][1] = len > 1e-12f ? uy * inv : 0.0f;
-        m.mu[j][2] = len > 1e-12f ? uz * inv : 1.0f;
-        m.kappa[j] = expf(clampf(o[5 * j + 3], kGuideLogKappaMin, kGuideLogKappaMax));
-        m.pi[j] = expf(o[5 * j + 4] - mx);
-        sum += m.pi[j];
-    }
-    const float inv_sum = 1.0f / sum;
-    for (int j = 0; j < kGuideLobes; j++) m.pi[j] *= inv_sum;
+    for (int j = 0; j < kGuideLobes; j++) sum += expf(o[5 * j + 4] - mx);
+    inv_sum = 1.0f / sum;
 }
 
-PT_HD float guide_pdf(const GuideMixture& m, V3 w) {
+// Lobe j of the mixture from its 5 raw outputs o5 = o + 5j.
+PT_HD void guide_lobe(const float* o5, float mx, float inv_sum, float mu[3], float& kappa, float& pi, float& c) {
+    const float ux = o5[0], uy = o5[1], uz = o5[2];
+    const float len = sqrtf(ux * ux + uy * uy + uz * uz);
+    const float inv = 1.0f / fmaxf(len, 1e-12f);
+    mu[0] = len > 1e-12f ? ux * inv : 0.0f;
+    mu[1] = len > 1e-12f ? uy * inv : 0.0f;
+    mu[2] = len > 1e-12f ? uz * inv : 1.0f;
+    kappa = expf(clampf(o5[3], kGuideLogKappaMin, kGuideLogKappaMax));
+    pi = expf(o5[4] - mx) * inv_sum;
+    c = logf(fmaxf(pi, 1e-30f)) + vmf_log_pdf(kappa, 1.0f);
+}
+
+// Mixture parameters from the 40 raw network outputs.
+PT_HD void guide_mixture(const float o[kGuideOut], GuideMixture& m) {
+    float mx, inv_sum;
+    guide_softmax(o, mx, inv_sum);
+    for (int j = 0; j < kGuideLobes; j++) guide_lobe(o + 5 * j, mx, inv_sum, m.mu[j], m.kappa[j], m.pi[j], m.c[j]);
+}
+
+// Mixtures of a whole bounce stored component-major: component k of entry e
+// at p[k * stride + e] (mu 3j+a, kappa 24+j, pi 32+j, c 40+j), so that a wave
+// reads and writes each component of 64 consecutive entries in one line.
+constexpr int kGuideMixComponents = 48;
+struct MixSoA {
+    const float* p;          // component 0 of this entry
+    uint32_t stride;
+    PT_HD float mu(int j, int a) const { return p[(size_t)(3 * j + a) * stride]; }
+    PT_HD float kappa(int j) const { return p[(size_t)(24 + j) * stride]; }
+    PT_HD float pi(int j) const { return p[(size_t)(32 + j) * stride]; }
+    PT_HD float c(int j) const { return p[(size_t)(40 + j) * stride]; }
+};
+
+// Mixture density at unit direction w.
+template <class M>
+PT_HD float guide_pdf(const M& m, V3 w) {
     float q = 0.0f;
     for (int j = 0; j < kGuideLobes; j++) {
-        const float d = m.mu[j][0] * w.x + m.mu[j][1] * w.y + m.mu[j][2] * w.z;
-        q += m.pi[j] * expf(vmf_log_pdf(m.kappa[j], d));
+        const float d = m.mu(j, 0) * w.x + m.mu(j, 1) * w.y + m.mu(j, 2) * w.z;
+        q += expf(m.c(j) + m.kappa(j) * (d - 1.0f));
     }
     return q;
+}
+
+// The mixture folded onto the hemisphere of n: directions below the surface
+// are mirrored across its plane, so the density at w (w . n >= 0) is
+// q(w) + q(w - 2 (w . n) n).
+PT_HD V3 guide_mirror(V3 w, V3 n) { return w - n * (2.0f * dot(w, n)); }
+template <class M>
+PT_HD float guide_pdf_folded(const M& m, V3 w, V3 n) {
+    return dot(w, n) < 0.0f ? 0.0f : guide_pdf(m, w) + guide_pdf(m, guide_mirror(w, n));
 }
 
 // Orthonormal basis around a unit vector (Duff et al. 2017, "Building an
@@ -140,22 +185,29 @@ PT_HD void guide_basis(V3 n, V3& t, V3& b) {
 
 // Direction from the mixture: u0 picks the lobe, (u1, u2) the direction
 // within it (Jakob 2012: cos = 1 + log(u + (1-u) e^{-2 kappa}) / kappa).
-PT_HD V3 guide_sample(const GuideMixture& m, float u0, float u1, float u2) {
+template <class M>
+PT_HD V3 guide_sample(const M& m, float u0, float u1, float u2) {
     int j = 0;
-    float acc = m.pi[0];
+    float acc = m.pi(0);
     while (j < kGuideLobes - 1 && u0 >= acc) {
         j++;
-        acc += m.pi[j];
+        acc += m.pi(j);
     }
-    const float k = m.kappa[j];
+    const float k = m.kappa(j);
     const float u = fmaxf(u1, 1e-7f);
     const float c = clampf(1.0f + logf(u + (1.0f - u) * expf(-2.0f * k)) / k, -1.0f, 1.0f);
     const float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
     const float phi = 2.0f * kPi * u2;
-    const V3 mu = v3(m.mu[j][0], m.mu[j][1], m.mu[j][2]);
+    const V3 mu = v3(m.mu(j, 0), m.mu(j, 1), m.mu(j, 2));
     V3 t, b;
     guide_basis(mu, t, b);
     return normalize(mu * c + t * (s * cosf(phi)) + b * (s * sinf(phi)));
+}
+
+template <class M>
+PT_HD V3 guide_sample_folded(const M& m, V3 n, float u0, float u1, float u2) {
+    const V3 w = guide_sample(m, u0, u1, u2);
+    return dot(w, n) < 0.0f ? guide_mirror(w, n) : w;
 }
 
 // d(loss)/d(raw outputs) of one record for loss = -weight * log q(wi).

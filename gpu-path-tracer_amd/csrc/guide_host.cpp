@@ -13,7 +13,7 @@ static_assert(PT_GUIDE_PARAMS == kGuideParams, "parameter count");
 static_assert(PT_GUIDE_INPUTS == kGuideDim && PT_GUIDE_OUTPUTS == kGuideDim, "layer width");
 
 namespace {
-constexpr uint32_t kDefaultRecordCapacity = 1u << 22;
+constexpr uint64_t kMaxRecordCapacity = 1ull << 26;
 }
 
 // Guide box from the bounds of scene 0 (the union of the root's child boxes),
@@ -54,9 +54,9 @@ GuideDev Renderer::guide_dev() const {
     g.grad = g_grad_.as<float>();
     g.m = g_m_.as<float>();
     g.v = g_v_.as<float>();
-    g.mix = g_mix_.as<GuideMixture>();
+    g.mix = g_mix_.as<float>();
+    g.mix_stride = g_pool_;
     g.recs = g_recs_.as<GuideRecord>();
-    g.rec_count = g_rec_count_.as<unsigned int>();
     g.rec_capacity = g_rec_capacity_;
     g.head[0] = g_head_[0].as<uint32_t>();
     g.head[1] = g_head_[1].as<uint32_t>();
@@ -92,7 +92,7 @@ pt_status Renderer::ensure_guide_pool(uint32_t cap) {
         g_head_[0].release();
         g_head_[1].release();
         g_sh_head_.release();
-        PT_TRY_HIP(g_mix_.allocate(sizeof(GuideMixture) * (size_t)pool_capacity_), "guide mixtures");
+        PT_TRY_HIP(g_mix_.allocate(sizeof(float) * kGuideMixComponents * (size_t)pool_capacity_), "guide mixtures");
         PT_TRY_HIP(g_head_[0].allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");
         PT_TRY_HIP(g_head_[1].allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");
         PT_TRY_HIP(g_sh_head_.allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");
@@ -104,7 +104,6 @@ pt_status Renderer::ensure_guide_pool(uint32_t cap) {
         PT_TRY_HIP(g_recs_.allocate(sizeof(GuideRecord) * (size_t)cap), "guide records");
         g_rec_capacity_ = cap;
     }
-    if (g_rec_count_.bytes() == 0) PT_TRY_HIP(g_rec_count_.allocate(64), "guide record count");
     return PT_OK;
 }
 
@@ -267,7 +266,9 @@ pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& c
         guide_box_from_scene();
         PT_TRY(ensure_guide());
         if (gs.reset) PT_TRY(GuideReset(gs.seed));
-        PT_TRY(ensure_guide_pool(gs.record_capacity ? gs.record_capacity : kDefaultRecordCapacity));
+        // default: room for four scatter vertices per sample of a pass
+        const uint64_t auto_cap = std::min<uint64_t>(4ull * views[0].sample_count, kMaxRecordCapacity);
+        PT_TRY(ensure_guide_pool(gs.record_capacity ? gs.record_capacity : (uint32_t)std::max<uint64_t>(auto_cap, 1024)));
         PT_TRY_HIP(g_loss_.allocate(sizeof(float) * std::max<size_t>(gs.train_steps, 16)), "guide loss");
     }
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -291,7 +292,6 @@ pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& c
         opt.clear = pass == 0;
         opt.guided = learn;
         opt.guide_fraction = trained ? gs.guide_fraction : 0.0f;
-        if (learn) PT_TRY_HIP(launch_zero_u32(stream_, g_rec_count_.as<unsigned int>(), 1), "guide record count");
         pt_frame_stats ps{};
         PT_TRY(run_views(views, depth, PT_FRAME_DEFAULT, &ps, nullptr, opt));
         total.render_ms += ps.render_ms;
@@ -303,9 +303,8 @@ pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& c
         if (!learn || pass + 1 == gs.passes || gs.train_steps == 0) continue;
 
         // train on this pass's records
-        uint32_t count = 0;
-        PT_TRY_HIP(hipMemcpy(&count, g_rec_count_.as<void>(), sizeof(uint32_t), hipMemcpyDeviceToHost), "guide record count");
-        count = std::min(count, g_rec_capacity_);
+        // run_views left the run's final counters in h_ctr_ring_[0]
+        const uint32_t count = std::min(h_ctr_ring_[0].records, g_rec_capacity_);
         gst.records = count;
         if (count == 0) continue;
         const GuideDev g = guide_dev();

@@ -805,6 +805,7 @@ __global__ void k_frame_begin(PathState p) {
         c->iterations = 0;
         c->invalid_pixels = 0;
         c->stack_overflow = 0;
+        c->records = 0;
         c->launched = gen;
         c->ext_rays = 0;
         c->shadow_rays = 0;
@@ -902,6 +903,9 @@ __global__ void k_prepare(PathState p, int cur) {
         c->ext_rays += traced;
         c->shadow_rays += c->shadow;
         if (traced) c->iterations++;
+        // the paths that continued this bounce own record slots
+        // [records, records + ray[nxt]) in queue order (guided shading)
+        c->records += c->ray[nxt];
         c->ray[nxt] += nr;
         c->launched += nr;
         c->ray[cur] = 0;
@@ -1034,6 +1038,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
     const int nxt = cur ^ 1;
     const uint32_t c0 = c->queue[0], c1 = c->queue[1], c2 = c->queue[2], c3 = c->queue[3];
     const uint32_t e1 = c0, e2 = c0 + c1, e3 = e2 + c2, total = e3 + c3;
+    const uint32_t rec_base = GUIDED ? c->records : 0u;
 
     for (uint32_t bbase = blockIdx.x * 256u; bbase < total; bbase += gridDim.x * 256u) {
         const uint32_t e = bbase + threadIdx.x;
@@ -1047,8 +1052,9 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
         float cur_ior = 1.0f;
         float scene_bits = 0.0f;
         uint32_t head = kInvalid;          // newest guide record of the path
-        bool want_rec = false;
-        GuideRecord rec;
+        bool rec_valid = false;            // the continuing path writes a training record
+        V3 rec_p = v3(0.0f), rec_n = v3(0.0f), rec_wo = v3(0.0f);
+        float rec_pdf = 0.0f, rec_cos = 0.0f;
         V3 add_L = v3(0.0f);
         bool add = false;
         uint32_t pixel = 0;
@@ -1102,12 +1108,11 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                 // guide: draw slot 7 picks guide or BSDF; the mixture of this entry
                 const float gfrac = GUIDED ? p.guide.cfg.guide_fraction : 0.0f;
                 float u7 = 1.0f;
-                GuideMixture gm;
+                const MixSoA gm{p.guide.mix + e, p.guide.mix_stride};
                 if (GUIDED && gfrac > 0.0f && q < 2) {
                     Draws d7 = dr;
                     for (int k = 0; k < 7; k++) d7.state = d7.state * kPcgMult + d7.inc;
                     u7 = d7.next();
-                    gm = p.guide.mix[e];
                 }
                 const bool by_guide = GUIDED && gfrac > 0.0f && u7 < gfrac;
                 float pdf_used = 0.0f;
@@ -1140,13 +1145,13 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                             }
                         }
                     }
-                    wi = by_guide ? guide_sample(gm, u2, u0, u1) : fr.from_local(cosine_hemisphere(u0, u1));
+                    wi = by_guide ? guide_sample_folded(gm, h.gn, u2, u0, u1) : fr.from_local(cosine_hemisphere(u0, u1));
                     const float ci = fmaxf(dot(fr.n, wi), 0.0f);
                     const float gci = fmaxf(dot(h.gn, wi), 0.0f);
                     if (!(ci <= 0.0f || gci <= 0.0f)) {
                         if (GUIDED && gfrac > 0.0f) {
                             // one-sample MIS: f cos / (a q + (1-a) cos/pi)
-                            const float pdf = gfrac * guide_pdf(gm, wi) + (1.0f - gfrac) * ci * kInvPi;
+                            const float pdf = gfrac * guide_pdf_folded(gm, wi, h.gn) + (1.0f - gfrac) * ci * kInvPi;
                             if (pdf > 1e-8f) {
                                 weight = albedo * (ci * kInvPi / pdf);
                                 pdf_used = pdf;
@@ -1201,7 +1206,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                         if (!(wsum <= 1e-6f)) {
                             const float pd = dw / wsum, ps = sw / wsum;
                             V3 wl;
-                            if (by_guide) wl = fr.to_local(guide_sample(gm, u2, u0, u1));
+                            if (by_guide) wl = fr.to_local(guide_sample_folded(gm, h.gn, u2, u0, u1));
                             else if (u2 < pd) wl = cosine_hemisphere(u0, u1);
                             else wl = reflect(-wo, ggx_sample_wm(wo, alpha, u0, u1));
                             const float ci = wl.z, co = wo.z;
@@ -1217,7 +1222,8 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                                         const float pdfs = ggx_dv(wo, wm, alpha) / (4.0f * owm);
                                         const V3 ww = fr.from_local(wl);
                                         float pdf = pd * pdfd + ps * pdfs;
-                                        if (GUIDED && gfrac > 0.0f) pdf = gfrac * guide_pdf(gm, ww) + (1.0f - gfrac) * pdf;
+                                        if (GUIDED && gfrac > 0.0f)
+                                            pdf = gfrac * guide_pdf_folded(gm, ww, h.gn) + (1.0f - gfrac) * pdf;
                                         if (!(pdf <= 1e-8f)) {
                                             if (!(dot(ww, h.gn) <= 0.0f)) {
                                                 wi = ww;
@@ -1248,20 +1254,12 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                         new_flags = k.depth | (m.y & 0x00FFFF00u) | (new_delta ? kDeltaBit : 0u);
                         to_ray = true;
                         if (GUIDED && q < 2) {
-                            want_rec = true;
-                            const V3 wo3 = v3(0.0f) - rdir;
-                            rec.p[0] = h.p.x, rec.p[1] = h.p.y, rec.p[2] = h.p.z;
-                            rec.prev = head;
-                            rec.n[0] = h.gn.x, rec.n[1] = h.gn.y, rec.n[2] = h.gn.z;
-                            rec.pdf = pdf_used;
-                            rec.wo[0] = wo3.x, rec.wo[1] = wo3.y, rec.wo[2] = wo3.z;
-                            rec.cos_i = fabsf(dot(fr.n, wi));
-                            rec.wi[0] = wi.x, rec.wi[1] = wi.y, rec.wi[2] = wi.z;
-                            rec.pad = 0.0f;
-                            rec.t_after[0] = k.thr.x, rec.t_after[1] = k.thr.y, rec.t_after[2] = k.thr.z;
-                            rec.pad2 = 0.0f;
-                            rec.li[0] = rec.li[1] = rec.li[2] = 0.0f;
-                            rec.pad3 = 0.0f;
+                            rec_valid = true;
+                            rec_p = h.p;
+                            rec_n = h.gn;
+                            rec_wo = v3(0.0f) - rdir;
+                            rec_pdf = pdf_used;
+                            rec_cos = fabsf(dot(fr.n, wi));
                         }
                     }
                 }
@@ -1275,25 +1273,34 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
         }
         uint32_t ray_pos, sh_pos;
         block_reserve2(to_ray, so.emit, &c->ray[nxt], &c->shadow, ray_pos, sh_pos);
-        uint32_t new_head = head;
         if (GUIDED) {
-            // wave-aggregated slot in the record buffer
-            const uint64_t mask = __ballot(want_rec);
-            uint32_t base = 0;
-            if (mask) {
-                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-                if (lane_id() == leader) base = atomicAdd(p.guide.rec_count, (uint32_t)__popcll(mask));
-                base = readlane_u32(base, leader);
-            }
-            if (want_rec) {
-                const uint32_t ridx = base + mbcnt(mask);
-                if (ridx < p.guide.rec_capacity) {
-                    p.guide.recs[ridx] = rec;
-                    new_head = ridx;
-                }
-            }
             if (so.emit) p.guide.sh_head[sh_pos] = head;
-            if (to_ray) (nxt ? p.guide.head[1] : p.guide.head[0])[ray_pos] = new_head;
+            if (to_ray) {
+                // record slot = records of earlier bounces + queue position; a
+                // mirror continuation leaves a zero record (no weight) in its slot
+                const uint32_t ridx = rec_base + ray_pos;
+                uint32_t new_head = head;
+                if (ridx < p.guide.rec_capacity) {
+                    float4* rp = reinterpret_cast<float4*>(p.guide.recs + ridx);
+                    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (rec_valid) {
+                        rp[0] = make_float4(rec_p.x, rec_p.y, rec_p.z, __uint_as_float(head));
+                        rp[1] = make_float4(rec_n.x, rec_n.y, rec_n.z, rec_pdf);
+                        rp[2] = make_float4(rec_wo.x, rec_wo.y, rec_wo.z, rec_cos);
+                        rp[3] = make_float4(new_d.x, new_d.y, new_d.z, 0.0f);
+                        rp[4] = make_float4(new_thr.x, new_thr.y, new_thr.z, 0.0f);
+                        new_head = ridx;
+                    } else {
+                        rp[0] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kInvalid));
+                        rp[1] = z;
+                        rp[2] = z;
+                        rp[3] = z;
+                        rp[4] = z;
+                    }
+                    rp[5] = z;
+                }
+                (nxt ? p.guide.head[1] : p.guide.head[0])[ray_pos] = new_head;
+            }
         }
         if (so.emit) {
             p.sh_o[sh_pos] = f4(so.o, so.tmax);

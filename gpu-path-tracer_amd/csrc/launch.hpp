@@ -34,9 +34,10 @@ struct GuideDev {
     float* grad;
     float* m;                       // Adam moments
     float* v;
-    GuideMixture* mix;              // per diffuse/GGX entry of the current bounce ([diffuse | ggx] order)
-    GuideRecord* recs;              // training records of the current pass
-    unsigned int* rec_count;
+    float* mix;                     // mixtures of the diffuse/GGX entries of the bounce ([diffuse | ggx]
+                                    // order), component-major with stride mix_stride (MixSoA)
+    uint32_t mix_stride;
+    GuideRecord* recs;              // training records of the current run, Counters::records of them
     uint32_t rec_capacity;
     uint32_t* head[2];              // per path position: newest record of the path, kInvalid if none
     uint32_t* sh_head;              // per shadow entry: newest record before its vertex

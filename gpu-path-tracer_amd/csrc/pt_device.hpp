@@ -265,7 +265,8 @@ struct Counters {
     uint32_t iterations;
     uint32_t invalid_pixels;
     uint32_t stack_overflow;
-    uint32_t pad[2];
+    uint32_t records;           // guide records of this run: record slots [0, records) are written
+    uint32_t pad;
     unsigned long long launched;
     unsigned long long ext_rays;
     unsigned long long shadow_rays;

@@ -368,7 +368,7 @@ typedef struct pt_guide_settings {
     float learning_rate;
     float guide_fraction;     /* probability of sampling the guide; 0 = plain BSDF sampling */
     float weight_max;         /* clamp of a record's loss weight luminance(L) cos / pdf */
-    uint32_t record_capacity; /* training records kept per pass; 0 = 2^22 */
+    uint32_t record_capacity; /* training records kept per pass; 0 = 4 per sample of a pass, at most 2^26 */
     uint32_t reset;           /* re-initialise the network before the first pass */
     uint64_t seed;            /* network initialisation and batch selection */
 } pt_guide_settings;
