@@ -262,12 +262,68 @@ def run_serve(a):
     r.close()
 
 
+def run_guide(a):
+    """Config 5: the 200k-triangle two-room scene lit through a 0.5 m gap,
+    1280x720, online-trained guiding (one-sample MIS with the BSDF) against
+    plain BSDF sampling at equal samples; the time to equal error is the
+    ratio of error x time (Monte Carlo error falls as 1/time)."""
+    import numpy as np
+    import ptcore
+    import scenes
+    room = scenes.guiding_room(200_000)
+    w, h = 1280, 720
+    r = ptcore.Renderer()
+    r.InitializeRenderingBuffers(w, h)
+    r.InitializeSceneBuffers(room)
+    out_cfgs = []
+    for nee in (1, 0):
+        base = dict(width=w, height=h, path_depth=8, use_nee=nee, rr_start_depth=3)
+        t0 = time.time()
+        ref_st, _ = r.RenderGuided(dict(base, samples_per_pixel=a.guide_ref_spp, seed=99), room.view, passes=4,
+                                   guide_fraction=0.0)
+        ref = r.GetFrameHdr(w, h)[..., :3].copy()
+        log(f"guide: NEE {nee} reference {a.guide_ref_spp} spp in {time.time() - t0:.1f}s")
+        m2 = float(ref.mean()) ** 2
+        spp = a.guide_spp
+        st = dict(base, samples_per_pixel=spp, seed=7)
+        r.RenderGuided(st, room.view, passes=4, guide_fraction=0.0)            # warm-up
+        sp, _ = r.RenderGuided(st, room.view, passes=4, guide_fraction=0.0)
+        e_p = float(np.mean((r.GetFrameHdr(w, h)[..., :3] - ref) ** 2)) / m2
+        sg, gs = r.RenderGuided(st, room.view, passes=4, train_steps=a.guide_steps, batch=16384, learning_rate=3e-3,
+                                guide_fraction=0.5, reset=True, seed=1)
+        e_g = float(np.mean((r.GetFrameHdr(w, h)[..., :3] - ref) ** 2)) / m2
+        ratio = e_g * sg["render_ms"] / (e_p * sp["render_ms"])
+        log(f"guide: NEE {nee} {spp} spp: plain {sp['render_ms']:.1f} ms nMSE {e_p:.4g}; guided {sg['render_ms']:.1f} ms "
+            f"(train {gs['train_ms']:.1f} ms, {gs['steps']} Adam steps, {gs['records']} records) nMSE {e_g:.4g}; "
+            f"time to equal error {ratio:.3f}x")
+        out_cfgs.append({"use_nee": nee, "spp": spp, "plain_ms": round(sp["render_ms"], 2), "plain_nmse": e_p,
+                         "guided_ms": round(sg["render_ms"], 2), "guided_train_ms": round(gs["train_ms"], 2),
+                         "guided_nmse": e_g, "error_ratio_equal_samples": round(e_g / e_p, 4),
+                         "time_to_equal_error_ratio": round(ratio, 4),
+                         "plain_mrays_s": round(_rays(sp) / sp["render_ms"] / 1e3, 1),
+                         "guided_mrays_s": round(_rays(sg) / sg["render_ms"] / 1e3, 1)})
+    out = {"metric": "time to equal error, guided / plain BSDF sampling (lower is better)",
+           "value": out_cfgs[0]["time_to_equal_error_ratio"], "unit": "x", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": out_cfgs[0]["guided_ms"], "higher_is_better": False, "scaling": "strong",
+           "vs_baseline": None, "dtype": "bf16 MLP / f32 render", "data": "synthetic",
+           "config": {"workload": f"guide: two rooms, {room.triangle_count} triangles, light behind a wall with a "
+                                  f"0.5 m gap, 1280x720, {a.guide_spp} spp in 4 passes, depth 8; nMSE against "
+                                  f"{a.guide_ref_spp} spp",
+                      "network": "MLP 64-64-64-64 BF16 (FP32 accumulate), 8 vMF lobes, one-sample MIS alpha 0.5"},
+           "runs": out_cfgs}
+    print(json.dumps(out), flush=True)
+    r.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=["production", "cornell", "batch", "serve"], default="production")
+    ap.add_argument("--config", choices=["production", "cornell", "batch", "serve", "guide"], default="production")
+    ap.add_argument("--guide-spp", type=int, default=256)
+    ap.add_argument("--guide-ref-spp", type=int, default=4096)
+    ap.add_argument("--guide-steps", type=int, default=256)
     ap.add_argument("--tuning", default=None, help="JSON written by autotune.py")
     ap.add_argument("--serve-scale", type=float, default=1.0, help="scale of the serving rooms' triangle counts")
     ap.add_argument("--serve-requests", type=int, default=48)
@@ -292,6 +348,9 @@ def main():
         return
     if a.config == "serve":
         run_serve(a)
+        return
+    if a.config == "guide":
+        run_guide(a)
         return
     import ptcore
     import scenes
