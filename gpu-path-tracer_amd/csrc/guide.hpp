@@ -66,7 +66,7 @@ struct GuideConfig {
     float lo[3];
     float guide_fraction;   // probability of sampling the guide (alpha)
     float inv_extent[3];
-    uint32_t enabled;       // 0: plain BSDF sampling
+    uint32_t rec_mask;      // a path writes training records when (sample index & rec_mask) == 0
 };
 
 // ---------------------------------------------------------------- features

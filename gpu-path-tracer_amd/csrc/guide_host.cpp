@@ -14,7 +14,43 @@ static_assert(PT_GUIDE_INPUTS == kGuideDim && PT_GUIDE_OUTPUTS == kGuideDim, "la
 
 namespace {
 constexpr uint64_t kMaxRecordCapacity = 1ull << 26;
+constexpr uint64_t kDefaultRecordCapacity = 1ull << 24;
+constexpr uint64_t kRecordsPerSample = 4;    // expected scatter vertices of a recording sample
+
+// Samples per pixel of each pass: doubling from pass to pass (the later, better
+// guided passes carry most of the image), or an even split when there are too
+// few samples; each pass is a list of chunks with a square sample count (the
+// stratified pixel grid of Renderer.cpp:366-367), smallest first.
+std::vector<std::vector<uint32_t>> pass_chunks(uint32_t spp, uint32_t passes) {
+    std::vector<uint32_t> n(passes);
+    const uint64_t parts = passes < 32 ? (1ull << passes) - 1ull : ~0ull;
+    uint64_t used = 0;
+    if ((uint64_t)spp >= parts) {
+        for (uint32_t k = 0; k + 1 < passes; k++) {
+            n[k] = (uint32_t)((uint64_t)spp * (1ull << k) / parts);
+            used += n[k];
+        }
+    } else {
+        for (uint32_t k = 0; k + 1 < passes; k++) {
+            n[k] = spp / passes;
+            used += n[k];
+        }
+    }
+    n[passes - 1] = (uint32_t)(spp - used);
+    std::vector<std::vector<uint32_t>> out(passes);
+    for (uint32_t k = 0; k < passes; k++) {
+        uint32_t left = n[k];
+        while (left > 0) {
+            uint32_t a = (uint32_t)std::sqrt((double)left);
+            while ((uint64_t)(a + 1) * (a + 1) <= left) a++;
+            while ((uint64_t)a * a > left) a--;
+            out[k].insert(out[k].begin(), a * a);
+            left -= a * a;
+        }
+    }
+    return out;
 }
+}  // namespace
 
 // Guide box from the bounds of scene 0 (the union of the root's child boxes),
 // unless the caller set one.
@@ -61,6 +97,8 @@ GuideDev Renderer::guide_dev() const {
     g.head[0] = g_head_[0].as<uint32_t>();
     g.head[1] = g_head_[1].as<uint32_t>();
     g.sh_head = g_sh_head_.as<uint32_t>();
+    g.rr_scale[0] = g_rr_scale_[0].as<float>();
+    g.rr_scale[1] = g_rr_scale_[1].as<float>();
     g.cfg = g_cfg_;
     return g;
 }
@@ -92,10 +130,14 @@ pt_status Renderer::ensure_guide_pool(uint32_t cap) {
         g_head_[0].release();
         g_head_[1].release();
         g_sh_head_.release();
+        g_rr_scale_[0].release();
+        g_rr_scale_[1].release();
         PT_TRY_HIP(g_mix_.allocate(sizeof(float) * kGuideMixComponents * (size_t)pool_capacity_), "guide mixtures");
         PT_TRY_HIP(g_head_[0].allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");
         PT_TRY_HIP(g_head_[1].allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");
         PT_TRY_HIP(g_sh_head_.allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");
+        PT_TRY_HIP(g_rr_scale_[0].allocate(sizeof(float) * (size_t)pool_capacity_), "guide roulette scales");
+        PT_TRY_HIP(g_rr_scale_[1].allocate(sizeof(float) * (size_t)pool_capacity_), "guide roulette scales");
         g_pool_ = pool_capacity_;
     }
     if (cap != g_rec_capacity_ || g_recs_.bytes() == 0) {
@@ -245,30 +287,32 @@ pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& c
     if (learn && gs.train_steps > 0 && !(gs.learning_rate > 0.0f))
         return fail(PT_ERROR_INVALID_ARGUMENT, "learning rate must be positive");
 
-    // one view per pass, the samples per pixel split evenly
+    // the clamped (square) samples per pixel, spread over the passes' chunks
     pt_view v{};
     v.scene = 0;
     v.settings = s;
-    v.settings.samples_per_pixel = std::max(1u, s.samples_per_pixel / gs.passes);
     v.camera = camera;
     v.state = state;
     std::vector<ViewRecord> views(1);
     uint32_t depth = 0;
     PT_TRY(make_view(v, views[0], depth));
-    const uint32_t spp_pass = views[0].spp;
-    views[0].inv_spp = 1.0f / (float)(spp_pass * gs.passes);
+    const uint32_t spp_total = views[0].spp;
+    if (spp_total < gs.passes) return fail(PT_ERROR_INVALID_ARGUMENT, "fewer samples per pixel than passes");
+    const std::vector<std::vector<uint32_t>> chunks = pass_chunks(spp_total, gs.passes);
     set_frame_size(views[0].width, views[0].height);
-    last_spp_ = spp_pass * gs.passes;
+    last_spp_ = spp_total;
     hdr_valid_ = false;
+    const uint64_t pixels = (uint64_t)views[0].width * views[0].height;
 
     PT_TRY(ensure_pool());
+    uint32_t rec_capacity = 0;
     if (learn) {
         guide_box_from_scene();
         PT_TRY(ensure_guide());
         if (gs.reset) PT_TRY(GuideReset(gs.seed));
-        // default: room for four scatter vertices per sample of a pass
-        const uint64_t auto_cap = std::min<uint64_t>(4ull * views[0].sample_count, kMaxRecordCapacity);
-        PT_TRY(ensure_guide_pool(gs.record_capacity ? gs.record_capacity : (uint32_t)std::max<uint64_t>(auto_cap, 1024)));
+        rec_capacity = gs.record_capacity ? (uint32_t)std::min<uint64_t>(gs.record_capacity, kMaxRecordCapacity)
+                                          : (uint32_t)kDefaultRecordCapacity;
+        PT_TRY(ensure_guide_pool(std::max<uint32_t>(rec_capacity, 1024)));
         PT_TRY_HIP(g_loss_.allocate(sizeof(float) * std::max<size_t>(gs.train_steps, 16)), "guide loss");
     }
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -286,24 +330,34 @@ pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& c
     pt_frame_stats total{};
     bool trained = learn && g_step_ > 0 && !gs.reset;
     float first_loss = 0.0f, last_loss = 0.0f;
+    uint32_t chunk_index = 0;
     for (uint32_t pass = 0; pass < gs.passes; pass++) {
-        views[0].seed = s.seed + 0x9E3779B97F4A7C15ull * pass;
-        RunOptions opt;
-        opt.clear = pass == 0;
-        opt.guided = learn;
-        opt.guide_fraction = trained ? gs.guide_fraction : 0.0f;
-        pt_frame_stats ps{};
-        PT_TRY(run_views(views, depth, PT_FRAME_DEFAULT, &ps, nullptr, opt));
-        total.render_ms += ps.render_ms;
-        total.extension_rays += ps.extension_rays;
-        total.shadow_rays += ps.shadow_rays;
-        total.samples += ps.samples;
-        total.iterations += ps.iterations;
-        gst.render_ms += ps.render_ms;
+        for (size_t ci = 0; ci < chunks[pass].size(); ci++, chunk_index++) {
+            v.settings.samples_per_pixel = chunks[pass][ci];
+            PT_TRY(make_view(v, views[0], depth));
+            views[0].seed = s.seed + 0x9E3779B97F4A7C15ull * chunk_index;
+            views[0].inv_spp = 1.0f / (float)spp_total;
+            RunOptions opt;
+            opt.clear = chunk_index == 0;
+            opt.guided = learn;
+            opt.guide_fraction = trained ? gs.guide_fraction : 0.0f;
+            // training records from one sample in 2^rec_shift, so that the
+            // expected vertices of the chunk fit the record buffer
+            const uint64_t want = (uint64_t)chunks[pass][ci] * pixels * kRecordsPerSample;
+            while (opt.rec_shift < 31 && (want >> opt.rec_shift) > rec_capacity) opt.rec_shift++;
+            pt_frame_stats ps{};
+            PT_TRY(run_views(views, depth, PT_FRAME_DEFAULT, &ps, nullptr, opt));
+            total.render_ms += ps.render_ms;
+            total.extension_rays += ps.extension_rays;
+            total.shadow_rays += ps.shadow_rays;
+            total.samples += ps.samples;
+            total.iterations += ps.iterations;
+            gst.render_ms += ps.render_ms;
+        }
         if (!learn || pass + 1 == gs.passes || gs.train_steps == 0) continue;
 
-        // train on this pass's records
-        // run_views left the run's final counters in h_ctr_ring_[0]
+        // train on the records of the pass's last (largest) chunk; run_views
+        // left the run's final counters in h_ctr_ring_[0]
         const uint32_t count = std::min(h_ctr_ring_[0].records, g_rec_capacity_);
         gst.records = count;
         if (count == 0) continue;

@@ -721,14 +721,16 @@ struct Continuation {
     V3 thr;
 };
 
-// ComputePathContinuation, MaterialEval.cu:129-146
+// ComputePathContinuation, MaterialEval.cu:129-146.  rr_scale multiplies the
+// throughput that sets the survival probability; it is exactly 1 for plain
+// BSDF sampling, which keeps the reference's roulette bit for bit.
 __device__ __forceinline__ Continuation continue_path(uint32_t depth, V3 thr, uint32_t limit, uint32_t rr_start,
-                                                      V3 weight, float rr) {
+                                                      V3 weight, float rr, float rr_scale = 1.0f) {
     const uint32_t next = depth + 1u;
     if (next >= limit) return Continuation{true, next, v3(0.0f)};
     const V3 nt = thr * weight;
     if (next < rr_start) return Continuation{false, next, nt};
-    const float q = clampf(maxc(nt), 0.0f, 1.0f);
+    const float q = clampf(maxc(nt) * rr_scale, 0.0f, 1.0f);
     if (rr > q) return Continuation{true, next, v3(0.0f)};
     return Continuation{false, next, nt / q};
 }
@@ -890,7 +892,10 @@ __global__ void k_generate(PathState p, int init, int nxt) {
         rd[pos] = f4(d, kMaxT);
         th[pos] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
         me[pos] = make_uint4(pixel, kDeltaBit | (v << 8), (uint32_t)sample, (uint32_t)(sample >> 32));
-        if (p.guide.head[0]) (nxt ? p.guide.head[1] : p.guide.head[0])[pos] = kInvalid;
+        if (p.guide.head[0]) {
+            (nxt ? p.guide.head[1] : p.guide.head[0])[pos] = kInvalid;
+            (nxt ? p.guide.rr_scale[1] : p.guide.rr_scale[0])[pos] = 1.0f;
+        }
     }
 }
 
@@ -907,9 +912,6 @@ __global__ void k_prepare(PathState p, int cur) {
         c->ext_rays += traced;
         c->shadow_rays += c->shadow;
         if (traced) c->iterations++;
-        // the paths that continued this bounce own record slots
-        // [records, records + ray[nxt]) in queue order (guided shading)
-        c->records += c->ray[nxt];
         c->ray[nxt] += nr;
         c->launched += nr;
         c->ray[cur] = 0;
@@ -1026,6 +1028,38 @@ __device__ __forceinline__ void block_reserve2(bool pred0, bool pred1, unsigned 
     __syncthreads();
 }
 
+// block_reserve2 with a third counter (guide training records).
+__device__ __forceinline__ void block_reserve3(bool pred0, bool pred1, bool pred2, unsigned int* c0, unsigned int* c1,
+                                               unsigned int* c2, uint32_t& pos0, uint32_t& pos1, uint32_t& pos2) {
+    __shared__ uint32_t s_off3[3][4];
+    const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t waves = blockDim.x >> 6;
+    const uint64_t m0 = __ballot(pred0), m1 = __ballot(pred1), m2 = __ballot(pred2);
+    if (lane == 0) {
+        s_off3[0][wave] = (uint32_t)__popcll(m0);
+        s_off3[1][wave] = (uint32_t)__popcll(m1);
+        s_off3[2][wave] = (uint32_t)__popcll(m2);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3u) {
+        const int k = (int)threadIdx.x;
+        unsigned int* ctr = k == 0 ? c0 : k == 1 ? c1 : c2;
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < waves; w++) total += s_off3[k][w];
+        uint32_t run = total ? atomicAdd(ctr, total) : 0u;
+        for (uint32_t w = 0; w < waves; w++) {
+            const uint32_t c = s_off3[k][w];
+            s_off3[k][w] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    pos0 = s_off3[0][wave] + mbcnt(m0);
+    pos1 = s_off3[1][wave] + mbcnt(m1);
+    pos2 = s_off3[2][wave] + mbcnt(m2);
+    __syncthreads();
+}
+
 // The four material stages of one bounce in one launch: element e of the
 // virtual range [diffuse | ggx | mirror | emissive] is queue entry e - offset.
 // Each wave walks 64 consecutive entries, so a wave shades one material except
@@ -1042,7 +1076,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
     const int nxt = cur ^ 1;
     const uint32_t c0 = c->queue[0], c1 = c->queue[1], c2 = c->queue[2], c3 = c->queue[3];
     const uint32_t e1 = c0, e2 = c0 + c1, e3 = e2 + c2, total = e3 + c3;
-    const uint32_t rec_base = GUIDED ? c->records : 0u;
+    const uint32_t rec_mask = GUIDED ? p.guide.cfg.rec_mask : 0u;
 
     for (uint32_t bbase = blockIdx.x * 256u; bbase < total; bbase += gridDim.x * 256u) {
         const uint32_t e = bbase + threadIdx.x;
@@ -1056,6 +1090,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
         float cur_ior = 1.0f;
         float scene_bits = 0.0f;
         uint32_t head = kInvalid;          // newest guide record of the path
+        float rrs = 1.0f;                  // guided roulette scale of the path
         bool rec_valid = false;            // the continuing path writes a training record
         V3 rec_p = v3(0.0f), rec_n = v3(0.0f), rec_wo = v3(0.0f);
         float rec_pdf = 0.0f, rec_cos = 0.0f;
@@ -1087,7 +1122,10 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
             cur_ior = th4.w;
             const float4 hh = p.hit[i];
             const uint32_t prim = __float_as_uint(hh.w);
-            if (GUIDED) head = (cur ? p.guide.head[1] : p.guide.head[0])[i];
+            if (GUIDED) {
+                head = (cur ? p.guide.head[1] : p.guide.head[0])[i];
+                rrs = (cur ? p.guide.rr_scale[1] : p.guide.rr_scale[0])[i];
+            }
             if (q == 3) {
                 // EvaluateEmissiveKernel, MaterialEval.cu:882-912
                 if (!view_nee || delta) {
@@ -1120,6 +1158,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                 }
                 const bool by_guide = GUIDED && gfrac > 0.0f && u7 < gfrac;
                 float pdf_used = 0.0f;
+                float pdf_bsdf = 0.0f;             // the BSDF-sampling density of wi (guided roulette)
                 const float u0 = dr.next(), u1 = dr.next(), u2 = dr.next(), u3 = dr.next();
                 float l0 = 0.0f, l1 = 0.0f, l2 = 0.0f;
                 if (nee) {
@@ -1159,6 +1198,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                             if (pdf > 1e-8f) {
                                 weight = albedo * (ci * kInvPi / pdf);
                                 pdf_used = pdf;
+                                pdf_bsdf = ci * kInvPi;
                                 scatter = true;
                             }
                         } else {
@@ -1226,6 +1266,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                                         const float pdfs = ggx_dv(wo, wm, alpha) / (4.0f * owm);
                                         const V3 ww = fr.from_local(wl);
                                         float pdf = pd * pdfd + ps * pdfs;
+                                        pdf_bsdf = pdf;
                                         if (GUIDED && gfrac > 0.0f)
                                             pdf = gfrac * guide_pdf_folded(gm, ww, h.gn) + (1.0f - gfrac) * pdf;
                                         if (!(pdf <= 1e-8f)) {
@@ -1249,7 +1290,13 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                     new_delta = true;
                 }
                 if (scatter) {
-                    const Continuation k = continue_path(depth, thr, limit, rr_start, weight, u3);
+                    // Guided paths: roulette on the throughput BSDF sampling would
+                    // have carried (carried throughput x prod of mixture pdf / BSDF
+                    // pdf), so a direction the guide favours is not killed for the
+                    // lower weight its higher density gives it.
+                    if (GUIDED && gfrac > 0.0f && q < 2)
+                        rrs = clampf(rrs * (pdf_bsdf > 0.0f ? pdf_used / pdf_bsdf : 1e6f), 1e-6f, 1e6f);
+                    const Continuation k = continue_path(depth, thr, limit, rr_start, weight, u3, GUIDED ? rrs : 1.0f);
                     if (!k.terminated) {
                         // SpawnScatteredRay, MaterialEval.cu:148-156
                         new_o = ro + rdir * hh.x + h.gn * 1e-4f;
@@ -1257,7 +1304,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                         new_thr = k.thr;
                         new_flags = k.depth | (m.y & 0x00FFFF00u) | (new_delta ? kDeltaBit : 0u);
                         to_ray = true;
-                        if (GUIDED && q < 2) {
+                        if (GUIDED && q < 2 && (m.z & rec_mask) == 0u) {
                             rec_valid = true;
                             rec_p = h.p;
                             rec_n = h.gn;
@@ -1275,35 +1322,28 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
             atomicAdd(acc + 1, add_L.y);
             atomicAdd(acc + 2, add_L.z);
         }
-        uint32_t ray_pos, sh_pos;
-        block_reserve2(to_ray, so.emit, &c->ray[nxt], &c->shadow, ray_pos, sh_pos);
+        uint32_t ray_pos, sh_pos, rec_pos = 0;
+        if (GUIDED)
+            block_reserve3(to_ray, so.emit, rec_valid, &c->ray[nxt], &c->shadow, &c->records, ray_pos, sh_pos, rec_pos);
+        else
+            block_reserve2(to_ray, so.emit, &c->ray[nxt], &c->shadow, ray_pos, sh_pos);
         if (GUIDED) {
             if (so.emit) p.guide.sh_head[sh_pos] = head;
             if (to_ray) {
-                // record slot = records of earlier bounces + queue position; a
-                // mirror continuation leaves a zero record (no weight) in its slot
-                const uint32_t ridx = rec_base + ray_pos;
+                // the recording paths' vertices take consecutive record slots
                 uint32_t new_head = head;
-                if (ridx < p.guide.rec_capacity) {
-                    float4* rp = reinterpret_cast<float4*>(p.guide.recs + ridx);
-                    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    if (rec_valid) {
-                        rp[0] = make_float4(rec_p.x, rec_p.y, rec_p.z, __uint_as_float(head));
-                        rp[1] = make_float4(rec_n.x, rec_n.y, rec_n.z, rec_pdf);
-                        rp[2] = make_float4(rec_wo.x, rec_wo.y, rec_wo.z, rec_cos);
-                        rp[3] = make_float4(new_d.x, new_d.y, new_d.z, 0.0f);
-                        rp[4] = make_float4(new_thr.x, new_thr.y, new_thr.z, 0.0f);
-                        new_head = ridx;
-                    } else {
-                        rp[0] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kInvalid));
-                        rp[1] = z;
-                        rp[2] = z;
-                        rp[3] = z;
-                        rp[4] = z;
-                    }
-                    rp[5] = z;
+                if (rec_valid && rec_pos < p.guide.rec_capacity) {
+                    float4* rp = reinterpret_cast<float4*>(p.guide.recs + rec_pos);
+                    rp[0] = make_float4(rec_p.x, rec_p.y, rec_p.z, __uint_as_float(head));
+                    rp[1] = make_float4(rec_n.x, rec_n.y, rec_n.z, rec_pdf);
+                    rp[2] = make_float4(rec_wo.x, rec_wo.y, rec_wo.z, rec_cos);
+                    rp[3] = make_float4(new_d.x, new_d.y, new_d.z, 0.0f);
+                    rp[4] = make_float4(new_thr.x, new_thr.y, new_thr.z, 0.0f);
+                    rp[5] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    new_head = rec_pos;
                 }
                 (nxt ? p.guide.head[1] : p.guide.head[0])[ray_pos] = new_head;
+                (nxt ? p.guide.rr_scale[1] : p.guide.rr_scale[0])[ray_pos] = rrs;
             }
         }
         if (so.emit) {

@@ -41,6 +41,7 @@ struct GuideDev {
     uint32_t rec_capacity;
     uint32_t* head[2];              // per path position: newest record of the path, kInvalid if none
     uint32_t* sh_head;              // per shadow entry: newest record before its vertex
+    float* rr_scale[2];             // per path position: BSDF-sampling throughput / carried throughput
     GuideConfig cfg;
 };
 

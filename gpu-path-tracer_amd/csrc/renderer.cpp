@@ -104,7 +104,7 @@ PathState Renderer::path_state(const LoopMode* mode) const {
     if (mode && mode->guided) {
         p.guide = guide_dev();
         p.guide.cfg.guide_fraction = mode->guide_fraction;
-        p.guide.cfg.enabled = 1;
+        p.guide.cfg.rec_mask = (1u << mode->rec_shift) - 1u;
     }
     for (int k = 0; k < 2; k++) {
         p.ray_o[k] = d_ray_o_[k].as<float4>();
@@ -418,6 +418,7 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
     }
     mode.guided = opt.guided;
     mode.guide_fraction = opt.guided ? opt.guide_fraction : 0.0f;
+    mode.rec_shift = opt.guided ? std::min(opt.rec_shift, 31u) : 0u;
     if (mode.guided && (mode.trace_multi || mode.shade_multi))
         return fail(PT_ERROR_INVALID_ARGUMENT, "guided rendering takes one view");
 

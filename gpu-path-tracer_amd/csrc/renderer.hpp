@@ -42,6 +42,7 @@ struct RunOptions {
     bool clear = true;           // zero the accumulation pixels first
     bool guided = false;         // guide MIS and training records (one view)
     float guide_fraction = 0.0f;
+    uint32_t rec_shift = 0;      // training records from one sample in 2^rec_shift
 };
 
 // A scene resident on the device: host copy (for the hooks and the BVH
@@ -131,8 +132,10 @@ private:
         uint32_t scene = 0;          // the one scene when !trace_multi
         bool guided = false;         // guide MIS and training records (one view)
         float guide_fraction = 0.0f; // baked into the captured kernels' arguments
+        uint32_t rec_shift = 0;      // likewise
         int key() const {
-            if (guided) return (1 << 30) | (int)(scene << 12) | (int)(guide_fraction * 1000.0f + 0.5f);
+            if (guided)
+                return (1 << 30) | (int)(rec_shift << 20) | (int)(scene << 12) | (int)(guide_fraction * 1000.0f + 0.5f);
             return trace_multi ? -1 - (int)shade_multi : (int)(scene * 2u + (shade_multi ? 1u : 0u));
         }
     };
@@ -233,7 +236,7 @@ private:
 
     // guide network and its training records
     DeviceBuffer g_w32_, g_w_, g_wt_, g_grad_, g_m_, g_v_, g_loss_;
-    DeviceBuffer g_mix_, g_recs_, g_head_[2], g_sh_head_;
+    DeviceBuffer g_mix_, g_recs_, g_head_[2], g_sh_head_, g_rr_scale_[2];
     uint32_t g_rec_capacity_ = 0;
     uint32_t g_pool_ = 0;            // path slots the per-path guide buffers cover
     uint32_t g_step_ = 0;            // Adam steps taken

@@ -362,13 +362,15 @@ typedef struct pt_guide_record {
 } pt_guide_record;
 
 typedef struct pt_guide_settings {
-    uint32_t passes;          /* render passes; the samples per pixel are split evenly among them */
+    uint32_t passes;          /* render passes (at most samples_per_pixel): pass k takes about 2^k / (2^passes - 1)
+                                 of the samples per pixel (an even split when there are fewer than 2^passes - 1) */
     uint32_t train_steps;     /* Adam steps after each pass but the last */
     uint32_t batch;           /* records per Adam step (multiple of 64) */
     float learning_rate;
     float guide_fraction;     /* probability of sampling the guide; 0 = plain BSDF sampling */
     float weight_max;         /* clamp of a record's loss weight luminance(L) cos / pdf */
-    uint32_t record_capacity; /* training records kept per pass; 0 = 4 per sample of a pass, at most 2^26 */
+    uint32_t record_capacity; /* training records kept per pass, at most 2^26; 0 = 2^24.  Records come from an
+                                 evenly spread subset of the samples (one in 2^k) sized to fit */
     uint32_t reset;           /* re-initialise the network before the first pass */
     uint64_t seed;            /* network initialisation and batch selection */
 } pt_guide_settings;
@@ -383,8 +385,9 @@ typedef struct pt_guide_stats {
     uint32_t passes;
 } pt_guide_stats;
 
-/* Renders one frame in `passes` passes (settings.samples_per_pixel / passes each);
- * the network is trained between passes.  The frame is read with pt_frame_copy and
+/* Renders one frame in `passes` passes of growing size; the network is trained
+ * between passes, and a freshly reset network is not sampled before its first
+ * training.  The frame is read with pt_frame_copy and
  * pt_frame_copy_hdr; stats->render_ms includes the training time. */
 pt_status pt_render_guided(pt_renderer* renderer, const pt_render_settings* settings,
                            const pt_camera* camera, const pt_motion_state* motion_state,
