@@ -10,6 +10,8 @@
 // Weights are kept as fp32 masters (with biases and Adam moments) and as bf16
 // copies in both [out][in] and [in][out] order, so every B fragment is one
 // 8-byte load.
+#include <algorithm>
+
 #include "guide.hpp"
 #include "launch.hpp"
 
@@ -138,133 +140,197 @@ __device__ __forceinline__ const float* bias_of(const GuideNet& n, int l) {
 
 // ---------------------------------------------------------------- inference forward
 //
-// Inference computes each layer transposed, D = W . X^T (A = W rows from
-// global memory, B = X rows from LDS), so that lane l holds outputs
-// (16 mt + 4 (l >> 4) + j) of sample 16 nt + (l & 15): four consecutive
-// features of one sample, stored with one 8-byte LDS write.  The two bf16
-// ping-pong tiles double as the fp32 output tile (17 KB of LDS per wave).
+// Inference keeps the activations in registers.  Each layer is computed
+// transposed, D = W . X^T, with v_mfma_f32_16x16x32_bf16: lane l = r + 16 kq
+// of a 16x16 output tile holds outputs 4 kq + j (j < 4) of sample r, which is
+// exactly what the lane needs as B operand of the next layer if the 32 inputs
+// of an MFMA are ordered (4 kq + j, 16 + 4 kq + j) per lane -- the weights are
+// stored in that order.  Only the first layer's input goes through LDS.  The
+// bf16 weights of all layers (32 KB) are staged in LDS once per block.
+//
+// The output layer's rows are reordered so that lane kq of output tile 0 (1)
+// holds lobe kq's (4 + kq's) mu xyz and log kappa, and tile 2 the 8 logits:
+// inference output o' maps to network output guide_out_row(o').
 
-typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ void mm_wxt(const uint16_t* x, const uint16_t* w, f32x4 acc[4][4]) {
-    const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+PT_HD int guide_out_row(int o) {
+    if (o < 32) return 5 * (o >> 2) + (o & 3);
+    if (o < 40) return 5 * (o - 32) + 4;
+    return o;
+}
+PT_HD int guide_out_slot(int row) {     // inverse of guide_out_row
+    if (row >= 40) return row;
+    const int j = row / 5, c = row % 5;
+    return c < 4 ? 4 * j + c : 32 + j;
+}
+
+// Position of weight W_l[o'][i] (o' in inference order) in the fragment-ordered
+// copy: [layer][mt][kt][lane][8].
+PT_HD uint32_t guide_frag_index(int l, int o, int i) {
+    const int mt = o >> 4, r = o & 15, kt = i >> 5, w = i & 31;
+    const int half = w >> 4, kq = (w & 15) >> 2, j = (w & 3) + 4 * half;
+    return ((((uint32_t)l * 4u + (uint32_t)mt) * 2u + (uint32_t)kt) * 64u + (uint32_t)(r + 16 * kq)) * 8u + (uint32_t)j;
+}
+
+constexpr int kInferWaves = 12;                       // waves per block, one block per CU
+constexpr int kInferBlock = 64 * kInferWaves;
+constexpr int kFragWeights = kGuideLayers * 4 * 2 * 64;   // bf16x8 fragments of all layers
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ __bf16 bf16_of(float f) {
+    const uint16_t h = to_bf16(f);
+    __bf16 b;
+    __builtin_memcpy(&b, &h, 2);
+    return b;
+}
+
+// Writes sample s's 64 inputs into the wave's fragment tile
+// xf[kt][nt][lane] (bf16x8), nt = s / 16.
+__device__ __forceinline__ void store_input_frags(bf16x8* xf, int s, const float f[kGuideDim]) {
+    const int nt = s >> 4, r = s & 15;
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++)
+    for (int kt = 0; kt < 2; kt++)
 #pragma unroll
-        for (int nt = 0; nt < 4; nt++) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int kq = 0; kq < 4; kq++) {
+            bf16x8 v;
 #pragma unroll
-    for (int kt = 0; kt < 4; kt++) {
-        bf16x4 wf[4];
+            for (int j = 0; j < 4; j++) {
+                v[j] = bf16_of(f[32 * kt + 4 * kq + j]);
+                v[4 + j] = bf16_of(f[32 * kt + 16 + 4 * kq + j]);
+            }
+            xf[(kt * 4 + nt) * 64 + r + 16 * kq] = v;
+        }
+}
+
+// Forward pass of the wave's 64-sample tile: hidden layers in registers, the
+// output layer's first MT 16-row tiles into out[mt][nt].
+template <int MT>
+__device__ __forceinline__ void tile_forward(const bf16x8* wsh, const float* bsh, const bf16x8* xf, f32x4 out[4][4]) {
+    const int lane = threadIdx.x & 63, kq = lane >> 4;
+    bf16x8 b[2][4];
 #pragma unroll
-        for (int mt = 0; mt < 4; mt++) wf[mt] = glb4(w, 16 * mt + r, 16 * kt + 4 * kq);
+    for (int kt = 0; kt < 2; kt++)
 #pragma unroll
-        for (int nt = 0; nt < 4; nt++) {
-            const bf16x4 xf = row4(x, 16 * nt + r, 16 * kt + 4 * kq);
+        for (int nt = 0; nt < 4; nt++) b[kt][nt] = xf[(kt * 4 + nt) * 64 + lane];
 #pragma unroll
-            for (int mt = 0; mt < 4; mt++)
-                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wf[mt], xf, acc[mt][nt], 0, 0, 0);
+    for (int l = 0; l < kGuideLayers; l++) {
+        const int mts = l + 1 < kGuideLayers ? 4 : MT;
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) out[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            if (mt >= mts) break;
+#pragma unroll
+            for (int kt = 0; kt < 2; kt++) {
+                const bf16x8 a = wsh[((l * 4 + mt) * 2 + kt) * 64 + lane];
+#pragma unroll
+                for (int nt = 0; nt < 4; nt++)
+                    out[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[kt][nt], out[mt][nt], 0, 0, 0);
+            }
+        }
+        const float* bias = bsh + l * kGuideDim;
+        if (l + 1 == kGuideLayers) {
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) {
+                if (mt >= mts) break;
+                const float4 bb = *reinterpret_cast<const float4*>(bias + 16 * mt + 4 * kq);
+#pragma unroll
+                for (int nt = 0; nt < 4; nt++) {
+                    out[mt][nt][0] += bb.x;
+                    out[mt][nt][1] += bb.y;
+                    out[mt][nt][2] += bb.z;
+                    out[mt][nt][3] += bb.w;
+                }
+            }
+            break;
+        }
+        // ReLU(out + bias) -> the next layer's B operands
+#pragma unroll
+        for (int kt = 0; kt < 2; kt++) {
+            const float4 b0 = *reinterpret_cast<const float4*>(bias + 32 * kt + 4 * kq);
+            const float4 b1 = *reinterpret_cast<const float4*>(bias + 32 * kt + 16 + 4 * kq);
+            const float lo[4] = {b0.x, b0.y, b0.z, b0.w}, hi[4] = {b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) {
+                bf16x8 v;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    v[j] = bf16_of(fmaxf(out[2 * kt][nt][j] + lo[j], 0.0f));
+                    v[4 + j] = bf16_of(fmaxf(out[2 * kt + 1][nt][j] + hi[j], 0.0f));
+                }
+                b[kt][nt] = v;
+            }
         }
     }
 }
 
-__device__ __forceinline__ void infer_hidden(const uint16_t* in, uint16_t* out, const uint16_t* w, const float* bias) {
-    f32x4 acc[4][4];
-    mm_wxt(in, w, acc);
-    const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
-#pragma unroll
-    for (int mt = 0; mt < 4; mt++) {
-        const int o = 16 * mt + 4 * kq;
-        const float4 b = *reinterpret_cast<const float4*>(bias + o);
-#pragma unroll
-        for (int nt = 0; nt < 4; nt++) {
-            u16x4 v;
-            v[0] = to_bf16(fmaxf(acc[mt][nt][0] + b.x, 0.0f));
-            v[1] = to_bf16(fmaxf(acc[mt][nt][1] + b.y, 0.0f));
-            v[2] = to_bf16(fmaxf(acc[mt][nt][2] + b.z, 0.0f));
-            v[3] = to_bf16(fmaxf(acc[mt][nt][3] + b.w, 0.0f));
-            *reinterpret_cast<u16x4*>(out + (16 * nt + r) * kLd + o) = v;
-        }
-    }
-}
-
-constexpr int kLdInfer = 68;            // fp32 row stride of the inference output tile
-
-// Four layers on the tile in buf (bf16 [64][kLd] features in the first half);
-// leaves the fp32 outputs as [sample][kLdInfer] over the whole buffer.
-__device__ __forceinline__ void infer_forward(uint16_t* buf, const GuideNet& net) {
-    uint16_t* xa = buf;
-    uint16_t* xb = buf + 64 * kLd;
-    __syncthreads();
-    infer_hidden(xa, xb, net.w, bias_of(net, 0));
-    __syncthreads();
-    infer_hidden(xb, xa, net.w + 1 * kGuideDim * kGuideDim, bias_of(net, 1));
-    __syncthreads();
-    infer_hidden(xa, xb, net.w + 2 * kGuideDim * kGuideDim, bias_of(net, 2));
-    __syncthreads();
-    f32x4 acc[4][4];
-    mm_wxt(xb, net.w + 3 * kGuideDim * kGuideDim, acc);
-    __syncthreads();   // every read of xb is done: the outputs overwrite both tiles
-    float* o32 = reinterpret_cast<float*>(buf);
-    const float* bias = bias_of(net, 3);
-    const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
-#pragma unroll
-    for (int mt = 0; mt < 4; mt++) {
-        const int o = 16 * mt + 4 * kq;
-        const float4 b = *reinterpret_cast<const float4*>(bias + o);
-#pragma unroll
-        for (int nt = 0; nt < 4; nt++)
-            *reinterpret_cast<float4*>(o32 + (16 * nt + r) * kLdInfer + o) =
-                make_float4(acc[mt][nt][0] + b.x, acc[mt][nt][1] + b.y, acc[mt][nt][2] + b.z, acc[mt][nt][3] + b.w);
-    }
+// Stages the fragment-ordered weights and the biases in LDS (whole block).
+__device__ __forceinline__ void stage_weights(bf16x8* wsh, float* bsh, const uint16_t* wf, const float* bf) {
+    const bf16x8* src = reinterpret_cast<const bf16x8*>(wf);
+    for (int k = threadIdx.x; k < kFragWeights; k += blockDim.x) wsh[k] = src[k];
+    for (int k = threadIdx.x; k < kGuideLayers * kGuideDim; k += blockDim.x) bsh[k] = bf[k];
     __syncthreads();
 }
 
-// One bf16 feature row (zeros past the features) with 8-byte stores.
-__device__ __forceinline__ void store_row(uint16_t* x, int s, const float f[kGuideFeatures], bool valid) {
+// Raw outputs of given inputs (hook for tests), in the network's output order.
+__global__ __launch_bounds__(kInferBlock) void k_guide_forward(const uint16_t* wf, const float* bf, const float* features,
+                                                               uint32_t n, float* out) {
+    __shared__ bf16x8 wsh[kFragWeights];
+    __shared__ float bsh[kGuideLayers * kGuideDim];
+    __shared__ bf16x8 xs[kInferWaves][2 * 4 * 64];
+    stage_weights(wsh, bsh, wf, bf);
+    const int wave = threadIdx.x >> 6, s = threadIdx.x & 63, r = s & 15, kq = s >> 4;
+    const uint32_t tiles = (n + 63) / 64, stride = gridDim.x * kInferWaves;
+    for (uint32_t t = blockIdx.x * kInferWaves + wave; t < tiles; t += stride) {
+        const uint32_t idx = t * 64u + (uint32_t)s;
+        float f[kGuideDim];
 #pragma unroll
-    for (int i = 0; i < kGuideDim; i += 4) {
-        u16x4 v;
+        for (int i = 0; i < kGuideDim; i++) f[i] = idx < n ? features[(size_t)idx * kGuideDim + i] : 0.0f;
+        store_input_frags(xs[wave], s, f);
+        wave_lds_sync();
+        f32x4 o[4][4];
+        tile_forward<4>(wsh, bsh, xs[wave], o);
+        wave_lds_sync();
 #pragma unroll
-        for (int c = 0; c < 4; c++) v[c] = to_bf16(valid && i + c < kGuideFeatures ? f[i + c] : 0.0f);
-        *reinterpret_cast<u16x4*>(x + s * kLd + i) = v;
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) {
+                const uint32_t e = t * 64u + 16u * nt + (uint32_t)r;
+                if (e < n)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) out[(size_t)e * kGuideDim + guide_out_row(16 * mt + 4 * kq + j)] = o[mt][nt][j];
+            }
     }
-}
-
-constexpr int kInferLds = 2 * 64 * kLd;   // uint16 elements: two bf16 tiles = one fp32 [64][68] tile
-static_assert(kInferLds * 2 == 64 * kLdInfer * 4, "the output tile fits the two feature tiles");
-
-// Raw outputs of given inputs (hook for tests).
-__global__ __launch_bounds__(64) void k_guide_forward(GuideNet net, const float* features, uint32_t n, float* out) {
-    __shared__ __attribute__((aligned(16))) uint16_t buf[kInferLds];
-    const int s = threadIdx.x;
-    const uint32_t idx = blockIdx.x * 64u + (uint32_t)s;
-    const bool valid = idx < n;
-    float f[kGuideFeatures];
-    for (int i = 0; i < kGuideFeatures; i++) f[i] = valid ? features[(size_t)idx * kGuideDim + i] : 0.0f;
-    store_row(buf, s, f, valid);
-    // inputs past the features are part of the hook's contract too
-    for (int i = kGuideFeatures; i < kGuideDim; i++) buf[s * kLd + i] = to_bf16(valid ? features[(size_t)idx * kGuideDim + i] : 0.0f);
-    infer_forward(buf, net);
-    const float* o32 = reinterpret_cast<const float*>(buf);
-    if (valid)
-        for (int o = 0; o < kGuideDim; o++) out[(size_t)idx * kGuideDim + o] = o32[s * kLdInfer + o];
 }
 
 // Mixture of every diffuse and GGX queue entry of the current bounce, in the
-// shade kernel's virtual order [diffuse | ggx].
-__global__ __launch_bounds__(64) void k_guide_infer(PathState p, SceneDev sc, GuideNet net, int cur) {
-    __shared__ __attribute__((aligned(16))) uint16_t buf[kInferLds];
+// shade kernel's virtual order [diffuse | ggx], packed as in guide.hpp.
+__global__ __launch_bounds__(kInferBlock) void k_guide_infer(PathState p, SceneDev sc, int cur) {
+    __shared__ bf16x8 wsh[kFragWeights];
+    __shared__ float bsh[kGuideLayers * kGuideDim];
+    __shared__ bf16x8 xs[kInferWaves][2 * 4 * 64];
+    stage_weights(wsh, bsh, p.guide.wf, p.guide.bf);
     const uint32_t c0 = p.ctr->queue[0], c1 = p.ctr->queue[1];
     const uint32_t total = c0 + c1;
     const GuideConfig g = p.guide.cfg;
-    const int s = threadIdx.x;
-    for (uint32_t base = blockIdx.x * 64u; base < total; base += gridDim.x * 64u) {
-        const uint32_t e = base + (uint32_t)s;
-        const bool valid = e < total;
-        uint16_t* row = buf + s * kLd;
+    const int wave = threadIdx.x >> 6, s = threadIdx.x & 63, r = s & 15, kq = s >> 4;
+    const uint32_t tiles = (total + 63) / 64, stride = gridDim.x * kInferWaves;
+    uint32_t* mix = p.guide.mix;
+    const size_t ms = p.guide.mix_stride;
+    for (uint32_t t = blockIdx.x * kInferWaves + wave; t < tiles; t += stride) {
+        const uint32_t e = t * 64u + (uint32_t)s;
+        float f[kGuideDim];
 #pragma unroll
-        for (int k = 0; k < kGuideDim; k += 4) *reinterpret_cast<u16x4*>(row + k) = u16x4{0, 0, 0, 0};
-        if (valid) {
+        for (int i = 0; i < kGuideDim; i++) f[i] = 0.0f;
+        if (e < total) {
             const uint32_t i = e < c0 ? p.q[0][e] : p.q[1][e - c0];
             const float4 o4 = (cur ? p.ray_o[1] : p.ray_o[0])[i];
             const float4 d4 = (cur ? p.ray_d[1] : p.ray_d[0])[i];
@@ -274,35 +340,60 @@ __global__ __launch_bounds__(64) void k_guide_infer(PathState p, SceneDev sc, Gu
             const float* gn = sc.shade[__float_as_uint(hh.w)].gn;
             V3 n = v3(gn[0], gn[1], gn[2]);
             if (dot(n, d) > 0.0f) n = -n;
-            guide_features_each(g, x, n, -d, [&](int k, float v) { row[k] = to_bf16(v); });
+            guide_features_each(g, x, n, -d, [&](int k, float v) { f[k] = v; });
         }
-        // re-read the weights from cache every tile: hoisting them out of the
-        // loop takes 128 registers and leaves one wave per SIMD
-        uint32_t opaque = 0;
-        asm volatile("" : "+s"(opaque));
-        GuideNet tile_net = net;
-        tile_net.w += opaque;
-        tile_net.w32 += opaque;
-        infer_forward(buf, tile_net);
-        if (valid) {
-            // lobe by lobe from the output row straight to the mixture record
-            const float* o = reinterpret_cast<const float*>(buf) + s * kLdInfer;
-            float mx, inv_sum;
-            guide_softmax(o, mx, inv_sum);
-            float* m = p.guide.mix + e;
-            const size_t st = p.guide.mix_stride;
-            for (int j = 0; j < kGuideLobes; j++) {
-                float mu[3], kappa, pi, c;
-                guide_lobe(o + 5 * j, mx, inv_sum, mu, kappa, pi, c);
-                m[(3 * j + 0) * st] = mu[0];
-                m[(3 * j + 1) * st] = mu[1];
-                m[(3 * j + 2) * st] = mu[2];
-                m[(24 + j) * st] = kappa;
-                m[(32 + j) * st] = pi;
-                m[(40 + j) * st] = c;
+        store_input_frags(xs[wave], s, f);
+        wave_lds_sync();
+        f32x4 o[4][4];
+        tile_forward<3>(wsh, bsh, xs[wave], o);
+        wave_lds_sync();
+        // lane (r, kq) holds, for samples 16 nt + r: lobe kq in o[0], lobe 4 + kq
+        // in o[1], logits 4 kq + j in o[2] (kq < 2)
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            float lg[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) lg[j] = o[2][nt][j];
+            float mx = fmaxf(fmaxf(lg[0], lg[1]), fmaxf(lg[2], lg[3]));
+            mx = fmaxf(mx, __shfl_xor(mx, 16));
+            float ex[4], sum = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                ex[j] = expf(lg[j] - mx);
+                sum += ex[j];
+            }
+            sum += __shfl_xor(sum, 16);
+            const float inv = 1.0f / sum;
+            // pi of lobe kq sits in lane r (component kq), of lobe 4 + kq in lane r + 16
+            float pa = 0.0f, pb = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float va = __shfl(ex[j] * inv, r), vb = __shfl(ex[j] * inv, r + 16);
+                pa = kq == j ? va : pa;
+                pb = kq == j ? vb : pb;
+            }
+            float mu[2][3], kap[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const f32x4 v = o[h][nt];
+                const float len = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                const bool ok = len > 1e-12f;
+                const float il = 1.0f / fmaxf(len, 1e-12f);
+                mu[h][0] = ok ? v[0] * il : 0.0f;
+                mu[h][1] = ok ? v[1] * il : 0.0f;
+                mu[h][2] = ok ? v[2] * il : 1.0f;
+                kap[h] = expf(clampf(v[3], kGuideLogKappaMin, kGuideLogKappaMax));
+            }
+            const uint32_t ee = t * 64u + 16u * nt + (uint32_t)r;
+            if (ee < total) {
+                uint32_t* w = mix + ee;
+                w[(size_t)(0 + kq) * ms] = guide_pack_half2(mu[0][0], mu[1][0]);
+                w[(size_t)(4 + kq) * ms] = guide_pack_half2(mu[0][1], mu[1][1]);
+                w[(size_t)(8 + kq) * ms] = guide_pack_half2(mu[0][2], mu[1][2]);
+                w[(size_t)(12 + kq) * ms] = guide_pack_half2(kap[0], kap[1]);
+                w[(size_t)(16 + kq) * ms] = guide_pack_half2(pa, pb);
             }
         }
-        __syncthreads();
     }
 }
 
@@ -409,8 +500,26 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const GuideRec
 }
 
 // Adam (Kingma and Ba 2015) on the fp32 masters, then the bf16 copies.
-__global__ void k_guide_adam(float* w32, float* grad, float* m, float* v, uint16_t* w, uint16_t* wt, float lr,
-                             float beta1, float beta2, float eps, float c1, float c2, float grad_scale) {
+// Writes parameter k of value x into the bf16 copies (weights) or the
+// inference bias table.
+__device__ __forceinline__ void guide_copies(uint32_t k, float x, uint16_t* w, uint16_t* wt, uint16_t* wf, float* bf) {
+    const uint32_t l = k / kGuideLayerParams, off = k % kGuideLayerParams;
+    const bool last = l + 1 == (uint32_t)kGuideLayers;
+    if (off < (uint32_t)(kGuideDim * kGuideDim)) {
+        const uint32_t o = off / kGuideDim, i = off % kGuideDim;
+        const uint16_t h = to_bf16(x);
+        w[l * kGuideDim * kGuideDim + o * kGuideDim + i] = h;
+        wt[l * kGuideDim * kGuideDim + i * kGuideDim + o] = h;
+        wf[guide_frag_index((int)l, last ? guide_out_slot((int)o) : (int)o, (int)i)] = h;
+    } else {
+        const uint32_t o = off - kGuideDim * kGuideDim;
+        bf[l * kGuideDim + (last ? (uint32_t)guide_out_slot((int)o) : o)] = x;
+    }
+}
+
+__global__ void k_guide_adam(float* w32, float* grad, float* m, float* v, uint16_t* w, uint16_t* wt, uint16_t* wf,
+                             float* bf, float lr, float beta1, float beta2, float eps, float c1, float c2,
+                             float grad_scale) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (uint32_t)kGuideParams) return;
     const float gk = grad[k] * grad_scale;
@@ -421,45 +530,34 @@ __global__ void k_guide_adam(float* w32, float* grad, float* m, float* v, uint16
     const float wk = w32[k] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
     w32[k] = wk;
     grad[k] = 0.0f;
-    const uint32_t l = k / kGuideLayerParams, off = k % kGuideLayerParams;
-    if (off < (uint32_t)(kGuideDim * kGuideDim)) {
-        const uint32_t o = off / kGuideDim, i = off % kGuideDim;
-        const uint16_t h = to_bf16(wk);
-        w[l * kGuideDim * kGuideDim + o * kGuideDim + i] = h;
-        wt[l * kGuideDim * kGuideDim + i * kGuideDim + o] = h;
-    }
+    guide_copies(k, wk, w, wt, wf, bf);
 }
 
 // bf16 copies of the masters (after an upload).
-__global__ void k_guide_sync(const float* w32, uint16_t* w, uint16_t* wt) {
+__global__ void k_guide_sync(const float* w32, uint16_t* w, uint16_t* wt, uint16_t* wf, float* bf) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (uint32_t)kGuideParams) return;
-    const uint32_t l = k / kGuideLayerParams, off = k % kGuideLayerParams;
-    if (off < (uint32_t)(kGuideDim * kGuideDim)) {
-        const uint32_t o = off / kGuideDim, i = off % kGuideDim;
-        const uint16_t h = to_bf16(w32[k]);
-        w[l * kGuideDim * kGuideDim + o * kGuideDim + i] = h;
-        wt[l * kGuideDim * kGuideDim + i * kGuideDim + o] = h;
-    }
+    guide_copies(k, w32[k], w, wt, wf, bf);
 }
 
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_guide_sync(hipStream_t s, const GuideDev& g) {
-    hipLaunchKernelGGL(k_guide_sync, dim3((kGuideParams + 255) / 256), dim3(256), 0, s, g.w32, g.w, g.wt);
+    hipLaunchKernelGGL(k_guide_sync, dim3((kGuideParams + 255) / 256), dim3(256), 0, s, g.w32, g.w, g.wt,
+                       g.wf, g.bf);
     return hipGetLastError();
 }
 
 hipError_t launch_guide_forward(hipStream_t s, const GuideDev& g, const float* features, uint32_t n, float* out) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_guide_forward, dim3((n + 63) / 64), dim3(64), 0, s, GuideNet{g.w, g.wt, g.w32}, features, n,
-                       out);
+    const uint32_t tiles = (n + 63) / 64;
+    const uint32_t grid = std::max(1u, std::min((tiles + kInferWaves - 1) / kInferWaves, 1024u));
+    hipLaunchKernelGGL(k_guide_forward, dim3(grid), dim3(kInferBlock), 0, s, g.wf, g.bf, features, n, out);
     return hipGetLastError();
 }
 
 hipError_t launch_guide_infer(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, uint32_t grid) {
-    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64), 0, s, p, sc,
-                       GuideNet{p.guide.w, p.guide.wt, p.guide.w32}, cur);
+    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(kInferBlock), 0, s, p, sc, cur);
     return hipGetLastError();
 }
 
@@ -475,7 +573,8 @@ hipError_t launch_guide_adam(hipStream_t s, const GuideDev& g, float lr, uint32_
     const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
     const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
     hipLaunchKernelGGL(k_guide_adam, dim3((kGuideParams + 255) / 256), dim3(256), 0, s, g.w32, g.grad, g.m, g.v, g.w,
-                       g.wt, lr, beta1, beta2, eps, c1, c2, grad_scale);
+                       g.wt, g.wf, g.bf, lr, beta1, beta2, eps, c1, c2,
+                       grad_scale);
     return hipGetLastError();
 }
 

@@ -140,37 +140,88 @@ PT_HD void guide_mixture(const float o[kGuideOut], GuideMixture& m) {
     for (int j = 0; j < kGuideLobes; j++) guide_lobe(o + 5 * j, mx, inv_sum, m.mu[j], m.kappa[j], m.pi[j], m.c[j]);
 }
 
-// Mixtures of a whole bounce stored component-major: component k of entry e
-// at p[k * stride + e] (mu 3j+a, kappa 24+j, pi 32+j, c 40+j), so that a wave
-// reads and writes each component of 64 consecutive entries in one line.
-constexpr int kGuideMixComponents = 48;
-struct MixSoA {
-    const float* p;          // component 0 of this entry
-    uint32_t stride;
-    PT_HD float mu(int j, int a) const { return p[(size_t)(3 * j + a) * stride]; }
-    PT_HD float kappa(int j) const { return p[(size_t)(24 + j) * stride]; }
-    PT_HD float pi(int j) const { return p[(size_t)(32 + j) * stride]; }
-    PT_HD float c(int j) const { return p[(size_t)(40 + j) * stride]; }
+// Mixtures of a whole bounce, packed: 8 lobes x (mu xyz, kappa, pi) as fp16,
+// lobes t and t + 4 in one 32-bit word.  Word 4 c + t of entry e lies at
+// w[(4 c + t) * stride + e] (c: 0-2 mu, 3 kappa, 4 pi), so that a wave reads
+// and writes each word of 64 consecutive entries in one line.
+constexpr int kGuideMixWords = 20;
+
+PT_HD uint32_t guide_pack_half2(float lo, float hi) {
+    const _Float16 a = (_Float16)lo, b = (_Float16)hi;
+    uint16_t ua, ub;
+    __builtin_memcpy(&ua, &a, 2);
+    __builtin_memcpy(&ub, &b, 2);
+    return (uint32_t)ua | ((uint32_t)ub << 16);
+}
+PT_HD float guide_half_lo(uint32_t w) {
+    const uint16_t u = (uint16_t)(w & 0xFFFFu);
+    _Float16 h;
+    __builtin_memcpy(&h, &u, 2);
+    return (float)h;
+}
+PT_HD float guide_half_hi(uint32_t w) {
+    const uint16_t u = (uint16_t)(w >> 16);
+    _Float16 h;
+    __builtin_memcpy(&h, &u, 2);
+    return (float)h;
+}
+
+// Lobe j of a packed mixture: unit mu (the stored fp16 vector normalised, as
+// the sampler and the density both use it), kappa and the unnormalised weight.
+struct GuideLobe {
+    V3 mu;
+    float kappa, pi;
 };
 
-// Mixture density at unit direction w.
-template <class M>
-PT_HD float guide_pdf(const M& m, V3 w) {
-    float q = 0.0f;
-    for (int j = 0; j < kGuideLobes; j++) {
-        const float d = m.mu(j, 0) * w.x + m.mu(j, 1) * w.y + m.mu(j, 2) * w.z;
-        q += expf(m.c(j) + m.kappa(j) * (d - 1.0f));
+// One shading point's packed mixture, read where it is used (the words stay in
+// cache) so that no lobe table occupies registers across the shading code.
+// The weights are renormalised to sum to one after the fp16 round trip, and the
+// density is q(w) = sum_j pi_j kappa_j / (2 pi (1 - e^{-2 kappa_j})) e^{kappa_j (mu_j . w - 1)}.
+struct GuideMixRef {
+    const uint32_t* w;       // word 0 of this entry
+    uint32_t stride;
+    PT_HD uint32_t word(int k) const { return w[(size_t)k * stride]; }
+    PT_HD static float half_of(uint32_t v, int h) { return h ? guide_half_hi(v) : guide_half_lo(v); }
+    PT_HD static GuideLobe make(float x, float y, float z, float k, float p) {
+        GuideLobe l;
+        const float len2 = x * x + y * y + z * z;
+        const bool ok = len2 > 1e-12f;
+        const float inv = ok ? 1.0f / sqrtf(len2) : 0.0f;
+        l.mu = ok ? v3(x * inv, y * inv, z * inv) : v3(0.0f, 0.0f, 1.0f);
+        l.kappa = fmaxf(k, 1e-3f);
+        l.pi = fmaxf(p, 0.0f);
+        return l;
     }
-    return q;
-}
+    PT_HD GuideLobe lobe(int j) const {
+        const int t = j & 3, h = j >> 2;
+        return make(half_of(word(t), h), half_of(word(4 + t), h), half_of(word(8 + t), h), half_of(word(12 + t), h),
+                    half_of(word(16 + t), h));
+    }
+    PT_HD float weight(int j) const { return fmaxf(half_of(word(16 + (j & 3)), j >> 2), 0.0f); }
+};
+
+PT_HD float guide_vmf_norm(float kappa) { return kappa / (2.0f * kPi * (1.0f - expf(-2.0f * kappa))); }
 
 // The mixture folded onto the hemisphere of n: directions below the surface
 // are mirrored across its plane, so the density at w (w . n >= 0) is
 // q(w) + q(w - 2 (w . n) n).
 PT_HD V3 guide_mirror(V3 w, V3 n) { return w - n * (2.0f * dot(w, n)); }
-template <class M>
-PT_HD float guide_pdf_folded(const M& m, V3 w, V3 n) {
-    return dot(w, n) < 0.0f ? 0.0f : guide_pdf(m, w) + guide_pdf(m, guide_mirror(w, n));
+PT_HD float guide_pdf_folded(const GuideMixRef& m, V3 w, V3 n) {
+    if (dot(w, n) < 0.0f) return 0.0f;
+    const V3 wr = guide_mirror(w, n);
+    float q = 0.0f, sum = 0.0f;
+    for (int t = 0; t < 4; t++) {
+        const uint32_t x = m.word(t), y = m.word(4 + t), z = m.word(8 + t), k = m.word(12 + t), p = m.word(16 + t);
+        for (int h = 0; h < 2; h++) {
+            const GuideLobe l = GuideMixRef::make(GuideMixRef::half_of(x, h), GuideMixRef::half_of(y, h),
+                                                  GuideMixRef::half_of(z, h), GuideMixRef::half_of(k, h),
+                                                  GuideMixRef::half_of(p, h));
+            const float a = l.pi * guide_vmf_norm(l.kappa);
+            q += a * (expf(l.kappa * (dot(l.mu, w) - 1.0f)) + expf(l.kappa * (dot(l.mu, wr) - 1.0f)));
+            sum += l.pi;
+        }
+    }
+    return sum > 0.0f ? q / sum : 0.0f;
 }
 
 // Orthonormal basis around a unit vector (Duff et al. 2017, "Building an
@@ -183,29 +234,31 @@ PT_HD void guide_basis(V3 n, V3& t, V3& b) {
     b = v3(c, s + n.y * n.y * a, -n.y);
 }
 
-// Direction from the mixture: u0 picks the lobe, (u1, u2) the direction
-// within it (Jakob 2012: cos = 1 + log(u + (1-u) e^{-2 kappa}) / kappa).
-template <class M>
-PT_HD V3 guide_sample(const M& m, float u0, float u1, float u2) {
+// Direction from the mixture: u0 picks the lobe (weights renormalised as in
+// the density), (u1, u2) the direction within it (Jakob 2012:
+// cos = 1 + log(u + (1-u) e^{-2 kappa}) / kappa).
+PT_HD V3 guide_sample(const GuideMixRef& m, float u0, float u1, float u2) {
+    float sum = 0.0f;
+    for (int j = 0; j < kGuideLobes; j++) sum += m.weight(j);
+    const float target = u0 * sum;
     int j = 0;
-    float acc = m.pi(0);
-    while (j < kGuideLobes - 1 && u0 >= acc) {
+    float acc = m.weight(0);
+    while (j < kGuideLobes - 1 && target >= acc) {
         j++;
-        acc += m.pi(j);
+        acc += m.weight(j);
     }
-    const float k = m.kappa(j);
+    const GuideLobe l = m.lobe(j);
+    const float k = l.kappa;
     const float u = fmaxf(u1, 1e-7f);
     const float c = clampf(1.0f + logf(u + (1.0f - u) * expf(-2.0f * k)) / k, -1.0f, 1.0f);
     const float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
     const float phi = 2.0f * kPi * u2;
-    const V3 mu = v3(m.mu(j, 0), m.mu(j, 1), m.mu(j, 2));
     V3 t, b;
-    guide_basis(mu, t, b);
-    return normalize(mu * c + t * (s * cosf(phi)) + b * (s * sinf(phi)));
+    guide_basis(l.mu, t, b);
+    return normalize(l.mu * c + t * (s * cosf(phi)) + b * (s * sinf(phi)));
 }
 
-template <class M>
-PT_HD V3 guide_sample_folded(const M& m, V3 n, float u0, float u1, float u2) {
+PT_HD V3 guide_sample_folded(const GuideMixRef& m, V3 n, float u0, float u1, float u2) {
     const V3 w = guide_sample(m, u0, u1, u2);
     return dot(w, n) < 0.0f ? guide_mirror(w, n) : w;
 }

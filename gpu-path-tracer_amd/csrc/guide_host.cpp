@@ -90,7 +90,9 @@ GuideDev Renderer::guide_dev() const {
     g.grad = g_grad_.as<float>();
     g.m = g_m_.as<float>();
     g.v = g_v_.as<float>();
-    g.mix = g_mix_.as<float>();
+    g.mix = g_mix_.as<uint32_t>();
+    g.wf = g_wf_.as<uint16_t>();
+    g.bf = g_bf_.as<float>();
     g.mix_stride = g_pool_;
     g.recs = g_recs_.as<GuideRecord>();
     g.rec_capacity = g_rec_capacity_;
@@ -108,6 +110,8 @@ pt_status Renderer::ensure_guide() {
         PT_TRY_HIP(g_w32_.allocate(sizeof(float) * kGuideParams), "guide weights");
         PT_TRY_HIP(g_w_.allocate(sizeof(uint16_t) * kGuideLayers * kGuideDim * kGuideDim), "guide weights");
         PT_TRY_HIP(g_wt_.allocate(sizeof(uint16_t) * kGuideLayers * kGuideDim * kGuideDim), "guide weights");
+        PT_TRY_HIP(g_wf_.allocate(sizeof(uint16_t) * kGuideLayers * kGuideDim * kGuideDim), "guide weights");
+        PT_TRY_HIP(g_bf_.allocate(sizeof(float) * kGuideLayers * kGuideDim), "guide biases");
         PT_TRY_HIP(g_grad_.allocate(sizeof(float) * kGuideParams), "guide gradient");
         PT_TRY_HIP(g_m_.allocate(sizeof(float) * kGuideParams), "guide Adam state");
         PT_TRY_HIP(g_v_.allocate(sizeof(float) * kGuideParams), "guide Adam state");
@@ -132,7 +136,7 @@ pt_status Renderer::ensure_guide_pool(uint32_t cap) {
         g_sh_head_.release();
         g_rr_scale_[0].release();
         g_rr_scale_[1].release();
-        PT_TRY_HIP(g_mix_.allocate(sizeof(float) * kGuideMixComponents * (size_t)pool_capacity_), "guide mixtures");
+        PT_TRY_HIP(g_mix_.allocate(sizeof(uint32_t) * kGuideMixWords * (size_t)pool_capacity_), "guide mixtures");
         PT_TRY_HIP(g_head_[0].allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");
         PT_TRY_HIP(g_head_[1].allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");
         PT_TRY_HIP(g_sh_head_.allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");

@@ -1150,7 +1150,7 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                 // guide: draw slot 7 picks guide or BSDF; the mixture of this entry
                 const float gfrac = GUIDED ? p.guide.cfg.guide_fraction : 0.0f;
                 float u7 = 1.0f;
-                const MixSoA gm{p.guide.mix + e, p.guide.mix_stride};
+                const GuideMixRef gm{p.guide.mix + e, p.guide.mix_stride};
                 if (GUIDED && gfrac > 0.0f && q < 2) {
                     Draws d7 = dr;
                     for (int k = 0; k < 7; k++) d7.state = d7.state * kPcgMult + d7.inc;

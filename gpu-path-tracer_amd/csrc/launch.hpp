@@ -34,9 +34,11 @@ struct GuideDev {
     float* grad;
     float* m;                       // Adam moments
     float* v;
-    float* mix;                     // mixtures of the diffuse/GGX entries of the bounce ([diffuse | ggx]
-                                    // order), component-major with stride mix_stride (MixSoA)
+    uint32_t* mix;                  // mixtures of the diffuse/GGX entries of the bounce ([diffuse | ggx]
+                                    // order), kGuideMixWords packed words each, word-major with stride mix_stride
     uint32_t mix_stride;
+    uint16_t* wf;                   // bf16 weights in the inference kernel's fragment order
+    float* bf;                      // biases in the inference kernel's output order
     GuideRecord* recs;              // training records of the current run, Counters::records of them
     uint32_t rec_capacity;
     uint32_t* head[2];              // per path position: newest record of the path, kInvalid if none

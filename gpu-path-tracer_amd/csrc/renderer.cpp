@@ -314,7 +314,7 @@ pt_status Renderer::launch_iteration(int cur, bool count, const LoopMode& mode) 
     PT_TRY_HIP(launch_closest(stream_, p, sc, cur, count, mode.trace_multi, trace_), "closest-hit launch");
     PT_TRY_HIP(launch_bin(stream_, p, cur, bin_grid()), "binning launch");
     if (mode.guided && mode.guide_fraction > 0.0f)
-        PT_TRY_HIP(launch_guide_infer(stream_, p, sc, cur, (uint32_t)std::max(1, cu_count_) * 16u), "guide inference");
+        PT_TRY_HIP(launch_guide_infer(stream_, p, sc, cur, (uint32_t)std::max(1, cu_count_)), "guide inference");
     PT_TRY_HIP(launch_shade(stream_, p, sc, cur, mode.shade_multi, mode.guided, grid, 256), "shade launch");
     PT_TRY_HIP(launch_shadow(stream_, p, sc, count, mode.trace_multi, trace_), "shadow launch");
     PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
@@ -481,7 +481,7 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
             PT_TRY_HIP(launch_bin(stream_, p, cur, bin_grid()), "binning launch");
             PT_TRY_HIP(hipEventRecord(ev[2], stream_), "event");
             if (mode.guided && mode.guide_fraction > 0.0f)
-                PT_TRY_HIP(launch_guide_infer(stream_, p, sc, cur, (uint32_t)std::max(1, cu_count_) * 16u), "guide inference");
+                PT_TRY_HIP(launch_guide_infer(stream_, p, sc, cur, (uint32_t)std::max(1, cu_count_)), "guide inference");
             PT_TRY_HIP(launch_shade(stream_, p, sc, cur, mode.shade_multi, mode.guided, grid, 256), "shade launch");
             PT_TRY_HIP(hipEventRecord(ev[3], stream_), "event");
             PT_TRY_HIP(launch_shadow(stream_, p, sc, false, mode.trace_multi, trace_), "shadow launch");
