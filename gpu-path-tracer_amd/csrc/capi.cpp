@@ -190,6 +190,10 @@ pt_status pt_guide_forward(pt_renderer* r, const float* inputs, uint32_t count, 
     return guarded(r, [&](pt::Renderer& x) { return x.GuideForward(inputs, count, outputs); });
 }
 
+pt_status pt_guide_mixture(pt_renderer* r, const float* inputs, uint32_t count, float* lobes) {
+    return guarded(r, [&](pt::Renderer& x) { return x.GuideMixture(inputs, count, lobes); });
+}
+
 pt_status pt_guide_gradient(pt_renderer* r, const pt_guide_record* records, uint32_t count, float weight_max,
                             float* gradient, float* loss) {
     return guarded(r, [&](pt::Renderer& x) { return x.GuideGradient(records, count, weight_max, gradient, loss); });

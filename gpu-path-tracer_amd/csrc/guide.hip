@@ -311,37 +311,57 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_forward(const uint16_t* w
     }
 }
 
-// Mixture of every diffuse and GGX queue entry of the current bounce, in the
-// shade kernel's virtual order [diffuse | ggx], packed as in guide.hpp.
-__global__ __launch_bounds__(kInferBlock) void k_guide_infer(PathState p, SceneDev sc, int cur) {
+// Inputs of the inference kernel: the diffuse and GGX queue entries of the
+// current bounce, in the shade kernel's virtual order [diffuse | ggx].
+struct InferPathSrc {
+    PathState p;
+    SceneDev sc;
+    int cur;
+    __device__ __forceinline__ uint32_t count() const { return p.ctr->queue[0] + p.ctr->queue[1]; }
+    __device__ __forceinline__ void features(uint32_t e, float f[kGuideDim]) const {
+        const uint32_t c0 = p.ctr->queue[0];
+        const uint32_t i = e < c0 ? p.q[0][e] : p.q[1][e - c0];
+        const float4 o4 = (cur ? p.ray_o[1] : p.ray_o[0])[i];
+        const float4 d4 = (cur ? p.ray_d[1] : p.ray_d[0])[i];
+        const float4 hh = p.hit[i];
+        const V3 d = xyz(d4);
+        const V3 x = xyz(o4) + d * hh.x;
+        const float* gn = sc.shade[__float_as_uint(hh.w)].gn;
+        V3 n = v3(gn[0], gn[1], gn[2]);
+        if (dot(n, d) > 0.0f) n = -n;
+        guide_features_each(p.guide.cfg, x, n, -d, [&](int k, float v) { f[k] = v; });
+    }
+};
+
+// Inputs given as rows of 64 floats (hook for tests).
+struct InferFeatureSrc {
+    const float* x;
+    uint32_t n;
+    __device__ __forceinline__ uint32_t count() const { return n; }
+    __device__ __forceinline__ void features(uint32_t e, float f[kGuideDim]) const {
+#pragma unroll
+        for (int k = 0; k < kGuideDim; k++) f[k] = x[(size_t)e * kGuideDim + k];
+    }
+};
+
+// Packed mixtures (guide.hpp) of every input of `src`, word k of entry e at
+// mix[k * ms + e].
+template <class Src>
+__global__ __launch_bounds__(kInferBlock) void k_guide_infer(Src src, const uint16_t* wf, const float* bf, uint32_t* mix,
+                                                             uint32_t ms) {
     __shared__ bf16x8 wsh[kFragWeights];
     __shared__ float bsh[kGuideLayers * kGuideDim];
     __shared__ bf16x8 xs[kInferWaves][2 * 4 * 64];
-    stage_weights(wsh, bsh, p.guide.wf, p.guide.bf);
-    const uint32_t c0 = p.ctr->queue[0], c1 = p.ctr->queue[1];
-    const uint32_t total = c0 + c1;
-    const GuideConfig g = p.guide.cfg;
+    stage_weights(wsh, bsh, wf, bf);
+    const uint32_t total = src.count();
     const int wave = threadIdx.x >> 6, s = threadIdx.x & 63, r = s & 15, kq = s >> 4;
     const uint32_t tiles = (total + 63) / 64, stride = gridDim.x * kInferWaves;
-    uint32_t* mix = p.guide.mix;
-    const size_t ms = p.guide.mix_stride;
     for (uint32_t t = blockIdx.x * kInferWaves + wave; t < tiles; t += stride) {
         const uint32_t e = t * 64u + (uint32_t)s;
         float f[kGuideDim];
 #pragma unroll
         for (int i = 0; i < kGuideDim; i++) f[i] = 0.0f;
-        if (e < total) {
-            const uint32_t i = e < c0 ? p.q[0][e] : p.q[1][e - c0];
-            const float4 o4 = (cur ? p.ray_o[1] : p.ray_o[0])[i];
-            const float4 d4 = (cur ? p.ray_d[1] : p.ray_d[0])[i];
-            const float4 hh = p.hit[i];
-            const V3 d = xyz(d4);
-            const V3 x = xyz(o4) + d * hh.x;
-            const float* gn = sc.shade[__float_as_uint(hh.w)].gn;
-            V3 n = v3(gn[0], gn[1], gn[2]);
-            if (dot(n, d) > 0.0f) n = -n;
-            guide_features_each(g, x, n, -d, [&](int k, float v) { f[k] = v; });
-        }
+        if (e < total) src.features(e, f);
         store_input_frags(xs[wave], s, f);
         wave_lds_sync();
         f32x4 o[4][4];
@@ -557,7 +577,17 @@ hipError_t launch_guide_forward(hipStream_t s, const GuideDev& g, const float* f
 }
 
 hipError_t launch_guide_infer(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, uint32_t grid) {
-    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(kInferBlock), 0, s, p, sc, cur);
+    hipLaunchKernelGGL(k_guide_infer<InferPathSrc>, dim3(grid), dim3(kInferBlock), 0, s, InferPathSrc{p, sc, cur},
+                       p.guide.wf, p.guide.bf, p.guide.mix, p.guide.mix_stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_guide_mixture(hipStream_t s, const GuideDev& g, const float* features, uint32_t n, uint32_t* words) {
+    if (n == 0) return hipSuccess;
+    const uint32_t tiles = (n + 63) / 64;
+    const uint32_t grid = std::max(1u, std::min((tiles + kInferWaves - 1) / kInferWaves, 1024u));
+    hipLaunchKernelGGL(k_guide_infer<InferFeatureSrc>, dim3(grid), dim3(kInferBlock), 0, s, InferFeatureSrc{features, n},
+                       g.wf, g.bf, words, n);
     return hipGetLastError();
 }
 

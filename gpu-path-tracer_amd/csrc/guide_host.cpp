@@ -230,6 +230,32 @@ pt_status Renderer::GuideForward(const float* in, uint32_t n, float* out) {
     return PT_OK;
 }
 
+pt_status Renderer::GuideMixture(const float* in, uint32_t n, float* lobes) {
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    if (n == 0) return PT_OK;
+    if (!in || !lobes) return fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
+    PT_TRY(ensure_guide());
+    const size_t in_bytes = (size_t)n * kGuideDim * sizeof(float);
+    const size_t words = (size_t)n * kGuideMixWords;
+    PT_TRY_HIP(d_hook_in_.allocate(in_bytes), "hook input");
+    PT_TRY_HIP(d_hook_out_.allocate(words * sizeof(uint32_t)), "hook output");
+    PT_TRY_HIP(hipMemcpyAsync(d_hook_in_.as<void>(), in, in_bytes, hipMemcpyHostToDevice, stream_), "hook input");
+    PT_TRY_HIP(launch_guide_mixture(stream_, guide_dev(), d_hook_in_.as<float>(), n, d_hook_out_.as<uint32_t>()),
+               "guide mixture");
+    std::vector<uint32_t> w(words);
+    PT_TRY_HIP(hipMemcpyAsync(w.data(), d_hook_out_.as<void>(), words * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+               "hook output");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "hook sync");
+    // word 4 c + t of entry e holds component c of lobes t (low half) and t + 4
+    for (size_t e = 0; e < n; e++)
+        for (int j = 0; j < kGuideLobes; j++)
+            for (int c = 0; c < 5; c++) {
+                const uint32_t v = w[(size_t)(4 * c + (j & 3)) * n + e];
+                lobes[(e * kGuideLobes + j) * 5 + c] = j < 4 ? guide_half_lo(v) : guide_half_hi(v);
+            }
+    return PT_OK;
+}
+
 pt_status Renderer::GuideGradient(const pt_guide_record* recs, uint32_t n, float w_max, float* grad, float* loss) {
     if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
     if (!recs || !grad || !loss) return fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");

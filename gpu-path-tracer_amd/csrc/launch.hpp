@@ -123,6 +123,7 @@ hipError_t launch_alias_sample(hipStream_t s, const SceneDev& sc, const float* u
 // Guide network (guide.hip).
 hipError_t launch_guide_sync(hipStream_t s, const GuideDev& g);
 hipError_t launch_guide_forward(hipStream_t s, const GuideDev& g, const float* features, uint32_t n, float* out);
+hipError_t launch_guide_mixture(hipStream_t s, const GuideDev& g, const float* features, uint32_t n, uint32_t* words);
 hipError_t launch_guide_infer(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, uint32_t grid);
 hipError_t launch_guide_train(hipStream_t s, const GuideDev& g, const GuideRecord* recs, uint32_t count,
                               uint32_t batch, uint32_t seed, float w_max, float* loss);

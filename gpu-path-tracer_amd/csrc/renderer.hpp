@@ -90,6 +90,7 @@ public:
     pt_status GuideSetWeights(const float* w, uint32_t n);
     pt_status GuideSetBox(const float lo[3], const float hi[3]);
     pt_status GuideForward(const float* features, uint32_t n, float* out);
+    pt_status GuideMixture(const float* features, uint32_t n, float* lobes);
     pt_status GuideGradient(const pt_guide_record* recs, uint32_t n, float w_max, float* grad, float* loss);
     pt_status GuideTrain(const pt_guide_record* recs, uint32_t n, uint32_t steps, uint32_t batch, float lr, float w_max,
                          float* losses);

@@ -165,6 +165,7 @@ SIGNATURES = {
     "pt_guide_set_weights": (C.c_int, [_P, _P, _U32]),
     "pt_guide_set_box": (C.c_int, [_P, _P, _P]),
     "pt_guide_forward": (C.c_int, [_P, _P, _U32, _P]),
+    "pt_guide_mixture": (C.c_int, [_P, _P, _U32, _P]),
     "pt_guide_gradient": (C.c_int, [_P, _P, _U32, C.c_float, _P, _P]),
     "pt_guide_train": (C.c_int, [_P, _P, _U32, _U32, _U32, C.c_float, C.c_float, _P]),
     "pt_trace_closest": (C.c_int, [_P, _P, _U32, _P]),
@@ -412,6 +413,13 @@ class Renderer:
         out = np.zeros((len(x), GUIDE_DIM), np.float32)
         self._check(self.lib.pt_guide_forward(self.h, _ptr(x), len(x), _ptr(out)))
         return out
+
+    def guide_mixture(self, inputs):
+        """(mu [n][8][3], kappa [n][8], pi [n][8]) as the inference kernel stores them."""
+        x = np.ascontiguousarray(inputs, np.float32)
+        out = np.zeros((len(x), 8, 5), np.float32)
+        self._check(self.lib.pt_guide_mixture(self.h, _ptr(x), len(x), _ptr(out)))
+        return out[..., :3], out[..., 3], out[..., 4]
 
     def guide_gradient(self, records, weight_max=1e30):
         r = np.ascontiguousarray(records, GUIDE_RECORD_DTYPE)

@@ -401,6 +401,9 @@ pt_status pt_guide_set_weights(pt_renderer* renderer, const float* weights, uint
 pt_status pt_guide_set_box(pt_renderer* renderer, const float lo[3], const float hi[3]);
 /* Raw network outputs [n][64] of inputs [n][64] (the MFMA forward pass). */
 pt_status pt_guide_forward(pt_renderer* renderer, const float* inputs, uint32_t count, float* outputs);
+/* The mixtures the renderer's inference kernel stores for inputs [n][64], as it
+ * stores them (fp16, decoded): lobes[n][8][5] = mu xyz, kappa, pi. */
+pt_status pt_guide_mixture(pt_renderer* renderer, const float* inputs, uint32_t count, float* lobes);
 /* Gradient (PT_GUIDE_PARAMS floats) of sum_i -w_i log q(wi_i) over the records, and
  * the loss, without changing the network. */
 pt_status pt_guide_gradient(pt_renderer* renderer, const pt_guide_record* records, uint32_t count,

@@ -50,6 +50,32 @@ def test_guide_forward_matches_oracle(gpu_renderer):
     assert np.allclose(gpu_renderer.guide_forward(x[perm]), got[perm], rtol=1e-6, atol=1e-6)
 
 
+def test_guide_mixture_matches_oracle(gpu_renderer):
+    """The inference kernel's stored mixtures (fp16) against the oracle's
+    mixture transform of the same network: the renderer's kernel on inputs
+    given as features, compared with the oracle's bf16 forward pass and with
+    the mixture of the MFMA forward's own raw outputs (which isolates the
+    output transform and the fp16 packing)."""
+    w = G.init_weights(6, scale=3.0)
+    gpu_renderer.guide_set_weights(w)
+    rng = np.random.default_rng(9)
+    n = 1000                                                   # a partial last tile
+    x = G.features(np.zeros(3), np.ones(3), rng.uniform(0.0, 1.0, (n, 3)), _unit(rng.normal(size=(n, 3))),
+                   _unit(rng.normal(size=(n, 3)))).astype(np.float32)
+    mu, kappa, pi = gpu_renderer.guide_mixture(x)
+    mu_r, kappa_r, pi_r = G.mixture(gpu_renderer.guide_forward(x))
+    assert np.abs(mu - mu_r).max() < 1e-3
+    assert np.abs(kappa / kappa_r - 1.0).max() < 1e-3
+    assert np.abs(pi - pi_r).max() < 1e-3
+    assert np.abs(np.linalg.norm(mu, axis=-1) - 1.0).max() < 2e-3
+    assert np.abs(pi.sum(-1) - 1.0).max() < 4e-3
+    mu_o, kappa_o, pi_o = G.mixture(G.forward(w, x, round=True)[0])
+    assert kappa_r.std() > 0.1 and pi_r.std() > 0.01           # the transform sees spread-out outputs
+    assert np.abs(pi - pi_o).max() < 2e-2
+    assert np.median(np.abs(mu - mu_o)) < 1e-2
+    assert np.median(np.abs(kappa / kappa_o - 1.0)) < 2e-2
+
+
 def test_guide_gradient_matches_oracle(gpu_renderer):
     w = G.init_weights(4)
     gpu_renderer.guide_set_weights(w)
