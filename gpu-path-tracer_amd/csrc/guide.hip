@@ -149,20 +149,26 @@ __device__ __forceinline__ const float* bias_of(const GuideNet& n, int l) {
 // bf16 weights of all layers (32 KB) are staged in LDS once per block.
 //
 // The output layer's rows are reordered so that lane kq of output tile 0 (1)
-// holds lobe kq's (4 + kq's) mu xyz and log kappa, and tile 2 the 8 logits:
-// inference output o' maps to network output guide_out_row(o').
+// holds lobe kq's (4 + kq's) mu xyz and log kappa, and of tile 2 the logits of
+// lobes kq and 4 + kq (then two unused outputs): inference output o' maps to
+// network output guide_out_row(o').
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 PT_HD int guide_out_row(int o) {
     if (o < 32) return 5 * (o >> 2) + (o & 3);
-    if (o < 40) return 5 * (o - 32) + 4;
+    if (o < 48) {
+        const int kq = (o - 32) >> 2, c = (o - 32) & 3;
+        return c == 0 ? 5 * kq + 4 : c == 1 ? 5 * (4 + kq) + 4 : 40 + 2 * kq + (c - 2);
+    }
     return o;
 }
 PT_HD int guide_out_slot(int row) {     // inverse of guide_out_row
-    if (row >= 40) return row;
+    if (row >= 48) return row;
+    if (row >= 40) return 32 + 4 * ((row - 40) >> 1) + 2 + ((row - 40) & 1);
     const int j = row / 5, c = row % 5;
-    return c < 4 ? 4 * j + c : 32 + j;
+    if (c < 4) return 4 * j + c;
+    return j < 4 ? 32 + 4 * j : 32 + 4 * (j - 4) + 1;
 }
 
 // Position of weight W_l[o'][i] (o' in inference order) in the fragment-ordered
@@ -173,9 +179,23 @@ PT_HD uint32_t guide_frag_index(int l, int o, int i) {
     return ((((uint32_t)l * 4u + (uint32_t)mt) * 2u + (uint32_t)kt) * 64u + (uint32_t)(r + 16 * kq)) * 8u + (uint32_t)j;
 }
 
-constexpr int kInferWaves = 12;                       // waves per block, one block per CU
-constexpr int kInferBlock = 64 * kInferWaves;
 constexpr int kFragWeights = kGuideLayers * 4 * 2 * 64;   // bf16x8 fragments of all layers
+
+// LDS of an inference block: the weight fragments of layers 0-2 and of the
+// first MT 16-row tiles of the output layer, the biases, and one input tile
+// per wave.  One block per CU: 16 waves when the unused output rows are left
+// out (159 KB), 12 with every output row.
+template <int WAVES, int MT>
+struct InferLds {
+    static constexpr int kWeights = (3 * 4 + MT) * 2 * 64;
+    bf16x8 w[kWeights];
+    float b[kGuideLayers * kGuideDim];
+    bf16x8 x[WAVES][2 * 4 * 64];
+};
+constexpr int kInferWaves = 16;                       // waves of an inference block
+constexpr int kForwardWaves = 12;                     // waves of a (hook) forward block
+static_assert(sizeof(InferLds<kInferWaves, 3>) <= 160 * 1024, "inference LDS");
+static_assert(sizeof(InferLds<kForwardWaves, 4>) <= 160 * 1024, "forward LDS");
 
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -183,33 +203,50 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ __bf16 bf16_of(float f) {
-    const uint16_t h = to_bf16(f);
-    __bf16 b;
-    __builtin_memcpy(&b, &h, 2);
-    return b;
+typedef float f32x8 __attribute__((ext_vector_type(8)));
+typedef short i16x8 __attribute__((ext_vector_type(8)));
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// Two floats to packed bf16 (round to nearest even, as to_bf16): one
+// v_cvt_pk_bf16_f32.
+__device__ __forceinline__ uint32_t cvt_pk_bf16(float lo, float hi) {
+    const bf16x2 h = __builtin_convertvector(f32x2{lo, hi}, bf16x2);
+    uint32_t r;
+    __builtin_memcpy(&r, &h, sizeof(r));
+    return r;
+}
+
+__device__ __forceinline__ bf16x8 to_bf16x8(const f32x8 v) {
+    const uint4 u = make_uint4(cvt_pk_bf16(v[0], v[1]), cvt_pk_bf16(v[2], v[3]), cvt_pk_bf16(v[4], v[5]),
+                               cvt_pk_bf16(v[6], v[7]));
+    bf16x8 r;
+    __builtin_memcpy(&r, &u, sizeof(r));
+    return r;
 }
 
 // Writes sample s's 64 inputs into the wave's fragment tile
-// xf[kt][nt][lane] (bf16x8), nt = s / 16.
+// xf[kt][nt][lane] (bf16x8), nt = s / 16 (round to nearest even, as to_bf16).
 __device__ __forceinline__ void store_input_frags(bf16x8* xf, int s, const float f[kGuideDim]) {
     const int nt = s >> 4, r = s & 15;
 #pragma unroll
     for (int kt = 0; kt < 2; kt++)
 #pragma unroll
         for (int kq = 0; kq < 4; kq++) {
-            bf16x8 v;
+            f32x8 v;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                v[j] = bf16_of(f[32 * kt + 4 * kq + j]);
-                v[4 + j] = bf16_of(f[32 * kt + 16 + 4 * kq + j]);
+                v[j] = f[32 * kt + 4 * kq + j];
+                v[4 + j] = f[32 * kt + 16 + 4 * kq + j];
             }
-            xf[(kt * 4 + nt) * 64 + r + 16 * kq] = v;
+            xf[(kt * 4 + nt) * 64 + r + 16 * kq] = to_bf16x8(v);
         }
 }
 
 // Forward pass of the wave's 64-sample tile: hidden layers in registers, the
-// output layer's first MT 16-row tiles into out[mt][nt].
+// output layer's first MT 16-row tiles into out[mt][nt].  The accumulators
+// start from the biases.
 template <int MT>
 __device__ __forceinline__ void tile_forward(const bf16x8* wsh, const float* bsh, const bf16x8* xf, f32x4 out[4][4]) {
     const int lane = threadIdx.x & 63, kq = lane >> 4;
@@ -221,10 +258,13 @@ __device__ __forceinline__ void tile_forward(const bf16x8* wsh, const float* bsh
 #pragma unroll
     for (int l = 0; l < kGuideLayers; l++) {
         const int mts = l + 1 < kGuideLayers ? 4 : MT;
+        const float* bias = bsh + l * kGuideDim;
 #pragma unroll
-        for (int mt = 0; mt < 4; mt++)
+        for (int mt = 0; mt < 4; mt++) {
+            const float4 bb = *reinterpret_cast<const float4*>(bias + 16 * mt + 4 * kq);
 #pragma unroll
-            for (int nt = 0; nt < 4; nt++) out[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int nt = 0; nt < 4; nt++) out[mt][nt] = f32x4{bb.x, bb.y, bb.z, bb.w};
+        }
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) {
             if (mt >= mts) break;
@@ -236,60 +276,48 @@ __device__ __forceinline__ void tile_forward(const bf16x8* wsh, const float* bsh
                     out[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[kt][nt], out[mt][nt], 0, 0, 0);
             }
         }
-        const float* bias = bsh + l * kGuideDim;
-        if (l + 1 == kGuideLayers) {
+        if (l + 1 == kGuideLayers) break;
+        // ReLU -> the next layer's B operands: round to bf16, then clear the
+        // negative values as 16-bit integers (a set sign bit is a negative
+        // int16), which equals rounding the ReLU output
 #pragma unroll
-            for (int mt = 0; mt < 4; mt++) {
-                if (mt >= mts) break;
-                const float4 bb = *reinterpret_cast<const float4*>(bias + 16 * mt + 4 * kq);
-#pragma unroll
-                for (int nt = 0; nt < 4; nt++) {
-                    out[mt][nt][0] += bb.x;
-                    out[mt][nt][1] += bb.y;
-                    out[mt][nt][2] += bb.z;
-                    out[mt][nt][3] += bb.w;
-                }
-            }
-            break;
-        }
-        // ReLU(out + bias) -> the next layer's B operands
-#pragma unroll
-        for (int kt = 0; kt < 2; kt++) {
-            const float4 b0 = *reinterpret_cast<const float4*>(bias + 32 * kt + 4 * kq);
-            const float4 b1 = *reinterpret_cast<const float4*>(bias + 32 * kt + 16 + 4 * kq);
-            const float lo[4] = {b0.x, b0.y, b0.z, b0.w}, hi[4] = {b1.x, b1.y, b1.z, b1.w};
+        for (int kt = 0; kt < 2; kt++)
 #pragma unroll
             for (int nt = 0; nt < 4; nt++) {
-                bf16x8 v;
+                f32x8 v;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    v[j] = bf16_of(fmaxf(out[2 * kt][nt][j] + lo[j], 0.0f));
-                    v[4 + j] = bf16_of(fmaxf(out[2 * kt + 1][nt][j] + hi[j], 0.0f));
+                    v[j] = out[2 * kt][nt][j];
+                    v[4 + j] = out[2 * kt + 1][nt][j];
                 }
-                b[kt][nt] = v;
+                const bf16x8 h = to_bf16x8(v);
+                i16x8 iv;
+                __builtin_memcpy(&iv, &h, sizeof(iv));
+                iv = __builtin_elementwise_max(iv, i16x8{0, 0, 0, 0, 0, 0, 0, 0});
+                __builtin_memcpy(&b[kt][nt], &iv, sizeof(iv));
             }
-        }
     }
 }
 
-// Stages the fragment-ordered weights and the biases in LDS (whole block).
-__device__ __forceinline__ void stage_weights(bf16x8* wsh, float* bsh, const uint16_t* wf, const float* bf) {
+// Stages the first `frags` weight fragments and the biases in LDS (whole block).
+__device__ __forceinline__ void stage_weights(bf16x8* wsh, float* bsh, const uint16_t* wf, const float* bf, int frags) {
     const bf16x8* src = reinterpret_cast<const bf16x8*>(wf);
-    for (int k = threadIdx.x; k < kFragWeights; k += blockDim.x) wsh[k] = src[k];
+    for (int k = threadIdx.x; k < frags; k += blockDim.x) wsh[k] = src[k];
     for (int k = threadIdx.x; k < kGuideLayers * kGuideDim; k += blockDim.x) bsh[k] = bf[k];
     __syncthreads();
 }
 
 // Raw outputs of given inputs (hook for tests), in the network's output order.
-__global__ __launch_bounds__(kInferBlock) void k_guide_forward(const uint16_t* wf, const float* bf, const float* features,
-                                                               uint32_t n, float* out) {
-    __shared__ bf16x8 wsh[kFragWeights];
-    __shared__ float bsh[kGuideLayers * kGuideDim];
-    __shared__ bf16x8 xs[kInferWaves][2 * 4 * 64];
-    stage_weights(wsh, bsh, wf, bf);
+__global__ __launch_bounds__(64 * kForwardWaves) void k_guide_forward(const uint16_t* wf, const float* bf,
+                                                                      const float* features, uint32_t n, float* out) {
+    __shared__ InferLds<kForwardWaves, 4> lds;
+    bf16x8* wsh = lds.w;
+    float* bsh = lds.b;
+    bf16x8 (*xs)[2 * 4 * 64] = lds.x;
+    stage_weights(wsh, bsh, wf, bf, InferLds<kForwardWaves, 4>::kWeights);
     const int wave = threadIdx.x >> 6, s = threadIdx.x & 63, r = s & 15, kq = s >> 4;
-    const uint32_t tiles = (n + 63) / 64, stride = gridDim.x * kInferWaves;
-    for (uint32_t t = blockIdx.x * kInferWaves + wave; t < tiles; t += stride) {
+    const uint32_t tiles = (n + 63) / 64, stride = gridDim.x * kForwardWaves;
+    for (uint32_t t = blockIdx.x * kForwardWaves + wave; t < tiles; t += stride) {
         const uint32_t idx = t * 64u + (uint32_t)s;
         float f[kGuideDim];
 #pragma unroll
@@ -347,12 +375,13 @@ struct InferFeatureSrc {
 // Packed mixtures (guide.hpp) of every input of `src`, word k of entry e at
 // mix[k * ms + e].
 template <class Src>
-__global__ __launch_bounds__(kInferBlock) void k_guide_infer(Src src, const uint16_t* wf, const float* bf, uint32_t* mix,
-                                                             uint32_t ms) {
-    __shared__ bf16x8 wsh[kFragWeights];
-    __shared__ float bsh[kGuideLayers * kGuideDim];
-    __shared__ bf16x8 xs[kInferWaves][2 * 4 * 64];
-    stage_weights(wsh, bsh, wf, bf);
+__global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(Src src, const uint16_t* wf, const float* bf,
+                                                                 uint32_t* mix, uint32_t ms) {
+    __shared__ InferLds<kInferWaves, 3> lds;
+    bf16x8* wsh = lds.w;
+    float* bsh = lds.b;
+    bf16x8 (*xs)[2 * 4 * 64] = lds.x;
+    stage_weights(wsh, bsh, wf, bf, InferLds<kInferWaves, 3>::kWeights);
     const uint32_t total = src.count();
     const int wave = threadIdx.x >> 6, s = threadIdx.x & 63, r = s & 15, kq = s >> 4;
     const uint32_t tiles = (total + 63) / 64, stride = gridDim.x * kInferWaves;
@@ -368,41 +397,32 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(Src src, const uint
         tile_forward<3>(wsh, bsh, xs[wave], o);
         wave_lds_sync();
         // lane (r, kq) holds, for samples 16 nt + r: lobe kq in o[0], lobe 4 + kq
-        // in o[1], logits 4 kq + j in o[2] (kq < 2)
+        // in o[1], their logits in o[2][.][0] and [1]; the softmax spans the
+        // four lanes r + 16 kq.  The stored fp16 mixture is renormalised where
+        // it is read, so fast exp and reciprocals suffice here.
 #pragma unroll
         for (int nt = 0; nt < 4; nt++) {
-            float lg[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) lg[j] = o[2][nt][j];
-            float mx = fmaxf(fmaxf(lg[0], lg[1]), fmaxf(lg[2], lg[3]));
+            const float la = o[2][nt][0], lb = o[2][nt][1];
+            float mx = fmaxf(la, lb);
             mx = fmaxf(mx, __shfl_xor(mx, 16));
-            float ex[4], sum = 0.0f;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                ex[j] = expf(lg[j] - mx);
-                sum += ex[j];
-            }
+            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            const float ea = __expf(la - mx), eb = __expf(lb - mx);
+            float sum = ea + eb;
             sum += __shfl_xor(sum, 16);
-            const float inv = 1.0f / sum;
-            // pi of lobe kq sits in lane r (component kq), of lobe 4 + kq in lane r + 16
-            float pa = 0.0f, pb = 0.0f;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const float va = __shfl(ex[j] * inv, r), vb = __shfl(ex[j] * inv, r + 16);
-                pa = kq == j ? va : pa;
-                pb = kq == j ? vb : pb;
-            }
+            sum += __shfl_xor(sum, 32);
+            const float inv = __frcp_rn(sum);
+            const float pa = ea * inv, pb = eb * inv;
             float mu[2][3], kap[2];
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const f32x4 v = o[h][nt];
-                const float len = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                const bool ok = len > 1e-12f;
-                const float il = 1.0f / fmaxf(len, 1e-12f);
+                const float len2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                const bool ok = len2 > 1e-24f;
+                const float il = rsqrtf(fmaxf(len2, 1e-24f));
                 mu[h][0] = ok ? v[0] * il : 0.0f;
                 mu[h][1] = ok ? v[1] * il : 0.0f;
                 mu[h][2] = ok ? v[2] * il : 1.0f;
-                kap[h] = expf(clampf(v[3], kGuideLogKappaMin, kGuideLogKappaMax));
+                kap[h] = __expf(clampf(v[3], kGuideLogKappaMin, kGuideLogKappaMax));
             }
             const uint32_t ee = t * 64u + 16u * nt + (uint32_t)r;
             if (ee < total) {
@@ -438,11 +458,12 @@ PT_HD float guide_weight(const GuideRecord& r, float w_max) {
 }
 
 // Gradient of sum_b -w_b log q(wi_b) over `batch` records (picked by hash
-// when `seed` != 0, else records [0, batch)), added to grad; loss summed into
-// *loss.
+// when `seed` != 0, else records [0, batch)): block k writes the gradient of
+// its 64 records to part[k][kGuideParams] (no atomics: the sum over blocks is
+// taken by the Adam step); the loss is summed into *loss.
 __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const GuideRecord* recs, uint32_t count,
                                                    uint32_t batch, uint32_t seed, GuideConfig g, float w_max,
-                                                   float* grad, float* loss) {
+                                                   float* part, float* loss) {
     __shared__ uint16_t x0[64 * kLd], h1[64 * kLd], h2[64 * kLd], h3[64 * kLd], d4[64 * kLd];
     __shared__ float o32[64 * kLdOut];
     const int s = threadIdx.x;
@@ -488,7 +509,7 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const GuideRec
 #pragma unroll
     for (int l = kGuideLayers - 1; l >= 0; l--) {
         uint16_t* act = l == 0 ? x0 : l == 1 ? h1 : l == 2 ? h2 : h3;   // input of layer l
-        float* gw = grad + l * kGuideLayerParams;
+        float* gw = part + (size_t)blockIdx.x * kGuideParams + l * kGuideLayerParams;
         f32x4 acc[4][4];
         mm_dtx(dl, act, acc);
 #pragma unroll
@@ -497,10 +518,10 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const GuideRec
             for (int nt = 0; nt < 4; nt++)
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    atomicAdd(gw + (16 * mt + 4 * kq + j) * kGuideDim + 16 * nt + r, acc[mt][nt][j]);
+                    gw[(16 * mt + 4 * kq + j) * kGuideDim + 16 * nt + r] = acc[mt][nt][j];
         float db = 0.0f;
         for (int t = 0; t < 64; t++) db += from_bf16(dl[t * kLd + lane]);
-        atomicAdd(gw + kGuideDim * kGuideDim + lane, db);
+        gw[kGuideDim * kGuideDim + lane] = db;
         __syncthreads();
         if (l == 0) break;
         // delta of layer l-1 outputs = (dl W_l) * relu'(act), written over act
@@ -537,19 +558,44 @@ __device__ __forceinline__ void guide_copies(uint32_t k, float x, uint16_t* w, u
     }
 }
 
-__global__ void k_guide_adam(float* w32, float* grad, float* m, float* v, uint16_t* w, uint16_t* wt, uint16_t* wf,
-                             float* bf, float lr, float beta1, float beta2, float eps, float c1, float c2,
-                             float grad_scale) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (uint32_t)kGuideParams) return;
-    const float gk = grad[k] * grad_scale;
+// Sum of the partial gradients of parameters [64 blockIdx.x, +64): the four
+// waves of the block sum every fourth partial (coalesced 256-byte rows), then
+// combine through LDS.  Returns the sum in the first wave (other waves: 0).
+constexpr int kSumBlock = 256;
+__device__ __forceinline__ float sum_parts(const float* part, uint32_t parts, uint32_t k) {
+    __shared__ float red[kSumBlock];
+    const uint32_t wave = threadIdx.x >> 6;
+    float s = 0.0f;
+    if (k < (uint32_t)kGuideParams)
+        for (uint32_t b = wave; b < parts; b += kSumBlock / 64) s += part[(size_t)b * kGuideParams + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    float t = 0.0f;
+    if (wave == 0) t = red[threadIdx.x] + red[threadIdx.x + 64] + red[threadIdx.x + 128] + red[threadIdx.x + 192];
+    return t;
+}
+
+// Gradient = sum of the per-block partial gradients (hook for tests).
+__global__ __launch_bounds__(kSumBlock) void k_guide_sum(const float* part, uint32_t parts, float* grad) {
+    const uint32_t k = blockIdx.x * 64u + (threadIdx.x & 63u);
+    const float g = sum_parts(part, parts, k);
+    if (threadIdx.x < 64u && k < (uint32_t)kGuideParams) grad[k] = g;
+}
+
+__global__ __launch_bounds__(kSumBlock) void k_guide_adam(float* w32, const float* part, uint32_t parts, float* m,
+                                                          float* v, uint16_t* w, uint16_t* wt, uint16_t* wf, float* bf,
+                                                          float lr, float beta1, float beta2, float eps, float c1,
+                                                          float c2, float grad_scale) {
+    const uint32_t k = blockIdx.x * 64u + (threadIdx.x & 63u);
+    const float gsum = sum_parts(part, parts, k);
+    if (threadIdx.x >= 64u || k >= (uint32_t)kGuideParams) return;
+    const float gk = gsum * grad_scale;
     const float mk = beta1 * m[k] + (1.0f - beta1) * gk;
     const float vk = beta2 * v[k] + (1.0f - beta2) * gk * gk;
     m[k] = mk;
     v[k] = vk;
     const float wk = w32[k] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
     w32[k] = wk;
-    grad[k] = 0.0f;
     guide_copies(k, wk, w, wt, wf, bf);
 }
 
@@ -571,13 +617,13 @@ hipError_t launch_guide_sync(hipStream_t s, const GuideDev& g) {
 hipError_t launch_guide_forward(hipStream_t s, const GuideDev& g, const float* features, uint32_t n, float* out) {
     if (n == 0) return hipSuccess;
     const uint32_t tiles = (n + 63) / 64;
-    const uint32_t grid = std::max(1u, std::min((tiles + kInferWaves - 1) / kInferWaves, 1024u));
-    hipLaunchKernelGGL(k_guide_forward, dim3(grid), dim3(kInferBlock), 0, s, g.wf, g.bf, features, n, out);
+    const uint32_t grid = std::max(1u, std::min((tiles + kForwardWaves - 1) / kForwardWaves, 1024u));
+    hipLaunchKernelGGL(k_guide_forward, dim3(grid), dim3(64 * kForwardWaves), 0, s, g.wf, g.bf, features, n, out);
     return hipGetLastError();
 }
 
 hipError_t launch_guide_infer(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, uint32_t grid) {
-    hipLaunchKernelGGL(k_guide_infer<InferPathSrc>, dim3(grid), dim3(kInferBlock), 0, s, InferPathSrc{p, sc, cur},
+    hipLaunchKernelGGL(k_guide_infer<InferPathSrc>, dim3(grid), dim3(64 * kInferWaves), 0, s, InferPathSrc{p, sc, cur},
                        p.guide.wf, p.guide.bf, p.guide.mix, p.guide.mix_stride);
     return hipGetLastError();
 }
@@ -586,25 +632,30 @@ hipError_t launch_guide_mixture(hipStream_t s, const GuideDev& g, const float* f
     if (n == 0) return hipSuccess;
     const uint32_t tiles = (n + 63) / 64;
     const uint32_t grid = std::max(1u, std::min((tiles + kInferWaves - 1) / kInferWaves, 1024u));
-    hipLaunchKernelGGL(k_guide_infer<InferFeatureSrc>, dim3(grid), dim3(kInferBlock), 0, s, InferFeatureSrc{features, n},
+    hipLaunchKernelGGL(k_guide_infer<InferFeatureSrc>, dim3(grid), dim3(64 * kInferWaves), 0, s, InferFeatureSrc{features, n},
                        g.wf, g.bf, words, n);
     return hipGetLastError();
 }
 
 hipError_t launch_guide_train(hipStream_t s, const GuideDev& g, const GuideRecord* recs, uint32_t count,
-                              uint32_t batch, uint32_t seed, float w_max, float* loss) {
+                              uint32_t batch, uint32_t seed, float w_max, float* part, float* loss) {
     if (batch == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_guide_train, dim3((batch + 63) / 64), dim3(64), 0, s, GuideNet{g.w, g.wt, g.w32}, recs,
-                       count, batch, seed, g.cfg, w_max, g.grad, loss);
+    hipLaunchKernelGGL(k_guide_train, dim3(guide_parts(batch)), dim3(64), 0, s, GuideNet{g.w, g.wt, g.w32}, recs,
+                       count, batch, seed, g.cfg, w_max, part, loss);
     return hipGetLastError();
 }
 
-hipError_t launch_guide_adam(hipStream_t s, const GuideDev& g, float lr, uint32_t step, float grad_scale) {
+hipError_t launch_guide_sum(hipStream_t s, const float* part, uint32_t parts, float* grad) {
+    hipLaunchKernelGGL(k_guide_sum, dim3((kGuideParams + 63) / 64), dim3(kSumBlock), 0, s, part, parts, grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_guide_adam(hipStream_t s, const GuideDev& g, const float* part, uint32_t parts, float lr,
+                             uint32_t step, float grad_scale) {
     const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
     const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
-    hipLaunchKernelGGL(k_guide_adam, dim3((kGuideParams + 255) / 256), dim3(256), 0, s, g.w32, g.grad, g.m, g.v, g.w,
-                       g.wt, g.wf, g.bf, lr, beta1, beta2, eps, c1, c2,
-                       grad_scale);
+    hipLaunchKernelGGL(k_guide_adam, dim3((kGuideParams + 63) / 64), dim3(kSumBlock), 0, s, g.w32, part, parts, g.m, g.v,
+                       g.w, g.wt, g.wf, g.bf, lr, beta1, beta2, eps, c1, c2, grad_scale);
     return hipGetLastError();
 }
 

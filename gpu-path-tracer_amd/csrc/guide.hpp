@@ -75,7 +75,7 @@ struct GuideConfig {
 // normalised position, its sin/cos frequency encoding, the normal and the
 // outgoing direction (45 values; the rest of the 64 inputs are zero).
 template <class Put>
-PT_HD void guide_features_each(const GuideConfig& g, V3 p, V3 n, V3 wo, Put put) {
+__device__ void guide_features_each(const GuideConfig& g, V3 p, V3 n, V3 wo, Put put) {
     const float q[3] = {(p.x - g.lo[0]) * g.inv_extent[0], (p.y - g.lo[1]) * g.inv_extent[1],
                         (p.z - g.lo[2]) * g.inv_extent[2]};
     int k = 0;
@@ -83,7 +83,7 @@ PT_HD void guide_features_each(const GuideConfig& g, V3 p, V3 n, V3 wo, Put put)
     for (int a = 0; a < 3; a++) {
         // octave 0 directly, the others by doubling: sin 2x = 2 sin x cos x,
         // cos 2x = 1 - 2 sin^2 x
-        float sn = sinf(kPi * q[a]), cs = cosf(kPi * q[a]);
+        float sn = __sinf(kPi * q[a]), cs = __cosf(kPi * q[a]);
         for (int o = 0; o < kGuideOctaves; o++) {
             put(k++, sn);
             put(k++, cs);
@@ -100,7 +100,7 @@ PT_HD void guide_features_each(const GuideConfig& g, V3 p, V3 n, V3 wo, Put put)
     put(k++, wo.z);
 }
 
-PT_HD void guide_features(const GuideConfig& g, V3 p, V3 n, V3 wo, float f[kGuideFeatures]) {
+__device__ void guide_features(const GuideConfig& g, V3 p, V3 n, V3 wo, float f[kGuideFeatures]) {
     guide_features_each(g, p, n, wo, [&](int k, float v) { f[k] = v; });
 }
 

@@ -263,13 +263,18 @@ pt_status Renderer::GuideGradient(const pt_guide_record* recs, uint32_t n, float
     PT_TRY_HIP(g_loss_.allocate(sizeof(float) * 16), "guide loss");
     PT_TRY_HIP(d_hook_in_.allocate(sizeof(GuideRecord) * (size_t)std::max(n, 1u)), "hook input");
     PT_TRY_HIP(hipMemcpyAsync(d_hook_in_.as<void>(), recs, sizeof(GuideRecord) * (size_t)n, hipMemcpyHostToDevice, stream_), "hook input");
-    PT_TRY_HIP(hipMemsetAsync(g_grad_.as<void>(), 0, sizeof(float) * kGuideParams, stream_), "guide gradient");
+    PT_TRY_HIP(g_part_.allocate(sizeof(float) * kGuideParams * guide_parts(std::max(n, 1u))), "guide partial gradients");
     PT_TRY_HIP(hipMemsetAsync(g_loss_.as<void>(), 0, sizeof(float), stream_), "guide loss");
-    PT_TRY_HIP(launch_guide_train(stream_, guide_dev(), d_hook_in_.as<GuideRecord>(), n, n, 0, w_max, g_loss_.as<float>()),
+    PT_TRY_HIP(launch_guide_train(stream_, guide_dev(), d_hook_in_.as<GuideRecord>(), n, n, 0, w_max, g_part_.as<float>(),
+                                  g_loss_.as<float>()),
                "guide gradient");
+    if (n > 0) {
+        PT_TRY_HIP(launch_guide_sum(stream_, g_part_.as<float>(), guide_parts(n), g_grad_.as<float>()), "guide gradient");
+    } else {
+        PT_TRY_HIP(hipMemsetAsync(g_grad_.as<void>(), 0, sizeof(float) * kGuideParams, stream_), "guide gradient");
+    }
     PT_TRY_HIP(hipMemcpyAsync(grad, g_grad_.as<void>(), sizeof(float) * kGuideParams, hipMemcpyDeviceToHost, stream_), "guide gradient");
     PT_TRY_HIP(hipMemcpyAsync(loss, g_loss_.as<void>(), sizeof(float), hipMemcpyDeviceToHost, stream_), "guide loss");
-    PT_TRY_HIP(hipMemsetAsync(g_grad_.as<void>(), 0, sizeof(float) * kGuideParams, stream_), "guide gradient");
     PT_TRY_HIP(hipStreamSynchronize(stream_), "hook sync");
     return PT_OK;
 }
@@ -285,13 +290,16 @@ pt_status Renderer::GuideTrain(const pt_guide_record* recs, uint32_t n, uint32_t
     PT_TRY_HIP(d_hook_in_.allocate(sizeof(GuideRecord) * (size_t)n), "hook input");
     PT_TRY_HIP(hipMemcpyAsync(d_hook_in_.as<void>(), recs, sizeof(GuideRecord) * (size_t)n, hipMemcpyHostToDevice, stream_), "hook input");
     PT_TRY_HIP(hipMemsetAsync(g_loss_.as<void>(), 0, sizeof(float) * std::max<size_t>(steps, 16), stream_), "guide loss");
+    PT_TRY_HIP(g_part_.allocate(sizeof(float) * kGuideParams * guide_parts(batch)), "guide partial gradients");
     const GuideDev g = guide_dev();
     for (uint32_t s = 0; s < steps; s++) {
         g_step_++;
         const uint32_t seed = (g_step_ * 2654435761u) | 1u;
-        PT_TRY_HIP(launch_guide_train(stream_, g, d_hook_in_.as<GuideRecord>(), n, batch, seed, w_max, g_loss_.as<float>() + s),
+        PT_TRY_HIP(launch_guide_train(stream_, g, d_hook_in_.as<GuideRecord>(), n, batch, seed, w_max,
+                                      g_part_.as<float>(), g_loss_.as<float>() + s),
                    "guide training");
-        PT_TRY_HIP(launch_guide_adam(stream_, g, lr, g_step_, 1.0f / (float)batch), "guide Adam step");
+        PT_TRY_HIP(launch_guide_adam(stream_, g, g_part_.as<float>(), guide_parts(batch), lr, g_step_, 1.0f / (float)batch),
+                   "guide Adam step");
     }
     if (losses && steps) {
         PT_TRY_HIP(hipMemcpyAsync(losses, g_loss_.as<void>(), sizeof(float) * steps, hipMemcpyDeviceToHost, stream_), "guide loss");
@@ -392,15 +400,19 @@ pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& c
         gst.records = count;
         if (count == 0) continue;
         const GuideDev g = guide_dev();
+        PT_TRY_HIP(g_part_.allocate(sizeof(float) * kGuideParams * guide_parts(gs.batch)), "guide partial gradients");
         PT_TRY_HIP(hipMemsetAsync(g_loss_.as<void>(), 0, sizeof(float) * gs.train_steps, stream_), "guide loss");
         PT_TRY_HIP(hipEventRecord(t0, stream_), "event");
         for (uint32_t k = 0; k < gs.train_steps; k++) {
             g_step_++;
             const uint32_t seed = (uint32_t)(gs.seed * 0x9E3779B1u) ^ (g_step_ * 2654435761u) ^ 1u;
             PT_TRY_HIP(launch_guide_train(stream_, g, g.recs, count, gs.batch, seed | 1u,
-                                          gs.weight_max > 0.0f ? gs.weight_max : 1e30f, g_loss_.as<float>() + k),
+                                          gs.weight_max > 0.0f ? gs.weight_max : 1e30f, g_part_.as<float>(),
+                                          g_loss_.as<float>() + k),
                        "guide training");
-            PT_TRY_HIP(launch_guide_adam(stream_, g, gs.learning_rate, g_step_, 1.0f / (float)gs.batch), "guide Adam step");
+            PT_TRY_HIP(launch_guide_adam(stream_, g, g_part_.as<float>(), guide_parts(gs.batch), gs.learning_rate, g_step_,
+                                         1.0f / (float)gs.batch),
+                       "guide Adam step");
         }
         PT_TRY_HIP(hipEventRecord(t1, stream_), "event");
         std::vector<float> losses(gs.train_steps);

@@ -125,9 +125,13 @@ hipError_t launch_guide_sync(hipStream_t s, const GuideDev& g);
 hipError_t launch_guide_forward(hipStream_t s, const GuideDev& g, const float* features, uint32_t n, float* out);
 hipError_t launch_guide_mixture(hipStream_t s, const GuideDev& g, const float* features, uint32_t n, uint32_t* words);
 hipError_t launch_guide_infer(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, uint32_t grid);
+// Blocks (64 records each) of a training step: the number of partial gradients.
+inline uint32_t guide_parts(uint32_t batch) { return (batch + 63) / 64; }
 hipError_t launch_guide_train(hipStream_t s, const GuideDev& g, const GuideRecord* recs, uint32_t count,
-                              uint32_t batch, uint32_t seed, float w_max, float* loss);
-hipError_t launch_guide_adam(hipStream_t s, const GuideDev& g, float lr, uint32_t step, float grad_scale);
+                              uint32_t batch, uint32_t seed, float w_max, float* part, float* loss);
+hipError_t launch_guide_sum(hipStream_t s, const float* part, uint32_t parts, float* grad);
+hipError_t launch_guide_adam(hipStream_t s, const GuideDev& g, const float* part, uint32_t parts, float lr,
+                             uint32_t step, float grad_scale);
 
 // Stream-ordered copy of `bytes` (a multiple of 4) from host memory to device
 // memory, passed as kernel arguments: the host buffer may be reused at once.
