@@ -177,36 +177,40 @@ struct GuideLobe {
 // cache) so that no lobe table occupies registers across the shading code.
 // The weights are renormalised to sum to one after the fp16 round trip, and the
 // density is q(w) = sum_j pi_j kappa_j / (2 pi (1 - e^{-2 kappa_j})) e^{kappa_j (mu_j . w - 1)}.
+// Sampling and density share make() and the hardware exp/rcp/rsqrt: both see
+// the same lobes, and the density is exact to a few ulp.
 struct GuideMixRef {
     const uint32_t* w;       // word 0 of this entry
     uint32_t stride;
-    PT_HD uint32_t word(int k) const { return w[(size_t)k * stride]; }
-    PT_HD static float half_of(uint32_t v, int h) { return h ? guide_half_hi(v) : guide_half_lo(v); }
-    PT_HD static GuideLobe make(float x, float y, float z, float k, float p) {
+    __device__ uint32_t word(int k) const { return w[(size_t)k * stride]; }
+    __device__ static float half_of(uint32_t v, int h) { return h ? guide_half_hi(v) : guide_half_lo(v); }
+    __device__ static GuideLobe make(float x, float y, float z, float k, float p) {
         GuideLobe l;
         const float len2 = x * x + y * y + z * z;
         const bool ok = len2 > 1e-12f;
-        const float inv = ok ? 1.0f / sqrtf(len2) : 0.0f;
+        const float inv = ok ? rsqrtf(len2) : 0.0f;
         l.mu = ok ? v3(x * inv, y * inv, z * inv) : v3(0.0f, 0.0f, 1.0f);
         l.kappa = fmaxf(k, 1e-3f);
         l.pi = fmaxf(p, 0.0f);
         return l;
     }
-    PT_HD GuideLobe lobe(int j) const {
+    __device__ GuideLobe lobe(int j) const {
         const int t = j & 3, h = j >> 2;
         return make(half_of(word(t), h), half_of(word(4 + t), h), half_of(word(8 + t), h), half_of(word(12 + t), h),
                     half_of(word(16 + t), h));
     }
-    PT_HD float weight(int j) const { return fmaxf(half_of(word(16 + (j & 3)), j >> 2), 0.0f); }
+    __device__ float weight(int j) const { return fmaxf(half_of(word(16 + (j & 3)), j >> 2), 0.0f); }
 };
 
-PT_HD float guide_vmf_norm(float kappa) { return kappa / (2.0f * kPi * (1.0f - expf(-2.0f * kappa))); }
+__device__ __forceinline__ float guide_vmf_norm(float kappa) {
+    return __fdividef(kappa, 2.0f * kPi * (1.0f - __expf(-2.0f * kappa)));
+}
 
 // The mixture folded onto the hemisphere of n: directions below the surface
 // are mirrored across its plane, so the density at w (w . n >= 0) is
 // q(w) + q(w - 2 (w . n) n).
 PT_HD V3 guide_mirror(V3 w, V3 n) { return w - n * (2.0f * dot(w, n)); }
-PT_HD float guide_pdf_folded(const GuideMixRef& m, V3 w, V3 n) {
+__device__ float guide_pdf_folded(const GuideMixRef& m, V3 w, V3 n) {
     if (dot(w, n) < 0.0f) return 0.0f;
     const V3 wr = guide_mirror(w, n);
     float q = 0.0f, sum = 0.0f;
@@ -217,11 +221,11 @@ PT_HD float guide_pdf_folded(const GuideMixRef& m, V3 w, V3 n) {
                                                   GuideMixRef::half_of(z, h), GuideMixRef::half_of(k, h),
                                                   GuideMixRef::half_of(p, h));
             const float a = l.pi * guide_vmf_norm(l.kappa);
-            q += a * (expf(l.kappa * (dot(l.mu, w) - 1.0f)) + expf(l.kappa * (dot(l.mu, wr) - 1.0f)));
+            q += a * (__expf(l.kappa * (dot(l.mu, w) - 1.0f)) + __expf(l.kappa * (dot(l.mu, wr) - 1.0f)));
             sum += l.pi;
         }
     }
-    return sum > 0.0f ? q / sum : 0.0f;
+    return sum > 0.0f ? __fdividef(q, sum) : 0.0f;
 }
 
 // Orthonormal basis around a unit vector (Duff et al. 2017, "Building an
@@ -237,7 +241,7 @@ PT_HD void guide_basis(V3 n, V3& t, V3& b) {
 // Direction from the mixture: u0 picks the lobe (weights renormalised as in
 // the density), (u1, u2) the direction within it (Jakob 2012:
 // cos = 1 + log(u + (1-u) e^{-2 kappa}) / kappa).
-PT_HD V3 guide_sample(const GuideMixRef& m, float u0, float u1, float u2) {
+__device__ V3 guide_sample(const GuideMixRef& m, float u0, float u1, float u2) {
     float sum = 0.0f;
     for (int j = 0; j < kGuideLobes; j++) sum += m.weight(j);
     const float target = u0 * sum;
@@ -250,15 +254,15 @@ PT_HD V3 guide_sample(const GuideMixRef& m, float u0, float u1, float u2) {
     const GuideLobe l = m.lobe(j);
     const float k = l.kappa;
     const float u = fmaxf(u1, 1e-7f);
-    const float c = clampf(1.0f + logf(u + (1.0f - u) * expf(-2.0f * k)) / k, -1.0f, 1.0f);
+    const float c = clampf(1.0f + __fdividef(__logf(u + (1.0f - u) * __expf(-2.0f * k)), k), -1.0f, 1.0f);
     const float s = sqrtf(fmaxf(0.0f, 1.0f - c * c));
     const float phi = 2.0f * kPi * u2;
     V3 t, b;
     guide_basis(l.mu, t, b);
-    return normalize(l.mu * c + t * (s * cosf(phi)) + b * (s * sinf(phi)));
+    return normalize(l.mu * c + t * (s * __cosf(phi)) + b * (s * __sinf(phi)));
 }
 
-PT_HD V3 guide_sample_folded(const GuideMixRef& m, V3 n, float u0, float u1, float u2) {
+__device__ V3 guide_sample_folded(const GuideMixRef& m, V3 n, float u0, float u1, float u2) {
     const V3 w = guide_sample(m, u0, u1, u2);
     return dot(w, n) < 0.0f ? guide_mirror(w, n) : w;
 }

@@ -90,10 +90,8 @@ GuideDev Renderer::guide_dev() const {
     g.grad = g_grad_.as<float>();
     g.m = g_m_.as<float>();
     g.v = g_v_.as<float>();
-    g.mix = g_mix_.as<uint32_t>();
     g.wf = g_wf_.as<uint16_t>();
     g.bf = g_bf_.as<float>();
-    g.mix_stride = g_pool_;
     g.recs = g_recs_.as<GuideRecord>();
     g.rec_capacity = g_rec_capacity_;
     g.head[0] = g_head_[0].as<uint32_t>();
@@ -128,15 +126,13 @@ pt_status Renderer::ensure_guide() {
 }
 
 pt_status Renderer::ensure_guide_pool(uint32_t cap) {
-    if (g_pool_ != pool_capacity_ || g_mix_.bytes() == 0) {
+    if (g_pool_ != pool_capacity_ || g_head_[0].bytes() == 0) {
         invalidate_graph();
-        g_mix_.release();
         g_head_[0].release();
         g_head_[1].release();
         g_sh_head_.release();
         g_rr_scale_[0].release();
         g_rr_scale_[1].release();
-        PT_TRY_HIP(g_mix_.allocate(sizeof(uint32_t) * kGuideMixWords * (size_t)pool_capacity_), "guide mixtures");
         PT_TRY_HIP(g_head_[0].allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");
         PT_TRY_HIP(g_head_[1].allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");
         PT_TRY_HIP(g_sh_head_.allocate(sizeof(uint32_t) * (size_t)pool_capacity_), "guide record heads");

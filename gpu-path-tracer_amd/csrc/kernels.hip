@@ -17,6 +17,7 @@
 // lane: same-address atomics are expensive across the eight XCDs.
 #include <hip/hip_runtime.h>
 
+#include "guide_net.hpp"
 #include "launch.hpp"
 
 namespace pt {
@@ -999,9 +1000,11 @@ __global__ __launch_bounds__(kBinBlock) void k_bin_batch(const uint8_t* qid, uin
 
 // Block-wide reservation for two output queues: one atomicAdd per queue per
 // block.  Every thread of the block must call it.
+constexpr uint32_t kMaxBlockWaves = 16;   // largest block of the kernels that reserve queue space
+
 __device__ __forceinline__ void block_reserve2(bool pred0, bool pred1, unsigned int* c0, unsigned int* c1,
                                                uint32_t& pos0, uint32_t& pos1) {
-    __shared__ uint32_t s_off[2][4];
+    __shared__ uint32_t s_off[2][kMaxBlockWaves];
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
     const uint32_t waves = blockDim.x >> 6;
     const uint64_t m0 = __ballot(pred0), m1 = __ballot(pred1);
@@ -1031,7 +1034,7 @@ __device__ __forceinline__ void block_reserve2(bool pred0, bool pred1, unsigned 
 // block_reserve2 with a third counter (guide training records).
 __device__ __forceinline__ void block_reserve3(bool pred0, bool pred1, bool pred2, unsigned int* c0, unsigned int* c1,
                                                unsigned int* c2, uint32_t& pos0, uint32_t& pos1, uint32_t& pos2) {
-    __shared__ uint32_t s_off3[3][4];
+    __shared__ uint32_t s_off3[3][kMaxBlockWaves];
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
     const uint32_t waves = blockDim.x >> 6;
     const uint64_t m0 = __ballot(pred0), m1 = __ballot(pred1), m2 = __ballot(pred2);
@@ -1070,17 +1073,49 @@ __device__ __forceinline__ void block_reserve3(bool pred0, bool pred1, bool pred
 // BSDF, each continuing diffuse/GGX vertex writes a training record, and
 // radiance reaching the pixel is credited to the path's records.
 template <bool MULTI, bool GUIDED>
-__global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cur) {
+__global__ __launch_bounds__(GUIDED ? 64 * kInferWaves : 256) void k_shade(PathState p, SceneDev sc1, int cur) {
+    constexpr uint32_t kBlock = GUIDED ? 64u * kInferWaves : 256u;
     const FrameParams* fp = p.fp;
     Counters* c = p.ctr;
     const int nxt = cur ^ 1;
     const uint32_t c0 = c->queue[0], c1 = c->queue[1], c2 = c->queue[2], c3 = c->queue[3];
     const uint32_t e1 = c0, e2 = c0 + c1, e3 = e2 + c2, total = e3 + c3;
     const uint32_t rec_mask = GUIDED ? p.guide.cfg.rec_mask : 0u;
+    const float gfrac = GUIDED ? p.guide.cfg.guide_fraction : 0.0f;
+    // GUIDED: the guide network in LDS; each wave infers the mixtures of its 64
+    // entries into its own tile, which the lanes then read while shading
+    InferLds<kInferWaves, 3>* gl = nullptr;
+    if constexpr (GUIDED) {
+        __shared__ InferLds<kInferWaves, 3> glds;
+        gl = &glds;
+        if (gfrac > 0.0f) stage_weights(gl->w, gl->b, p.guide.wf, p.guide.bf, InferLds<kInferWaves, 3>::kWeights);
+    }
 
-    for (uint32_t bbase = blockIdx.x * 256u; bbase < total; bbase += gridDim.x * 256u) {
+    for (uint32_t bbase = blockIdx.x * kBlock; bbase < total; bbase += gridDim.x * kBlock) {
         const uint32_t e = bbase + threadIdx.x;
         const bool valid = e < total;
+        uint32_t* mixw = nullptr;          // this wave's mixture words: word k of lane s at mixw[64 k + s]
+        if constexpr (GUIDED) {
+            if (gfrac > 0.0f) {
+                const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+                bf16x8* xt = gl->x[wave];
+                mixw = reinterpret_cast<uint32_t*>(xt);
+                const bool need = valid && e < e2;
+                if (__ballot(need) != 0) {
+                    float f[kGuideDim];
+#pragma unroll
+                    for (int k = 0; k < kGuideDim; k++) f[k] = 0.0f;
+                    if (need) InferPathSrc{p, sc1, cur}.features(e, f);
+                    store_input_frags(xt, lane, f);
+                    wave_lds_sync();
+                    f32x4 o[4][4];
+                    tile_forward<3>(gl->w, gl->b, xt, o);
+                    wave_lds_sync();   // the tile's inputs are read: its words take their place
+                    mixture_words(o, [&](int k, uint32_t sidx, uint32_t v) { mixw[64 * k + sidx] = v; });
+                    wave_lds_sync();
+                }
+            }
+        }
         bool to_ray = false;
         ShadowOut so;
         so.emit = false;
@@ -1148,9 +1183,8 @@ __global__ __launch_bounds__(256) void k_shade(PathState p, SceneDev sc1, int cu
                 const Frame3 fr = shading_frame(h, nts);
                 Draws dr = draws_at(fp, vr->seed, ((uint64_t)m.w << 32) | m.z, depth);
                 // guide: draw slot 7 picks guide or BSDF; the mixture of this entry
-                const float gfrac = GUIDED ? p.guide.cfg.guide_fraction : 0.0f;
                 float u7 = 1.0f;
-                const GuideMixRef gm{p.guide.mix + e, p.guide.mix_stride};
+                const GuideMixRef gm{mixw + (threadIdx.x & 63), 64};
                 if (GUIDED && gfrac > 0.0f && q < 2) {
                     Draws d7 = dr;
                     for (int k = 0; k < 7; k++) d7.state = d7.state * kPcgMult + d7.inc;
@@ -1517,7 +1551,11 @@ hipError_t launch_bin(hipStream_t s, const PathState& p, int cur, uint32_t grid)
 hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool multi, bool guided,
                         uint32_t grid, uint32_t block) {
     if (block != 256 || (multi && guided)) return hipErrorInvalidValue;
-    if (guided) hipLaunchKernelGGL((k_shade<false, true>), dim3(grid), dim3(256), 0, s, p, sc, cur);
+    // the guided kernel holds the guide network in 159 KB of LDS: one block of
+    // 16 waves per CU where the others run eight blocks of four
+    if (guided)
+        hipLaunchKernelGGL((k_shade<false, true>), dim3(std::max(1u, grid / 8u)), dim3(64 * kInferWaves), 0, s, p, sc,
+                           cur);
     else if (multi) hipLaunchKernelGGL((k_shade<true, false>), dim3(grid), dim3(256), 0, s, p, sc, cur);
     else hipLaunchKernelGGL((k_shade<false, false>), dim3(grid), dim3(256), 0, s, p, sc, cur);
     return hipGetLastError();

@@ -34,9 +34,6 @@ struct GuideDev {
     float* grad;
     float* m;                       // Adam moments
     float* v;
-    uint32_t* mix;                  // mixtures of the diffuse/GGX entries of the bounce ([diffuse | ggx]
-                                    // order), kGuideMixWords packed words each, word-major with stride mix_stride
-    uint32_t mix_stride;
     uint16_t* wf;                   // bf16 weights in the inference kernel's fragment order
     float* bf;                      // biases in the inference kernel's output order
     GuideRecord* recs;              // training records of the current run, Counters::records of them
@@ -124,7 +121,6 @@ hipError_t launch_alias_sample(hipStream_t s, const SceneDev& sc, const float* u
 hipError_t launch_guide_sync(hipStream_t s, const GuideDev& g);
 hipError_t launch_guide_forward(hipStream_t s, const GuideDev& g, const float* features, uint32_t n, float* out);
 hipError_t launch_guide_mixture(hipStream_t s, const GuideDev& g, const float* features, uint32_t n, uint32_t* words);
-hipError_t launch_guide_infer(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, uint32_t grid);
 // Blocks (64 records each) of a training step: the number of partial gradients.
 inline uint32_t guide_parts(uint32_t batch) { return (batch + 63) / 64; }
 hipError_t launch_guide_train(hipStream_t s, const GuideDev& g, const GuideRecord* recs, uint32_t count,

@@ -313,8 +313,6 @@ pt_status Renderer::launch_iteration(int cur, bool count, const LoopMode& mode) 
     const uint32_t grid = (uint32_t)std::max(1, cu_count_) * 8u;
     PT_TRY_HIP(launch_closest(stream_, p, sc, cur, count, mode.trace_multi, trace_), "closest-hit launch");
     PT_TRY_HIP(launch_bin(stream_, p, cur, bin_grid()), "binning launch");
-    if (mode.guided && mode.guide_fraction > 0.0f)
-        PT_TRY_HIP(launch_guide_infer(stream_, p, sc, cur, (uint32_t)std::max(1, cu_count_)), "guide inference");
     PT_TRY_HIP(launch_shade(stream_, p, sc, cur, mode.shade_multi, mode.guided, grid, 256), "shade launch");
     PT_TRY_HIP(launch_shadow(stream_, p, sc, count, mode.trace_multi, trace_), "shadow launch");
     PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
@@ -480,8 +478,6 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
             PT_TRY_HIP(hipEventRecord(ev[1], stream_), "event");
             PT_TRY_HIP(launch_bin(stream_, p, cur, bin_grid()), "binning launch");
             PT_TRY_HIP(hipEventRecord(ev[2], stream_), "event");
-            if (mode.guided && mode.guide_fraction > 0.0f)
-                PT_TRY_HIP(launch_guide_infer(stream_, p, sc, cur, (uint32_t)std::max(1, cu_count_)), "guide inference");
             PT_TRY_HIP(launch_shade(stream_, p, sc, cur, mode.shade_multi, mode.guided, grid, 256), "shade launch");
             PT_TRY_HIP(hipEventRecord(ev[3], stream_), "event");
             PT_TRY_HIP(launch_shadow(stream_, p, sc, false, mode.trace_multi, trace_), "shadow launch");

@@ -237,7 +237,7 @@ private:
 
     // guide network and its training records
     DeviceBuffer g_w32_, g_w_, g_wt_, g_wf_, g_bf_, g_grad_, g_m_, g_v_, g_loss_, g_part_;
-    DeviceBuffer g_mix_, g_recs_, g_head_[2], g_sh_head_, g_rr_scale_[2];
+    DeviceBuffer g_recs_, g_head_[2], g_sh_head_, g_rr_scale_[2];
     uint32_t g_rec_capacity_ = 0;
     uint32_t g_pool_ = 0;            // path slots the per-path guide buffers cover
     uint32_t g_step_ = 0;            // Adam steps taken
