@@ -340,9 +340,18 @@ constexpr int kSumBlock = 256;
 __device__ __forceinline__ float sum_parts(const float* part, uint32_t parts, uint32_t k) {
     __shared__ float red[kSumBlock];
     const uint32_t wave = threadIdx.x >> 6;
+    constexpr uint32_t kStep = kSumBlock / 64, kUnroll = 8;   // eight independent loads in flight
+    float acc[kUnroll] = {};
+    if (k < (uint32_t)kGuideParams) {
+        uint32_t b = wave;
+        for (; b + (kUnroll - 1) * kStep < parts; b += kUnroll * kStep)
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; u++) acc[u] += part[(size_t)(b + u * kStep) * kGuideParams + k];
+        for (; b < parts; b += kStep) acc[0] += part[(size_t)b * kGuideParams + k];
+    }
     float s = 0.0f;
-    if (k < (uint32_t)kGuideParams)
-        for (uint32_t b = wave; b < parts; b += kSumBlock / 64) s += part[(size_t)b * kGuideParams + k];
+#pragma unroll
+    for (uint32_t u = 0; u < kUnroll; u++) s += acc[u];
     red[threadIdx.x] = s;
     __syncthreads();
     float t = 0.0f;
