@@ -45,82 +45,69 @@ __device__ __forceinline__ bf16x4 glb4(const uint16_t* w, int row, int col) {
     return *reinterpret_cast<const bf16x4*>(w + row * kGuideDim + col);
 }
 
-// acc[mt][nt] <- A . B^T with A a [64][64] LDS tile and B a [64][64] global
-// matrix stored row-major (B row n = output n).  Lane l holds element
-// (16 mt + 4 (l >> 4) + j, 16 nt + (l & 15)) of the result in acc[mt][nt][j].
-__device__ __forceinline__ void mm_abt(const uint16_t* a, const uint16_t* b, f32x4 acc[4][4]) {
+// The training kernel splits each 64-sample tile over four waves: wave w
+// computes the 16 result columns 16 w .. 16 w + 15 of every product.
+
+// acc[mt] <- columns 16 nt .. 16 nt + 15 of A . B^T, A a [64][64] LDS tile and
+// B a [64][64] global matrix stored row-major (B row n = output n).  Lane l
+// holds element (16 mt + 4 (l >> 4) + j, 16 nt + (l & 15)) in acc[mt][j].
+__device__ __forceinline__ void mm_abt(const uint16_t* a, const uint16_t* b, int nt, f32x4 acc[4]) {
     const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-        for (int nt = 0; nt < 4; nt++) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int mt = 0; mt < 4; mt++) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kt = 0; kt < 4; kt++) {
-        bf16x4 bf[4];
-#pragma unroll
-        for (int nt = 0; nt < 4; nt++) bf[nt] = glb4(b, 16 * nt + r, 16 * kt + 4 * kq);
+        const bf16x4 bf = glb4(b, 16 * nt + r, 16 * kt + 4 * kq);
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) {
             const bf16x4 af = row4(a, 16 * mt + r, 16 * kt + 4 * kq);
-#pragma unroll
-            for (int nt = 0; nt < 4; nt++)
-                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf[nt], acc[mt][nt], 0, 0, 0);
+            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf, acc[mt], 0, 0, 0);
         }
     }
 }
 
-// acc[mt][nt] <- D^T . X over the 64 samples: D and X are LDS tiles.
-__device__ __forceinline__ void mm_dtx(const uint16_t* d, const uint16_t* x, f32x4 acc[4][4]) {
+// acc[mt] <- columns 16 nt .. of D^T . X over the 64 samples (LDS tiles).
+__device__ __forceinline__ void mm_dtx(const uint16_t* d, const uint16_t* x, int nt, f32x4 acc[4]) {
     const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-        for (int nt = 0; nt < 4; nt++) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int mt = 0; mt < 4; mt++) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kt = 0; kt < 4; kt++) {
-        bf16x4 xf[4];
-#pragma unroll
-        for (int nt = 0; nt < 4; nt++) xf[nt] = col4(x, 16 * kt + 4 * kq, 16 * nt + r);
+        const bf16x4 xf = col4(x, 16 * kt + 4 * kq, 16 * nt + r);
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) {
             const bf16x4 df = col4(d, 16 * kt + 4 * kq, 16 * mt + r);
-#pragma unroll
-            for (int nt = 0; nt < 4; nt++)
-                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(df, xf[nt], acc[mt][nt], 0, 0, 0);
+            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(df, xf, acc[mt], 0, 0, 0);
         }
     }
 }
 
-// Hidden layer: out = relu(in W^T + b) as bf16.
-__device__ __forceinline__ void layer_hidden(const uint16_t* in, uint16_t* out, const uint16_t* w, const float* bias) {
-    f32x4 acc[4][4];
-    mm_abt(in, w, acc);
+// Hidden layer, columns 16 nt ..: out = relu(in W^T + b) as bf16.
+__device__ __forceinline__ void layer_hidden(const uint16_t* in, uint16_t* out, const uint16_t* w, const float* bias,
+                                             int nt) {
+    f32x4 acc[4];
+    mm_abt(in, w, nt, acc);
     const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+    const int o = 16 * nt + r;
+    const float b = bias[o];
 #pragma unroll
     for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-        for (int nt = 0; nt < 4; nt++) {
-            const int o = 16 * nt + r;
-            const float b = bias[o];
-#pragma unroll
-            for (int j = 0; j < 4; j++) out[(16 * mt + 4 * kq + j) * kLd + o] = to_bf16(fmaxf(acc[mt][nt][j] + b, 0.0f));
-        }
+        for (int j = 0; j < 4; j++) out[(16 * mt + 4 * kq + j) * kLd + o] = to_bf16(fmaxf(acc[mt][j] + b, 0.0f));
 }
 
-// Output layer: fp32 [sample][kLdOut].
-__device__ __forceinline__ void layer_output(const uint16_t* in, float* out, const uint16_t* w, const float* bias) {
-    f32x4 acc[4][4];
-    mm_abt(in, w, acc);
+// Output layer, columns 16 nt ..: fp32 [sample][kLdOut].
+__device__ __forceinline__ void layer_output(const uint16_t* in, float* out, const uint16_t* w, const float* bias,
+                                             int nt) {
+    f32x4 acc[4];
+    mm_abt(in, w, nt, acc);
     const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+    const int o = 16 * nt + r;
+    const float b = bias[o];
 #pragma unroll
     for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-        for (int nt = 0; nt < 4; nt++) {
-            const int o = 16 * nt + r;
-            const float b = bias[o];
-#pragma unroll
-            for (int j = 0; j < 4; j++) out[(16 * mt + 4 * kq + j) * kLdOut + o] = acc[mt][nt][j] + b;
-        }
+        for (int j = 0; j < 4; j++) out[(16 * mt + 4 * kq + j) * kLdOut + o] = acc[mt][j] + b;
 }
 
 // Writes the features of this lane's sample as one bf16 row (zeros past 45).
@@ -236,80 +223,87 @@ PT_HD float guide_weight(const GuideRecord& r, float w_max) {
 // when `seed` != 0, else records [0, batch)): block k writes the gradient of
 // its 64 records to part[k][kGuideParams] (no atomics: the sum over blocks is
 // taken by the Adam step); the loss is summed into *loss.
-__global__ __launch_bounds__(64) void k_guide_train(GuideNet net, const GuideRecord* recs, uint32_t count,
-                                                   uint32_t batch, uint32_t seed, GuideConfig g, float w_max,
-                                                   float* part, float* loss) {
+constexpr int kTrainWaves = 4;       // waves sharing one 64-record tile
+
+__global__ __launch_bounds__(64 * kTrainWaves) void k_guide_train(GuideNet net, const GuideRecord* recs, uint32_t count,
+                                                                 uint32_t batch, uint32_t seed, GuideConfig g,
+                                                                 float w_max, float* part, float* loss) {
     __shared__ uint16_t x0[64 * kLd], h1[64 * kLd], h2[64 * kLd], h3[64 * kLd], d4[64 * kLd];
     __shared__ float o32[64 * kLdOut];
-    const int s = threadIdx.x;
+    const int wave = threadIdx.x >> 6, s = threadIdx.x & 63;
     const uint32_t b = blockIdx.x * 64u + (uint32_t)s;
     const bool valid = b < batch && count > 0;
+    // wave 0 owns the per-record work (features, loss), every wave a quarter of
+    // each product
     GuideRecord rec{};
-    float f[kGuideFeatures];
-    if (valid) {
-        rec = recs[seed ? pick_record(seed, b, count) : b];
-        guide_features(g, v3(rec.p[0], rec.p[1], rec.p[2]), v3(rec.n[0], rec.n[1], rec.n[2]),
-                       v3(rec.wo[0], rec.wo[1], rec.wo[2]), f);
+    if (wave == 0) {
+        float f[kGuideFeatures];
+        if (valid) {
+            rec = recs[seed ? pick_record(seed, b, count) : b];
+            guide_features(g, v3(rec.p[0], rec.p[1], rec.p[2]), v3(rec.n[0], rec.n[1], rec.n[2]),
+                           v3(rec.wo[0], rec.wo[1], rec.wo[2]), f);
+        }
+        store_features(x0, s, f, valid);
     }
-    store_features(x0, s, f, valid);
     __syncthreads();
-    layer_hidden(x0, h1, net.w, bias_of(net, 0));
+    layer_hidden(x0, h1, net.w, bias_of(net, 0), wave);
     __syncthreads();
-    layer_hidden(h1, h2, net.w + 1 * kGuideDim * kGuideDim, bias_of(net, 1));
+    layer_hidden(h1, h2, net.w + 1 * kGuideDim * kGuideDim, bias_of(net, 1), wave);
     __syncthreads();
-    layer_hidden(h2, h3, net.w + 2 * kGuideDim * kGuideDim, bias_of(net, 2));
+    layer_hidden(h2, h3, net.w + 2 * kGuideDim * kGuideDim, bias_of(net, 2), wave);
     __syncthreads();
-    layer_output(h3, o32, net.w + 3 * kGuideDim * kGuideDim, bias_of(net, 3));
+    layer_output(h3, o32, net.w + 3 * kGuideDim * kGuideDim, bias_of(net, 3), wave);
     __syncthreads();
 
-    // d loss / d outputs of this lane's sample
-    float gout[kGuideOut];
-    float lsum = 0.0f;
-    if (valid) {
-        float o[kGuideOut];
-        for (int k = 0; k < kGuideOut; k++) o[k] = o32[s * kLdOut + k];
-        const float w = guide_weight(rec, w_max);
-        const float logq = guide_loss_grad(o, v3(rec.wi[0], rec.wi[1], rec.wi[2]), w, gout);
-        lsum = -w * logq;
-    } else {
-        for (int k = 0; k < kGuideOut; k++) gout[k] = 0.0f;
+    if (wave == 0) {
+        // d loss / d outputs of this lane's record
+        float gout[kGuideOut];
+        float lsum = 0.0f;
+        if (valid) {
+            float o[kGuideOut];
+            for (int k = 0; k < kGuideOut; k++) o[k] = o32[s * kLdOut + k];
+            const float w = guide_weight(rec, w_max);
+            const float logq = guide_loss_grad(o, v3(rec.wi[0], rec.wi[1], rec.wi[2]), w, gout);
+            lsum = -w * logq;
+        } else {
+            for (int k = 0; k < kGuideOut; k++) gout[k] = 0.0f;
+        }
+        for (int k = 0; k < kGuideDim; k++) d4[s * kLd + k] = to_bf16(k < kGuideOut ? gout[k] : 0.0f);
+        for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+        if (s == 0) atomicAdd(loss, lsum);
     }
-    for (int k = 0; k < kGuideDim; k++) d4[s * kLd + k] = to_bf16(k < kGuideOut ? gout[k] : 0.0f);
-    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
-    if (s == 0) atomicAdd(loss, lsum);
     __syncthreads();
 
-    const int lane = s, r = lane & 15, kq = lane >> 4;
+    const int r = s & 15, kq = s >> 4, col = 16 * wave + r;
     uint16_t* dl = d4;                         // delta of the current layer's outputs
 #pragma unroll
     for (int l = kGuideLayers - 1; l >= 0; l--) {
         uint16_t* act = l == 0 ? x0 : l == 1 ? h1 : l == 2 ? h2 : h3;   // input of layer l
         float* gw = part + (size_t)blockIdx.x * kGuideParams + l * kGuideLayerParams;
-        f32x4 acc[4][4];
-        mm_dtx(dl, act, acc);
+        f32x4 acc[4];
+        mm_dtx(dl, act, wave, acc);
 #pragma unroll
         for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-            for (int nt = 0; nt < 4; nt++)
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-                    gw[(16 * mt + 4 * kq + j) * kGuideDim + 16 * nt + r] = acc[mt][nt][j];
+            for (int j = 0; j < 4; j++) gw[(16 * mt + 4 * kq + j) * kGuideDim + col] = acc[mt][j];
+        // bias gradient of output `col`: the four lane groups sum 16 records each
         float db = 0.0f;
-        for (int t = 0; t < 64; t++) db += from_bf16(dl[t * kLd + lane]);
-        gw[kGuideDim * kGuideDim + lane] = db;
+#pragma unroll
+        for (int t = 0; t < 16; t++) db += from_bf16(dl[(16 * kq + t) * kLd + col]);
+        db += __shfl_xor(db, 16);
+        db += __shfl_xor(db, 32);
+        if (kq == 0) gw[kGuideDim * kGuideDim + col] = db;
         __syncthreads();
         if (l == 0) break;
         // delta of layer l-1 outputs = (dl W_l) * relu'(act), written over act
-        mm_abt(dl, net.wt + l * kGuideDim * kGuideDim, acc);
+        mm_abt(dl, net.wt + l * kGuideDim * kGuideDim, wave, acc);
 #pragma unroll
         for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-            for (int nt = 0; nt < 4; nt++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int at = (16 * mt + 4 * kq + j) * kLd + 16 * nt + r;
-                    act[at] = to_bf16(from_bf16(act[at]) > 0.0f ? acc[mt][nt][j] : 0.0f);
-                }
+            for (int j = 0; j < 4; j++) {
+                const int at = (16 * mt + 4 * kq + j) * kLd + col;
+                act[at] = to_bf16(from_bf16(act[at]) > 0.0f ? acc[mt][j] : 0.0f);
+            }
         __syncthreads();
         dl = act;
     }
@@ -418,7 +412,7 @@ hipError_t launch_guide_mixture(hipStream_t s, const GuideDev& g, const float* f
 hipError_t launch_guide_train(hipStream_t s, const GuideDev& g, const GuideRecord* recs, uint32_t count,
                               uint32_t batch, uint32_t seed, float w_max, float* part, float* loss) {
     if (batch == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_guide_train, dim3(guide_parts(batch)), dim3(64), 0, s, GuideNet{g.w, g.wt, g.w32}, recs,
+    hipLaunchKernelGGL(k_guide_train, dim3(guide_parts(batch)), dim3(64 * kTrainWaves), 0, s, GuideNet{g.w, g.wt, g.w32}, recs,
                        count, batch, seed, g.cfg, w_max, part, loss);
     return hipGetLastError();
 }
