@@ -171,11 +171,14 @@ def test_guided_render_is_unbiased(room_renderer):
         assert abs(d.mean()) < 3 * d.std() / math.sqrt(d.size), (name, d.mean(), d.std())
 
 
-def test_trained_guide_gains_over_bsdf_sampling(room_renderer):
-    """What the trained network gains over BSDF sampling at equal samples
-    (NEE on)."""
+@pytest.mark.parametrize("nee, trained_max, online_max", [(1, 0.75, 0.85), (0, 0.4, 0.5)])
+def test_trained_guide_gains_over_bsdf_sampling(room_renderer, nee, trained_max, online_max):
+    """What the network gains over BSDF sampling at equal samples: the error of
+    the online-trained render and of the trained guide alone, relative to
+    plain BSDF sampling (measured: about 0.57 and 0.53 with NEE, 0.27 and 0.17
+    without)."""
     r, room = room_renderer
-    ref, plain, online, trained = _render_set(r, room, 320, 180, nee=1)
+    ref, plain, online, trained = _render_set(r, room, 320, 180, nee=nee)
     e_plain, e_online, e_trained = _nmse(plain, ref), _nmse(online, ref), _nmse(trained, ref)
-    assert e_trained < 0.93 * e_plain, (e_trained, e_plain)
-    assert e_online < e_plain, (e_online, e_plain)
+    assert e_trained < trained_max * e_plain, (e_trained, e_plain)
+    assert e_online < online_max * e_plain, (e_online, e_plain)
