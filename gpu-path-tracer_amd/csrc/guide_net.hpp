@@ -135,19 +135,16 @@ __device__ __forceinline__ void tile_forward(const bf16x8* wsh, const float* bsh
         const float* bias = bsh + l * kGuideDim;
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) {
-            const float4 bb = *reinterpret_cast<const float4*>(bias + 16 * mt + 4 * kq);
-#pragma unroll
-            for (int nt = 0; nt < 4; nt++) out[mt][nt] = f32x4{bb.x, bb.y, bb.z, bb.w};
-        }
-#pragma unroll
-        for (int mt = 0; mt < 4; mt++) {
             if (mt >= mts) break;
+            // the first product of each tile takes the bias as its accumulator input
+            const float4 bb = *reinterpret_cast<const float4*>(bias + 16 * mt + 4 * kq);
+            const f32x4 b4 = f32x4{bb.x, bb.y, bb.z, bb.w};
+            const bf16x8 a0 = wsh[((l * 4 + mt) * 2 + 0) * 64 + lane];
+            const bf16x8 a1 = wsh[((l * 4 + mt) * 2 + 1) * 64 + lane];
 #pragma unroll
-            for (int kt = 0; kt < 2; kt++) {
-                const bf16x8 a = wsh[((l * 4 + mt) * 2 + kt) * 64 + lane];
-#pragma unroll
-                for (int nt = 0; nt < 4; nt++)
-                    out[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[kt][nt], out[mt][nt], 0, 0, 0);
+            for (int nt = 0; nt < 4; nt++) {
+                out[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b[0][nt], b4, 0, 0, 0);
+                out[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b[1][nt], out[mt][nt], 0, 0, 0);
             }
         }
         if (l + 1 == kGuideLayers) break;
