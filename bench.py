@@ -296,10 +296,26 @@ def run_guide(a):
         log(f"guide: NEE {nee} {spp} spp: plain {sp['render_ms']:.1f} ms nMSE {e_p:.4g}; guided {sg['render_ms']:.1f} ms "
             f"(train {gs['train_ms']:.1f} ms, {gs['steps']} Adam steps, {gs['records']} records) nMSE {e_g:.4g}; "
             f"time to equal error {ratio:.3f}x")
+        # the next frame of a progressive render: other samples, the network kept
+        # from the frame above and refined with a quarter of the Adam steps (no
+        # plain first pass)
+        st2 = dict(st, seed=8)
+        sp2, _ = r.RenderGuided(st2, room.view, passes=4, guide_fraction=0.0)
+        e_p2 = float(np.mean((r.GetFrameHdr(w, h)[..., :3] - ref) ** 2)) / m2
+        sw, gw = r.RenderGuided(st2, room.view, passes=4, train_steps=max(1, a.guide_steps // 4), batch=16384,
+                                learning_rate=3e-3, guide_fraction=0.5, reset=False, seed=2)
+        e_w = float(np.mean((r.GetFrameHdr(w, h)[..., :3] - ref) ** 2)) / m2
+        ratio_w = e_w * sw["render_ms"] / (e_p2 * sp2["render_ms"])
+        log(f"guide: NEE {nee} next frame, network kept: guided {sw['render_ms']:.1f} ms (train {gw['train_ms']:.1f} ms) "
+            f"nMSE {e_w:.4g} vs plain {sp2['render_ms']:.1f} ms nMSE {e_p2:.4g}; time to equal error {ratio_w:.3f}x")
         out_cfgs.append({"use_nee": nee, "spp": spp, "plain_ms": round(sp["render_ms"], 2), "plain_nmse": e_p,
                          "guided_ms": round(sg["render_ms"], 2), "guided_train_ms": round(gs["train_ms"], 2),
                          "guided_nmse": e_g, "error_ratio_equal_samples": round(e_g / e_p, 4),
                          "time_to_equal_error_ratio": round(ratio, 4),
+                         "next_frame_network_kept": {"guided_ms": round(sw["render_ms"], 2),
+                                                     "guided_train_ms": round(gw["train_ms"], 2), "guided_nmse": e_w,
+                                                     "plain_ms": round(sp2["render_ms"], 2), "plain_nmse": e_p2,
+                                                     "time_to_equal_error_ratio": round(ratio_w, 4)},
                          "plain_mrays_s": round(_rays(sp) / sp["render_ms"] / 1e3, 1),
                          "guided_mrays_s": round(_rays(sg) / sg["render_ms"] / 1e3, 1)})
     out = {"metric": "time to equal error, guided / plain BSDF sampling (lower is better)",

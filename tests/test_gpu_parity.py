@@ -96,6 +96,30 @@ def test_c1_render_matches_oracle(c1, cornell, cornell_oracle):
     assert abs(mg - mo) < 3 * se
 
 
+@pytest.mark.parametrize("change, spp", [
+    (dict(path_depth=1), 64),                           # camera hit and its NEE only
+    (dict(path_depth=2, use_nee=0), 64),                # one bounce, emission found by BSDF sampling
+    (dict(use_nee=0, rr_start_depth=64), 64),           # roulette never starts
+    (dict(rr_start_depth=1), 64),                       # roulette from the first bounce
+    (dict(samples_per_pixel=10), 9),                    # rounded down to a square (Renderer.cpp:366-367)
+    (dict(width=40, height=16), 64),                    # another aspect ratio
+])
+def test_c1_render_settings_match_oracle(c1, cornell, cornell_oracle, change, spp):
+    """Render settings at their edges: the same sample streams give the
+    oracle's image and ray counts."""
+    st = dict(width=32, height=24, samples_per_pixel=64, path_depth=8, use_nee=1, rr_start_depth=3, seed=5)
+    st.update(change)
+    w, h = st["width"], st["height"]
+    stats = c1.RenderFrame(st, cornell.view)
+    hdr = c1.GetFrameHdr(w, h).reshape(-1, 4)
+    ref, ext, sh = cornell_oracle.render(st, cornell.view)
+    assert stats["samples"] == w * h * spp
+    assert abs(stats["extension_rays"] - ext) <= 0.01 * ext
+    assert abs(stats["shadow_rays"] - sh) <= 0.01 * max(sh, 1)
+    rm = _relmse(hdr[:, :3], ref[:, :3])
+    assert rm < 1e-3, rm
+
+
 def test_c1_render_graph_equals_stage_by_stage(c1, cornell):
     st = dict(width=32, height=32, samples_per_pixel=16, path_depth=8, use_nee=1, seed=2)
     s1 = c1.RenderFrame(st, cornell.view)
