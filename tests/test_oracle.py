@@ -89,6 +89,26 @@ def test_reference_traversal_equals_brute_force_except_ties(cornell_oracle):
     assert diff.mean() < 0.01
 
 
+def test_brute_force_tie_rule_on_degenerate_geometry(oracle_lib):
+    """Duplicated, coplanar and zero-area triangles: the reference-order
+    traversal and the brute force differ only on exact ties in t, where the
+    brute force keeps the lowest primitive id; zero-area triangles never hit."""
+    import test_degenerate as D
+    s = D._torture_scene()
+    o = oracle_lib.OracleScene(s)
+    rays = D._rays(1 << 15, 11)
+    ref = o.closest(rays)
+    brute = o.closest(rays, brute=True)
+    diff = (ref != brute).any(axis=1)
+    assert diff.sum() > 100                                  # the scene is full of ties
+    assert np.array_equal(ref[diff, 1], brute[diff, 1])      # same t
+    assert (brute[diff, 0] < ref[diff, 0]).all()             # lower id wins
+    hit = brute[:, 0] != 0xFFFFFFFF
+    assert not np.isin(brute[hit, 0], list(D.ZERO_AREA)).any()
+    # the duplicates of grid triangles 0-39 never beat their originals
+    assert not np.isin(brute[hit, 0], list(D.DUPLICATES)).any()
+
+
 def test_any_hit_equals_closest_below_tmax(cornell, cornell_oracle):
     rays = scenes.random_rays(50000, (-1, 0, -1), (1, 2, 1), 6)
     rng = np.random.default_rng(0)
