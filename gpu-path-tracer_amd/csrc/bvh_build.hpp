@@ -19,6 +19,7 @@ struct Bvh4Build {
     uint32_t leaf_count = 0;
     uint32_t max_leaf = 0;
     uint32_t width = 4;               // children per node: 4, or 2 (no collapse)
+    uint32_t device_nodes = 0;        // node count of a tree built on the device (nodes and order stay empty)
 };
 
 // The nodes of a width-2 build in the 64-byte layout.

@@ -1,4 +1,5 @@
 #include "renderer.hpp"
+#include "bvh_gpu.hpp"
 #include "server.hpp"
 
 #include <algorithm>
@@ -46,6 +47,7 @@ Renderer::Renderer() {
     tuning_.bvh_traversal_cost = 1.0f;
     tuning_.trace_refill = 16;
     tuning_.bvh_width = 4;
+    tuning_.bvh_builder = PT_BVH_BUILDER_HOST;
 }
 
 Renderer::~Renderer() {
@@ -245,23 +247,53 @@ pt_status Renderer::upload_scene(uint32_t id, const pt_scene_desc& desc) {
     bo.width = tuning_.bvh_width;
     auto rs = std::make_unique<ResidentScene>();
     SceneError err{PT_OK, ""};
-    if (!build_host_scene(desc, bo, rs->host, err)) return fail(err.status, err.message);
-    const HostScene& h = rs->host;
+    const bool gpu_bvh = tuning_.bvh_builder == PT_BVH_BUILDER_GPU && desc.triangle_count > 0;
+    if (!build_host_scene(desc, bo, rs->host, err, !gpu_bvh)) return fail(err.status, err.message);
+    HostScene& h = rs->host;
     auto upload = [&](DeviceBuffer& b, const void* src, size_t bytes, const char* what) -> pt_status {
         PT_TRY_HIP(b.allocate(bytes), what);
         if (bytes) PT_TRY_HIP(hipMemcpyAsync(b.as<void>(), src, bytes, hipMemcpyHostToDevice, stream_), what);
         return PT_OK;
     };
-    if (h.bvh.width == 2) {
+    if (gpu_bvh) {
+        // positions and material ids to the device, then the linear-BVH build there
+        const uint32_t T = desc.triangle_count;
+        DeviceBuffer pos, mids, nodes;
+        PT_TRY(upload(pos, desc.positions, (size_t)T * 9 * sizeof(float), "positions"));
+        PT_TRY(upload(mids, desc.material_ids, (size_t)T * sizeof(uint32_t), "material ids"));
+        PT_TRY(upload(rs->materials, h.materials.data(), h.materials.size() * sizeof(MaterialRecord), "materials"));
+        const uint32_t cap = gpu_bvh4_node_bound(T);
+        PT_TRY_HIP(nodes.allocate((size_t)cap * sizeof(Bvh4Node)), "BVH nodes");
+        PT_TRY_HIP(rs->tris.allocate((size_t)T * sizeof(TriRecord)), "triangle records");
+        PT_TRY_HIP(rs->order.allocate((size_t)T * sizeof(uint32_t)), "BVH order");
+        GpuBvhStats st;
+        PT_TRY_HIP(gpu_build_bvh4(stream_, pos.as<float>(), mids.as<uint32_t>(), rs->materials.as<MaterialRecord>(), T,
+                                  bo.max_leaf_size, nodes.as<Bvh4Node>(), cap, rs->tris.as<TriRecord>(),
+                                  rs->order.as<uint32_t>(), st),
+                   "GPU BVH build");
+        // keep only the nodes written
+        PT_TRY_HIP(rs->nodes.allocate((size_t)st.node_count * sizeof(Bvh4Node)), "BVH nodes");
+        PT_TRY_HIP(hipMemcpyAsync(rs->nodes.as<void>(), nodes.as<void>(), (size_t)st.node_count * sizeof(Bvh4Node),
+                                  hipMemcpyDeviceToDevice, stream_),
+                   "BVH nodes");
+        PT_TRY_HIP(hipStreamSynchronize(stream_), "BVH nodes");
+        h.bvh.width = 4;
+        h.bvh.depth = st.depth;
+        h.bvh.leaf_count = st.leaf_count;
+        h.bvh.max_leaf = st.max_leaf;
+        h.bvh.device_nodes = st.node_count;
+    } else if (h.bvh.width == 2) {
         const std::vector<Bvh2Node> n2 = to_bvh2(h.bvh);
         PT_TRY(upload(rs->nodes, n2.data(), n2.size() * sizeof(Bvh2Node), "BVH nodes"));
         PT_TRY_HIP(hipStreamSynchronize(stream_), "BVH nodes");
     } else {
         PT_TRY(upload(rs->nodes, h.bvh.nodes.data(), h.bvh.nodes.size() * sizeof(Bvh4Node), "BVH nodes"));
     }
-    PT_TRY(upload(rs->tris, h.tris.data(), h.tris.size() * sizeof(TriRecord), "triangle records"));
+    if (!gpu_bvh) {
+        PT_TRY(upload(rs->tris, h.tris.data(), h.tris.size() * sizeof(TriRecord), "triangle records"));
+        PT_TRY(upload(rs->materials, h.materials.data(), h.materials.size() * sizeof(MaterialRecord), "materials"));
+    }
     PT_TRY(upload(rs->shade, h.shade.data(), h.shade.size() * sizeof(ShadeRecord), "shading records"));
-    PT_TRY(upload(rs->materials, h.materials.data(), h.materials.size() * sizeof(MaterialRecord), "materials"));
     PT_TRY(upload(rs->lights, h.lights.data(), h.lights.size() * sizeof(LightRecord), "lights"));
     PT_TRY(upload(rs->bins, h.bins.data(), h.bins.size() * sizeof(AliasBin), "alias table"));
     const SceneDev d = rs->dev();
@@ -975,7 +1007,7 @@ pt_status Renderer::GetAliasTable(pt_alias_bin* bins, uint32_t cap, uint32_t* co
 
 pt_status Renderer::GetBvhInfo(pt_bvh_info& info) {
     if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
-    info.node_count = (uint32_t)host0().bvh.nodes.size();
+    info.node_count = host0().bvh.device_nodes ? host0().bvh.device_nodes : (uint32_t)host0().bvh.nodes.size();
     info.leaf_count = host0().bvh.leaf_count;
     info.max_depth = host0().bvh.depth;
     info.max_leaf_size = host0().bvh.max_leaf;
@@ -989,6 +1021,23 @@ pt_status Renderer::GetBvhInfo(pt_bvh_info& info) {
 
 pt_status Renderer::GetBvh4(void* nodes, uint32_t node_cap, uint32_t* order, uint32_t order_cap) {
     if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (host0().bvh.device_nodes) {
+        // built on the device: read the tree back
+        const ResidentScene& rs = *scenes_[0];
+        const uint32_t nn = host0().bvh.device_nodes, T = host0().tri_count;
+        if (nodes && node_cap < nn) return fail(PT_ERROR_INVALID_ARGUMENT, "node capacity too small");
+        if (order && order_cap < T) return fail(PT_ERROR_INVALID_ARGUMENT, "order capacity too small");
+        if (nodes)
+            PT_TRY_HIP(hipMemcpyAsync(nodes, rs.nodes.as<void>(), (size_t)nn * sizeof(Bvh4Node), hipMemcpyDeviceToHost,
+                                      stream_),
+                       "BVH nodes");
+        if (order)
+            PT_TRY_HIP(hipMemcpyAsync(order, rs.order.as<void>(), (size_t)T * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                      stream_),
+                       "BVH order");
+        PT_TRY_HIP(hipStreamSynchronize(stream_), "BVH read-back");
+        return PT_OK;
+    }
     if (nodes) {
         if (node_cap < host0().bvh.nodes.size()) return fail(PT_ERROR_INVALID_ARGUMENT, "node capacity too small");
         std::memcpy(nodes, host0().bvh.nodes.data(), host0().bvh.nodes.size() * sizeof(Bvh4Node));
@@ -1010,6 +1059,10 @@ pt_status Renderer::SetTuning(const pt_tuning& t) {
     if (!trace_config_supported(t.trace_block, t.trace_block >= 256 ? 8 : 16))
         return fail(PT_ERROR_INVALID_ARGUMENT, "trace_block must be 64, 128, 256 or 512");
     if (t.bvh_width != 2 && t.bvh_width != 4) return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_width must be 2 or 4");
+    if (t.bvh_builder != PT_BVH_BUILDER_HOST && t.bvh_builder != PT_BVH_BUILDER_GPU)
+        return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_builder must be PT_BVH_BUILDER_HOST or PT_BVH_BUILDER_GPU");
+    if (t.bvh_builder == PT_BVH_BUILDER_GPU && t.bvh_width != 4)
+        return fail(PT_ERROR_INVALID_ARGUMENT, "the GPU BVH builder emits 4-wide nodes");
     if (t.shade_block != 256) return fail(PT_ERROR_INVALID_ARGUMENT, "shade_block must be 256");
     if (t.pool_size < 1024 || t.pool_size > (1u << 26)) return fail(PT_ERROR_INVALID_ARGUMENT, "pool_size must be in [1024, 2^26]");
     if (t.trace_refill < 1 || t.trace_refill > 64) return fail(PT_ERROR_INVALID_ARGUMENT, "trace_refill must be in [1, 64]");

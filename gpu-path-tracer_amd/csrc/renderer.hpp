@@ -50,6 +50,7 @@ struct RunOptions {
 struct ResidentScene {
     HostScene host;
     DeviceBuffer nodes, tris, shade, materials, lights, bins;
+    DeviceBuffer order;        // GPU-built BVH: original triangle of each BVH slot (the host copy is empty)
     SceneDev dev() const;
 };
 

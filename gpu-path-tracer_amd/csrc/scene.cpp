@@ -72,7 +72,8 @@ V3 fallback_tangent(V3 n) {
 
 }  // namespace
 
-bool build_host_scene(const pt_scene_desc& d, const BvhBuildOptions& bvh_options, HostScene& out, SceneError& err) {
+bool build_host_scene(const pt_scene_desc& d, const BvhBuildOptions& bvh_options, HostScene& out, SceneError& err,
+                      bool build_bvh) {
     const uint32_t T = d.triangle_count;
     if (T > 0 && (!d.positions || !d.normals || !d.material_ids)) {
         err = {PT_ERROR_INVALID_ARGUMENT, "scene needs positions, normals and material ids"};
@@ -192,6 +193,7 @@ bool build_host_scene(const pt_scene_desc& d, const BvhBuildOptions& bvh_options
         }
     }
 
+    if (!build_bvh) return true;
     try {
         out.bvh = build_bvh4(d.positions, T, bvh_options);
     } catch (const std::exception& e) {

@@ -38,7 +38,10 @@ struct SceneError {
     pt_status status;
     std::string message;
 };
-bool build_host_scene(const pt_scene_desc& desc, const BvhBuildOptions& bvh_options, HostScene& out, SceneError& err);
+// With build_bvh false the BVH and the BVH-ordered triangle records are left
+// to the caller (the GPU builder).
+bool build_host_scene(const pt_scene_desc& desc, const BvhBuildOptions& bvh_options, HostScene& out, SceneError& err,
+                      bool build_bvh = true);
 
 // MakeDeviceCamera (Renderer.cpp:237-257).
 CameraBasis make_camera_basis(const pt_camera& camera, float aspect, const pt_motion_state& state);

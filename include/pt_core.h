@@ -224,7 +224,13 @@ typedef struct pt_tuning {
     float bvh_traversal_cost;    /* SAH cost of a node visit relative to a triangle test; next upload */
     uint32_t trace_refill;       /* idle lanes of a wave that trigger fetching new rays (1..64) */
     uint32_t bvh_width;          /* children per BVH node: 4 (128-byte nodes) or 2 (64-byte); next upload */
+    uint32_t bvh_builder;        /* PT_BVH_BUILDER_*: where the next uploads build their BVH */
 } pt_tuning;
+
+/* BVH builders: the host's binned-SAH build (best trees), or a linear BVH over
+ * Morton codes built on the GPU (fast for scenes of millions of triangles; 4-wide
+ * nodes only).  Traversal results do not depend on the tree. */
+enum { PT_BVH_BUILDER_HOST = 0, PT_BVH_BUILDER_GPU = 1 };
 
 /* ---- lifetime (Renderer ctor/dtor, Renderer.hpp:106-112) ---- */
 pt_status pt_renderer_create(pt_renderer** out);
