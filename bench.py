@@ -262,6 +262,55 @@ def run_serve(a):
     r.close()
 
 
+def run_ingest(a):
+    """Scene ingestion: the host's binned-SAH build against the GPU's linear
+    BVH, on the production scene (983,042 triangles) and a config-1 room of
+    20M triangles: scene upload + BVH build wall time, tree size, and the
+    production frame's rays/s over each tree."""
+    import ptcore
+    import scenes
+    t0 = time.time()
+    prod = scenes.production_scene()
+    room = scenes.cornell_scaled(int(a.ingest_triangles))
+    log(f"ingest: scenes of {prod.triangle_count} and {room.triangle_count} triangles generated in "
+        f"{time.time() - t0:.1f}s")
+    r = ptcore.Renderer()
+    st = dict(prod.settings)
+    r.InitializeRenderingBuffers(st["width"], st["height"])
+    runs = []
+    for name, scene in (("spheres_983042", prod), (f"room_{room.triangle_count}", room)):
+        for builder, label in ((0, "host binned SAH"), (1, "GPU linear BVH")):
+            r.set_tuning(bvh_builder=builder)
+            r.InitializeSceneBuffers(prod)                      # warm: allocations, first-use costs
+            t0 = time.perf_counter()
+            r.InitializeSceneBuffers(scene)
+            ms = (time.perf_counter() - t0) * 1e3
+            info = r.bvh_info()
+            entry = {"scene": name, "triangles": scene.triangle_count, "builder": label,
+                     "upload_build_ms": round(ms, 1), "nodes": info["node_count"], "depth": info["max_depth"]}
+            if scene is prod:
+                r.RenderFrame(st, prod.view)
+                rays, dev = 0, 0.0
+                for _ in range(2):
+                    f = r.RenderFrame(st, prod.view)
+                    rays += _rays(f)
+                    dev += f["render_ms"]
+                entry["frame_mrays_s"] = round(rays / dev / 1e3, 1)
+            log(f"ingest: {name} with the {label}: upload + build {ms:.0f} ms, {info['node_count']} nodes, depth "
+                f"{info['max_depth']}" + (f", production frame {entry['frame_mrays_s']} Mrays/s"
+                                          if "frame_mrays_s" in entry else ""))
+            runs.append(entry)
+    big = [e for e in runs if e["scene"] != "spheres_983042"]
+    out = {"metric": f"scene upload + BVH build time, {room.triangle_count}-triangle room (GPU builder)",
+           "value": big[1]["upload_build_ms"], "unit": "ms", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": big[1]["upload_build_ms"], "higher_is_better": False, "scaling": "strong",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"ingest: production scene and a {room.triangle_count}-triangle config-1 room"},
+           "host_builder_ms": big[0]["upload_build_ms"], "runs": runs}
+    print(json.dumps(out), flush=True)
+    r.close()
+
+
 def run_guide(a):
     """Config 5: the 200k-triangle two-room scene lit through a 0.5 m gap,
     1280x720, online-trained guiding (one-sample MIS with the BSDF) against
@@ -336,7 +385,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=["production", "cornell", "batch", "serve", "guide"], default="production")
+    ap.add_argument("--config", choices=["production", "cornell", "batch", "serve", "guide", "ingest"],
+                    default="production")
+    ap.add_argument("--ingest-triangles", type=float, default=20e6, help="room size of --config ingest")
     ap.add_argument("--guide-spp", type=int, default=256)
     ap.add_argument("--guide-ref-spp", type=int, default=4096)
     ap.add_argument("--guide-steps", type=int, default=256)
@@ -367,6 +418,9 @@ def main():
         return
     if a.config == "guide":
         run_guide(a)
+        return
+    if a.config == "ingest":
+        run_ingest(a)
         return
     import ptcore
     import scenes

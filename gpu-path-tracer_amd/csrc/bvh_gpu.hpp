@@ -16,6 +16,9 @@ struct GpuBvhStats {
     uint32_t bvh2_levels = 0;  // levels of the binary tree it was collapsed from
 };
 
+// Largest leaf the renderer asks of the GPU builder.
+constexpr uint32_t kGpuBvhLeaf = 2;
+
 // Room the builder needs for n triangles: one BVH4 node per binary internal node at most.
 inline uint32_t gpu_bvh4_node_bound(uint32_t n) { return n > 1 ? n - 1 : 1; }
 

@@ -229,7 +229,8 @@ typedef struct pt_tuning {
 
 /* BVH builders: the host's binned-SAH build (best trees), or a linear BVH over
  * Morton codes built on the GPU (fast for scenes of millions of triangles; 4-wide
- * nodes only).  Traversal results do not depend on the tree. */
+ * nodes only; leaves of at most min(bvh_max_leaf, 2) triangles).  Traversal
+ * results do not depend on the tree. */
 enum { PT_BVH_BUILDER_HOST = 0, PT_BVH_BUILDER_GPU = 1 };
 
 /* ---- lifetime (Renderer ctor/dtor, Renderer.hpp:106-112) ---- */
