@@ -17,7 +17,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["kernels.hip", "guide.hip", "bvh_gpu.hip", "bvh_build.cpp", "scene.cpp", "renderer.cpp", "server.cpp", "guide_host.cpp",
+SOURCES = ["kernels.hip", "guide.hip", "bvh_gpu.hip", "bvh_build.cpp", "scene.cpp", "renderer.cpp", "server.cpp", "guide_host.cpp", "obj_import.cpp",
            "capi.cpp"]
 
 # -ffp-contract=off: no FMA contraction, so float expressions round the same

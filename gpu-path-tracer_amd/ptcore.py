@@ -180,6 +180,9 @@ SIGNATURES = {
     "pt_get_bvh_info": (C.c_int, [_P, C.POINTER(BvhInfo)]),
     "pt_get_bvh4": (C.c_int, [_P, _P, _U32, _P, _U32]),
     "pt_build_bvh4": (C.c_int, [_P, _U32, _U32, _P, _U32, C.POINTER(_U32), _P, C.POINTER(_U32)]),
+    "pt_import_obj": (C.c_int, [C.c_char_p, C.POINTER(_P), C.c_char_p, C.c_size_t]),
+    "pt_imported_scene_desc": (C.POINTER(SceneDesc), [_P]),
+    "pt_imported_scene_free": (None, [_P]),
     "pt_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "pt_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
 }
@@ -556,6 +559,36 @@ def build_bvh4(positions, max_leaf_size=0):
     if st != PT_OK:
         raise PtError(st, "pt_build_bvh4 failed")
     return nodes, order, depth.value
+
+
+def import_obj(path):
+    """pt_import_obj: an OBJ file and its MTL libraries as a scenes.Scene
+    (arrays copied out of the library's import)."""
+    import scenes
+    lib = load_library()
+    h = _P()
+    err = C.create_string_buffer(1024)
+    st = lib.pt_import_obj(os.fspath(path).encode(), C.byref(h), err, len(err))
+    if st != PT_OK:
+        raise PtError(st, err.value.decode(errors="replace"))
+    try:
+        d = lib.pt_imported_scene_desc(h).contents
+        T, M = d.triangle_count, d.material_count
+
+        def arr(ptr, dtype, shape):
+            n = int(np.prod(shape))
+            if n == 0:
+                return np.zeros(shape, dtype)
+            buf = (C.c_char * (n * np.dtype(dtype).itemsize)).from_address(ptr)
+            return np.frombuffer(buf, dtype).reshape(shape).copy()
+
+        return scenes.Scene(name=os.path.basename(os.fspath(path)), positions=arr(d.positions, np.float32, (T, 3, 3)),
+                            normals=arr(d.normals, np.float32, (T, 3, 3)),
+                            tangents=arr(d.tangents, np.float32, (T, 3, 4)),
+                            material_ids=arr(d.material_ids, np.uint32, (T,)),
+                            materials=arr(d.materials, scenes.MATERIAL_DTYPE, (M,)))
+    finally:
+        lib.pt_imported_scene_free(h)
 
 
 def hits_as_fields(hits):

@@ -45,7 +45,7 @@ typedef enum pt_status {
     PT_ERROR_DEVICE = 4,          /* a HIP runtime call failed */
     PT_ERROR_OUT_OF_MEMORY = 5,
     PT_ERROR_UNSUPPORTED = 6,     /* shading model outside this core (Normal, Phong) */
-    PT_ERROR_IO = 7               /* frame could not be saved */
+    PT_ERROR_IO = 7               /* a file could not be read or written */
 } pt_status;
 
 /* Values of GlobalShadingModel (Core/Include/Core/Graphics/Cuda/PathTracing/Material.hpp:8-17). */
@@ -454,6 +454,24 @@ pt_status pt_get_bvh4(pt_renderer* renderer, void* nodes, uint32_t node_capacity
 pt_status pt_build_bvh4(const float* positions, uint32_t triangle_count, uint32_t max_leaf_size,
                         void* nodes, uint32_t node_capacity, uint32_t* node_count,
                         uint32_t* prim_order, uint32_t* depth);
+
+/* ---- scene ingestion (ObjLoader.cpp:67-234, Manager.cpp:311-412) ----
+ * Reads a Wavefront OBJ and the MTL libraries it names (next to it) into a
+ * scene for pt_initialize_scene_buffers / pt_add_scene, as the reference's
+ * import does: faces fan-triangulated, per-vertex normals and uvs required,
+ * triangles grouped per shape (o/g) and material; the MTL key `surface`
+ * selects the model (unlit, unknown or absent: diffuse); Kd and Ks pass
+ * through sRGB u8 quantisation (0.73 -> 222 -> 0.7304); roughness and metallic
+ * are the default RMA texel (1, 0); Ke is kept as fp32; Ni is the ior; the
+ * normal-map constant is (128, 128, 255); faces without a material get the
+ * default material (white).  Tangents come from the uv derivatives (w =
+ * handedness); the reference uses MikkTSpace.  On failure *out is NULL and
+ * `error` receives the message. */
+typedef struct pt_imported_scene pt_imported_scene;
+pt_status pt_import_obj(const char* path, pt_imported_scene** out, char* error, size_t error_capacity);
+/* The scene's description; valid until pt_imported_scene_free. */
+const pt_scene_desc* pt_imported_scene_desc(const pt_imported_scene* scene);
+void pt_imported_scene_free(pt_imported_scene* scene);
 
 /* ---- tuning ---- */
 pt_status pt_get_tuning(pt_renderer* renderer, pt_tuning* tuning);
