@@ -18,7 +18,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["kernels.hip", "guide.hip", "bvh_gpu.hip", "bvh_build.cpp", "scene.cpp", "renderer.cpp", "server.cpp", "guide_host.cpp", "obj_import.cpp",
-           "capi.cpp"]
+           "tangent_space.cpp", "image_io.cpp", "capi.cpp"]
 
 # -ffp-contract=off: no FMA contraction, so float expressions round the same
 # way on the CPU and the GPU.  The packed-fp32 feature is switched off for all
@@ -65,7 +65,7 @@ def build(jobs=None, verbose=False, out=LIB, defines=()):
             if warn and verbose:
                 print(warn, flush=True)
     tmp = out + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + sorted(objs) + ["-lpthread"]
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + sorted(objs) + ["-lpthread", "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stdout}\n{r.stderr}")

@@ -3,6 +3,7 @@
 #include <cstring>
 #include <new>
 
+#include "image_io.hpp"
 #include "renderer.hpp"
 
 struct pt_renderer {
@@ -91,6 +92,33 @@ pt_status pt_frame_copy(pt_renderer* r, uint32_t* rgba, size_t pixel_count) {
 
 pt_status pt_frame_copy_hdr(pt_renderer* r, float* rgba, size_t pixel_count) {
     return guarded(r, [&](pt::Renderer& x) { return x.CopyFrameHdr(rgba, pixel_count); });
+}
+
+pt_status pt_save_frame(pt_renderer* r, const char* path) {
+    return guarded(r, [&](pt::Renderer& x) { return x.SaveFrame(path); });
+}
+
+pt_status pt_write_image(const char* path, uint32_t width, uint32_t height, uint32_t pixel_format,
+                         const void* pixels, int bottom_up) {
+    const std::string p = path ? path : "";
+    pt::ImageKind kind;
+    std::string err;
+    pt::ImageStatus s = pt::image_kind(p, width, height, kind, err);
+    if (s == pt::ImageStatus::Ok) {
+        if (!pixels) return PT_ERROR_INVALID_ARGUMENT;
+        if (kind == pt::ImageKind::Png && pixel_format == PT_PIXELS_RGBA8)
+            s = pt::write_png_rgba8(p, width, height, (const uint32_t*)pixels, bottom_up != 0, err);
+        else if (kind == pt::ImageKind::Exr && pixel_format == PT_PIXELS_RGBA32F)
+            s = pt::write_exr_rgba32f(p, width, height, (const float*)pixels, bottom_up != 0, err);
+        else
+            return PT_ERROR_UNSUPPORTED;
+    }
+    switch (s) {
+        case pt::ImageStatus::Ok: return PT_OK;
+        case pt::ImageStatus::InvalidArgument: return PT_ERROR_INVALID_ARGUMENT;
+        case pt::ImageStatus::Unsupported: return PT_ERROR_UNSUPPORTED;
+        default: return PT_ERROR_IO;
+    }
 }
 
 size_t pt_peek_last_error(const pt_renderer* r, char* buf, size_t capacity) {

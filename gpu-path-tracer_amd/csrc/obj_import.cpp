@@ -3,8 +3,9 @@
 // and uvs required, vertices deduplicated by (v, vt, vn), submeshes per shape
 // and material) and Manager.cpp:311-412 (constant material channels stored as
 // 1x1 textures: colour and specular as sRGB u8, the default RMA texel, fp32
-// emission, the default normal-map texel).  The parser is written from the
-// OBJ/MTL format description; the reference parses with tinyobjloader.
+// emission, the default normal-map texel), with tangents by Mikkelsen's
+// method per submesh (TangentSpace.cpp:85-89).  The parser is written from
+// the OBJ/MTL format description; the reference parses with tinyobjloader.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "pt_core.h"
+#include "tangent_space.hpp"
 
 struct pt_imported_scene {
     std::vector<float> positions, normals, tangents;
@@ -169,57 +171,6 @@ struct Shape {
     std::vector<Face> faces;
 };
 
-struct V3d {
-    double x, y, z;
-};
-
-// Per-vertex tangents of one submesh from the uv derivatives of its faces,
-// accumulated over the faces sharing the vertex, made orthogonal to the
-// normal; w = handedness of (normal, tangent, bitangent).
-void tangents_of(const std::vector<float>& pos, const std::vector<float>& uv, const std::vector<float>& nrm,
-                 const std::vector<uint32_t>& idx, std::vector<float>& tan4) {
-    const size_t nv = pos.size() / 3;
-    std::vector<V3d> t(nv, V3d{0, 0, 0}), b(nv, V3d{0, 0, 0});
-    for (size_t f = 0; f + 2 < idx.size(); f += 3) {
-        const uint32_t i0 = idx[f], i1 = idx[f + 1], i2 = idx[f + 2];
-        const double e1[3] = {pos[3 * i1] - (double)pos[3 * i0], pos[3 * i1 + 1] - (double)pos[3 * i0 + 1],
-                              pos[3 * i1 + 2] - (double)pos[3 * i0 + 2]};
-        const double e2[3] = {pos[3 * i2] - (double)pos[3 * i0], pos[3 * i2 + 1] - (double)pos[3 * i0 + 1],
-                              pos[3 * i2 + 2] - (double)pos[3 * i0 + 2]};
-        const double du1 = uv[2 * i1] - (double)uv[2 * i0], dv1 = uv[2 * i1 + 1] - (double)uv[2 * i0 + 1];
-        const double du2 = uv[2 * i2] - (double)uv[2 * i0], dv2 = uv[2 * i2 + 1] - (double)uv[2 * i0 + 1];
-        const double det = du1 * dv2 - du2 * dv1;
-        if (std::fabs(det) < 1e-20) continue;
-        const double r = 1.0 / det;
-        const V3d tt{(e1[0] * dv2 - e2[0] * dv1) * r, (e1[1] * dv2 - e2[1] * dv1) * r, (e1[2] * dv2 - e2[2] * dv1) * r};
-        const V3d bb{(e2[0] * du1 - e1[0] * du2) * r, (e2[1] * du1 - e1[1] * du2) * r, (e2[2] * du1 - e1[2] * du2) * r};
-        for (uint32_t i : {i0, i1, i2}) {
-            t[i] = V3d{t[i].x + tt.x, t[i].y + tt.y, t[i].z + tt.z};
-            b[i] = V3d{b[i].x + bb.x, b[i].y + bb.y, b[i].z + bb.z};
-        }
-    }
-    tan4.assign(nv * 4, 0.0f);
-    for (size_t i = 0; i < nv; i++) {
-        const V3d n{nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]};
-        const double tn = t[i].x * n.x + t[i].y * n.y + t[i].z * n.z;
-        V3d o{t[i].x - n.x * tn, t[i].y - n.y * tn, t[i].z - n.z * tn};
-        double len = std::sqrt(o.x * o.x + o.y * o.y + o.z * o.z);
-        if (!(len > 1e-12)) {
-            // no usable uv derivative: any unit vector orthogonal to n
-            const V3d a = std::fabs(n.x) < 0.9 ? V3d{1, 0, 0} : V3d{0, 1, 0};
-            const double an = a.x * n.x + a.y * n.y + a.z * n.z;
-            o = V3d{a.x - n.x * an, a.y - n.y * an, a.z - n.z * an};
-            len = std::sqrt(o.x * o.x + o.y * o.y + o.z * o.z);
-        }
-        const V3d c{n.y * o.z - n.z * o.y, n.z * o.x - n.x * o.z, n.x * o.y - n.y * o.x};
-        const double hand = c.x * b[i].x + c.y * b[i].y + c.z * b[i].z < 0.0 ? -1.0 : 1.0;
-        tan4[4 * i + 0] = (float)(o.x / len);
-        tan4[4 * i + 1] = (float)(o.y / len);
-        tan4[4 * i + 2] = (float)(o.z / len);
-        tan4[4 * i + 3] = (float)hand;
-    }
-}
-
 void import_obj(const std::filesystem::path& path, pt_imported_scene& out) {
     std::ifstream in(path);
     if (!in) throw ImportError{"cannot open " + path.string()};
@@ -318,19 +269,19 @@ void import_obj(const std::filesystem::path& path, pt_imported_scene& out) {
                     if (fresh) {
                         pos.insert(pos.end(), {vp[3 * c.v], vp[3 * c.v + 1], vp[3 * c.v + 2]});
                         uv.insert(uv.end(), {vt[2 * c.t], vt[2 * c.t + 1]});
-                        double x = vn[3 * c.n], y = vn[3 * c.n + 1], z = vn[3 * c.n + 2];
-                        const double l = std::sqrt(x * x + y * y + z * z);
-                        if (l > 0.0) {
-                            x /= l;
-                            y /= l;
-                            z /= l;
-                        }
-                        nrm.insert(nrm.end(), {(float)x, (float)y, (float)z});
+                        // glm::normalize in float (ObjLoader.cpp:123): v * (1 / sqrt(v.v));
+                        // a zero normal is kept as zero
+                        const float x = vn[3 * c.n], y = vn[3 * c.n + 1], z = vn[3 * c.n + 2];
+                        const float d = x * x + y * y + z * z;
+                        const float r = d > 0.0f ? 1.0f / std::sqrt(d) : 0.0f;
+                        nrm.insert(nrm.end(), {x * r, y * r, z * r});
                     }
                     idx.push_back(it->second);
                 }
-            std::vector<float> tan4;
-            tangents_of(pos, uv, nrm, idx, tan4);
+            // per-submesh tangents (ObjLoader.cpp:150-152)
+            std::vector<float> tan4(pos.size() / 3 * 4);
+            mikk_tangents(pos.data(), nrm.data(), uv.data(), (uint32_t)(pos.size() / 3), idx.data(), idx.size(),
+                          tan4.data());
             const uint32_t mid = mat >= 0 ? (uint32_t)mat : default_id;
             need_default = need_default || mat < 0;
             for (size_t k = 0; k < idx.size(); k++) {

@@ -1,4 +1,5 @@
 #include "renderer.hpp"
+#include "image_io.hpp"
 #include "bvh_gpu.hpp"
 #include "server.hpp"
 
@@ -810,35 +811,16 @@ size_t Renderer::ConsumeLastError(char* buf, size_t cap) {
     return k;
 }
 
-// ---------------------------------------------------------------- PNG (stored deflate)
+// ---------------------------------------------------------------- image files
 
 namespace {
-uint32_t crc32_update(uint32_t crc, const uint8_t* p, size_t n) {
-    static uint32_t table[256];
-    static bool init = false;
-    if (!init) {
-        for (uint32_t i = 0; i < 256; i++) {
-            uint32_t c = i;
-            for (int k = 0; k < 8; k++) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
-            table[i] = c;
-        }
-        init = true;
+pt_status image_status(ImageStatus s) {
+    switch (s) {
+        case ImageStatus::Ok: return PT_OK;
+        case ImageStatus::InvalidArgument: return PT_ERROR_INVALID_ARGUMENT;
+        case ImageStatus::Unsupported: return PT_ERROR_UNSUPPORTED;
+        default: return PT_ERROR_IO;
     }
-    for (size_t i = 0; i < n; i++) crc = table[(crc ^ p[i]) & 0xFFu] ^ (crc >> 8);
-    return crc;
-}
-void put_be32(std::vector<uint8_t>& v, uint32_t x) {
-    v.push_back((uint8_t)(x >> 24));
-    v.push_back((uint8_t)(x >> 16));
-    v.push_back((uint8_t)(x >> 8));
-    v.push_back((uint8_t)x);
-}
-void chunk(std::vector<uint8_t>& out, const char* type, const std::vector<uint8_t>& data) {
-    put_be32(out, (uint32_t)data.size());
-    std::vector<uint8_t> td(type, type + 4);
-    td.insert(td.end(), data.begin(), data.end());
-    out.insert(out.end(), td.begin(), td.end());
-    put_be32(out, crc32_update(0xFFFFFFFFu, td.data(), td.size()) ^ 0xFFFFFFFFu);
 }
 }  // namespace
 
@@ -848,55 +830,23 @@ pt_status Renderer::save_frame_png(const std::string& path) {
         std::scoped_lock lock(frame_mutex_);
         px = frame_;
     }
-    const uint32_t W = width_, H = height_;
-    std::vector<uint8_t> raw;
-    raw.reserve((size_t)H * (1 + 4 * (size_t)W));
-    for (uint32_t row = 0; row < H; row++) {
-        const uint32_t y = H - 1 - row;   // row 0 of the image is the top: flip (Renderer.cpp:672)
-        raw.push_back(0);
-        for (uint32_t x = 0; x < W; x++) {
-            const uint32_t c = px[(size_t)y * W + x];
-            raw.push_back((uint8_t)c);
-            raw.push_back((uint8_t)(c >> 8));
-            raw.push_back((uint8_t)(c >> 16));
-            raw.push_back((uint8_t)(c >> 24));
-        }
-    }
-    std::vector<uint8_t> z = {0x78, 0x01};
-    uint32_t a = 1, b = 0;
-    for (uint8_t v : raw) {
-        a = (a + v) % 65521u;
-        b = (b + a) % 65521u;
-    }
-    size_t off = 0;
-    do {
-        const size_t len = std::min<size_t>(65535, raw.size() - off);
-        const bool last = off + len == raw.size();
-        z.push_back(last ? 1 : 0);
-        z.push_back((uint8_t)len);
-        z.push_back((uint8_t)(len >> 8));
-        z.push_back((uint8_t)~len);
-        z.push_back((uint8_t)(~len >> 8));
-        z.insert(z.end(), raw.begin() + off, raw.begin() + off + len);
-        off += len;
-    } while (off < raw.size());
-    put_be32(z, (b << 16) | a);
-    std::vector<uint8_t> png = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
-    std::vector<uint8_t> ihdr;
-    put_be32(ihdr, W);
-    put_be32(ihdr, H);
-    ihdr.insert(ihdr.end(), {8, 6, 0, 0, 0});
-    chunk(png, "IHDR", ihdr);
-    chunk(png, "IDAT", z);
-    chunk(png, "IEND", {});
-    std::error_code ec;
-    std::filesystem::create_directories(std::filesystem::path(path).parent_path(), ec);
-    FILE* f = std::fopen(path.c_str(), "wb");
-    if (!f) return fail(PT_ERROR_IO, "cannot open " + path);
-    const size_t wrote = std::fwrite(png.data(), 1, png.size(), f);
-    std::fclose(f);
-    if (wrote != png.size()) return fail(PT_ERROR_IO, "short write to " + path);
-    return PT_OK;
+    std::string err;
+    const ImageStatus s = write_png_rgba8(path, width_, height_, px.data(), true, err);
+    return s == ImageStatus::Ok ? PT_OK : fail(image_status(s), err);
+}
+
+pt_status Renderer::SaveFrame(const char* path) {
+    const std::string p = path ? path : "";
+    ImageKind kind;
+    std::string err;
+    ImageStatus s = image_kind(p, width_, height_, kind, err);
+    if (s != ImageStatus::Ok) return fail(image_status(s), err);
+    if (kind == ImageKind::Png) return save_frame_png(p);
+    std::vector<float> hdr(4 * (size_t)width_ * height_);
+    const pt_status c = CopyFrameHdr(hdr.data(), (size_t)width_ * height_);
+    if (c != PT_OK) return c;
+    s = write_exr_rgba32f(p, width_, height_, hdr.data(), true, err);
+    return s == ImageStatus::Ok ? PT_OK : fail(image_status(s), err);
 }
 
 // ---------------------------------------------------------------- hooks

@@ -72,6 +72,8 @@ public:
     bool IsRendering() const { return is_rendering_.load(); }
     void GetProgress(pt_progress& p) const;
     pt_status CopyFrame(uint32_t* rgba, size_t n);
+    // the frame as ".png" (RGBA8) or ".exr" (float radiance of the last single frame)
+    pt_status SaveFrame(const char* path);
     pt_status CopyFrameHdr(float* rgba, size_t n);
     size_t PeekLastError(char* buf, size_t cap) const;
     size_t ConsumeLastError(char* buf, size_t cap);

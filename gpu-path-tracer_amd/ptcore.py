@@ -183,6 +183,9 @@ SIGNATURES = {
     "pt_import_obj": (C.c_int, [C.c_char_p, C.POINTER(_P), C.c_char_p, C.c_size_t]),
     "pt_imported_scene_desc": (C.POINTER(SceneDesc), [_P]),
     "pt_imported_scene_free": (None, [_P]),
+    "pt_generate_tangents": (C.c_int, [_P, _P, _P, C.c_uint32, _P, C.c_uint32, _P]),
+    "pt_write_image": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_int]),
+    "pt_save_frame": (C.c_int, [_P, C.c_char_p]),
     "pt_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "pt_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
 }
@@ -332,6 +335,10 @@ class Renderer:
         out = np.zeros((height, width, 4), np.float32)
         self._check(self.lib.pt_frame_copy_hdr(self.h, _ptr(out), width * height))
         return out
+
+    def SaveFrame(self, path):
+        """pt_save_frame: ".png" (RGBA8 frame) or ".exr" (float radiance)."""
+        self._check(self.lib.pt_save_frame(self.h, os.fspath(path).encode()))
 
     def PeekLastError(self):
         buf = C.create_string_buffer(4096)
@@ -589,6 +596,43 @@ def import_obj(path):
                             materials=arr(d.materials, scenes.MATERIAL_DTYPE, (M,)))
     finally:
         lib.pt_imported_scene_free(h)
+
+
+def generate_tangents(positions, normals, uvs, indices):
+    """pt_generate_tangents: (V, 4) float32 per-vertex tangents (xyz, sign) of
+    the triangles `indices` over per-vertex positions, unit normals and uvs."""
+    lib = load_library()
+    P = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    N = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    U = np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
+    idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    if not (len(P) == len(N) == len(U)):
+        raise ValueError("positions, normals and uvs need one row per vertex")
+    out = np.zeros((len(P), 4), np.float32)
+    st = lib.pt_generate_tangents(_ptr(P), _ptr(N), _ptr(U), len(P), _ptr(idx), len(idx), _ptr(out))
+    if st != PT_OK:
+        raise PtError(st, "pt_generate_tangents: index count not a multiple of 3 or an index out of range")
+    return out
+
+
+PT_PIXELS_RGBA8 = 0
+PT_PIXELS_RGBA32F = 1
+
+
+def write_image(path, pixels, bottom_up=False):
+    """pt_write_image: (H, W) uint32 RGBA8 words to ".png", or (H, W, 4)
+    float32 RGBA to ".exr"; row 0 is the top unless `bottom_up`."""
+    lib = load_library()
+    a = np.ascontiguousarray(pixels)
+    if a.dtype == np.uint32 and a.ndim == 2:
+        fmt, (h, w) = PT_PIXELS_RGBA8, a.shape
+    elif a.dtype == np.float32 and a.ndim == 3 and a.shape[2] == 4:
+        fmt, (h, w) = PT_PIXELS_RGBA32F, a.shape[:2]
+    else:
+        raise ValueError("pixels: (H, W) uint32 RGBA8 or (H, W, 4) float32")
+    st = lib.pt_write_image(os.fspath(path).encode(), w, h, fmt, _ptr(a), int(bool(bottom_up)))
+    if st != PT_OK:
+        raise PtError(st, f"pt_write_image({os.fspath(path)!r}) failed")
 
 
 def hits_as_fields(hits):

@@ -263,6 +263,12 @@ pt_status pt_get_progress(const pt_renderer* renderer, pt_progress* out);
 pt_status pt_frame_copy(pt_renderer* renderer, uint32_t* rgba, size_t pixel_count);
 /* accumulation / spp as float RGBA (needed for relMSE; no reference counterpart). */
 pt_status pt_frame_copy_hdr(pt_renderer* renderer, float* rgba, size_t pixel_count);
+/* The frame as an image file, by extension (SaveImage, ImageIO.cpp:119-160, as the
+ * render loop's PNG save uses it, Renderer.cpp:649-675): ".png" writes the RGBA8
+ * frame of pt_frame_copy, ".exr" the float radiance of pt_frame_copy_hdr (needs a
+ * finished pt_render_frame); top row first.  Empty path, no extension ->
+ * PT_ERROR_INVALID_ARGUMENT; another extension -> PT_ERROR_UNSUPPORTED. */
+pt_status pt_save_frame(pt_renderer* renderer, const char* path);
 /* Renderer::PeekLastError / ConsumeLastError, Renderer.cpp:308-320.
  * Copy the parked render-thread error into buf; returns its length, 0 if none. */
 size_t pt_peek_last_error(const pt_renderer* renderer, char* buf, size_t capacity);
@@ -464,14 +470,39 @@ pt_status pt_build_bvh4(const float* positions, uint32_t triangle_count, uint32_
  * through sRGB u8 quantisation (0.73 -> 222 -> 0.7304); roughness and metallic
  * are the default RMA texel (1, 0); Ke is kept as fp32; Ni is the ior; the
  * normal-map constant is (128, 128, 255); faces without a material get the
- * default material (white).  Tangents come from the uv derivatives (w =
- * handedness); the reference uses MikkTSpace.  On failure *out is NULL and
- * `error` receives the message. */
+ * default material (white).  Tangents per submesh as pt_generate_tangents
+ * computes them.  On failure *out is NULL and `error` receives the message. */
 typedef struct pt_imported_scene pt_imported_scene;
 pt_status pt_import_obj(const char* path, pt_imported_scene** out, char* error, size_t error_capacity);
 /* The scene's description; valid until pt_imported_scene_free. */
 const pt_scene_desc* pt_imported_scene_desc(const pt_imported_scene* scene);
 void pt_imported_scene_free(pt_imported_scene* scene);
+/* Per-vertex tangents of an indexed triangle mesh by Mikkelsen's method
+ * (mikktspace, 180-degree threshold), as GenerateTangentSpace does at import
+ * (TangentSpace.cpp:57-89): tangents[4 * v] = (x, y, z, sign); where faces
+ * sharing a vertex disagree, the last face in index order sets it; vertices no
+ * face uses get zeros.  positions/normals: float3 per vertex (normals unit),
+ * uvs: float2 per vertex.  PT_ERROR_INVALID_ARGUMENT for an index count that
+ * is not a multiple of 3 or an index >= vertex_count. */
+pt_status pt_generate_tangents(const float* positions, const float* normals, const float* uvs,
+                               uint32_t vertex_count, const uint32_t* indices, uint32_t index_count,
+                               float* tangents);
+
+/* ---- image files (ImageIO.cpp:119-160) ----
+ * Pixels to a file whose extension picks the container: ".png" takes
+ * PT_PIXELS_RGBA8 (little-endian RGBA8 words; an 8-bit RGBA PNG), ".exr" takes
+ * PT_PIXELS_RGBA32F (float RGBA; an OpenEXR scanline file with FLOAT channels,
+ * no compression).  bottom_up != 0: row 0 of `pixels` is the bottom row (the
+ * frame buffers' order) and is written last.  Errors as SaveImage: empty path,
+ * missing extension, zero width or height -> PT_ERROR_INVALID_ARGUMENT; another
+ * extension, or pixels the container does not take -> PT_ERROR_UNSUPPORTED;
+ * the file cannot be written -> PT_ERROR_IO.  No GPU is needed. */
+typedef enum pt_pixel_format {
+    PT_PIXELS_RGBA8 = 0,
+    PT_PIXELS_RGBA32F = 1
+} pt_pixel_format;
+pt_status pt_write_image(const char* path, uint32_t width, uint32_t height, uint32_t pixel_format,
+                         const void* pixels, int bottom_up);
 
 /* ---- tuning ---- */
 pt_status pt_get_tuning(pt_renderer* renderer, pt_tuning* tuning);

@@ -169,3 +169,28 @@ def tonemap(hdr):
     out = np.zeros(len(hdr), np.uint32)
     lib().orc_tonemap(_p(hdr), len(hdr), _p(out))
     return out
+
+
+REF_MIKK = os.path.join(HERE, "_ref", "libmikk_ref.so")
+_mikk = None
+
+
+def reference_tangents(positions, normals, uvs, indices):
+    """The reference's own mikktspace (External/mikktspace/mikktspace.c, built
+    by oracle/build.py into oracle/_ref) driven as the reference's import
+    drives it (TangentSpace.cpp:14-89): (V, 4) float32 per-vertex tangents."""
+    global _mikk
+    if _mikk is None:
+        if not os.path.exists(REF_MIKK):
+            raise RuntimeError(f"{REF_MIKK} missing: run `python oracle/build.py` where the reference is present")
+        L = C.CDLL(REF_MIKK)
+        L.mikk_ref_tangents.restype = C.c_int
+        L.mikk_ref_tangents.argtypes = [_P, _P, _P, _U32, _P, _U32, _P]
+        _mikk = L
+    P = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    N = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    U = np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
+    idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    out = np.zeros((len(P), 4), np.float32)
+    _mikk.mikk_ref_tangents(_p(P), _p(N), _p(U), len(P), _p(idx), len(idx), _p(out))
+    return out

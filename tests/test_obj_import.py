@@ -2,8 +2,11 @@
 the reference's material conversion (sRGB u8 quantisation of Kd and Ks, the
 `surface` key, default RMA and normal-map texels, fp32 emission), polygon
 triangulation, relative indices, required normals and uvs, default material,
-tangents; and on the GPU a render of an imported room against the oracle."""
+tangents equal to the reference's mikktspace; and on the GPU a render of an
+imported room against the oracle."""
 import math
+import os
+import sys
 
 import numpy as np
 import pytest
@@ -121,6 +124,39 @@ def test_import_missing_file(tmp_path):
     with pytest.raises(ptcore.PtError) as e:
         ptcore.import_obj(tmp_path / "missing.obj")
     assert "cannot open" in str(e.value)
+
+
+def test_imported_tangents_equal_reference_mikktspace(tmp_path):
+    """A bumpy grid with mirrored uvs written as OBJ (one index per corner for
+    v, vt and vn): the imported per-corner tangents equal the reference's
+    mikktspace run over the same vertices, normals as imported."""
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+    import build as oracle_build
+    import oracle
+    if not os.path.exists(oracle.REF_MIKK) and oracle_build.build_ref() is None:
+        pytest.skip("the reference's mikktspace sources are not present and oracle/_ref is not built")
+    n = 12
+    x, y = np.meshgrid(np.linspace(0, 1, n + 1), np.linspace(0, 1, n + 1))
+    z = 0.2 * np.sin(5 * x) * np.cos(4 * y)
+    P = np.stack([x, y, z], -1).reshape(-1, 3).astype(np.float32)
+    N = np.stack([-np.cos(5 * x) * np.cos(4 * y), 0.8 * np.sin(5 * x) * np.sin(4 * y), np.ones_like(x)], -1)
+    N = N.reshape(-1, 3).astype(np.float32)                         # not unit: the import normalises
+    U = np.stack([np.abs(x - 0.5), y], -1).reshape(-1, 2).astype(np.float32)
+    i = np.arange(n)[:, None] * (n + 1) + np.arange(n)[None, :]
+    q = np.stack([i, i + 1, i + n + 2, i + n + 1], -1).reshape(-1, 4)
+    lines = ["v %.9g %.9g %.9g" % tuple(p) for p in P] + ["vt %.9g %.9g" % tuple(u) for u in U] + \
+            ["vn %.9g %.9g %.9g" % tuple(v) for v in N]
+    lines += ["f " + " ".join(f"{k + 1}/{k + 1}/{k + 1}" for k in quad) for quad in q]   # quads, fan-split
+    (tmp_path / "g.obj").write_text("\n".join(lines) + "\n")
+    s = ptcore.import_obj(tmp_path / "g.obj")
+    idx = np.concatenate([q[:, [0, 1, 2]], q[:, [0, 2, 3]]], 1).reshape(-1)
+    assert s.triangle_count == 2 * n * n
+    Nv = np.zeros_like(P)
+    Nv[idx] = s.normals.reshape(-1, 3)
+    assert np.allclose(np.linalg.norm(Nv, axis=1), 1.0, atol=1e-6)
+    want = oracle.reference_tangents(P, Nv, U, idx)[idx]
+    assert np.array_equal(s.tangents.reshape(-1, 4), want)
+    assert (want[:, 3] == 1).any() and (want[:, 3] == -1).any()    # both sides of the mirror
 
 
 @pytest.mark.gpu
