@@ -23,6 +23,7 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes of one closest-hit ray: ray record (origin|tMin, direction|tMax)
 # 32 B in, hit record + queue id 17 B out, 128 B per BVH4 node, 48 B per triangle
 RAY_IN_B, HIT_OUT_B, NODE_B, TRI_B = 32, 17, 128, 48
+QNODE_B = 64        # quantized BVH4 node
 
 
 def log(msg):
@@ -399,6 +400,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--trace-block", type=int, default=0)
     ap.add_argument("--pool", type=int, default=0)
+    ap.add_argument("--quantized", type=int, choices=[0, 1, 2, 3], default=None,
+                    help="pt_tuning.bvh_quantized: bit 0 closest hit, bit 1 any hit on 64-byte quantized nodes "
+                         "(default: the library's, 2)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scale reference for FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args()
@@ -431,13 +435,16 @@ def main():
     view = scene.view
     log(f"scene {scene.name}: {scene.triangle_count} triangles, generated in {time.time() - t0:.1f}s")
     r = ptcore.Renderer()
-    if a.trace_block or a.pool:
+    if a.trace_block or a.pool or a.quantized is not None:
         kw = {}
         if a.trace_block:
             kw["trace_block"] = a.trace_block
         if a.pool:
             kw["pool_size"] = a.pool
+        if a.quantized is not None:
+            kw["bvh_quantized"] = a.quantized
         r.set_tuning(**kw)
+    node_b = QNODE_B if r.get_tuning()["bvh_quantized"] & 1 else NODE_B     # closest-hit node format
     r.InitializeRenderingBuffers(settings["width"], settings["height"])
     t0 = time.time()
     r.InitializeSceneBuffers(scene)
@@ -465,7 +472,7 @@ def main():
     ext = inst["extension_rays"]
     nodes_per_ray = inst["closest_nodes"] / max(1, ext)
     tris_per_ray = inst["closest_tris"] / max(1, ext)
-    bytes_per_ray = RAY_IN_B + HIT_OUT_B + NODE_B * nodes_per_ray + TRI_B * tris_per_ray
+    bytes_per_ray = RAY_IN_B + HIT_OUT_B + node_b * nodes_per_ray + TRI_B * tris_per_ray
     launches = max(1, prof["closest_launches"])
     bytes_per_launch = bytes_per_ray * prof["extension_rays"] / launches
     avg_launch_ms = prof["closest_ms"] / launches

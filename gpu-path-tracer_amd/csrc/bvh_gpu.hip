@@ -462,4 +462,18 @@ hipError_t gpu_build_bvh4(hipStream_t s, const float* d_positions, const uint32_
     return hipSuccess;
 }
 
+__global__ void k_quantize_bvh4(const Bvh4Node* nodes, uint32_t n, Bvh4QNode* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Bvh4QNode q;
+    quantize_bvh4_node(nodes[i], q);
+    out[i] = q;
+}
+
+hipError_t gpu_quantize_bvh4(hipStream_t s, const Bvh4Node* d_nodes, uint32_t n, Bvh4QNode* d_out) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_quantize_bvh4, dim3((n + 255) / 256), dim3(256), 0, s, d_nodes, n, d_out);
+    return hipGetLastError();
+}
+
 }  // namespace pt

@@ -31,4 +31,8 @@ hipError_t gpu_build_bvh4(hipStream_t s, const float* d_positions, const uint32_
                           const MaterialRecord* d_materials, uint32_t n, uint32_t max_leaf, Bvh4Node* d_nodes,
                           uint32_t node_capacity, TriRecord* d_tris, uint32_t* d_order, GpuBvhStats& stats);
 
+// Quantized copies of n 4-wide nodes (the same bytes as quantize_bvh4_node on
+// the host); asynchronous on stream s.
+hipError_t gpu_quantize_bvh4(hipStream_t s, const Bvh4Node* d_nodes, uint32_t n, Bvh4QNode* d_out);
+
 }  // namespace pt

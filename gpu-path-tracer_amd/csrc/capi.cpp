@@ -94,6 +94,18 @@ pt_status pt_frame_copy_hdr(pt_renderer* r, float* rgba, size_t pixel_count) {
     return guarded(r, [&](pt::Renderer& x) { return x.CopyFrameHdr(rgba, pixel_count); });
 }
 
+pt_status pt_get_bvh4_quantized(pt_renderer* r, void* nodes, uint32_t node_capacity) {
+    return guarded(r, [&](pt::Renderer& x) { return x.GetBvh4Quantized(nodes, node_capacity); });
+}
+
+pt_status pt_quantize_bvh4(const void* nodes, uint32_t count, void* quantized) {
+    if (count && (!nodes || !quantized)) return PT_ERROR_INVALID_ARGUMENT;
+    const pt::Bvh4Node* in = static_cast<const pt::Bvh4Node*>(nodes);
+    pt::Bvh4QNode* out = static_cast<pt::Bvh4QNode*>(quantized);
+    for (uint32_t i = 0; i < count; i++) pt::quantize_bvh4_node(in[i], out[i]);
+    return PT_OK;
+}
+
 pt_status pt_save_frame(pt_renderer* r, const char* path) {
     return guarded(r, [&](pt::Renderer& x) { return x.SaveFrame(path); });
 }

@@ -178,17 +178,55 @@ __device__ __forceinline__ void box4(const Bvh4Node* node, const RayFrame& r, fl
     }
 }
 
-// Entry distances and references of the children of a node of width W.
-template <int W>
+// Node kinds of the traversal kernels: fp32 BVH4 (128 B), fp32 BVH2 (64 B) and
+// quantized BVH4 (64 B); `arity` children per node.
+constexpr int kNodeBvh2 = 2;
+constexpr int kNodeBvh4 = 4;
+constexpr int kNodeBvh4Q = 5;
+template <int K>
 struct NodeOf;
 template <>
-struct NodeOf<4> {
+struct NodeOf<kNodeBvh4> {
     using type = Bvh4Node;
+    static constexpr int arity = 4;
 };
 template <>
-struct NodeOf<2> {
+struct NodeOf<kNodeBvh2> {
     using type = Bvh2Node;
+    static constexpr int arity = 2;
 };
+template <>
+struct NodeOf<kNodeBvh4Q> {
+    using type = Bvh4QNode;
+    static constexpr int arity = 4;
+};
+
+// Quantized node: decode the four child boxes (one 64-byte line: origin and
+// scale exponents, lo rows, hi rows, references) and run the same slab test.
+__device__ __forceinline__ void children(const Bvh4QNode* node, const RayFrame& r, float t0[4], uint32_t c[4]) {
+    const uint4* p = reinterpret_cast<const uint4*>(node);
+    const uint4 a = p[0], b = p[1], d = p[2], e = p[3];
+    const float ox = __uint_as_float(a.x), oy = __uint_as_float(a.y), oz = __uint_as_float(a.z);
+    const float sx = qnode_scale(a.w & 0xFFu), sy = qnode_scale((a.w >> 8) & 0xFFu), sz = qnode_scale((a.w >> 16) & 0xFFu);
+    // rows: lo x, y, z = b.x, b.y, b.z; hi x, y, z = b.w, d.x, d.y
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint32_t sh = 8u * (uint32_t)i;
+        const float lx = qnode_decode(ox, (b.x >> sh) & 0xFFu, sx), hx = qnode_decode(ox, (b.w >> sh) & 0xFFu, sx);
+        const float ly = qnode_decode(oy, (b.y >> sh) & 0xFFu, sy), hy = qnode_decode(oy, (d.x >> sh) & 0xFFu, sy);
+        const float lz = qnode_decode(oz, (b.z >> sh) & 0xFFu, sz), hz = qnode_decode(oz, (d.y >> sh) & 0xFFu, sz);
+        const float ax = fmaf(lx, r.rd.x, -r.ord.x), bx = fmaf(hx, r.rd.x, -r.ord.x);
+        const float ay = fmaf(ly, r.rd.y, -r.ord.y), by = fmaf(hy, r.rd.y, -r.ord.y);
+        const float az = fmaf(lz, r.rd.z, -r.ord.z), bz = fmaf(hz, r.rd.z, -r.ord.z);
+        const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), r.tmin));
+        const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), r.tmax));
+        t0[i] = tn <= tf ? tn : __builtin_inff();
+    }
+    c[0] = d.z;
+    c[1] = d.w;
+    c[2] = e.x;
+    c[3] = e.y;
+}
 
 __device__ __forceinline__ void children(const Bvh4Node* node, const RayFrame& r, float t0[4], uint32_t c[4]) {
     box4(node, r, r.tmax, t0);
@@ -239,10 +277,10 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32
 // global area past `LDS` entries.  Each entry is (node ref, entry distance) so
 // popped subtrees that lie beyond the current hit are skipped without a fetch.
 
-template <int BLOCK, int LDS, int W, bool ANY, bool COUNT, bool MULTI, class Src>
+template <int BLOCK, int LDS, int NODE, bool ANY, bool COUNT, bool MULTI, class Src>
 __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev& sc, const SceneDev* scenes,
                                                  const TraceConfig& cfg) {
-    using Node = typename NodeOf<W>::type;
+    using Node = typename NodeOf<NODE>::type;
     __shared__ uint2 stack[LDS][BLOCK];
     const uint32_t tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
@@ -266,7 +304,8 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
     uint32_t chunk_next = min(wave_id * chunk, n), chunk_end = min(wave_id * chunk + chunk, n);
     bool exhausted = chunk_next >= n;
     uint32_t item = 0, ref = 0, sp = 0;
-    const Node* nodes_base = static_cast<const Node*>(sc.nodes);   // per lane when MULTI: the ray's scene
+    // per lane when MULTI: the ray's scene
+    const Node* nodes_base = static_cast<const Node*>(NODE == kNodeBvh4Q ? sc.qnodes : sc.nodes);
     const TriRecord* tris_base = sc.tris;
     RayFrame r;
     float best_t = 0.0f, best_u = 0.0f, best_v = 0.0f;
@@ -341,7 +380,8 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                         uint32_t scene = 0;
                         item = src.load(chunk_next + lane_rank, o, d, tmin, tmax, scene);
                         if (MULTI) {
-                            nodes_base = static_cast<const Node*>(scenes[scene].nodes);
+                            nodes_base = static_cast<const Node*>(NODE == kNodeBvh4Q ? scenes[scene].qnodes
+                                                                                     : scenes[scene].nodes);
                             tris_base = scenes[scene].tris;
                         }
                         setup_ray(r, o, d, tmin, tmax);
@@ -367,6 +407,7 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
             const bool step = active && !fin && !ref_is_leaf(ref);
             if (__ballot(step) == 0) break;
             if (step) {
+                constexpr int W = NodeOf<NODE>::arity;
                 float t0[W];
                 uint32_t c[W];
                 children(nodes_base + ref, r, t0, c);
@@ -1482,26 +1523,28 @@ hipError_t dispatch_block(hipStream_t s, const Src& src, const SceneDev& sc, con
 template <bool ANY, bool COUNT, bool MULTI, class Src>
 hipError_t dispatch_trace(hipStream_t s, const Src& src, const SceneDev& sc, const SceneDev* scenes,
                           const TraceConfig& cfg) {
-    if (sc.node_width == 4) return dispatch_block<4, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
-    if (sc.node_width == 2) return dispatch_block<2, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
+    if (sc.node_width == 4 && (sc.node_quantized & (ANY ? 2u : 1u)))
+        return dispatch_block<kNodeBvh4Q, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
+    if (sc.node_width == 4) return dispatch_block<kNodeBvh4, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
+    if (sc.node_width == 2) return dispatch_block<kNodeBvh2, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
     return hipErrorInvalidValue;
 }
 
-template <int BLOCK, int LDS, bool ANY, bool COUNT, class Src>
+template <int BLOCK, int LDS, int NODE, bool ANY, bool COUNT, class Src>
 int occ_one() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<BLOCK, LDS, 4, ANY, COUNT, false, Src>, BLOCK, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<BLOCK, LDS, NODE, ANY, COUNT, false, Src>, BLOCK, 0) !=
         hipSuccess)
         return 0;
     return nb;
 }
 
-template <bool ANY, bool COUNT, class Src>
+template <int NODE, bool ANY, bool COUNT, class Src>
 int occ(const TraceConfig& cfg) {
-    if (cfg.block == 64 && cfg.lds_stack == 16) return occ_one<64, 16, ANY, COUNT, Src>();
-    if (cfg.block == 128 && cfg.lds_stack == 16) return occ_one<128, 16, ANY, COUNT, Src>();
-    if (cfg.block == 256 && cfg.lds_stack == 8) return occ_one<256, 8, ANY, COUNT, Src>();
-    if (cfg.block == 512 && cfg.lds_stack == 8) return occ_one<512, 8, ANY, COUNT, Src>();
+    if (cfg.block == 64 && cfg.lds_stack == 16) return occ_one<64, 16, NODE, ANY, COUNT, Src>();
+    if (cfg.block == 128 && cfg.lds_stack == 16) return occ_one<128, 16, NODE, ANY, COUNT, Src>();
+    if (cfg.block == 256 && cfg.lds_stack == 8) return occ_one<256, 8, NODE, ANY, COUNT, Src>();
+    if (cfg.block == 512 && cfg.lds_stack == 8) return occ_one<512, 8, NODE, ANY, COUNT, Src>();
     return 0;
 }
 
@@ -1512,9 +1555,19 @@ bool trace_config_supported(uint32_t block, uint32_t lds_stack) {
            (block == 512 && lds_stack == 8);
 }
 
-int trace_occupancy(const TraceConfig& cfg, bool any, bool count) {
-    if (any) return count ? occ<true, true, WavefrontShadowSrc>(cfg) : occ<true, false, WavefrontShadowSrc>(cfg);
-    return count ? occ<false, true, WavefrontClosestSrc>(cfg) : occ<false, false, WavefrontClosestSrc>(cfg);
+int trace_occupancy(const TraceConfig& cfg, bool any, bool count, bool quantized) {
+    if (quantized) {
+        if (any)
+            return count ? occ<kNodeBvh4Q, true, true, WavefrontShadowSrc>(cfg)
+                         : occ<kNodeBvh4Q, true, false, WavefrontShadowSrc>(cfg);
+        return count ? occ<kNodeBvh4Q, false, true, WavefrontClosestSrc>(cfg)
+                     : occ<kNodeBvh4Q, false, false, WavefrontClosestSrc>(cfg);
+    }
+    if (any)
+        return count ? occ<kNodeBvh4, true, true, WavefrontShadowSrc>(cfg)
+                     : occ<kNodeBvh4, true, false, WavefrontShadowSrc>(cfg);
+    return count ? occ<kNodeBvh4, false, true, WavefrontClosestSrc>(cfg)
+                 : occ<kNodeBvh4, false, false, WavefrontClosestSrc>(cfg);
 }
 
 hipError_t launch_frame_begin(hipStream_t s, const PathState& p) {

@@ -15,6 +15,7 @@ constexpr int kQueueSlots = 5;
 
 struct SceneDev {
     const void* nodes;              // Bvh4Node[] or Bvh2Node[] by node_width
+    const void* qnodes;             // Bvh4QNode[]: the same 4-wide tree quantized, or null
     const TriRecord* tris;          // BVH order
     const ShadeRecord* shade;       // original order
     const MaterialRecord* materials;
@@ -23,7 +24,8 @@ struct SceneDev {
     uint32_t light_count;
     uint32_t tri_count;
     uint32_t node_width;            // 4 or 2
-    uint32_t pad;
+    uint32_t node_quantized;        // traversals that use qnodes: bit 0 closest hit, bit 1 any hit
+    uint32_t pad[2];
 };
 
 // Device state of the guide network and of its training records.
@@ -81,7 +83,7 @@ struct TraceConfig {
 };
 
 // Blocks of `block` threads that fit on one CU for the traversal kernels.
-int trace_occupancy(const TraceConfig& cfg, bool any, bool count);
+int trace_occupancy(const TraceConfig& cfg, bool any, bool count, bool quantized);
 bool trace_config_supported(uint32_t block, uint32_t lds_stack);
 
 hipError_t launch_frame_begin(hipStream_t s, const PathState& p);

@@ -77,6 +77,90 @@ struct alignas(64) Bvh2Node {
 };
 static_assert(sizeof(Bvh2Node) == 64, "BVH2 node is 64 bytes");
 
+// Quantized BVH4 node: 64 bytes, the same four references, child boxes as
+// 8-bit offsets from the node's own box corner.  Axis a of child i decodes to
+//   lo = origin[a] + float(lo byte i of row a) * 2^(e_a - 127)
+// (likewise hi), in float arithmetic without contraction; the quantizer picks
+// each byte so that the decoded box contains the child's fp32 box bit for bit,
+// so every ray that enters the fp32 box enters the decoded one.
+struct alignas(64) Bvh4QNode {
+    float origin[3];
+    uint32_t scale;      // bytes 0..2: float exponent field e_a of axis a's scale
+    uint32_t lo[3];      // row per axis, byte i = child i
+    uint32_t hi[3];
+    uint32_t child[4];
+    uint32_t pad[2];
+};
+static_assert(sizeof(Bvh4QNode) == 64, "quantized BVH4 node is 64 bytes");
+
+PT_HD float qnode_decode(float origin, uint32_t q, float scale) { return origin + (float)q * scale; }
+PT_HD float qnode_scale(uint32_t e) { return __builtin_bit_cast(float, e << 23); }
+
+// Smallest scale exponent (1..254) whose byte 255 reaches `hi` from `origin`.
+PT_HD uint32_t qnode_exponent(float origin, float hi) {
+    auto reaches = [&](uint32_t e) { return qnode_decode(origin, 255u, qnode_scale(e)) >= hi; };
+    const float extent = hi - origin;
+    uint32_t e = 1;
+    if (extent > 0.0f) {
+        // 2^(e - 127) >= extent / 255 as a first guess, then the exact minimum
+        const uint32_t ebits = (__builtin_bit_cast(uint32_t, extent * (1.0f / 255.0f)) >> 23) & 0xFFu;
+        e = ebits < 1u ? 1u : (ebits > 253u ? 254u : ebits + 1u);
+        while (e > 1u && reaches(e - 1u)) e--;
+        while (e < 254u && !reaches(e)) e++;
+    }
+    return e;
+}
+
+// Largest byte whose decoded value is <= lo, and smallest whose value is >= hi.
+PT_HD uint32_t qnode_floor(float origin, float scale, float lo) {
+    uint32_t q = 0;
+    for (uint32_t step = 128; step > 0; step >>= 1)
+        if (q + step <= 255u && qnode_decode(origin, q + step, scale) <= lo) q += step;
+    return q;
+}
+PT_HD uint32_t qnode_ceil(float origin, float scale, float hi) {
+    uint32_t q = 255;
+    for (uint32_t step = 128; step > 0; step >>= 1)
+        if (q >= step && qnode_decode(origin, q - step, scale) >= hi) q -= step;
+    return q;
+}
+
+// fp32 BVH4 node -> quantized node (host and device give identical bytes).
+// Empty child slots (reference 0xFFFFFFFF) keep lo 255, hi 0.
+PT_HD void quantize_bvh4_node(const Bvh4Node& n, Bvh4QNode& q) {
+    const float* lo[3] = {n.lox, n.loy, n.loz};
+    const float* hi[3] = {n.hix, n.hiy, n.hiz};
+    q.scale = 0u;
+    q.pad[0] = q.pad[1] = 0u;
+    for (int a = 0; a < 3; a++) {
+        float mn = 0.0f, mx = 0.0f;
+        bool any = false;
+        for (int i = 0; i < 4; i++) {
+            if (n.child[i] == 0xFFFFFFFFu) continue;
+            mn = any ? (lo[a][i] < mn ? lo[a][i] : mn) : lo[a][i];
+            mx = any ? (hi[a][i] > mx ? hi[a][i] : mx) : hi[a][i];
+            any = true;
+        }
+        const uint32_t e = qnode_exponent(mn, mx);
+        const float s = qnode_scale(e);
+        q.origin[a] = mn;
+        q.scale |= e << (8 * a);
+        uint32_t rl = 0, rh = 0;
+        for (int i = 0; i < 4; i++) {
+            uint32_t bl = 255u, bh = 0u;
+            if (n.child[i] != 0xFFFFFFFFu) {
+                bl = qnode_floor(mn, s, lo[a][i]);
+                bh = qnode_ceil(mn, s, hi[a][i]);
+            }
+            rl |= bl << (8 * i);
+            rh |= bh << (8 * i);
+        }
+        q.lo[a] = rl;
+        q.hi[a] = rh;
+    }
+    for (int i = 0; i < 4; i++) q.child[i] = n.child[i];
+}
+
 constexpr uint32_t kLeafBit = 0x80000000u;
 PT_HD bool ref_is_leaf(uint32_t r) { return (r & kLeafBit) != 0u; }
 PT_HD uint32_t leaf_first(uint32_t r) { return (r & 0x7FFFFFFFu) >> 4; }

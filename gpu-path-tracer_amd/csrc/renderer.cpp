@@ -49,6 +49,7 @@ Renderer::Renderer() {
     tuning_.trace_refill = 16;
     tuning_.bvh_width = 4;
     tuning_.bvh_builder = PT_BVH_BUILDER_HOST;
+    tuning_.bvh_quantized = 2;          // shadow rays on 64-byte quantized nodes
 }
 
 Renderer::~Renderer() {
@@ -135,6 +136,7 @@ PathState Renderer::path_state(const LoopMode* mode) const {
 SceneDev ResidentScene::dev() const {
     SceneDev s{};
     s.nodes = nodes.as<void>();
+    s.qnodes = quantized ? qnodes.as<void>() : nullptr;
     s.tris = tris.as<TriRecord>();
     s.shade = shade.as<ShadeRecord>();
     s.materials = materials.as<MaterialRecord>();
@@ -143,6 +145,7 @@ SceneDev ResidentScene::dev() const {
     s.light_count = (uint32_t)host.lights.size();
     s.tri_count = host.tri_count;
     s.node_width = host.bvh.width;
+    s.node_quantized = quantized;
     return s;
 }
 
@@ -210,9 +213,17 @@ pt_status Renderer::configure_trace() {
     cfg.refill = tuning_.trace_refill;
     if (!trace_config_supported(cfg.block, cfg.lds_stack))
         return fail(PT_ERROR_INVALID_ARGUMENT, "unsupported traversal block size " + std::to_string(cfg.block));
-    int occ_closest = trace_occupancy(cfg, false, false);
-    int occ_shadow = trace_occupancy(cfg, true, false);
-    int occ = std::max(1, std::min(occ_closest, occ_shadow));
+    // the smallest occupancy over the kernels the resident scenes use
+    int occ = 1 << 20;
+    bool any_scene = false;
+    for (const auto& sc : scenes_) {
+        if (!sc) continue;
+        any_scene = true;
+        occ = std::min({occ, trace_occupancy(cfg, false, false, (sc->quantized & 1u) != 0),
+                        trace_occupancy(cfg, true, false, (sc->quantized & 2u) != 0)});
+    }
+    if (!any_scene) occ = std::min(trace_occupancy(cfg, false, false, false), trace_occupancy(cfg, true, false, false));
+    occ = std::max(1, occ);
     if (tuning_.trace_blocks_per_cu) occ = (int)tuning_.trace_blocks_per_cu;
     cfg.blocks = (uint32_t)(occ * std::max(1, cu_count_));
     // worst-case stack over every resident scene: a level of a width-W BVH
@@ -293,6 +304,21 @@ pt_status Renderer::upload_scene(uint32_t id, const pt_scene_desc& desc) {
         PT_TRY_HIP(hipStreamSynchronize(stream_), "BVH nodes");
     } else {
         PT_TRY(upload(rs->nodes, h.bvh.nodes.data(), h.bvh.nodes.size() * sizeof(Bvh4Node), "BVH nodes"));
+    }
+    if (tuning_.bvh_quantized && h.bvh.width == 4) {
+        // 64-byte quantized copy of the 4-wide nodes for traversal
+        if (gpu_bvh) {
+            const uint32_t nn = h.bvh.device_nodes;
+            PT_TRY_HIP(rs->qnodes.allocate((size_t)nn * sizeof(Bvh4QNode)), "quantized BVH nodes");
+            PT_TRY_HIP(gpu_quantize_bvh4(stream_, rs->nodes.as<Bvh4Node>(), nn, rs->qnodes.as<Bvh4QNode>()),
+                       "BVH node quantization");
+        } else {
+            std::vector<Bvh4QNode> q(h.bvh.nodes.size());
+            for (size_t i = 0; i < q.size(); i++) quantize_bvh4_node(h.bvh.nodes[i], q[i]);
+            PT_TRY(upload(rs->qnodes, q.data(), q.size() * sizeof(Bvh4QNode), "quantized BVH nodes"));
+            PT_TRY_HIP(hipStreamSynchronize(stream_), "quantized BVH nodes");
+        }
+        rs->quantized = tuning_.bvh_quantized;
     }
     if (!gpu_bvh) {
         PT_TRY(upload(rs->tris, h.tris.data(), h.tris.size() * sizeof(TriRecord), "triangle records"));
@@ -448,8 +474,9 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
     mode.shade_multi = n > 1;
     for (const ViewRecord& v : views) {
         mode.trace_multi = mode.trace_multi || v.scene != mode.scene;
-        if (scenes_[v.scene]->host.bvh.width != scenes_[mode.scene]->host.bvh.width)
-            return fail(PT_ERROR_INVALID_ARGUMENT, "the scenes of a batch must share one BVH node width");
+        if (scenes_[v.scene]->host.bvh.width != scenes_[mode.scene]->host.bvh.width ||
+            scenes_[v.scene]->quantized != scenes_[mode.scene]->quantized)
+            return fail(PT_ERROR_INVALID_ARGUMENT, "the scenes of a batch must share one BVH node format");
     }
     mode.guided = opt.guided;
     mode.guide_fraction = opt.guided ? opt.guide_fraction : 0.0f;
@@ -1003,6 +1030,18 @@ pt_status Renderer::GetBvh4(void* nodes, uint32_t node_cap, uint32_t* order, uin
     return PT_OK;
 }
 
+pt_status Renderer::GetBvh4Quantized(void* nodes, uint32_t node_cap) {
+    if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    const ResidentScene& rs = *scenes_[0];
+    if (!rs.quantized) return fail(PT_ERROR_INVALID_STATE, "scene 0 was uploaded without quantized nodes");
+    const size_t nn = rs.host.bvh.device_nodes ? rs.host.bvh.device_nodes : rs.host.bvh.nodes.size();
+    if (!nodes || node_cap < nn) return fail(PT_ERROR_INVALID_ARGUMENT, "node capacity too small");
+    PT_TRY_HIP(hipMemcpyAsync(nodes, rs.qnodes.as<void>(), nn * sizeof(Bvh4QNode), hipMemcpyDeviceToHost, stream_),
+               "quantized BVH nodes");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "BVH read-back");
+    return PT_OK;
+}
+
 pt_status Renderer::GetTuning(pt_tuning& t) const {
     t = tuning_;
     return PT_OK;
@@ -1017,6 +1056,7 @@ pt_status Renderer::SetTuning(const pt_tuning& t) {
         return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_builder must be PT_BVH_BUILDER_HOST or PT_BVH_BUILDER_GPU");
     if (t.bvh_builder == PT_BVH_BUILDER_GPU && t.bvh_width != 4)
         return fail(PT_ERROR_INVALID_ARGUMENT, "the GPU BVH builder emits 4-wide nodes");
+    if (t.bvh_quantized > 3) return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_quantized must be in [0, 3]");
     if (t.shade_block != 256) return fail(PT_ERROR_INVALID_ARGUMENT, "shade_block must be 256");
     if (t.pool_size < 1024 || t.pool_size > (1u << 26)) return fail(PT_ERROR_INVALID_ARGUMENT, "pool_size must be in [1024, 2^26]");
     if (t.trace_refill < 1 || t.trace_refill > 64) return fail(PT_ERROR_INVALID_ARGUMENT, "trace_refill must be in [1, 64]");

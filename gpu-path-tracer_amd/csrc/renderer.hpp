@@ -51,6 +51,8 @@ struct ResidentScene {
     HostScene host;
     DeviceBuffer nodes, tris, shade, materials, lights, bins;
     DeviceBuffer order;        // GPU-built BVH: original triangle of each BVH slot (the host copy is empty)
+    DeviceBuffer qnodes;       // quantized copy of the 4-wide nodes
+    uint32_t quantized = 0;    // traversals that use qnodes: bit 0 closest hit, bit 1 any hit
     SceneDev dev() const;
 };
 
@@ -115,6 +117,7 @@ public:
     pt_status GetAliasTable(pt_alias_bin* bins, uint32_t cap, uint32_t* count);
     pt_status GetBvhInfo(pt_bvh_info& info);
     pt_status GetBvh4(void* nodes, uint32_t node_cap, uint32_t* order, uint32_t order_cap);
+    pt_status GetBvh4Quantized(void* nodes, uint32_t node_cap);
     pt_status GetTuning(pt_tuning& t) const;
     pt_status SetTuning(const pt_tuning& t);
 

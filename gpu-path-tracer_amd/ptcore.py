@@ -124,7 +124,7 @@ class Tuning(C.Structure):
     _fields_ = [("trace_block", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("shade_block", C.c_uint32),
                 ("pool_size", C.c_uint32), ("use_graph", C.c_uint32), ("bvh_max_leaf", C.c_uint32),
                 ("bvh_traversal_cost", C.c_float), ("trace_refill", C.c_uint32), ("bvh_width", C.c_uint32),
-                ("bvh_builder", C.c_uint32)]
+                ("bvh_builder", C.c_uint32), ("bvh_quantized", C.c_uint32)]
 
 
 _P = C.c_void_p
@@ -186,6 +186,8 @@ SIGNATURES = {
     "pt_generate_tangents": (C.c_int, [_P, _P, _P, C.c_uint32, _P, C.c_uint32, _P]),
     "pt_write_image": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_int]),
     "pt_save_frame": (C.c_int, [_P, C.c_char_p]),
+    "pt_quantize_bvh4": (C.c_int, [_P, C.c_uint32, _P]),
+    "pt_get_bvh4_quantized": (C.c_int, [_P, _P, C.c_uint32]),
     "pt_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "pt_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
 }
@@ -539,6 +541,13 @@ class Renderer:
         self._check(self.lib.pt_get_bvh4(self.h, _ptr(nodes), info["node_count"], _ptr(order), len(order)))
         return nodes, order
 
+    def bvh4_quantized(self):
+        """(N, 16) uint32 rows of the 64-byte quantized nodes scene 0 is traversed with."""
+        n = self.bvh_info()["node_count"]
+        out = np.zeros((n, 16), np.uint32)
+        self._check(self.lib.pt_get_bvh4_quantized(self.h, _ptr(out), n))
+        return out
+
     def get_tuning(self):
         t = Tuning()
         self._check(self.lib.pt_get_tuning(self.h, C.byref(t)))
@@ -566,6 +575,18 @@ def build_bvh4(positions, max_leaf_size=0):
     if st != PT_OK:
         raise PtError(st, "pt_build_bvh4 failed")
     return nodes, order, depth.value
+
+
+def quantize_bvh4(nodes):
+    """pt_quantize_bvh4: (N, 32) uint32 rows of 128-byte BVH4 nodes -> (N, 16)
+    uint32 rows of 64-byte quantized nodes."""
+    lib = load_library()
+    a = np.ascontiguousarray(nodes, np.uint32).reshape(-1, 32)
+    out = np.zeros((len(a), 16), np.uint32)
+    st = lib.pt_quantize_bvh4(_ptr(a), len(a), _ptr(out))
+    if st != PT_OK:
+        raise PtError(st, "pt_quantize_bvh4 failed")
+    return out
 
 
 def import_obj(path):

@@ -225,6 +225,9 @@ typedef struct pt_tuning {
     uint32_t trace_refill;       /* idle lanes of a wave that trigger fetching new rays (1..64) */
     uint32_t bvh_width;          /* children per BVH node: 4 (128-byte nodes) or 2 (64-byte); next upload */
     uint32_t bvh_builder;        /* PT_BVH_BUILDER_*: where the next uploads build their BVH */
+    uint32_t bvh_quantized;      /* traversals on 64-byte nodes whose child boxes are 8-bit offsets
+                                    (conservative, results unchanged): bit 0 closest hit, bit 1 any
+                                    hit; default 2; 4-wide trees only; next upload */
 } pt_tuning;
 
 /* BVH builders: the host's binned-SAH build (best trees), or a linear BVH over
@@ -453,6 +456,10 @@ pt_status pt_get_bvh_info(pt_renderer* renderer, pt_bvh_info* info);
 pt_status pt_get_bvh4(pt_renderer* renderer, void* nodes, uint32_t node_capacity,
                       uint32_t* prim_order, uint32_t prim_capacity);
 
+/* The quantized nodes of scene 0 (pt_tuning.bvh_quantized != 0 at upload), 64 B each,
+ * as uploaded (node_count of pt_get_bvh_info); PT_ERROR_INVALID_STATE without them. */
+pt_status pt_get_bvh4_quantized(pt_renderer* renderer, void* nodes, uint32_t node_capacity);
+
 /* Host-only BVH build of the core (no GPU needed): the 128-byte BVH4 nodes and
  * BVH-order -> original prim permutation that InitializeSceneBuffers uploads.
  * Replaces HostBvh + DeviceBvh::BuildFlattenSync (HostBvh.cpp:51-100, DeviceBvh.cpp:7-52).
@@ -460,6 +467,12 @@ pt_status pt_get_bvh4(pt_renderer* renderer, void* nodes, uint32_t node_capacity
 pt_status pt_build_bvh4(const float* positions, uint32_t triangle_count, uint32_t max_leaf_size,
                         void* nodes, uint32_t node_capacity, uint32_t* node_count,
                         uint32_t* prim_order, uint32_t* depth);
+
+/* The 64-byte quantized form of `count` BVH4 nodes of pt_build_bvh4 (the nodes
+ * traversed when pt_tuning.bvh_quantized is set; no reference counterpart):
+ * per node the box corner (3 floats), the three scale exponents, 8-bit child
+ * bounds and the same child references.  Host only. */
+pt_status pt_quantize_bvh4(const void* nodes, uint32_t count, void* quantized);
 
 /* ---- scene ingestion (ObjLoader.cpp:67-234, Manager.cpp:311-412) ----
  * Reads a Wavefront OBJ and the MTL libraries it names (next to it) into a

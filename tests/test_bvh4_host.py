@@ -65,3 +65,59 @@ def test_bvh4_rejects_non_finite():
     tri = np.array([[[0, 0, 0], [np.nan, 0, 0], [0, 1, 0]]], np.float32)
     with pytest.raises(ptcore.PtError):
         ptcore.build_bvh4(tri)
+
+
+def _decode(q):
+    """(N, 16) uint32 quantized nodes -> lo, hi (N, 4, 3) float32 and child refs,
+    in float32 arithmetic: origin + float(byte) * 2^(e - 127)."""
+    origin = q[:, 0:3].view(np.float32)
+    e = np.stack([(q[:, 3] >> (8 * a)) & 255 for a in range(3)], 1)
+    scale = (e.astype(np.uint32) << 23).view(np.float32)
+    byte = lambda rows: np.stack([(rows >> (8 * i)) & 255 for i in range(4)], 1).astype(np.float32)   # (N, 4, 3)
+    lo = origin[:, None, :] + byte(q[:, 4:7]) * scale[:, None, :]
+    hi = origin[:, None, :] + byte(q[:, 7:10]) * scale[:, None, :]
+    return lo.astype(np.float32), hi.astype(np.float32), scale, q[:, 10:14]
+
+
+@pytest.mark.parametrize("make", [
+    lambda: scenes.spheres_scene(count=6, subdivisions=4, lights=64, seed=3),
+    scenes.cornell_box,
+    # tiny triangles far from the origin: scale steps below the coordinates' ulp
+    lambda: scenes.Scene(name="far", positions=(np.random.default_rng(4).uniform(-1e-3, 1e-3, (500, 3, 3)) +
+                                                np.float32([1e4, -3e4, 7e3])).astype(np.float32),
+                         normals=np.zeros((500, 3, 3), np.float32), tangents=np.zeros((500, 3, 4), np.float32),
+                         material_ids=np.zeros(500, np.uint32), materials=np.stack([scenes.material(1)])),
+])
+def test_quantized_nodes_contain_the_fp32_boxes(make):
+    """Every decoded child box contains the fp32 child box; each byte is the
+    tightest one (one step further would cut into the box) and each scale the
+    smallest power of two whose byte 255 reaches the node's upper corner."""
+    s = make()
+    nodes, _, _ = ptcore.build_bvh4(s.positions)
+    q = ptcore.quantize_bvh4(nodes)
+    lo, hi, scale, child = _decode(q)
+    assert np.array_equal(child, nodes[:, 24:28])
+    f = nodes[:, :24].view(np.float32)
+    flo = np.stack([f[:, 0:4], f[:, 8:12], f[:, 16:20]], 2)        # (N, 4, 3)
+    fhi = np.stack([f[:, 4:8], f[:, 12:16], f[:, 20:24]], 2)
+    valid = (nodes[:, 24:28] != 0xFFFFFFFF)[:, :, None] & np.ones((1, 1, 3), bool)
+    assert (lo[valid] <= flo[valid]).all() and (hi[valid] >= fhi[valid]).all()
+    # tightness: the next byte up (down) decodes past the fp32 bound
+    origin = q[:, 0:3].view(np.float32)[:, None, :]
+    bl = np.stack([(q[:, 4:7] >> (8 * i)) & 255 for i in range(4)], 1)
+    bh = np.stack([(q[:, 7:10] >> (8 * i)) & 255 for i in range(4)], 1)
+    s3 = scale[:, None, :]
+    up = origin + np.minimum(bl + 1, 255).astype(np.float32) * s3
+    down = origin + np.maximum(bh.astype(np.int64) - 1, 0).astype(np.float32) * s3
+    assert ((up > flo) | (bl == 255))[valid].all()
+    assert ((down < fhi) | (bh == 0))[valid].all()
+    # scale: byte 255 reaches the node's corner, half the scale would not
+    node_hi = np.where(valid, fhi, -np.inf).max(1)
+    o2 = q[:, 0:3].view(np.float32)
+    assert (o2 + np.float32(255) * scale >= node_hi).all()
+    e = (scale.view(np.uint32) >> 23)
+    half = ((e - 1).astype(np.uint32) << 23).view(np.float32)
+    assert ((o2 + np.float32(255) * half < node_hi) | (e == 1)).all()
+    # the byte grid is fine: decoded boxes exceed the fp32 ones by at most one step (plus rounding)
+    slack = np.maximum(s3, np.spacing(np.abs(origin)) * 2)
+    assert ((flo - lo)[valid] <= (slack + 0 * lo)[valid] * 1.0001).all()

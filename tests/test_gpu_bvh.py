@@ -42,6 +42,18 @@ def test_gpu_bvh_cornell_structure_and_hits(gpu_builder, cornell, cornell_oracle
     assert np.array_equal(r.trace_any(sh), cornell_oracle.any(sh, brute=True))
 
 
+def test_gpu_quantized_nodes_equal_host_quantizer(gpu_builder, cornell):
+    """Nodes quantized on the GPU hold the bytes the host quantizer gives for
+    the same fp32 nodes: the traversal sees one format whichever side made it."""
+    r = gpu_builder
+    r.set_tuning(bvh_quantized=3)
+    r.InitializeRenderingBuffers(32, 32)
+    r.InitializeSceneBuffers(cornell)
+    nodes, _ = r.bvh4()
+    q_dev = r.bvh4_quantized()
+    assert np.array_equal(q_dev, ptcore.quantize_bvh4(nodes))
+
+
 @pytest.mark.parametrize("kind", ["one", "two", "same"])
 def test_gpu_bvh_tiny_and_identical(gpu_builder, oracle_lib, kind):
     """One triangle, two, and 300 copies of one triangle (equal Morton codes)."""
@@ -86,6 +98,11 @@ def test_gpu_bvh_production_scene_equals_host_tree(gpu_renderer):
         assert np.array_equal(r.trace_any(sh), host_occ)
         stats = r.RenderFrame(st, s.view)
         img = r.GetFrameHdr(96, 64)
+        # the GPU-built tree traversed as quantized 64-byte nodes (quantized on the GPU)
+        r.set_tuning(bvh_builder=1, bvh_quantized=3)
+        r.InitializeSceneBuffers(s)
+        assert np.array_equal(r.trace_closest(rays), host_hits)
+        assert np.array_equal(r.trace_any(sh), host_occ)
     finally:
         r.set_tuning(**tun)
     assert stats["extension_rays"] == host_stats["extension_rays"]

@@ -1,7 +1,8 @@
 """The tuning space of the traversal: node width 2 (64-byte nodes) or 4
-(128-byte nodes) and block sizes 64..512.  Every point must return the same
-closest hits, bit for bit (the (t, prim) minimum does not depend on the tree),
-and the same images."""
+(128-byte nodes, or 64-byte quantized ones) and block sizes 64..512.  Every
+point must return the same closest hits, bit for bit (the (t, prim) minimum
+does not depend on the tree or on how conservative its boxes are), and the
+same images."""
 import numpy as np
 import pytest
 
@@ -10,7 +11,9 @@ import scenes
 
 pytestmark = pytest.mark.gpu
 
-POINTS = [(4, 256), (4, 512), (2, 64), (2, 256), (2, 512), (4, 64), (4, 128)]
+# (node width, block, bvh_quantized: bit 0 closest hit, bit 1 any hit on 64-byte quantized nodes)
+POINTS = [(4, 256, 0), (4, 512, 0), (2, 64, 2), (2, 256, 0), (2, 512, 2), (4, 64, 0), (4, 128, 2),
+          (4, 256, 3), (4, 64, 1), (4, 512, 3), (4, 256, 2)]
 
 
 @pytest.fixture(scope="module")
@@ -23,10 +26,10 @@ def brute(cornell_oracle, rays):
     return cornell_oracle.closest(rays, brute=True)
 
 
-@pytest.mark.parametrize("width,block", POINTS)
-def test_tuning_point_is_bit_exact(cornell, cornell_oracle, rays, brute, width, block):
+@pytest.mark.parametrize("width,block,quantized", POINTS)
+def test_tuning_point_is_bit_exact(cornell, cornell_oracle, rays, brute, width, block, quantized):
     with ptcore.Renderer() as r:
-        r.set_tuning(bvh_width=width, trace_block=block)
+        r.set_tuning(bvh_width=width, trace_block=block, bvh_quantized=quantized)
         r.InitializeRenderingBuffers(32, 32)
         r.InitializeSceneBuffers(cornell)
         info = r.bvh_info()
@@ -54,9 +57,9 @@ def test_width2_tree_is_a_binary_bvh(cornell):
 def test_images_equal_across_tuning_points(cornell):
     st = dict(width=32, height=32, samples_per_pixel=16, path_depth=8, use_nee=1, seed=5)
     imgs, rays = [], []
-    for width, block in POINTS[:3]:
+    for width, block, quantized in (POINTS[0], POINTS[2], POINTS[7]):
         with ptcore.Renderer() as r:
-            r.set_tuning(bvh_width=width, trace_block=block)
+            r.set_tuning(bvh_width=width, trace_block=block, bvh_quantized=quantized)
             r.InitializeRenderingBuffers(32, 32)
             r.InitializeSceneBuffers(cornell)
             s = r.RenderFrame(st, cornell.view)
@@ -68,7 +71,8 @@ def test_images_equal_across_tuning_points(cornell):
 
 
 def test_width2_production_scene_hits(oracle_lib):
-    """983,042 triangles with 64-byte nodes: hits equal the oracle's on a
+    """983,042 triangles with 64-byte nodes, 128-byte nodes and quantized
+    64-byte nodes: equal hits and occlusion, and equal to the oracle's on a
     seeded subset (ties between coplanar triangles aside)."""
     s = scenes.production_scene()
     rays = scenes.random_rays(1 << 20, (-10, -10, -10), (110, 110, 110), 302)
@@ -77,10 +81,17 @@ def test_width2_production_scene_hits(oracle_lib):
         r.InitializeRenderingBuffers(32, 32)
         r.InitializeSceneBuffers(s)
         hits = r.trace_closest(rays)
-        r.set_tuning(bvh_width=4)
+        r.set_tuning(bvh_width=4, bvh_quantized=0)
         r.InitializeSceneBuffers(s)
         hits4 = r.trace_closest(rays)
-    assert np.array_equal(hits, hits4)
+        sh = scenes.shadow_rays(s, 1 << 18, 304)
+        occ4 = r.trace_any(sh)
+        r.set_tuning(bvh_quantized=3)
+        r.InitializeSceneBuffers(s)
+        hits4q = r.trace_closest(rays)
+        occ4q = r.trace_any(sh)
+    assert np.array_equal(hits, hits4) and np.array_equal(hits4, hits4q)
+    assert np.array_equal(occ4, occ4q)
     o = oracle_lib.OracleScene(s)
     sub = np.random.default_rng(1).choice(len(rays), 16384, replace=False)
     ref = o.closest(rays[sub])
@@ -90,7 +101,7 @@ def test_width2_production_scene_hits(oracle_lib):
 
 
 def test_invalid_tuning_is_rejected(gpu_renderer):
-    for bad in (dict(bvh_width=3), dict(trace_block=1024), dict(trace_block=96)):
+    for bad in (dict(bvh_width=3), dict(trace_block=1024), dict(trace_block=96), dict(bvh_quantized=4)):
         with pytest.raises(ptcore.PtError) as e:
             gpu_renderer.set_tuning(**bad)
         assert e.value.status == 1
