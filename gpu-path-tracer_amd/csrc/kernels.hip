@@ -526,8 +526,9 @@ __device__ __forceinline__ void guide_credit(GuideRecord* recs, uint32_t head, V
 
 struct WavefrontShadowSrc {
     PathState p;
-    __device__ __forceinline__ uint32_t count() const { return p.ctr->shadow; }
-    __device__ __forceinline__ unsigned int* work() const { return &p.ctr->work_shadow; }
+    int cur;   // parity of the iteration whose shade pushed the rays
+    __device__ __forceinline__ uint32_t count() const { return p.ctr->shadow[cur]; }
+    __device__ __forceinline__ unsigned int* work() const { return &p.ctr->work_shadow[cur]; }
     __device__ __forceinline__ uint32_t load(uint32_t i, V3& o, V3& d, float& tmin, float& tmax, uint32_t& scene) const {
         const float4 a = p.sh_o[i], b = p.sh_d[i];
         o = xyz(a);
@@ -846,9 +847,9 @@ __global__ void k_frame_begin(PathState p) {
         c->ray[0] = gen;
         c->ray[1] = 0;
         for (int i = 0; i < kQueueSlots; i++) c->queue[i] = 0;
-        c->shadow = 0;
+        c->shadow[0] = c->shadow[1] = 0;
         c->work_closest = 0;
-        c->work_shadow = 0;
+        c->work_shadow[0] = c->work_shadow[1] = 0;
         c->active = gen;
         c->iterations = 0;
         c->invalid_pixels = 0;
@@ -952,15 +953,17 @@ __global__ void k_prepare(PathState p, int cur) {
         const uint32_t nr = (uint64_t)free_slots < remaining ? free_slots : (uint32_t)remaining;
         const uint32_t traced = c->ray[cur];
         c->ext_rays += traced;
-        c->shadow_rays += c->shadow;
+        c->shadow_rays += c->shadow[cur];
         if (traced) c->iterations++;
         c->ray[nxt] += nr;
         c->launched += nr;
         c->ray[cur] = 0;
         for (int i = 0; i < kQueueSlots; i++) c->queue[i] = 0;
-        c->shadow = 0;
+        // this iteration's shadow rays may still be in flight on another
+        // stream: clear the pair the next shade pushes into
+        c->shadow[nxt] = 0;
         c->work_closest = 0;
-        c->work_shadow = 0;
+        c->work_shadow[nxt] = 0;
         c->active = c->ray[nxt];
     }
 }
@@ -1399,9 +1402,10 @@ __global__ __launch_bounds__(GUIDED ? 64 * kInferWaves : 256) void k_shade(PathS
         }
         uint32_t ray_pos, sh_pos, rec_pos = 0;
         if (GUIDED)
-            block_reserve3(to_ray, so.emit, rec_valid, &c->ray[nxt], &c->shadow, &c->records, ray_pos, sh_pos, rec_pos);
+            block_reserve3(to_ray, so.emit, rec_valid, &c->ray[nxt], &c->shadow[cur], &c->records, ray_pos, sh_pos,
+                           rec_pos);
         else
-            block_reserve2(to_ray, so.emit, &c->ray[nxt], &c->shadow, ray_pos, sh_pos);
+            block_reserve2(to_ray, so.emit, &c->ray[nxt], &c->shadow[cur], ray_pos, sh_pos);
         if (GUIDED) {
             if (so.emit) p.guide.sh_head[sh_pos] = head;
             if (to_ray) {
@@ -1614,9 +1618,9 @@ hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, i
     return hipGetLastError();
 }
 
-hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, bool count, bool multi,
+hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool count, bool multi,
                          const TraceConfig& cfg) {
-    WavefrontShadowSrc src{p};
+    WavefrontShadowSrc src{p, cur};
     if (multi)
         return count ? dispatch_trace<true, true, true>(s, src, sc, p.scenes, cfg)
                      : dispatch_trace<true, false, true>(s, src, sc, p.scenes, cfg);

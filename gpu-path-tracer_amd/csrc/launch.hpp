@@ -100,7 +100,8 @@ hipError_t launch_bin(hipStream_t s, const PathState& p, int cur, uint32_t grid)
 // records (single view only).
 hipError_t launch_shade(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool multi, bool guided,
                         uint32_t grid, uint32_t block);
-hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, bool count, bool multi,
+// Any-hit traversal of the shadow rays that the shade of iteration parity `cur` pushed.
+hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, int cur, bool count, bool multi,
                          const TraceConfig& cfg);
 hipError_t launch_prepare(hipStream_t s, const PathState& p, int cur);
 // Tone-maps pixels [begin, end) of the views with virtual index in [lo, hi);

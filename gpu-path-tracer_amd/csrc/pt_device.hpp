@@ -342,15 +342,14 @@ struct FrameParams {
 struct Counters {
     uint32_t ray[2];
     uint32_t queue[kQueueCount + 1];   // diffuse, ggx, mirror, emissive, regen
-    uint32_t shadow;
+    uint32_t shadow[2];         // shadow rays pushed by the shade of an iteration of parity p
     uint32_t work_closest;
-    uint32_t work_shadow;
+    uint32_t work_shadow[2];
     uint32_t active;            // ray count of the next iteration
     uint32_t iterations;
     uint32_t invalid_pixels;
     uint32_t stack_overflow;
     uint32_t records;           // guide records of this run: record slots [0, records) are written
-    uint32_t pad;
     unsigned long long launched;
     unsigned long long ext_rays;
     unsigned long long shadow_rays;

@@ -58,9 +58,14 @@ Renderer::~Renderer() {
     invalidate_graph();
     for (auto& e : ring_events_)
         if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < 2; i++) {
+        if (fork_events_[i]) (void)hipEventDestroy(fork_events_[i]);
+        if (join_events_[i]) (void)hipEventDestroy(join_events_[i]);
+    }
     if (h_fp_) (void)hipHostFree(h_fp_);
     if (h_ctr_ring_) (void)hipHostFree(h_ctr_ring_);
     if (h_frame_staging_) (void)hipHostFree(h_frame_staging_);
+    if (side_stream_) (void)hipStreamDestroy(side_stream_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -78,6 +83,11 @@ pt_status Renderer::init() {
     PT_TRY_HIP(hipGetDevice(&device_), "hipGetDevice");
     PT_TRY_HIP(hipDeviceGetAttribute(&cu_count_, hipDeviceAttributeMultiprocessorCount, device_), "CU count");
     PT_TRY_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    PT_TRY_HIP(hipStreamCreateWithFlags(&side_stream_, hipStreamNonBlocking), "hipStreamCreate");
+    for (int i = 0; i < 2; i++) {
+        PT_TRY_HIP(hipEventCreateWithFlags(&fork_events_[i], hipEventDisableTiming), "hipEventCreate");
+        PT_TRY_HIP(hipEventCreateWithFlags(&join_events_[i], hipEventDisableTiming), "hipEventCreate");
+    }
     PT_TRY_HIP(hipHostMalloc((void**)&h_fp_, sizeof(FrameParams), hipHostMallocDefault), "hipHostMalloc");
     PT_TRY_HIP(hipHostMalloc((void**)&h_ctr_ring_, sizeof(Counters) * kRing, hipHostMallocDefault), "hipHostMalloc");
     for (auto& e : ring_events_) PT_TRY_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
@@ -232,11 +242,15 @@ pt_status Renderer::configure_trace() {
     for (const auto& sc : scenes_)
         if (sc) stack_total_ = std::max(stack_total_, (sc->host.bvh.width - 1u) * sc->host.bvh.depth + 2u);
     cfg.spill_depth = stack_total_ > cfg.lds_stack ? stack_total_ - cfg.lds_stack : 0u;
-    const size_t spill_bytes = (size_t)cfg.spill_depth * cfg.blocks * cfg.block * sizeof(uint2);
-    PT_TRY_HIP(d_spill_.allocate(spill_bytes), "traversal stack spill area");
+    // two spill areas: the any-hit traversal of one bounce runs concurrently
+    // with the next bounce's closest-hit traversal
+    const size_t spill_entries = (size_t)cfg.spill_depth * cfg.blocks * cfg.block;
+    PT_TRY_HIP(d_spill_.allocate(2 * spill_entries * sizeof(uint2)), "traversal stack spill area");
     cfg.spill = d_spill_.as<uint2>();
     cfg.overflow = d_overflow_.as<unsigned int>();
     trace_ = cfg;
+    shadow_trace_ = cfg;
+    shadow_trace_.spill = d_spill_.as<uint2>() + spill_entries;
     return PT_OK;
 }
 
@@ -370,14 +384,31 @@ pt_status Renderer::Clear(uint8_t r, uint8_t g, uint8_t b, uint8_t a) {
     return PT_OK;
 }
 
-pt_status Renderer::launch_iteration(int cur, bool count, const LoopMode& mode) {
+pt_status Renderer::join_shadow() {
+    if (shadow_pending_ < 0) return PT_OK;
+    PT_TRY_HIP(hipStreamWaitEvent(stream_, join_events_[shadow_pending_], 0), "join");
+    shadow_pending_ = -1;
+    return PT_OK;
+}
+
+pt_status Renderer::launch_iteration(int cur, bool count, const LoopMode& mode, bool fork) {
     const PathState p = path_state(&mode);
     const SceneDev sc = scene_dev(mode.scene);
     const uint32_t grid = (uint32_t)std::max(1, cu_count_) * 8u;
     PT_TRY_HIP(launch_closest(stream_, p, sc, cur, count, mode.trace_multi, trace_), "closest-hit launch");
     PT_TRY_HIP(launch_bin(stream_, p, cur, bin_grid()), "binning launch");
+    // the shadow queue and the guide records are rewritten by this shade
+    PT_TRY(join_shadow());
     PT_TRY_HIP(launch_shade(stream_, p, sc, cur, mode.shade_multi, mode.guided, grid, 256), "shade launch");
-    PT_TRY_HIP(launch_shadow(stream_, p, sc, count, mode.trace_multi, trace_), "shadow launch");
+    if (fork) {
+        PT_TRY_HIP(hipEventRecord(fork_events_[cur], stream_), "fork");
+        PT_TRY_HIP(hipStreamWaitEvent(side_stream_, fork_events_[cur], 0), "fork");
+        PT_TRY_HIP(launch_shadow(side_stream_, p, sc, cur, count, mode.trace_multi, shadow_trace_), "shadow launch");
+        PT_TRY_HIP(hipEventRecord(join_events_[cur], side_stream_), "join");
+        shadow_pending_ = cur;
+    } else {
+        PT_TRY_HIP(launch_shadow(stream_, p, sc, cur, count, mode.trace_multi, shadow_trace_), "shadow launch");
+    }
     PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
     PT_TRY_HIP(launch_prepare(stream_, p, cur), "prepare launch");
     return PT_OK;
@@ -391,8 +422,13 @@ pt_status Renderer::ensure_graph(const LoopMode& mode, hipGraphExec_t& exec) {
             return PT_OK;
         }
     PT_TRY_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "graph capture");
-    pt_status s = launch_iteration(0, false, mode);
-    if (s == PT_OK) s = launch_iteration(1, false, mode);
+    // kGraphIterations bounces, each any-hit launch overlapping the next
+    // bounce's first half; the last one joins before the graph ends
+    shadow_pending_ = -1;
+    pt_status s = PT_OK;
+    for (int i = 0; i < kGraphIterations && s == PT_OK; i++) s = launch_iteration(i & 1, false, mode, true);
+    if (s == PT_OK) s = join_shadow();
+    shadow_pending_ = -1;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(stream_, &g);
     if (s != PT_OK) {
@@ -544,7 +580,7 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
             PT_TRY_HIP(hipEventRecord(ev[2], stream_), "event");
             PT_TRY_HIP(launch_shade(stream_, p, sc, cur, mode.shade_multi, mode.guided, grid, 256), "shade launch");
             PT_TRY_HIP(hipEventRecord(ev[3], stream_), "event");
-            PT_TRY_HIP(launch_shadow(stream_, p, sc, false, mode.trace_multi, trace_), "shadow launch");
+            PT_TRY_HIP(launch_shadow(stream_, p, sc, cur, false, mode.trace_multi, shadow_trace_), "shadow launch");
             PT_TRY_HIP(hipEventRecord(ev[4], stream_), "event");
             PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
             PT_TRY_HIP(hipEventRecord(ev[5], stream_), "event");
@@ -569,7 +605,8 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
         const int lag = 2;
         int cur = 0;
         for (uint64_t k = 0;; k++) {
-            if (k * (use_graph ? 2 : 1) > max_iters) return fail(PT_ERROR_DEVICE, "wavefront did not terminate");
+            if (k * (use_graph ? kGraphIterations : 1) > max_iters)
+                return fail(PT_ERROR_DEVICE, "wavefront did not terminate");
             if (use_graph) {
                 PT_TRY_HIP(hipGraphLaunch(graph_exec, stream_), "graph launch");
             } else {

@@ -164,7 +164,11 @@ private:
                         const std::atomic<bool>* stop, const RunOptions& opt = RunOptions());
     pt_status render_frame(const pt_render_settings& s, const pt_camera& camera, const pt_motion_state& state,
                            uint32_t flags, pt_frame_stats* stats, const std::atomic<bool>* stop);
-    pt_status launch_iteration(int cur, bool count, const LoopMode& mode);
+    // One bounce.  With `fork` the any-hit traversal runs on side_stream_ and
+    // overlaps the regeneration, the bookkeeping and the next bounce's
+    // closest-hit traversal and binning; the next shade waits for it.
+    pt_status launch_iteration(int cur, bool count, const LoopMode& mode, bool fork = false);
+    pt_status join_shadow();
     void server_loop();
     void readback_loop();
     pt_status server_step();
@@ -184,6 +188,9 @@ private:
     int device_ = 0;
     int cu_count_ = 0;
     hipStream_t stream_ = nullptr;
+    hipStream_t side_stream_ = nullptr;     // any-hit traversal overlapped with the next bounce (graphs)
+    hipEvent_t fork_events_[2] = {}, join_events_[2] = {};
+    int shadow_pending_ = -1;               // parity of a forked any-hit launch not yet joined
     std::string last_error_;
     std::mutex message_mutex_;
 
@@ -201,6 +208,7 @@ private:
     FrameParams* h_fp_ = nullptr;           // pinned staging
     Counters* h_ctr_ring_ = nullptr;        // pinned, kRing entries
     static constexpr int kRing = 4;
+    static constexpr int kGraphIterations = 4;   // bounces per graph launch (even: parity returns to 0)
     hipEvent_t ring_events_[kRing] = {};
 
     // pixels: accumulation capacity for all views of a run; width_ x height_
@@ -217,6 +225,7 @@ private:
 
     // traversal
     TraceConfig trace_{};
+    TraceConfig shadow_trace_{};            // the same, with its own stack spill area
     DeviceBuffer d_spill_;
     uint32_t stack_total_ = 0;
 

@@ -324,12 +324,11 @@ pt_status Renderer::server_step() {
         }
         if (S.header_dirty) PT_TRY(write_header());
 
-        // ---- two bounce iterations
+        // ---- kGraphIterations bounce iterations
         if (server_graph_) {
             PT_TRY_HIP(hipGraphLaunch(server_graph_, stream_), "graph launch");
         } else {
-            PT_TRY(launch_iteration(0, false, server_mode_));
-            PT_TRY(launch_iteration(1, false, server_mode_));
+            for (int i = 0; i < kGraphIterations; i++) PT_TRY(launch_iteration(i & 1, false, server_mode_));
         }
         const int slot = (int)(k % kRing);
         PT_TRY_HIP(hipMemcpyAsync(h_ctr_ring_ + slot, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream_), "counters");
@@ -341,7 +340,7 @@ pt_status Renderer::server_step() {
         const int old = (int)((k - 1 - lag) % kRing);
         PT_TRY_HIP(hipEventSynchronize(ring_events_[old]), "sync");
         const Counters c = h_ctr_ring_[old];
-        const int64_t its = 2 * (int64_t)(k - lag);
+        const int64_t its = kGraphIterations * (int64_t)(k - lag);
         if (c.stack_overflow) return fail(PT_ERROR_DEVICE, "traversal stack overflow");
         {
             std::scoped_lock lock(S.mu);
