@@ -24,6 +24,9 @@ HBM_PEAK_GBS = 8000.0
 # 32 B in, hit record + queue id 17 B out, 128 B per BVH4 node, 48 B per triangle
 RAY_IN_B, HIT_OUT_B, NODE_B, TRI_B = 32, 17, 128, 48
 QNODE_B = 64        # quantized BVH4 node
+# one shadow ray: its 48-byte record (origin|tMax, direction|scene, radiance|pixel)
+# in; the 12-byte radiance add of an unoccluded ray is not counted
+SHADOW_IN_B = 48
 
 
 def log(msg):
@@ -75,11 +78,11 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes": n * 4, "repeats": 3}), flush=True)
 
 
-def measured_traffic():
-    """HBM bytes per closest-hit launch from the committed rocprofv3 --pmc
-    summary of this workload (FETCH_SIZE and WRITE_SIZE scaled by the copy
-    calibration), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic_closest_production.json")
+def measured_traffic(kind):
+    """HBM bytes per launch of the closest-hit ("closest") or any-hit ("any")
+    traversal from the committed rocprofv3 --pmc summary of this workload
+    (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kind}_production.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -444,7 +447,9 @@ def main():
         if a.quantized is not None:
             kw["bvh_quantized"] = a.quantized
         r.set_tuning(**kw)
-    node_b = QNODE_B if r.get_tuning()["bvh_quantized"] & 1 else NODE_B     # closest-hit node format
+    quant = r.get_tuning()["bvh_quantized"]
+    node_b = QNODE_B if quant & 1 else NODE_B           # closest-hit node format
+    node_b_any = QNODE_B if quant & 2 else NODE_B       # any-hit node format
     r.InitializeRenderingBuffers(settings["width"], settings["height"])
     t0 = time.time()
     r.InitializeSceneBuffers(scene)
@@ -484,14 +489,32 @@ def main():
     log(f"closest-hit: {launches} launches, avg {avg_launch_ms:.3f} ms, {nodes_per_ray:.2f} nodes + "
         f"{tris_per_ray:.2f} tris per ray = {bytes_per_ray:.0f} B/ray; shade {prof['shade_ms']:.1f} ms, "
         f"shadow {prof['shadow_ms']:.1f} ms, regen {prof['regen_ms']:.1f} ms, bin {prof['bin_ms']:.1f} ms, closest {prof['closest_ms']:.1f} ms")
-    traffic = measured_traffic()
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
-                "traffic_detail": traffic,
-                "kernel": "k_trace<closest>", "bytes_per_ray": round(bytes_per_ray, 1),
-                "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3),
-                "avg_launch_ms": round(avg_launch_ms, 4), "rays_per_launch": round(prof["extension_rays"] / launches, 1)}
+    # the roofline of the dominant traversal kernel (the larger share of the
+    # profiled frame), and the other one beside it
+    sh_launches = launches
+    sh_bytes_per_ray = SHADOW_IN_B + node_b_any * sh_nodes + TRI_B * sh_tris
+    sh_avg_ms = prof["shadow_ms"] / sh_launches
+    sh_bytes_per_launch = sh_bytes_per_ray * prof["shadow_rays"] / sh_launches
+    sh_achieved = sh_bytes_per_launch / (sh_avg_ms * 1e-3) / 1e9
+
+    def roof(kind, kernel, achieved_gbs, bpr, nodes, tris, avg_ms, rays_per_launch, node_bytes):
+        traffic = measured_traffic(kind)
+        return {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
+                "traffic": traffic["hbm_bytes_per_launch"] if traffic else None, "traffic_detail": traffic,
+                "kernel": kernel, "bytes_per_ray": round(bpr, 1), "node_bytes": node_bytes,
+                "nodes_per_ray": round(nodes, 3), "tris_per_ray": round(tris, 3),
+                "avg_launch_ms": round(avg_ms, 4), "rays_per_launch": round(rays_per_launch, 1)}
+
+    r_closest = roof("closest", "k_trace<closest>", achieved, bytes_per_ray, nodes_per_ray, tris_per_ray, avg_launch_ms,
+                     prof["extension_rays"] / launches, node_b)
+    r_any = roof("any", "k_trace<any>", sh_achieved, sh_bytes_per_ray, sh_nodes, sh_tris, sh_avg_ms,
+                 prof["shadow_rays"] / sh_launches, node_b_any)
+    log(f"any-hit: avg {sh_avg_ms:.3f} ms per launch, {sh_bytes_per_ray:.0f} B/ray, {sh_achieved:.0f} GB/s algorithmic")
+    if prof["shadow_ms"] >= prof["closest_ms"]:
+        roofline, other = r_any, ("roofline_closest", r_closest)
+    else:
+        roofline, other = r_closest, ("roofline_any", r_any)
 
     serving = None
     if not a.no_serving and a.config == "production":
@@ -518,7 +541,7 @@ def main():
                    "triangles": scene.triangle_count, "width": settings["width"], "height": settings["height"],
                    "spp": settings["samples_per_pixel"], "path_depth": settings["path_depth"],
                    "rays_per_frame": rays // a.steps, "device_ms_per_frame": round(dev_ms / a.steps, 3)},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, other[0]: other[1], "cpu_baseline": cpu,
     }
     if serving:
         out["p50_ms"] = serving["rates"][0]["p50_ms"]
