@@ -359,6 +359,10 @@ pt_status Renderer::server_step() {
         std::vector<std::shared_ptr<ServerRequest>> finished;
         for (auto& r : S.active)
             if (r->launched_at >= 0 && its >= r->launched_at + (int64_t)r->depth) finished.push_back(r);
+        // requests seen finished in one snapshot go out in the order they finished
+        std::stable_sort(finished.begin(), finished.end(), [](const auto& a, const auto& b) {
+            return a->launched_at + (int64_t)a->depth < b->launched_at + (int64_t)b->depth;
+        });
         for (auto& r : finished) {
             PT_TRY_HIP(launch_postprocess(stream_, p, (uint32_t)r->last_virtual, (uint32_t)r->last_virtual + 1u,
                                           r->accum_offset, r->accum_offset + r->pixels, true), "postprocess");
