@@ -92,6 +92,9 @@ __device__ __forceinline__ void add_counts(unsigned long long* nodes_ctr, unsign
 #define PT_ANY_SORTED 1    // any-hit rays also visit the nearest child first (occluders
                            // near the origin end the ray sooner: -14% shadow time measured)
 #endif
+#ifndef PT_TRACE_ATTR
+#define PT_TRACE_ATTR      // extra kernel attributes of k_trace for variant builds (none by default:
+#endif                     // 7 or 8 waves per SIMD measured slower than the compiler's 6)
 #ifndef PT_WHILE_WHILE
 #define PT_WHILE_WHILE 0   // if-if measured faster on gfx950 (one step per outer iteration)
 #endif
@@ -201,27 +204,55 @@ struct NodeOf<kNodeBvh4Q> {
     static constexpr int arity = 4;
 };
 
-// Quantized node: decode the four child boxes (one 64-byte line: origin and
-// scale exponents, lo rows, hi rows, references) and run the same slab test.
+// Byte i of a quantized row as a float (one v_cvt_f32_ubyte<i>).
+template <int I>
+__device__ __forceinline__ float qbyte(uint32_t row) {
+    return (float)((row >> (8 * I)) & 0xFFu);
+}
+
+// Entry distance of child I of a quantized node: per axis the plane slab
+// distance q*A + B (A = scale*rd, B = origin*rd - o*rd) of the near and far
+// byte rows, then the usual interval intersection with [tmin, tmax].
+template <int I>
+__device__ __forceinline__ float qchild(uint32_t nx, uint32_t fx, uint32_t ny, uint32_t fy, uint32_t nz, uint32_t fz,
+                                        float ax, float bx, float ay, float by, float az, float bz,
+                                        const RayFrame& r) {
+    const float tnx = fmaf(qbyte<I>(nx), ax, bx), tfx = fmaf(qbyte<I>(fx), ax, bx);
+    const float tny = fmaf(qbyte<I>(ny), ay, by), tfy = fmaf(qbyte<I>(fy), ay, by);
+    const float tnz = fmaf(qbyte<I>(nz), az, bz), tfz = fmaf(qbyte<I>(fz), az, bz);
+    const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tmin));
+    const float tf = fminf(fminf(tfx, tfy), fminf(tfz, r.tmax));
+    return tn <= tf ? tn : __builtin_inff();
+}
+
+// Quantized node: the four child boxes of one 64-byte line (origin and scale
+// exponents, lo rows, hi rows, references) under the slab test.  A plane q of
+// axis a lies at origin_a + q*scale_a, the product exact (scale_a is a power
+// of two), so its slab distance (origin_a + q*scale_a - o_a)*rd_a is formed as
+// q*(scale_a*rd_a) + (origin_a*rd_a - o_a*rd_a): scale_a*rd_a is again exact
+// and each distance takes one fma and one byte conversion.  The distance is
+// monotonic in q with the sign of rd_a, so the near plane of every child is
+// the lo row where rd_a >= 0 and the hi row otherwise, and no per-child
+// min/max is needed.  An empty slot (lo byte 255, hi byte 0) has its near
+// plane beyond its far plane; its reference is kInvalid as well.
 __device__ __forceinline__ void children(const Bvh4QNode* node, const RayFrame& r, float t0[4], uint32_t c[4]) {
     const uint4* p = reinterpret_cast<const uint4*>(node);
     const uint4 a = p[0], b = p[1], d = p[2], e = p[3];
-    const float ox = __uint_as_float(a.x), oy = __uint_as_float(a.y), oz = __uint_as_float(a.z);
-    const float sx = qnode_scale(a.w & 0xFFu), sy = qnode_scale((a.w >> 8) & 0xFFu), sz = qnode_scale((a.w >> 16) & 0xFFu);
+    const float ax = qnode_scale(a.w & 0xFFu) * r.rd.x;
+    const float ay = qnode_scale((a.w >> 8) & 0xFFu) * r.rd.y;
+    const float az = qnode_scale((a.w >> 16) & 0xFFu) * r.rd.z;
+    const float bx = fmaf(__uint_as_float(a.x), r.rd.x, -r.ord.x);
+    const float by = fmaf(__uint_as_float(a.y), r.rd.y, -r.ord.y);
+    const float bz = fmaf(__uint_as_float(a.z), r.rd.z, -r.ord.z);
     // rows: lo x, y, z = b.x, b.y, b.z; hi x, y, z = b.w, d.x, d.y
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const uint32_t sh = 8u * (uint32_t)i;
-        const float lx = qnode_decode(ox, (b.x >> sh) & 0xFFu, sx), hx = qnode_decode(ox, (b.w >> sh) & 0xFFu, sx);
-        const float ly = qnode_decode(oy, (b.y >> sh) & 0xFFu, sy), hy = qnode_decode(oy, (d.x >> sh) & 0xFFu, sy);
-        const float lz = qnode_decode(oz, (b.z >> sh) & 0xFFu, sz), hz = qnode_decode(oz, (d.y >> sh) & 0xFFu, sz);
-        const float ax = fmaf(lx, r.rd.x, -r.ord.x), bx = fmaf(hx, r.rd.x, -r.ord.x);
-        const float ay = fmaf(ly, r.rd.y, -r.ord.y), by = fmaf(hy, r.rd.y, -r.ord.y);
-        const float az = fmaf(lz, r.rd.z, -r.ord.z), bz = fmaf(hz, r.rd.z, -r.ord.z);
-        const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), r.tmin));
-        const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), r.tmax));
-        t0[i] = tn <= tf ? tn : __builtin_inff();
-    }
+    const bool px = r.rd.x >= 0.0f, py = r.rd.y >= 0.0f, pz = r.rd.z >= 0.0f;
+    const uint32_t nx = px ? b.x : b.w, fx = px ? b.w : b.x;
+    const uint32_t ny = py ? b.y : d.x, fy = py ? d.x : b.y;
+    const uint32_t nz = pz ? b.z : d.y, fz = pz ? d.y : b.z;
+    t0[0] = qchild<0>(nx, fx, ny, fy, nz, fz, ax, bx, ay, by, az, bz, r);
+    t0[1] = qchild<1>(nx, fx, ny, fy, nz, fz, ax, bx, ay, by, az, bz, r);
+    t0[2] = qchild<2>(nx, fx, ny, fy, nz, fz, ax, bx, ay, by, az, bz, r);
+    t0[3] = qchild<3>(nx, fx, ny, fy, nz, fz, ax, bx, ay, by, az, bz, r);
     c[0] = d.z;
     c[1] = d.w;
     c[2] = e.x;
@@ -599,7 +630,7 @@ struct BatchSrc {
 };
 
 template <int BLOCK, int LDS, int W, bool ANY, bool COUNT, bool MULTI, class Src>
-__global__ __launch_bounds__(BLOCK) void k_trace(Src src, SceneDev sc, const SceneDev* scenes, TraceConfig cfg) {
+__global__ __launch_bounds__(BLOCK) PT_TRACE_ATTR void k_trace(Src src, SceneDev sc, const SceneDev* scenes, TraceConfig cfg) {
     trace_persistent<BLOCK, LDS, W, ANY, COUNT, MULTI, Src>(src, sc, scenes, cfg);
 }
 
