@@ -216,7 +216,8 @@ typedef struct pt_bvh_info {
 /* Kernel launch and build parameters picked by the autotuner. */
 typedef struct pt_tuning {
     uint32_t trace_block;        /* threads per block of the traversal kernels: 64, 128, 256 or 512 */
-    uint32_t trace_blocks_per_cu;/* 0 = as many as fit */
+    uint32_t trace_blocks_per_cu;/* persistent grid of the closest-hit traversal (and of the any-hit one
+                                    when shadow_blocks_per_cu is 0), blocks per CU: 0 = as many as fit */
     uint32_t shade_block;        /* 256 */
     uint32_t pool_size;          /* path slots (queue capacity) */
     uint32_t use_graph;          /* replay the bounce loop as a hipGraph */
@@ -228,6 +229,11 @@ typedef struct pt_tuning {
     uint32_t bvh_quantized;      /* traversals on 64-byte nodes whose child boxes are 8-bit offsets
                                     (conservative, results unchanged): bit 0 closest hit, bit 1 any
                                     hit; default 2; 4-wide trees only; next upload */
+    uint32_t shadow_blocks_per_cu;/* persistent grid of the any-hit traversal, blocks per CU; 0 =
+                                    trace_blocks_per_cu if that is set, else the whole blocks that fit
+                                    while one wave per SIMD stays free.  In the replayed bounce loop
+                                    each bounce's any-hit launch runs beside the next closest-hit
+                                    launch, and the free slots let the two share the CUs */
 } pt_tuning;
 
 /* BVH builders: the host's binned-SAH build (best trees), or a linear BVH over

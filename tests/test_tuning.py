@@ -70,6 +70,25 @@ def test_images_equal_across_tuning_points(cornell):
         assert np.allclose(im, imgs[0], rtol=1e-4, atol=1e-5)
 
 
+def test_images_equal_across_traversal_grids(cornell):
+    """The persistent grids of the two traversals (the automatic one, whose
+    any-hit grid leaves a wave per SIMD to the concurrent closest-hit launch,
+    full grids, and a single block per CU each) render the same frame."""
+    st = dict(width=32, height=32, samples_per_pixel=16, path_depth=8, use_nee=1, seed=6)
+    imgs, rays = [], []
+    with ptcore.Renderer() as r:
+        r.InitializeRenderingBuffers(32, 32)
+        r.InitializeSceneBuffers(cornell)
+        for closest, any_hit in ((0, 0), (6, 6), (6, 1), (1, 1), (0, 3)):
+            r.set_tuning(trace_blocks_per_cu=closest, shadow_blocks_per_cu=any_hit)
+            s = r.RenderFrame(st, cornell.view)
+            imgs.append(r.GetFrameHdr(32, 32))
+            rays.append((s["extension_rays"], s["shadow_rays"]))
+    assert len(set(rays)) == 1
+    for im in imgs[1:]:
+        assert np.allclose(im, imgs[0], rtol=1e-4, atol=1e-5)
+
+
 def test_width2_production_scene_hits(oracle_lib):
     """983,042 triangles with 64-byte nodes, 128-byte nodes and quantized
     64-byte nodes: equal hits and occlusion, and equal to the oracle's on a
@@ -101,7 +120,8 @@ def test_width2_production_scene_hits(oracle_lib):
 
 
 def test_invalid_tuning_is_rejected(gpu_renderer):
-    for bad in (dict(bvh_width=3), dict(trace_block=1024), dict(trace_block=96), dict(bvh_quantized=4)):
+    for bad in (dict(bvh_width=3), dict(trace_block=1024), dict(trace_block=96), dict(bvh_quantized=4),
+                dict(trace_blocks_per_cu=65), dict(shadow_blocks_per_cu=65)):
         with pytest.raises(ptcore.PtError) as e:
             gpu_renderer.set_tuning(**bad)
         assert e.value.status == 1
