@@ -92,9 +92,10 @@ __device__ __forceinline__ void add_counts(unsigned long long* nodes_ctr, unsign
 #define PT_ANY_SORTED 1    // any-hit rays also visit the nearest child first (occluders
                            // near the origin end the ray sooner: -14% shadow time measured)
 #endif
-#ifndef PT_TRACE_ATTR
-#define PT_TRACE_ATTR      // extra kernel attributes of k_trace for variant builds (none by default:
-#endif                     // 7 or 8 waves per SIMD measured slower than the compiler's 6)
+#ifndef PT_TRACE_WAVES
+#define PT_TRACE_WAVES 6   // minimum waves per SIMD of the traversal kernels: 80 VGPRs, which every
+#endif                     // one-scene kernel fits anyway and the several-scene closest-hit kernel
+                           // reaches without spilling (82 otherwise: 5 waves); 7 or 8 measured slower
 #ifndef PT_WHILE_WHILE
 #define PT_WHILE_WHILE 0   // if-if measured faster on gfx950 (one step per outer iteration)
 #endif
@@ -630,7 +631,8 @@ struct BatchSrc {
 };
 
 template <int BLOCK, int LDS, int W, bool ANY, bool COUNT, bool MULTI, class Src>
-__global__ __launch_bounds__(BLOCK) PT_TRACE_ATTR void k_trace(Src src, SceneDev sc, const SceneDev* scenes, TraceConfig cfg) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : PT_TRACE_WAVES)))
+void k_trace(Src src, SceneDev sc, const SceneDev* scenes, TraceConfig cfg) {
     trace_persistent<BLOCK, LDS, W, ANY, COUNT, MULTI, Src>(src, sc, scenes, cfg);
 }
 
@@ -1565,22 +1567,28 @@ hipError_t dispatch_trace(hipStream_t s, const Src& src, const SceneDev& sc, con
     return hipErrorInvalidValue;
 }
 
-template <int BLOCK, int LDS, int NODE, bool ANY, bool COUNT, class Src>
+template <int BLOCK, int LDS, int NODE, bool ANY, bool COUNT, bool MULTI, class Src>
 int occ_one() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<BLOCK, LDS, NODE, ANY, COUNT, false, Src>, BLOCK, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<BLOCK, LDS, NODE, ANY, COUNT, MULTI, Src>, BLOCK, 0) !=
         hipSuccess)
         return 0;
     return nb;
 }
 
-template <int NODE, bool ANY, bool COUNT, class Src>
+template <int NODE, bool ANY, bool COUNT, bool MULTI, class Src>
 int occ(const TraceConfig& cfg) {
-    if (cfg.block == 64 && cfg.lds_stack == 16) return occ_one<64, 16, NODE, ANY, COUNT, Src>();
-    if (cfg.block == 128 && cfg.lds_stack == 16) return occ_one<128, 16, NODE, ANY, COUNT, Src>();
-    if (cfg.block == 256 && cfg.lds_stack == 8) return occ_one<256, 8, NODE, ANY, COUNT, Src>();
-    if (cfg.block == 512 && cfg.lds_stack == 8) return occ_one<512, 8, NODE, ANY, COUNT, Src>();
+    if (cfg.block == 64 && cfg.lds_stack == 16) return occ_one<64, 16, NODE, ANY, COUNT, MULTI, Src>();
+    if (cfg.block == 128 && cfg.lds_stack == 16) return occ_one<128, 16, NODE, ANY, COUNT, MULTI, Src>();
+    if (cfg.block == 256 && cfg.lds_stack == 8) return occ_one<256, 8, NODE, ANY, COUNT, MULTI, Src>();
+    if (cfg.block == 512 && cfg.lds_stack == 8) return occ_one<512, 8, NODE, ANY, COUNT, MULTI, Src>();
     return 0;
+}
+
+template <int NODE, bool MULTI>
+int occ_kind(const TraceConfig& cfg, bool any, bool count) {
+    if (any) return count ? occ<NODE, true, true, MULTI, WavefrontShadowSrc>(cfg) : occ<NODE, true, false, MULTI, WavefrontShadowSrc>(cfg);
+    return count ? occ<NODE, false, true, MULTI, WavefrontClosestSrc>(cfg) : occ<NODE, false, false, MULTI, WavefrontClosestSrc>(cfg);
 }
 
 }  // namespace
@@ -1590,19 +1598,9 @@ bool trace_config_supported(uint32_t block, uint32_t lds_stack) {
            (block == 512 && lds_stack == 8);
 }
 
-int trace_occupancy(const TraceConfig& cfg, bool any, bool count, bool quantized) {
-    if (quantized) {
-        if (any)
-            return count ? occ<kNodeBvh4Q, true, true, WavefrontShadowSrc>(cfg)
-                         : occ<kNodeBvh4Q, true, false, WavefrontShadowSrc>(cfg);
-        return count ? occ<kNodeBvh4Q, false, true, WavefrontClosestSrc>(cfg)
-                     : occ<kNodeBvh4Q, false, false, WavefrontClosestSrc>(cfg);
-    }
-    if (any)
-        return count ? occ<kNodeBvh4, true, true, WavefrontShadowSrc>(cfg)
-                     : occ<kNodeBvh4, true, false, WavefrontShadowSrc>(cfg);
-    return count ? occ<kNodeBvh4, false, true, WavefrontClosestSrc>(cfg)
-                 : occ<kNodeBvh4, false, false, WavefrontClosestSrc>(cfg);
+int trace_occupancy(const TraceConfig& cfg, bool any, bool count, bool quantized, bool multi) {
+    if (quantized) return multi ? occ_kind<kNodeBvh4Q, true>(cfg, any, count) : occ_kind<kNodeBvh4Q, false>(cfg, any, count);
+    return multi ? occ_kind<kNodeBvh4, true>(cfg, any, count) : occ_kind<kNodeBvh4, false>(cfg, any, count);
 }
 
 hipError_t launch_frame_begin(hipStream_t s, const PathState& p) {

@@ -83,7 +83,7 @@ struct TraceConfig {
 };
 
 // Blocks of `block` threads that fit on one CU for the traversal kernels.
-int trace_occupancy(const TraceConfig& cfg, bool any, bool count, bool quantized);
+int trace_occupancy(const TraceConfig& cfg, bool any, bool count, bool quantized, bool multi);
 bool trace_config_supported(uint32_t block, uint32_t lds_stack);
 
 hipError_t launch_frame_begin(hipStream_t s, const PathState& p);

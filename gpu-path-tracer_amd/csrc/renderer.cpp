@@ -224,51 +224,59 @@ pt_status Renderer::configure_trace() {
     cfg.refill = tuning_.trace_refill;
     if (!trace_config_supported(cfg.block, cfg.lds_stack))
         return fail(PT_ERROR_INVALID_ARGUMENT, "unsupported traversal block size " + std::to_string(cfg.block));
-    // per traversal, the smallest occupancy over the kernels the resident scenes use
-    int occ_closest = 1 << 20, occ_any = 1 << 20;
-    bool any_scene = false;
-    for (const auto& sc : scenes_) {
-        if (!sc) continue;
-        any_scene = true;
-        occ_closest = std::min(occ_closest, trace_occupancy(cfg, false, false, (sc->quantized & 1u) != 0));
-        occ_any = std::min(occ_any, trace_occupancy(cfg, true, false, (sc->quantized & 2u) != 0));
-    }
-    if (!any_scene) {
-        occ_closest = trace_occupancy(cfg, false, false, false);
-        occ_any = trace_occupancy(cfg, true, false, false);
-    }
-    occ_closest = std::max(1, occ_closest);
-    occ_any = std::max(1, occ_any);
-    if (tuning_.trace_blocks_per_cu) {
-        occ_closest = occ_any = (int)tuning_.trace_blocks_per_cu;
-    } else {
-        // the any-hit grid leaves one wave per SIMD (four per CU) free, so that
-        // the closest-hit launch it overlaps in the bounce loop starts at once
-        // (production frame: 7.8 -> 8.15 Grays/s at 256-thread blocks)
-        const int wpb = (int)cfg.block / 64;
-        occ_any = std::max(1, (occ_any * wpb - 4) / wpb);
-    }
-    if (tuning_.shadow_blocks_per_cu) occ_any = (int)tuning_.shadow_blocks_per_cu;
+    // per traversal and kernel form (one scene or several per launch), the
+    // smallest occupancy over the kernels the resident scenes use
+    int occ[2][2];   // [multi][any]
+    for (int m = 0; m < 2; m++)
+        for (int a = 0; a < 2; a++) {
+            occ[m][a] = 1 << 20;
+            bool any_scene = false;
+            for (const auto& sc : scenes_) {
+                if (!sc) continue;
+                any_scene = true;
+                occ[m][a] = std::min(occ[m][a], trace_occupancy(cfg, a != 0, false,
+                                                                (sc->quantized & (a ? 2u : 1u)) != 0, m != 0));
+            }
+            if (!any_scene) occ[m][a] = trace_occupancy(cfg, a != 0, false, false, m != 0);
+            occ[m][a] = std::max(1, occ[m][a]);
+        }
     const uint32_t cus = (uint32_t)std::max(1, cu_count_);
-    cfg.blocks = (uint32_t)occ_closest * cus;
+    uint32_t blocks[2][2];
+    for (int m = 0; m < 2; m++) {
+        int bc = occ[m][0], ba = occ[m][1];
+        if (tuning_.trace_blocks_per_cu) {
+            bc = ba = (int)tuning_.trace_blocks_per_cu;
+        } else {
+            // the any-hit grid leaves one wave per SIMD (four per CU) free, so that
+            // the closest-hit launch it overlaps in the bounce loop starts at once
+            // (production frame: 7.8 -> 8.15 Grays/s at 256-thread blocks)
+            const int wpb = (int)cfg.block / 64;
+            ba = std::max(1, (ba * wpb - 4) / wpb);
+        }
+        if (tuning_.shadow_blocks_per_cu) ba = (int)tuning_.shadow_blocks_per_cu;
+        blocks[m][0] = (uint32_t)bc * cus;
+        blocks[m][1] = (uint32_t)ba * cus;
+    }
     // worst-case stack over every resident scene: a level of a width-W BVH
     // pushes at most W-1 siblings
     stack_total_ = 3u + 2u;
     for (const auto& sc : scenes_)
         if (sc) stack_total_ = std::max(stack_total_, (sc->host.bvh.width - 1u) * sc->host.bvh.depth + 2u);
     cfg.spill_depth = stack_total_ > cfg.lds_stack ? stack_total_ - cfg.lds_stack : 0u;
-    // two spill areas: the any-hit traversal of one bounce runs concurrently
-    // with the next bounce's closest-hit traversal
-    const uint32_t shadow_blocks = (uint32_t)occ_any * cus;
-    const size_t spill_entries = (size_t)cfg.spill_depth * cfg.blocks * cfg.block;
-    const size_t shadow_spill_entries = (size_t)cfg.spill_depth * shadow_blocks * cfg.block;
-    PT_TRY_HIP(d_spill_.allocate((spill_entries + shadow_spill_entries) * sizeof(uint2)), "traversal stack spill area");
-    cfg.spill = d_spill_.as<uint2>();
+    // two spill areas, each sized for the larger of its two grids: the any-hit
+    // traversal of one bounce runs concurrently with the next bounce's
+    // closest-hit traversal
+    const size_t spill_closest = (size_t)cfg.spill_depth * std::max(blocks[0][0], blocks[1][0]) * cfg.block;
+    const size_t spill_any = (size_t)cfg.spill_depth * std::max(blocks[0][1], blocks[1][1]) * cfg.block;
+    PT_TRY_HIP(d_spill_.allocate((spill_closest + spill_any) * sizeof(uint2)), "traversal stack spill area");
     cfg.overflow = d_overflow_.as<unsigned int>();
-    trace_ = cfg;
-    shadow_trace_ = cfg;
-    shadow_trace_.blocks = shadow_blocks;
-    shadow_trace_.spill = d_spill_.as<uint2>() + spill_entries;
+    TraceConfig* out[2][2] = {{&trace_, &shadow_trace_}, {&trace_multi_, &shadow_trace_multi_}};
+    for (int m = 0; m < 2; m++)
+        for (int a = 0; a < 2; a++) {
+            *out[m][a] = cfg;
+            out[m][a]->blocks = blocks[m][a];
+            out[m][a]->spill = d_spill_.as<uint2>() + (a ? spill_closest : 0);
+        }
     return PT_OK;
 }
 
@@ -413,7 +421,9 @@ pt_status Renderer::launch_iteration(int cur, bool count, const LoopMode& mode, 
     const PathState p = path_state(&mode);
     const SceneDev sc = scene_dev(mode.scene);
     const uint32_t grid = (uint32_t)std::max(1, cu_count_) * 8u;
-    PT_TRY_HIP(launch_closest(stream_, p, sc, cur, count, mode.trace_multi, trace_), "closest-hit launch");
+    const TraceConfig& tc = mode.trace_multi ? trace_multi_ : trace_;
+    const TraceConfig& ta = mode.trace_multi ? shadow_trace_multi_ : shadow_trace_;
+    PT_TRY_HIP(launch_closest(stream_, p, sc, cur, count, mode.trace_multi, tc), "closest-hit launch");
     PT_TRY_HIP(launch_bin(stream_, p, cur, bin_grid()), "binning launch");
     // the shadow queue and the guide records are rewritten by this shade
     PT_TRY(join_shadow());
@@ -421,11 +431,11 @@ pt_status Renderer::launch_iteration(int cur, bool count, const LoopMode& mode, 
     if (fork) {
         PT_TRY_HIP(hipEventRecord(fork_events_[cur], stream_), "fork");
         PT_TRY_HIP(hipStreamWaitEvent(side_stream_, fork_events_[cur], 0), "fork");
-        PT_TRY_HIP(launch_shadow(side_stream_, p, sc, cur, count, mode.trace_multi, shadow_trace_), "shadow launch");
+        PT_TRY_HIP(launch_shadow(side_stream_, p, sc, cur, count, mode.trace_multi, ta), "shadow launch");
         PT_TRY_HIP(hipEventRecord(join_events_[cur], side_stream_), "join");
         shadow_pending_ = cur;
     } else {
-        PT_TRY_HIP(launch_shadow(stream_, p, sc, cur, count, mode.trace_multi, shadow_trace_), "shadow launch");
+        PT_TRY_HIP(launch_shadow(stream_, p, sc, cur, count, mode.trace_multi, ta), "shadow launch");
     }
     PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
     PT_TRY_HIP(launch_prepare(stream_, p, cur), "prepare launch");
@@ -592,13 +602,15 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
         for (uint64_t it = 0;; it++) {
             if (it > max_iters) return fail(PT_ERROR_DEVICE, "wavefront did not terminate");
             PT_TRY_HIP(hipEventRecord(ev[0], stream_), "event");
-            PT_TRY_HIP(launch_closest(stream_, p, sc, cur, false, mode.trace_multi, trace_), "closest-hit launch");
+            PT_TRY_HIP(launch_closest(stream_, p, sc, cur, false, mode.trace_multi,
+                                      mode.trace_multi ? trace_multi_ : trace_), "closest-hit launch");
             PT_TRY_HIP(hipEventRecord(ev[1], stream_), "event");
             PT_TRY_HIP(launch_bin(stream_, p, cur, bin_grid()), "binning launch");
             PT_TRY_HIP(hipEventRecord(ev[2], stream_), "event");
             PT_TRY_HIP(launch_shade(stream_, p, sc, cur, mode.shade_multi, mode.guided, grid, 256), "shade launch");
             PT_TRY_HIP(hipEventRecord(ev[3], stream_), "event");
-            PT_TRY_HIP(launch_shadow(stream_, p, sc, cur, false, mode.trace_multi, shadow_trace_), "shadow launch");
+            PT_TRY_HIP(launch_shadow(stream_, p, sc, cur, false, mode.trace_multi,
+                                     mode.trace_multi ? shadow_trace_multi_ : shadow_trace_), "shadow launch");
             PT_TRY_HIP(hipEventRecord(ev[4], stream_), "event");
             PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
             PT_TRY_HIP(hipEventRecord(ev[5], stream_), "event");

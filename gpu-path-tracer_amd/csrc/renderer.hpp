@@ -224,8 +224,10 @@ private:
     uint32_t last_spp_ = 1;
 
     // traversal
-    TraceConfig trace_{};
-    TraceConfig shadow_trace_{};            // the same, with its own stack spill area
+    TraceConfig trace_{};                   // closest hit, one scene per launch
+    TraceConfig shadow_trace_{};            // any hit, with its own stack spill area
+    TraceConfig trace_multi_{};             // the same two for launches over several scenes, whose
+    TraceConfig shadow_trace_multi_{};      //   kernels hold more registers (grids sized to their occupancy)
     DeviceBuffer d_spill_;
     uint32_t stack_total_ = 0;
 

@@ -137,6 +137,15 @@ def _nmse(a, ref):
     return float(np.mean((a - ref) ** 2)) / float(ref.mean()) ** 2
 
 
+def _nmse_trimmed(a, ref, q=0.999):
+    """_nmse with each pixel's squared error capped at its 99.9th percentile:
+    at 256 spp without NEE a handful of paths that reach the light through the
+    gap carry most of a dark image's squared error, for either sampler, and
+    one of them can swing the ratio of two such errors several-fold."""
+    e = ((a - ref) ** 2).mean(-1).ravel()
+    return float(np.minimum(e, np.quantile(e, q)).mean()) / float(ref.mean()) ** 2
+
+
 def _render_set(r, room, w, h, nee):
     """Reference (16,384 spp), plain and guided (online-trained, then the trained
     guide alone) images at 256 spp."""
@@ -175,10 +184,13 @@ def test_guided_render_is_unbiased(room_renderer):
 def test_trained_guide_gains_over_bsdf_sampling(room_renderer, nee, trained_max, online_max):
     """What the network gains over BSDF sampling at equal samples: the error of
     the online-trained render and of the trained guide alone, relative to
-    plain BSDF sampling (measured: about 0.57 and 0.53 with NEE, 0.27 and 0.17
-    without)."""
+    plain BSDF sampling, with the error of the brightest 0.1 % of pixels capped
+    (_nmse_trimmed; measured over five runs: 0.54-0.57 and 0.57-0.62 with NEE,
+    0.23-0.31 and 0.11-0.15 without; untrimmed 0.55-0.57 and 0.56-0.62, 0.26-0.35
+    and 0.14-0.19)."""
     r, room = room_renderer
     ref, plain, online, trained = _render_set(r, room, 320, 180, nee=nee)
-    e_plain, e_online, e_trained = _nmse(plain, ref), _nmse(online, ref), _nmse(trained, ref)
+    e_plain, e_online, e_trained = (_nmse_trimmed(plain, ref), _nmse_trimmed(online, ref),
+                                    _nmse_trimmed(trained, ref))
     assert e_trained < trained_max * e_plain, (e_trained, e_plain)
     assert e_online < online_max * e_plain, (e_online, e_plain)
