@@ -372,18 +372,19 @@ class Renderer:
     def RemoveScene(self, scene_id):
         self._check(self.lib.pt_remove_scene(self.h, scene_id))
 
-    def RenderBatch(self, items, flags=FRAME_DEFAULT, rgba=False):
+    def RenderBatch(self, items, flags=FRAME_DEFAULT, rgba=False, hdr=True):
         """items: [(scene_id, settings dict, scenes.View)], rendered in one
-        wavefront.  Returns ([hdr (h, w, 4) per view], stats), plus the RGBA8
-        images (h, w) when rgba=True."""
+        wavefront.  Returns ([hdr (h, w, 4) per view, or None when hdr=False],
+        stats), plus the RGBA8 images (h, w) when rgba=True."""
         n = len(items)
         views = (View * n)(*[_view(sid, st, v) for sid, st, v in items])
+        want_hdr = hdr
         hdr, ldr = [], []
         for i in range(n):
             w, h = views[i].settings.width, views[i].settings.height
-            hdr.append(np.zeros((h, w, 4), np.float32))
+            hdr.append(np.zeros((h, w, 4), np.float32) if want_hdr else None)
             ldr.append(np.zeros((h, w), np.uint32) if rgba else None)
-        hp = (C.c_void_p * n)(*[a.ctypes.data for a in hdr])
+        hp = (C.c_void_p * n)(*[a.ctypes.data if a is not None else None for a in hdr])
         lp = (C.c_void_p * n)(*[a.ctypes.data if a is not None else None for a in ldr])
         st = FrameStats()
         self._check(self.lib.pt_render_batch(self.h, views, n, hp, lp, flags, C.byref(st)))

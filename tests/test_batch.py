@@ -134,3 +134,21 @@ def test_add_remove_scene_slots(two_scenes, cornell, spheres):
     again = r.AddScene(cornell)
     assert again == extra
     r.RemoveScene(again)
+
+
+def test_batch_outputs_are_optional(two_scenes, cornell, spheres):
+    """The HDR and RGBA8 outputs of pt_render_batch are each optional: a view
+    rendered for its RGBA8 image alone gets the same ray counts, and an image
+    that tonemaps the HDR image rendered with it."""
+    r, sid = two_scenes
+    it = _items(cornell, spheres, sid)[0]
+    (hdr,), st, (rgba,) = r.RenderBatch([it], rgba=True)
+    (none,), st2, (rgba2,) = r.RenderBatch([it], rgba=True, hdr=False)
+    assert none is None
+    assert (st["extension_rays"], st["shadow_rays"]) == (st2["extension_rays"], st2["shadow_rays"])
+    # float-atomic order differs between runs: compare channels to one 8-bit step
+    a = rgba.view(np.uint8).reshape(-1, 4).astype(int)
+    b = rgba2.view(np.uint8).reshape(-1, 4).astype(int)
+    assert np.abs(a - b).max() <= 1
+    assert (a[:, 3] == 255).all()
+    assert hdr.shape == (32, 32, 4) and np.isfinite(hdr).all()
