@@ -241,21 +241,23 @@ pt_status Renderer::configure_trace() {
             occ[m][a] = std::max(1, occ[m][a]);
         }
     const uint32_t cus = (uint32_t)std::max(1, cu_count_);
-    uint32_t blocks[2][2];
+    uint32_t blocks[2][3];   // [multi][closest, any hit overlapped, any hit alone]
     for (int m = 0; m < 2; m++) {
-        int bc = occ[m][0], ba = occ[m][1];
+        int bc = occ[m][0], ba = occ[m][1], alone = occ[m][1];
         if (tuning_.trace_blocks_per_cu) {
-            bc = ba = (int)tuning_.trace_blocks_per_cu;
+            bc = ba = alone = (int)tuning_.trace_blocks_per_cu;
         } else {
-            // the any-hit grid leaves one wave per SIMD (four per CU) free, so that
-            // the closest-hit launch it overlaps in the bounce loop starts at once
-            // (production frame: 7.8 -> 8.15 Grays/s at 256-thread blocks)
+            // the overlapped any-hit grid leaves one wave per SIMD (four per CU)
+            // free, so that the closest-hit launch it overlaps in the bounce loop
+            // starts at once (production frame: 7.8 -> 8.15 Grays/s at 256-thread
+            // blocks); alone it takes the whole occupancy
             const int wpb = (int)cfg.block / 64;
             ba = std::max(1, (ba * wpb - 4) / wpb);
         }
-        if (tuning_.shadow_blocks_per_cu) ba = (int)tuning_.shadow_blocks_per_cu;
+        if (tuning_.shadow_blocks_per_cu) ba = alone = (int)tuning_.shadow_blocks_per_cu;
         blocks[m][0] = (uint32_t)bc * cus;
         blocks[m][1] = (uint32_t)ba * cus;
+        blocks[m][2] = (uint32_t)alone * cus;
     }
     // worst-case stack over every resident scene: a level of a width-W BVH
     // pushes at most W-1 siblings
@@ -267,12 +269,14 @@ pt_status Renderer::configure_trace() {
     // traversal of one bounce runs concurrently with the next bounce's
     // closest-hit traversal
     const size_t spill_closest = (size_t)cfg.spill_depth * std::max(blocks[0][0], blocks[1][0]) * cfg.block;
-    const size_t spill_any = (size_t)cfg.spill_depth * std::max(blocks[0][1], blocks[1][1]) * cfg.block;
+    const size_t spill_any = (size_t)cfg.spill_depth *
+                             std::max({blocks[0][1], blocks[1][1], blocks[0][2], blocks[1][2]}) * cfg.block;
     PT_TRY_HIP(d_spill_.allocate((spill_closest + spill_any) * sizeof(uint2)), "traversal stack spill area");
     cfg.overflow = d_overflow_.as<unsigned int>();
-    TraceConfig* out[2][2] = {{&trace_, &shadow_trace_}, {&trace_multi_, &shadow_trace_multi_}};
+    TraceConfig* out[2][3] = {{&trace_, &shadow_trace_, &shadow_alone_},
+                              {&trace_multi_, &shadow_trace_multi_, &shadow_alone_multi_}};
     for (int m = 0; m < 2; m++)
-        for (int a = 0; a < 2; a++) {
+        for (int a = 0; a < 3; a++) {
             *out[m][a] = cfg;
             out[m][a]->blocks = blocks[m][a];
             out[m][a]->spill = d_spill_.as<uint2>() + (a ? spill_closest : 0);
@@ -422,7 +426,8 @@ pt_status Renderer::launch_iteration(int cur, bool count, const LoopMode& mode, 
     const SceneDev sc = scene_dev(mode.scene);
     const uint32_t grid = (uint32_t)std::max(1, cu_count_) * 8u;
     const TraceConfig& tc = mode.trace_multi ? trace_multi_ : trace_;
-    const TraceConfig& ta = mode.trace_multi ? shadow_trace_multi_ : shadow_trace_;
+    const TraceConfig& ta = fork ? (mode.trace_multi ? shadow_trace_multi_ : shadow_trace_)
+                                 : (mode.trace_multi ? shadow_alone_multi_ : shadow_alone_);
     PT_TRY_HIP(launch_closest(stream_, p, sc, cur, count, mode.trace_multi, tc), "closest-hit launch");
     PT_TRY_HIP(launch_bin(stream_, p, cur, bin_grid()), "binning launch");
     // the shadow queue and the guide records are rewritten by this shade
@@ -610,7 +615,7 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
             PT_TRY_HIP(launch_shade(stream_, p, sc, cur, mode.shade_multi, mode.guided, grid, 256), "shade launch");
             PT_TRY_HIP(hipEventRecord(ev[3], stream_), "event");
             PT_TRY_HIP(launch_shadow(stream_, p, sc, cur, false, mode.trace_multi,
-                                     mode.trace_multi ? shadow_trace_multi_ : shadow_trace_), "shadow launch");
+                                     mode.trace_multi ? shadow_alone_multi_ : shadow_alone_), "shadow launch");
             PT_TRY_HIP(hipEventRecord(ev[4], stream_), "event");
             PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
             PT_TRY_HIP(hipEventRecord(ev[5], stream_), "event");

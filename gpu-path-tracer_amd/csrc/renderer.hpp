@@ -228,6 +228,8 @@ private:
     TraceConfig shadow_trace_{};            // any hit, with its own stack spill area
     TraceConfig trace_multi_{};             // the same two for launches over several scenes, whose
     TraceConfig shadow_trace_multi_{};      //   kernels hold more registers (grids sized to their occupancy)
+    TraceConfig shadow_alone_{};            // any hit where no closest-hit launch runs beside it
+    TraceConfig shadow_alone_multi_{};      //   (stage-by-stage loops): the whole occupancy
     DeviceBuffer d_spill_;
     uint32_t stack_total_ = 0;
 

@@ -230,10 +230,11 @@ typedef struct pt_tuning {
                                     (conservative, results unchanged): bit 0 closest hit, bit 1 any
                                     hit; default 2; 4-wide trees only; next upload */
     uint32_t shadow_blocks_per_cu;/* persistent grid of the any-hit traversal, blocks per CU; 0 =
-                                    trace_blocks_per_cu if that is set, else the whole blocks that fit
-                                    while one wave per SIMD stays free.  In the replayed bounce loop
-                                    each bounce's any-hit launch runs beside the next closest-hit
-                                    launch, and the free slots let the two share the CUs */
+                                    trace_blocks_per_cu if that is set, else as many as fit, less one
+                                    wave per SIMD where the launch overlaps a closest-hit launch: in
+                                    the replayed bounce loop each bounce's any-hit launch runs beside
+                                    the next closest-hit launch, and the free slots let the two share
+                                    the CUs */
 } pt_tuning;
 
 /* BVH builders: the host's binned-SAH build (best trees), or a linear BVH over
