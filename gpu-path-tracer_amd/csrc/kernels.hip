@@ -1158,6 +1158,7 @@ __global__ __launch_bounds__(GUIDED ? 64 * kInferWaves : 256) void k_shade(PathS
     const uint32_t c0 = c->queue[0], c1 = c->queue[1], c2 = c->queue[2], c3 = c->queue[3];
     const uint32_t e1 = c0, e2 = c0 + c1, e3 = e2 + c2, total = e3 + c3;
     const uint32_t rec_mask = GUIDED ? p.guide.cfg.rec_mask : 0u;
+    const PcgJump jump7 = pcg_jump(7u);
     const float gfrac = GUIDED ? p.guide.cfg.guide_fraction : 0.0f;
     // GUIDED: the guide network in LDS; each wave infers the mixtures of its 64
     // entries into its own tile, which the lanes then read while shading
@@ -1263,8 +1264,9 @@ __global__ __launch_bounds__(GUIDED ? 64 * kInferWaves : 256) void k_shade(PathS
                 float u7 = 1.0f;
                 const GuideMixRef gm{mixw + (threadIdx.x & 63), 64};
                 if (GUIDED && gfrac > 0.0f && q < 2) {
+                    // draw slot 7 of the bounce: seven steps ahead in one affine jump
                     Draws d7 = dr;
-                    for (int k = 0; k < 7; k++) d7.state = d7.state * kPcgMult + d7.inc;
+                    d7.state = jump7.mult * dr.state + jump7.plus * dr.inc;
                     u7 = d7.next();
                 }
                 const bool by_guide = GUIDED && gfrac > 0.0f && u7 < gfrac;
