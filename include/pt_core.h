@@ -317,7 +317,7 @@ typedef struct pt_server_config {
     uint64_t accum_budget_bytes;  /* device memory for admitted requests' pixels, 20 B per
                                      pixel; 0 = 2 GiB.  A request larger than it is rejected. */
     uint64_t window_samples;      /* samples queued on the device ahead of the launch front;
-                                     0 = twice the path pool */
+                                     0 = four times the path pool */
     uint64_t chunk_samples;       /* largest piece of one request scheduled at once; 0 = pool/4 */
     uint32_t max_requests;        /* requests admitted at once, 1..2048; 0 = 1024 */
     uint32_t reserved;
