@@ -395,8 +395,8 @@ def run_guide(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", choices=["production", "cornell", "batch", "serve", "guide", "ingest"],
                     default="production")
     ap.add_argument("--ingest-triangles", type=float, default=20e6, help="room size of --config ingest")
