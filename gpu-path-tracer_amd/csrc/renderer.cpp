@@ -637,7 +637,10 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
             if (stop && stop->load()) return PT_OK;
         }
     } else {
-        const int lag = 2;
+        // the host reads the counters of the previous graph launch while the
+        // current one runs: enough to keep the queue ahead of the device
+        // (two launches of lag measured no faster, and overshot the frame's end)
+        const int lag = 1;
         int cur = 0;
         for (uint64_t k = 0;; k++) {
             if (k * (use_graph ? kGraphIterations : 1) > max_iters)

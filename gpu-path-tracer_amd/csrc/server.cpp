@@ -53,11 +53,11 @@ pt_status Renderer::ServerStart(const pt_server_config& cfg_in) {
     if (budget_pixels < 1) return fail(PT_ERROR_INVALID_ARGUMENT, "accum_budget_bytes holds no pixel");
     PT_TRY(ensure_pool());
     // the window covers what the wavefront regenerates while the host reads its
-    // counters two graph launches late (about a pool per three bounces): with
-    // two pools the path pool ran part-empty (eight rooms of 1-20M triangles:
-    // 0.91x the rays/s of one request at a time; four pools 1.00x).  A deeper
-    // window commits samples further ahead of newly arriving requests: eight
-    // pools gave the same rays/s at a higher median latency
+    // counters a graph launch late (about a pool per three bounces): with two
+    // pools the path pool ran part-empty (eight rooms of 1-20M triangles: 0.97x
+    // the rays/s of one request at a time; four pools 1.00x).  A deeper window
+    // commits samples further ahead of newly arriving requests: eight pools
+    // gave the same rays/s at a higher median latency
     if (cfg.window_samples == 0) cfg.window_samples = 4ull * pool_capacity_;
     if (cfg.chunk_samples == 0) cfg.chunk_samples = std::max<uint64_t>(pool_capacity_ / 4, 4096);
     PT_TRY(ensure_accum((uint32_t)budget_pixels));
@@ -246,7 +246,7 @@ void Renderer::server_loop() {
 pt_status Renderer::server_step() {
     ServerState& S = *server_;
     const PathState p = path_state();
-    const int lag = 2;
+    const int lag = 1;   // counters of the launch before the one in flight
     uint64_t k = 0;   // graph launches (two iterations each)
 
     auto write_header = [&]() -> pt_status {
