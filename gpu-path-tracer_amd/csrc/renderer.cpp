@@ -216,6 +216,14 @@ void Renderer::set_frame_size(uint32_t w, uint32_t h) {
     hdr_valid_ = false;
 }
 
+namespace {
+#ifndef PT_CLOSEST_CHUNK
+#define PT_CLOSEST_CHUNK 512
+#endif
+constexpr uint32_t kClosestHitChunk = PT_CLOSEST_CHUNK;
+constexpr uint32_t kAnyHitChunk = 256;
+}  // namespace
+
 pt_status Renderer::configure_trace() {
     invalidate_graph();
     TraceConfig cfg;
@@ -280,6 +288,13 @@ pt_status Renderer::configure_trace() {
             *out[m][a] = cfg;
             out[m][a]->blocks = blocks[m][a];
             out[m][a]->spill = d_spill_.as<uint2>() + (a ? spill_closest : 0);
+            // largest ray chunk a wave claims with one atomic on the launch's
+            // work counter, a single address all eight XCDs contend for: the
+            // closest-hit launch (16.7M rays of the production frame) spent a
+            // fifth of its time in those claims at 256 rays (12.8 -> 9.9 ms per
+            // frame at 512); the any-hit launch's fewer, longer rays balance
+            // better with smaller chunks
+            out[m][a]->chunk = a ? kAnyHitChunk : kClosestHitChunk;
         }
     return PT_OK;
 }
