@@ -672,12 +672,36 @@ __device__ __forceinline__ V3 fix_normal(V3 gn, V3 d) { return dot(gn, d) > 0.0f
 
 __device__ __forceinline__ float luminance(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
+// sin and cos of a sampling angle x = 2 pi u, 0 <= x < 2 pi (the reference
+// calls the device libm, MaterialEval.cu:228,591).  Quadrant reduction by
+// pi/2 split into three parts (Cody & Waite 1980; the first two parts have 17
+// significant bits, so q * part is exact for q <= 4) and the Taylor
+// polynomials of degree 9 and 10 on |r| <= pi/4: at most 1.6 ulp from the
+// exact values, and with contraction off every operation rounds as the CPU
+// oracle's, so a sample's directions are identical on both.
+__device__ __forceinline__ void sincos_angle(float x, float* s, float* c) {
+    constexpr float kTwoOverPi = 0x1.45f306p-1f;
+    constexpr float kP1 = 0x1.921ep+0f, kP2 = 0x1.b544p-16f, kP3 = 0x1.0b4612p-34f;
+    const float q = rintf(x * kTwoOverPi);
+    float r = x - q * kP1;
+    r = r - q * kP2;
+    r = r - q * kP3;
+    const float r2 = r * r;
+    const float sp = r + r * (r2 * (-0x1.555556p-3f + r2 * (0x1.111112p-7f + r2 * (-0x1.a01a02p-13f + r2 * 0x1.71de3ap-19f))));
+    const float cp = (1.0f - 0.5f * r2) +
+                     (r2 * r2) * (0x1.555556p-5f + r2 * (-0x1.6c16c2p-10f + r2 * (0x1.a01a02p-16f + r2 * -0x1.27e4fcp-22f)));
+    const int k = (int)q & 3;
+    const float a = (k & 1) ? cp : sp, b = (k & 1) ? sp : cp;
+    *s = (k & 2) ? -a : a;
+    *c = ((k + 1) & 2) ? -b : b;
+}
+
 __device__ __forceinline__ V3 cosine_hemisphere(float u0, float u1) {
     const float phi = 2.0f * kPi * u0;
     const float cos_t = sqrtf(u1);
     const float sin_t = sqrtf(1.0f - u1);
     float s, c;
-    sincosf(phi, &s, &c);
+    sincos_angle(phi, &s, &c);
     return v3(sin_t * c, sin_t * s, cos_t);
 }
 
@@ -717,8 +741,11 @@ __device__ __forceinline__ float ggx_d(V3 wm, float alpha) {
 __device__ __forceinline__ float ggx_dv(V3 w, V3 wm, float alpha) {
     return ggx_g1(w, alpha) / fabsf(w.z) * ggx_d(wm, alpha) * fabsf(dot(w, wm));
 }
+// FresnelSchlick, MaterialEval.cu:647-650, with (1 - c)^5 as three products
 __device__ __forceinline__ V3 fresnel_schlick(V3 f0, float cos_h) {
-    const float k = powf(1.0f - cos_h, 5.0f);
+    const float x = 1.0f - cos_h;
+    const float x2 = x * x;
+    const float k = (x2 * x2) * x;
     return f0 + (v3(1.0f) - f0) * k;
 }
 // Visible-normal sampling (Heitz 2018), TrowbridgeSampleWm, MaterialEval.cu:594-610
@@ -730,7 +757,7 @@ __device__ __forceinline__ V3 ggx_sample_wm(V3 w, float alpha, float u0, float u
     const float rr = sqrtf(u0);
     const float theta = 2.0f * kPi * u1;
     float s, c;
-    sincosf(theta, &s, &c);
+    sincos_angle(theta, &s, &c);
     float px = rr * c, py = rr * s;
     const float h = sqrtf(1.0f - px * px);
     const float lt = (1.0f + wh.z) * 0.5f;

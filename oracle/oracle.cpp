@@ -447,11 +447,41 @@ Frame build_frame(F3 n, F3 t, float sign) {   // Math.hpp:136-148
 F3 fix_normal(F3 gn, F3 d) { return dot(gn, d) > 0.0f ? -gn : gn; }   // MaterialEval.cu:25-30
 float luminance(F3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
+// sin and cos of a sampling angle x = 2 pi u, 0 <= x < 2 pi.  The reference
+// calls the device libm here (MaterialEval.cu:228,591), whose last-ulp
+// behaviour is not reproducible off NVIDIA hardware; the core and this oracle
+// both evaluate the same published method instead, operation for operation:
+// quadrant reduction by pi/2 split into three parts (Cody & Waite 1980; the
+// first two parts have 17 significant bits, so q * part is exact for q <= 4,
+// and x - q * P1 is exact by Sterbenz's lemma) and the Taylor polynomials of
+// degree 9 (sin) and 10 (cos) on |r| <= pi/4 (truncation below 2e-9).
+// tests/test_oracle.py bounds its error against float64 sin/cos.
+void sincos_angle(float x, float* s, float* c) {
+    constexpr float kTwoOverPi = 0x1.45f306p-1f;
+    constexpr float kP1 = 0x1.921ep+0f, kP2 = 0x1.b544p-16f, kP3 = 0x1.0b4612p-34f;
+    const float q = std::nearbyint(x * kTwoOverPi);
+    float r = x - q * kP1;
+    r = r - q * kP2;
+    r = r - q * kP3;
+    const float r2 = r * r;
+    const float sp = r + r * (r2 * (-0x1.555556p-3f + r2 * (0x1.111112p-7f + r2 * (-0x1.a01a02p-13f + r2 * 0x1.71de3ap-19f))));
+    const float cp = (1.0f - 0.5f * r2) +
+                     (r2 * r2) * (0x1.555556p-5f + r2 * (-0x1.6c16c2p-10f + r2 * (0x1.a01a02p-16f + r2 * -0x1.27e4fcp-22f)));
+    switch ((int)q & 3) {
+        case 0: *s = sp; *c = cp; break;
+        case 1: *s = cp; *c = -sp; break;
+        case 2: *s = -sp; *c = -cp; break;
+        default: *s = -cp; *c = sp; break;
+    }
+}
+
 F3 cosine_hemisphere(float u0, float u1) {   // MaterialEval.cu:222-229
     const float phi = 2.0f * kPi * u0;
     const float cosTheta = std::sqrt(u1);
     const float sinTheta = std::sqrt(1.0f - u1);
-    return f3(sinTheta * std::cos(phi), sinTheta * std::sin(phi), cosTheta);
+    float sp, cp;
+    sincos_angle(phi, &sp, &cp);
+    return f3(sinTheta * cp, sinTheta * sp, cosTheta);
 }
 
 float cos2t(F3 w) { return w.z * w.z; }
@@ -483,7 +513,13 @@ float dggx(F3 wm, float alpha) {   // MaterialEval.cu:631-640
     return 1 / (kPi * sqr(alpha) * cos4 * sqr(1 + e));
 }
 float dvis(F3 w, F3 wm, float alpha) { return g1(w, alpha) / std::fabs(w.z) * dggx(wm, alpha) * std::fabs(dot(w, wm)); }
-F3 fresnel(F3 f0, float c) { return f0 + (f3(1.0f) - f0) * std::pow(1.0f - c, 5.0f); }
+// FresnelSchlick, MaterialEval.cu:647-650: (1 - c)^5 as three products in
+// place of powf (same value up to rounding; identical on the core and here).
+F3 fresnel(F3 f0, float c) {
+    const float x = 1.0f - c;
+    const float x2 = x * x;
+    return f0 + (f3(1.0f) - f0) * ((x2 * x2) * x);
+}
 F3 sample_wm(F3 w, float alpha, float u0, float u1) {   // MaterialEval.cu:587-610
     F3 wh = normalize(f3(alpha * w.x, alpha * w.y, w.z));
     if (wh.z < 0) wh = -wh;
@@ -491,12 +527,97 @@ F3 sample_wm(F3 w, float alpha, float u0, float u1) {   // MaterialEval.cu:587-6
     const F3 t2 = cross(wh, t1);
     const float r = std::sqrt(u0);
     const float theta = 2.0f * kPi * u1;
-    float px = r * std::cos(theta), py = r * std::sin(theta);
+    float st, ct;
+    sincos_angle(theta, &st, &ct);
+    float px = r * ct, py = r * st;
     const float h = std::sqrt(1.0f - px * px);
     py = h + ((1 + wh.z) * 0.5f) * (py - h);
     const float pz = std::sqrt(std::fmax(0.0f, 1.0f - (px * px + py * py)));
     const F3 nh = px * t1 + py * t2 + pz * wh;
     return normalize(f3(alpha * nh.x, alpha * nh.y, std::fmax(1e-6f, nh.z)));
+}
+
+// GGX constants of one hit, MaterialEval.cu:699-701,747-750,860-861:
+// alpha = max(roughness^2, 1e-3), F0 = lerp(((eta_i - eta)/(eta_i + eta))^2, base, metallic),
+// diffuse colour base * (1 - metallic).
+struct GgxParams {
+    F3 f0, diffc;
+    float alpha;
+};
+GgxParams ggx_params(const Material& m, float cur_ior) {
+    const F3 basec = f3(m.color[0], m.color[1], m.color[2]);
+    const float metallic = m.metallic;
+    const float df0 = sqr((cur_ior - m.ior) / (cur_ior + m.ior));
+    return GgxParams{lerp(f3(df0), basec, metallic), basec * (1.0f - metallic), std::fmax(m.roughness * m.roughness, 0.001f)};
+}
+
+// BSDF of a light sample, EvaluateGgxDirect, MaterialEval.cu:652-718 (local
+// directions): (1 - F) diffuse / pi + F D G / (4 cos_i cos_o); false where the
+// reference rejects the sample (either direction below the surface, a half
+// vector below it, or wo.wm / wi.wm <= 1e-6).
+bool ggx_direct_bsdf(const GgxParams& g, F3 wo, F3 wl, F3* f) {
+    const float ci = wl.z, co = wo.z;
+    if (ci <= 0.0f || co <= 0.0f) return false;
+    const F3 wm = normalize(wo + wl);
+    if (wm.z <= 0.0f) return false;
+    const float owm = dot(wo, wm), iwm = dot(wl, wm);
+    if (owm <= 1e-6f || iwm <= 1e-6f) return false;
+    const F3 F = fresnel(g.f0, clampf(owm, 0.0f, 1.0f));
+    const F3 fd = (f3(1.0f) - F) * g.diffc * kInvPi;
+    const F3 fs = F * dggx(wm, g.alpha) * gsmith(wo, wl, g.alpha) / (4.0f * ci * co);
+    *f = fd + fs;
+    return true;
+}
+
+// Lobe probabilities of the indirect sample, MaterialEval.cu:752-760: by the
+// luminance of the diffuse colour and of F0; false if both are ~0.
+bool ggx_lobes(const GgxParams& g, float* pd, float* ps) {
+    const float dw = luminance(g.diffc), sw = luminance(g.f0);
+    const float ws = dw + sw;
+    if (ws <= 1e-6f) return false;
+    *pd = dw / ws;
+    *ps = sw / ws;
+    return true;
+}
+
+// Solid-angle density of the indirect sample's direction wl (z > 0):
+// pd cos / pi + ps D_wo(wm) / (4 wo.wm), MaterialEval.cu:805-807.
+float ggx_pdf(float pd, float ps, F3 wo, F3 wl, F3 wm, float owm, float alpha) {
+    return pd * (wl.z * kInvPi) + ps * (dvis(wo, wm, alpha) / (4.0f * owm));
+}
+
+// EvaluateGgxIndirect, MaterialEval.cu:720-836, in local coordinates: pick a
+// lobe with u2, sample a cosine direction or reflect wo about a visible
+// normal, then the BSDF, the mixture density and the weight f cos / pdf.
+// false where the reference drops the path (before its world-space
+// geometric-normal test, which the caller applies).
+struct GgxSample {
+    F3 wl, f, weight;
+    float pdf;
+};
+bool ggx_indirect(const GgxParams& g, F3 wo, float u0, float u1, float u2, GgxSample* out) {
+    if (!(wo.z > 0.0f)) return false;
+    float pd, ps;
+    if (!ggx_lobes(g, &pd, &ps)) return false;
+    F3 wl;
+    if (u2 < pd) wl = cosine_hemisphere(u0, u1);
+    else wl = reflect(-wo, sample_wm(wo, g.alpha, u0, u1));
+    const float ci = wl.z, co = wo.z;
+    if (ci <= 0.0f) return false;
+    const F3 wm = normalize(wl + wo);
+    if (wm.z <= 0.0f) return false;
+    const float owm = dot(wo, wm);
+    if (owm <= 1e-6f) return false;
+    const F3 F = fresnel(g.f0, clampf(owm, 0.0f, 1.0f));
+    const F3 fd = (f3(1.0f) - F) * g.diffc * kInvPi;
+    const F3 fs = F * dggx(wm, g.alpha) * gsmith(wo, wl, g.alpha) / (4.0f * ci * co);
+    const float pdf = ggx_pdf(pd, ps, wo, wl, wm, owm, g.alpha);
+    if (pdf <= 1e-8f) return false;
+    out->wl = wl;
+    out->f = fd + fs;
+    out->pdf = pdf;
+    out->weight = (fd + fs) * (ci / pdf);
+    return true;
 }
 
 struct RenderParams {
@@ -619,12 +740,7 @@ F3 trace_sample(const Scene& s, const RenderParams& rp, uint64_t sample, RenderC
                 scatter = true;
             }
         } else if (tr.model == Ggx) {   // MaterialEval.cu:652-880
-            const F3 basec = f3(m.color[0], m.color[1], m.color[2]);
-            const float alpha = std::fmax(m.roughness * m.roughness, 0.001f);
-            const float metallic = m.metallic;
-            const float df0 = sqr((cur_ior - m.ior) / (cur_ior + m.ior));
-            const F3 f0 = lerp(f3(df0), basec, metallic);
-            const F3 diffc = basec * (1.0f - metallic);
+            const GgxParams g = ggx_params(m, cur_ior);
             const F3 wo = fr.to_local(f3(0.0f) - ray.d);
             if (nee) {
                 const LightHit lh = sample_light(l0, l1, l2);
@@ -634,53 +750,22 @@ F3 trace_sample(const Scene& s, const RenderParams& rp, uint64_t sample, RenderC
                         const float cl = std::fmax(dot(lh.gn, f3(0.0f) - lh.wi), 0.0f);
                         if (!(cl <= 0.0f)) {
                             const F3 wl = fr.to_local(lh.wi);
-                            const float ci = wl.z, co = wo.z;
-                            if (!(ci <= 0.0f || co <= 0.0f)) {
-                                const F3 wm = normalize(wo + wl);
-                                if (!(wm.z <= 0.0f)) {
-                                    const float owm = dot(wo, wm), iwm = dot(wl, wm);
-                                    if (!(owm <= 1e-6f || iwm <= 1e-6f)) {
-                                        const F3 F = fresnel(f0, clampf(owm, 0.0f, 1.0f));
-                                        const F3 fd = (f3(1.0f) - F) * diffc * kInvPi;
-                                        const F3 fs = F * dggx(wm, alpha) * gsmith(wo, wl, alpha) / (4.0f * ci * co);
-                                        const float G = ci * cl / lh.d2;
-                                        shadow(lh.p, lh.emission * thr * (fd + fs) * G / lh.pdf);
-                                    }
-                                }
+                            F3 f;
+                            if (ggx_direct_bsdf(g, wo, wl, &f)) {
+                                const float G = wl.z * cl / lh.d2;
+                                shadow(lh.p, lh.emission * thr * f * G / lh.pdf);
                             }
                         }
                     }
                 }
             }
-            if (wo.z > 0.0f) {
-                const float dw = luminance(diffc), sw = luminance(f0);
-                const float ws = dw + sw;
-                if (!(ws <= 1e-6f)) {
-                    const float pd = dw / ws, ps = sw / ws;
-                    F3 wl;
-                    if (u2 < pd) wl = cosine_hemisphere(u0, u1);
-                    else wl = reflect(-wo, sample_wm(wo, alpha, u0, u1));
-                    const float ci = wl.z, co = wo.z;
-                    if (!(ci <= 0.0f)) {
-                        const F3 wm = normalize(wl + wo);
-                        if (!(wm.z <= 0.0f)) {
-                            const float owm = dot(wo, wm);
-                            if (!(owm <= 1e-6f)) {
-                                const F3 F = fresnel(f0, clampf(owm, 0.0f, 1.0f));
-                                const F3 fd = (f3(1.0f) - F) * diffc * kInvPi;
-                                const F3 fs = F * dggx(wm, alpha) * gsmith(wo, wl, alpha) / (4.0f * ci * co);
-                                const float pdf = pd * (ci * kInvPi) + ps * (dvis(wo, wm, alpha) / (4.0f * owm));
-                                if (!(pdf <= 1e-8f)) {
-                                    const F3 ww = fr.from_local(wl);
-                                    if (!(dot(ww, gn) <= 0.0f)) {
-                                        wi = ww;
-                                        weight = (fd + fs) * (ci / pdf);
-                                        scatter = true;
-                                    }
-                                }
-                            }
-                        }
-                    }
+            GgxSample gs;
+            if (ggx_indirect(g, wo, u0, u1, u2, &gs)) {
+                const F3 ww = fr.from_local(gs.wl);
+                if (!(dot(ww, gn) <= 0.0f)) {
+                    wi = ww;
+                    weight = gs.weight;
+                    scatter = true;
                 }
             }
         } else if (tr.model == Mirror) {   // MaterialEval.cu:352-392
@@ -1112,6 +1197,89 @@ void orc_tonemap(const float* hdr, uint32_t n, uint32_t* rgba) {
             out |= b << (8 * k);
         }
         rgba[i] = out;
+    }
+}
+
+
+// ---- shading functions, exported for the closed-form tests (test_oracle.py)
+
+void orc_sincos(const float* x, uint32_t n, float* s, float* c) {
+    for (uint32_t i = 0; i < n; i++) sincos_angle(x[i], s + i, c + i);
+}
+
+void orc_cosine_hemisphere(const float* u2, uint32_t n, float* w3) {
+    for (uint32_t i = 0; i < n; i++) {
+        const F3 w = cosine_hemisphere(u2[2 * i], u2[2 * i + 1]);
+        w3[3 * i] = w.x, w3[3 * i + 1] = w.y, w3[3 * i + 2] = w.z;
+    }
+}
+
+static F3 ld3(const float* p, uint32_t i) { return f3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+static void st3(float* p, uint32_t i, F3 v) { p[3 * i] = v.x, p[3 * i + 1] = v.y, p[3 * i + 2] = v.z; }
+
+// D(wm) (what = 0), Lambda(wm) (1), G1(wm) (2) of one alpha
+void orc_ggx_term(int what, float alpha, const float* wm3, uint32_t n, float* out) {
+    for (uint32_t i = 0; i < n; i++) {
+        const F3 w = ld3(wm3, i);
+        out[i] = what == 0 ? dggx(w, alpha) : what == 1 ? lambda(w, alpha) : g1(w, alpha);
+    }
+}
+
+// D_wo(wm) = G1(wo) / |wo.z| D(wm) |wo.wm| (the visible-normal density)
+void orc_ggx_dvis(float alpha, const float* wo3, const float* wm3, uint32_t n, float* out) {
+    for (uint32_t i = 0; i < n; i++) out[i] = dvis(ld3(wo3, i), ld3(wm3, i), alpha);
+}
+
+void orc_ggx_sample_wm(float alpha, const float* wo3, const float* u2, uint32_t n, float* wm3) {
+    for (uint32_t i = 0; i < n; i++) st3(wm3, i, sample_wm(ld3(wo3, i), alpha, u2[2 * i], u2[2 * i + 1]));
+}
+
+// material = {r, g, b, roughness, metallic, ior}; the incoming medium has ior 1
+static GgxParams params_of(const float* m6) {
+    Material m{};
+    m.model = Ggx;
+    m.color[0] = m6[0], m.color[1] = m6[1], m.color[2] = m6[2];
+    m.roughness = m6[3];
+    m.metallic = m6[4];
+    m.ior = m6[5];
+    return ggx_params(m, 1.0f);
+}
+
+void orc_ggx_bsdf(const float* m6, const float* wo3, const float* wl3, uint32_t n, float* f3out, uint32_t* ok) {
+    const GgxParams g = params_of(m6);
+    for (uint32_t i = 0; i < n; i++) {
+        F3 f = f3(0.0f);
+        ok[i] = ggx_direct_bsdf(g, ld3(wo3, i), ld3(wl3, i), &f) ? 1u : 0u;
+        st3(f3out, i, f);
+    }
+}
+
+// density of the indirect sample's direction at wl (z > 0 and wo.wm > 0), 0 otherwise
+void orc_ggx_pdf(const float* m6, const float* wo3, const float* wl3, uint32_t n, float* pdf) {
+    const GgxParams g = params_of(m6);
+    float pd = 0.0f, ps = 0.0f;
+    const bool lobes = ggx_lobes(g, &pd, &ps);
+    for (uint32_t i = 0; i < n; i++) {
+        const F3 wo = ld3(wo3, i), wl = ld3(wl3, i);
+        pdf[i] = 0.0f;
+        if (!lobes || wl.z <= 0.0f) continue;
+        const F3 wm = normalize(wl + wo);
+        const float owm = dot(wo, wm);
+        if (owm <= 0.0f) continue;
+        pdf[i] = ggx_pdf(pd, ps, wo, wl, wm, owm, g.alpha);
+    }
+}
+
+void orc_ggx_sample(const float* m6, const float* wo3, const float* u3, uint32_t n, float* wl3, float* f3out,
+                    float* weight3, float* pdf, uint32_t* ok) {
+    const GgxParams g = params_of(m6);
+    for (uint32_t i = 0; i < n; i++) {
+        GgxSample gs{f3(0.0f), f3(0.0f), f3(0.0f), 0.0f};
+        ok[i] = ggx_indirect(g, ld3(wo3, i), u3[3 * i], u3[3 * i + 1], u3[3 * i + 2], &gs) ? 1u : 0u;
+        st3(wl3, i, gs.wl);
+        st3(f3out, i, gs.f);
+        st3(weight3, i, gs.weight);
+        pdf[i] = gs.pdf;
     }
 }
 

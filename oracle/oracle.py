@@ -36,6 +36,14 @@ def lib():
             "orc_camera_basis": (None, [C.c_float, C.c_float, _P, _P, _P]),
             "orc_render": (None, [_P, _P, C.c_uint64, _P, _U32, _U32, _U32, _P, _P]),
             "orc_tonemap": (None, [_P, _U32, _P]),
+            "orc_sincos": (None, [_P, _U32, _P, _P]),
+            "orc_cosine_hemisphere": (None, [_P, _U32, _P]),
+            "orc_ggx_term": (None, [C.c_int, C.c_float, _P, _U32, _P]),
+            "orc_ggx_dvis": (None, [C.c_float, _P, _P, _U32, _P]),
+            "orc_ggx_sample_wm": (None, [C.c_float, _P, _P, _U32, _P]),
+            "orc_ggx_bsdf": (None, [_P, _P, _P, _U32, _P, _P]),
+            "orc_ggx_pdf": (None, [_P, _P, _P, _U32, _P]),
+            "orc_ggx_sample": (None, [_P, _P, _P, _U32, _P, _P, _P, _P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -194,3 +202,82 @@ def reference_tangents(positions, normals, uvs, indices):
     out = np.zeros((len(P), 4), np.float32)
     _mikk.mikk_ref_tangents(_p(P), _p(N), _p(U), len(P), _p(idx), len(idx), _p(out))
     return out
+
+
+# ---- shading functions of the oracle (the ones its render calls)
+
+def _f32(a, cols=None):
+    a = np.ascontiguousarray(a, np.float32)
+    return a if cols is None else a.reshape(-1, cols)
+
+
+def sincos(x):
+    """(sin x, cos x) of sampling angles 0 <= x < 2 pi, as the render evaluates them."""
+    x = _f32(x).ravel()
+    s, c = np.zeros_like(x), np.zeros_like(x)
+    lib().orc_sincos(_p(x), len(x), _p(s), _p(c))
+    return s, c
+
+
+def cosine_hemisphere(u2):
+    u2 = _f32(u2, 2)
+    w = np.zeros((len(u2), 3), np.float32)
+    lib().orc_cosine_hemisphere(_p(u2), len(u2), _p(w))
+    return w
+
+
+def ggx_term(what, alpha, wm):
+    """what: "D", "lambda" or "G1" of local directions wm (n, 3)."""
+    wm = _f32(wm, 3)
+    out = np.zeros(len(wm), np.float32)
+    lib().orc_ggx_term({"D": 0, "lambda": 1, "G1": 2}[what], C.c_float(alpha), _p(wm), len(wm), _p(out))
+    return out
+
+
+def ggx_dvis(alpha, wo, wm):
+    wo, wm = _f32(wo, 3), _f32(wm, 3)
+    out = np.zeros(len(wm), np.float32)
+    lib().orc_ggx_dvis(C.c_float(alpha), _p(wo), _p(wm), len(wm), _p(out))
+    return out
+
+
+def ggx_sample_wm(alpha, wo, u2):
+    wo, u2 = _f32(wo, 3), _f32(u2, 2)
+    wm = np.zeros((len(u2), 3), np.float32)
+    lib().orc_ggx_sample_wm(C.c_float(alpha), _p(wo), _p(u2), len(u2), _p(wm))
+    return wm
+
+
+def _mat6(color, roughness, metallic, ior):
+    return np.array(list(color) + [roughness, metallic, ior], np.float32)
+
+
+def ggx_bsdf(material, wo, wl):
+    """f(wo, wl) of the NEE path and whether the sample is accepted; material =
+    (color, roughness, metallic, ior)."""
+    m = _mat6(*material)
+    wo, wl = _f32(wo, 3), _f32(wl, 3)
+    f = np.zeros((len(wl), 3), np.float32)
+    ok = np.zeros(len(wl), np.uint32)
+    lib().orc_ggx_bsdf(_p(m), _p(wo), _p(wl), len(wl), _p(f), _p(ok))
+    return f, ok.astype(bool)
+
+
+def ggx_pdf(material, wo, wl):
+    m = _mat6(*material)
+    wo, wl = _f32(wo, 3), _f32(wl, 3)
+    pdf = np.zeros(len(wl), np.float32)
+    lib().orc_ggx_pdf(_p(m), _p(wo), _p(wl), len(wl), _p(pdf))
+    return pdf
+
+
+def ggx_sample(material, wo, u3):
+    """The indirect GGX sample: (wl, f, weight, pdf, accepted)."""
+    m = _mat6(*material)
+    wo, u3 = _f32(wo, 3), _f32(u3, 3)
+    n = len(u3)
+    wl, f, wt = (np.zeros((n, 3), np.float32) for _ in range(3))
+    pdf = np.zeros(n, np.float32)
+    ok = np.zeros(n, np.uint32)
+    lib().orc_ggx_sample(_p(m), _p(wo), _p(u3), n, _p(wl), _p(f), _p(wt), _p(pdf), _p(ok))
+    return wl, f, wt, pdf, ok.astype(bool)

@@ -230,37 +230,19 @@ def test_tonemap_exact_values(oracle_lib):
 
 # ---------------------------------------------------------------- shading, closed forms
 
-def _furnace(albedo, emission, depth, model=scenes.MODEL_DIFFUSE):
-    """Closed box whose every wall is diffuse with albedo rho and also emits
-    Le uniformly: radiance at depth limit D without RR is
-    Le * (1 - rho^D) / (1 - rho) for any path.  Walls are modelled as
-    an inner emissive-free box plus NEE off."""
+def test_direct_view_of_the_emitter_reads_le(oracle_lib):
+    """Depth 1 without NEE traces camera rays only: a pixel reads Le times the
+    fraction of its samples that hit the emitter (EvaluateEmissiveKernel,
+    MaterialEval.cu:882-912), so every pixel is a multiple of Le / spp.  The
+    closed forms of diffuse, GGX and mirror transport are in
+    test_shading_closed_forms.py."""
     s = scenes.cornell_box()
-    return s
-
-
-def test_white_furnace_closed_form(oracle_lib):
-    """Sphere of diffuse walls (albedo rho) with a camera inside, no NEE: each
-    bounce hits a wall and terminates only at the depth limit.  The emissive
-    hits count radiance on camera rays only (delta) when NEE is on, so with
-    NEE off and emission on a small inner light the estimator converges to
-    the geometric-series answer; here we check the pure reflection series
-    with an emissive shell seen through a diffuse inner shell of holes-free
-    walls by rendering depth 1..4 and comparing ratios."""
-    rho = 0.5
-    # emissive triangle covering the whole view: depth 1 radiance = Le
-    s = scenes.cornell_box()
-    mats = s.materials.copy()
-    for k in range(3):
-        mats[k]["color"] = (rho, rho, rho)
-    s.materials = mats
     o = oracle_lib.OracleScene(s)
     settings = dict(width=8, height=8, samples_per_pixel=4, path_depth=1, use_nee=0, rr_start_depth=100, seed=1)
     hdr1, ext1, sh1 = o.render(settings, s.view)
     assert sh1 == 0 and ext1 == 8 * 8 * 4   # depth 1: camera rays only, no NEE
-    light_px = hdr1[:, 0] > 0
-    # camera rays that hit the emitter see exactly Le = 15
-    assert np.allclose(hdr1[light_px, :3] * 4 % 15, 0, atol=1e-3)
+    assert (hdr1[:, 0] > 0).any()
+    assert np.allclose(hdr1[:, :3] * 4 % 15, 0, atol=1e-3)
 
 
 def test_nee_and_bsdf_sampling_agree(cornell, cornell_oracle):
