@@ -110,8 +110,8 @@ pt_status pt_save_frame(pt_renderer* r, const char* path) {
     return guarded(r, [&](pt::Renderer& x) { return x.SaveFrame(path); });
 }
 
-pt_status pt_write_image(const char* path, uint32_t width, uint32_t height, uint32_t pixel_format,
-                         const void* pixels, int bottom_up) {
+static pt_status write_image(const char* path, uint32_t width, uint32_t height, uint32_t pixel_format,
+                             const void* pixels, int bottom_up) {
     const std::string p = path ? path : "";
     pt::ImageKind kind;
     std::string err;
@@ -130,6 +130,19 @@ pt_status pt_write_image(const char* path, uint32_t width, uint32_t height, uint
         case pt::ImageStatus::InvalidArgument: return PT_ERROR_INVALID_ARGUMENT;
         case pt::ImageStatus::Unsupported: return PT_ERROR_UNSUPPORTED;
         default: return PT_ERROR_IO;
+    }
+}
+
+// ImageIO::SaveImage (ImageIO.cpp:119-160); no exception crosses the C ABI:
+// the encoders build whole-image buffers (bad_alloc) and std::filesystem throws
+pt_status pt_write_image(const char* path, uint32_t width, uint32_t height, uint32_t pixel_format,
+                         const void* pixels, int bottom_up) {
+    try {
+        return write_image(path, width, height, pixel_format, pixels, bottom_up);
+    } catch (const std::bad_alloc&) {
+        return PT_ERROR_OUT_OF_MEMORY;
+    } catch (...) {
+        return PT_ERROR_IO;
     }
 }
 

@@ -317,7 +317,7 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
     const uint32_t tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
     const uint32_t stride = gridDim.x * BLOCK;
-    const uint32_t cap = LDS + cfg.spill_depth;
+    const uint32_t cap = min(cfg.stack_cap, (uint32_t)LDS + cfg.spill_depth);
     const uint32_t n = src.count();
     unsigned int* work = src.work();
     // chunk size from the ray count: about four chunks per wave of the grid,
@@ -1515,12 +1515,13 @@ __device__ __forceinline__ uint32_t view_of_pixel(const ViewRecord* views, uint3
 // PostprocessKernel, Image.cu:34-54, over pixels [begin, end) of the views
 // with virtual index in [lo, hi).  With `scale`, the accumulation pixels are
 // replaced by radiance / spp, ready for a plain copy to the caller.
-__global__ void k_postprocess(PathState p, uint32_t lo, uint32_t hi, uint32_t begin, uint32_t end, uint32_t scale) {
+__global__ void k_postprocess(PathState p, uint32_t lo, uint32_t hi, uint32_t begin, uint32_t end, uint32_t scale,
+                              uint32_t* invalid) {
     for (uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x) {
         const float inv_spp = p.views[view_of_pixel(p.views, lo, hi, i)].inv_spp;
         const float4 a = p.accum[i];
         const bool ok = isfinite(a.x) && a.x >= 0.0f && isfinite(a.y) && a.y >= 0.0f && isfinite(a.z) && a.z >= 0.0f;
-        if (!ok) atomicAdd(&p.ctr->invalid_pixels, 1u);
+        if (!ok) atomicAdd(invalid, 1u);
         const float inv_gamma = 1.0f / 2.2f;
         float c[3] = {a.x * inv_spp, a.y * inv_spp, a.z * inv_spp};
         uint32_t out = 0xFF000000u;
@@ -1692,9 +1693,11 @@ hipError_t launch_prepare(hipStream_t s, const PathState& p, int cur) {
 }
 
 hipError_t launch_postprocess(hipStream_t s, const PathState& p, uint32_t lo, uint32_t hi, uint32_t begin,
-                              uint32_t end, bool scale) {
+                              uint32_t end, bool scale, uint32_t* invalid) {
     if (end <= begin || hi <= lo) return hipSuccess;
-    hipLaunchKernelGGL(k_postprocess, dim3(grid_for(end - begin, 256, 4096)), dim3(256), 0, s, p, lo, hi, begin, end, scale ? 1u : 0u);
+    if (!invalid) invalid = &p.ctr->invalid_pixels;
+    hipLaunchKernelGGL(k_postprocess, dim3(grid_for(end - begin, 256, 4096)), dim3(256), 0, s, p, lo, hi, begin, end,
+                       scale ? 1u : 0u, invalid);
     return hipGetLastError();
 }
 

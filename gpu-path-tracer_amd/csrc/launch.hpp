@@ -79,7 +79,8 @@ struct TraceConfig {
     uint32_t chunk = 256;           // rays claimed per wave with one atomic
     uint32_t spill_depth = 0;       // stack entries per lane in the global spill area
     uint2* spill = nullptr;         // [spill_depth][blocks*block]
-    unsigned int* overflow = nullptr; // set when a push would exceed lds_stack+spill_depth
+    uint32_t stack_cap = 0;         // stack entries per lane, at most lds_stack + spill_depth
+    unsigned int* overflow = nullptr; // set when a push would exceed stack_cap
 };
 
 // Blocks of `block` threads that fit on one CU for the traversal kernels.
@@ -105,9 +106,11 @@ hipError_t launch_shadow(hipStream_t s, const PathState& p, const SceneDev& sc, 
                          const TraceConfig& cfg);
 hipError_t launch_prepare(hipStream_t s, const PathState& p, int cur);
 // Tone-maps pixels [begin, end) of the views with virtual index in [lo, hi);
-// with `scale` also divides the accumulation pixels by spp in place.
+// with `scale` also divides the accumulation pixels by spp in place.  Pixels
+// with non-finite or negative radiance are counted in *invalid (default: the
+// frame counters' invalid_pixels).
 hipError_t launch_postprocess(hipStream_t s, const PathState& p, uint32_t lo, uint32_t hi, uint32_t begin,
-                              uint32_t end, bool scale = false);
+                              uint32_t end, bool scale = false, uint32_t* invalid = nullptr);
 
 // Batch hooks.  rays: n x 2 float4 (origin|tMin, direction|tMax).
 // Closest: hits[i] in pt_hit layout (prim bits, t, u, v); any: occluded[i].  nodes/tris may be null.

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <filesystem>
@@ -65,6 +66,7 @@ Renderer::~Renderer() {
     }
     if (h_fp_) (void)hipHostFree(h_fp_);
     if (h_ctr_ring_) (void)hipHostFree(h_ctr_ring_);
+    if (h_req_invalid_) (void)hipHostFree(h_req_invalid_);
     if (h_frame_staging_) (void)hipHostFree(h_frame_staging_);
     if (side_stream_) (void)hipStreamDestroy(side_stream_);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -280,7 +282,11 @@ pt_status Renderer::configure_trace() {
     const size_t spill_any = (size_t)cfg.spill_depth *
                              std::max({blocks[0][1], blocks[1][1], blocks[0][2], blocks[1][2]}) * cfg.block;
     PT_TRY_HIP(d_spill_.allocate((spill_closest + spill_any) * sizeof(uint2)), "traversal stack spill area");
-    cfg.overflow = d_overflow_.as<unsigned int>();
+    cfg.stack_cap = cfg.lds_stack + cfg.spill_depth;
+    if (tuning_.stack_limit) cfg.stack_cap = std::min(cfg.stack_cap, tuning_.stack_limit);
+    // renders report an overflow in the frame's counters (cleared by the
+    // frame's first kernel, read after the frame); the hooks use their own flag
+    cfg.overflow = reinterpret_cast<unsigned int*>(d_ctr_.as<char>() + offsetof(Counters, stack_overflow));
     TraceConfig* out[2][3] = {{&trace_, &shadow_trace_, &shadow_alone_},
                               {&trace_multi_, &shadow_trace_multi_, &shadow_alone_multi_}};
     for (int m = 0; m < 2; m++)
@@ -296,6 +302,8 @@ pt_status Renderer::configure_trace() {
             // better with smaller chunks
             out[m][a]->chunk = a ? kAnyHitChunk : kClosestHitChunk;
         }
+    hook_trace_ = trace_;
+    hook_trace_.overflow = d_overflow_.as<unsigned int>();
     return PT_OK;
 }
 
@@ -996,15 +1004,23 @@ pt_status Renderer::TraceBatch(const pt_ray* rays, uint32_t n, bool any, bool de
         d_nodes = d_hook_aux_.as<uint32_t>();
         d_tris = d_nodes + n;
     }
+    // the overflow flag is cleared here and read by the synchronous forms; the
+    // asynchronous device form leaves it set for the next synchronous call
     PT_TRY_HIP(launch_zero_u32(s, d_work_.as<unsigned int>(), 1), "zero");
+    if (!device || nodes || tris) PT_TRY_HIP(launch_zero_u32(s, d_overflow_.as<unsigned int>(), 1), "zero");
     PT_TRY_HIP(launch_trace_batch(s, scene_dev(), d_in, n, any ? nullptr : static_cast<float4*>(d_out),
                                   any ? static_cast<uint32_t*>(d_out) : nullptr, d_nodes, d_tris,
-                                  d_work_.as<unsigned int>(), any, trace_),
+                                  d_work_.as<unsigned int>(), any, hook_trace_),
                "trace launch");
     if (!device) PT_TRY_HIP(hipMemcpyAsync(out, d_out, (size_t)n * out_elem, hipMemcpyDeviceToHost, s), "hook output");
     if (nodes) PT_TRY_HIP(hipMemcpyAsync(nodes, d_nodes, (size_t)n * 4, hipMemcpyDeviceToHost, s), "hook counts");
     if (tris) PT_TRY_HIP(hipMemcpyAsync(tris, d_tris, (size_t)n * 4, hipMemcpyDeviceToHost, s), "hook counts");
-    if (!device || nodes || tris) PT_TRY_HIP(hipStreamSynchronize(s), "hook sync");
+    if (!device || nodes || tris) {
+        uint32_t overflow = 0;
+        PT_TRY_HIP(hipMemcpyAsync(&overflow, d_overflow_.as<void>(), 4, hipMemcpyDeviceToHost, s), "overflow flag");
+        PT_TRY_HIP(hipStreamSynchronize(s), "hook sync");
+        if (overflow) return fail(PT_ERROR_DEVICE, "traversal stack overflow");
+    }
     return PT_OK;
 }
 
@@ -1024,12 +1040,16 @@ pt_status Renderer::Classify(const pt_ray* rays, uint32_t n, uint32_t* queue_of_
     unsigned int* qcounts = reinterpret_cast<unsigned int*>(qbase + (size_t)kQueueSlots * n);
     PT_TRY_HIP(launch_zero_u32(stream_, d_work_.as<unsigned int>(), 1), "zero");
     PT_TRY_HIP(launch_zero_u32(stream_, qcounts, kQueueSlots), "zero");
+    PT_TRY_HIP(launch_zero_u32(stream_, d_overflow_.as<unsigned int>(), 1), "zero");
     PT_TRY_HIP(launch_classify(stream_, scene_dev(), d_hook_in_.as<float4>(), n, d_hook_aux_.as<uint8_t>(), queues,
-                               qcounts, d_work_.as<unsigned int>(), trace_, bin_grid()),
+                               qcounts, d_work_.as<unsigned int>(), hook_trace_, bin_grid()),
                "classify launch");
     std::vector<uint32_t> host((size_t)n * kQueueSlots + 16);
+    uint32_t overflow = 0;
     PT_TRY_HIP(hipMemcpyAsync(host.data(), qbase, ((size_t)n * kQueueSlots + kQueueSlots) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "hook queues");
+    PT_TRY_HIP(hipMemcpyAsync(&overflow, d_overflow_.as<void>(), 4, hipMemcpyDeviceToHost, stream_), "overflow flag");
     PT_TRY_HIP(hipStreamSynchronize(stream_), "hook sync");
+    if (overflow) return fail(PT_ERROR_DEVICE, "traversal stack overflow");
     std::vector<uint8_t> seen(n, 0);
     for (uint32_t i = 0; i < n; i++) queue_of_ray[i] = PT_INVALID_INDEX;
     uint64_t total = 0;

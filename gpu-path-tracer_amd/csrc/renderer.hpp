@@ -230,6 +230,9 @@ private:
     TraceConfig shadow_trace_multi_{};      //   kernels hold more registers (grids sized to their occupancy)
     TraceConfig shadow_alone_{};            // any hit where no closest-hit launch runs beside it
     TraceConfig shadow_alone_multi_{};      //   (stage-by-stage loops): the whole occupancy
+    TraceConfig hook_trace_{};              // trace_ for the caller-batch hooks, with their own overflow flag
+    DeviceBuffer d_req_invalid_;            // server: per-request invalid-pixel counters
+    uint32_t* h_req_invalid_ = nullptr;     //   and their pinned host copies
     DeviceBuffer d_spill_;
     uint32_t stack_total_ = 0;
 

@@ -31,6 +31,9 @@ namespace {
 constexpr uint64_t kDefaultBudget = 2ull << 30;
 constexpr uint32_t kDefaultMaxRequests = 1024;
 constexpr uint32_t kMaxRequests = 2048;
+// invalid-pixel counters, each held by one request from its postprocess to
+// its readback (the server thread waits for one if all are held)
+constexpr uint32_t kInvalidSlots = 4096;
 constexpr uint64_t kPixelBytes = sizeof(float4) + sizeof(uint32_t);
 }  // namespace
 
@@ -63,9 +66,15 @@ pt_status Renderer::ServerStart(const pt_server_config& cfg_in) {
     PT_TRY(ensure_accum((uint32_t)budget_pixels));
     hdr_valid_ = false;
 
+    if (!h_req_invalid_) {
+        PT_TRY_HIP(d_req_invalid_.allocate(kInvalidSlots * sizeof(uint32_t)), "request counters");
+        PT_TRY_HIP(hipHostMalloc((void**)&h_req_invalid_, kInvalidSlots * sizeof(uint32_t), hipHostMallocDefault),
+                   "hipHostMalloc");
+    }
     auto S = std::make_unique<ServerState>();
     S->cfg = cfg;
     S->arena.reset((uint32_t)budget_pixels);
+    for (uint32_t k = kInvalidSlots; k-- > 0;) S->free_slots.push_back(k);
     PT_TRY_HIP(hipStreamCreateWithFlags(&S->rb_stream, hipStreamNonBlocking), "hipStreamCreate");
 
     FrameParams& fp = *h_fp_;
@@ -161,6 +170,8 @@ pt_status Renderer::ServerWait(uint64_t id, uint32_t timeout_ms, pt_request_info
     S.cv_done.wait_for(lock, std::chrono::milliseconds(timeout_ms),
                        [&]() { return r->state == PT_REQUEST_DONE || r->state == PT_REQUEST_FAILED; });
     fill_info(*r, info);
+    if (r->state == PT_REQUEST_FAILED && !r->message.empty())
+        return fail(PT_ERROR_DEVICE, "request " + std::to_string(id) + ": " + r->message);
     return PT_OK;
 }
 
@@ -370,8 +381,20 @@ pt_status Renderer::server_step() {
             return a->launched_at + (int64_t)a->depth < b->launched_at + (int64_t)b->depth;
         });
         for (auto& r : finished) {
+            {
+                std::unique_lock lock(S.mu);
+                S.cv_work.wait(lock, [&]() { return !S.free_slots.empty(); });
+                r->invalid_slot = S.free_slots.back();
+                S.free_slots.pop_back();
+            }
+            // count this request's invalid pixels in its own counter, copied to
+            // pinned memory before the event the readback waits on
+            uint32_t* slot = d_req_invalid_.as<uint32_t>() + r->invalid_slot;
+            PT_TRY_HIP(launch_zero_u32(stream_, slot, 1), "zero");
             PT_TRY_HIP(launch_postprocess(stream_, p, (uint32_t)r->last_virtual, (uint32_t)r->last_virtual + 1u,
-                                          r->accum_offset, r->accum_offset + r->pixels, true), "postprocess");
+                                          r->accum_offset, r->accum_offset + r->pixels, true, slot), "postprocess");
+            PT_TRY_HIP(hipMemcpyAsync(h_req_invalid_ + r->invalid_slot, slot, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                      stream_), "request counter");
             hipEvent_t e = nullptr;
             {
                 std::scoped_lock lock(S.mu);
@@ -414,12 +437,19 @@ void Renderer::readback_loop() {
             err = hipMemcpyAsync(r->hdr, d_accum_.as<float4>() + r->accum_offset, np * sizeof(float4),
                                  hipMemcpyDeviceToHost, S.rb_stream);
         if (err == hipSuccess) err = hipStreamSynchronize(S.rb_stream);
+        const uint32_t invalid = h_req_invalid_[r->invalid_slot];
         {
             std::scoped_lock lock(S.mu);
             S.arena.release(r->accum_offset, r->pixels);
             S.free_events.push_back(e);
+            S.free_slots.push_back(r->invalid_slot);
             r->t_done = S.now_ms();
-            if (err == hipSuccess) {
+            if (err == hipSuccess && invalid) {
+                // RenderFrame's check of the same counter (Renderer::run_views)
+                r->state = PT_REQUEST_FAILED;
+                r->message = std::to_string(invalid) + " pixels hold non-finite or negative radiance";
+                S.failed++;
+            } else if (err == hipSuccess) {
                 r->state = PT_REQUEST_DONE;
                 S.done++;
                 S.samples_done += r->total;

@@ -79,7 +79,9 @@ struct ServerRequest {
     uint64_t first_virtual = 0;      // view-ring index of its first and last chunk
     uint64_t last_virtual = 0;
     int64_t launched_at = -1;        // iterations run when every sample had been launched
+    uint32_t invalid_slot = 0xFFFFFFFFu; // counter of its non-finite or negative pixels (postprocess to readback)
     uint32_t state = PT_REQUEST_QUEUED;
+    std::string message;             // why it failed
     double t_submit = 0, t_admit = 0, t_done = 0;
 };
 
@@ -108,6 +110,7 @@ struct ServerState {
     std::unordered_map<uint64_t, std::shared_ptr<ServerRequest>> all;
     std::deque<std::pair<std::shared_ptr<ServerRequest>, hipEvent_t>> rb_queue;
     std::vector<hipEvent_t> free_events;
+    std::vector<uint32_t> free_slots;    // invalid-pixel counters not held by a request
     PixelArena arena;
     uint64_t next_id = 1;
 

@@ -124,7 +124,8 @@ class Tuning(C.Structure):
     _fields_ = [("trace_block", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("shade_block", C.c_uint32),
                 ("pool_size", C.c_uint32), ("use_graph", C.c_uint32), ("bvh_max_leaf", C.c_uint32),
                 ("bvh_traversal_cost", C.c_float), ("trace_refill", C.c_uint32), ("bvh_width", C.c_uint32),
-                ("bvh_builder", C.c_uint32), ("bvh_quantized", C.c_uint32), ("shadow_blocks_per_cu", C.c_uint32)]
+                ("bvh_builder", C.c_uint32), ("bvh_quantized", C.c_uint32), ("shadow_blocks_per_cu", C.c_uint32),
+                ("stack_limit", C.c_uint32)]
 
 
 _P = C.c_void_p

@@ -189,6 +189,33 @@ def cornell_box(tessellation=1):
     return scene
 
 
+def specular_room():
+    """The config-1 room with GGX and mirror surfaces where the light lands:
+    a copper-like GGX floor, GGX side walls (a red rough dielectric, a green
+    half-metal), a mirror back wall, a smooth GGX metal short box and a tinted
+    mirror tall box; only the ceiling stays diffuse.  36 triangles."""
+    s = cornell_box()
+    mats = np.stack([
+        material(MODEL_DIFFUSE, color=(0.73, 0.73, 0.73)),                                       # 0 ceiling
+        material(MODEL_GGX, color=(0.65, 0.05, 0.05), roughness=0.5, metallic=0.0, ior=1.5),      # 1 left
+        material(MODEL_GGX, color=(0.12, 0.45, 0.15), roughness=0.7, metallic=0.3, ior=1.5),     # 2 right
+        material(MODEL_EMISSIVE, color=(1.0, 1.0, 1.0), emission=(15.0, 15.0, 15.0)),            # 3 light
+        material(MODEL_GGX, color=(0.95, 0.64, 0.54), roughness=0.25, metallic=1.0, ior=1.5),    # 4 floor
+        material(MODEL_MIRROR, specular=(0.95, 0.95, 0.95)),                                      # 5 back wall
+        material(MODEL_GGX, color=(0.9, 0.9, 0.9), roughness=0.08, metallic=1.0, ior=1.5),       # 6 short box
+        material(MODEL_MIRROR, specular=(0.8, 0.9, 1.0)),                                         # 7 tall box
+    ])
+    mid = s.material_ids.copy()
+    mid[0:2] = 4          # floor
+    mid[4:6] = 5          # back wall
+    mid[10:22] = 6        # short box
+    mid[22:34] = 7        # tall box
+    s.material_ids = mid
+    s.materials = mats
+    s.name = "specular_room_36"
+    return s
+
+
 def cornell_scaled(triangles):
     """The config-1 room tessellated to about `triangles` triangles."""
     return cornell_box(max(1, int(round(math.sqrt(triangles / 36.0)))))

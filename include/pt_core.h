@@ -235,6 +235,11 @@ typedef struct pt_tuning {
                                     the replayed bounce loop each bounce's any-hit launch runs beside
                                     the next closest-hit launch, and the free slots let the two share
                                     the CUs */
+    uint32_t stack_limit;        /* 0: each traversal's stack (LDS + global spill) holds the deepest
+                                    resident BVH's worst case, so it cannot overflow; n > 0 caps it
+                                    at n entries per ray.  A traversal whose stack would overflow
+                                    drops the subtree and the call fails with PT_ERROR_DEVICE
+                                    "traversal stack overflow" (a test of that path) */
 } pt_tuning;
 
 /* BVH builders: the host's binned-SAH build (best trees), or a linear BVH over

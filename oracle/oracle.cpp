@@ -391,6 +391,40 @@ bool any_ref(const Scene& s, const Ray& ray) {
 }
 
 // Brute force over all triangles, minimum under the (t, prim id) order.
+// The reference's traversal (closest_ref) with the acceptance test of
+// closest_brute: a triangle at exactly the best t replaces it when its
+// primitive id is lower.  The slab test keeps every box whose entry is at or
+// before the best t, so this returns the (t, prim id) minimum of all
+// triangles, as brute force does, in the time of a BVH walk.
+HitRec closest_tprim(const Scene& s, Ray ray) {
+    HitRec h{kInvalid, 0.0f, 0.0f, 0.0f};
+    uint32_t best_prim = kInvalid;
+    const F3 inv = f3(1.0f) / ray.d;
+    uint32_t stack[70];
+    uint32_t sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const Node& node = s.nodes[stack[--sp]];
+        if (!aabb_intersect(node.bounds, ray, inv)) continue;
+        if (node.leaf()) {
+            for (uint32_t i = 0; i < node.count; i++) {
+                float t, u, v;
+                if (!mt(s.tris[node.first + i], ray.o, ray.d, ray.tmin, t, u, v)) continue;
+                const uint32_t prim = s.tris[node.first + i].prim;
+                if (t < ray.tmax || (t == ray.tmax && best_prim != kInvalid && prim < best_prim)) {
+                    ray.tmax = t;
+                    best_prim = prim;
+                    h = HitRec{node.first + i, t, u, v};
+                }
+            }
+        } else {
+            stack[sp++] = node.left;
+            stack[sp++] = node.right;
+        }
+    }
+    return h;
+}
+
 HitRec closest_brute(const Scene& s, const Ray& ray) {
     HitRec h{kInvalid, 0.0f, 0.0f, 0.0f};
     uint32_t best_prim = kInvalid;
@@ -622,6 +656,7 @@ bool ggx_indirect(const GgxParams& g, F3 wo, float u0, float u1, float u2, GgxSa
 
 struct RenderParams {
     uint32_t width, height, spp, axis, depth, use_nee, rr_start;
+    uint32_t tprim;   // 1: closest hits under the (t, prim id) order (the core's), 0: the reference's order
     uint64_t seed;
     F3 origin, llc, horizontal, vertical;
 };
@@ -655,7 +690,7 @@ F3 trace_sample(const Scene& s, const RenderParams& rp, uint64_t sample, RenderC
     const bool nee = rp.use_nee && !s.lights.empty();
     while (true) {
         rc.ext++;
-        const HitRec h = closest_ref(s, ray, nullptr, nullptr);
+        const HitRec h = rp.tprim ? closest_tprim(s, ray) : closest_ref(s, ray, nullptr, nullptr);
         if (h.tri == kInvalid) break;
         const Triangle& tr = s.tris[h.tri];
         const Material& m = s.materials[tr.material];
@@ -1017,6 +1052,7 @@ void orc_trace_closest(void* h, const float* rays, uint32_t n, void* hits, int m
         if (s.tris.empty()) hr = HitRec{kInvalid, 0, 0, 0};
         else if (mode == 0)
             hr = closest_ref(s, ray, nodes_visited ? nodes_visited + i : nullptr, tris_tested ? tris_tested + i : nullptr);
+        else if (mode == 2) hr = closest_tprim(s, ray);
         else hr = closest_brute(s, ray);
         uint32_t* o = static_cast<uint32_t*>(hits) + 4 * i;
         if (hr.tri == kInvalid) {
@@ -1049,7 +1085,7 @@ void orc_classify(void* h, const float* rays, uint32_t n, uint32_t* queue, uint3
     parallel_for(n, threads, [&](uint64_t i) {
         const float* r = rays + 8 * i;
         const Ray ray{f3(r[0], r[1], r[2]), f3(r[4], r[5], r[6]), r[3], r[7]};
-        const HitRec hr = s.tris.empty() ? HitRec{kInvalid, 0, 0, 0} : closest_brute(s, ray);
+        const HitRec hr = s.tris.empty() ? HitRec{kInvalid, 0, 0, 0} : closest_tprim(s, ray);
         uint32_t q = 4;
         if (hr.tri != kInvalid) {
             switch (s.tris[hr.tri].model) {
@@ -1151,6 +1187,7 @@ void orc_render(void* h, const uint32_t* settings, uint64_t seed, const float* c
     rp.depth = std::min(std::max(settings[3], 1u), 100u);
     rp.use_nee = settings[4];
     rp.rr_start = settings[5] ? settings[5] : 3u;
+    rp.tprim = settings[6];
     rp.seed = seed;
     rp.origin = f3(camera12[0], camera12[1], camera12[2]);
     rp.llc = f3(camera12[3], camera12[4], camera12[5]);

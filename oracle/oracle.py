@@ -83,12 +83,16 @@ class OracleScene:
             lib().orc_scene_destroy(self.h)
             self.h = None
 
-    def closest(self, rays, brute=False, threads=None, counts=False):
+    def closest(self, rays, brute=False, threads=None, counts=False, tprim=False):
+        """Closest hits (prim, t, u, v as uint32 words).  Default: the reference's
+        traversal order (first found wins at equal t); brute: all triangles under
+        the (t, prim id) order; tprim: the same order through the BVH."""
         rays = np.ascontiguousarray(rays, np.float32)
         hits = np.zeros((len(rays), 4), np.uint32)
         nv = np.zeros(len(rays), np.uint32) if counts else None
         nt = np.zeros(len(rays), np.uint32) if counts else None
-        lib().orc_trace_closest(self.h, _p(rays), len(rays), _p(hits), 1 if brute else 0,
+        mode = 1 if brute else 2 if tprim else 0
+        lib().orc_trace_closest(self.h, _p(rays), len(rays), _p(hits), mode,
                                 threads or default_threads(), _p(nv), _p(nt))
         return (hits, nv, nt) if counts else hits
 
@@ -128,12 +132,15 @@ class OracleScene:
         lib().orc_scene_bvh(self.h, _p(nodes), _p(order))
         return nodes, order
 
-    def render(self, settings, view, pixel_begin=0, pixel_end=None, threads=None):
-        """HDR (accumulated radiance / spp) of pixels [pixel_begin, pixel_end)."""
+    def render(self, settings, view, pixel_begin=0, pixel_end=None, threads=None, tprim=False):
+        """HDR (accumulated radiance / spp) of pixels [pixel_begin, pixel_end).
+        tprim: closest hits under the (t, prim id) order, the core's rule,
+        instead of the reference's traversal order (they differ only where
+        two triangles are hit at exactly the same t)."""
         w, h = settings["width"], settings["height"]
         pixel_end = w * h if pixel_end is None else pixel_end
         s = np.array([w, h, settings["samples_per_pixel"], settings["path_depth"], settings.get("use_nee", 1),
-                      settings.get("rr_start_depth", 3)], np.uint32)
+                      settings.get("rr_start_depth", 3), 1 if tprim else 0], np.uint32)
         cam = camera_basis(view, w / h)
         out = np.zeros((pixel_end - pixel_begin, 4), np.float32)
         cnt = np.zeros(2, np.uint64)

@@ -229,3 +229,59 @@ def ggx_floor_expected(material, wo, p, le=0.1, n_theta=1200, n_phi=1200):
     f = ggx_f64(material, wol, wi)
     w = (1.0 / n_theta) * (2 * math.pi / n_phi) / 2.0
     return le * (f * inside[:, None]).sum(0) * w
+
+
+# ---------------------------------------------------------------- checks shared by the oracle and GPU tests
+
+def check_plates(hdr, view, w, h, rho, le, nee, depth, blk=8):
+    """E[L] = rho Le F(p) per 8 x 8 block within 5 standard errors (0 at depth 1
+    without NEE).  hdr: (w h, 4) radiance / spp."""
+    p = floor_points(view, w, h)
+    want = rho * le * plate_form_factor(p[:, 0], p[:, 2])
+    got = hdr[:, 0].astype(np.float64)
+    assert np.array_equal(hdr[:, 0], hdr[:, 1]) and np.array_equal(hdr[:, 0], hdr[:, 2])
+    if depth == 1 and not nee:
+        assert (got == 0).all()
+        return
+    res = (got - want).reshape(h // blk, blk, w // blk, blk)
+    mean, se = res.mean((1, 3)), res.std((1, 3)) / blk
+    assert (np.abs(mean) < 5 * se + 1e-4).all(), (mean, se)
+    assert abs(res.mean()) < 4 * res.std() / math.sqrt(w * h) + 1e-5
+
+
+def check_corridor(hdr, view, w, h, depth, spp, rr_start):
+    """Without roulette every pixel whose samples all reflect k times reads
+    s^k Le (0 for k >= depth); with roulette from the first bounce the pixels
+    are unbiased around it (z-scores of the exact per-pixel variance)."""
+    want, same = corridor_expected(view, w, h, depth)
+    ks = corridor_bounces(view, w, h)
+    assert same.mean() > 0.6 and len(set(ks[same])) >= 4
+    got = hdr[same, :3].astype(np.float64)
+    if rr_start >= depth:
+        assert np.allclose(got, want[same], rtol=1e-5, atol=1e-6)
+        return
+    k = ks[same]
+    sv = np.asarray(CORRIDOR_S, np.float64)
+    le = np.asarray(CORRIDOR_LE, np.float64)
+    p = sv.max() ** k
+    v = le[None] * (sv / sv.max())[None] ** k[:, None]
+    se = np.sqrt(v * v * (p * (1 - p))[:, None] / spp) + 1e-9
+    z = (got - want[same]) / se
+    assert np.allclose(got[k == 0], want[same][k == 0])
+    assert abs(z.mean()) < 0.25 and (np.abs(z) < 6.5).all(), (z.mean(), np.abs(z).max())
+
+
+def check_ggx_floor(hdr, view, w, h, material, blk=4):
+    """Block means against Le times the quadrature of the published BSDF over
+    the emitter's directions at each block's centre."""
+    img = hdr[:, :3].astype(np.float64).reshape(h // blk, blk, w // blk, blk, 3)
+    got, se = img.mean((1, 3)), img.std((1, 3)) / blk
+    cam = np.asarray(view.position, np.float64)
+    centres = floor_points(view, w // blk, h // blk)
+    want = np.zeros_like(got)
+    for i, q in enumerate(centres):
+        wo = cam - q
+        wo /= np.linalg.norm(wo)
+        want.reshape(-1, 3)[i] = ggx_floor_expected(material, np.array([wo[0], wo[2], wo[1]]), q, n_theta=800,
+                                                    n_phi=800)
+    assert (np.abs(got - want) < 5 * se + 3e-3 * want).all(), (got, want, se)

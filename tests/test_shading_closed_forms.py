@@ -359,19 +359,9 @@ def test_diffuse_plates_match_form_factor(oracle_lib, nee, depth, rr):
     w = h = 32
     st = dict(width=w, height=h, samples_per_pixel=256, path_depth=depth, use_nee=nee, rr_start_depth=rr, seed=7)
     hdr, ext, sh = o.render(st, s.view)
-    p = cf.floor_points(s.view, w, h)
-    want = rho * le * cf.plate_form_factor(p[:, 0], p[:, 2])
-    got = hdr[:, 0].astype(np.float64)
-    assert np.array_equal(hdr[:, 0], hdr[:, 1]) and np.array_equal(hdr[:, 0], hdr[:, 2])
+    cf.check_plates(hdr, s.view, w, h, rho, le, nee, depth)
     if depth == 1 and not nee:
-        assert (got == 0).all() and sh == 0
-        return
-    # 8 x 8 blocks: mean residual against the closed form within 5 standard errors
-    blk = 8
-    res = (got - want).reshape(h // blk, blk, w // blk, blk)
-    mean, se = res.mean((1, 3)), res.std((1, 3)) / blk
-    assert (np.abs(mean) < 5 * se + 1e-4).all(), (mean, se)
-    assert abs(res.mean()) < 4 * res.std() / math.sqrt(w * h) + 1e-5
+        assert sh == 0
 
 
 def test_mirror_corridor_is_a_power_series(oracle_lib):
@@ -385,10 +375,8 @@ def test_mirror_corridor_is_a_power_series(oracle_lib):
     for depth in (8, 4):
         st = dict(width=w, height=h, samples_per_pixel=16, path_depth=depth, use_nee=1, rr_start_depth=100, seed=3)
         hdr, _, sh = o.render(st, s.view)
-        want, same = cf.corridor_expected(s.view, w, h, depth)
-        ks = cf.corridor_bounces(s.view, w, h)
-        assert same.mean() > 0.6 and len(set(ks[same])) >= 4 and sh == 0
-        assert np.allclose(hdr[same, :3], want[same], rtol=1e-5, atol=1e-6)
+        assert sh == 0
+        cf.check_corridor(hdr, s.view, w, h, depth, 16, 100)
 
 
 def test_mirror_corridor_roulette_keeps_the_mean(oracle_lib):
@@ -399,18 +387,7 @@ def test_mirror_corridor_roulette_keeps_the_mean(oracle_lib):
     w, h = 64, 16
     st = dict(width=w, height=h, samples_per_pixel=256, path_depth=16, use_nee=1, rr_start_depth=1, seed=4)
     hdr, _, _ = o.render(st, s.view)
-    want, same = cf.corridor_expected(s.view, w, h, 16)
-    k = cf.corridor_bounces(s.view, w, h)[same]
-    got = hdr[same, :3].astype(np.float64)
-    # per sample: Le (s / s_max)^k with probability s_max^k, else 0
-    sv = np.asarray(cf.CORRIDOR_S, np.float64)
-    le = np.asarray(cf.CORRIDOR_LE, np.float64)
-    p = sv.max() ** k
-    v = le[None] * (sv / sv.max())[None] ** k[:, None]
-    se = np.sqrt(v * v * (p * (1 - p))[:, None] / 256) + 1e-9
-    z = (got - want[same]) / se
-    assert np.allclose(got[k == 0], want[same][k == 0])
-    assert abs(z.mean()) < 0.25 and (np.abs(z) < 6.5).all()
+    cf.check_corridor(hdr, s.view, w, h, 16, 256, 1)
 
 
 @pytest.mark.parametrize("material", [GGX_MATERIALS[0], GGX_MATERIALS[1], GGX_MATERIALS[2]])
@@ -425,16 +402,4 @@ def test_ggx_floor_matches_quadrature(oracle_lib, material, nee):
     w = h = 16
     st = dict(width=w, height=h, samples_per_pixel=1024, path_depth=2, use_nee=nee, rr_start_depth=100, seed=11)
     hdr, _, _ = o.render(st, s.view)
-    blk = 4
-    img = hdr[:, :3].astype(np.float64).reshape(h // blk, blk, w // blk, blk, 3)
-    got, se = img.mean((1, 3)), img.std((1, 3)) / blk
-    # the closed form at each block's centre (it varies slowly across a block)
-    cam = np.asarray(s.view.position, np.float64)
-    centres = cf.floor_points(s.view, w // blk, h // blk)
-    want = np.zeros_like(got)
-    for i, q in enumerate(centres):
-        wo = cam - q
-        wo /= np.linalg.norm(wo)
-        want.reshape(-1, 3)[i] = cf.ggx_floor_expected(material, np.array([wo[0], wo[2], wo[1]]), q, n_theta=800,
-                                                       n_phi=800)
-    assert (np.abs(got - want) < 5 * se + 3e-3 * want).all(), (got, want, se)
+    cf.check_ggx_floor(hdr, s.view, w, h, material)
