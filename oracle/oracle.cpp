@@ -661,8 +661,15 @@ struct RenderParams {
     F3 origin, llc, horizontal, vertical;
 };
 
+// One traced ray of a logged sample: kind 0 = extension ray (result: prim id
+// bits, t, u, v), 1 = shadow ray (result: occluded, radiance before the clamp).
+struct PathRecord {
+    float kind, o[3], tmin, d[3], tmax, result[4], depth, pad[2];
+};
+
 struct RenderCounters {
     uint64_t ext = 0, shadow = 0;
+    std::vector<PathRecord>* log = nullptr;
 };
 
 // (radiance of one sample) Follows one sample index through the wavefront
@@ -691,6 +698,13 @@ F3 trace_sample(const Scene& s, const RenderParams& rp, uint64_t sample, RenderC
     while (true) {
         rc.ext++;
         const HitRec h = rp.tprim ? closest_tprim(s, ray) : closest_ref(s, ray, nullptr, nullptr);
+        if (rc.log) {
+            PathRecord pr{0.0f, {ray.o.x, ray.o.y, ray.o.z}, ray.tmin, {ray.d.x, ray.d.y, ray.d.z}, ray.tmax,
+                          {0, h.t, h.u, h.v}, (float)depth, {0, 0}};
+            const uint32_t prim = h.tri == kInvalid ? kInvalid : s.tris[h.tri].prim;
+            std::memcpy(&pr.result[0], &prim, 4);
+            rc.log->push_back(pr);
+        }
         if (h.tri == kInvalid) break;
         const Triangle& tr = s.tris[h.tri];
         const Material& m = s.materials[tr.material];
@@ -721,7 +735,12 @@ F3 trace_sample(const Scene& s, const RenderParams& rp, uint64_t sample, RenderC
             const float dist = length(sv);
             const Ray sr{so, sv / dist, kMinT, dist - 1e-4f};
             rc.shadow++;
-            if (!any_ref(s, sr)) {
+            const bool occluded = any_ref(s, sr);
+            if (rc.log)
+                rc.log->push_back(PathRecord{1.0f, {sr.o.x, sr.o.y, sr.o.z}, sr.tmin, {sr.d.x, sr.d.y, sr.d.z}, sr.tmax,
+                                             {occluded ? 1.0f : 0.0f, radiance.x, radiance.y, radiance.z},
+                                             (float)depth, {0, 0}});
+            if (!occluded) {
                 const float l2s = dot(radiance, radiance);
                 if (l2s > 20.0f * 20.0f) radiance = radiance * (20.0f * rsqrt_host(l2s));
                 L = L + radiance;
@@ -1172,9 +1191,7 @@ void orc_camera_basis(float fov_y, float aspect, const float* pos, const float* 
 // accumulated radiance / spp as RGBA float (alpha 0) for those pixels, in
 // double-precision accumulation.  settings: width, height, spp, depth, use_nee,
 // rr_start; camera12 from orc_camera_basis.
-void orc_render(void* h, const uint32_t* settings, uint64_t seed, const float* camera12, uint32_t pixel_begin,
-                uint32_t pixel_end, uint32_t threads, float* hdr, uint64_t* counters2) {
-    const Scene& s = *static_cast<Scene*>(h);
+static RenderParams make_params(const uint32_t* settings, uint64_t seed, const float* camera12) {
     RenderParams rp;
     rp.width = settings[0];
     rp.height = settings[1];
@@ -1193,6 +1210,13 @@ void orc_render(void* h, const uint32_t* settings, uint64_t seed, const float* c
     rp.llc = f3(camera12[3], camera12[4], camera12[5]);
     rp.horizontal = f3(camera12[6], camera12[7], camera12[8]);
     rp.vertical = f3(camera12[9], camera12[10], camera12[11]);
+    return rp;
+}
+
+void orc_render(void* h, const uint32_t* settings, uint64_t seed, const float* camera12, uint32_t pixel_begin,
+                uint32_t pixel_end, uint32_t threads, float* hdr, uint64_t* counters2) {
+    const Scene& s = *static_cast<Scene*>(h);
+    const RenderParams rp = make_params(settings, seed, camera12);
     const uint32_t npix = pixel_end - pixel_begin;
     std::atomic<uint64_t> ext{0}, sh{0};
     parallel_for(npix, threads, [&](uint64_t k) {
@@ -1219,6 +1243,25 @@ void orc_render(void* h, const uint32_t* settings, uint64_t seed, const float* c
         counters2[0] = ext.load();
         counters2[1] = sh.load();
     }
+}
+
+// The path of one sample, as every traced ray with its result (16 floats per
+// record, PathRecord), for comparing a sample's rays with the core's
+// traversal hooks; returns the record count (records beyond `capacity` are
+// dropped) and the sample's radiance in L3.
+uint32_t orc_trace_path(void* h, const uint32_t* settings, uint64_t seed, const float* camera12, uint64_t sample,
+                        float* records, uint32_t capacity, float* L3) {
+    const Scene& s = *static_cast<Scene*>(h);
+    RenderParams rp = make_params(settings, seed, camera12);
+    std::vector<PathRecord> log;
+    RenderCounters rc;
+    rc.log = &log;
+    uint32_t px;
+    const F3 L = trace_sample(s, rp, sample, rc, &px);
+    L3[0] = L.x, L3[1] = L.y, L3[2] = L.z;
+    const uint32_t n = (uint32_t)std::min<size_t>(log.size(), capacity);
+    std::memcpy(records, log.data(), n * sizeof(PathRecord));
+    return (uint32_t)log.size();
 }
 
 // PostprocessKernel, Image.cu:34-54: hdr already divided by spp.

@@ -36,6 +36,7 @@ def lib():
             "orc_camera_basis": (None, [C.c_float, C.c_float, _P, _P, _P]),
             "orc_render": (None, [_P, _P, C.c_uint64, _P, _U32, _U32, _U32, _P, _P]),
             "orc_tonemap": (None, [_P, _U32, _P]),
+            "orc_trace_path": (_U32, [_P, _P, C.c_uint64, _P, C.c_uint64, _P, _U32, _P]),
             "orc_sincos": (None, [_P, _U32, _P, _P]),
             "orc_cosine_hemisphere": (None, [_P, _U32, _P]),
             "orc_ggx_term": (None, [C.c_int, C.c_float, _P, _U32, _P]),
@@ -97,6 +98,8 @@ class OracleScene:
         return (hits, nv, nt) if counts else hits
 
     def any(self, rays, brute=False, threads=None):
+        """Occlusion (any triangle with t in [t_min, t_max)): the reference's
+        traversal, or all triangles (brute)."""
         rays = np.ascontiguousarray(rays, np.float32)
         occ = np.zeros(len(rays), np.uint32)
         lib().orc_trace_any(self.h, _p(rays), len(rays), _p(occ), 1 if brute else 0, threads or default_threads())
@@ -147,6 +150,26 @@ class OracleScene:
         lib().orc_render(self.h, _p(s), settings.get("seed", 1), _p(cam), pixel_begin, pixel_end,
                          threads or default_threads(), _p(out), _p(cnt))
         return out, int(cnt[0]), int(cnt[1])
+
+    def trace_path(self, settings, view, sample, tprim=True):
+        """Every ray of one sample's path: (records, radiance).  records is a
+        structured array: kind (0 extension, 1 shadow), ray (pt_ray layout:
+        origin, t_min, direction, t_max), result (extension: prim id bits, t,
+        u, v; shadow: occluded, radiance before the clamp), depth."""
+        w, h = settings["width"], settings["height"]
+        s = np.array([w, h, settings["samples_per_pixel"], settings["path_depth"], settings.get("use_nee", 1),
+                      settings.get("rr_start_depth", 3), 1 if tprim else 0], np.uint32)
+        cam = camera_basis(view, w / h)
+        rec = np.zeros((256, 16), np.float32)
+        L = np.zeros(3, np.float32)
+        n = lib().orc_trace_path(self.h, _p(s), settings.get("seed", 1), _p(cam), sample, _p(rec), 256, _p(L))
+        rec = rec[:min(n, 256)]
+        out = np.zeros(len(rec), [("kind", "<i4"), ("ray", "<f4", 8), ("result", "<f4", 4), ("depth", "<i4")])
+        out["kind"] = rec[:, 0].astype(np.int32)
+        out["ray"] = rec[:, 1:9]
+        out["result"] = rec[:, 9:13]
+        out["depth"] = rec[:, 13].astype(np.int32)
+        return out, L
 
 
 def camera_basis(view, aspect):

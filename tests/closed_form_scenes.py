@@ -239,7 +239,8 @@ def check_plates(hdr, view, w, h, rho, le, nee, depth, blk=8):
     p = floor_points(view, w, h)
     want = rho * le * plate_form_factor(p[:, 0], p[:, 2])
     got = hdr[:, 0].astype(np.float64)
-    assert np.array_equal(hdr[:, 0], hdr[:, 1]) and np.array_equal(hdr[:, 0], hdr[:, 2])
+    # grey: the channels differ only by the order of their float additions
+    assert np.allclose(hdr[:, 0], hdr[:, 1], rtol=1e-5) and np.allclose(hdr[:, 0], hdr[:, 2], rtol=1e-5)
     if depth == 1 and not nee:
         assert (got == 0).all()
         return

@@ -299,11 +299,55 @@ def test_c2_queue_membership_and_alias(c2_pair, oracle_lib):
     assert np.array_equal(idx, oi) and np.array_equal(pmf, op)
 
 
+def _exact_occlusion(scene, ray):
+    """Whether the segment [t_min, t_max) of `ray` meets any triangle, solved in
+    float64 (Cramer's rule over every triangle)."""
+    P = scene.positions.astype(np.float64)
+    o, d = ray[0:3].astype(np.float64), ray[4:7].astype(np.float64)
+    v0 = P[:, 0]
+    e1, e2 = P[:, 1] - v0, P[:, 2] - v0
+    pv = np.cross(d[None], e2)
+    det = (e1 * pv).sum(1)
+    ok = np.abs(det) > 1e-300
+    inv = np.where(ok, 1.0 / np.where(ok, det, 1.0), 0.0)
+    sv = o[None] - v0
+    u = (sv * pv).sum(1) * inv
+    q = np.cross(sv, e1)
+    v = (q * d[None]).sum(1) * inv
+    t = (e2 * q).sum(1) * inv
+    hit = ok & (u >= 0) & (v >= 0) & (u + v <= 1) & (t >= ray[3]) & (t < ray[7])
+    return bool(hit.any())
+
+
+def _explain_mismatch(scene, r, o, st, pixel):
+    """Replays every ray of the pixel's samples (oracle path log) through the
+    core's hooks: extension rays must hit identically; a shadow ray may differ
+    only where float32 Moller-Trumbore over all triangles disagrees with the
+    exact float64 answer (a false hit or miss of the float test on a triangle's
+    edge, which box culling then keeps or drops depending on the tree)."""
+    explained = 0
+    for c in range(st["samples_per_pixel"]):
+        rec, _ = o.trace_path(st, scene.view, pixel * st["samples_per_pixel"] + c)
+        ext, sh = rec["ray"][rec["kind"] == 0], rec["ray"][rec["kind"] == 1]
+        if len(ext):
+            assert np.array_equal(r.trace_closest(ext), rec["result"][rec["kind"] == 0].view(np.uint32))
+        if len(sh):
+            want = (rec["result"][rec["kind"] == 1][:, 0] > 0).astype(np.uint32)
+            for k in np.nonzero(r.trace_any(sh) != want)[0]:
+                brute32 = bool(o.any(sh[k:k + 1], brute=True)[0])
+                assert brute32 != _exact_occlusion(scene, sh[k]), (pixel, c, sh[k].tolist())
+                explained += 1
+    return explained
+
+
 def test_c2_production_frame_matches_oracle(c2_pair):
     """The benchmarked frame (1920 x 1080, 64 spp, depth 8, NEE, roulette from
     bounce 3): a band of 16 rows through the image centre (30,720 pixels,
-    1,966,080 samples) against the oracle pixel by pixel, relMSE and 3 sigma;
-    and the same view at 240 x 135 whole, with equal ray counts."""
+    1,966,080 samples) against the oracle pixel by pixel, and the same view at
+    240 x 135 whole with equal ray counts.  Pixels agree to the rounding of
+    their float sums except where a shadow ray's float32 triangle test is
+    itself wrong (edge-grazing false hits, decided by box culling): at most
+    1 in 10,000 pixels, each one traced to such a ray."""
     s, r, o = c2_pair
     st = dict(s.settings)
     r.InitializeRenderingBuffers(1920, 1080)
@@ -313,8 +357,11 @@ def test_c2_production_frame_matches_oracle(c2_pair):
     ref, _, _ = o.render(st, s.view, pixel_begin=lo, pixel_end=hi, tprim=True)
     g = hdr[lo:hi, :3]
     assert (ref[:, :3] > 0).mean() > 0.5
-    assert np.allclose(g, ref[:, :3], rtol=2e-5, atol=1e-6)
-    assert _relmse(g, ref[:, :3]) < 1e-10
+    bad = np.nonzero(~np.isclose(g, ref[:, :3], rtol=2e-5, atol=1e-6).all(1))[0]
+    assert len(bad) <= 1e-4 * len(g), len(bad)
+    for k in bad:
+        assert _explain_mismatch(s, r, o, st, lo + int(k)) > 0
+    assert _relmse(g, ref[:, :3]) < 1e-8
     se = math.sqrt(ref[:, :3].mean(1).var() / len(ref))
     assert abs(g.mean() - ref[:, :3].mean()) < 3 * se
     small = dict(st, width=240, height=135)
@@ -322,4 +369,7 @@ def test_c2_production_frame_matches_oracle(c2_pair):
     hdr = r.GetFrameHdr(240, 135).reshape(-1, 4)
     ref, ext, sh = o.render(small, s.view, tprim=True)
     assert stats["extension_rays"] == ext and stats["shadow_rays"] == sh
-    assert np.allclose(hdr[:, :3], ref[:, :3], rtol=2e-5, atol=1e-6)
+    bad = np.nonzero(~np.isclose(hdr[:, :3], ref[:, :3], rtol=2e-5, atol=1e-6).all(1))[0]
+    assert len(bad) <= 1e-4 * len(hdr), len(bad)
+    for k in bad:
+        assert _explain_mismatch(s, r, o, small, int(k)) > 0
