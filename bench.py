@@ -93,6 +93,29 @@ def _rays(st):
     return st["extension_rays"] + st["shadow_rays"]
 
 
+def occupancy_summary(occ):
+    """Queue occupancy of the profiled frame's iterations (pt_get_queue_occupancy):
+    the extension-ray and shadow queues filled against the path pool, the
+    material queues' share of the hits, and each iteration's fills."""
+    import numpy as np
+    if len(occ) == 0:
+        return None
+    cap = int(occ["capacity"][0])
+    ext = occ["extension"].astype(np.float64)
+    mat = occ["material"].sum(0).astype(np.float64)
+    total = max(1.0, ext.sum())
+    return {
+        "capacity": cap, "iterations": int(len(occ)),
+        "ray_fill_mean": round(float(ext.mean() / cap), 4), "ray_fill_min": round(float(ext.min() / cap), 4),
+        "shadow_fill_mean": round(float(occ["shadow"].mean() / cap), 4),
+        "share": {"diffuse": round(float(mat[0] / total), 4), "ggx": round(float(mat[1] / total), 4),
+                  "mirror": round(float(mat[2] / total), 4), "emissive": round(float(mat[3] / total), 4),
+                  "missed": round(float(occ["missed"].sum()) / total, 4)},
+        "ray_fill": [round(float(x) / cap, 4) for x in ext],
+        "shadow_fill": [round(float(x) / cap, 4) for x in occ["shadow"]],
+    }
+
+
 def sequential_then_batched(r, items, steps, log_prefix):
     """Device time of rendering `items` one view at a time (plain rendering,
     the reference's one-request model) and all in one batch.  Returns
@@ -481,6 +504,11 @@ def main():
 
     # live per-kernel timing (hipEvents on the renderer's stream) and visit counts
     prof = r.RenderFrame(settings, view, ptcore.FRAME_PROFILE)
+    occupancy = occupancy_summary(r.queue_occupancy())
+    if occupancy:
+        log(f"queue occupancy: {occupancy['iterations']} iterations, ray queue {occupancy['ray_fill_mean']:.3f} "
+            f"of {occupancy['capacity']} slots on average (min {occupancy['ray_fill_min']:.3f}), shadow queue "
+            f"{occupancy['shadow_fill_mean']:.3f}; shares {occupancy['share']}")
     inst = r.RenderFrame(settings, view, ptcore.FRAME_INSTRUMENT)
     ext = inst["extension_rays"]
     nodes_per_ray = inst["closest_nodes"] / max(1, ext)
@@ -507,8 +535,15 @@ def main():
 
     def roof(kind, kernel, achieved_gbs, bpr, nodes, tris, avg_ms, rays_per_launch, node_bytes):
         traffic = measured_traffic(kind)
+        # the same launch's measured HBM bytes (rocprofv3 counters, per launch)
+        # over its live duration: what HBM actually carried
+        hbm = None
+        if traffic:
+            hbm = traffic["hbm_bytes_per_ray"] * rays_per_launch / (avg_ms * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
+                "hbm_achieved": round(hbm, 1) if hbm else None,
+                "hbm_frac": round(hbm / HBM_PEAK_GBS, 4) if hbm else None,
                 "traffic": traffic["hbm_bytes_per_launch"] if traffic else None, "traffic_detail": traffic,
                 "kernel": kernel, "bytes_per_ray": round(bpr, 1), "node_bytes": node_bytes,
                 "nodes_per_ray": round(nodes, 3), "tris_per_ray": round(tris, 3),
@@ -549,7 +584,7 @@ def main():
                    "triangles": scene.triangle_count, "width": settings["width"], "height": settings["height"],
                    "spp": settings["samples_per_pixel"], "path_depth": settings["path_depth"],
                    "rays_per_frame": rays // a.steps, "device_ms_per_frame": round(dev_ms / a.steps, 3)},
-        "roofline": roofline, other[0]: other[1], "cpu_baseline": cpu,
+        "roofline": roofline, other[0]: other[1], "cpu_baseline": cpu, "queue_occupancy": occupancy,
     }
     if serving:
         out["p50_ms"] = serving["rates"][0]["p50_ms"]

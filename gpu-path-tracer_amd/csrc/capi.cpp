@@ -289,6 +289,14 @@ pt_status pt_trace_closest_counts(pt_renderer* r, const pt_ray* rays, uint32_t c
     });
 }
 
+pt_status pt_trace_any_counts(pt_renderer* r, const pt_ray* rays, uint32_t count, uint32_t* occluded,
+                              uint32_t* nodes_visited, uint32_t* tris_tested) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!nodes_visited || !tris_tested) return x.fail(PT_ERROR_INVALID_ARGUMENT, "count arrays must not be NULL");
+        return x.TraceBatch(rays, count, true, false, occluded, nodes_visited, tris_tested, nullptr);
+    });
+}
+
 pt_status pt_classify_hits(pt_renderer* r, const pt_ray* rays, uint32_t count, uint32_t* queue_of_ray,
                            uint32_t* queue_counts) {
     return guarded(r, [&](pt::Renderer& x) { return x.Classify(rays, count, queue_of_ray, queue_counts); });
@@ -306,6 +314,13 @@ pt_status pt_get_bvh_info(pt_renderer* r, pt_bvh_info* info) {
     return guarded(r, [&](pt::Renderer& x) {
         if (!info) return x.fail(PT_ERROR_INVALID_ARGUMENT, "info is NULL");
         return x.GetBvhInfo(*info);
+    });
+}
+
+pt_status pt_get_queue_occupancy(pt_renderer* r, pt_queue_occupancy* iterations, uint32_t capacity, uint32_t* count) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!count) return x.fail(PT_ERROR_INVALID_ARGUMENT, "count is NULL");
+        return x.GetQueueOccupancy(iterations, capacity, *count);
     });
 }
 

@@ -627,6 +627,7 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
             }
         } g7{ev};
         int cur = 0;
+        queue_history_.clear();
         for (uint64_t it = 0;; it++) {
             if (it > max_iters) return fail(PT_ERROR_DEVICE, "wavefront did not terminate");
             PT_TRY_HIP(hipEventRecord(ev[0], stream_), "event");
@@ -642,6 +643,8 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
             PT_TRY_HIP(hipEventRecord(ev[4], stream_), "event");
             PT_TRY_HIP(launch_generate(stream_, p, false, cur ^ 1, grid, 256), "regenerate launch");
             PT_TRY_HIP(hipEventRecord(ev[5], stream_), "event");
+            // the iteration's queue sizes, before the prepare kernel resets them
+            PT_TRY_HIP(hipMemcpyAsync(h_ctr_ring_ + 1, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream_), "counters");
             PT_TRY_HIP(launch_prepare(stream_, p, cur), "prepare launch");
             PT_TRY_HIP(hipMemcpyAsync(h_ctr_ring_, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream_), "counters");
             PT_TRY_HIP(hipEventRecord(ev[6], stream_), "event");
@@ -654,6 +657,17 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
             shadow_ms += ms[3];
             regen_ms += ms[4];
             closest_launches++;
+            {
+                const Counters& q = h_ctr_ring_[1];
+                pt_queue_occupancy o{};
+                o.extension = q.ray[cur];
+                for (int k = 0; k < 4; k++) o.material[k] = q.queue[k];
+                o.missed = q.queue[4];
+                o.shadow = q.shadow[cur];
+                o.next = h_ctr_ring_[0].ray[cur ^ 1];
+                o.capacity = pool_capacity_;
+                if (o.extension) queue_history_.push_back(o);
+            }
             done_samples_.store(h_ctr_ring_[0].launched, std::memory_order_relaxed);
             cur ^= 1;
             if (h_ctr_ring_[0].active == 0) break;
@@ -1149,6 +1163,12 @@ pt_status Renderer::GetBvh4Quantized(void* nodes, uint32_t node_cap) {
     PT_TRY_HIP(hipMemcpyAsync(nodes, rs.qnodes.as<void>(), nn * sizeof(Bvh4QNode), hipMemcpyDeviceToHost, stream_),
                "quantized BVH nodes");
     PT_TRY_HIP(hipStreamSynchronize(stream_), "BVH read-back");
+    return PT_OK;
+}
+
+pt_status Renderer::GetQueueOccupancy(pt_queue_occupancy* out, uint32_t cap, uint32_t& count) const {
+    count = (uint32_t)queue_history_.size();
+    if (out) std::memcpy(out, queue_history_.data(), std::min<size_t>(cap, queue_history_.size()) * sizeof(*out));
     return PT_OK;
 }
 

@@ -116,6 +116,7 @@ public:
     pt_status AliasSample(const float* u, uint32_t n, uint32_t* index, float* pmf);
     pt_status GetAliasTable(pt_alias_bin* bins, uint32_t cap, uint32_t* count);
     pt_status GetBvhInfo(pt_bvh_info& info);
+    pt_status GetQueueOccupancy(pt_queue_occupancy* out, uint32_t cap, uint32_t& count) const;
     pt_status GetBvh4(void* nodes, uint32_t node_cap, uint32_t* order, uint32_t order_cap);
     pt_status GetBvh4Quantized(void* nodes, uint32_t node_cap);
     pt_status GetTuning(pt_tuning& t) const;
@@ -207,6 +208,7 @@ private:
     DeviceBuffer d_ctr_, d_fp_, d_work_, d_overflow_;
     FrameParams* h_fp_ = nullptr;           // pinned staging
     Counters* h_ctr_ring_ = nullptr;        // pinned, kRing entries
+    std::vector<pt_queue_occupancy> queue_history_;   // iterations of the last profiled frame
     static constexpr int kRing = 4;
     static constexpr int kGraphIterations = 4;   // bounces per graph launch (even: parity returns to 0)
     hipEvent_t ring_events_[kRing] = {};

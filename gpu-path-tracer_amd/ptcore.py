@@ -120,6 +120,15 @@ class BvhInfo(C.Structure):
                 ("stack_lds", C.c_uint32), ("stack_total", C.c_uint32), ("node_width", C.c_uint32)]
 
 
+class QueueOccupancy(C.Structure):
+    _fields_ = [("extension", C.c_uint32), ("material", C.c_uint32 * 4), ("missed", C.c_uint32),
+                ("shadow", C.c_uint32), ("next", C.c_uint32), ("capacity", C.c_uint32)]
+
+
+QUEUE_OCCUPANCY_DTYPE = np.dtype([("extension", "<u4"), ("material", "<u4", 4), ("missed", "<u4"),
+                                  ("shadow", "<u4"), ("next", "<u4"), ("capacity", "<u4")])
+
+
 class Tuning(C.Structure):
     _fields_ = [("trace_block", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("shade_block", C.c_uint32),
                 ("pool_size", C.c_uint32), ("use_graph", C.c_uint32), ("bvh_max_leaf", C.c_uint32),
@@ -175,6 +184,8 @@ SIGNATURES = {
     "pt_trace_closest_device": (C.c_int, [_P, _P, _U32, _P, _P]),
     "pt_trace_any_device": (C.c_int, [_P, _P, _U32, _P, _P]),
     "pt_trace_closest_counts": (C.c_int, [_P, _P, _U32, _P, _P, _P]),
+    "pt_trace_any_counts": (C.c_int, [_P, _P, _U32, _P, _P, _P]),
+    "pt_get_queue_occupancy": (C.c_int, [_P, _P, _U32, C.POINTER(C.c_uint32)]),
     "pt_classify_hits": (C.c_int, [_P, _P, _U32, _P, _P]),
     "pt_alias_sample": (C.c_int, [_P, _P, _U32, _P, _P]),
     "pt_get_alias_table": (C.c_int, [_P, _P, _U32, C.POINTER(_U32)]),
@@ -509,6 +520,14 @@ class Renderer:
                                                      _ptr(tris)))
         return hits, nodes, tris
 
+    def trace_any_counts(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        occ = np.zeros(len(rays), np.uint32)
+        nodes = np.zeros(len(rays), np.uint32)
+        tris = np.zeros(len(rays), np.uint32)
+        self._check(self.lib.pt_trace_any_counts(self.h, _ptr(rays), len(rays), _ptr(occ), _ptr(nodes), _ptr(tris)))
+        return occ, nodes, tris
+
     def classify(self, rays):
         rays = np.ascontiguousarray(rays, np.float32)
         q = np.zeros(len(rays), np.uint32)
@@ -535,6 +554,16 @@ class Renderer:
         b = BvhInfo()
         self._check(self.lib.pt_get_bvh_info(self.h, C.byref(b)))
         return {k: getattr(b, k) for k, _ in b._fields_}
+
+    def queue_occupancy(self):
+        """Per-iteration queue sizes of the last FRAME_PROFILE frame
+        (QUEUE_OCCUPANCY_DTYPE records)."""
+        n = C.c_uint32(0)
+        self._check(self.lib.pt_get_queue_occupancy(self.h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, QUEUE_OCCUPANCY_DTYPE)
+        if n.value:
+            self._check(self.lib.pt_get_queue_occupancy(self.h, _ptr(out), n.value, C.byref(n)))
+        return out
 
     def bvh4(self):
         info = self.bvh_info()

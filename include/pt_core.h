@@ -453,6 +453,10 @@ pt_status pt_trace_any_device(pt_renderer* renderer, const pt_ray* rays_device, 
 /* Per-ray visit counts of the closest-hit traversal (the algorithmic-byte model). */
 pt_status pt_trace_closest_counts(pt_renderer* renderer, const pt_ray* rays, uint32_t count,
                                   pt_hit* hits, uint32_t* nodes_visited, uint32_t* tris_tested);
+/* The same for the any-hit traversal (the shadow rays' node kind, quantized by
+ * default): occlusion and per-ray node and triangle visits. */
+pt_status pt_trace_any_counts(pt_renderer* renderer, const pt_ray* rays, uint32_t count,
+                              uint32_t* occluded, uint32_t* nodes_visited, uint32_t* tris_tested);
 /* Queue membership of the closest-hit push (Intersect.cu:115-122): queue_of_ray[i] is a
  * PT_QUEUE_* value; queue_counts[PT_QUEUE_COUNT] receives the queue sizes. */
 pt_status pt_classify_hits(pt_renderer* renderer, const pt_ray* rays, uint32_t count,
@@ -464,6 +468,20 @@ pt_status pt_alias_sample(pt_renderer* renderer, const float* u, uint32_t count,
 pt_status pt_get_alias_table(pt_renderer* renderer, pt_alias_bin* bins, uint32_t capacity,
                              uint32_t* count);
 pt_status pt_get_bvh_info(pt_renderer* renderer, pt_bvh_info* info);
+
+/* Queue occupancy of one wavefront iteration: what the reference reads back
+ * every iteration to size the next one (Renderer.cpp:587-600), here kept for
+ * each iteration of the last frame rendered with PT_FRAME_PROFILE. */
+typedef struct pt_queue_occupancy {
+    uint32_t extension;          /* rays traced by the iteration's closest-hit launch */
+    uint32_t material[4];        /* entries of the diffuse, GGX, mirror and emissive queues */
+    uint32_t missed;             /* rays that left the scene (their paths end) */
+    uint32_t shadow;             /* shadow rays pushed by the shade */
+    uint32_t next;               /* rays queued for the next iteration: continuing plus new camera paths */
+    uint32_t capacity;           /* path slots, the capacity of every queue */
+} pt_queue_occupancy;
+pt_status pt_get_queue_occupancy(pt_renderer* renderer, pt_queue_occupancy* iterations, uint32_t capacity,
+                                 uint32_t* count);
 /* Raw BVH4 nodes (128 B each) and triangle order, for external traversal checks. */
 pt_status pt_get_bvh4(pt_renderer* renderer, void* nodes, uint32_t node_capacity,
                       uint32_t* prim_order, uint32_t prim_capacity);

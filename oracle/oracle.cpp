@@ -886,6 +886,87 @@ inline float safe_rcp(float x) {
     return 1.0f / ax;
 }
 
+// Scale 2^(e - 127) of a quantized node's axis, and byte k of a bound row.
+inline float qscale(uint32_t e) {
+    const uint32_t b = e << 23;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+inline float qbyte(uint32_t row, int k) { return (float)((row >> (8 * k)) & 0xFFu); }
+
+bool bvh4q_any(const Scene& s, const std::vector<uint32_t>& by_prim, const uint32_t* qnodes, const uint32_t* order,
+               const Ray& ray, Bvh4Walk& w) {
+    const F3 rd = f3(safe_rcp(ray.d.x), safe_rcp(ray.d.y), safe_rcp(ray.d.z));
+    const F3 ord = ray.o * rd;
+    std::vector<uint32_t> stack;
+    uint32_t ref = 0;
+    const float inf = std::numeric_limits<float>::infinity();
+    while (true) {
+        if (!(ref & 0x80000000u)) {
+            // 16 words: origin x, y, z | scale exponents | lo x, lo y, lo z, hi x | hi y, hi z, child 0, 1 | child 2, 3
+            const uint32_t* q = qnodes + 16 * (size_t)ref;
+            w.nodes++;
+            float o[3];
+            std::memcpy(o, q, sizeof(o));
+            const float ax = qscale(q[3] & 0xFFu) * rd.x, ay = qscale((q[3] >> 8) & 0xFFu) * rd.y,
+                        az = qscale((q[3] >> 16) & 0xFFu) * rd.z;
+            const float bx = std::fma(o[0], rd.x, -ord.x), by = std::fma(o[1], rd.y, -ord.y),
+                        bz = std::fma(o[2], rd.z, -ord.z);
+            // the near plane of every child is its lo row where the direction is >= 0
+            const bool px = rd.x >= 0.0f, py = rd.y >= 0.0f, pz = rd.z >= 0.0f;
+            const uint32_t nx = px ? q[4] : q[7], fx = px ? q[7] : q[4];
+            const uint32_t ny = py ? q[5] : q[8], fy = py ? q[8] : q[5];
+            const uint32_t nz = pz ? q[6] : q[9], fz = pz ? q[9] : q[6];
+            uint32_t c[4] = {q[10], q[11], q[12], q[13]};
+            float t0[4];
+            for (int k = 0; k < 4; k++) {
+                const float tnx = std::fma(qbyte(nx, k), ax, bx), tfx = std::fma(qbyte(fx, k), ax, bx);
+                const float tny = std::fma(qbyte(ny, k), ay, by), tfy = std::fma(qbyte(fy, k), ay, by);
+                const float tnz = std::fma(qbyte(nz, k), az, bz), tfz = std::fma(qbyte(fz, k), az, bz);
+                const float tn = std::fmax(std::fmax(tnx, tny), std::fmax(tnz, ray.tmin));
+                const float tf = std::fmin(std::fmin(tfx, tfy), std::fmin(tfz, ray.tmax));
+                t0[k] = tn <= tf ? tn : inf;
+            }
+            auto cs = [&](int a, int b) {
+                if (t0[b] < t0[a]) {
+                    std::swap(t0[a], t0[b]);
+                    std::swap(c[a], c[b]);
+                }
+            };
+            cs(0, 1);
+            cs(2, 3);
+            cs(0, 2);
+            cs(1, 3);
+            cs(1, 2);
+            uint32_t next = kInvalid;
+            for (int k = 3; k >= 0; k--) {
+                if (t0[k] != inf && c[k] != kInvalid) {
+                    if (next != kInvalid) stack.push_back(next);
+                    next = c[k];
+                }
+            }
+            w.max_stack = std::max(w.max_stack, (uint32_t)stack.size());
+            if (next != kInvalid) {
+                ref = next;
+                continue;
+            }
+        } else {
+            const uint32_t first = (ref & 0x7FFFFFFFu) >> 4, cnt = (ref & 15u) + 1u;
+            w.tris += cnt;
+            bool hit = false;
+            for (uint32_t k = 0; k < cnt; k++) {
+                float t, u, v;
+                if (mt(s.tris[by_prim[order[first + k]]], ray.o, ray.d, ray.tmin, t, u, v) && t < ray.tmax) hit = true;
+            }
+            if (hit) return true;
+        }
+        if (stack.empty()) return false;
+        ref = stack.back();
+        stack.pop_back();
+    }
+}
+
 HitRec bvh4_closest(const Scene& s, const std::vector<uint32_t>& by_prim, const uint32_t* nodes, const uint32_t* order,
                     const Ray& ray, Bvh4Walk& w) {
     const F3 rd = f3(safe_rcp(ray.d.x), safe_rcp(ray.d.y), safe_rcp(ray.d.z));
@@ -1120,6 +1201,23 @@ void orc_classify(void* h, const float* rays, uint32_t n, uint32_t* queue, uint3
 }
 
 // Closest hit over the core's BVH4 (nodes: N x 32 u32, order: BVH slot -> prim).
+// Any hit over the core's 64-byte quantized BVH4 in its kernel's visit order
+// (nearest child first, the farther ones pushed; a hit ends the ray after its
+// leaf), counting node steps and triangle tests per ray: the visits behind the
+// any-hit traversal's algorithmic bytes.
+void orc_bvh4q_any(void* h, const uint32_t* qnodes, const uint32_t* order, const float* rays, uint32_t n,
+                   uint32_t* occluded, uint32_t* nodes_visited, uint32_t* tris_tested, uint32_t threads) {
+    const Scene& s = *static_cast<Scene*>(h);
+    parallel_for(n, threads, [&](uint64_t i) {
+        const float* r = rays + 8 * i;
+        const Ray ray{f3(r[0], r[1], r[2]), f3(r[4], r[5], r[6]), r[3], r[7]};
+        Bvh4Walk w;
+        occluded[i] = bvh4q_any(s, s.by_prim, qnodes, order, ray, w) ? 1u : 0u;
+        if (nodes_visited) nodes_visited[i] = w.nodes;
+        if (tris_tested) tris_tested[i] = w.tris;
+    });
+}
+
 void orc_bvh4_closest(void* h, const uint32_t* nodes, const uint32_t* order, const float* rays, uint32_t n, void* hits,
                       uint32_t* nodes_visited, uint32_t* tris_tested, uint32_t* max_stack, uint32_t threads) {
     const Scene& s = *static_cast<Scene*>(h);

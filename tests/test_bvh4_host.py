@@ -121,3 +121,35 @@ def test_quantized_nodes_contain_the_fp32_boxes(make):
     # the byte grid is fine: decoded boxes exceed the fp32 ones by at most one step (plus rounding)
     slack = np.maximum(s3, np.spacing(np.abs(origin)) * 2)
     assert ((flo - lo)[valid] <= (slack + 0 * lo)[valid] * 1.0001).all()
+
+
+@pytest.mark.parametrize("make", [scenes.cornell_box, lambda: scenes.spheres_scene(count=4, subdivisions=4, lights=64)])
+def test_quantized_any_hit_walk(make, oracle_lib):
+    """The oracle's walk of the quantized BVH4 in the any-hit kernel's order
+    (whose per-ray visits the roofline's bytes count): occlusion equal to brute
+    force on random segments and on shadow segments to the emitters; a ray
+    missing the root box costs one node and no triangle; an unoccluded ray,
+    which cannot stop early, tests every triangle of the leaves it reaches at
+    least as often as the closest-hit walk of the fp32 tree, whose boxes it
+    contains."""
+    s = make()
+    nodes, order, depth = ptcore.build_bvh4(s.positions)
+    q = ptcore.quantize_bvh4(nodes)
+    o = oracle_lib.OracleScene(s)
+    lo, hi = s.bounds()
+    rays = scenes.random_rays(20000, lo - 1, hi + 1, 78)
+    rays[:, 7] = np.random.default_rng(3).uniform(0.01, float(np.linalg.norm(hi - lo)), len(rays)).astype(np.float32)
+    occ, nv, nt = o.bvh4q_any(q, order, rays)
+    assert np.array_equal(occ, o.any(rays, brute=True))
+    assert 0.05 < occ.mean() < 0.95 and nv.min() >= 1
+    sh = scenes.shadow_rays(s, 20000, 79)
+    occ_s, _, _ = o.bvh4q_any(q, order, sh)
+    assert np.array_equal(occ_s, o.any(sh, brute=True))
+    away = np.zeros((1, 8), np.float32)
+    away[0, :3] = hi + 5
+    away[0, 3], away[0, 4:7], away[0, 7] = 1e-4, 0.57735026, 1e34
+    occ1, nv1, nt1 = o.bvh4q_any(q, order, away)
+    assert occ1[0] == 0 and nv1[0] == 1 and nt1[0] == 0
+    free = occ == 0
+    _, cnv, cnt, _ = o.bvh4_closest(nodes, order, rays[free])
+    assert (nt[free] >= cnt).all() and (nv[free] >= cnv).all()

@@ -91,7 +91,7 @@ _LAYOUTS = {
     "pt_frame_stats": ptcore.FrameStats, "pt_alias_bin": ptcore.AliasBin, "pt_bvh_info": ptcore.BvhInfo,
     "pt_tuning": ptcore.Tuning, "pt_server_config": ptcore.ServerConfig, "pt_request_info": ptcore.RequestInfo,
     "pt_server_stats": ptcore.ServerStats, "pt_guide_settings": ptcore.GuideSettings,
-    "pt_guide_stats": ptcore.GuideStats,
+    "pt_guide_stats": ptcore.GuideStats, "pt_queue_occupancy": ptcore.QueueOccupancy,
 }
 
 

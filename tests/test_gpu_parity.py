@@ -212,6 +212,43 @@ def test_stack_overflow_is_reported(gpu_renderer, cornell, cornell_oracle):
     r.RenderFrame(st, cornell.view)
 
 
+@pytest.mark.parametrize("make", [scenes.cornell_box, scenes.specular_room])
+def test_queue_occupancy_per_bounce_equals_the_oracle(gpu_renderer, oracle_lib, make):
+    """Per-iteration queue sizes of a profiled frame (the reference's
+    per-iteration counter readback, Renderer.cpp:587-600).  With every sample
+    in the pool at once, iteration k holds bounce k of every path, so the
+    extension-ray count, the four material queues, the missed rays and the
+    shadow rays of each iteration equal the oracle's per-bounce counts over the
+    logged paths of all 4,096 samples."""
+    s = make()
+    r = gpu_renderer
+    r.InitializeSceneBuffers(s)
+    st = dict(width=32, height=32, samples_per_pixel=4, path_depth=8, use_nee=1, rr_start_depth=3, seed=31)
+    stats = r.RenderFrame(st, s.view, ptcore.FRAME_PROFILE)
+    occ = r.queue_occupancy()
+    assert occ["extension"].sum() == stats["extension_rays"] and occ["shadow"].sum() == stats["shadow_rays"]
+    assert (occ["material"].sum(1) + occ["missed"] == occ["extension"]).all()
+    assert (occ["next"][:-1] == occ["extension"][1:]).all() and occ["next"][-1] == 0
+    o = oracle_lib.OracleScene(s)
+    qmap = {scenes.MODEL_DIFFUSE: 0, scenes.MODEL_GGX: 1, scenes.MODEL_MIRROR: 2, scenes.MODEL_EMISSIVE: 3}
+    want = np.zeros((len(occ), 6), np.int64)     # 4 material queues, missed, shadow
+    for sample in range(32 * 32 * 4):
+        rec, _ = o.trace_path(st, s.view, sample)
+        for k in range(len(rec)):
+            d = rec["depth"][k]
+            if rec["kind"][k] == 1:
+                want[d, 5] += 1
+                continue
+            prim = rec["result"][k].view(np.uint32)[0]
+            if prim == ptcore.INVALID:
+                want[d, 4] += 1
+            else:
+                want[d, qmap[int(s.materials["model"][s.material_ids[prim]])]] += 1
+    got = np.concatenate([occ["material"], occ["missed"][:, None], occ["shadow"][:, None]], 1)
+    assert np.array_equal(got, want), (got, want)
+    assert occ["extension"][0] == 32 * 32 * 4 and (occ["capacity"] >= 4096).all()
+
+
 @pytest.fixture(scope="module")
 def c2_pair(oracle_lib):
     s = scenes.production_scene()
@@ -279,6 +316,29 @@ def test_c2_any_hit_matches_oracle(c2_pair):
     rays = scenes.random_rays(1 << 20, (-10, -10, -10), (110, 110, 110), 204)
     rays[:, 7] = np.random.default_rng(2).uniform(0.1, 150.0, len(rays)).astype(np.float32)
     assert np.array_equal(r.trace_any(rays), o.any(rays))
+
+
+def test_c2_visit_counts_equal_the_oracle_walk(c2_pair):
+    """The node and triangle visits behind the roofline's algorithmic bytes:
+    per ray, the kernels' COUNT counters equal the oracle's walk of the same
+    uploaded trees in the same order, for any hit over the 64-byte quantized
+    nodes (262,144 shadow segments to the emitters) and closest hit over the
+    128-byte nodes (262,144 random rays); the any-hit results equal brute force
+    on a subset."""
+    s, r, o = c2_pair
+    nodes, order = r.bvh4()
+    q = r.bvh4_quantized()
+    sh = scenes.shadow_rays(s, 1 << 18, 205)
+    occ, nv, nt = r.trace_any_counts(sh)
+    oocc, onv, ont = o.bvh4q_any(q, order, sh)
+    assert np.array_equal(occ, oocc) and np.array_equal(nv, onv) and np.array_equal(nt, ont)
+    assert np.array_equal(occ[:512], o.any(sh[:512], brute=True))
+    rays = scenes.random_rays(1 << 18, (-10, -10, -10), (110, 110, 110), 206)
+    hits, nv, nt = r.trace_closest_counts(rays)
+    ohits, onv, ont, _ = o.bvh4_closest(nodes, order, rays)
+    assert np.array_equal(hits, ohits) and np.array_equal(nv, onv) and np.array_equal(nt, ont)
+    print(f"any hit: {nv.mean():.3f} nodes, {nt.mean():.3f} triangles per shadow segment; "
+          f"closest hit: {onv.mean():.3f} nodes, {ont.mean():.3f} triangles per random ray")
 
 
 def test_c2_queue_membership_and_alias(c2_pair, oracle_lib):
