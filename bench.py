@@ -89,6 +89,140 @@ def measured_traffic(kind):
         return json.load(f)
 
 
+# ---------------------------------------------------------------- rocprofv3 summaries
+# The profiles/ summaries are written by these functions from the CSV output of
+# rocprofv3 -f csv runs of this script (commands in each summary).
+
+def _short(name):
+    """Kernel name without its argument list."""
+    name = name.strip()
+    if name.startswith("void "):
+        name = name[5:]
+    depth = 0
+    for i, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return name[:i]
+    return name
+
+
+def _csvs(d, suffix):
+    import glob
+    return sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True))
+
+
+def summarize_kernel_trace(d, command, title):
+    """Markdown table (calls, total, average, share) per kernel from the
+    *_kernel_trace.csv of a rocprofv3 --kernel-trace run in directory d."""
+    import csv
+    rows = {}
+    for path in _csvs(d, "_kernel_trace.csv"):
+        with open(path) as f:
+            for rec in csv.DictReader(f):
+                k = _short(rec["Kernel_Name"])
+                dt = (int(rec["End_Timestamp"]) - int(rec["Start_Timestamp"])) * 1e-3   # us
+                c = rows.setdefault(k, [0, 0.0])
+                c[0] += 1
+                c[1] += dt
+    total = sum(v[1] for v in rows.values()) or 1.0
+    out = [f"# {title}", "", "Command (MI355X, from the repo root, `export TMPDIR=/tmp`):", "", "```", command, "```", "",
+           "| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
+    for k, (n, t) in sorted(rows.items(), key=lambda kv: -kv[1][1]):
+        out.append(f"| `{k}` | {n} | {t / 1e3:.2f} | {t / n:.1f} | {100 * t / total:.1f} |")
+    return "\n".join(out) + "\n"
+
+
+def _pmc_table(d, min_us=20.0):
+    """{kernel: {counter: mean per dispatch, "us": mean duration}} over the
+    dispatches longer than min_us of the *_counter_collection.csv in d."""
+    import csv
+    acc = {}
+    for path in _csvs(d, "_counter_collection.csv"):
+        with open(path) as f:
+            for rec in csv.DictReader(f):
+                us = (int(rec["End_Timestamp"]) - int(rec["Start_Timestamp"])) * 1e-3
+                if us < min_us:
+                    continue
+                k = _short(rec["Kernel_Name"])
+                key = (rec["Dispatch_Id"], rec["Counter_Name"])
+                a = acc.setdefault(k, {"_disp": {}, "_us": {}})
+                a["_disp"][key] = a["_disp"].get(key, 0.0) + float(rec["Counter_Value"])
+                a["_us"][rec["Dispatch_Id"]] = us
+    out = {}
+    for k, a in acc.items():
+        sums, counts = {}, {}
+        for (_, c), v in a["_disp"].items():
+            sums[c] = sums.get(c, 0.0) + v
+            counts[c] = counts.get(c, 0) + 1
+        row = {c: sums[c] / counts[c] for c in sums}
+        row["us"] = sum(a["_us"].values()) / len(a["_us"])
+        row["dispatches"] = len(a["_us"])
+        out[k] = row
+    return out
+
+
+def summarize_pmc(cal_fetch, cal_write, dirs, bench_json, command):
+    """The PMC summary of a workload: per kernel HBM bytes (FETCH_SIZE and
+    WRITE_SIZE scaled by the same counters of a 1 GiB device-to-device copy),
+    L2 hit rate and VALU use, from one rocprofv3 --pmc run per counter group;
+    returns (markdown, {"any": traffic, "closest": traffic}) where traffic is
+    the per-launch record bench.py reads into roofline.traffic."""
+    copy_kib = (1 << 30) / 1024.0
+
+    def cal(d, counter):
+        t = _pmc_table(d, min_us=50.0)
+        v = max(row.get(counter, 0.0) for row in t.values())
+        return copy_kib / v
+    fs, ws = cal(cal_fetch, "FETCH_SIZE"), cal(cal_write, "WRITE_SIZE")
+    rows = {}
+    for d in dirs:
+        for k, row in _pmc_table(d).items():
+            rows.setdefault(k, {}).update(row)
+    with open(bench_json) as f:
+        line = json.loads([l for l in f if l.startswith("{")][-1])
+    rpl = {}
+    for key in ("roofline", "roofline_closest", "roofline_any"):
+        if key in line and line[key]:
+            rpl["any" if "any" in line[key]["kernel"] else "closest"] = line[key]
+    md = ["| kernel | avg us | HBM read MB | HBM write MB | L2 hit | VALU lane util | VALU issue / CU / cycle |",
+          "|---|---:|---:|---:|---:|---:|---:|"]
+    traffic = {}
+    for k, r in sorted(rows.items(), key=lambda kv: -kv[1].get("us", 0) * kv[1].get("dispatches", 0)):
+        rd = r.get("FETCH_SIZE", 0.0) * fs * 1024
+        wr = r.get("WRITE_SIZE", 0.0) * ws * 1024
+        hit, miss = r.get("TCC_HIT_sum"), r.get("TCC_MISS_sum")
+        l2 = hit / (hit + miss) if hit is not None and miss is not None and hit + miss > 0 else None
+        lane = (r["SQ_THREAD_CYCLES_VALU"] / (64 * r["SQ_ACTIVE_INST_VALU"])
+                if r.get("SQ_THREAD_CYCLES_VALU") and r.get("SQ_ACTIVE_INST_VALU") else None)
+        issue = (r["SQ_ACTIVE_INST_VALU"] / (256 * r["GRBM_GUI_ACTIVE"] / 8)
+                 if r.get("SQ_ACTIVE_INST_VALU") and r.get("GRBM_GUI_ACTIVE") else None)
+        fmt = lambda x, n=2: "-" if x is None else f"{x:.{n}f}"
+        md.append(f"| `{k}` | {r.get('us', 0):.1f} | {rd / 1e6:.1f} | {wr / 1e6:.1f} | {fmt(l2)} | {fmt(lane)} | {fmt(issue)} |")
+        kind = ("any" if "WavefrontShadowSrc" in k else "closest" if "WavefrontClosestSrc" in k else None)
+        if kind and ", false, false, pt::" in k and kind in rpl:
+            rays = rpl[kind]["rays_per_launch"]
+            traffic[kind] = {"kernel": k, "hbm_bytes_per_launch": int(rd + wr), "hbm_read_bytes_per_launch": int(rd),
+                             "hbm_write_bytes_per_launch": int(wr), "fetch_scale": round(fs, 4),
+                             "write_scale": round(ws, 4), "rays_per_launch": rays,
+                             "hbm_bytes_per_ray": round((rd + wr) / rays, 1),
+                             "algorithmic_bytes_per_ray": rpl[kind]["bytes_per_ray"],
+                             "l2_hit": None if l2 is None else round(l2, 3),
+                             "source": "profiles/pmc_production_1080p64spp.md"}
+    head = ["# PMC counters: production frame (983,042 triangles, 1920x1080, 64 spp)", "",
+            "One counter group per run (`bench.py --summarize-pmc` wrote this file from their CSV output):", "",
+            "```", command, "```", "",
+            f"Copy calibration: FETCH_SIZE x {fs:.3f}, WRITE_SIZE x {ws:.3f} KiB (a 1 GiB device-to-device copy, "
+            "`bench.py --calibrate-copy`, under the same counters).", "",
+            "Averages per dispatch over the dispatches longer than 20 us (counter runs serialize dispatches).", "",
+            "Derived columns: HBM read = FETCH_SIZE x scale KiB, HBM write = WRITE_SIZE x scale KiB; L2 hit = "
+            "TCC_HIT / (TCC_HIT + TCC_MISS); VALU lane util = SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU); "
+            "VALU issue per CU per cycle = SQ_ACTIVE_INST_VALU / (256 CUs x GRBM_GUI_ACTIVE / 8 XCDs).", ""]
+    return "\n".join(head + md) + "\n", traffic
+
+
 def _rays(st):
     return st["extension_rays"] + st["shadow_rays"]
 
@@ -439,9 +573,37 @@ def main():
                          "(default: the library's, 2)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scale reference for FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--summarize-kernel-trace", metavar="DIR", default=None,
+                    help="write a markdown summary of the rocprofv3 --kernel-trace CSV output in DIR to --out")
+    ap.add_argument("--summarize-pmc", nargs="+", metavar="DIR", default=None,
+                    help="CAL_FETCH CAL_WRITE RUN_DIR... [--bench-json FILE]: PMC summary to --out and "
+                         "profiles/traffic_{any,closest}_production.json")
+    ap.add_argument("--bench-json", default=None, help="the bench.py output line of the profiled runs")
+    ap.add_argument("--command", default="", help="the profiling command, recorded in the summary")
+    ap.add_argument("--title", default="Kernel trace")
+    ap.add_argument("--out", default=None)
     a = ap.parse_args()
     if a.calibrate_copy:
         calibrate_copy()
+        return
+    if a.summarize_kernel_trace:
+        text = summarize_kernel_trace(a.summarize_kernel_trace, a.command, a.title)
+        if a.out:
+            with open(a.out, "w") as f:
+                f.write(text)
+        print(text)
+        return
+    if a.summarize_pmc:
+        md, traffic = summarize_pmc(a.summarize_pmc[0], a.summarize_pmc[1], a.summarize_pmc[2:], a.bench_json,
+                                    a.command)
+        if a.out:
+            with open(a.out, "w") as f:
+                f.write(md)
+        for kind, t in traffic.items():
+            with open(os.path.join(ROOT, "profiles", f"traffic_{kind}_production.json"), "w") as f:
+                json.dump(t, f, indent=1)
+        print(md)
+        print(json.dumps(traffic, indent=1))
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (samples would shard by "

@@ -289,6 +289,13 @@ pt_status pt_trace_closest_counts(pt_renderer* r, const pt_ray* rays, uint32_t c
     });
 }
 
+void pt_frame_size(const pt_render_settings* settings, uint32_t* width, uint32_t* height) {
+    uint32_t w = 1, h = 1;
+    if (settings) pt::frame_size(*settings, w, h);
+    if (width) *width = w;
+    if (height) *height = h;
+}
+
 pt_status pt_trace_any_counts(pt_renderer* r, const pt_ray* rays, uint32_t count, uint32_t* occluded,
                               uint32_t* nodes_visited, uint32_t* tris_tested) {
     return guarded(r, [&](pt::Renderer& x) {

@@ -522,6 +522,12 @@ ClampedSettings clamp_settings(const pt_render_settings& s) {
 }
 }  // namespace
 
+void frame_size(const pt_render_settings& s, uint32_t& width, uint32_t& height) {
+    const ClampedSettings c = clamp_settings(s);
+    width = c.width;
+    height = c.height;
+}
+
 // View record of `v` with settings clamped as in Renderer.cpp:354-372; the
 // caller fills sample_begin and accum_offset.
 pt_status Renderer::make_view(const pt_view& v, ViewRecord& rec, uint32_t& depth) {

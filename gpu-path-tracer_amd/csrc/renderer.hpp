@@ -37,6 +37,9 @@ private:
 
 struct ServerState;   // server.hpp
 
+// Width and height of a frame rendered with `s` (clamped as RenderFrame does).
+void frame_size(const pt_render_settings& s, uint32_t& width, uint32_t& height);
+
 // Options of one wavefront run over a set of views.
 struct RunOptions {
     bool clear = true;           // zero the accumulation pixels first

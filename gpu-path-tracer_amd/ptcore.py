@@ -185,6 +185,7 @@ SIGNATURES = {
     "pt_trace_any_device": (C.c_int, [_P, _P, _U32, _P, _P]),
     "pt_trace_closest_counts": (C.c_int, [_P, _P, _U32, _P, _P, _P]),
     "pt_trace_any_counts": (C.c_int, [_P, _P, _U32, _P, _P, _P]),
+    "pt_frame_size": (None, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "pt_get_queue_occupancy": (C.c_int, [_P, _P, _U32, C.POINTER(C.c_uint32)]),
     "pt_classify_hits": (C.c_int, [_P, _P, _U32, _P, _P]),
     "pt_alias_sample": (C.c_int, [_P, _P, _U32, _P, _P]),
@@ -393,7 +394,7 @@ class Renderer:
         want_hdr = hdr
         hdr, ldr = [], []
         for i in range(n):
-            w, h = views[i].settings.width, views[i].settings.height
+            w, h = self.frame_size(views[i].settings)
             hdr.append(np.zeros((h, w, 4), np.float32) if want_hdr else None)
             ldr.append(np.zeros((h, w), np.uint32) if rgba else None)
         hp = (C.c_void_p * n)(*[a.ctypes.data if a is not None else None for a in hdr])
@@ -471,13 +472,20 @@ class Renderer:
         """Queues one view; returns its request id.  The result arrays are
         held here until ServerResult takes them."""
         v = _view(scene_id, settings, view)
-        w, h = v.settings.width, v.settings.height
+        w, h = self.frame_size(v.settings)
         out_hdr = np.zeros((h, w, 4), np.float32) if hdr else None
         out_rgba = np.zeros((h, w), np.uint32) if rgba else None
         rid = C.c_uint64()
         self._check(self.lib.pt_server_submit(self.h, C.byref(v), _ptr(out_hdr), _ptr(out_rgba), C.byref(rid)))
         self._requests[rid.value] = (out_hdr, out_rgba)
         return rid.value
+
+    def frame_size(self, settings):
+        """(width, height) of a frame rendered with `settings` (a RenderSettings),
+        as the library clamps them: the size of every result buffer."""
+        w, h = C.c_uint32(), C.c_uint32()
+        self.lib.pt_frame_size(C.byref(settings), C.byref(w), C.byref(h))
+        return w.value, h.value
 
     def ServerWait(self, request_id, timeout_ms=120000):
         info = RequestInfo()

@@ -299,13 +299,18 @@ pt_status pt_render_frame(pt_renderer* renderer, const pt_render_settings* setti
 /* Uploads a further scene (same validation as pt_initialize_scene_buffers) and
  * returns its id in 1..63. */
 pt_status pt_add_scene(pt_renderer* renderer, const pt_scene_desc* scene, uint32_t* scene_id);
+/* The frame size a render of `settings` produces (Renderer.cpp:354-372 clamps it):
+ * the pixel count every result buffer of that view must hold. */
+void pt_frame_size(const pt_render_settings* settings, uint32_t* width, uint32_t* height);
 /* Frees a resident scene (id 0 included). */
 pt_status pt_remove_scene(pt_renderer* renderer, uint32_t scene_id);
 /* Renders view_count (1..4096) views, of any resident scenes, in one wavefront: the
  * bounce-loop kernels take paths of every view from one path pool.  Each view's
  * image equals the image it gets rendered alone.  hdr[i] (w*h float RGBA,
  * radiance / spp) and rgba8[i] (w*h RGBA8, row 0 = bottom) receive view i; either
- * array, or any entry, may be NULL.  stats aggregate the whole batch. */
+ * array, or any entry, may be NULL.  w and h are the view's settings clamped as
+ * RenderFrame clamps them (1..7680 x 1..4320, pt_frame_size).  stats aggregate
+ * the whole batch. */
 pt_status pt_render_batch(pt_renderer* renderer, const pt_view* views, uint32_t view_count,
                           float* const* hdr, uint32_t* const* rgba8, uint32_t frame_flags,
                           pt_frame_stats* stats);

@@ -120,3 +120,16 @@ def test_struct_layouts_match_header(tmp_path):
             assert got[(cname, f)] == getattr(cls, f).offset, (cname, f)
     assert got[("pt_material", "size")] == scenes.MATERIAL_DTYPE.itemsize
     assert got[("pt_guide_record", "size")] == ptcore.GUIDE_RECORD_DTYPE.itemsize
+
+
+def test_frame_size_is_the_clamped_size():
+    """pt_frame_size reports the size a render of the settings has, clamped as
+    RenderFrame clamps it (Renderer.cpp:354-372): the pixel count every result
+    buffer of a batch or a served request must hold."""
+    lib = ptcore.load_library()
+    for (w, h), want in (((1920, 1080), (1920, 1080)), ((0, 0), (1, 1)), ((9000, 5000), (7680, 4320)),
+                         ((7680, 1), (7680, 1))):
+        s = ptcore.RenderSettings(w, h, 16, 8, 1, 3, 1)
+        gw, gh = ctypes.c_uint32(), ctypes.c_uint32()
+        lib.pt_frame_size(ctypes.byref(s), ctypes.byref(gw), ctypes.byref(gh))
+        assert (gw.value, gh.value) == want
