@@ -274,6 +274,13 @@ pt_status pt_trace_closest_device(pt_renderer* r, const pt_ray* rays_device, uin
     });
 }
 
+pt_status pt_trace_closest_device_noalloc(pt_renderer* r, const pt_ray* rays_device, uint32_t count,
+                                          pt_hit* hits_device, void* stream) {
+    return guarded(r, [&](pt::Renderer& x) {
+        return x.TraceClosestNoAlloc(rays_device, count, hits_device, (hipStream_t)stream);
+    });
+}
+
 pt_status pt_trace_any_device(pt_renderer* r, const pt_ray* rays_device, uint32_t count, uint32_t* occluded_device,
                               void* stream) {
     return guarded(r, [&](pt::Renderer& x) {

@@ -20,6 +20,8 @@
 #include "guide_net.hpp"
 #include "launch.hpp"
 
+#include <mutex>
+
 namespace pt {
 
 // ================================================================ wave helpers
@@ -1554,6 +1556,12 @@ __global__ void k_write_words(uint32_t* dst, WordPayload src, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src.w[i];
 }
 
+// The allocation-free trace's scratch (counters + stack spill).  In host code
+// `scratch` names a host-side stand-in: only the address that
+// hipGetSymbolAddress returns for the current device may reach a kernel or a
+// copy.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 __global__ void k_zero_u32(unsigned int* ptr, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ptr[i] = 0u;
 }
@@ -1728,6 +1736,25 @@ hipError_t launch_classify(hipStream_t s, const SceneDev& sc, const float4* rays
 hipError_t launch_alias_sample(hipStream_t s, const SceneDev& sc, const float* u, uint32_t n, uint32_t* index, float* pmf) {
     hipLaunchKernelGGL(k_alias_sample, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, sc, u, n, index, pmf);
     return hipGetLastError();
+}
+
+hipError_t noalloc_scratch_address(void** ptr) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    return hipSuccess;
 }
 
 hipError_t launch_write_words(hipStream_t s, void* dst, const void* src, size_t bytes) {

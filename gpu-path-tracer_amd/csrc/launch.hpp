@@ -141,4 +141,12 @@ constexpr uint32_t kWritePayloadWords = 896;
 hipError_t launch_write_words(hipStream_t s, void* dst, const void* src, size_t bytes);
 hipError_t launch_zero_u32(hipStream_t s, unsigned int* p, uint32_t n);
 
+// Scratch of the allocation-free closest-hit trace (pt_trace_closest_device_noalloc):
+// one file-scope device array per GPU, [0, 256) work counter and overflow flag,
+// the rest the traversal's global stack spill area.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+constexpr size_t kNoAllocSpillOffset = 256;
+// Device address of that array on the current device, looked up once per device.
+hipError_t noalloc_scratch_address(void** ptr);
+
 }  // namespace pt

@@ -105,6 +105,8 @@ pt_status Renderer::init() {
     PT_TRY_HIP(hipMemsetAsync(d_ctr_.as<void>(), 0, sizeof(Counters), stream_), "memset");
     PT_TRY_HIP(hipMemsetAsync(d_overflow_.as<void>(), 0, 64, stream_), "memset");
     PT_TRY_HIP(hipStreamSynchronize(stream_), "sync");
+    // the allocation-free trace's scratch on this device (an address lookup, no allocation)
+    PT_TRY_HIP(noalloc_scratch_address(reinterpret_cast<void**>(&noalloc_scratch_)), "hipGetSymbolAddress(scratch)");
     return PT_OK;
 }
 
@@ -1041,6 +1043,39 @@ pt_status Renderer::TraceBatch(const pt_ray* rays, uint32_t n, bool any, bool de
         PT_TRY_HIP(hipStreamSynchronize(s), "hook sync");
         if (overflow) return fail(PT_ERROR_DEVICE, "traversal stack overflow");
     }
+    return PT_OK;
+}
+
+// The closest-hit trace of caller device buffers with no allocation: its work
+// counter, overflow flag and stack spill area live in the static scratch
+// array, and the persistent grid shrinks until the spill area of the deepest
+// resident BVH fits it.  The stack keeps its full depth (the spill depth is the
+// tree's worst case), so it cannot overflow.  Every call on one GPU shares the
+// array: calls that overlap on one GPU race.
+pt_status Renderer::TraceClosestNoAlloc(const pt_ray* rays, uint32_t n, pt_hit* hits, hipStream_t stream) {
+    if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Renderer is already rendering");
+    if (n == 0) return PT_OK;
+    if (!rays || !hits) return fail(PT_ERROR_INVALID_ARGUMENT, "rays and hits must not be NULL");
+    if (!noalloc_scratch_) return fail(PT_ERROR_DEVICE, "no scratch address on this device");
+    TraceConfig cfg = hook_trace_;
+    cfg.stack_cap = cfg.lds_stack + cfg.spill_depth;
+    const size_t per_block = (size_t)cfg.spill_depth * cfg.block * sizeof(uint2);
+    const size_t room = kNoAllocScratchBytes - kNoAllocSpillOffset;
+    if (per_block > room)
+        return fail(PT_ERROR_OUT_OF_RANGE, "the BVH's traversal stack needs " + std::to_string(per_block) +
+                                               " bytes of scratch per block; the no-alloc trace holds " +
+                                               std::to_string(room));
+    if (per_block) cfg.blocks = (uint32_t)std::max<size_t>(1, std::min<size_t>(cfg.blocks, room / per_block));
+    unsigned int* work = reinterpret_cast<unsigned int*>(noalloc_scratch_);
+    cfg.overflow = work + 1;
+    cfg.spill = reinterpret_cast<uint2*>(noalloc_scratch_ + kNoAllocSpillOffset);
+    hipStream_t s = stream ? stream : stream_;
+    // the counter and the flag start at zero: a fill kernel (no memset node in a captured graph)
+    PT_TRY_HIP(launch_zero_u32(s, work, 2), "zero");
+    PT_TRY_HIP(launch_trace_batch(s, scene_dev(), reinterpret_cast<const float4*>(rays), n,
+                                  reinterpret_cast<float4*>(hits), nullptr, nullptr, nullptr, work, false, cfg),
+               "trace launch");
     return PT_OK;
 }
 

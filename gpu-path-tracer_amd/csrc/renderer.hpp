@@ -116,6 +116,9 @@ public:
     pt_status TraceBatch(const pt_ray* rays, uint32_t n, bool any, bool device, void* out, uint32_t* nodes,
                          uint32_t* tris, hipStream_t stream);
     pt_status Classify(const pt_ray* rays, uint32_t n, uint32_t* queue_of_ray, uint32_t* counts);
+    // TraceBatch(closest, device) with its scratch in the static device array
+    // (launch.hpp): allocates nothing
+    pt_status TraceClosestNoAlloc(const pt_ray* rays_device, uint32_t n, pt_hit* hits_device, hipStream_t stream);
     pt_status AliasSample(const float* u, uint32_t n, uint32_t* index, float* pmf);
     pt_status GetAliasTable(pt_alias_bin* bins, uint32_t cap, uint32_t* count);
     pt_status GetBvhInfo(pt_bvh_info& info);
@@ -237,6 +240,7 @@ private:
     TraceConfig shadow_alone_multi_{};      //   (stage-by-stage loops): the whole occupancy
     TraceConfig hook_trace_{};              // trace_ for the caller-batch hooks, with their own overflow flag
     DeviceBuffer d_req_invalid_;            // server: per-request invalid-pixel counters
+    unsigned char* noalloc_scratch_ = nullptr;  // this device's address of the no-alloc scratch array
     uint32_t* h_req_invalid_ = nullptr;     //   and their pinned host copies
     DeviceBuffer d_spill_;
     uint32_t stack_total_ = 0;

@@ -185,6 +185,7 @@ SIGNATURES = {
     "pt_trace_any_device": (C.c_int, [_P, _P, _U32, _P, _P]),
     "pt_trace_closest_counts": (C.c_int, [_P, _P, _U32, _P, _P, _P]),
     "pt_trace_any_counts": (C.c_int, [_P, _P, _U32, _P, _P, _P]),
+    "pt_trace_closest_device_noalloc": (C.c_int, [_P, _P, _U32, _P, _P]),
     "pt_frame_size": (None, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "pt_get_queue_occupancy": (C.c_int, [_P, _P, _U32, C.POINTER(C.c_uint32)]),
     "pt_classify_hits": (C.c_int, [_P, _P, _U32, _P, _P]),

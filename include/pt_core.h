@@ -455,6 +455,18 @@ pt_status pt_trace_closest_device(pt_renderer* renderer, const pt_ray* rays_devi
                                   pt_hit* hits_device, void* stream);
 pt_status pt_trace_any_device(pt_renderer* renderer, const pt_ray* rays_device, uint32_t count,
                               uint32_t* occluded_device, void* stream);
+/* pt_trace_closest_device that allocates nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch (work counter, overflow flag, traversal stack spill) is one static
+ * 32 MiB device array per GPU, held until the process ends; the persistent
+ * grid shrinks until the deepest resident BVH's stack spill fits it, and a tree
+ * whose spill does not fit one block is rejected with PT_ERROR_OUT_OF_RANGE.
+ * Every call on one GPU shares that array: calls that overlap on one GPU race.
+ * Asynchronous on `stream` (NULL: the renderer's stream); same hits as
+ * pt_trace_closest_device. */
+pt_status pt_trace_closest_device_noalloc(pt_renderer* renderer, const pt_ray* rays_device, uint32_t count,
+                                          pt_hit* hits_device, void* stream);
 /* Per-ray visit counts of the closest-hit traversal (the algorithmic-byte model). */
 pt_status pt_trace_closest_counts(pt_renderer* renderer, const pt_ray* rays, uint32_t count,
                                   pt_hit* hits, uint32_t* nodes_visited, uint32_t* tris_tested);
