@@ -20,6 +20,7 @@ STATUS_NAMES = {0: "PT_OK", 1: "PT_ERROR_INVALID_ARGUMENT", 2: "PT_ERROR_INVALID
 QUEUE_DIFFUSE, QUEUE_GGX, QUEUE_MIRROR, QUEUE_EMISSIVE, QUEUE_REGEN = range(5)
 INVALID = 0xFFFFFFFF
 REQUEST_QUEUED, REQUEST_RUNNING, REQUEST_DONE, REQUEST_FAILED = 0, 1, 2, 3
+BVH_BUILDER_HOST, BVH_BUILDER_GPU = 0, 1
 FRAME_DEFAULT, FRAME_PROFILE, FRAME_INSTRUMENT, FRAME_NO_GRAPH = 0, 1, 2, 4
 
 
@@ -389,14 +390,15 @@ class Renderer:
     def RenderBatch(self, items, flags=FRAME_DEFAULT, rgba=False, hdr=True):
         """items: [(scene_id, settings dict, scenes.View)], rendered in one
         wavefront.  Returns ([hdr (h, w, 4) per view, or None when hdr=False],
-        stats), plus the RGBA8 images (h, w) when rgba=True."""
+        stats), plus the RGBA8 images (h, w) when rgba=True.  hdr may also be
+        a list of booleans, one per view."""
         n = len(items)
         views = (View * n)(*[_view(sid, st, v) for sid, st, v in items])
-        want_hdr = hdr
+        want_hdr = hdr if isinstance(hdr, (list, tuple)) else [hdr] * n
         hdr, ldr = [], []
         for i in range(n):
             w, h = self.frame_size(views[i].settings)
-            hdr.append(np.zeros((h, w, 4), np.float32) if want_hdr else None)
+            hdr.append(np.zeros((h, w, 4), np.float32) if want_hdr[i] else None)
             ldr.append(np.zeros((h, w), np.uint32) if rgba else None)
         hp = (C.c_void_p * n)(*[a.ctypes.data if a is not None else None for a in hdr])
         lp = (C.c_void_p * n)(*[a.ctypes.data if a is not None else None for a in ldr])

@@ -163,3 +163,22 @@ def test_server_errors_and_lifecycle(server, cornell, spheres):
     # the renderer works normally afterwards
     (hdr,), _ = r.RenderBatch([(sid, st, spheres.view)])
     assert np.isfinite(hdr).all()
+
+
+def test_served_request_equals_the_oracle(server, cornell, oracle_lib):
+    """A request that went through the server (split into chunks and mixed with
+    another) against the oracle's render of the same view (Renderer.cpp:330-399
+    renders one request at a time): equal to the rounding of the float sums."""
+    r, sid = server
+    st = dict(width=48, height=40, samples_per_pixel=16, path_depth=8, use_nee=1, rr_start_depth=3, seed=21)
+    r.ServerStart(chunk_samples=4001, window_samples=16000)
+    rid = r.ServerSubmit(0, st, cornell.view, hdr=True)
+    other = r.ServerSubmit(sid, dict(st, seed=22), cornell.view, hdr=True)
+    info = r.ServerWait(rid)
+    r.ServerWait(other)
+    r.ServerStop()
+    assert info["state"] == ptcore.REQUEST_DONE
+    hdr, _ = r.ServerResult(rid)
+    r.ServerResult(other)
+    ref, _, _ = oracle_lib.OracleScene(cornell).render(st, cornell.view, tprim=True)
+    assert np.allclose(hdr.reshape(-1, 4)[:, :3], ref[:, :3], rtol=2e-5, atol=1e-6)
