@@ -24,6 +24,7 @@ HBM_PEAK_GBS = 8000.0
 # 32 B in, hit record + queue id 17 B out, 128 B per BVH4 node, 48 B per triangle
 RAY_IN_B, HIT_OUT_B, NODE_B, TRI_B = 32, 17, 128, 48
 QNODE_B = 64        # quantized BVH4 node
+Q8NODE_B = 96       # 8-wide quantized node (any hit): the 96 bytes of its 128-byte line a visit reads
 # one shadow ray: its 48-byte record (origin|tMax, direction|scene, radiance|pixel)
 # in; the 12-byte radiance add of an unoccluded ray is not counted
 SHADOW_IN_B = 48
@@ -568,9 +569,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--trace-block", type=int, default=0)
     ap.add_argument("--pool", type=int, default=0)
-    ap.add_argument("--quantized", type=int, choices=[0, 1, 2, 3], default=None,
-                    help="pt_tuning.bvh_quantized: bit 0 closest hit, bit 1 any hit on 64-byte quantized nodes "
-                         "(default: the library's, 2)")
+    ap.add_argument("--quantized", type=int, choices=range(8), default=None,
+                    help="pt_tuning.bvh_quantized: bit 0 closest hit, bit 1 any hit on 64-byte quantized nodes, "
+                         "bit 2 any hit on 128-byte 8-wide quantized nodes (default: the library's)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scale reference for FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--summarize-kernel-trace", metavar="DIR", default=None,
@@ -641,12 +642,13 @@ def main():
             kw["bvh_quantized"] = a.quantized
         r.set_tuning(**kw)
     quant = r.get_tuning()["bvh_quantized"]
-    node_b = QNODE_B if quant & 1 else NODE_B           # closest-hit node format
-    node_b_any = QNODE_B if quant & 2 else NODE_B       # any-hit node format
     r.InitializeRenderingBuffers(settings["width"], settings["height"])
     t0 = time.time()
     r.InitializeSceneBuffers(scene)
     bvh = r.bvh_info()
+    node_b = QNODE_B if quant & 1 else NODE_B           # closest-hit node format
+    # any-hit node format: the 8-wide tree where it was built (4-wide host build)
+    node_b_any = Q8NODE_B if bvh["bvh8_node_count"] else QNODE_B if quant & 2 else NODE_B
     log(f"upload + BVH4 build {time.time() - t0:.1f}s: {bvh}")
 
     for i in range(a.warmup):

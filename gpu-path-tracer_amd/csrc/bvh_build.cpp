@@ -220,6 +220,82 @@ uint32_t emit(const TNode* n, Bvh4Build& out, uint32_t level) {
     return index;
 }
 
+// Emits the 8-wide quantized node that replaces BVH2 node `n` (internal), in
+// DFS preorder: up to eight children by expanding the largest-area internal
+// child (the rule of emit), child boxes padded as in set_child, then quantized.
+uint32_t emit8(const TNode* n, Bvh4Build& out, uint32_t level) {
+    const uint32_t index = (uint32_t)out.qnodes8.size();
+    out.qnodes8.emplace_back();
+    out.depth8 = std::max(out.depth8, level);
+    const TNode* ch[8] = {n->left.get(), n->right.get(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int count = 2;
+    while (count < 8) {
+        int pick = -1;
+        float best = -1.0f;
+        for (int i = 0; i < count; i++) {
+            if (!ch[i]->leaf() && ch[i]->box.area() > best) {
+                best = ch[i]->box.area();
+                pick = i;
+            }
+        }
+        if (pick < 0) break;
+        const TNode* c = ch[pick];
+        ch[pick] = c->left.get();
+        ch[count++] = c->right.get();
+    }
+    uint32_t refs[8];
+    float lo[3][8] = {}, hi[3][8] = {};
+    for (int i = 0; i < 8; i++) refs[i] = kInvalid;
+    for (int i = 0; i < count; i++) {
+        const TNode* c = ch[i];
+        if (c->leaf()) {
+            if (c->count > kMaxLeafSize) throw std::runtime_error("BVH leaf larger than 16 triangles");
+            refs[i] = make_leaf_ref(c->begin, c->count);
+        } else {
+            refs[i] = emit8(c, out, level + 1);
+        }
+        float l[3] = {c->box.lo[0], c->box.lo[1], c->box.lo[2]};
+        float h[3] = {c->box.hi[0], c->box.hi[1], c->box.hi[2]};
+        pad_bounds(l, h);
+        for (int a = 0; a < 3; a++) {
+            lo[a][i] = l[a];
+            hi[a][i] = h[a];
+        }
+    }
+    Bvh8QNode& q = out.qnodes8[index];   // the recursion above may have moved the array
+    quantize_children(lo, hi, refs, 8, q.origin, q.scale, q.lo, q.hi);
+    for (int i = 0; i < 8; i++) {
+        q.child[i] = refs[i];
+        q.pad[i] = 0u;
+    }
+    return index;
+}
+
+// An 8-wide root over a single leaf (or none).
+void root8(const TNode* leaf, Bvh4Build& out) {
+    out.qnodes8.emplace_back();
+    out.depth8 = 1;
+    uint32_t refs[8];
+    float lo[3][8] = {}, hi[3][8] = {};
+    for (int i = 0; i < 8; i++) refs[i] = kInvalid;
+    if (leaf) {
+        refs[0] = make_leaf_ref(leaf->begin, leaf->count);
+        float l[3] = {leaf->box.lo[0], leaf->box.lo[1], leaf->box.lo[2]};
+        float h[3] = {leaf->box.hi[0], leaf->box.hi[1], leaf->box.hi[2]};
+        pad_bounds(l, h);
+        for (int a = 0; a < 3; a++) {
+            lo[a][0] = l[a];
+            hi[a][0] = h[a];
+        }
+    }
+    Bvh8QNode& q = out.qnodes8[0];
+    quantize_children(lo, hi, refs, 8, q.origin, q.scale, q.lo, q.hi);
+    for (int i = 0; i < 8; i++) {
+        q.child[i] = refs[i];
+        q.pad[i] = 0u;
+    }
+}
+
 }  // namespace
 
 std::vector<Bvh2Node> to_bvh2(const Bvh4Build& b) {
@@ -256,6 +332,7 @@ Bvh4Build build_bvh4(const float* positions, uint32_t T, const BvhBuildOptions& 
         throw std::runtime_error("max_leaf_size must be in [1, 16]");
     if (options.bins < 2 || options.bins > 256) throw std::runtime_error("bins must be in [2, 256]");
     if (options.width != 2 && options.width != 4) throw std::runtime_error("BVH node width must be 2 or 4");
+    if (options.wide8 && options.width != 4) throw std::runtime_error("8-wide nodes need a 4-wide build");
     if ((uint64_t)T > (uint64_t)kMaxLeafFirst) throw std::runtime_error("scene exceeds 2^27 triangles");
 
     Builder b;
@@ -286,6 +363,7 @@ Bvh4Build build_bvh4(const float* positions, uint32_t T, const BvhBuildOptions& 
         out.nodes.emplace_back();
         clear_node(out.nodes[0]);
         out.depth = 1;
+        if (options.wide8) root8(nullptr, out);
         return out;
     }
     std::unique_ptr<TNode> root = b.build(0, T, 0);
@@ -297,6 +375,11 @@ Bvh4Build build_bvh4(const float* positions, uint32_t T, const BvhBuildOptions& 
         set_child(out.nodes[0], 0, root->box, leaf_ref_checked(root.get(), out), root->count);
     } else {
         emit(root.get(), out, 1);
+    }
+    if (options.wide8) {
+        out.qnodes8.reserve(T / 8 + 8);
+        if (root->leaf()) root8(root.get(), out);
+        else emit8(root.get(), out, 1);
     }
     out.order = std::move(b.idx);
     return out;

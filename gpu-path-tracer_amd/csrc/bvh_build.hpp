@@ -20,6 +20,10 @@ struct Bvh4Build {
     uint32_t max_leaf = 0;
     uint32_t width = 4;               // children per node: 4, or 2 (no collapse)
     uint32_t device_nodes = 0;        // node count of a tree built on the device (nodes and order stay empty)
+    // the same binary tree collapsed into 8-wide quantized nodes (BvhBuildOptions::wide8):
+    // the same leaves, so the same triangle order
+    std::vector<Bvh8QNode> qnodes8;
+    uint32_t depth8 = 0;
 };
 
 // The nodes of a width-2 build in the 64-byte layout.
@@ -32,6 +36,7 @@ struct BvhBuildOptions {
     float intersection_cost = 1.0f;
     uint32_t threads = 0;             // 0 = hardware concurrency
     uint32_t width = 4;               // 4: collapse the BVH2 into BVH4 nodes; 2: keep it
+    bool wide8 = false;               // also collapse it into 8-wide quantized nodes (width 4 only)
 };
 
 // positions: [T][3][3] floats.  Throws std::runtime_error on bad input.

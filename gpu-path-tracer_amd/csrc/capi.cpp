@@ -331,6 +331,13 @@ pt_status pt_get_bvh_info(pt_renderer* r, pt_bvh_info* info) {
     });
 }
 
+pt_status pt_get_bvh8_quantized(pt_renderer* r, void* nodes, uint32_t node_capacity, uint32_t* node_count) {
+    return guarded(r, [&](pt::Renderer& x) {
+        if (!node_count) return x.fail(PT_ERROR_INVALID_ARGUMENT, "node_count is NULL");
+        return x.GetBvh8Quantized(nodes, node_capacity, *node_count);
+    });
+}
+
 pt_status pt_get_queue_occupancy(pt_renderer* r, pt_queue_occupancy* iterations, uint32_t capacity, uint32_t* count) {
     return guarded(r, [&](pt::Renderer& x) {
         if (!count) return x.fail(PT_ERROR_INVALID_ARGUMENT, "count is NULL");
@@ -354,6 +361,29 @@ pt_status pt_set_tuning(pt_renderer* r, const pt_tuning* tuning) {
         if (!tuning) return x.fail(PT_ERROR_INVALID_ARGUMENT, "tuning is NULL");
         return x.SetTuning(*tuning);
     });
+}
+
+pt_status pt_build_bvh8_quantized(const float* positions, uint32_t triangle_count, uint32_t max_leaf_size,
+                                  void* nodes, uint32_t node_capacity, uint32_t* node_count, uint32_t* prim_order,
+                                  uint32_t* depth) {
+    if ((triangle_count > 0 && !positions) || !node_count) return PT_ERROR_INVALID_ARGUMENT;
+    try {
+        pt::BvhBuildOptions o;
+        if (max_leaf_size) o.max_leaf_size = max_leaf_size;
+        o.wide8 = true;
+        const pt::Bvh4Build b = pt::build_bvh4(positions, triangle_count, o);
+        *node_count = (uint32_t)b.qnodes8.size();
+        if (depth) *depth = b.depth8;
+        if (!nodes) return PT_OK;
+        if (node_capacity < b.qnodes8.size()) return PT_ERROR_INVALID_ARGUMENT;
+        std::memcpy(nodes, b.qnodes8.data(), b.qnodes8.size() * sizeof(pt::Bvh8QNode));
+        if (prim_order) std::memcpy(prim_order, b.order.data(), b.order.size() * sizeof(uint32_t));
+        return PT_OK;
+    } catch (const std::bad_alloc&) {
+        return PT_ERROR_OUT_OF_MEMORY;
+    } catch (const std::exception&) {
+        return PT_ERROR_INVALID_ARGUMENT;
+    }
 }
 
 pt_status pt_build_bvh4(const float* positions, uint32_t triangle_count, uint32_t max_leaf_size, void* nodes,

@@ -189,6 +189,7 @@ __device__ __forceinline__ void box4(const Bvh4Node* node, const RayFrame& r, fl
 constexpr int kNodeBvh2 = 2;
 constexpr int kNodeBvh4 = 4;
 constexpr int kNodeBvh4Q = 5;
+constexpr int kNodeBvh8Q = 8;
 template <int K>
 struct NodeOf;
 template <>
@@ -205,6 +206,11 @@ template <>
 struct NodeOf<kNodeBvh4Q> {
     using type = Bvh4QNode;
     static constexpr int arity = 4;
+};
+template <>
+struct NodeOf<kNodeBvh8Q> {
+    using type = Bvh8QNode;
+    static constexpr int arity = 8;
 };
 
 // Byte i of a quantized row as a float (one v_cvt_f32_ubyte<i>).
@@ -260,6 +266,41 @@ __device__ __forceinline__ void children(const Bvh4QNode* node, const RayFrame& 
     c[1] = d.w;
     c[2] = e.x;
     c[3] = e.y;
+}
+
+// Quantized 8-wide node: the scheme of the 4-wide one over two words per bound
+// row (children 0-3 and 4-7), six 16-byte loads of the 128-byte line.
+__device__ __forceinline__ void children(const Bvh8QNode* node, const RayFrame& r, float t0[8], uint32_t c[8]) {
+    const uint4* p = reinterpret_cast<const uint4*>(node);
+    const uint4 a = p[0], l0 = p[1], l1 = p[2], h1 = p[3], c0 = p[4], c1 = p[5];
+    const float ax = qnode_scale(a.w & 0xFFu) * r.rd.x;
+    const float ay = qnode_scale((a.w >> 8) & 0xFFu) * r.rd.y;
+    const float az = qnode_scale((a.w >> 16) & 0xFFu) * r.rd.z;
+    const float bx = fmaf(__uint_as_float(a.x), r.rd.x, -r.ord.x);
+    const float by = fmaf(__uint_as_float(a.y), r.rd.y, -r.ord.y);
+    const float bz = fmaf(__uint_as_float(a.z), r.rd.z, -r.ord.z);
+    // rows: lo x = (l0.x, l0.y), lo y = (l0.z, l0.w), lo z = (l1.x, l1.y);
+    //       hi x = (l1.z, l1.w), hi y = (h1.x, h1.y), hi z = (h1.z, h1.w)
+    const bool px = r.rd.x >= 0.0f, py = r.rd.y >= 0.0f, pz = r.rd.z >= 0.0f;
+    const uint32_t nx0 = px ? l0.x : l1.z, fx0 = px ? l1.z : l0.x, nx1 = px ? l0.y : l1.w, fx1 = px ? l1.w : l0.y;
+    const uint32_t ny0 = py ? l0.z : h1.x, fy0 = py ? h1.x : l0.z, ny1 = py ? l0.w : h1.y, fy1 = py ? h1.y : l0.w;
+    const uint32_t nz0 = pz ? l1.x : h1.z, fz0 = pz ? h1.z : l1.x, nz1 = pz ? l1.y : h1.w, fz1 = pz ? h1.w : l1.y;
+    t0[0] = qchild<0>(nx0, fx0, ny0, fy0, nz0, fz0, ax, bx, ay, by, az, bz, r);
+    t0[1] = qchild<1>(nx0, fx0, ny0, fy0, nz0, fz0, ax, bx, ay, by, az, bz, r);
+    t0[2] = qchild<2>(nx0, fx0, ny0, fy0, nz0, fz0, ax, bx, ay, by, az, bz, r);
+    t0[3] = qchild<3>(nx0, fx0, ny0, fy0, nz0, fz0, ax, bx, ay, by, az, bz, r);
+    t0[4] = qchild<0>(nx1, fx1, ny1, fy1, nz1, fz1, ax, bx, ay, by, az, bz, r);
+    t0[5] = qchild<1>(nx1, fx1, ny1, fy1, nz1, fz1, ax, bx, ay, by, az, bz, r);
+    t0[6] = qchild<2>(nx1, fx1, ny1, fy1, nz1, fz1, ax, bx, ay, by, az, bz, r);
+    t0[7] = qchild<3>(nx1, fx1, ny1, fy1, nz1, fz1, ax, bx, ay, by, az, bz, r);
+    c[0] = c0.x;
+    c[1] = c0.y;
+    c[2] = c0.z;
+    c[3] = c0.w;
+    c[4] = c1.x;
+    c[5] = c1.y;
+    c[6] = c1.z;
+    c[7] = c1.w;
 }
 
 __device__ __forceinline__ void children(const Bvh4Node* node, const RayFrame& r, float t0[4], uint32_t c[4]) {
@@ -339,7 +380,8 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
     bool exhausted = chunk_next >= n;
     uint32_t item = 0, ref = 0, sp = 0;
     // per lane when MULTI: the ray's scene
-    const Node* nodes_base = static_cast<const Node*>(NODE == kNodeBvh4Q ? sc.qnodes : sc.nodes);
+    const Node* nodes_base =
+        static_cast<const Node*>(NODE == kNodeBvh8Q ? sc.qnodes8 : NODE == kNodeBvh4Q ? sc.qnodes : sc.nodes);
     const TriRecord* tris_base = sc.tris;
     RayFrame r;
     float best_t = 0.0f, best_u = 0.0f, best_v = 0.0f;
@@ -414,8 +456,9 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                         uint32_t scene = 0;
                         item = src.load(chunk_next + lane_rank, o, d, tmin, tmax, scene);
                         if (MULTI) {
-                            nodes_base = static_cast<const Node*>(NODE == kNodeBvh4Q ? scenes[scene].qnodes
-                                                                                     : scenes[scene].nodes);
+                            nodes_base = static_cast<const Node*>(NODE == kNodeBvh8Q   ? scenes[scene].qnodes8
+                                                                  : NODE == kNodeBvh4Q ? scenes[scene].qnodes
+                                                                                       : scenes[scene].nodes);
                             tris_base = scenes[scene].tris;
                         }
                         setup_ray(r, o, d, tmin, tmax);
@@ -448,7 +491,28 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                 if (COUNT) nodes++;
                 if (!ANY || PT_ANY_SORTED) {
                     // nearest child first: sort the (distance, ref) pairs
-                    if (W == 4) {
+                    if (W == 8) {
+                        // Batcher's odd-even merge sort, 19 comparators
+                        cswap(t0[0], c[0], t0[1], c[1]);
+                        cswap(t0[2], c[2], t0[3], c[3]);
+                        cswap(t0[4], c[4], t0[5], c[5]);
+                        cswap(t0[6], c[6], t0[7], c[7]);
+                        cswap(t0[0], c[0], t0[2], c[2]);
+                        cswap(t0[1], c[1], t0[3], c[3]);
+                        cswap(t0[4], c[4], t0[6], c[6]);
+                        cswap(t0[5], c[5], t0[7], c[7]);
+                        cswap(t0[1], c[1], t0[2], c[2]);
+                        cswap(t0[5], c[5], t0[6], c[6]);
+                        cswap(t0[0], c[0], t0[4], c[4]);
+                        cswap(t0[1], c[1], t0[5], c[5]);
+                        cswap(t0[2], c[2], t0[6], c[6]);
+                        cswap(t0[3], c[3], t0[7], c[7]);
+                        cswap(t0[2], c[2], t0[4], c[4]);
+                        cswap(t0[3], c[3], t0[5], c[5]);
+                        cswap(t0[1], c[1], t0[2], c[2]);
+                        cswap(t0[3], c[3], t0[4], c[4]);
+                        cswap(t0[5], c[5], t0[6], c[6]);
+                    } else if (W == 4) {
                         cswap(t0[0], c[0], t0[1], c[1]);
                         cswap(t0[W - 2], c[W - 2], t0[W - 1], c[W - 1]);
                         cswap(t0[0], c[0], t0[W - 2], c[W - 2]);
@@ -1598,6 +1662,9 @@ hipError_t dispatch_block(hipStream_t s, const Src& src, const SceneDev& sc, con
 template <bool ANY, bool COUNT, bool MULTI, class Src>
 hipError_t dispatch_trace(hipStream_t s, const Src& src, const SceneDev& sc, const SceneDev* scenes,
                           const TraceConfig& cfg) {
+    if constexpr (ANY) {
+        if (sc.node_width == 4 && sc.qnodes8) return dispatch_block<kNodeBvh8Q, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
+    }
     if (sc.node_width == 4 && (sc.node_quantized & (ANY ? 2u : 1u)))
         return dispatch_block<kNodeBvh4Q, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
     if (sc.node_width == 4) return dispatch_block<kNodeBvh4, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
@@ -1629,6 +1696,13 @@ int occ_kind(const TraceConfig& cfg, bool any, bool count) {
     return count ? occ<NODE, false, true, MULTI, WavefrontClosestSrc>(cfg) : occ<NODE, false, false, MULTI, WavefrontClosestSrc>(cfg);
 }
 
+// the 8-wide quantized nodes serve the any-hit traversal only
+template <bool MULTI>
+int occ_any8(const TraceConfig& cfg, bool count) {
+    return count ? occ<kNodeBvh8Q, true, true, MULTI, WavefrontShadowSrc>(cfg)
+                 : occ<kNodeBvh8Q, true, false, MULTI, WavefrontShadowSrc>(cfg);
+}
+
 }  // namespace
 
 bool trace_config_supported(uint32_t block, uint32_t lds_stack) {
@@ -1636,8 +1710,9 @@ bool trace_config_supported(uint32_t block, uint32_t lds_stack) {
            (block == 512 && lds_stack == 8);
 }
 
-int trace_occupancy(const TraceConfig& cfg, bool any, bool count, bool quantized, bool multi) {
-    if (quantized) return multi ? occ_kind<kNodeBvh4Q, true>(cfg, any, count) : occ_kind<kNodeBvh4Q, false>(cfg, any, count);
+int trace_occupancy(const TraceConfig& cfg, bool any, bool count, int node_kind, bool multi) {
+    if (node_kind == 2 && any) return multi ? occ_any8<true>(cfg, count) : occ_any8<false>(cfg, count);
+    if (node_kind >= 1) return multi ? occ_kind<kNodeBvh4Q, true>(cfg, any, count) : occ_kind<kNodeBvh4Q, false>(cfg, any, count);
     return multi ? occ_kind<kNodeBvh4, true>(cfg, any, count) : occ_kind<kNodeBvh4, false>(cfg, any, count);
 }
 

@@ -16,6 +16,7 @@ constexpr int kQueueSlots = 5;
 struct SceneDev {
     const void* nodes;              // Bvh4Node[] or Bvh2Node[] by node_width
     const void* qnodes;             // Bvh4QNode[]: the same 4-wide tree quantized, or null
+    const void* qnodes8;            // Bvh8QNode[]: the same binary tree 8-wide and quantized (any hit), or null
     const TriRecord* tris;          // BVH order
     const ShadeRecord* shade;       // original order
     const MaterialRecord* materials;
@@ -84,7 +85,8 @@ struct TraceConfig {
 };
 
 // Blocks of `block` threads that fit on one CU for the traversal kernels.
-int trace_occupancy(const TraceConfig& cfg, bool any, bool count, bool quantized, bool multi);
+// node_kind: 0 fp32 BVH4, 1 quantized BVH4, 2 quantized BVH8 (any hit only)
+int trace_occupancy(const TraceConfig& cfg, bool any, bool count, int node_kind, bool multi);
 bool trace_config_supported(uint32_t block, uint32_t lds_stack);
 
 hipError_t launch_frame_begin(hipStream_t s, const PathState& p);

@@ -161,6 +161,59 @@ PT_HD void quantize_bvh4_node(const Bvh4Node& n, Bvh4QNode& q) {
     for (int i = 0; i < 4; i++) q.child[i] = n.child[i];
 }
 
+// Quantized BVH8 node: one 128-byte line, eight children under the same
+// byte scheme as Bvh4QNode.  Bound rows hold children 0-3 in word 2a and 4-7
+// in word 2a + 1 of axis a, so the node reads as six 16-byte loads:
+// (origin, scale), (lo x0 x1 y0 y1), (lo z0 z1, hi x0 x1), (hi y0 y1 z0 z1),
+// children 0-3, children 4-7.
+struct alignas(128) Bvh8QNode {
+    float origin[3];
+    uint32_t scale;
+    uint32_t lo[6];
+    uint32_t hi[6];
+    uint32_t child[8];
+    uint32_t pad[8];
+};
+static_assert(sizeof(Bvh8QNode) == 128, "quantized BVH8 node fills one cache line");
+
+// Quantizes `n` child boxes (lo/hi [axis][child], padded fp32; ref 0xFFFFFFFF
+// = empty slot) into origin, scale exponents and byte rows (byte i of row word
+// a * words + i / 4); the rule of quantize_bvh4_node.
+PT_HD void quantize_children(const float lo[3][8], const float hi[3][8], const uint32_t* child, int n,
+                             float origin[3], uint32_t& scale, uint32_t* lo_rows, uint32_t* hi_rows) {
+    const int words = (n + 3) / 4;
+    scale = 0u;
+    for (int a = 0; a < 3; a++) {
+        float mn = 0.0f, mx = 0.0f;
+        bool any = false;
+        for (int i = 0; i < n; i++) {
+            if (child[i] == 0xFFFFFFFFu) continue;
+            mn = any ? (lo[a][i] < mn ? lo[a][i] : mn) : lo[a][i];
+            mx = any ? (hi[a][i] > mx ? hi[a][i] : mx) : hi[a][i];
+            any = true;
+        }
+        const uint32_t e = qnode_exponent(mn, mx);
+        const float s = qnode_scale(e);
+        origin[a] = mn;
+        scale |= e << (8 * a);
+        for (int w = 0; w < words; w++) {
+            uint32_t rl = 0, rh = 0;
+            for (int k = 0; k < 4; k++) {
+                const int i = 4 * w + k;
+                uint32_t bl = 255u, bh = 0u;
+                if (i < n && child[i] != 0xFFFFFFFFu) {
+                    bl = qnode_floor(mn, s, lo[a][i]);
+                    bh = qnode_ceil(mn, s, hi[a][i]);
+                }
+                rl |= bl << (8 * k);
+                rh |= bh << (8 * k);
+            }
+            lo_rows[a * words + w] = rl;
+            hi_rows[a * words + w] = rh;
+        }
+    }
+}
+
 constexpr uint32_t kLeafBit = 0x80000000u;
 PT_HD bool ref_is_leaf(uint32_t r) { return (r & kLeafBit) != 0u; }
 PT_HD uint32_t leaf_first(uint32_t r) { return (r & 0x7FFFFFFFu) >> 4; }

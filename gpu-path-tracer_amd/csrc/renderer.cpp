@@ -51,7 +51,8 @@ Renderer::Renderer() {
     tuning_.trace_refill = 16;
     tuning_.bvh_width = 4;
     tuning_.bvh_builder = PT_BVH_BUILDER_HOST;
-    tuning_.bvh_quantized = 2;          // shadow rays on 64-byte quantized nodes
+    tuning_.bvh_quantized = 6;          // shadow rays on 8-wide quantized nodes (64-byte 4-wide ones
+                                        // where the tree is built on the device)
 }
 
 Renderer::~Renderer() {
@@ -152,6 +153,7 @@ SceneDev ResidentScene::dev() const {
     SceneDev s{};
     s.nodes = nodes.as<void>();
     s.qnodes = quantized ? qnodes.as<void>() : nullptr;
+    s.qnodes8 = depth8 ? qnodes8.as<void>() : nullptr;
     s.tris = tris.as<TriRecord>();
     s.shade = shade.as<ShadeRecord>();
     s.materials = materials.as<MaterialRecord>();
@@ -246,10 +248,10 @@ pt_status Renderer::configure_trace() {
             for (const auto& sc : scenes_) {
                 if (!sc) continue;
                 any_scene = true;
-                occ[m][a] = std::min(occ[m][a], trace_occupancy(cfg, a != 0, false,
-                                                                (sc->quantized & (a ? 2u : 1u)) != 0, m != 0));
+                const int kind = (a && sc->depth8) ? 2 : (sc->quantized & (a ? 2u : 1u)) ? 1 : 0;
+                occ[m][a] = std::min(occ[m][a], trace_occupancy(cfg, a != 0, false, kind, m != 0));
             }
-            if (!any_scene) occ[m][a] = trace_occupancy(cfg, a != 0, false, false, m != 0);
+            if (!any_scene) occ[m][a] = trace_occupancy(cfg, a != 0, false, 0, m != 0);
             occ[m][a] = std::max(1, occ[m][a]);
         }
     const uint32_t cus = (uint32_t)std::max(1, cu_count_);
@@ -275,7 +277,10 @@ pt_status Renderer::configure_trace() {
     // pushes at most W-1 siblings
     stack_total_ = 3u + 2u;
     for (const auto& sc : scenes_)
-        if (sc) stack_total_ = std::max(stack_total_, (sc->host.bvh.width - 1u) * sc->host.bvh.depth + 2u);
+        if (sc) {
+            stack_total_ = std::max(stack_total_, (sc->host.bvh.width - 1u) * sc->host.bvh.depth + 2u);
+            if (sc->depth8) stack_total_ = std::max(stack_total_, 7u * sc->depth8 + 2u);
+        }
     cfg.spill_depth = stack_total_ > cfg.lds_stack ? stack_total_ - cfg.lds_stack : 0u;
     // two spill areas, each sized for the larger of its two grids: the any-hit
     // traversal of one bounce runs concurrently with the next bounce's
@@ -329,6 +334,8 @@ pt_status Renderer::upload_scene(uint32_t id, const pt_scene_desc& desc) {
     auto rs = std::make_unique<ResidentScene>();
     SceneError err{PT_OK, ""};
     const bool gpu_bvh = tuning_.bvh_builder == PT_BVH_BUILDER_GPU && desc.triangle_count > 0;
+    // shadow rays on 8-wide quantized nodes: a collapse of the host's binary tree
+    bo.wide8 = (tuning_.bvh_quantized & 4u) != 0 && bo.width == 4 && !gpu_bvh;
     if (!build_host_scene(desc, bo, rs->host, err, !gpu_bvh)) return fail(err.status, err.message);
     HostScene& h = rs->host;
     auto upload = [&](DeviceBuffer& b, const void* src, size_t bytes, const char* what) -> pt_status {
@@ -388,6 +395,13 @@ pt_status Renderer::upload_scene(uint32_t id, const pt_scene_desc& desc) {
             PT_TRY_HIP(hipStreamSynchronize(stream_), "quantized BVH nodes");
         }
         rs->quantized = tuning_.bvh_quantized;
+    }
+    if (!h.bvh.qnodes8.empty()) {
+        // the 8-wide quantized collapse of the same binary tree, for shadow rays
+        PT_TRY(upload(rs->qnodes8, h.bvh.qnodes8.data(), h.bvh.qnodes8.size() * sizeof(Bvh8QNode),
+                      "8-wide quantized BVH nodes"));
+        PT_TRY_HIP(hipStreamSynchronize(stream_), "8-wide quantized BVH nodes");
+        rs->depth8 = h.bvh.depth8;
     }
     if (!gpu_bvh) {
         PT_TRY(upload(rs->tris, h.tris.data(), h.tris.size() * sizeof(TriRecord), "triangle records"));
@@ -575,7 +589,8 @@ pt_status Renderer::run_views(const std::vector<ViewRecord>& views, uint32_t max
     for (const ViewRecord& v : views) {
         mode.trace_multi = mode.trace_multi || v.scene != mode.scene;
         if (scenes_[v.scene]->host.bvh.width != scenes_[mode.scene]->host.bvh.width ||
-            scenes_[v.scene]->quantized != scenes_[mode.scene]->quantized)
+            scenes_[v.scene]->quantized != scenes_[mode.scene]->quantized ||
+            (scenes_[v.scene]->depth8 != 0) != (scenes_[mode.scene]->depth8 != 0))
             return fail(PT_ERROR_INVALID_ARGUMENT, "the scenes of a batch must share one BVH node format");
     }
     mode.guided = opt.guided;
@@ -1162,6 +1177,8 @@ pt_status Renderer::GetBvhInfo(pt_bvh_info& info) {
     info.stack_lds = trace_.lds_stack;
     info.stack_total = trace_.lds_stack + trace_.spill_depth;
     info.node_width = host0().bvh.width;
+    info.bvh8_node_count = scenes_[0]->depth8 ? (uint32_t)host0().bvh.qnodes8.size() : 0u;
+    info.bvh8_depth = scenes_[0]->depth8;
     return PT_OK;
 }
 
@@ -1192,6 +1209,19 @@ pt_status Renderer::GetBvh4(void* nodes, uint32_t node_cap, uint32_t* order, uin
         if (order_cap < host0().bvh.order.size()) return fail(PT_ERROR_INVALID_ARGUMENT, "order capacity too small");
         std::memcpy(order, host0().bvh.order.data(), host0().bvh.order.size() * sizeof(uint32_t));
     }
+    return PT_OK;
+}
+
+pt_status Renderer::GetBvh8Quantized(void* nodes, uint32_t node_cap, uint32_t& count) {
+    if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
+    const ResidentScene& rs = *scenes_[0];
+    count = (uint32_t)rs.host.bvh.qnodes8.size();
+    if (!rs.depth8) return fail(PT_ERROR_INVALID_STATE, "scene 0 was uploaded without 8-wide quantized nodes");
+    if (!nodes) return PT_OK;
+    if (node_cap < count) return fail(PT_ERROR_INVALID_ARGUMENT, "node capacity too small");
+    PT_TRY_HIP(hipMemcpyAsync(nodes, rs.qnodes8.as<void>(), (size_t)count * sizeof(Bvh8QNode), hipMemcpyDeviceToHost,
+                              stream_), "8-wide quantized BVH nodes");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "BVH read-back");
     return PT_OK;
 }
 
@@ -1227,7 +1257,7 @@ pt_status Renderer::SetTuning(const pt_tuning& t) {
         return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_builder must be PT_BVH_BUILDER_HOST or PT_BVH_BUILDER_GPU");
     if (t.bvh_builder == PT_BVH_BUILDER_GPU && t.bvh_width != 4)
         return fail(PT_ERROR_INVALID_ARGUMENT, "the GPU BVH builder emits 4-wide nodes");
-    if (t.bvh_quantized > 3) return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_quantized must be in [0, 3]");
+    if (t.bvh_quantized > 7) return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_quantized must be in [0, 7]");
     if (t.shade_block != 256) return fail(PT_ERROR_INVALID_ARGUMENT, "shade_block must be 256");
     if (t.trace_blocks_per_cu > 64 || t.shadow_blocks_per_cu > 64)
         return fail(PT_ERROR_INVALID_ARGUMENT, "traversal blocks per CU must be at most 64");

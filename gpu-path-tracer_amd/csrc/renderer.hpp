@@ -55,7 +55,9 @@ struct ResidentScene {
     DeviceBuffer nodes, tris, shade, materials, lights, bins;
     DeviceBuffer order;        // GPU-built BVH: original triangle of each BVH slot (the host copy is empty)
     DeviceBuffer qnodes;       // quantized copy of the 4-wide nodes
+    DeviceBuffer qnodes8;      // the same binary tree 8-wide and quantized (any hit), or empty
     uint32_t quantized = 0;    // traversals that use qnodes: bit 0 closest hit, bit 1 any hit
+    uint32_t depth8 = 0;       // levels of the 8-wide tree (0: none)
     SceneDev dev() const;
 };
 
@@ -125,6 +127,7 @@ public:
     pt_status GetQueueOccupancy(pt_queue_occupancy* out, uint32_t cap, uint32_t& count) const;
     pt_status GetBvh4(void* nodes, uint32_t node_cap, uint32_t* order, uint32_t order_cap);
     pt_status GetBvh4Quantized(void* nodes, uint32_t node_cap);
+    pt_status GetBvh8Quantized(void* nodes, uint32_t node_cap, uint32_t& count);
     pt_status GetTuning(pt_tuning& t) const;
     pt_status SetTuning(const pt_tuning& t);
 

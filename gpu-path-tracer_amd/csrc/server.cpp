@@ -45,8 +45,10 @@ pt_status Renderer::ServerStart(const pt_server_config& cfg_in) {
         if (scenes_[k]) first_scene = k;
     if (first_scene == kMaxScenes) return fail(PT_ERROR_INVALID_STATE, "no resident scene");
     for (uint32_t k = 0; k < kMaxScenes; k++)
-        if (scenes_[k] && scenes_[k]->host.bvh.width != scenes_[first_scene]->host.bvh.width)
-            return fail(PT_ERROR_INVALID_STATE, "the resident scenes must share one BVH node width");
+        if (scenes_[k] && (scenes_[k]->host.bvh.width != scenes_[first_scene]->host.bvh.width ||
+                           scenes_[k]->quantized != scenes_[first_scene]->quantized ||
+                           (scenes_[k]->depth8 != 0) != (scenes_[first_scene]->depth8 != 0)))
+            return fail(PT_ERROR_INVALID_STATE, "the resident scenes must share one BVH node format");
     pt_server_config cfg = cfg_in;
     if (cfg.max_requests == 0) cfg.max_requests = kDefaultMaxRequests;
     if (cfg.max_requests > kMaxRequests)

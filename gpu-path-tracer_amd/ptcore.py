@@ -118,7 +118,8 @@ ALIAS_DTYPE = np.dtype([("q", "<f4"), ("p", "<f4"), ("alias", "<u4"), ("value", 
 class BvhInfo(C.Structure):
     _fields_ = [("node_count", C.c_uint32), ("leaf_count", C.c_uint32), ("max_depth", C.c_uint32),
                 ("max_leaf_size", C.c_uint32), ("triangle_count", C.c_uint32), ("light_count", C.c_uint32),
-                ("stack_lds", C.c_uint32), ("stack_total", C.c_uint32), ("node_width", C.c_uint32)]
+                ("stack_lds", C.c_uint32), ("stack_total", C.c_uint32), ("node_width", C.c_uint32),
+                ("bvh8_node_count", C.c_uint32), ("bvh8_depth", C.c_uint32)]
 
 
 class QueueOccupancy(C.Structure):
@@ -202,7 +203,10 @@ SIGNATURES = {
     "pt_write_image": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_int]),
     "pt_save_frame": (C.c_int, [_P, C.c_char_p]),
     "pt_quantize_bvh4": (C.c_int, [_P, C.c_uint32, _P]),
+    "pt_build_bvh8_quantized": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.POINTER(C.c_uint32), _P,
+                                          C.POINTER(C.c_uint32)]),
     "pt_get_bvh4_quantized": (C.c_int, [_P, _P, C.c_uint32]),
+    "pt_get_bvh8_quantized": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "pt_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "pt_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
 }
@@ -583,6 +587,14 @@ class Renderer:
         self._check(self.lib.pt_get_bvh4(self.h, _ptr(nodes), info["node_count"], _ptr(order), len(order)))
         return nodes, order
 
+    def bvh8_quantized(self):
+        """(N, 32) uint32 rows of the 128-byte 8-wide quantized nodes of scene 0."""
+        n = C.c_uint32(0)
+        self._check(self.lib.pt_get_bvh8_quantized(self.h, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 32), np.uint32)
+        self._check(self.lib.pt_get_bvh8_quantized(self.h, _ptr(out), n.value, C.byref(n)))
+        return out
+
     def bvh4_quantized(self):
         """(N, 16) uint32 rows of the 64-byte quantized nodes scene 0 is traversed with."""
         n = self.bvh_info()["node_count"]
@@ -616,6 +628,24 @@ def build_bvh4(positions, max_leaf_size=0):
                            C.byref(depth))
     if st != PT_OK:
         raise PtError(st, "pt_build_bvh4 failed")
+    return nodes, order, depth.value
+
+
+def build_bvh8_quantized(positions, max_leaf_size=0):
+    """Host-only 8-wide quantized build: (nodes (N, 32) uint32 rows of 128 B,
+    prim order (that of build_bvh4), depth)."""
+    lib = load_library()
+    P = np.ascontiguousarray(positions, np.float32)
+    n, depth = C.c_uint32(), C.c_uint32()
+    st = lib.pt_build_bvh8_quantized(_ptr(P), len(P), max_leaf_size, None, 0, C.byref(n), None, C.byref(depth))
+    if st != PT_OK:
+        raise PtError(st, "pt_build_bvh8_quantized failed")
+    nodes = np.zeros((n.value, 32), np.uint32)
+    order = np.zeros(len(P), np.uint32)
+    st = lib.pt_build_bvh8_quantized(_ptr(P), len(P), max_leaf_size, _ptr(nodes), n.value, C.byref(n), _ptr(order),
+                                     C.byref(depth))
+    if st != PT_OK:
+        raise PtError(st, "pt_build_bvh8_quantized failed")
     return nodes, order, depth.value
 
 

@@ -211,6 +211,8 @@ typedef struct pt_bvh_info {
     uint32_t stack_lds;       /* traversal-stack entries held in LDS per lane */
     uint32_t stack_total;     /* worst-case stack entries (LDS + spill) */
     uint32_t node_width;      /* 4 or 2 children per node */
+    uint32_t bvh8_node_count; /* 128-byte 8-wide quantized nodes traversed by any hit (0: none built) */
+    uint32_t bvh8_depth;      /* their levels */
 } pt_bvh_info;
 
 /* Kernel launch and build parameters picked by the autotuner. */
@@ -228,7 +230,9 @@ typedef struct pt_tuning {
     uint32_t bvh_builder;        /* PT_BVH_BUILDER_*: where the next uploads build their BVH */
     uint32_t bvh_quantized;      /* traversals on 64-byte nodes whose child boxes are 8-bit offsets
                                     (conservative, results unchanged): bit 0 closest hit, bit 1 any
-                                    hit; default 2; 4-wide trees only; next upload */
+                                    hit; bit 2: any hit on 128-byte 8-wide quantized nodes instead,
+                                    a collapse of the host builder's tree; 4-wide trees only; next
+                                    upload; default 6 */
     uint32_t shadow_blocks_per_cu;/* persistent grid of the any-hit traversal, blocks per CU; 0 =
                                     trace_blocks_per_cu if that is set, else as many as fit, less one
                                     wave per SIMD where the launch overlaps a closest-hit launch: in
@@ -506,6 +510,10 @@ pt_status pt_get_bvh4(pt_renderer* renderer, void* nodes, uint32_t node_capacity
 /* The quantized nodes of scene 0 (pt_tuning.bvh_quantized != 0 at upload), 64 B each,
  * as uploaded (node_count of pt_get_bvh_info); PT_ERROR_INVALID_STATE without them. */
 pt_status pt_get_bvh4_quantized(pt_renderer* renderer, void* nodes, uint32_t node_capacity);
+/* The 8-wide quantized nodes of scene 0 (pt_tuning.bvh_quantized bit 2 at a host-built
+ * upload), 128 B each, as uploaded; *node_count receives their number (nodes == NULL:
+ * only the count).  PT_ERROR_INVALID_STATE without them. */
+pt_status pt_get_bvh8_quantized(pt_renderer* renderer, void* nodes, uint32_t node_capacity, uint32_t* node_count);
 
 /* Host-only BVH build of the core (no GPU needed): the 128-byte BVH4 nodes and
  * BVH-order -> original prim permutation that InitializeSceneBuffers uploads.
@@ -514,6 +522,15 @@ pt_status pt_get_bvh4_quantized(pt_renderer* renderer, void* nodes, uint32_t nod
 pt_status pt_build_bvh4(const float* positions, uint32_t triangle_count, uint32_t max_leaf_size,
                         void* nodes, uint32_t node_capacity, uint32_t* node_count,
                         uint32_t* prim_order, uint32_t* depth);
+
+/* The same build collapsed into 8-wide quantized nodes (128 B each: box corner,
+ * three scale exponents, 8-bit bounds of eight children, eight references; the
+ * nodes the any-hit traversal walks when pt_tuning.bvh_quantized has bit 2), with
+ * the same leaves and triangle order as pt_build_bvh4.  Host only; no reference
+ * counterpart.  With nodes == NULL only *node_count and *depth are written. */
+pt_status pt_build_bvh8_quantized(const float* positions, uint32_t triangle_count, uint32_t max_leaf_size,
+                                  void* nodes, uint32_t node_capacity, uint32_t* node_count,
+                                  uint32_t* prim_order, uint32_t* depth);
 
 /* The 64-byte quantized form of `count` BVH4 nodes of pt_build_bvh4 (the nodes
  * traversed when pt_tuning.bvh_quantized is set; no reference counterpart):

@@ -30,6 +30,7 @@ def lib():
             "orc_classify": (None, [_P, _P, _U32, _P, _U32]),
             "orc_bvh4_closest": (None, [_P, _P, _P, _P, _U32, _P, _P, _P, _P, _U32]),
             "orc_bvh4q_any": (None, [_P, _P, _P, _P, _U32, _P, _P, _P, _U32]),
+            "orc_bvh8q_any": (None, [_P, _P, _P, _P, _U32, _P, _P, _P, _P, _U32]),
             "orc_alias_build": (None, [_P, _U32, _P]),
             "orc_alias_sample": (None, [_P, _U32, _P, _U32, _P, _P]),
             "orc_scene_alias": (None, [_P, _P]),
@@ -136,6 +137,18 @@ class OracleScene:
         lib().orc_bvh4q_any(self.h, _p(qnodes), _p(order), _p(rays), n, _p(occ), _p(nv), _p(nt),
                             threads or default_threads())
         return occ, nv, nt
+
+    def bvh8q_any(self, qnodes, order, rays, threads=None):
+        """Any hit over the core's 8-wide quantized BVH in the GPU kernel's
+        order: occlusion, per-ray node / triangle visits and peak stack depth."""
+        qnodes = np.ascontiguousarray(qnodes, np.uint32)
+        order = np.ascontiguousarray(order, np.uint32)
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = len(rays)
+        occ, nv, nt, ms = (np.zeros(n, np.uint32) for _ in range(4))
+        lib().orc_bvh8q_any(self.h, _p(qnodes), _p(order), _p(rays), n, _p(occ), _p(nv), _p(nt), _p(ms),
+                            threads or default_threads())
+        return occ, nv, nt, ms
 
     def alias_table(self):
         bins = np.zeros(self.info["lights"], BIN_DTYPE)

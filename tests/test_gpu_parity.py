@@ -318,27 +318,70 @@ def test_c2_any_hit_matches_oracle(c2_pair):
     assert np.array_equal(r.trace_any(rays), o.any(rays))
 
 
+def _any_hit_walk(r, o, rays):
+    """The oracle's walk of the any-hit tree a renderer traverses: the 8-wide
+    quantized nodes where they were built, else the 4-wide quantized ones."""
+    _, order = r.bvh4()
+    if r.bvh_info()["bvh8_node_count"]:
+        occ, nv, nt, _ = o.bvh8q_any(r.bvh8_quantized(), order, rays)
+    else:
+        occ, nv, nt = o.bvh4q_any(r.bvh4_quantized(), order, rays)
+    return occ, nv, nt
+
+
 def test_c2_visit_counts_equal_the_oracle_walk(c2_pair):
     """The node and triangle visits behind the roofline's algorithmic bytes:
     per ray, the kernels' COUNT counters equal the oracle's walk of the same
-    uploaded trees in the same order, for any hit over the 64-byte quantized
+    uploaded trees in the same order, for any hit over the default quantized
     nodes (262,144 shadow segments to the emitters) and closest hit over the
     128-byte nodes (262,144 random rays); the any-hit results equal brute force
     on a subset."""
     s, r, o = c2_pair
     nodes, order = r.bvh4()
-    q = r.bvh4_quantized()
     sh = scenes.shadow_rays(s, 1 << 18, 205)
     occ, nv, nt = r.trace_any_counts(sh)
-    oocc, onv, ont = o.bvh4q_any(q, order, sh)
+    oocc, onv, ont = _any_hit_walk(r, o, sh)
     assert np.array_equal(occ, oocc) and np.array_equal(nv, onv) and np.array_equal(nt, ont)
     assert np.array_equal(occ[:512], o.any(sh[:512], brute=True))
     rays = scenes.random_rays(1 << 18, (-10, -10, -10), (110, 110, 110), 206)
-    hits, nv, nt = r.trace_closest_counts(rays)
-    ohits, onv, ont, _ = o.bvh4_closest(nodes, order, rays)
-    assert np.array_equal(hits, ohits) and np.array_equal(nv, onv) and np.array_equal(nt, ont)
+    hits, cnv, cnt = r.trace_closest_counts(rays)
+    ohits, ocnv, ocnt, _ = o.bvh4_closest(nodes, order, rays)
+    assert np.array_equal(hits, ohits) and np.array_equal(cnv, ocnv) and np.array_equal(cnt, ocnt)
     print(f"any hit: {nv.mean():.3f} nodes, {nt.mean():.3f} triangles per shadow segment; "
-          f"closest hit: {onv.mean():.3f} nodes, {ont.mean():.3f} triangles per random ray")
+          f"closest hit: {cnv.mean():.3f} nodes, {cnt.mean():.3f} triangles per random ray")
+
+
+@pytest.mark.parametrize("quantized", [2, 6])
+def test_c2_any_hit_node_formats_equal_the_oracle_walk(c2_pair, quantized):
+    """Any hit over the 64-byte 4-wide quantized nodes (tuning bit 1) and over
+    the 128-byte 8-wide ones (bit 2): per ray, occlusion and node / triangle
+    visits equal the oracle's walk of the uploaded tree on 262,144 shadow
+    segments, occlusion equals the oracle's on 2^20, and the 8-wide walk costs
+    fewer node visits than the 4-wide one."""
+    s, _, o = c2_pair
+    with ptcore.Renderer() as r:
+        r.set_tuning(bvh_quantized=quantized)
+        r.InitializeRenderingBuffers(64, 64)
+        r.InitializeSceneBuffers(s)
+        info = r.bvh_info()
+        _, order = r.bvh4()
+        q4 = r.bvh4_quantized()
+        sh = scenes.shadow_rays(s, 1 << 18, 205)
+        occ, nv, nt = r.trace_any_counts(sh)
+        if quantized & 4:
+            q8 = r.bvh8_quantized()
+            assert info["bvh8_node_count"] == len(q8) and 1 <= info["bvh8_depth"] <= info["max_depth"]
+            oocc, onv, ont, ms = o.bvh8q_any(q8, order, sh)
+            assert ms.max() <= 7 * info["bvh8_depth"] + 2
+            _, nv4, _ = o.bvh4q_any(q4, order, sh)
+            assert nv.mean() < nv4.mean()
+        else:
+            assert info["bvh8_node_count"] == 0
+            oocc, onv, ont = o.bvh4q_any(q4, order, sh)
+        assert np.array_equal(occ, oocc) and np.array_equal(nv, onv) and np.array_equal(nt, ont)
+        big = scenes.shadow_rays(s, 1 << 20, 203)
+        assert np.array_equal(r.trace_any(big), o.any(big))
+        print(f"bvh_quantized={quantized}: {nv.mean():.3f} nodes, {nt.mean():.3f} triangles per shadow segment")
 
 
 def test_c2_queue_membership_and_alias(c2_pair, oracle_lib):

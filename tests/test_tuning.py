@@ -1,5 +1,6 @@
 """The tuning space of the traversal: node width 2 (64-byte nodes) or 4
-(128-byte nodes, or 64-byte quantized ones) and block sizes 64..512.  Every
+(128-byte nodes, 64-byte quantized ones, or for any hit 128-byte 8-wide
+quantized ones) and block sizes 64..512.  Every
 point must return the same closest hits, bit for bit (the (t, prim) minimum
 does not depend on the tree or on how conservative its boxes are), and the
 same images."""
@@ -11,9 +12,11 @@ import scenes
 
 pytestmark = pytest.mark.gpu
 
-# (node width, block, bvh_quantized: bit 0 closest hit, bit 1 any hit on 64-byte quantized nodes)
+# (node width, block, bvh_quantized: bit 0 closest hit, bit 1 any hit on 64-byte quantized nodes,
+#  bit 2 any hit on 128-byte 8-wide quantized nodes)
 POINTS = [(4, 256, 0), (4, 512, 0), (2, 64, 2), (2, 256, 0), (2, 512, 2), (4, 64, 0), (4, 128, 2),
-          (4, 256, 3), (4, 64, 1), (4, 512, 3), (4, 256, 2)]
+          (4, 256, 3), (4, 64, 1), (4, 512, 3), (4, 256, 2), (4, 256, 6), (4, 512, 7), (4, 64, 4),
+          (2, 256, 6)]
 
 
 @pytest.fixture(scope="module")
@@ -34,6 +37,7 @@ def test_tuning_point_is_bit_exact(cornell, cornell_oracle, rays, brute, width, 
         r.InitializeSceneBuffers(cornell)
         info = r.bvh_info()
         assert info["node_width"] == width
+        assert (info["bvh8_node_count"] > 0) == bool(quantized & 4 and width == 4)
         assert np.array_equal(r.trace_closest(rays), brute)
         sh = scenes.shadow_rays(cornell, 65536, 303)
         assert np.array_equal(r.trace_any(sh), cornell_oracle.any(sh))
@@ -120,7 +124,7 @@ def test_width2_production_scene_hits(oracle_lib):
 
 
 def test_invalid_tuning_is_rejected(gpu_renderer):
-    for bad in (dict(bvh_width=3), dict(trace_block=1024), dict(trace_block=96), dict(bvh_quantized=4),
+    for bad in (dict(bvh_width=3), dict(trace_block=1024), dict(trace_block=96), dict(bvh_quantized=8),
                 dict(trace_blocks_per_cu=65), dict(shadow_blocks_per_cu=65)):
         with pytest.raises(ptcore.PtError) as e:
             gpu_renderer.set_tuning(**bad)
