@@ -20,6 +20,12 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-path-tracer_amd"))
 
 METRIC = "Mrays/s (extension+shadow) on 1 MI355X at equal relMSE; p50/p99 request latency"
 HBM_PEAK_GBS = 8000.0
+BF16_MFMA_PEAK_TFLOPS = 2500.0     # dense bf16 MFMA of one MI355X
+# one training record of the guide network (4 layers of 64 x 64, bf16 MFMA):
+# the forward pass (4 products), the weight gradients (4) and the input deltas
+# of layers 3..1 (3), 2 * 64 * 64 flops each
+GUIDE_TRAIN_FLOPS_PER_RECORD = 11 * 2 * 64 * 64
+GUIDE_PARAMS = 16640               # fp32 partial gradient of one 64-record block
 # algorithmic bytes of one closest-hit ray: ray record (origin|tMin, direction|tMax)
 # 32 B in, hit record + queue id 17 B out, 128 B per BVH4 node, 48 B per triangle
 RAY_IN_B, HIT_OUT_B, NODE_B, TRI_B = 32, 17, 128, 48
@@ -79,15 +85,17 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes": n * 4, "repeats": 3}), flush=True)
 
 
-def measured_traffic(kind):
+def measured_traffic(kind, node_bytes):
     """HBM bytes per launch of the closest-hit ("closest") or any-hit ("any")
     traversal from the committed rocprofv3 --pmc summary of this workload
-    (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration), or None."""
+    (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration), or None when
+    there is none or it was measured on another node format."""
     path = os.path.join(ROOT, "profiles", f"traffic_{kind}_production.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)
+        t = json.load(f)
+    return t if t.get("node_bytes") == node_bytes else None
 
 
 # ---------------------------------------------------------------- rocprofv3 summaries
@@ -115,24 +123,64 @@ def _csvs(d, suffix):
     return sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True))
 
 
-def summarize_kernel_trace(d, command, title):
+def summarize_kernel_trace(d, command, title, bench_json=None):
     """Markdown table (calls, total, average, share) per kernel from the
-    *_kernel_trace.csv of a rocprofv3 --kernel-trace run in directory d."""
+    *_kernel_trace.csv of a rocprofv3 --kernel-trace run in directory d; with
+    the bench line of the same run, its live launch timing, roofline and
+    per-iteration queue occupancy beside it."""
     import csv
-    rows = {}
+    disp = []
     for path in _csvs(d, "_kernel_trace.csv"):
         with open(path) as f:
             for rec in csv.DictReader(f):
-                k = _short(rec["Kernel_Name"])
-                dt = (int(rec["End_Timestamp"]) - int(rec["Start_Timestamp"])) * 1e-3   # us
-                c = rows.setdefault(k, [0, 0.0])
-                c[0] += 1
-                c[1] += dt
+                disp.append((int(rec["Start_Timestamp"]), int(rec["End_Timestamp"]), _short(rec["Kernel_Name"])))
+    disp.sort()
+    # a dispatch runs alone when no other dispatch's interval meets its own
+    alone = [True] * len(disp)
+    for i, (s, e, _) in enumerate(disp):
+        j = i + 1
+        while j < len(disp) and disp[j][0] < e:
+            alone[i] = alone[j] = False
+            j += 1
+    rows = {}
+    for (s, e, k), a in zip(disp, alone):
+        dt = (e - s) * 1e-3   # us
+        c = rows.setdefault(k, [0, 0.0, 0, 0.0])
+        c[0] += 1
+        c[1] += dt
+        if a and dt >= 20.0:
+            c[2] += 1
+            c[3] += dt
     total = sum(v[1] for v in rows.values()) or 1.0
     out = [f"# {title}", "", "Command (MI355X, from the repo root, `export TMPDIR=/tmp`):", "", "```", command, "```", "",
-           "| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
-    for k, (n, t) in sorted(rows.items(), key=lambda kv: -kv[1][1]):
-        out.append(f"| `{k}` | {n} | {t / 1e3:.2f} | {t / n:.1f} | {100 * t / total:.1f} |")
+           "Alone: the dispatches of at least 20 us that overlap no other dispatch (the traversals of the profiled "
+           "frame, which runs its stages one after another; in the replayed bounce loop each any-hit launch "
+           "overlaps the next closest-hit launch).", "",
+           "| kernel | calls | total ms | avg us | % | alone calls | alone avg us |",
+           "|---|---:|---:|---:|---:|---:|---:|"]
+    for k, (n, t, na, ta) in sorted(rows.items(), key=lambda kv: -kv[1][1]):
+        out.append(f"| `{k}` | {n} | {t / 1e3:.2f} | {t / n:.1f} | {100 * t / total:.1f} | {na} | "
+                   f"{(ta / na if na else 0.0):.1f} |")
+    if bench_json:
+        with open(bench_json) as f:
+            line = json.loads([l for l in f if l.startswith("{")][-1])
+        out += ["", "The same run's bench line (hipEvents on the renderer's stream, profiled frame):", ""]
+        for key in ("roofline", "roofline_closest", "roofline_any"):
+            r = line.get(key)
+            if r:
+                out.append(f"* `{key}`: {r['kernel']}, {r['avg_launch_ms']} ms per launch of {r['rays_per_launch']:.0f} "
+                           f"rays, {r['nodes_per_ray']} nodes ({r['node_bytes']} B) + {r['tris_per_ray']} triangles "
+                           f"= {r['bytes_per_ray']} B per ray: {r['achieved']} GB/s algorithmic, frac {r['frac']}"
+                           + (f"; measured HBM {r['hbm_achieved']} GB/s, hbm_frac {r['hbm_frac']}"
+                              if r.get("hbm_achieved") else ""))
+        occ = line.get("queue_occupancy")
+        if occ:
+            out += ["", f"Queue occupancy per wavefront iteration (pt_get_queue_occupancy; path pool "
+                        f"{occ['capacity']} slots; extension and shadow rays queued against the pool; hit shares "
+                        f"over the frame: {occ['share']}):", "",
+                    "| iteration | extension rays / pool | shadow rays / pool |", "|---:|---:|---:|"]
+            for i, (e, s) in enumerate(zip(occ["ray_fill"], occ["shadow_fill"])):
+                out.append(f"| {i} | {e:.4f} | {s:.4f} |")
     return "\n".join(out) + "\n"
 
 
@@ -210,6 +258,7 @@ def summarize_pmc(cal_fetch, cal_write, dirs, bench_json, command):
                              "write_scale": round(ws, 4), "rays_per_launch": rays,
                              "hbm_bytes_per_ray": round((rd + wr) / rays, 1),
                              "algorithmic_bytes_per_ray": rpl[kind]["bytes_per_ray"],
+                             "node_bytes": rpl[kind]["node_bytes"],
                              "l2_hit": None if l2 is None else round(l2, 3),
                              "source": "profiles/pmc_production_1080p64spp.md"}
     head = ["# PMC counters: production frame (983,042 triangles, 1920x1080, 64 spp)", "",
@@ -527,7 +576,21 @@ def run_guide(a):
         ratio_w = e_w * sw["render_ms"] / (e_p2 * sp2["render_ms"])
         log(f"guide: NEE {nee} next frame, network kept: guided {sw['render_ms']:.1f} ms (train {gw['train_ms']:.1f} ms) "
             f"nMSE {e_w:.4g} vs plain {sp2['render_ms']:.1f} ms nMSE {e_p2:.4g}; time to equal error {ratio_w:.3f}x")
+        # the training kernel against the bf16 MFMA peak: one launch is a
+        # forward + backward over `batch` records; each 64-record block also
+        # writes its fp32 partial gradient, which the Adam step sums
+        batch = 16384
+        us = gs["train_kernel_us"]
+        tflops = GUIDE_TRAIN_FLOPS_PER_RECORD * batch / (us * 1e-6) / 1e12 if us else 0.0
+        hbm = (batch // 64 * GUIDE_PARAMS * 4 + batch * 96) / (us * 1e-6) / 1e9 if us else 0.0
+        train_roof = {"bound": "mfma", "kernel": "k_guide_train", "achieved": round(tflops, 2),
+                      "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / BF16_MFMA_PEAK_TFLOPS, 4),
+                      "flops_per_record": GUIDE_TRAIN_FLOPS_PER_RECORD, "records_per_launch": batch,
+                      "avg_launch_us": round(us, 2), "hbm_gbs": round(hbm, 1)}
+        log(f"guide: k_guide_train {us:.1f} us per {batch}-record launch = {tflops:.1f} TFLOP/s bf16 "
+            f"({100 * tflops / BF16_MFMA_PEAK_TFLOPS:.2f} % of peak), {hbm:.0f} GB/s of partial gradients and records")
         out_cfgs.append({"use_nee": nee, "spp": spp, "plain_ms": round(sp["render_ms"], 2), "plain_nmse": e_p,
+                         "roofline_train": train_roof,
                          "guided_ms": round(sg["render_ms"], 2), "guided_train_ms": round(gs["train_ms"], 2),
                          "guided_nmse": e_g, "error_ratio_equal_samples": round(e_g / e_p, 4),
                          "time_to_equal_error_ratio": round(ratio, 4),
@@ -545,7 +608,7 @@ def run_guide(a):
                                   f"0.5 m gap, 1280x720, {a.guide_spp} spp in 4 passes, depth 8; nMSE against "
                                   f"{a.guide_ref_spp} spp",
                       "network": "MLP 64-64-64-64 BF16 (FP32 accumulate), 8 vMF lobes, one-sample MIS alpha 0.5"},
-           "runs": out_cfgs}
+           "roofline": out_cfgs[0]["roofline_train"], "runs": out_cfgs}
     print(json.dumps(out), flush=True)
     r.close()
 
@@ -588,7 +651,7 @@ def main():
         calibrate_copy()
         return
     if a.summarize_kernel_trace:
-        text = summarize_kernel_trace(a.summarize_kernel_trace, a.command, a.title)
+        text = summarize_kernel_trace(a.summarize_kernel_trace, a.command, a.title, a.bench_json)
         if a.out:
             with open(a.out, "w") as f:
                 f.write(text)
@@ -698,7 +761,7 @@ def main():
     sh_achieved = sh_bytes_per_launch / (sh_avg_ms * 1e-3) / 1e9
 
     def roof(kind, kernel, achieved_gbs, bpr, nodes, tris, avg_ms, rays_per_launch, node_bytes):
-        traffic = measured_traffic(kind)
+        traffic = measured_traffic(kind, node_bytes)
         # the same launch's measured HBM bytes (rocprofv3 counters, per launch)
         # over its live duration: what HBM actually carried
         hbm = None

@@ -352,13 +352,23 @@ pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& c
     hipEvent_t t0 = nullptr, t1 = nullptr;
     PT_TRY_HIP(hipEventCreate(&t0), "event");
     PT_TRY_HIP(hipEventCreate(&t1), "event");
+    // the first kTimedTrainSteps training launches of each pass are bracketed
+    // by their own events: the training kernel's duration without the Adam step
+    constexpr uint32_t kTimedTrainSteps = 8;
+    hipEvent_t tk[2 * kTimedTrainSteps] = {};
     struct EventGuard {
         hipEvent_t a, b;
+        hipEvent_t* k;
         ~EventGuard() {
             (void)hipEventDestroy(a);
             (void)hipEventDestroy(b);
+            for (uint32_t i = 0; i < 2 * kTimedTrainSteps; i++)
+                if (k[i]) (void)hipEventDestroy(k[i]);
         }
-    } guard{t0, t1};
+    } guard{t0, t1, tk};
+    for (auto& e : tk) PT_TRY_HIP(hipEventCreate(&e), "event");
+    double train_kernel_ms = 0.0;
+    uint32_t train_kernel_n = 0;
 
     pt_guide_stats gst{};
     pt_frame_stats total{};
@@ -402,10 +412,12 @@ pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& c
         for (uint32_t k = 0; k < gs.train_steps; k++) {
             g_step_++;
             const uint32_t seed = (uint32_t)(gs.seed * 0x9E3779B1u) ^ (g_step_ * 2654435761u) ^ 1u;
+            if (k < kTimedTrainSteps) PT_TRY_HIP(hipEventRecord(tk[2 * k], stream_), "event");
             PT_TRY_HIP(launch_guide_train(stream_, g, g.recs, count, gs.batch, seed | 1u,
                                           gs.weight_max > 0.0f ? gs.weight_max : 1e30f, g_part_.as<float>(),
                                           g_loss_.as<float>() + k),
                        "guide training");
+            if (k < kTimedTrainSteps) PT_TRY_HIP(hipEventRecord(tk[2 * k + 1], stream_), "event");
             PT_TRY_HIP(launch_guide_adam(stream_, g, g_part_.as<float>(), guide_parts(gs.batch), gs.learning_rate, g_step_,
                                          1.0f / (float)gs.batch),
                        "guide Adam step");
@@ -417,6 +429,12 @@ pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& c
         float ms = 0.0f;
         PT_TRY_HIP(hipEventElapsedTime(&ms, t0, t1), "elapsed");
         gst.train_ms += ms;
+        for (uint32_t k = 0; k < std::min(gs.train_steps, kTimedTrainSteps); k++) {
+            float km = 0.0f;
+            PT_TRY_HIP(hipEventElapsedTime(&km, tk[2 * k], tk[2 * k + 1]), "elapsed");
+            train_kernel_ms += km;
+            train_kernel_n++;
+        }
         total.render_ms += ms;
         if (gst.steps == 0) first_loss = losses[0] / (float)gs.batch;
         last_loss = losses[gs.train_steps - 1] / (float)gs.batch;
@@ -424,6 +442,7 @@ pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& c
         trained = true;
     }
     gst.passes = gs.passes;
+    gst.train_kernel_us = train_kernel_n ? 1e3 * train_kernel_ms / train_kernel_n : 0.0;
     gst.loss_first = first_loss;
     gst.loss_last = last_loss;
 

@@ -420,6 +420,8 @@ typedef struct pt_guide_stats {
     float loss_last;
     uint32_t steps;           /* Adam steps taken in this call */
     uint32_t passes;
+    double train_kernel_us;   /* average duration of one training forward+backward launch (the first 8 of each
+                                 pass, each between its own events; the Adam step not included) */
 } pt_guide_stats;
 
 /* Renders one frame in `passes` passes of growing size; the network is trained

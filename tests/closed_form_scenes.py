@@ -21,7 +21,16 @@ use:
 * ggx floor: a GGX floor under a large emissive rectangle; the expected
   radiance is Le * integral of f(wo, wi) cos(theta_i) over the directions that
   reach the rectangle, computed by quadrature from the published GGX formulas
-  (Walter et al. 2007; Heitz 2014 for the Smith term) in float64.
+  (Walter et al. 2007; Heitz 2014 for the Smith term) in float64;
+* sphere furnace: the inside of a sphere whose triangles are diffuse (albedo
+  rho) or emissive (radiance Le, a fraction f of the area).  Between two points
+  x, y of a sphere of radius R, cos(theta_x) = cos(theta_y) = |x - y| / (2R),
+  so the form factor from x to an area dA is dA / (4 pi R^2) wherever x and dA
+  lie: a cosine-sampled bounce lands on emissive area with probability f, and a
+  light sample by area carries rho f Le exactly.  A diffuse point then reads the
+  multi-bounce series rho f Le (1 + q + ... + q^(n-1)), q = rho (1 - f), with
+  n = depth bounces' light samples under NEE and n = depth - 1 emitter hits
+  without it: the diffuse furnace of the material model.
 """
 import math
 
@@ -229,6 +238,86 @@ def ggx_floor_expected(material, wo, p, le=0.1, n_theta=1200, n_phi=1200):
     f = ggx_f64(material, wol, wi)
     w = (1.0 / n_theta) * (2 * math.pi / n_phi) / 2.0
     return le * (f * inside[:, None]).sum(0) * w
+
+
+# ---------------------------------------------------------------- sphere furnace
+
+FURNACE_R = 2.0                        # every segment stays below the 20-unit shadow clamp
+FURNACE_EMISSIVE = (0, 3, 8, 14, 17)   # base faces of the icosphere that emit: f = 1/4
+
+
+def sphere_furnace(rho=0.8, le=2.0, subdivisions=5):
+    """Inside of an icosphere (20 * 4^s flat triangles, wound and normalled
+    inward), the patches of FURNACE_EMISSIVE emissive and the rest diffuse;
+    camera at the centre."""
+    unit = scenes.icosphere(subdivisions)[:, [0, 2, 1]]
+    per = unit.shape[0] // 20
+    P = unit * FURNACE_R
+    N = -unit
+    Tn = scenes._sphere_tangents(N)
+    T = np.concatenate([Tn, np.ones(Tn.shape[:-1] + (1,))], axis=-1)
+    M = np.zeros(len(P), np.uint32)
+    for f in FURNACE_EMISSIVE:
+        M[f * per:(f + 1) * per] = 1
+    mats = [scenes.material(scenes.MODEL_DIFFUSE, color=(rho, rho, rho)),
+            scenes.material(scenes.MODEL_EMISSIVE, emission=(le, le, le))]
+    pos = (0.0, 0.0, 0.0)
+    view = scenes.View(fov_y=math.radians(70.0), position=pos,
+                       rotation=scenes.look_at_quaternion(pos, (0.3, -0.2, -1.0)))
+    return scenes.Scene(name="sphere_furnace", positions=P.astype(np.float32), normals=N.astype(np.float32),
+                        tangents=T.astype(np.float32), material_ids=M, materials=np.stack(mats), view=view,
+                        settings={})
+
+
+def furnace_fraction(scene):
+    """Emissive share f of the furnace's area (float64 over its fp32 triangles)."""
+    P = scene.positions.astype(np.float64)
+    area = 0.5 * np.linalg.norm(np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), axis=1)
+    return float(area[scene.material_ids == 1].sum() / area.sum())
+
+
+def furnace_expected(rho, le, f, depth, nee):
+    """E[L] of a diffuse point: rho f Le sum_{k < n} (rho (1 - f))^k, n = depth
+    with NEE, depth - 1 without."""
+    n = depth if nee else depth - 1
+    q = rho * (1.0 - f)
+    return rho * f * le * sum(q ** k for k in range(n))
+
+
+def furnace_pixels(scene, closest, w, h, material=0):
+    """Pixels whose four corner rays all hit triangles of `material` (0
+    diffuse, 1 emissive): none straddles an emitter edge.  `closest` maps
+    (n, 8) rays to (n, 4) hit records."""
+    o, llc, hor, ver = camera_basis64(scene.view, w / h)
+    ok = np.ones(w * h, bool)
+    for fx in (0.0, 1.0):
+        for fy in (0.0, 1.0):
+            x, y = np.meshgrid((np.arange(w) + fx) / w, (np.arange(h) + fy) / h)
+            d = llc[None, None] + x[..., None] * hor + y[..., None] * ver - o
+            d /= np.linalg.norm(d, axis=-1, keepdims=True)
+            rays = np.zeros((w * h, 8), np.float32)
+            rays[:, 0:3] = o
+            rays[:, 3] = 1e-4
+            rays[:, 4:7] = d.reshape(-1, 3)
+            rays[:, 7] = 1e34
+            prim = closest(rays)[:, 0]
+            ok &= scene.material_ids[prim] == material
+    return ok
+
+
+def check_furnace(hdr, mask, rho, le, f, depth, nee):
+    """The diffuse pixels' mean within 4 standard errors plus 3e-4 relative
+    (the flat facets bend the sphere's identity by about 1e-4) of the series;
+    with NEE at depth 1 every pixel is within 1e-2 of rho f Le (one light
+    sample per sample, carrying rho f Le up to the facets)."""
+    want = furnace_expected(rho, le, f, depth, nee)
+    g = hdr[mask, :3].astype(np.float64)
+    assert np.allclose(g[:, 0], g[:, 1], rtol=1e-5) and np.allclose(g[:, 0], g[:, 2], rtol=1e-5)
+    v = g[:, 0]
+    se = v.std() / math.sqrt(len(v))
+    assert abs(v.mean() - want) < 4 * se + 3e-4 * want, (v.mean(), want, se)
+    if nee and depth == 1:
+        assert np.allclose(v, want, rtol=1e-2)
 
 
 # ---------------------------------------------------------------- checks shared by the oracle and GPU tests

@@ -39,6 +39,7 @@ SCENES = {
     "plates": cf.plates,
     "corridor": cf.corridor,
     "ggx_floor": lambda: cf.ggx_floor(((0.95, 0.64, 0.54), 0.3, 0.7, 1.5)),
+    "sphere_furnace": cf.sphere_furnace,
 }
 
 
@@ -121,6 +122,23 @@ def test_gpu_mirror_corridor_power_series(gpu, depth, spp, rr):
     hdr, stats = _render(gpu, s, st)
     assert stats["shadow_rays"] == 0
     cf.check_corridor(hdr, s.view, 64, 16, depth, spp, rr)
+
+
+@pytest.mark.parametrize("nee, depth, rr", [(1, 1, 100), (1, 8, 100), (0, 3, 100), (0, 8, 100), (1, 8, 1), (0, 8, 2)])
+def test_gpu_diffuse_sphere_furnace_is_a_geometric_series(gpu, nee, depth, rr):
+    """Inside a sphere a quarter emissive, diffuse pixels read rho f Le
+    (1 + q + ... + q^(n-1)), q = rho (1 - f), n = depth with NEE and depth - 1
+    without; emissive pixels read Le."""
+    rho, le = 0.8, 2.0
+    s = cf.sphere_furnace(rho, le)
+    w = h = 32
+    st = dict(width=w, height=h, samples_per_pixel=256, path_depth=depth, use_nee=nee, rr_start_depth=rr, seed=4)
+    hdr, _ = _render(gpu, s, st)
+    mask = cf.furnace_pixels(s, gpu.trace_closest, w, h)
+    assert mask.sum() > 600
+    cf.check_furnace(hdr, mask, rho, le, cf.furnace_fraction(s), depth, nee)
+    emit = cf.furnace_pixels(s, gpu.trace_closest, w, h, material=1)
+    assert emit.sum() > 50 and np.allclose(hdr[emit, :3], le, rtol=1e-5)
 
 
 @pytest.mark.parametrize("material", [((1.0, 1.0, 1.0), 0.5, 1.0, 1.5), ((0.8, 0.4, 0.2), 0.3, 0.5, 1.5),
