@@ -98,6 +98,9 @@ __device__ __forceinline__ void add_counts(unsigned long long* nodes_ctr, unsign
 #define PT_TRACE_WAVES 6   // minimum waves per SIMD of the traversal kernels: 80 VGPRs, which every
 #endif                     // one-scene kernel fits anyway and the several-scene closest-hit kernel
                            // reaches without spilling (82 otherwise: 5 waves); 7 or 8 measured slower
+#ifndef PT_TOP_TABLE
+#define PT_TOP_TABLE 1     // single-scene traversal blocks read the top of the tree from LDS
+#endif
 #ifndef PT_WHILE_WHILE
 #define PT_WHILE_WHILE 0   // if-if measured faster on gfx950 (one step per outer iteration)
 #endif
@@ -184,12 +187,7 @@ __device__ __forceinline__ void box4(const Bvh4Node* node, const RayFrame& r, fl
     }
 }
 
-// Node kinds of the traversal kernels: fp32 BVH4 (128 B), fp32 BVH2 (64 B) and
-// quantized BVH4 (64 B); `arity` children per node.
-constexpr int kNodeBvh2 = 2;
-constexpr int kNodeBvh4 = 4;
-constexpr int kNodeBvh4Q = 5;
-constexpr int kNodeBvh8Q = 8;
+// `arity` children per node of each node kind.
 template <int K>
 struct NodeOf;
 template <>
@@ -357,7 +355,33 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                                                  const TraceConfig& cfg) {
     using Node = typename NodeOf<NODE>::type;
     __shared__ uint2 stack[LDS][BLOCK];
+    // The top of the tree in LDS (single-scene launches): the whole table from
+    // 256 threads up, 32 bytes per thread below, so that small blocks keep
+    // their occupancy; references past a block's share go back to HBM.
+    constexpr uint32_t kTopMax = kTopBytes / sizeof(Node);
+    constexpr uint32_t kTopCap =
+        (MULTI || !PT_TOP_TABLE) ? 0u : (BLOCK >= 256 ? kTopBytes : (uint32_t)BLOCK * 32u) / (uint32_t)sizeof(Node);
+    __shared__ Node top[kTopCap ? kTopCap : 1];
     const uint32_t tid = threadIdx.x;
+    const void* top_src = sc.top[NODE == kNodeBvh8Q ? 2 : NODE == kNodeBvh4Q ? 1 : 0];
+    const uint32_t root = (kTopCap && top_src) ? kTopBit : 0u;
+    if (kTopCap && top_src) {
+        const uint4* s4 = static_cast<const uint4*>(top_src);
+        uint4* d4 = reinterpret_cast<uint4*>(top);
+        constexpr uint32_t kWords = kTopCap * (uint32_t)(sizeof(Node) / 16);
+        for (uint32_t i = tid; i < kWords; i += BLOCK) d4[i] = s4[i];
+        __syncthreads();
+        if (kTopCap < kTopMax) {
+            const uint32_t* gidx =
+                reinterpret_cast<const uint32_t*>(static_cast<const char*>(top_src) + kTopMax * sizeof(Node));
+            constexpr uint32_t A = (uint32_t)NodeOf<NODE>::arity;
+            for (uint32_t i = tid; i < kTopCap * A; i += BLOCK) {
+                uint32_t& c = top[i / A].child[i % A];
+                if (ref_is_top(c) && (c & (kTopBit - 1u)) >= kTopCap) c = gidx[c & (kTopBit - 1u)];
+            }
+            __syncthreads();
+        }
+    }
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
     const uint32_t stride = gridDim.x * BLOCK;
     const uint32_t cap = min(cfg.stack_cap, (uint32_t)LDS + cfg.spill_depth);
@@ -467,7 +491,7 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                         best_t = tmax;
                         best_u = best_v = 0.0f;
                         nodes = tris = 0;
-                        ref = 0;   // root node
+                        ref = root;
                         sp = 0;
                         fin = false;
                         active = true;
@@ -487,7 +511,8 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                 constexpr int W = NodeOf<NODE>::arity;
                 float t0[W];
                 uint32_t c[W];
-                children(nodes_base + ref, r, t0, c);
+                const Node* np = (kTopCap && (ref & kTopBit)) ? top + (ref & (kTopBit - 1u)) : nodes_base + ref;
+                children(np, r, t0, c);
                 if (COUNT) nodes++;
                 if (!ANY || PT_ANY_SORTED) {
                     // nearest child first: sort the (distance, ref) pairs
@@ -1830,6 +1855,62 @@ hipError_t noalloc_scratch_address(void** ptr) {
     }
     *ptr = addr[dev];
     return hipSuccess;
+}
+
+// Top-of-tree table of a tree (one thread): breadth-first from the root, each
+// internal child that still fits gets the next table slot and its reference
+// in the copied parent becomes kTopBit | slot.
+template <class Node, int A>
+__global__ void k_build_top(const Node* nodes, Node* table) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    constexpr uint32_t kMax = kTopBytes / sizeof(Node);
+    uint32_t* gidx = reinterpret_cast<uint32_t*>(table + kMax);
+    uint32_t count = 1;
+    gidx[0] = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        Node n = nodes[gidx[i]];
+        for (int k = 0; k < A; k++) {
+            const uint32_t c = n.child[k];
+            if (c == kInvalid || ref_is_leaf(c) || count >= kMax) continue;
+            gidx[count] = c;
+            n.child[k] = kTopBit | count;
+            count++;
+        }
+        table[i] = n;
+    }
+    for (uint32_t i = count; i < kMax; i++) {
+        table[i] = Node{};
+        gidx[i] = 0u;
+    }
+}
+
+size_t top_table_bytes(int kind) {
+    const size_t node = (kind == kNodeBvh4 || kind == kNodeBvh8Q) ? 128 : 64;
+    return kTopBytes + kTopBytes / node * sizeof(uint32_t);
+}
+
+hipError_t launch_build_top(hipStream_t s, int kind, const void* nodes, void* table) {
+    switch (kind) {
+    case kNodeBvh4:
+        hipLaunchKernelGGL((k_build_top<Bvh4Node, 4>), dim3(1), dim3(64), 0, s, static_cast<const Bvh4Node*>(nodes),
+                           static_cast<Bvh4Node*>(table));
+        break;
+    case kNodeBvh2:
+        hipLaunchKernelGGL((k_build_top<Bvh2Node, 2>), dim3(1), dim3(64), 0, s, static_cast<const Bvh2Node*>(nodes),
+                           static_cast<Bvh2Node*>(table));
+        break;
+    case kNodeBvh4Q:
+        hipLaunchKernelGGL((k_build_top<Bvh4QNode, 4>), dim3(1), dim3(64), 0, s, static_cast<const Bvh4QNode*>(nodes),
+                           static_cast<Bvh4QNode*>(table));
+        break;
+    case kNodeBvh8Q:
+        hipLaunchKernelGGL((k_build_top<Bvh8QNode, 8>), dim3(1), dim3(64), 0, s, static_cast<const Bvh8QNode*>(nodes),
+                           static_cast<Bvh8QNode*>(table));
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_write_words(hipStream_t s, void* dst, const void* src, size_t bytes) {

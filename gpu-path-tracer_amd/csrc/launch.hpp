@@ -13,6 +13,13 @@ namespace pt {
 constexpr int kQueueRegen = 4;
 constexpr int kQueueSlots = 5;
 
+// Node kinds of the traversal kernels: fp32 BVH4 (128 B), fp32 BVH2 (64 B),
+// quantized BVH4 (64 B) and quantized BVH8 (128 B).
+constexpr int kNodeBvh2 = 2;
+constexpr int kNodeBvh4 = 4;
+constexpr int kNodeBvh4Q = 5;
+constexpr int kNodeBvh8Q = 8;
+
 struct SceneDev {
     const void* nodes;              // Bvh4Node[] or Bvh2Node[] by node_width
     const void* qnodes;             // Bvh4QNode[]: the same 4-wide tree quantized, or null
@@ -26,8 +33,15 @@ struct SceneDev {
     uint32_t tri_count;
     uint32_t node_width;            // 4 or 2
     uint32_t node_quantized;        // traversals that use qnodes: bit 0 closest hit, bit 1 any hit
-    uint32_t pad[2];
+    const void* top[3];             // top-of-tree tables of nodes, qnodes and qnodes8 (null: none)
 };
+
+// Writes the top-of-tree table of `nodes` (kind: kNodeBvh4, kNodeBvh2,
+// kNodeBvh4Q or kNodeBvh8Q) into `table` (kTopBytes of nodes, then their
+// global indices): one thread, a breadth-first walk from the root.
+hipError_t launch_build_top(hipStream_t s, int kind, const void* nodes, void* table);
+// Bytes of a top-of-tree table for node kind `kind`.
+size_t top_table_bytes(int kind);
 
 // Device state of the guide network and of its training records.
 struct GuideDev {

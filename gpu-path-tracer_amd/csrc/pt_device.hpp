@@ -224,6 +224,17 @@ PT_HD uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
 constexpr uint32_t kMaxLeafSize = 16;
 constexpr uint32_t kMaxLeafFirst = (1u << 27) - 1u;
 
+// Top-of-tree table: the first kTopBytes / sizeof(node) nodes of a tree in
+// breadth-first order, copied with every internal child reference that lands
+// in the table rewritten to kTopBit | its table index, followed by the global
+// index of each table node (uint32 each).  Traversal blocks keep the table in
+// LDS; an internal reference with kTopBit set reads its node there.  Internal
+// node indices stay below kTopBit.
+constexpr uint32_t kTopBit = 0x40000000u;
+constexpr uint32_t kTopBytes = 8192;
+constexpr uint32_t kMaxNodeIndex = kTopBit - 1u;
+PT_HD bool ref_is_top(uint32_t r) { return (r & (kLeafBit | kTopBit)) == kTopBit; }
+
 // Intersection record, 48 B, in BVH order (TriangleIntersection, Triangle.hpp:8-17):
 //   v0.xyz | original prim id,  e1.xyz | queue id,  e2.xyz | material id
 struct alignas(16) TriRecord {

@@ -163,6 +163,9 @@ SceneDev ResidentScene::dev() const {
     s.tri_count = host.tri_count;
     s.node_width = host.bvh.width;
     s.node_quantized = quantized;
+    for (int k = 0; k < 3; k++) s.top[k] = top[k].bytes() ? top[k].as<void>() : nullptr;
+    if (!quantized) s.top[1] = nullptr;
+    if (!depth8) s.top[2] = nullptr;
     return s;
 }
 
@@ -403,6 +406,16 @@ pt_status Renderer::upload_scene(uint32_t id, const pt_scene_desc& desc) {
         PT_TRY_HIP(hipStreamSynchronize(stream_), "8-wide quantized BVH nodes");
         rs->depth8 = h.bvh.depth8;
     }
+    // top-of-tree tables, which the traversal blocks keep in LDS
+    const uint64_t node_count = gpu_bvh ? h.bvh.device_nodes : h.bvh.nodes.size();
+    if (node_count > kMaxNodeIndex) return fail(PT_ERROR_OUT_OF_RANGE, "the BVH has more than 2^30 nodes");
+    auto build_top = [&](DeviceBuffer& table, int kind, const DeviceBuffer& nodes) -> hipError_t {
+        hipError_t e = table.allocate(top_table_bytes(kind));
+        return e != hipSuccess ? e : launch_build_top(stream_, kind, nodes.as<void>(), table.as<void>());
+    };
+    PT_TRY_HIP(build_top(rs->top[0], h.bvh.width == 2 ? kNodeBvh2 : kNodeBvh4, rs->nodes), "top-of-tree table");
+    if (rs->quantized) PT_TRY_HIP(build_top(rs->top[1], kNodeBvh4Q, rs->qnodes), "top-of-tree table");
+    if (rs->depth8) PT_TRY_HIP(build_top(rs->top[2], kNodeBvh8Q, rs->qnodes8), "top-of-tree table");
     if (!gpu_bvh) {
         PT_TRY(upload(rs->tris, h.tris.data(), h.tris.size() * sizeof(TriRecord), "triangle records"));
         PT_TRY(upload(rs->materials, h.materials.data(), h.materials.size() * sizeof(MaterialRecord), "materials"));

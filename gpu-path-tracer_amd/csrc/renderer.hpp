@@ -56,6 +56,7 @@ struct ResidentScene {
     DeviceBuffer order;        // GPU-built BVH: original triangle of each BVH slot (the host copy is empty)
     DeviceBuffer qnodes;       // quantized copy of the 4-wide nodes
     DeviceBuffer qnodes8;      // the same binary tree 8-wide and quantized (any hit), or empty
+    DeviceBuffer top[3];       // top-of-tree tables of nodes, qnodes and qnodes8
     uint32_t quantized = 0;    // traversals that use qnodes: bit 0 closest hit, bit 1 any hit
     uint32_t depth8 = 0;       // levels of the 8-wide tree (0: none)
     SceneDev dev() const;
