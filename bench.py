@@ -307,12 +307,14 @@ def sequential_then_batched(r, items, steps, log_prefix):
     difference of a few views batched vs alone)."""
     import numpy as np
     alone, seq_ms, seq_rays = [], 0.0, 0
+    per_view = []
     for it in items:                              # warm-up: allocation, graphs
         r.RenderBatch([it])
     for k, it in enumerate(items):
         (hdr,), st = r.RenderBatch([it])
         seq_ms += st["render_ms"]
         seq_rays += _rays(st)
+        per_view.append((st["render_ms"], _rays(st)))
         if k < 6:
             alone.append(hdr)
     r.RenderBatch(items)
@@ -326,7 +328,50 @@ def sequential_then_batched(r, items, steps, log_prefix):
     diff = max(float(np.max(np.abs(h - a) / (np.abs(a) + 1e-3))) for h, a in zip(hdrs, alone))
     log(f"{log_prefix}: sequential {seq_ms:.1f} ms ({seq_rays} rays), batched {b_ms:.1f} ms ({brays} rays), "
         f"max rel diff batched vs alone {diff:.2e}")
-    return seq_rays / seq_ms / 1e3, brays / b_ms / 1e3, b_ms, brays, diff
+    return seq_rays / seq_ms / 1e3, brays / b_ms / 1e3, b_ms, brays, diff, per_view
+
+
+# Size classes of a view by its sample count (pixels x spp after rounding to a
+# square): below 2^21 a view cannot fill the 2^25-slot path pool even for one
+# bounce; above 2^25 it fills the pool for several bounces by itself.
+BATCH_CLASSES = (("small", 0, 1 << 21), ("medium", 1 << 21, 1 << 25), ("large", 1 << 25, 1 << 62))
+
+
+def view_samples(st):
+    import math
+    return st["width"] * st["height"] * int(math.isqrt(st["samples_per_pixel"])) ** 2
+
+
+def batch_classes(r, items, per_view, steps):
+    """Per size class: the views rendered one at a time (device time, from the
+    sequential pass) against the class's views batched alone, with the fixed
+    cost of an alone render (the time of its smallest views)."""
+    out = []
+    for name, lo, hi in BATCH_CLASSES:
+        ks = [k for k, (_, st, _) in enumerate(items) if lo <= view_samples(st) < hi]
+        if not ks:
+            continue
+        cls = [items[k] for k in ks]
+        seq_ms = sum(per_view[k][0] for k in ks)
+        seq_rays = sum(per_view[k][1] for k in ks)
+        r.RenderBatch(cls)
+        ms = []
+        for _ in range(max(1, steps)):
+            _, st = r.RenderBatch(cls)
+            ms.append(st["render_ms"])
+            brays = _rays(st)
+        ms.sort()
+        b_ms = ms[len(ms) // 2]
+        row = {"class": name, "views": len(ks), "samples": int(sum(view_samples(items[k][1]) for k in ks)),
+               "alone_ms": round(seq_ms, 2), "batched_ms": round(b_ms, 2),
+               "alone_mrays_s": round(seq_rays / seq_ms / 1e3, 1), "batched_mrays_s": round(brays / b_ms / 1e3, 1),
+               "speedup": round(seq_ms / b_ms * brays / seq_rays, 3),
+               "alone_ms_per_view_min": round(min(per_view[k][0] for k in ks), 3)}
+        log(f"batch class {name}: {row['views']} views, {row['samples']} samples: alone {seq_ms:.1f} ms "
+            f"({row['alone_mrays_s']} Mrays/s), batched {b_ms:.1f} ms ({row['batched_mrays_s']} Mrays/s), "
+            f"{row['speedup']}x")
+        out.append(row)
+    return out
 
 
 def serve_run(r, requests, rate_hz, seed):
@@ -438,7 +483,8 @@ def run_batch(a):
     ids = upload_all(r, sc)
     log(f"scenes uploaded in {time.time() - t0:.1f}s")
     items = [(ids[i], st, v) for i, st, v in views]
-    seq, bat, b_ms, brays, diff = sequential_then_batched(r, items, a.steps, "batch")
+    seq, bat, b_ms, brays, diff, per_view = sequential_then_batched(r, items, a.steps, "batch")
+    classes = batch_classes(r, items, per_view, a.steps)
     out = {"metric": "Mrays/s (extension+shadow), 84 views of 32 scenes in one batched wavefront",
            "value": round(bat, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": a.steps, "warmup": 1,
            "ms_per_step": round(b_ms, 3), "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
@@ -448,7 +494,7 @@ def run_batch(a):
                       "triangles": sum(s.triangle_count for s in sc), "views": len(views), "rays": brays,
                       "tuning": r.get_tuning()},
            "sequential_mrays_s": round(seq, 2), "speedup_vs_sequential": round(bat / seq, 3),
-           "max_rel_diff_batched_vs_alone": diff}
+           "max_rel_diff_batched_vs_alone": diff, "classes": classes}
     print(json.dumps(out), flush=True)
     r.close()
 

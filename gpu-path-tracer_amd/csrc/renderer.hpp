@@ -283,6 +283,7 @@ private:
     std::unique_ptr<ServerState> server_;
     LoopMode server_mode_;
     hipGraphExec_t server_graph_ = nullptr;
+    std::vector<hipGraphExec_t> server_scene_graphs_;   // by scene id: the one-scene traversal loops
 };
 
 #define PT_TRY_HIP(expr, what)                              \

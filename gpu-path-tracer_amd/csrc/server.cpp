@@ -95,8 +95,19 @@ pt_status Renderer::ServerStart(const pt_server_config& cfg_in) {
     server_mode_.shade_multi = true;
     server_mode_.scene = first_scene;
     server_graph_ = nullptr;
+    server_scene_graphs_.assign(kMaxScenes, nullptr);
     if (tuning_.use_graph) {
         pt_status s = ensure_graph(server_mode_, server_graph_);
+        // the one-scene loops of every resident scene, captured now: a capture
+        // while the server threads run can be invalidated by the read-back
+        // thread's copies
+        for (uint32_t id = 0; id < kMaxScenes && s == PT_OK; id++) {
+            if (!scenes_[id]) continue;
+            LoopMode m = server_mode_;
+            m.trace_multi = false;
+            m.scene = id;
+            s = ensure_graph(m, server_scene_graphs_[id]);
+        }
         if (s != PT_OK) {
             (void)hipStreamDestroy(S->rb_stream);
             return s;
@@ -343,11 +354,33 @@ pt_status Renderer::server_step() {
         }
         if (S.header_dirty) PT_TRY(write_header());
 
-        // ---- kGraphIterations bounce iterations
-        if (server_graph_) {
-            PT_TRY_HIP(hipGraphLaunch(server_graph_, stream_), "graph launch");
+        // ---- kGraphIterations bounce iterations.  Every path in flight and
+        // every queued sample belongs to an active request (a request leaves
+        // the active set only once its last path has ended), so while all
+        // active requests view one scene the launch may use the one-scene
+        // traversal kernels, which read the top of that scene's tree from LDS.
+        LoopMode mode = server_mode_;
+        {
+            uint32_t scene = kInvalid;
+            bool one = true;
+            for (auto& r : S.active) {
+                if (scene == kInvalid) scene = r->rec.scene;
+                one = one && r->rec.scene == scene;
+            }
+            if (one && scene != kInvalid) {
+                mode.trace_multi = false;
+                mode.scene = scene;
+            }
+        }
+        hipGraphExec_t graph = server_graph_;
+        if (graph && !mode.trace_multi) {
+            if (server_scene_graphs_[mode.scene]) graph = server_scene_graphs_[mode.scene];
+            else mode = server_mode_;
+        }
+        if (graph) {
+            PT_TRY_HIP(hipGraphLaunch(graph, stream_), "graph launch");
         } else {
-            for (int i = 0; i < kGraphIterations; i++) PT_TRY(launch_iteration(i & 1, false, server_mode_));
+            for (int i = 0; i < kGraphIterations; i++) PT_TRY(launch_iteration(i & 1, false, mode));
         }
         const int slot = (int)(k % kRing);
         PT_TRY_HIP(hipMemcpyAsync(h_ctr_ring_ + slot, p.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream_), "counters");
