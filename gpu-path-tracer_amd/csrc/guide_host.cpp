@@ -333,7 +333,7 @@ pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& c
     const uint32_t spp_total = views[0].spp;
     if (spp_total < gs.passes) return fail(PT_ERROR_INVALID_ARGUMENT, "fewer samples per pixel than passes");
     const std::vector<std::vector<uint32_t>> chunks = pass_chunks(spp_total, gs.passes);
-    set_frame_size(views[0].width, views[0].height);
+    PT_TRY(set_frame_size(views[0].width, views[0].height));
     last_spp_ = spp_total;
     hdr_valid_ = false;
     const uint64_t pixels = (uint64_t)views[0].width * views[0].height;
@@ -446,13 +446,7 @@ pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& c
     gst.loss_first = first_loss;
     gst.loss_last = last_loss;
 
-    const size_t npix = (size_t)width_ * height_;
-    PT_TRY_HIP(hipMemcpyAsync(h_frame_staging_, d_frame_.as<void>(), npix * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "framebuffer");
-    PT_TRY_HIP(hipStreamSynchronize(stream_), "frame sync");
-    {
-        std::scoped_lock lock(frame_mutex_);
-        frame_.assign(h_frame_staging_, h_frame_staging_ + npix);
-    }
+    PT_TRY(publish_frame());
     hdr_valid_ = true;
     if (stats) *stats = total;
     if (gstats) *gstats = gst;

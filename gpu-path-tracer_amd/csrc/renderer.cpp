@@ -68,7 +68,8 @@ Renderer::~Renderer() {
     if (h_fp_) (void)hipHostFree(h_fp_);
     if (h_ctr_ring_) (void)hipHostFree(h_ctr_ring_);
     if (h_req_invalid_) (void)hipHostFree(h_req_invalid_);
-    if (h_frame_staging_) (void)hipHostFree(h_frame_staging_);
+    for (uint32_t* b : h_frame_)
+        if (b) (void)hipHostFree(b);
     if (side_stream_) (void)hipStreamDestroy(side_stream_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -202,13 +203,6 @@ pt_status Renderer::ensure_accum(uint32_t pixels) {
     accum_capacity_ = 0;
     PT_TRY_HIP(d_accum_.allocate(n * sizeof(float4)), "accumulation buffer");
     PT_TRY_HIP(d_frame_.allocate(n * sizeof(uint32_t)), "framebuffer");
-    if (h_frame_staging_pixels_ < n) {
-        if (h_frame_staging_) (void)hipHostFree(h_frame_staging_);
-        h_frame_staging_ = nullptr;
-        h_frame_staging_pixels_ = 0;
-        PT_TRY_HIP(hipHostMalloc((void**)&h_frame_staging_, n * sizeof(uint32_t), hipHostMallocDefault), "pinned framebuffer");
-        h_frame_staging_pixels_ = n;
-    }
     PT_TRY_HIP(hipMemsetAsync(d_accum_.as<void>(), 0, n * sizeof(float4), stream_), "memset");
     PT_TRY_HIP(hipMemsetAsync(d_frame_.as<void>(), 0, n * sizeof(uint32_t), stream_), "memset");
     PT_TRY_HIP(hipStreamSynchronize(stream_), "sync");
@@ -216,13 +210,39 @@ pt_status Renderer::ensure_accum(uint32_t pixels) {
     return PT_OK;
 }
 
-void Renderer::set_frame_size(uint32_t w, uint32_t h) {
-    if (w == width_ && h == height_) return;
+pt_status Renderer::set_frame_size(uint32_t w, uint32_t h) {
+    if (w == width_ && h == height_ && h_frame_[0]) return PT_OK;
+    const size_t n = std::max<size_t>((size_t)w * h, 1);
     std::scoped_lock lock(frame_mutex_);
-    frame_.assign((size_t)w * h, 0u);
+    if (h_frame_capacity_ < n) {
+        for (uint32_t*& b : h_frame_) {
+            if (b) (void)hipHostFree(b);
+            b = nullptr;
+        }
+        h_frame_capacity_ = 0;
+        width_ = height_ = 0;
+        for (uint32_t*& b : h_frame_)
+            PT_TRY_HIP(hipHostMalloc((void**)&b, n * sizeof(uint32_t), hipHostMallocDefault), "pinned framebuffer");
+        h_frame_capacity_ = n;
+    }
+    std::memset(h_frame_[h_front_], 0, (size_t)w * h * sizeof(uint32_t));
     width_ = w;
     height_ = h;
     hdr_valid_ = false;
+    return PT_OK;
+}
+
+// Copies the tone-mapped frame of the last run into the back buffer and makes
+// it the front one.  Only the rendering thread writes the back buffer; readers
+// hold frame_mutex_ while they copy the front one.
+pt_status Renderer::publish_frame() {
+    const size_t npix = (size_t)width_ * height_;
+    uint32_t* back = h_frame_[h_front_ ^ 1];
+    PT_TRY_HIP(hipMemcpyAsync(back, d_frame_.as<void>(), npix * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "framebuffer");
+    PT_TRY_HIP(hipStreamSynchronize(stream_), "frame sync");
+    std::scoped_lock lock(frame_mutex_);
+    h_front_ ^= 1;
+    return PT_OK;
 }
 
 namespace {
@@ -323,7 +343,7 @@ pt_status Renderer::InitializeRenderingBuffers(uint32_t width, uint32_t height) 
     height = std::min(std::max(height, kMinHeight), kMaxHeight);
     PT_TRY(ensure_pool());
     PT_TRY(ensure_accum(width * height));
-    set_frame_size(width, height);
+    PT_TRY(set_frame_size(width, height));
     return PT_OK;
 }
 
@@ -462,7 +482,7 @@ pt_status Renderer::Clear(uint8_t r, uint8_t g, uint8_t b, uint8_t a) {
     if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Cannot clear renderer while rendering");
     const uint32_t c = (uint32_t)r | ((uint32_t)g << 8) | ((uint32_t)b << 16) | ((uint32_t)a << 24);
     std::scoped_lock lock(frame_mutex_);
-    std::fill(frame_.begin(), frame_.end(), c);
+    if (h_frame_[h_front_]) std::fill(h_frame_[h_front_], h_frame_[h_front_] + (size_t)width_ * height_, c);
     return PT_OK;
 }
 
@@ -781,18 +801,12 @@ pt_status Renderer::render_frame(const pt_render_settings& s, const pt_camera& c
     std::vector<ViewRecord> views(1);
     uint32_t depth = 0;
     PT_TRY(make_view(v, views[0], depth));
-    set_frame_size(views[0].width, views[0].height);
+    PT_TRY(set_frame_size(views[0].width, views[0].height));
     last_spp_ = views[0].spp;
     hdr_valid_ = false;
     PT_TRY(run_views(views, depth, flags, stats, stop));
     if (stop && stop->load()) return PT_OK;
-    const size_t npix = (size_t)width_ * height_;
-    PT_TRY_HIP(hipMemcpyAsync(h_frame_staging_, d_frame_.as<void>(), npix * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "framebuffer");
-    PT_TRY_HIP(hipStreamSynchronize(stream_), "frame sync");
-    {
-        std::scoped_lock lock(frame_mutex_);
-        frame_.assign(h_frame_staging_, h_frame_staging_ + npix);
-    }
+    PT_TRY(publish_frame());
     hdr_valid_ = true;
     return PT_OK;
 }
@@ -853,7 +867,7 @@ pt_status Renderer::StartSimulation(const pt_render_settings& s, const pt_camera
     const ClampedSettings c = clamp_settings(s);
     PT_TRY(ensure_pool());
     PT_TRY(ensure_accum(c.width * c.height));
-    set_frame_size(c.width, c.height);
+    PT_TRY(set_frame_size(c.width, c.height));
     settings_ = s;
     camera_ = camera;
     motion_.assign(states, states + n);
@@ -946,8 +960,9 @@ void Renderer::GetProgress(pt_progress& p) const {
 
 pt_status Renderer::CopyFrame(uint32_t* rgba, size_t n) {
     std::scoped_lock lock(frame_mutex_);
-    if (!rgba || n < frame_.size()) return fail(PT_ERROR_INVALID_ARGUMENT, "frame buffer too small");
-    std::memcpy(rgba, frame_.data(), frame_.size() * sizeof(uint32_t));
+    const size_t npix = (size_t)width_ * height_;
+    if (!rgba || n < npix) return fail(PT_ERROR_INVALID_ARGUMENT, "frame buffer too small");
+    if (npix) std::memcpy(rgba, h_frame_[h_front_], npix * sizeof(uint32_t));
     return PT_OK;
 }
 
@@ -1003,7 +1018,7 @@ pt_status Renderer::save_frame_png(const std::string& path) {
     std::vector<uint32_t> px;
     {
         std::scoped_lock lock(frame_mutex_);
-        px = frame_;
+        if (h_frame_[h_front_]) px.assign(h_frame_[h_front_], h_frame_[h_front_] + (size_t)width_ * height_);
     }
     std::string err;
     const ImageStatus s = write_png_rgba8(path, width_, height_, px.data(), true, err);

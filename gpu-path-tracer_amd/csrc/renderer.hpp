@@ -166,7 +166,8 @@ private:
     pt_status check_hip(hipError_t e, const char* what);
     pt_status ensure_pool();
     pt_status ensure_accum(uint32_t pixels);
-    void set_frame_size(uint32_t width, uint32_t height);
+    pt_status set_frame_size(uint32_t width, uint32_t height);
+    pt_status publish_frame();
     pt_status configure_trace();
     pt_status upload_scene(uint32_t id, const pt_scene_desc& desc);
     pt_status make_view(const pt_view& v, ViewRecord& rec, uint32_t& depth);
@@ -229,9 +230,12 @@ private:
     uint32_t accum_capacity_ = 0;
     bool hdr_valid_ = false;                // accum[0, w*h) holds the last single frame
     DeviceBuffer d_accum_, d_frame_;
-    uint32_t* h_frame_staging_ = nullptr;   // pinned
-    size_t h_frame_staging_pixels_ = 0;
-    std::vector<uint32_t> frame_;           // last finished frame, guarded by frame_mutex_
+    // The last finished frame (RGBA8, width_ x height_): two pinned buffers.
+    // Readers copy the front one under frame_mutex_; a finished frame is copied
+    // from the device into the back one and made the front by a swap.
+    uint32_t* h_frame_[2] = {nullptr, nullptr};
+    size_t h_frame_capacity_ = 0;           // pixels per buffer
+    int h_front_ = 0;
     mutable std::mutex frame_mutex_;
     uint32_t last_spp_ = 1;
 
