@@ -355,15 +355,17 @@ pt_status Renderer::server_step() {
         if (S.header_dirty) PT_TRY(write_header());
 
         // ---- kGraphIterations bounce iterations.  Every path in flight and
-        // every queued sample belongs to an active request (a request leaves
-        // the active set only once its last path has ended), so while all
-        // active requests view one scene the launch may use the one-scene
+        // every queued sample belongs to an active request with scheduled
+        // samples (a request leaves the active set only once its last path has
+        // ended; one with nothing scheduled has nothing on the device), so
+        // while those requests view one scene the launch may use the one-scene
         // traversal kernels, which read the top of that scene's tree from LDS.
         LoopMode mode = server_mode_;
         {
             uint32_t scene = kInvalid;
             bool one = true;
             for (auto& r : S.active) {
+                if (r->scheduled == 0) continue;
                 if (scene == kInvalid) scene = r->rec.scene;
                 one = one && r->rec.scene == scene;
             }
