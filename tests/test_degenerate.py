@@ -113,3 +113,59 @@ def test_degenerate_render_matches_oracle(torture):
     ref, ext, sh = o.render(st, s.view)
     assert abs(stats["extension_rays"] - ext) <= 0.01 * ext and abs(stats["shadow_rays"] - sh) <= 0.01 * sh
     assert float(np.mean((hdr[:, :3] - ref[:, :3]) ** 2 / (ref[:, :3] ** 2 + 1e-2))) < 1e-3
+
+
+def _dark_room():
+    """The box room with its ceiling light made a white wall: no emissive
+    triangle, so no light table."""
+    s = scenes.cornell_box()
+    mats = s.materials.copy()
+    for k in range(len(mats)):
+        if mats[k]["model"] == scenes.MODEL_EMISSIVE:
+            mats[k] = scenes.material(scenes.MODEL_DIFFUSE, color=(0.73, 0.73, 0.73))
+    s.materials = mats
+    return s
+
+
+@pytest.mark.parametrize("nee", [1, 0])
+def test_scene_without_emitters_renders_black(oracle_lib, nee):
+    """No emissive triangle: next-event estimation has nothing to sample and is
+    skipped (the reference would sample an empty alias table), every pixel is
+    zero, the ray counts equal the oracle's, and the alias hook refuses."""
+    s = _dark_room()
+    r = ptcore.Renderer()
+    try:
+        r.InitializeRenderingBuffers(32, 32)
+        r.InitializeSceneBuffers(s)
+        assert r.bvh_info()["light_count"] == 0
+        st = dict(width=32, height=32, samples_per_pixel=16, path_depth=8, use_nee=nee, rr_start_depth=3, seed=4)
+        stats = r.RenderFrame(st, s.view)
+        hdr = r.GetFrameHdr(32, 32)
+        assert np.all(hdr[..., :3] == 0.0)
+        assert (r.GetFrame(32, 32) == 0xFF000000).all()
+        _, ext, sh = oracle_lib.OracleScene(s).render(st, s.view)
+        assert stats["extension_rays"] == ext and stats["shadow_rays"] == sh == 0
+        with pytest.raises(ptcore.PtError) as e:
+            r.alias_sample(np.array([0.5], np.float32))
+        assert e.value.status == 2   # PT_ERROR_INVALID_STATE
+    finally:
+        r.close()
+
+
+def test_view_that_misses_everything(oracle_lib):
+    """A camera looking away from all geometry: one extension ray per sample,
+    no shadow ray, a black frame."""
+    s = scenes.cornell_box()
+    away = scenes.View(s.view.fov_y, (0.0, 1.0, 50.0), scenes.look_at_quaternion((0.0, 1.0, 50.0), (0.0, 1.0, 100.0)))
+    r = ptcore.Renderer()
+    try:
+        r.InitializeRenderingBuffers(24, 16)
+        r.InitializeSceneBuffers(s)
+        st = dict(width=24, height=16, samples_per_pixel=4, path_depth=8, use_nee=1, rr_start_depth=3, seed=9)
+        stats = r.RenderFrame(st, away)
+        assert stats["extension_rays"] == 24 * 16 * 4 and stats["shadow_rays"] == 0
+        assert np.all(r.GetFrameHdr(24, 16)[..., :3] == 0.0)
+        _, ext, sh = oracle_lib.OracleScene(s).render(st, away)
+        assert (ext, sh) == (24 * 16 * 4, 0)
+    finally:
+        r.close()
