@@ -28,7 +28,7 @@ import ptcore  # noqa: E402
 import scenes  # noqa: E402
 
 BLOCKS = (64, 128, 256, 512)
-POOLS = (1 << 22, 1 << 23, 1 << 24, 1 << 25)
+POOLS = (1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27)
 WIDTHS = (4, 2)
 TUNED_KEYS = ("trace_block", "pool_size", "bvh_width")
 

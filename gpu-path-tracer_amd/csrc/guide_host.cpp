@@ -338,7 +338,12 @@ pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& c
     hdr_valid_ = false;
     const uint64_t pixels = (uint64_t)views[0].width * views[0].height;
 
-    PT_TRY(ensure_pool());
+    // the pool for the largest pass chunk, before the guide's per-slot arrays
+    // are sized to it (the passes' own runs then find it large enough)
+    uint32_t max_chunk = 0;
+    for (const auto& pass : chunks)
+        for (uint32_t c : pass) max_chunk = std::max(max_chunk, c);
+    PT_TRY(ensure_pool((uint64_t)max_chunk * pixels));
     uint32_t rec_capacity = 0;
     if (learn) {
         guide_box_from_scene();

@@ -164,7 +164,7 @@ private:
         hipGraphExec_t exec;
     };
     pt_status check_hip(hipError_t e, const char* what);
-    pt_status ensure_pool();
+    pt_status ensure_pool(uint64_t samples = 0, bool exact = false);
     pt_status ensure_accum(uint32_t pixels);
     pt_status set_frame_size(uint32_t width, uint32_t height);
     pt_status publish_frame();

@@ -35,6 +35,7 @@ constexpr uint32_t kMaxRequests = 2048;
 // its readback (the server thread waits for one if all are held)
 constexpr uint32_t kInvalidSlots = 4096;
 constexpr uint64_t kPixelBytes = sizeof(float4) + sizeof(uint32_t);
+constexpr uint64_t kServerAutoPool = 1ull << 25;   // path slots of a server started with an automatic pool
 }  // namespace
 
 pt_status Renderer::ServerStart(const pt_server_config& cfg_in) {
@@ -56,7 +57,11 @@ pt_status Renderer::ServerStart(const pt_server_config& cfg_in) {
     if (cfg.accum_budget_bytes == 0) cfg.accum_budget_bytes = kDefaultBudget;
     const uint64_t budget_pixels = std::min<uint64_t>(cfg.accum_budget_bytes / kPixelBytes, 0x7FFFFFFFull);
     if (budget_pixels < 1) return fail(PT_ERROR_INVALID_ARGUMENT, "accum_budget_bytes holds no pixel");
-    PT_TRY(ensure_pool());
+    // requests are not known yet: an automatic pool is set to 2^25 slots with a
+    // window of four pools (eight rooms of 1-20M triangles: 2^27 slots and
+    // their four-pool window gave 0.96x of one request at a time against
+    // 0.97x, and p50 474 / 972 ms against 360 / 414 ms at 50 / 80 % load)
+    PT_TRY(ensure_pool(kServerAutoPool, true));
     // the window covers what the wavefront regenerates while the host reads its
     // counters a graph launch late (about a pool per three bounces): with two
     // pools the path pool ran part-empty (eight rooms of 1-20M triangles: 0.97x

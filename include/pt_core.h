@@ -221,7 +221,8 @@ typedef struct pt_tuning {
     uint32_t trace_blocks_per_cu;/* persistent grid of the closest-hit traversal (and of the any-hit one
                                     when shadow_blocks_per_cu is 0), blocks per CU: 0 = as many as fit */
     uint32_t shade_block;        /* 256 */
-    uint32_t pool_size;          /* path slots (queue capacity) */
+    uint32_t pool_size;          /* path slots (queue capacity): 1024..2^27, or 0 = automatic (default): the next
+                                    power of two of the largest run so far, 2^22..2^27 slots (128 B each + queues) */
     uint32_t use_graph;          /* replay the bounce loop as a hipGraph */
     uint32_t bvh_max_leaf;       /* 1..16 triangles; applied at the next scene upload */
     float bvh_traversal_cost;    /* SAH cost of a node visit relative to a triangle test; next upload */
