@@ -90,6 +90,14 @@ pt_status pt_frame_copy(pt_renderer* r, uint32_t* rgba, size_t pixel_count) {
     return guarded(r, [&](pt::Renderer& x) { return x.CopyFrame(rgba, pixel_count); });
 }
 
+pt_status pt_frame_lock(pt_renderer* r, const uint32_t** rgba, size_t* pixel_count) {
+    return guarded(r, [&](pt::Renderer& x) { return x.LockFrame(rgba, pixel_count); });
+}
+
+pt_status pt_frame_unlock(pt_renderer* r) {
+    return guarded(r, [&](pt::Renderer& x) { return x.UnlockFrame(); });
+}
+
 pt_status pt_frame_copy_hdr(pt_renderer* r, float* rgba, size_t pixel_count) {
     return guarded(r, [&](pt::Renderer& x) { return x.CopyFrameHdr(rgba, pixel_count); });
 }

@@ -227,6 +227,7 @@ pt_status Renderer::ensure_accum(uint32_t pixels) {
 }
 
 pt_status Renderer::set_frame_size(uint32_t w, uint32_t h) {
+    if (frame_locked_here()) return fail(PT_ERROR_INVALID_STATE, "this thread holds the frame lock (pt_frame_lock)");
     if (w == width_ && h == height_ && h_frame_[0]) return PT_OK;
     const size_t n = std::max<size_t>((size_t)w * h, 1);
     std::scoped_lock lock(frame_mutex_);
@@ -252,6 +253,7 @@ pt_status Renderer::set_frame_size(uint32_t w, uint32_t h) {
 // it the front one.  Only the rendering thread writes the back buffer; readers
 // hold frame_mutex_ while they copy the front one.
 pt_status Renderer::publish_frame() {
+    if (frame_locked_here()) return fail(PT_ERROR_INVALID_STATE, "this thread holds the frame lock (pt_frame_lock)");
     const size_t npix = (size_t)width_ * height_;
     uint32_t* back = h_frame_[h_front_ ^ 1];
     PT_TRY_HIP(hipMemcpyAsync(back, d_frame_.as<void>(), npix * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "framebuffer");
@@ -496,6 +498,7 @@ pt_status Renderer::RemoveScene(uint32_t id) {
 
 pt_status Renderer::Clear(uint8_t r, uint8_t g, uint8_t b, uint8_t a) {
     if (IsRendering()) return fail(PT_ERROR_INVALID_STATE, "Cannot clear renderer while rendering");
+    if (frame_locked_here()) return fail(PT_ERROR_INVALID_STATE, "this thread holds the frame lock (pt_frame_lock)");
     const uint32_t c = (uint32_t)r | ((uint32_t)g << 8) | ((uint32_t)b << 16) | ((uint32_t)a << 24);
     std::scoped_lock lock(frame_mutex_);
     if (h_frame_[h_front_]) std::fill(h_frame_[h_front_], h_frame_[h_front_] + (size_t)width_ * height_, c);
@@ -975,10 +978,30 @@ void Renderer::GetProgress(pt_progress& p) const {
 }
 
 pt_status Renderer::CopyFrame(uint32_t* rgba, size_t n) {
+    if (frame_locked_here()) return fail(PT_ERROR_INVALID_STATE, "this thread holds the frame lock (pt_frame_lock)");
     std::scoped_lock lock(frame_mutex_);
     const size_t npix = (size_t)width_ * height_;
     if (!rgba || n < npix) return fail(PT_ERROR_INVALID_ARGUMENT, "frame buffer too small");
     if (npix) std::memcpy(rgba, h_frame_[h_front_], npix * sizeof(uint32_t));
+    return PT_OK;
+}
+
+// FrameView::Lock() (Renderer.hpp:45-104) without a copy: the front buffer
+// under the frame lock until UnlockFrame on the same thread.
+pt_status Renderer::LockFrame(const uint32_t** rgba, size_t* n) {
+    if (!rgba || !n) return fail(PT_ERROR_INVALID_ARGUMENT, "NULL argument");
+    if (frame_locked_here()) return fail(PT_ERROR_INVALID_STATE, "this thread already holds the frame lock");
+    frame_mutex_.lock();
+    frame_lock_owner_.store(std::this_thread::get_id());
+    *rgba = h_frame_[h_front_];
+    *n = h_frame_[h_front_] ? (size_t)width_ * height_ : 0;
+    return PT_OK;
+}
+
+pt_status Renderer::UnlockFrame() {
+    if (!frame_locked_here()) return fail(PT_ERROR_INVALID_STATE, "this thread does not hold the frame lock");
+    frame_lock_owner_.store(std::thread::id());
+    frame_mutex_.unlock();
     return PT_OK;
 }
 
@@ -1031,6 +1054,7 @@ pt_status image_status(ImageStatus s) {
 }  // namespace
 
 pt_status Renderer::save_frame_png(const std::string& path) {
+    if (frame_locked_here()) return fail(PT_ERROR_INVALID_STATE, "this thread holds the frame lock (pt_frame_lock)");
     std::vector<uint32_t> px;
     {
         std::scoped_lock lock(frame_mutex_);

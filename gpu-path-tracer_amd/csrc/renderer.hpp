@@ -80,6 +80,8 @@ public:
     bool IsRendering() const { return is_rendering_.load(); }
     void GetProgress(pt_progress& p) const;
     pt_status CopyFrame(uint32_t* rgba, size_t n);
+    pt_status LockFrame(const uint32_t** rgba, size_t* n);
+    pt_status UnlockFrame();
     // the frame as ".png" (RGBA8) or ".exr" (float radiance of the last single frame)
     pt_status SaveFrame(const char* path);
     pt_status CopyFrameHdr(float* rgba, size_t n);
@@ -237,6 +239,9 @@ private:
     size_t h_frame_capacity_ = 0;           // pixels per buffer
     int h_front_ = 0;
     mutable std::mutex frame_mutex_;
+    // the thread holding frame_mutex_ through LockFrame (default id: none)
+    std::atomic<std::thread::id> frame_lock_owner_{};
+    bool frame_locked_here() const { return frame_lock_owner_.load() == std::this_thread::get_id(); }
     uint32_t last_spp_ = 1;
 
     // traversal

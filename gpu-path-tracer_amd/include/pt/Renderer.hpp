@@ -24,16 +24,32 @@ struct Result {
 
 class Renderer {
 public:
-    // FrameView::Lock() -> LockedFrameView (Renderer.hpp:45-104); the data is a
-    // copy taken under the renderer's frame lock.
+    // FrameView::Lock() -> LockedFrameView (Renderer.hpp:45-104): the published
+    // frame itself, held under the renderer's frame lock until the view is
+    // destroyed (pt_frame_lock / pt_frame_unlock). Destroy it on the thread that
+    // made it; while it lives, that thread's frame-taking calls fail.
     class LockedFrameView {
     public:
-        const uint32_t* GetData() const { return pixels_.data(); }
-        size_t GetSize() const { return pixels_.size(); }
+        LockedFrameView(LockedFrameView&& o) noexcept
+            : h_(std::exchange(o.h_, nullptr)), data_(o.data_), size_(o.size_) {}
+        LockedFrameView& operator=(LockedFrameView&&) = delete;
+        LockedFrameView(const LockedFrameView&) = delete;
+        LockedFrameView& operator=(const LockedFrameView&) = delete;
+        ~LockedFrameView() {
+            if (h_) pt_frame_unlock(h_);
+        }
+        const uint32_t* GetData() const { return data_; }
+        size_t GetSize() const { return size_; }
         size_t GetElementSize() const { return sizeof(uint32_t); }
     private:
         friend class Renderer;
-        std::vector<uint32_t> pixels_;
+        explicit LockedFrameView(pt_renderer* h) {
+            if (pt_frame_lock(h, &data_, &size_) == PT_OK) h_ = h;
+            else { data_ = nullptr; size_ = 0; }
+        }
+        pt_renderer* h_ = nullptr;
+        const uint32_t* data_ = nullptr;
+        size_t size_ = 0;
     };
 
     class FrameView {
@@ -41,12 +57,7 @@ public:
         explicit FrameView(const Renderer& r) : r_(&r) {}
         size_t GetSize() const { return (size_t)r_->GetPixelGridWidth() * r_->GetPixelGridHeight(); }
         size_t GetElementSize() const { return sizeof(uint32_t); }
-        LockedFrameView Lock() const {
-            LockedFrameView v;
-            v.pixels_.resize(GetSize());
-            if (!v.pixels_.empty()) pt_frame_copy(r_->h_, v.pixels_.data(), v.pixels_.size());
-            return v;
-        }
+        LockedFrameView Lock() const { return LockedFrameView(r_->h_); }
     private:
         const Renderer* r_;
     };

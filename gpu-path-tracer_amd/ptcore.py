@@ -5,6 +5,7 @@ the reference Renderer's interface
 The library is the product: there is no fallback.  Loading fails loudly if
 libptcore.so has not been built.
 """
+import contextlib
 import ctypes as C
 import os
 
@@ -159,6 +160,8 @@ SIGNATURES = {
     "pt_get_progress": (C.c_int, [_P, C.POINTER(Progress)]),
     "pt_frame_copy": (C.c_int, [_P, _P, C.c_size_t]),
     "pt_frame_copy_hdr": (C.c_int, [_P, _P, C.c_size_t]),
+    "pt_frame_lock": (C.c_int, [_P, C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.c_size_t)]),
+    "pt_frame_unlock": (C.c_int, [_P]),
     "pt_peek_last_error": (C.c_size_t, [_P, C.c_char_p, C.c_size_t]),
     "pt_consume_last_error": (C.c_size_t, [_P, C.c_char_p, C.c_size_t]),
     "pt_render_frame": (C.c_int, [_P, C.POINTER(RenderSettings), C.POINTER(Camera), C.POINTER(MotionState), _U32,
@@ -352,6 +355,19 @@ class Renderer:
         out = np.zeros(width * height, np.uint32)
         self._check(self.lib.pt_frame_copy(self.h, _ptr(out), out.size))
         return out.reshape(height, width)
+
+    @contextlib.contextmanager
+    def LockFrame(self):
+        """pt_frame_lock / pt_frame_unlock: the published RGBA8 frame without a
+        copy, as a (pixels,) uint32 view valid only inside the ``with`` block
+        (FrameView::Lock(), Renderer.hpp:45-104)."""
+        ptr = C.POINTER(C.c_uint32)()
+        n = C.c_size_t()
+        self._check(self.lib.pt_frame_lock(self.h, C.byref(ptr), C.byref(n)))
+        try:
+            yield np.ctypeslib.as_array(ptr, (n.value,)) if n.value else np.zeros(0, np.uint32)
+        finally:
+            self._check(self.lib.pt_frame_unlock(self.h))
 
     def GetFrameHdr(self, width, height):
         out = np.zeros((height, width, 4), np.float32)

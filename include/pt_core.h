@@ -281,6 +281,16 @@ int pt_is_rendering(const pt_renderer* renderer);
 pt_status pt_get_progress(const pt_renderer* renderer, pt_progress* out);
 /* FrameView::Lock()->GetData(), Renderer.hpp:45-104: RGBA8 pixels, row 0 = bottom. */
 pt_status pt_frame_copy(pt_renderer* renderer, uint32_t* rgba, size_t pixel_count);
+/* FrameView::Lock() without a copy (Renderer.hpp:45-104, LockedFrameView::GetData
+ * and GetSize): takes the renderer's frame lock and hands out the current RGBA8
+ * frame (row 0 = bottom; NULL and 0 before a frame size is set) until
+ * pt_frame_unlock, which the same thread must call.  While it is held, a render
+ * that finishes waits to publish its frame; on the locking thread every call that
+ * takes the frame (pt_frame_copy, pt_clear, pt_save_frame, the render and
+ * buffer-initialising calls) fails with PT_ERROR_INVALID_STATE instead of
+ * deadlocking.  Unlock before pt_renderer_destroy. */
+pt_status pt_frame_lock(pt_renderer* renderer, const uint32_t** rgba, size_t* pixel_count);
+pt_status pt_frame_unlock(pt_renderer* renderer);
 /* accumulation / spp as float RGBA (needed for relMSE; no reference counterpart). */
 pt_status pt_frame_copy_hdr(pt_renderer* renderer, float* rgba, size_t pixel_count);
 /* The frame as an image file, by extension (SaveImage, ImageIO.cpp:119-160, as the

@@ -58,3 +58,52 @@ def test_frames_follow_renders_sizes_and_clear(oracle_lib):
         assert np.array_equal(r.GetFrame(32, 32), frames[2])
     finally:
         r.close()
+
+
+def test_frame_lock_is_the_published_frame_without_a_copy():
+    """pt_frame_lock hands out the published frame itself (FrameView::Lock(),
+    Renderer.hpp:45-104): it equals the copy, a second lock or a frame-taking
+    call on the locking thread is refused with PT_ERROR_INVALID_STATE, and a
+    reader on another thread waits until the lock is released."""
+    import threading
+    import time
+
+    s = scenes.cornell_box()
+    r = ptcore.Renderer()
+    try:
+        r.InitializeRenderingBuffers(32, 32)
+        r.InitializeSceneBuffers(s)
+        r.RenderFrame(_settings(32, 32, 5), s.view)
+        copy = r.GetFrame(32, 32)
+        with r.LockFrame() as px:
+            assert px.shape == (32 * 32,)
+            assert np.array_equal(px.reshape(32, 32), copy)
+            for call in (lambda: r.GetFrame(32, 32), lambda: r.Clear(0, 0, 0, 0),
+                         lambda: r.RenderFrame(_settings(32, 32, 6), s.view),
+                         lambda: r.InitializeRenderingBuffers(16, 16)):
+                with pytest.raises(ptcore.PtError) as e:
+                    call()
+                assert e.value.status == 2   # PT_ERROR_INVALID_STATE
+            with pytest.raises(ptcore.PtError) as e:
+                with r.LockFrame():
+                    pass
+            assert e.value.status == 2
+
+            got = {}
+            t = threading.Thread(target=lambda: got.setdefault("f", r.GetFrame(32, 32)))
+            t.start()
+            time.sleep(0.3)
+            assert t.is_alive() and "f" not in got   # blocked on the frame lock
+        t.join(30)
+        assert not t.is_alive() and np.array_equal(got["f"], copy)
+        with pytest.raises(ptcore.PtError) as e:
+            r._check(r.lib.pt_frame_unlock(r.h))   # not locked by this thread
+        assert e.value.status == 2
+
+        # after the lock: the calls work again and the frame follows the next render
+        r.RenderFrame(_settings(32, 32, 6), s.view)
+        with r.LockFrame() as px:
+            locked = px.reshape(32, 32).copy()
+        assert np.array_equal(locked, r.GetFrame(32, 32)) and not np.array_equal(locked, copy)
+    finally:
+        r.close()
