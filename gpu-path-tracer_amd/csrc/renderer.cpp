@@ -59,6 +59,9 @@ Renderer::Renderer() {
 }
 
 Renderer::~Renderer() {
+    // a frame lock still held by the destroying thread is released first, so
+    // that the render thread can finish its last publish
+    if (frame_locked_here()) (void)UnlockFrame();
     if (server_) (void)ServerStop(nullptr);
     (void)StopRendering();
     invalidate_graph();
@@ -926,6 +929,8 @@ pt_status Renderer::ResumeSimulation(uint32_t start_frame) {
 
 pt_status Renderer::StopRendering() {
     if (server_) return fail(PT_ERROR_INVALID_STATE, "the server is running: stop it with pt_server_stop");
+    // the render thread may be waiting for the frame lock to publish a frame
+    if (frame_locked_here()) return fail(PT_ERROR_INVALID_STATE, "this thread holds the frame lock (pt_frame_lock)");
     if (render_thread_.joinable()) {
         stop_requested_.store(true);
         render_thread_.join();

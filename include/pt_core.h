@@ -287,8 +287,9 @@ pt_status pt_frame_copy(pt_renderer* renderer, uint32_t* rgba, size_t pixel_coun
  * pt_frame_unlock, which the same thread must call.  While it is held, a render
  * that finishes waits to publish its frame; on the locking thread every call that
  * takes the frame (pt_frame_copy, pt_clear, pt_save_frame, the render and
- * buffer-initialising calls) fails with PT_ERROR_INVALID_STATE instead of
- * deadlocking.  Unlock before pt_renderer_destroy. */
+ * buffer-initialising calls) or waits for the render thread (pt_stop_rendering)
+ * fails with PT_ERROR_INVALID_STATE instead of deadlocking.  pt_renderer_destroy
+ * on the locking thread releases the lock first. */
 pt_status pt_frame_lock(pt_renderer* renderer, const uint32_t** rgba, size_t* pixel_count);
 pt_status pt_frame_unlock(pt_renderer* renderer);
 /* accumulation / spp as float RGBA (needed for relMSE; no reference counterpart). */

@@ -3,6 +3,8 @@ Clear, RenderFrame's copy of the finished frame to the host, the frame copy a
 caller reads): every finished render becomes the frame a reader copies, a
 size change starts from a cleared frame of the new size, Clear paints the
 current frame, and a caller buffer smaller than the frame is refused."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -80,7 +82,7 @@ def test_frame_lock_is_the_published_frame_without_a_copy():
             assert np.array_equal(px.reshape(32, 32), copy)
             for call in (lambda: r.GetFrame(32, 32), lambda: r.Clear(0, 0, 0, 0),
                          lambda: r.RenderFrame(_settings(32, 32, 6), s.view),
-                         lambda: r.InitializeRenderingBuffers(16, 16)):
+                         lambda: r.InitializeRenderingBuffers(16, 16), r.StopRendering):
                 with pytest.raises(ptcore.PtError) as e:
                     call()
                 assert e.value.status == 2   # PT_ERROR_INVALID_STATE
@@ -105,5 +107,10 @@ def test_frame_lock_is_the_published_frame_without_a_copy():
         with r.LockFrame() as px:
             locked = px.reshape(32, 32).copy()
         assert np.array_equal(locked, r.GetFrame(32, 32)) and not np.array_equal(locked, copy)
+
+        # destroying the renderer on the locking thread releases the lock
+        ptr = ctypes.POINTER(ctypes.c_uint32)()
+        n = ctypes.c_size_t()
+        assert r.lib.pt_frame_lock(r.h, ctypes.byref(ptr), ctypes.byref(n)) == 0 and n.value == 32 * 32
     finally:
         r.close()
