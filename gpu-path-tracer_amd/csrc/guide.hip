@@ -327,14 +327,15 @@ __device__ __forceinline__ void guide_copies(uint32_t k, float x, uint16_t* w, u
     }
 }
 
-// Sum of the partial gradients of parameters [64 blockIdx.x, +64): the four
-// waves of the block sum every fourth partial (coalesced 256-byte rows), then
-// combine through LDS.  Returns the sum in the first wave (other waves: 0).
-constexpr int kSumBlock = 256;
+// Sum of the partial gradients of parameters [64 blockIdx.x, +64): the 16
+// waves of the block sum every 16th partial (coalesced 256-byte rows, eight
+// loads in flight per lane), then combine through LDS in a fixed order.
+// Returns the sum in the first wave (other waves: 0).
+constexpr int kSumBlock = 1024;
 __device__ __forceinline__ float sum_parts(const float* part, uint32_t parts, uint32_t k) {
     __shared__ float red[kSumBlock];
     const uint32_t wave = threadIdx.x >> 6;
-    constexpr uint32_t kStep = kSumBlock / 64, kUnroll = 8;   // eight independent loads in flight
+    constexpr uint32_t kStep = kSumBlock / 64, kUnroll = 8;
     float acc[kUnroll] = {};
     if (k < (uint32_t)kGuideParams) {
         uint32_t b = wave;
@@ -349,7 +350,9 @@ __device__ __forceinline__ float sum_parts(const float* part, uint32_t parts, ui
     red[threadIdx.x] = s;
     __syncthreads();
     float t = 0.0f;
-    if (wave == 0) t = red[threadIdx.x] + red[threadIdx.x + 64] + red[threadIdx.x + 128] + red[threadIdx.x + 192];
+    if (wave == 0)
+#pragma unroll
+        for (uint32_t w = 0; w < kStep; w++) t += red[threadIdx.x + 64 * w];
     return t;
 }
 
