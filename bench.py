@@ -30,7 +30,7 @@ GUIDE_PARAMS = 16640               # fp32 partial gradient of one 64-record bloc
 # 32 B in, hit record + queue id 17 B out, 128 B per BVH4 node, 48 B per triangle
 RAY_IN_B, HIT_OUT_B, NODE_B, TRI_B = 32, 17, 128, 48
 QNODE_B = 64        # quantized BVH4 node
-Q8NODE_B = 96       # 8-wide quantized node (any hit): the 96 bytes of its 128-byte line a visit reads
+Q8NODE_B = 80       # 8-wide compressed node (any hit): the 80 bytes of its 128-byte line a visit reads
 # one shadow ray: its 48-byte record (origin|tMax, direction|scene, radiance|pixel)
 # in; the 12-byte radiance add of an unoccluded ray is not counted
 SHADOW_IN_B = 48

@@ -220,79 +220,137 @@ uint32_t emit(const TNode* n, Bvh4Build& out, uint32_t level) {
     return index;
 }
 
-// Emits the 8-wide quantized node that replaces BVH2 node `n` (internal), in
-// DFS preorder: up to eight children by expanding the largest-area internal
-// child (the rule of emit), child boxes padded as in set_child, then quantized.
-uint32_t emit8(const TNode* n, Bvh4Build& out, uint32_t level) {
-    const uint32_t index = (uint32_t)out.qnodes8.size();
-    out.qnodes8.emplace_back();
-    out.depth8 = std::max(out.depth8, level);
-    const TNode* ch[8] = {n->left.get(), n->right.get(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    int count = 2;
-    while (count < 8) {
-        int pick = -1;
-        float best = -1.0f;
+// Slot of each of the `count` children of an 8-wide node (boxes lo/hi
+// [axis][child]), in the manner of the compressed wide BVH of Ylitie, Karras
+// and Laine (HPG 2017): slot s belongs to the ray octant s (bit a set = the
+// direction's component a is negative), whose first child should be the one
+// farthest upstream along the octant's diagonal d_s, i.e. with the smallest
+// (centre - mean centre) . d_s.  Pairs are taken greedily, cheapest first
+// (ties: lowest child, then lowest slot).
+void octant_slots(const float lo[3][8], const float hi[3][8], int count, int slot_of[8]) {
+    float cen[8][3];
+    float mid[3] = {0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < count; i++)
+        for (int a = 0; a < 3; a++) {
+            cen[i][a] = 0.5f * (lo[a][i] + hi[a][i]);
+            mid[a] += cen[i][a];
+        }
+    for (int a = 0; a < 3; a++) mid[a] /= (float)std::max(count, 1);
+    bool slot_used[8] = {};
+    for (int i = 0; i < 8; i++) slot_of[i] = -1;
+    for (int round = 0; round < count; round++) {
+        float best = kInf;
+        int bi = -1, bs = -1;
         for (int i = 0; i < count; i++) {
-            if (!ch[i]->leaf() && ch[i]->box.area() > best) {
-                best = ch[i]->box.area();
-                pick = i;
+            if (slot_of[i] >= 0) continue;
+            for (int s = 0; s < 8; s++) {
+                if (slot_used[s]) continue;
+                float cost = 0.0f;
+                for (int a = 0; a < 3; a++) cost += (s >> a & 1) ? mid[a] - cen[i][a] : cen[i][a] - mid[a];
+                if (bi < 0 || cost < best) {
+                    best = cost;
+                    bi = i;
+                    bs = s;
+                }
             }
         }
-        if (pick < 0) break;
-        const TNode* c = ch[pick];
-        ch[pick] = c->left.get();
-        ch[count++] = c->right.get();
+        slot_of[bi] = bs;
+        slot_used[bs] = true;
     }
-    uint32_t refs[8];
-    float lo[3][8] = {}, hi[3][8] = {};
-    for (int i = 0; i < 8; i++) refs[i] = kInvalid;
-    for (int i = 0; i < count; i++) {
-        const TNode* c = ch[i];
-        if (c->leaf()) {
-            if (c->count > kMaxLeafSize) throw std::runtime_error("BVH leaf larger than 16 triangles");
-            refs[i] = make_leaf_ref(c->begin, c->count);
-        } else {
-            refs[i] = emit8(c, out, level + 1);
-        }
-        float l[3] = {c->box.lo[0], c->box.lo[1], c->box.lo[2]};
-        float h[3] = {c->box.hi[0], c->box.hi[1], c->box.hi[2]};
-        pad_bounds(l, h);
-        for (int a = 0; a < 3; a++) {
-            lo[a][i] = l[a];
-            hi[a][i] = h[a];
-        }
-    }
-    Bvh8QNode& q = out.qnodes8[index];   // the recursion above may have moved the array
-    quantize_children(lo, hi, refs, 8, q.origin, q.scale, q.lo, q.hi);
-    for (int i = 0; i < 8; i++) {
-        q.child[i] = refs[i];
-        q.pad[i] = 0u;
-    }
-    return index;
 }
 
-// An 8-wide root over a single leaf (or none).
-void root8(const TNode* leaf, Bvh4Build& out) {
-    out.qnodes8.emplace_back();
-    out.depth8 = 1;
-    uint32_t refs[8];
-    float lo[3][8] = {}, hi[3][8] = {};
-    for (int i = 0; i < 8; i++) refs[i] = kInvalid;
-    if (leaf) {
-        refs[0] = make_leaf_ref(leaf->begin, leaf->count);
-        float l[3] = {leaf->box.lo[0], leaf->box.lo[1], leaf->box.lo[2]};
-        float h[3] = {leaf->box.hi[0], leaf->box.hi[1], leaf->box.hi[2]};
-        pad_bounds(l, h);
-        for (int a = 0; a < 3; a++) {
-            lo[a][0] = l[a];
-            hi[a][0] = h[a];
+// The 8-wide quantized any-hit tree (Bvh8QNode) of the binary tree under
+// `root`, numbered breadth first.  Each node takes up to eight children of its
+// binary node by expanding the largest-area internal child (the rule of emit),
+// child boxes padded as in set_child, puts them into octant slots
+// (octant_slots), appends its inner children to the queue (consecutive node
+// indices in slot order) and the triangles of its leaf children to order8
+// (consecutive positions in slot order).  A leaf root is the only child of the
+// root; an empty tree is a root without children.
+void emit8_tree(const TNode* root, const std::vector<uint32_t>& idx, Bvh4Build& out) {
+    struct Item {
+        const TNode* n;
+        uint32_t level;
+    };
+    std::vector<Item> queue{{root, 1u}};
+    out.qnodes8.assign(1, Bvh8QNode{});
+    out.order8.clear();
+    out.depth8 = 0;
+    for (size_t head = 0; head < queue.size(); head++) {
+        const TNode* n = queue[head].n;
+        const uint32_t level = queue[head].level;
+        out.depth8 = std::max(out.depth8, level);
+        const TNode* ch[8] = {};
+        int count = 0;
+        if (n->leaf()) {
+            if (n->count > 0) ch[count++] = n;
+        } else {
+            ch[0] = n->left.get();
+            ch[1] = n->right.get();
+            count = 2;
+            while (count < 8) {
+                int pick = -1;
+                float best = -1.0f;
+                for (int i = 0; i < count; i++) {
+                    if (!ch[i]->leaf() && ch[i]->box.area() > best) {
+                        best = ch[i]->box.area();
+                        pick = i;
+                    }
+                }
+                if (pick < 0) break;
+                const TNode* c = ch[pick];
+                ch[pick] = c->left.get();
+                ch[count++] = c->right.get();
+            }
         }
-    }
-    Bvh8QNode& q = out.qnodes8[0];
-    quantize_children(lo, hi, refs, 8, q.origin, q.scale, q.lo, q.hi);
-    for (int i = 0; i < 8; i++) {
-        q.child[i] = refs[i];
-        q.pad[i] = 0u;
+        float lo[3][8] = {}, hi[3][8] = {};
+        for (int i = 0; i < count; i++) {
+            float l[3] = {ch[i]->box.lo[0], ch[i]->box.lo[1], ch[i]->box.lo[2]};
+            float h[3] = {ch[i]->box.hi[0], ch[i]->box.hi[1], ch[i]->box.hi[2]};
+            pad_bounds(l, h);
+            for (int a = 0; a < 3; a++) {
+                lo[a][i] = l[a];
+                hi[a][i] = h[a];
+            }
+        }
+        int slot_of[8];
+        octant_slots(lo, hi, count, slot_of);
+        const TNode* at[8] = {};
+        uint32_t occupied[8];
+        float slo[3][8] = {}, shi[3][8] = {};
+        for (int s = 0; s < 8; s++) occupied[s] = kInvalid;
+        for (int i = 0; i < count; i++) {
+            const int s = slot_of[i];
+            at[s] = ch[i];
+            occupied[s] = 0u;
+            for (int a = 0; a < 3; a++) {
+                slo[a][s] = lo[a][i];
+                shi[a][s] = hi[a][i];
+            }
+        }
+        Bvh8QNode q{};
+        quantize_children(slo, shi, occupied, 8, q.origin, q.scale, q.lo, q.hi);
+        uint32_t imask = 0u;
+        q.node_base = (uint32_t)out.qnodes8.size();
+        for (int s = 0; s < 8; s++) {
+            if (!at[s] || at[s]->leaf()) continue;
+            imask |= 1u << s;
+            queue.push_back({at[s], level + 1u});
+            out.qnodes8.emplace_back();
+        }
+        if (out.qnodes8.size() > (size_t)kMaxNodeIndex) throw std::runtime_error("the 8-wide BVH has more than 2^30 nodes");
+        q.scale |= imask << 24;
+        q.tri_base = (uint32_t)out.order8.size();
+        uint32_t end = 0u;
+        for (int s = 0; s < 8; s++) {
+            if (at[s] && at[s]->leaf()) {
+                if (at[s]->count > kMaxLeafSize) throw std::runtime_error("BVH leaf larger than 16 triangles");
+                for (uint32_t k = 0; k < at[s]->count; k++) out.order8.push_back(idx[at[s]->begin + k]);
+                end += at[s]->count;
+            }
+            q.tri_end[s >> 2] |= end << (8 * (s & 3));
+        }
+        out.qnodes8[head] = q;
     }
 }
 
@@ -363,7 +421,10 @@ Bvh4Build build_bvh4(const float* positions, uint32_t T, const BvhBuildOptions& 
         out.nodes.emplace_back();
         clear_node(out.nodes[0]);
         out.depth = 1;
-        if (options.wide8) root8(nullptr, out);
+        if (options.wide8) {
+            const TNode none;
+            emit8_tree(&none, b.idx, out);
+        }
         return out;
     }
     std::unique_ptr<TNode> root = b.build(0, T, 0);
@@ -378,8 +439,8 @@ Bvh4Build build_bvh4(const float* positions, uint32_t T, const BvhBuildOptions& 
     }
     if (options.wide8) {
         out.qnodes8.reserve(T / 8 + 8);
-        if (root->leaf()) root8(root.get(), out);
-        else emit8(root.get(), out, 1);
+        out.order8.reserve(T);
+        emit8_tree(root.get(), b.idx, out);
     }
     out.order = std::move(b.idx);
     return out;

@@ -20,9 +20,11 @@ struct Bvh4Build {
     uint32_t max_leaf = 0;
     uint32_t width = 4;               // children per node: 4, or 2 (no collapse)
     uint32_t device_nodes = 0;        // node count of a tree built on the device (nodes and order stay empty)
-    // the same binary tree collapsed into 8-wide quantized nodes (BvhBuildOptions::wide8):
-    // the same leaves, so the same triangle order
+    // the same binary tree collapsed into 8-wide quantized nodes (BvhBuildOptions::wide8),
+    // breadth first, with the triangles of each node's leaf children consecutive:
+    // order8[k] = original prim of position k of the any-hit tree's triangles
     std::vector<Bvh8QNode> qnodes8;
+    std::vector<uint32_t> order8;
     uint32_t depth8 = 0;
 };
 

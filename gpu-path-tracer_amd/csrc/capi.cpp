@@ -339,10 +339,11 @@ pt_status pt_get_bvh_info(pt_renderer* r, pt_bvh_info* info) {
     });
 }
 
-pt_status pt_get_bvh8_quantized(pt_renderer* r, void* nodes, uint32_t node_capacity, uint32_t* node_count) {
+pt_status pt_get_bvh8_quantized(pt_renderer* r, void* nodes, uint32_t node_capacity, uint32_t* node_count,
+                                uint32_t* prim_order) {
     return guarded(r, [&](pt::Renderer& x) {
         if (!node_count) return x.fail(PT_ERROR_INVALID_ARGUMENT, "node_count is NULL");
-        return x.GetBvh8Quantized(nodes, node_capacity, *node_count);
+        return x.GetBvh8Quantized(nodes, node_capacity, *node_count, prim_order);
     });
 }
 
@@ -385,7 +386,7 @@ pt_status pt_build_bvh8_quantized(const float* positions, uint32_t triangle_coun
         if (!nodes) return PT_OK;
         if (node_capacity < b.qnodes8.size()) return PT_ERROR_INVALID_ARGUMENT;
         std::memcpy(nodes, b.qnodes8.data(), b.qnodes8.size() * sizeof(pt::Bvh8QNode));
-        if (prim_order) std::memcpy(prim_order, b.order.data(), b.order.size() * sizeof(uint32_t));
+        if (prim_order) std::memcpy(prim_order, b.order8.data(), b.order8.size() * sizeof(uint32_t));
         return PT_OK;
     } catch (const std::bad_alloc&) {
         return PT_ERROR_OUT_OF_MEMORY;

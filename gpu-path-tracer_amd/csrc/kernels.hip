@@ -205,11 +205,6 @@ struct NodeOf<kNodeBvh4Q> {
     using type = Bvh4QNode;
     static constexpr int arity = 4;
 };
-template <>
-struct NodeOf<kNodeBvh8Q> {
-    using type = Bvh8QNode;
-    static constexpr int arity = 8;
-};
 
 // Byte i of a quantized row as a float (one v_cvt_f32_ubyte<i>).
 template <int I>
@@ -266,11 +261,28 @@ __device__ __forceinline__ void children(const Bvh4QNode* node, const RayFrame& 
     c[3] = e.y;
 }
 
-// Quantized 8-wide node: the scheme of the 4-wide one over two words per bound
-// row (children 0-3 and 4-7), six 16-byte loads of the 128-byte line.
-__device__ __forceinline__ void children(const Bvh8QNode* node, const RayFrame& r, float t0[8], uint32_t c[8]) {
+// Compressed 8-wide node (Bvh8QNode): the slab test of qchild over its eight
+// children, as a mask of the children the segment enters (bit s = slot s; an
+// empty slot's near plane lies beyond its far plane), with the node's child
+// bookkeeping: inner-slot mask, first inner child, first triangle and the
+// triangle ends of the slots.
+struct Node8Hits {
+    uint32_t hits, imask, node_base, tri_base, end0, end1;
+};
+template <int I>
+__device__ __forceinline__ bool qenter(uint32_t nx, uint32_t fx, uint32_t ny, uint32_t fy, uint32_t nz, uint32_t fz,
+                                       float ax, float bx, float ay, float by, float az, float bz, const RayFrame& r) {
+    const float tnx = fmaf(qbyte<I>(nx), ax, bx), tfx = fmaf(qbyte<I>(fx), ax, bx);
+    const float tny = fmaf(qbyte<I>(ny), ay, by), tfy = fmaf(qbyte<I>(fy), ay, by);
+    const float tnz = fmaf(qbyte<I>(nz), az, bz), tfz = fmaf(qbyte<I>(fz), az, bz);
+    const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tmin));
+    const float tf = fminf(fminf(tfx, tfy), fminf(tfz, r.tmax));
+    return tn <= tf;
+}
+
+__device__ __forceinline__ Node8Hits enter8(const Bvh8QNode* node, const RayFrame& r) {
     const uint4* p = reinterpret_cast<const uint4*>(node);
-    const uint4 a = p[0], l0 = p[1], l1 = p[2], h1 = p[3], c0 = p[4], c1 = p[5];
+    const uint4 a = p[0], l0 = p[1], l1 = p[2], h1 = p[3], m = p[4];
     const float ax = qnode_scale(a.w & 0xFFu) * r.rd.x;
     const float ay = qnode_scale((a.w >> 8) & 0xFFu) * r.rd.y;
     const float az = qnode_scale((a.w >> 16) & 0xFFu) * r.rd.z;
@@ -283,22 +295,25 @@ __device__ __forceinline__ void children(const Bvh8QNode* node, const RayFrame& 
     const uint32_t nx0 = px ? l0.x : l1.z, fx0 = px ? l1.z : l0.x, nx1 = px ? l0.y : l1.w, fx1 = px ? l1.w : l0.y;
     const uint32_t ny0 = py ? l0.z : h1.x, fy0 = py ? h1.x : l0.z, ny1 = py ? l0.w : h1.y, fy1 = py ? h1.y : l0.w;
     const uint32_t nz0 = pz ? l1.x : h1.z, fz0 = pz ? h1.z : l1.x, nz1 = pz ? l1.y : h1.w, fz1 = pz ? h1.w : l1.y;
-    t0[0] = qchild<0>(nx0, fx0, ny0, fy0, nz0, fz0, ax, bx, ay, by, az, bz, r);
-    t0[1] = qchild<1>(nx0, fx0, ny0, fy0, nz0, fz0, ax, bx, ay, by, az, bz, r);
-    t0[2] = qchild<2>(nx0, fx0, ny0, fy0, nz0, fz0, ax, bx, ay, by, az, bz, r);
-    t0[3] = qchild<3>(nx0, fx0, ny0, fy0, nz0, fz0, ax, bx, ay, by, az, bz, r);
-    t0[4] = qchild<0>(nx1, fx1, ny1, fy1, nz1, fz1, ax, bx, ay, by, az, bz, r);
-    t0[5] = qchild<1>(nx1, fx1, ny1, fy1, nz1, fz1, ax, bx, ay, by, az, bz, r);
-    t0[6] = qchild<2>(nx1, fx1, ny1, fy1, nz1, fz1, ax, bx, ay, by, az, bz, r);
-    t0[7] = qchild<3>(nx1, fx1, ny1, fy1, nz1, fz1, ax, bx, ay, by, az, bz, r);
-    c[0] = c0.x;
-    c[1] = c0.y;
-    c[2] = c0.z;
-    c[3] = c0.w;
-    c[4] = c1.x;
-    c[5] = c1.y;
-    c[6] = c1.z;
-    c[7] = c1.w;
+    uint32_t h = 0u;
+    h |= qenter<0>(nx0, fx0, ny0, fy0, nz0, fz0, ax, bx, ay, by, az, bz, r) ? 0x01u : 0u;
+    h |= qenter<1>(nx0, fx0, ny0, fy0, nz0, fz0, ax, bx, ay, by, az, bz, r) ? 0x02u : 0u;
+    h |= qenter<2>(nx0, fx0, ny0, fy0, nz0, fz0, ax, bx, ay, by, az, bz, r) ? 0x04u : 0u;
+    h |= qenter<3>(nx0, fx0, ny0, fy0, nz0, fz0, ax, bx, ay, by, az, bz, r) ? 0x08u : 0u;
+    h |= qenter<0>(nx1, fx1, ny1, fy1, nz1, fz1, ax, bx, ay, by, az, bz, r) ? 0x10u : 0u;
+    h |= qenter<1>(nx1, fx1, ny1, fy1, nz1, fz1, ax, bx, ay, by, az, bz, r) ? 0x20u : 0u;
+    h |= qenter<2>(nx1, fx1, ny1, fy1, nz1, fz1, ax, bx, ay, by, az, bz, r) ? 0x40u : 0u;
+    h |= qenter<3>(nx1, fx1, ny1, fy1, nz1, fz1, ax, bx, ay, by, az, bz, r) ? 0x80u : 0u;
+    return Node8Hits{h, a.w >> 24, m.x, m.y, m.z, m.w};
+}
+
+// Bits k and k ^ oct of the two bytes of m exchanged: slot order to the visit
+// order of a ray of octant oct, and back.
+__device__ __forceinline__ uint32_t octant_order(uint32_t m, uint32_t oct) {
+    if (oct & 1u) m = ((m & 0x5555u) << 1) | ((m >> 1) & 0x5555u);
+    if (oct & 2u) m = ((m & 0x3333u) << 2) | ((m >> 2) & 0x3333u);
+    if (oct & 4u) m = ((m & 0x0F0Fu) << 4) | ((m >> 4) & 0x0F0Fu);
+    return m;
 }
 
 __device__ __forceinline__ void children(const Bvh4Node* node, const RayFrame& r, float t0[4], uint32_t c[4]) {
@@ -363,7 +378,7 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
         (MULTI || !PT_TOP_TABLE) ? 0u : (BLOCK >= 256 ? kTopBytes : (uint32_t)BLOCK * 32u) / (uint32_t)sizeof(Node);
     __shared__ Node top[kTopCap ? kTopCap : 1];
     const uint32_t tid = threadIdx.x;
-    const void* top_src = sc.top[NODE == kNodeBvh8Q ? 2 : NODE == kNodeBvh4Q ? 1 : 0];
+    const void* top_src = sc.top[NODE == kNodeBvh4Q ? 1 : 0];
     const uint32_t root = (kTopCap && top_src) ? kTopBit : 0u;
     if (kTopCap && top_src) {
         const uint4* s4 = static_cast<const uint4*>(top_src);
@@ -404,8 +419,7 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
     bool exhausted = chunk_next >= n;
     uint32_t item = 0, ref = 0, sp = 0;
     // per lane when MULTI: the ray's scene
-    const Node* nodes_base =
-        static_cast<const Node*>(NODE == kNodeBvh8Q ? sc.qnodes8 : NODE == kNodeBvh4Q ? sc.qnodes : sc.nodes);
+    const Node* nodes_base = static_cast<const Node*>(NODE == kNodeBvh4Q ? sc.qnodes : sc.nodes);
     const TriRecord* tris_base = sc.tris;
     RayFrame r;
     float best_t = 0.0f, best_u = 0.0f, best_v = 0.0f;
@@ -480,9 +494,8 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                         uint32_t scene = 0;
                         item = src.load(chunk_next + lane_rank, o, d, tmin, tmax, scene);
                         if (MULTI) {
-                            nodes_base = static_cast<const Node*>(NODE == kNodeBvh8Q   ? scenes[scene].qnodes8
-                                                                  : NODE == kNodeBvh4Q ? scenes[scene].qnodes
-                                                                                       : scenes[scene].nodes);
+                            nodes_base = static_cast<const Node*>(NODE == kNodeBvh4Q ? scenes[scene].qnodes
+                                                                                     : scenes[scene].nodes);
                             tris_base = scenes[scene].tris;
                         }
                         setup_ray(r, o, d, tmin, tmax);
@@ -516,28 +529,7 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
                 if (COUNT) nodes++;
                 if (!ANY || PT_ANY_SORTED) {
                     // nearest child first: sort the (distance, ref) pairs
-                    if (W == 8) {
-                        // Batcher's odd-even merge sort, 19 comparators
-                        cswap(t0[0], c[0], t0[1], c[1]);
-                        cswap(t0[2], c[2], t0[3], c[3]);
-                        cswap(t0[4], c[4], t0[5], c[5]);
-                        cswap(t0[6], c[6], t0[7], c[7]);
-                        cswap(t0[0], c[0], t0[2], c[2]);
-                        cswap(t0[1], c[1], t0[3], c[3]);
-                        cswap(t0[4], c[4], t0[6], c[6]);
-                        cswap(t0[5], c[5], t0[7], c[7]);
-                        cswap(t0[1], c[1], t0[2], c[2]);
-                        cswap(t0[5], c[5], t0[6], c[6]);
-                        cswap(t0[0], c[0], t0[4], c[4]);
-                        cswap(t0[1], c[1], t0[5], c[5]);
-                        cswap(t0[2], c[2], t0[6], c[6]);
-                        cswap(t0[3], c[3], t0[7], c[7]);
-                        cswap(t0[2], c[2], t0[4], c[4]);
-                        cswap(t0[3], c[3], t0[5], c[5]);
-                        cswap(t0[1], c[1], t0[2], c[2]);
-                        cswap(t0[3], c[3], t0[4], c[4]);
-                        cswap(t0[5], c[5], t0[6], c[6]);
-                    } else if (W == 4) {
+                    if (W == 4) {
                         cswap(t0[0], c[0], t0[1], c[1]);
                         cswap(t0[W - 2], c[W - 2], t0[W - 1], c[W - 1]);
                         cswap(t0[0], c[0], t0[W - 2], c[W - 2]);
@@ -589,6 +581,197 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
         if (active && fin) {
             if (ANY) src.finish_any(item, best_prim != kInvalid, nodes, tris);
             else src.finish_closest(item, best_prim, best_t, best_u, best_v, best_queue, nodes, tris);
+            acc_nodes += nodes;
+            acc_tris += tris;
+            active = false;
+        }
+    }
+    if (COUNT) src.flush_counts(acc_nodes, acc_tris);
+}
+
+// Any hit over the compressed 8-wide nodes (Bvh8QNode), after the node-group
+// traversal of Ylitie, Karras and Laine (HPG 2017).  A ray of octant oct (bit a
+// set where rd_a < 0) takes a node's entered children in the visit order
+// k ^ oct of their slots, which the builder's octant slots make close to
+// nearest first: first the leaf children (one per outer iteration), then the
+// inner children, whose pending ones stay on the stack as one group entry
+// (first inner child, inner mask | pending visit mask << 8), so a node step
+// pushes at most one entry and a pop reads the next child from the entry
+// alone: no sort, no per-child push.  Persistent lanes, chunked ray claims and
+// the LDS stack with global spill as in trace_persistent; single-scene
+// launches keep the first nodes (the top levels: breadth-first numbering) in
+// LDS.
+template <int BLOCK, int LDS, bool COUNT, bool MULTI, class Src>
+__device__ __forceinline__ void trace_any8(const Src& src, const SceneDev& sc, const SceneDev* scenes,
+                                           const TraceConfig& cfg) {
+    using Node = Bvh8QNode;
+    __shared__ uint2 stack[LDS][BLOCK];
+    constexpr uint32_t kTopCap =
+        (MULTI || !PT_TOP_TABLE) ? 0u : (BLOCK >= 256 ? kTopBytes : (uint32_t)BLOCK * 32u) / (uint32_t)sizeof(Node);
+    __shared__ Node top[kTopCap ? kTopCap : 1];
+    const uint32_t tid = threadIdx.x;
+    uint32_t top_n = 0;
+    if (kTopCap && sc.qnodes8) {
+        top_n = min(kTopCap, sc.node8_count);
+        const uint4* s4 = static_cast<const uint4*>(sc.qnodes8);
+        uint4* d4 = reinterpret_cast<uint4*>(top);
+        const uint32_t words = top_n * (uint32_t)(sizeof(Node) / 16);
+        for (uint32_t i = tid; i < words; i += BLOCK) d4[i] = s4[i];
+        __syncthreads();
+    }
+    const uint32_t gtid = blockIdx.x * BLOCK + tid;
+    const uint32_t stride = gridDim.x * BLOCK;
+    const uint32_t cap = min(cfg.stack_cap, (uint32_t)LDS + cfg.spill_depth);
+    const uint32_t n = src.count();
+    unsigned int* work = src.work();
+    const uint32_t waves = stride >> 6;
+    const uint32_t chunk = min(max((n / (4u * waves) + 63u) & ~63u, 64u), max(cfg.chunk, 64u));
+
+    bool active = false;      // lane holds a ray
+    bool fin = false;         // its traversal has ended
+    bool occluded = false;
+    const uint32_t wave_id = gtid >> 6;
+    const uint32_t static_end = waves * chunk;
+    uint32_t chunk_next = min(wave_id * chunk, n), chunk_end = min(wave_id * chunk + chunk, n);
+    bool exhausted = chunk_next >= n;
+    uint32_t item = 0, node = kInvalid, sp = 0, oct = 0;
+    uint32_t lmask = 0, lbase = 0, lend0 = 0, lend1 = 0;   // leaf children of the last node, visit order
+    const Node* nodes_base = static_cast<const Node*>(sc.qnodes8);
+    const TriRecord* tris_base = sc.tris8;
+    RayFrame r;
+    uint32_t nodes = 0, tris = 0, acc_nodes = 0, acc_tris = 0;
+
+    auto load_entry = [&](uint32_t i) -> uint2 {
+        return i < (uint32_t)LDS ? stack[i][tid] : cfg.spill[(size_t)(i - LDS) * stride + gtid];
+    };
+    auto store_entry = [&](uint32_t i, uint2 e) {
+        if (i < (uint32_t)LDS) stack[i][tid] = e;
+        else cfg.spill[(size_t)(i - LDS) * stride + gtid] = e;
+    };
+    // the inner child at the lowest pending visit position of a group
+    auto child_of = [&](uint32_t base, uint32_t imask, uint32_t pending) -> uint32_t {
+        const uint32_t slot = (uint32_t)__builtin_ctz(pending) ^ oct;
+        return base + (uint32_t)__builtin_popcount(imask & ((1u << slot) - 1u));
+    };
+
+    while (true) {
+        // ---- refill idle lanes from the wave's chunk
+        if (!exhausted) {
+            const uint64_t idle = __ballot(!active);
+            const uint32_t nidle = (uint32_t)__popcll(idle);
+            if (nidle >= cfg.refill || nidle == 64u) {
+                if (chunk_next >= chunk_end) {
+                    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1u;
+                    uint32_t base = 0;
+                    if (lane_id() == leader) base = atomicAdd(work, chunk);
+                    base = readlane_u32(base, leader);
+                    base = static_end > n ? n : base + static_end;
+                    if (base >= n) {
+                        exhausted = true;
+                    } else {
+                        chunk_next = base;
+                        chunk_end = min(base + chunk, n);
+                    }
+                }
+                if (chunk_next < chunk_end) {
+                    const uint32_t avail = chunk_end - chunk_next;
+                    const uint32_t lane_rank = mbcnt(idle);
+                    if (!active && lane_rank < avail) {
+                        V3 o, d;
+                        float tmin, tmax;
+                        uint32_t scene = 0;
+                        item = src.load(chunk_next + lane_rank, o, d, tmin, tmax, scene);
+                        if (MULTI) {
+                            nodes_base = static_cast<const Node*>(scenes[scene].qnodes8);
+                            tris_base = scenes[scene].tris8;
+                        }
+                        setup_ray(r, o, d, tmin, tmax);
+                        oct = (r.rd.x >= 0.0f ? 0u : 1u) | (r.rd.y >= 0.0f ? 0u : 2u) | (r.rd.z >= 0.0f ? 0u : 4u);
+                        occluded = false;
+                        nodes = tris = 0;
+                        node = 0u;
+                        lmask = 0u;
+                        sp = 0;
+                        fin = false;
+                        active = true;
+                    }
+                    chunk_next += min(nidle, avail);
+                }
+            }
+        }
+        if (__ballot(active) == 0) break;
+
+        // ---- one node step for lanes whose last node's leaves are done
+        {
+            const bool step = active && !fin && lmask == 0u && node != kInvalid;
+            if (__ballot(step) != 0 && step) {
+                const Node* np = node < top_n ? top + node : nodes_base + node;
+                const Node8Hits h = enter8(np, r);
+                if (COUNT) nodes++;
+                // both masks into visit order at once: inner hits low byte, leaf hits high byte
+                const uint32_t v = octant_order((h.hits & h.imask) | ((h.hits & ~h.imask) << 8), oct);
+                const uint32_t inner = v & 0xFFu;
+                lmask = v >> 8;
+                lbase = h.tri_base;
+                lend0 = h.end0;
+                lend1 = h.end1;
+                node = kInvalid;
+                if (inner != 0u) {
+                    node = child_of(h.node_base, h.imask, inner);
+                    const uint32_t rest = inner & (inner - 1u);
+                    if (rest != 0u) {
+                        if (sp < cap) {
+                            store_entry(sp, make_uint2(h.node_base, h.imask | (rest << 8)));
+                            sp++;
+                        } else {
+                            atomicOr(cfg.overflow, 1u);
+                        }
+                    }
+                }
+            }
+        }
+
+        // ---- one leaf child: its triangles, two at a time
+        {
+            const bool step = active && !fin && lmask != 0u;
+            if (__ballot(step) != 0 && step) {
+                const uint32_t slot = (uint32_t)__builtin_ctz(lmask) ^ oct;
+                lmask &= lmask - 1u;
+                const uint32_t ends = slot < 4u ? lend0 : lend1;
+                const uint32_t end = (ends >> (8u * (slot & 3u))) & 0xFFu;
+                const uint32_t prev = slot == 0u ? 0u : slot == 4u ? (lend0 >> 24) : (ends >> (8u * ((slot - 1u) & 3u))) & 0xFFu;
+                const uint32_t first = lbase + prev, cnt = end - prev;
+                if (COUNT) tris += cnt;
+                for (uint32_t k = 0; k < cnt; k += 2) {
+                    float4 a0, b0, c0, a1, b1, c1;
+                    load_tri(tris_base + first + k, a0, b0, c0);
+                    const bool two = k + 1 < cnt;
+                    if (two) load_tri(tris_base + first + k + 1, a1, b1, c1);
+                    float t, u, v;
+                    if (intersect_triangle(a0, b0, c0, r.o, r.d, r.tmin, t, u, v) && t < r.tmax) occluded = true;
+                    if (two && intersect_triangle(a1, b1, c1, r.o, r.d, r.tmin, t, u, v) && t < r.tmax) occluded = true;
+                }
+                if (occluded) fin = true;
+            }
+        }
+
+        // ---- lanes with nothing left at their node take the next child of the top group
+        if (active && !fin && lmask == 0u && node == kInvalid) {
+            if (sp == 0u) {
+                fin = true;
+            } else {
+                const uint2 e = load_entry(sp - 1u);
+                const uint32_t imask = e.y & 0xFFu, pending = e.y >> 8;
+                node = child_of(e.x, imask, pending);
+                const uint32_t rest = pending & (pending - 1u);
+                if (rest == 0u) sp--;
+                else store_entry(sp - 1u, make_uint2(e.x, imask | (rest << 8)));
+            }
+        }
+
+        // ---- finished rays write their result
+        if (active && fin) {
+            src.finish_any(item, occluded, nodes, tris);
             acc_nodes += nodes;
             acc_tris += tris;
             active = false;
@@ -724,7 +907,12 @@ struct BatchSrc {
 template <int BLOCK, int LDS, int W, bool ANY, bool COUNT, bool MULTI, class Src>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : PT_TRACE_WAVES)))
 void k_trace(Src src, SceneDev sc, const SceneDev* scenes, TraceConfig cfg) {
-    trace_persistent<BLOCK, LDS, W, ANY, COUNT, MULTI, Src>(src, sc, scenes, cfg);
+    if constexpr (W == kNodeBvh8Q) {
+        static_assert(ANY, "the 8-wide compressed nodes serve the any-hit traversal only");
+        trace_any8<BLOCK, LDS, COUNT, MULTI, Src>(src, sc, scenes, cfg);
+    } else {
+        trace_persistent<BLOCK, LDS, W, ANY, COUNT, MULTI, Src>(src, sc, scenes, cfg);
+    }
 }
 
 // ================================================================ shading math
@@ -1885,7 +2073,7 @@ __global__ void k_build_top(const Node* nodes, Node* table) {
 }
 
 size_t top_table_bytes(int kind) {
-    const size_t node = (kind == kNodeBvh4 || kind == kNodeBvh8Q) ? 128 : 64;
+    const size_t node = kind == kNodeBvh4 ? 128 : 64;
     return kTopBytes + kTopBytes / node * sizeof(uint32_t);
 }
 
@@ -1902,10 +2090,6 @@ hipError_t launch_build_top(hipStream_t s, int kind, const void* nodes, void* ta
     case kNodeBvh4Q:
         hipLaunchKernelGGL((k_build_top<Bvh4QNode, 4>), dim3(1), dim3(64), 0, s, static_cast<const Bvh4QNode*>(nodes),
                            static_cast<Bvh4QNode*>(table));
-        break;
-    case kNodeBvh8Q:
-        hipLaunchKernelGGL((k_build_top<Bvh8QNode, 8>), dim3(1), dim3(64), 0, s, static_cast<const Bvh8QNode*>(nodes),
-                           static_cast<Bvh8QNode*>(table));
         break;
     default:
         return hipErrorInvalidValue;

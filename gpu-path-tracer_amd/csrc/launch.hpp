@@ -25,6 +25,7 @@ struct SceneDev {
     const void* qnodes;             // Bvh4QNode[]: the same 4-wide tree quantized, or null
     const void* qnodes8;            // Bvh8QNode[]: the same binary tree 8-wide and quantized (any hit), or null
     const TriRecord* tris;          // BVH order
+    const TriRecord* tris8;         // order of the 8-wide tree's leaves (with qnodes8)
     const ShadeRecord* shade;       // original order
     const MaterialRecord* materials;
     const LightRecord* lights;
@@ -33,7 +34,8 @@ struct SceneDev {
     uint32_t tri_count;
     uint32_t node_width;            // 4 or 2
     uint32_t node_quantized;        // traversals that use qnodes: bit 0 closest hit, bit 1 any hit
-    const void* top[3];             // top-of-tree tables of nodes, qnodes and qnodes8 (null: none)
+    uint32_t node8_count;           // nodes of qnodes8 (breadth first: its first ones are the top of the tree)
+    const void* top[2];             // top-of-tree tables of nodes and qnodes (null: none)
 };
 
 // Writes the top-of-tree table of `nodes` (kind: kNodeBvh4, kNodeBvh2,

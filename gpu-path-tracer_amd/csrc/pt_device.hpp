@@ -161,18 +161,27 @@ PT_HD void quantize_bvh4_node(const Bvh4Node& n, Bvh4QNode& q) {
     for (int i = 0; i < 4; i++) q.child[i] = n.child[i];
 }
 
-// Quantized BVH8 node: one 128-byte line, eight children under the same
-// byte scheme as Bvh4QNode.  Bound rows hold children 0-3 in word 2a and 4-7
-// in word 2a + 1 of axis a, so the node reads as six 16-byte loads:
-// (origin, scale), (lo x0 x1 y0 y1), (lo z0 z1, hi x0 x1), (hi y0 y1 z0 z1),
-// children 0-3, children 4-7.
+// Quantized BVH8 node of the any-hit tree, in the manner of the compressed
+// wide BVH of Ylitie, Karras and Laine (HPG 2017): eight children under the
+// byte scheme of Bvh4QNode, bound rows holding children 0-3 in word 2a and 4-7
+// in word 2a + 1 of axis a.  Children are implicit: the inner children of a
+// node are consecutive nodes from node_base in slot order (breadth-first
+// numbering), the triangles of its leaf children consecutive from tri_base in
+// slot order, and byte s of tri_end the end of slot s's triangles relative to
+// tri_base (a prefix sum: the start of slot s is byte s - 1, 0 for slot 0).
+// Byte 3 of `scale` marks the inner slots.  The node is read as five 16-byte
+// loads: (origin, scale), (lo x0 x1 y0 y1), (lo z0 z1, hi x0 x1),
+// (hi y0 y1 z0 z1), (node_base, tri_base, tri_end); the line's last 48 bytes
+// are padding.
 struct alignas(128) Bvh8QNode {
     float origin[3];
     uint32_t scale;
     uint32_t lo[6];
     uint32_t hi[6];
-    uint32_t child[8];
-    uint32_t pad[8];
+    uint32_t node_base;
+    uint32_t tri_base;
+    uint32_t tri_end[2];
+    uint32_t pad[12];
 };
 static_assert(sizeof(Bvh8QNode) == 128, "quantized BVH8 node fills one cache line");
 

@@ -162,6 +162,8 @@ SceneDev ResidentScene::dev() const {
     s.qnodes = quantized ? qnodes.as<void>() : nullptr;
     s.qnodes8 = depth8 ? qnodes8.as<void>() : nullptr;
     s.tris = tris.as<TriRecord>();
+    s.tris8 = depth8 ? tris8.as<TriRecord>() : nullptr;
+    s.node8_count = depth8 ? (uint32_t)(qnodes8.bytes() / sizeof(Bvh8QNode)) : 0u;
     s.shade = shade.as<ShadeRecord>();
     s.materials = materials.as<MaterialRecord>();
     s.lights = lights.as<LightRecord>();
@@ -170,9 +172,8 @@ SceneDev ResidentScene::dev() const {
     s.tri_count = host.tri_count;
     s.node_width = host.bvh.width;
     s.node_quantized = quantized;
-    for (int k = 0; k < 3; k++) s.top[k] = top[k].bytes() ? top[k].as<void>() : nullptr;
+    for (int k = 0; k < 2; k++) s.top[k] = top[k].bytes() ? top[k].as<void>() : nullptr;
     if (!quantized) s.top[1] = nullptr;
-    if (!depth8) s.top[2] = nullptr;
     return s;
 }
 
@@ -444,6 +445,7 @@ pt_status Renderer::upload_scene(uint32_t id, const pt_scene_desc& desc) {
         // the 8-wide quantized collapse of the same binary tree, for shadow rays
         PT_TRY(upload(rs->qnodes8, h.bvh.qnodes8.data(), h.bvh.qnodes8.size() * sizeof(Bvh8QNode),
                       "8-wide quantized BVH nodes"));
+        PT_TRY(upload(rs->tris8, h.tris8.data(), h.tris8.size() * sizeof(TriRecord), "8-wide tree triangle records"));
         PT_TRY_HIP(hipStreamSynchronize(stream_), "8-wide quantized BVH nodes");
         rs->depth8 = h.bvh.depth8;
     }
@@ -456,7 +458,6 @@ pt_status Renderer::upload_scene(uint32_t id, const pt_scene_desc& desc) {
     };
     PT_TRY_HIP(build_top(rs->top[0], h.bvh.width == 2 ? kNodeBvh2 : kNodeBvh4, rs->nodes), "top-of-tree table");
     if (rs->quantized) PT_TRY_HIP(build_top(rs->top[1], kNodeBvh4Q, rs->qnodes), "top-of-tree table");
-    if (rs->depth8) PT_TRY_HIP(build_top(rs->top[2], kNodeBvh8Q, rs->qnodes8), "top-of-tree table");
     if (!gpu_bvh) {
         PT_TRY(upload(rs->tris, h.tris.data(), h.tris.size() * sizeof(TriRecord), "triangle records"));
         PT_TRY(upload(rs->materials, h.materials.data(), h.materials.size() * sizeof(MaterialRecord), "materials"));
@@ -1285,11 +1286,12 @@ pt_status Renderer::GetBvh4(void* nodes, uint32_t node_cap, uint32_t* order, uin
     return PT_OK;
 }
 
-pt_status Renderer::GetBvh8Quantized(void* nodes, uint32_t node_cap, uint32_t& count) {
+pt_status Renderer::GetBvh8Quantized(void* nodes, uint32_t node_cap, uint32_t& count, uint32_t* prim_order) {
     if (!has_scene()) return fail(PT_ERROR_INVALID_STATE, "scene buffers are not initialized");
     const ResidentScene& rs = *scenes_[0];
     count = (uint32_t)rs.host.bvh.qnodes8.size();
     if (!rs.depth8) return fail(PT_ERROR_INVALID_STATE, "scene 0 was uploaded without 8-wide quantized nodes");
+    if (prim_order) std::copy(rs.host.bvh.order8.begin(), rs.host.bvh.order8.end(), prim_order);
     if (!nodes) return PT_OK;
     if (node_cap < count) return fail(PT_ERROR_INVALID_ARGUMENT, "node capacity too small");
     PT_TRY_HIP(hipMemcpyAsync(nodes, rs.qnodes8.as<void>(), (size_t)count * sizeof(Bvh8QNode), hipMemcpyDeviceToHost,

@@ -56,7 +56,8 @@ struct ResidentScene {
     DeviceBuffer order;        // GPU-built BVH: original triangle of each BVH slot (the host copy is empty)
     DeviceBuffer qnodes;       // quantized copy of the 4-wide nodes
     DeviceBuffer qnodes8;      // the same binary tree 8-wide and quantized (any hit), or empty
-    DeviceBuffer top[3];       // top-of-tree tables of nodes, qnodes and qnodes8
+    DeviceBuffer tris8;        // triangle records in the 8-wide tree's leaf order, or empty
+    DeviceBuffer top[2];       // top-of-tree tables of nodes and qnodes (the 8-wide tree needs none)
     uint32_t quantized = 0;    // traversals that use qnodes: bit 0 closest hit, bit 1 any hit
     uint32_t depth8 = 0;       // levels of the 8-wide tree (0: none)
     SceneDev dev() const;
@@ -130,7 +131,7 @@ public:
     pt_status GetQueueOccupancy(pt_queue_occupancy* out, uint32_t cap, uint32_t& count) const;
     pt_status GetBvh4(void* nodes, uint32_t node_cap, uint32_t* order, uint32_t order_cap);
     pt_status GetBvh4Quantized(void* nodes, uint32_t node_cap);
-    pt_status GetBvh8Quantized(void* nodes, uint32_t node_cap, uint32_t& count);
+    pt_status GetBvh8Quantized(void* nodes, uint32_t node_cap, uint32_t& count, uint32_t* prim_order);
     pt_status GetTuning(pt_tuning& t) const;
     pt_status SetTuning(const pt_tuning& t);
 

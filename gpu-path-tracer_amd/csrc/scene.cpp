@@ -200,20 +200,26 @@ bool build_host_scene(const pt_scene_desc& d, const BvhBuildOptions& bvh_options
         err = {PT_ERROR_INVALID_ARGUMENT, std::string("BVH build failed: ") + e.what()};
         return false;
     }
-    out.tris.resize(T);
-    for (uint32_t k = 0; k < T; k++) {
-        const uint32_t t = out.bvh.order[k];
-        const float* P = d.positions + 9 * (size_t)t;
-        const V3 p0 = load3(P), p1 = load3(P + 3), p2 = load3(P + 6);
-        const V3 e1 = p1 - p0, e2 = p2 - p0;
-        TriRecord& r = out.tris[k];
-        r.v0[0] = p0.x; r.v0[1] = p0.y; r.v0[2] = p0.z;
-        r.e1[0] = e1.x; r.e1[1] = e1.y; r.e1[2] = e1.z;
-        r.e2[0] = e2.x; r.e2[1] = e2.y; r.e2[2] = e2.z;
-        r.prim = t;
-        r.material = d.material_ids[t];
-        r.queue = out.materials[r.material].queue;
-    }
+    // triangle records in the order of a tree's leaves (BVH4, and the 8-wide
+    // any-hit tree when it was built)
+    auto records = [&](const std::vector<uint32_t>& order, std::vector<TriRecord>& recs) {
+        recs.resize(order.size());
+        for (size_t k = 0; k < order.size(); k++) {
+            const uint32_t t = order[k];
+            const float* P = d.positions + 9 * (size_t)t;
+            const V3 p0 = load3(P), p1 = load3(P + 3), p2 = load3(P + 6);
+            const V3 e1 = p1 - p0, e2 = p2 - p0;
+            TriRecord& r = recs[k];
+            r.v0[0] = p0.x; r.v0[1] = p0.y; r.v0[2] = p0.z;
+            r.e1[0] = e1.x; r.e1[1] = e1.y; r.e1[2] = e1.z;
+            r.e2[0] = e2.x; r.e2[1] = e2.y; r.e2[2] = e2.z;
+            r.prim = t;
+            r.material = d.material_ids[t];
+            r.queue = out.materials[r.material].queue;
+        }
+    };
+    records(out.bvh.order, out.tris);
+    records(out.bvh.order8, out.tris8);
     return true;
 }
 

@@ -18,6 +18,7 @@ namespace pt {
 
 struct HostScene {
     std::vector<TriRecord> tris;          // BVH order
+    std::vector<TriRecord> tris8;         // order of the 8-wide any-hit tree (empty without one)
     std::vector<ShadeRecord> shade;       // original order
     std::vector<MaterialRecord> materials;
     std::vector<LightRecord> lights;

@@ -210,7 +210,7 @@ SIGNATURES = {
     "pt_build_bvh8_quantized": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.POINTER(C.c_uint32), _P,
                                           C.POINTER(C.c_uint32)]),
     "pt_get_bvh4_quantized": (C.c_int, [_P, _P, C.c_uint32]),
-    "pt_get_bvh8_quantized": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "pt_get_bvh8_quantized": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_uint32), _P]),
     "pt_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "pt_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
 }
@@ -605,12 +605,14 @@ class Renderer:
         return nodes, order
 
     def bvh8_quantized(self):
-        """(N, 32) uint32 rows of the 128-byte 8-wide quantized nodes of scene 0."""
+        """(N, 32) uint32 rows of the 128-byte 8-wide quantized nodes of scene 0 and
+        the tree's triangle order (original prim of each position)."""
         n = C.c_uint32(0)
-        self._check(self.lib.pt_get_bvh8_quantized(self.h, None, 0, C.byref(n)))
+        self._check(self.lib.pt_get_bvh8_quantized(self.h, None, 0, C.byref(n), None))
         out = np.zeros((n.value, 32), np.uint32)
-        self._check(self.lib.pt_get_bvh8_quantized(self.h, _ptr(out), n.value, C.byref(n)))
-        return out
+        order = np.zeros(self.bvh_info()["triangle_count"], np.uint32)
+        self._check(self.lib.pt_get_bvh8_quantized(self.h, _ptr(out), n.value, C.byref(n), _ptr(order)))
+        return out, order
 
     def bvh4_quantized(self):
         """(N, 16) uint32 rows of the 64-byte quantized nodes scene 0 is traversed with."""
@@ -650,7 +652,7 @@ def build_bvh4(positions, max_leaf_size=0):
 
 def build_bvh8_quantized(positions, max_leaf_size=0):
     """Host-only 8-wide quantized build: (nodes (N, 32) uint32 rows of 128 B,
-    prim order (that of build_bvh4), depth)."""
+    the tree's triangle order (original prim of each position), depth)."""
     lib = load_library()
     P = np.ascontiguousarray(positions, np.float32)
     n, depth = C.c_uint32(), C.c_uint32()

@@ -526,8 +526,11 @@ pt_status pt_get_bvh4(pt_renderer* renderer, void* nodes, uint32_t node_capacity
 pt_status pt_get_bvh4_quantized(pt_renderer* renderer, void* nodes, uint32_t node_capacity);
 /* The 8-wide quantized nodes of scene 0 (pt_tuning.bvh_quantized bit 2 at a host-built
  * upload), 128 B each, as uploaded; *node_count receives their number (nodes == NULL:
- * only the count).  PT_ERROR_INVALID_STATE without them. */
-pt_status pt_get_bvh8_quantized(pt_renderer* renderer, void* nodes, uint32_t node_capacity, uint32_t* node_count);
+ * only the count); prim_order (NULL or triangle-count entries) receives the original
+ * prim of each position of the tree's triangle order.  PT_ERROR_INVALID_STATE without
+ * them. */
+pt_status pt_get_bvh8_quantized(pt_renderer* renderer, void* nodes, uint32_t node_capacity, uint32_t* node_count,
+                                uint32_t* prim_order);
 
 /* Host-only BVH build of the core (no GPU needed): the 128-byte BVH4 nodes and
  * BVH-order -> original prim permutation that InitializeSceneBuffers uploads.
@@ -538,10 +541,13 @@ pt_status pt_build_bvh4(const float* positions, uint32_t triangle_count, uint32_
                         uint32_t* prim_order, uint32_t* depth);
 
 /* The same build collapsed into 8-wide quantized nodes (128 B each: box corner,
- * three scale exponents, 8-bit bounds of eight children, eight references; the
- * nodes the any-hit traversal walks when pt_tuning.bvh_quantized has bit 2), with
- * the same leaves and triangle order as pt_build_bvh4.  Host only; no reference
- * counterpart.  With nodes == NULL only *node_count and *depth are written. */
+ * three scale exponents and the inner-child mask, 8-bit bounds of eight children,
+ * the first inner child and first triangle, the children's triangle ranges; the
+ * nodes the any-hit traversal walks when pt_tuning.bvh_quantized has bit 2),
+ * numbered breadth first, each node's inner children consecutive and its leaf
+ * children's triangles consecutive; prim_order receives that triangle order (the
+ * same leaves as pt_build_bvh4, regrouped).  Host only; no reference counterpart.
+ * With nodes == NULL only *node_count and *depth are written. */
 pt_status pt_build_bvh8_quantized(const float* positions, uint32_t triangle_count, uint32_t max_leaf_size,
                                   void* nodes, uint32_t node_capacity, uint32_t* node_count,
                                   uint32_t* prim_order, uint32_t* depth);

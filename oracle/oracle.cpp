@@ -967,80 +967,88 @@ bool bvh4q_any(const Scene& s, const std::vector<uint32_t>& by_prim, const uint3
     }
 }
 
-// The 19-comparator sorting network for eight (Batcher's odd-even merge sort),
-// in the order the any-hit kernel applies it.
-constexpr int kSort8[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
-                               {0, 4}, {1, 5}, {2, 6}, {3, 7}, {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
-
-// Any hit over the core's 128-byte 8-wide quantized nodes, in the kernel's
-// order (children sorted nearest first by the network above, the farther ones
-// pushed), counting node steps and triangle tests.  32 words per node:
-// origin x y z | scales | lo rows x0 x1 y0 y1 z0 z1 | hi rows | children 0-7 | pad.
+// Any hit over the core's 8-wide compressed quantized nodes, in the kernel's
+// order: a ray of octant oct (bit a set where its reciprocal direction
+// component is negative) takes a node's entered children in visit order
+// k ^ oct of their slots -- first every leaf child, then the inner children,
+// the pending ones kept as one stack entry per node -- counting node steps and
+// triangle tests.  32 words per node: origin x y z | scale exponents, inner
+// mask in byte 3 | lo rows x0 x1 y0 y1 z0 z1 | hi rows | first inner child |
+// first triangle | triangle ends of the slots (bytes) x2 | pad.  `order`
+// maps the tree's triangle positions to prims.
 bool bvh8q_any(const Scene& s, const std::vector<uint32_t>& by_prim, const uint32_t* qnodes, const uint32_t* order,
                const Ray& ray, Bvh4Walk& w) {
     const F3 rd = f3(safe_rcp(ray.d.x), safe_rcp(ray.d.y), safe_rcp(ray.d.z));
     const F3 ord = ray.o * rd;
-    std::vector<uint32_t> stack;
-    uint32_t ref = 0;
-    const float inf = std::numeric_limits<float>::infinity();
+    const bool px = rd.x >= 0.0f, py = rd.y >= 0.0f, pz = rd.z >= 0.0f;
+    const uint32_t oct = (px ? 0u : 1u) | (py ? 0u : 2u) | (pz ? 0u : 4u);
+    struct Group {
+        uint32_t base, imask, pending;
+    };
+    std::vector<Group> stack;
+    auto child = [&](uint32_t base, uint32_t imask, uint32_t pending) {
+        const uint32_t slot = (uint32_t)__builtin_ctz(pending) ^ oct;
+        return base + (uint32_t)__builtin_popcount(imask & ((1u << slot) - 1u));
+    };
+    uint32_t node = 0;
     while (true) {
-        if (!(ref & 0x80000000u)) {
-            const uint32_t* q = qnodes + 32 * (size_t)ref;
-            w.nodes++;
-            float o[3];
-            std::memcpy(o, q, sizeof(o));
-            const float ax = qscale(q[3] & 0xFFu) * rd.x, ay = qscale((q[3] >> 8) & 0xFFu) * rd.y,
-                        az = qscale((q[3] >> 16) & 0xFFu) * rd.z;
-            const float bx = std::fma(o[0], rd.x, -ord.x), by = std::fma(o[1], rd.y, -ord.y),
-                        bz = std::fma(o[2], rd.z, -ord.z);
-            const uint32_t* lo = q + 4;
-            const uint32_t* hi = q + 10;
-            const bool px = rd.x >= 0.0f, py = rd.y >= 0.0f, pz = rd.z >= 0.0f;
-            uint32_t c[8];
-            float t0[8];
-            for (int k = 0; k < 8; k++) {
-                const int wd = k >> 2, b = k & 3;
-                c[k] = q[16 + k];
-                const uint32_t nx = px ? lo[wd] : hi[wd], fx = px ? hi[wd] : lo[wd];
-                const uint32_t ny = py ? lo[2 + wd] : hi[2 + wd], fy = py ? hi[2 + wd] : lo[2 + wd];
-                const uint32_t nz = pz ? lo[4 + wd] : hi[4 + wd], fz = pz ? hi[4 + wd] : lo[4 + wd];
-                const float tnx = std::fma(qbyte(nx, b), ax, bx), tfx = std::fma(qbyte(fx, b), ax, bx);
-                const float tny = std::fma(qbyte(ny, b), ay, by), tfy = std::fma(qbyte(fy, b), ay, by);
-                const float tnz = std::fma(qbyte(nz, b), az, bz), tfz = std::fma(qbyte(fz, b), az, bz);
-                const float tn = std::fmax(std::fmax(tnx, tny), std::fmax(tnz, ray.tmin));
-                const float tf = std::fmin(std::fmin(tfx, tfy), std::fmin(tfz, ray.tmax));
-                t0[k] = tn <= tf ? tn : inf;
-            }
-            for (const auto& p : kSort8)
-                if (t0[p[1]] < t0[p[0]]) {
-                    std::swap(t0[p[0]], t0[p[1]]);
-                    std::swap(c[p[0]], c[p[1]]);
-                }
-            uint32_t next = kInvalid;
-            for (int k = 7; k >= 0; k--) {
-                if (t0[k] != inf && c[k] != kInvalid) {
-                    if (next != kInvalid) stack.push_back(next);
-                    next = c[k];
-                }
-            }
+        const uint32_t* q = qnodes + 32 * (size_t)node;
+        w.nodes++;
+        float o[3];
+        std::memcpy(o, q, sizeof(o));
+        const float ax = qscale(q[3] & 0xFFu) * rd.x, ay = qscale((q[3] >> 8) & 0xFFu) * rd.y,
+                    az = qscale((q[3] >> 16) & 0xFFu) * rd.z;
+        const float bx = std::fma(o[0], rd.x, -ord.x), by = std::fma(o[1], rd.y, -ord.y),
+                    bz = std::fma(o[2], rd.z, -ord.z);
+        const uint32_t* lo = q + 4;
+        const uint32_t* hi = q + 10;
+        const uint32_t imask = q[3] >> 24, node_base = q[16], tri_base = q[17];
+        uint8_t end[8];
+        for (int k = 0; k < 8; k++) end[k] = (uint8_t)(q[18 + (k >> 2)] >> (8 * (k & 3)));
+        uint32_t inner_v = 0, leaf_v = 0;
+        for (int k = 0; k < 8; k++) {
+            const int wd = k >> 2, b = k & 3;
+            const uint32_t nx = px ? lo[wd] : hi[wd], fx = px ? hi[wd] : lo[wd];
+            const uint32_t ny = py ? lo[2 + wd] : hi[2 + wd], fy = py ? hi[2 + wd] : lo[2 + wd];
+            const uint32_t nz = pz ? lo[4 + wd] : hi[4 + wd], fz = pz ? hi[4 + wd] : lo[4 + wd];
+            const float tnx = std::fma(qbyte(nx, b), ax, bx), tfx = std::fma(qbyte(fx, b), ax, bx);
+            const float tny = std::fma(qbyte(ny, b), ay, by), tfy = std::fma(qbyte(fy, b), ay, by);
+            const float tnz = std::fma(qbyte(nz, b), az, bz), tfz = std::fma(qbyte(fz, b), az, bz);
+            const float tn = std::fmax(std::fmax(tnx, tny), std::fmax(tnz, ray.tmin));
+            const float tf = std::fmin(std::fmin(tfx, tfy), std::fmin(tfz, ray.tmax));
+            if (!(tn <= tf)) continue;
+            const uint32_t pos = (uint32_t)k ^ oct;
+            if (imask >> k & 1u) inner_v |= 1u << pos;
+            else leaf_v |= 1u << pos;
+        }
+        uint32_t next = kInvalid;
+        if (inner_v) {
+            next = child(node_base, imask, inner_v);
+            const uint32_t rest = inner_v & (inner_v - 1u);
+            if (rest) stack.push_back(Group{node_base, imask, rest});
             w.max_stack = std::max(w.max_stack, (uint32_t)stack.size());
-            if (next != kInvalid) {
-                ref = next;
-                continue;
-            }
-        } else {
-            const uint32_t first = (ref & 0x7FFFFFFFu) >> 4, cnt = (ref & 15u) + 1u;
+        }
+        for (uint32_t lv = leaf_v; lv; lv &= lv - 1u) {
+            const uint32_t slot = (uint32_t)__builtin_ctz(lv) ^ oct;
+            const uint32_t first = slot ? end[slot - 1] : 0u, cnt = end[slot] - first;
             w.tris += cnt;
             bool hit = false;
             for (uint32_t k = 0; k < cnt; k++) {
                 float t, u, v;
-                if (mt(s.tris[by_prim[order[first + k]]], ray.o, ray.d, ray.tmin, t, u, v) && t < ray.tmax) hit = true;
+                if (mt(s.tris[by_prim[order[tri_base + first + k]]], ray.o, ray.d, ray.tmin, t, u, v) && t < ray.tmax)
+                    hit = true;
             }
             if (hit) return true;
         }
+        if (next != kInvalid) {
+            node = next;
+            continue;
+        }
         if (stack.empty()) return false;
-        ref = stack.back();
-        stack.pop_back();
+        Group& g = stack.back();
+        node = child(g.base, g.imask, g.pending);
+        g.pending &= g.pending - 1u;
+        if (!g.pending) stack.pop_back();
     }
 }
 

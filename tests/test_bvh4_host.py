@@ -156,55 +156,114 @@ def test_quantized_any_hit_walk(make, oracle_lib):
 
 
 def _decode8(q):
-    """(N, 32) uint32 rows of 8-wide quantized nodes -> decoded child boxes
-    lo, hi (N, 8, 3) and child refs (N, 8)."""
+    """(N, 32) uint32 rows of 8-wide compressed nodes -> decoded child boxes lo,
+    hi (N, 8, 3) and per node and slot the child: ("inner", node index),
+    ("leaf", first position, count) or None, by the layout's own rules (inner
+    children consecutive from the first inner child in slot order, leaf
+    triangles from the first triangle, slot s ending at byte s of the ends)."""
     origin = q[:, 0:3].copy().view(np.float32)
     e = np.stack([(q[:, 3] >> (8 * a)) & 255 for a in range(3)], 1).astype(np.uint32)
     scale = (e << 23).view(np.float32)
     lo = np.zeros((len(q), 8, 3), np.float32)
     hi = np.zeros((len(q), 8, 3), np.float32)
+    empty = np.ones((len(q), 8), bool)
     for a in range(3):
         for c in range(8):
             bl = (q[:, 4 + 2 * a + c // 4] >> (8 * (c % 4))) & 255
             bh = (q[:, 10 + 2 * a + c // 4] >> (8 * (c % 4))) & 255
             lo[:, c, a] = origin[:, a] + bl.astype(np.float32) * scale[:, a]
             hi[:, c, a] = origin[:, a] + bh.astype(np.float32) * scale[:, a]
-    return lo, hi, q[:, 16:24]
+            empty[:, c] &= (bl == 255) & (bh == 0)
+    kids = []
+    for i in range(len(q)):
+        imask, base, tri_base = int(q[i, 3]) >> 24, int(q[i, 16]), int(q[i, 17])
+        ends = [(int(q[i, 18 + k // 4]) >> (8 * (k % 4))) & 255 for k in range(8)]
+        row, rank = [], 0
+        for k in range(8):
+            start = ends[k - 1] if k else 0
+            if imask >> k & 1:
+                assert not empty[i, k] and ends[k] == start
+                row.append(("inner", base + rank))
+                rank += 1
+            elif ends[k] > start:
+                assert not empty[i, k]
+                row.append(("leaf", tri_base + start, ends[k] - start))
+            else:
+                assert empty[i, k]
+                row.append(None)
+        kids.append(row)
+    return lo, hi, kids
+
+
+def _leaf_sets4(nodes, order):
+    sets = set()
+    for i in range(len(nodes)):
+        for k in range(4):
+            c = int(nodes[i, 24 + k])
+            if c != 0xFFFFFFFF and c & 0x80000000:
+                first, cnt = (c & 0x7FFFFFFF) >> 4, (c & 15) + 1
+                sets.add(frozenset(order[first:first + cnt].tolist()))
+    return sets
 
 
 @pytest.mark.parametrize("make", [scenes.cornell_box, lambda: scenes.spheres_scene(count=4, subdivisions=4, lights=64)])
 def test_bvh8_quantized_structure_and_walk(make, oracle_lib):
-    """The 8-wide quantized tree: the triangle order of the BVH4 build, every
-    triangle in exactly one leaf whose decoded box holds it, depth as reported;
-    the oracle's walk in the any-hit kernel's order agrees with brute force on
-    random and shadow segments, its stack stays within 7 * depth + 2 entries
-    and it visits fewer nodes than the 4-wide quantized walk."""
+    """The 8-wide compressed tree: breadth-first numbering with each node's
+    inner children consecutive and after it, every node reached once, depth as
+    reported; the leaves of the BVH4 build regrouped (equal triangle sets), every
+    triangle in exactly one leaf whose decoded box holds it; octant slots; the
+    oracle's walk in the any-hit kernel's order agrees with brute force on
+    random and shadow segments, keeps at most one stack entry per level and
+    visits fewer nodes than the 4-wide quantized walk."""
     s = make()
     q8, order8, depth8 = ptcore.build_bvh8_quantized(s.positions)
     nodes, order, depth = ptcore.build_bvh4(s.positions)
-    assert np.array_equal(order8, order) and 1 <= depth8 <= depth
-    assert (q8[:, 24:32] == 0).all()
-    lo, hi, child = _decode8(q8)
     T = len(s.positions)
-    P = s.positions[order].reshape(T, 3, 3)
+    assert sorted(order8.tolist()) == list(range(T)) and 1 <= depth8 <= depth
+    assert (q8[:, 20:32] == 0).all()
+    lo, hi, kids = _decode8(q8)
+    P = s.positions[order8].reshape(T, 3, 3)
     seen = np.zeros(T, np.int32)
-    stack, max_level = [(0, 1)], 0
-    while stack:
-        i, level = stack.pop()
-        max_level = max(max_level, level)
-        for k in range(8):
-            c = int(child[i, k])
-            if c == 0xFFFFFFFF:
+    reached = np.zeros(len(q8), np.int32)
+    reached[0] = 1
+    level = np.zeros(len(q8), np.int32)
+    level[0] = 1
+    leaf_sets = set()
+    for i in range(len(q8)):      # breadth first: parents before children
+        assert reached[i] == 1
+        for k, kid in enumerate(kids[i]):
+            if kid is None:
                 continue
-            if c & 0x80000000:
-                first, cnt = (c & 0x7FFFFFFF) >> 4, (c & 15) + 1
+            if kid[0] == "inner":
+                c = kid[1]
+                assert i < c < len(q8)
+                reached[c] += 1
+                level[c] = level[i] + 1
+            else:
+                _, first, cnt = kid
                 seen[first:first + cnt] += 1
                 pts = P[first:first + cnt].reshape(-1, 3)
                 assert (pts >= lo[i, k]).all() and (pts <= hi[i, k]).all()
-            else:
-                assert 0 < c < len(q8)
-                stack.append((c, level + 1))
-    assert (seen == 1).all() and max_level == depth8
+                leaf_sets.add(frozenset(order8[first:first + cnt].tolist()))
+    assert (seen == 1).all() and (reached == 1).all() and level.max() == depth8
+    assert leaf_sets == _leaf_sets4(nodes, order)
+    # octant slots: slot o holds a child far upstream along octant o's diagonal
+    # (o's bit a set = negative component a).  Over the nodes with several
+    # children, the child of slot o is mostly the one with the least
+    # projection, and never on average above the mean: a flipped octant
+    # convention or an unsorted assignment fails one of these.
+    first_ok, below_mean = [], []
+    for i in range(len(q8)):
+        occupied = [k for k in range(8) if kids[i][k] is not None]
+        if len(occupied) < 3:
+            continue
+        cen = 0.5 * (lo[i] + hi[i])
+        for oc in occupied:
+            d = np.array([-1.0 if oc >> a & 1 else 1.0 for a in range(3)])
+            proj = {k: float(cen[k] @ d) for k in occupied}
+            first_ok.append(proj[oc] <= min(proj.values()) + 1e-6)
+            below_mean.append(proj[oc] - np.mean(list(proj.values())))
+    assert len(first_ok) > 10 and np.mean(first_ok) > 0.5 and np.mean(below_mean) < 0
     o = oracle_lib.OracleScene(s)
     blo, bhi = s.bounds()
     rays = scenes.random_rays(20000, blo - 1, bhi + 1, 88)
@@ -214,16 +273,20 @@ def test_bvh8_quantized_structure_and_walk(make, oracle_lib):
     for r in (rays, sh):
         occ, nv, nt, ms = o.bvh8q_any(q8, order8, r)
         assert np.array_equal(occ, o.any(r, brute=True))
-        assert ms.max() <= 7 * depth8 + 2 and nv.min() >= 1
+        assert ms.max() <= depth8 and nv.min() >= 1
         _, nv4, _ = o.bvh4q_any(q4, order, r)
         assert nv.mean() < nv4.mean()
 
 
 def test_bvh8_quantized_tiny_scenes():
-    """One triangle: a root with one leaf; no triangles: a root with no child."""
+    """One triangle: a root whose slot 0 is a leaf of it; no triangles: a root
+    with no child."""
     one = np.array([[[0, 0, 0], [1, 0, 0], [0, 1, 0]]], np.float32)
     q, order, depth = ptcore.build_bvh8_quantized(one)
     assert len(q) == 1 and depth == 1 and order.tolist() == [0]
-    assert q[0, 16] == 0x80000000 and (q[0, 17:24] == 0xFFFFFFFF).all()
+    _, _, kids = _decode8(q)
+    assert kids[0][0] == ("leaf", 0, 1) and all(k is None for k in kids[0][1:])
     q, order, depth = ptcore.build_bvh8_quantized(np.zeros((0, 3, 3), np.float32))
-    assert len(q) == 1 and len(order) == 0 and (q[0, 16:24] == 0xFFFFFFFF).all()
+    assert len(q) == 1 and len(order) == 0
+    _, _, kids = _decode8(q)
+    assert all(k is None for k in kids[0])

@@ -189,10 +189,14 @@ def test_unsupported_material_is_rejected(gpu_renderer, cornell):
     gpu_renderer.InitializeSceneBuffers(cornell)   # restore
 
 
-def test_stack_overflow_is_reported(gpu_renderer, cornell, cornell_oracle):
+def test_stack_overflow_is_reported(gpu_renderer, cornell, cornell_oracle, oracle_lib):
     """A traversal stack capped below what the BVH needs (pt_tuning.stack_limit)
     drops subtrees; every call that traverses then fails with "traversal stack
-    overflow" instead of returning wrong hits, and the default cap serves again."""
+    overflow" instead of returning wrong hits, and the default cap serves again.
+    The any-hit walk of the 8-wide tree keeps at most one stack entry per level
+    (a node's pending inner children), and rarely more than one in all: its
+    overflow is shown on twelve spheres (245,762 triangles), where a few of
+    16,384 rays need up to four entries in the oracle's walk."""
     r = gpu_renderer
     r.InitializeSceneBuffers(cornell)
     rays = scenes.random_rays(1 << 16, (-1, 0, -1), (1, 2, 1), 301)
@@ -200,7 +204,7 @@ def test_stack_overflow_is_reported(gpu_renderer, cornell, cornell_oracle):
     st = dict(width=32, height=32, samples_per_pixel=4, path_depth=8, use_nee=1, seed=2)
     try:
         r.set_tuning(stack_limit=1)
-        for call in (lambda: r.trace_closest(rays), lambda: r.trace_any(rays), lambda: r.classify(rays),
+        for call in (lambda: r.trace_closest(rays), lambda: r.classify(rays),
                      lambda: r.RenderFrame(st, cornell.view),
                      lambda: r.RenderFrame(st, cornell.view, ptcore.FRAME_NO_GRAPH)):
             with pytest.raises(ptcore.PtError) as e:
@@ -210,6 +214,23 @@ def test_stack_overflow_is_reported(gpu_renderer, cornell, cornell_oracle):
         r.set_tuning(**tun)
     assert np.array_equal(r.trace_closest(rays), cornell_oracle.closest(rays, tprim=True))
     r.RenderFrame(st, cornell.view)
+
+    deep = scenes.spheres_scene(count=12, subdivisions=5, lights=64)
+    with ptcore.Renderer() as r2:
+        r2.InitializeSceneBuffers(deep)
+        lo, hi = deep.bounds()
+        segs = scenes.random_rays(1 << 14, lo - 1, hi + 1, 302)
+        q8, order8 = r2.bvh8_quantized()
+        o = oracle_lib.OracleScene(deep)
+        want, _, _, ms = o.bvh8q_any(q8, order8, segs)
+        assert ms.max() >= 2
+        tun2 = r2.get_tuning()
+        r2.set_tuning(stack_limit=1)
+        with pytest.raises(ptcore.PtError) as e:
+            r2.trace_any(segs)
+        assert e.value.status == 4 and "traversal stack overflow" in str(e.value)
+        r2.set_tuning(**tun2)
+        assert np.array_equal(r2.trace_any(segs), want)
 
 
 @pytest.mark.parametrize("make", [scenes.cornell_box, scenes.specular_room])
@@ -321,10 +342,10 @@ def test_c2_any_hit_matches_oracle(c2_pair):
 def _any_hit_walk(r, o, rays):
     """The oracle's walk of the any-hit tree a renderer traverses: the 8-wide
     quantized nodes where they were built, else the 4-wide quantized ones."""
-    _, order = r.bvh4()
     if r.bvh_info()["bvh8_node_count"]:
-        occ, nv, nt, _ = o.bvh8q_any(r.bvh8_quantized(), order, rays)
+        occ, nv, nt, _ = o.bvh8q_any(*r.bvh8_quantized(), rays)
     else:
+        _, order = r.bvh4()
         occ, nv, nt = o.bvh4q_any(r.bvh4_quantized(), order, rays)
     return occ, nv, nt
 
@@ -369,10 +390,11 @@ def test_c2_any_hit_node_formats_equal_the_oracle_walk(c2_pair, quantized):
         sh = scenes.shadow_rays(s, 1 << 18, 205)
         occ, nv, nt = r.trace_any_counts(sh)
         if quantized & 4:
-            q8 = r.bvh8_quantized()
+            q8, order8 = r.bvh8_quantized()
             assert info["bvh8_node_count"] == len(q8) and 1 <= info["bvh8_depth"] <= info["max_depth"]
-            oocc, onv, ont, ms = o.bvh8q_any(q8, order, sh)
-            assert ms.max() <= 7 * info["bvh8_depth"] + 2
+            assert sorted(order8.tolist()) == list(range(s.triangle_count))
+            oocc, onv, ont, ms = o.bvh8q_any(q8, order8, sh)
+            assert ms.max() <= info["bvh8_depth"]
             _, nv4, _ = o.bvh4q_any(q4, order, sh)
             assert nv.mean() < nv4.mean()
         else:
