@@ -224,9 +224,11 @@ uint32_t emit(const TNode* n, Bvh4Build& out, uint32_t level) {
 // [axis][child]), in the manner of the compressed wide BVH of Ylitie, Karras
 // and Laine (HPG 2017): slot s belongs to the ray octant s (bit a set = the
 // direction's component a is negative), whose first child should be the one
-// farthest upstream along the octant's diagonal d_s, i.e. with the smallest
-// (centre - mean centre) . d_s.  Pairs are taken greedily, cheapest first
-// (ties: lowest child, then lowest slot).
+// farthest upstream along the octant's diagonal d_s.  The cost of child i in
+// slot s is (centre_i - mean centre) . d_s, and the slots are the assignment
+// of least total cost, found by the Hungarian method (Kuhn 1955, in the
+// potentials form of Jonker and Volgenant) over the 8 x 8 cost matrix whose
+// rows past `count` are zero (empty slots).
 void octant_slots(const float lo[3][8], const float hi[3][8], int count, int slot_of[8]) {
     float cen[8][3];
     float mid[3] = {0.0f, 0.0f, 0.0f};
@@ -236,27 +238,58 @@ void octant_slots(const float lo[3][8], const float hi[3][8], int count, int slo
             mid[a] += cen[i][a];
         }
     for (int a = 0; a < 3; a++) mid[a] /= (float)std::max(count, 1);
-    bool slot_used[8] = {};
-    for (int i = 0; i < 8; i++) slot_of[i] = -1;
-    for (int round = 0; round < count; round++) {
-        float best = kInf;
-        int bi = -1, bs = -1;
-        for (int i = 0; i < count; i++) {
-            if (slot_of[i] >= 0) continue;
-            for (int s = 0; s < 8; s++) {
-                if (slot_used[s]) continue;
-                float cost = 0.0f;
-                for (int a = 0; a < 3; a++) cost += (s >> a & 1) ? mid[a] - cen[i][a] : cen[i][a] - mid[a];
-                if (bi < 0 || cost < best) {
-                    best = cost;
-                    bi = i;
-                    bs = s;
+    constexpr int n = 8;
+    double cost[n + 1][n + 1] = {};   // 1-based: cost[child][slot]
+    for (int i = 0; i < count; i++)
+        for (int s = 0; s < n; s++) {
+            double c = 0.0;
+            for (int a = 0; a < 3; a++) c += (s >> a & 1) ? mid[a] - cen[i][a] : cen[i][a] - mid[a];
+            cost[i + 1][s + 1] = c;
+        }
+    double u[n + 1] = {}, v[n + 1] = {};
+    int row_of[n + 1] = {}, way[n + 1] = {};   // row_of[slot]: child assigned to it
+    for (int i = 1; i <= n; i++) {
+        row_of[0] = i;
+        int j0 = 0;
+        double minv[n + 1];
+        bool used[n + 1] = {};
+        for (int j = 0; j <= n; j++) minv[j] = std::numeric_limits<double>::infinity();
+        do {
+            used[j0] = true;
+            const int i0 = row_of[j0];
+            double delta = std::numeric_limits<double>::infinity();
+            int j1 = 0;
+            for (int j = 1; j <= n; j++) {
+                if (used[j]) continue;
+                const double cur = cost[i0][j] - u[i0] - v[j];
+                if (cur < minv[j]) {
+                    minv[j] = cur;
+                    way[j] = j0;
+                }
+                if (minv[j] < delta) {
+                    delta = minv[j];
+                    j1 = j;
                 }
             }
-        }
-        slot_of[bi] = bs;
-        slot_used[bs] = true;
+            for (int j = 0; j <= n; j++) {
+                if (used[j]) {
+                    u[row_of[j]] += delta;
+                    v[j] -= delta;
+                } else {
+                    minv[j] -= delta;
+                }
+            }
+            j0 = j1;
+        } while (row_of[j0] != 0);
+        do {
+            const int j1 = way[j0];
+            row_of[j0] = row_of[j1];
+            j0 = j1;
+        } while (j0 != 0);
     }
+    for (int i = 0; i < n; i++) slot_of[i] = -1;
+    for (int j = 1; j <= n; j++)
+        if (row_of[j] >= 1 && row_of[j] <= count) slot_of[row_of[j] - 1] = j - 1;
 }
 
 // The 8-wide quantized any-hit tree (Bvh8QNode) of the binary tree under
