@@ -807,7 +807,8 @@ def main():
     sh_achieved = sh_bytes_per_launch / (sh_avg_ms * 1e-3) / 1e9
 
     def roof(kind, kernel, achieved_gbs, bpr, nodes, tris, avg_ms, rays_per_launch, node_bytes):
-        traffic = measured_traffic(kind, node_bytes)
+        # the committed counter runs are of the production frame: no traffic figure for other scenes
+        traffic = measured_traffic(kind, node_bytes) if a.config == "production" else None
         # the same launch's measured HBM bytes (rocprofv3 counters, per launch)
         # over its live duration: what HBM actually carried
         hbm = None
