@@ -290,3 +290,37 @@ def test_bvh8_quantized_tiny_scenes():
     assert len(q) == 1 and len(order) == 0
     _, _, kids = _decode8(q)
     assert all(k is None for k in kids[0])
+
+def _clusters(n_clusters=40, copies=16, seed=5):
+    """Clusters of near-identical triangles, which the SAH keeps as leaves of
+    `copies` triangles (a split cannot shrink their boxes)."""
+    rng = np.random.default_rng(seed)
+    base = rng.uniform(0, 10, (n_clusters, 1, 3, 3)).astype(np.float32)
+    P = np.repeat(base, copies, axis=1).reshape(-1, 3, 3)
+    P = (P + rng.uniform(-1e-3, 1e-3, P.shape)).astype(np.float32)
+    T = len(P)
+    tan = np.zeros((T, 3, 4), np.float32)
+    tan[..., 0] = tan[..., 3] = 1.0
+    nrm = np.zeros((T, 3, 3), np.float32)
+    nrm[..., 2] = 1.0
+    return scenes.Scene(name="clusters", positions=P, normals=nrm, tangents=tan,
+                        material_ids=np.zeros(T, np.uint32), materials=np.stack([scenes.material(1)]))
+
+
+def test_bvh8_sixteen_triangle_leaves(oracle_lib):
+    """Leaves of the largest size (16 triangles, so a node's triangle ends reach
+    128): the decoded ranges cover every triangle once and the walk equals brute
+    force."""
+    s = _clusters()
+    q8, order8, depth8 = ptcore.build_bvh8_quantized(s.positions, 16)
+    _, _, kids = _decode8(q8)
+    counts = [k[2] for row in kids for k in row if k and k[0] == "leaf"]
+    assert max(counts) == 16 and sum(counts) == s.triangle_count
+    ends = [(int(q8[i, 18 + k // 4]) >> (8 * (k % 4))) & 255 for i in range(len(q8)) for k in range(8)]
+    assert max(ends) > 64
+    o = oracle_lib.OracleScene(s)
+    lo, hi = s.bounds()
+    rays = scenes.random_rays(20000, lo - 1, hi + 1, 90)
+    rays[:, 7] = np.random.default_rng(6).uniform(0.01, 20.0, len(rays)).astype(np.float32)
+    occ, _, nt, _ = o.bvh8q_any(q8, order8, rays)
+    assert np.array_equal(occ, o.any(rays, brute=True)) and 0.01 < occ.mean() < 0.99 and nt.max() >= 16

@@ -169,3 +169,24 @@ def test_view_that_misses_everything(oracle_lib):
         assert (ext, sh) == (24 * 16 * 4, 0)
     finally:
         r.close()
+
+
+def test_sixteen_triangle_leaves_any_hit(oracle_lib):
+    """Leaves of the largest size (pt_tuning.bvh_max_leaf = 16) in the 8-wide
+    any-hit tree: occlusion equal to brute force and per-ray node and triangle
+    visits equal to the oracle's walk of the uploaded tree."""
+    from test_bvh4_host import _clusters
+    s = _clusters()
+    with ptcore.Renderer() as r:
+        r.set_tuning(bvh_max_leaf=16)
+        r.InitializeSceneBuffers(s)
+        q8, order8 = r.bvh8_quantized()
+        o = oracle_lib.OracleScene(s)
+        lo, hi = s.bounds()
+        rays = scenes.random_rays(1 << 16, lo - 1, hi + 1, 91)
+        rays[:, 7] = np.random.default_rng(7).uniform(0.01, 20.0, len(rays)).astype(np.float32)
+        occ, nv, nt = r.trace_any_counts(rays)
+        oocc, onv, ont, _ = o.bvh8q_any(q8, order8, rays)
+        assert np.array_equal(occ, o.any(rays, brute=True))
+        assert np.array_equal(occ, oocc) and np.array_equal(nv, onv) and np.array_equal(nt, ont)
+        assert nt.max() >= 16
