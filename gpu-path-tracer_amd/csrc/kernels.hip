@@ -1312,8 +1312,11 @@ __global__ void k_prepare(PathState p, int cur) {
 // owns one contiguous range: a counting pass, one atomicAdd per queue per
 // block, then a scatter pass in which every wave writes at its own running
 // offsets.  Five global atomics per block in place of one per finishing ray.
+// Both passes take kBinUnroll rows of the block per step, their queue ids
+// loaded together (independent loads in flight), then handled in index order.
 constexpr int kBinBlock = 256;
 constexpr int kBinWaves = kBinBlock / 64;
+constexpr int kBinUnroll = 4;
 
 template <class SlotOf>
 __device__ __forceinline__ void bin_entries(const uint8_t* qid, uint32_t n, const SlotOf& slot_of,
@@ -1324,11 +1327,17 @@ __device__ __forceinline__ void bin_entries(const uint8_t* qid, uint32_t n, cons
     const uint32_t b1 = min(b0 + per, n);
     const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
     uint32_t cnt[kQueueSlots] = {0, 0, 0, 0, 0};
-    for (uint32_t base = b0; base < b1; base += kBinBlock) {
-        const uint32_t i = base + threadIdx.x;
-        const int q = i < b1 ? (int)qid[i] : -1;
+    for (uint32_t base = b0; base < b1; base += kBinUnroll * kBinBlock) {
+        int qs[kBinUnroll];
 #pragma unroll
-        for (int k = 0; k < kQueueSlots; k++) cnt[k] += (uint32_t)__popcll(__ballot(q == k));
+        for (int u = 0; u < kBinUnroll; u++) {
+            const uint32_t i = base + (uint32_t)u * kBinBlock + threadIdx.x;
+            qs[u] = i < b1 ? (int)qid[i] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kBinUnroll; u++)
+#pragma unroll
+            for (int k = 0; k < kQueueSlots; k++) cnt[k] += (uint32_t)__popcll(__ballot(qs[u] == k));
     }
     if (lane == 0) {
 #pragma unroll
@@ -1350,15 +1359,24 @@ __device__ __forceinline__ void bin_entries(const uint8_t* qid, uint32_t n, cons
     uint32_t off[kQueueSlots];
 #pragma unroll
     for (int k = 0; k < kQueueSlots; k++) off[k] = s_cnt[k][wave];
-    for (uint32_t base = b0; base < b1; base += kBinBlock) {
-        const uint32_t i = base + threadIdx.x;
-        const int q = i < b1 ? (int)qid[i] : -1;
-        const uint32_t slot = q >= 0 ? slot_of(i) : 0u;
+    for (uint32_t base = b0; base < b1; base += kBinUnroll * kBinBlock) {
+        int qs[kBinUnroll];
 #pragma unroll
-        for (int k = 0; k < kQueueSlots; k++) {
-            const uint64_t m = __ballot(q == k);
-            if (q == k && queues[k]) queues[k][off[k] + mbcnt(m)] = slot;
-            off[k] += (uint32_t)__popcll(m);
+        for (int u = 0; u < kBinUnroll; u++) {
+            const uint32_t i = base + (uint32_t)u * kBinBlock + threadIdx.x;
+            qs[u] = i < b1 ? (int)qid[i] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kBinUnroll; u++) {
+            const uint32_t i = base + (uint32_t)u * kBinBlock + threadIdx.x;
+            const int q = qs[u];
+            const uint32_t slot = q >= 0 ? slot_of(i) : 0u;
+#pragma unroll
+            for (int k = 0; k < kQueueSlots; k++) {
+                const uint64_t m = __ballot(q == k);
+                if (q == k && queues[k]) queues[k][off[k] + mbcnt(m)] = slot;
+                off[k] += (uint32_t)__popcll(m);
+            }
         }
     }
 }
