@@ -213,12 +213,26 @@ def _pmc_table(d, min_us=20.0):
     return out
 
 
+# The counter runs behind profiles/pmc_production_1080p64spp.md (one group per
+# run: rocprofv3 does not split counters over passes), the default command
+# block of that summary.
+PMC_COMMANDS = """timeout -s KILL 240 rocprofv3 --pmc <group> -f csv -d gpurun_out/prof/pmc_<name> -o run -- python3 bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-serving
+#   fetch: FETCH_SIZE                 write: WRITE_SIZE
+#   tcc:   TCC_HIT_sum TCC_MISS_sum TCC_EA0_RDREQ_sum TCC_EA0_RDREQ_DRAM_sum
+#   sq1:   SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VMEM_RD SQ_INSTS_SALU SQ_INSTS_LDS SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_ACTIVE_INST_VALU
+#   sq2:   SQ_THREAD_CYCLES_VALU SQ_ACTIVE_INST_VALU SQ_INSTS_VALU GRBM_GUI_ACTIVE
+timeout -s KILL 240 rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/prof/pmc_cal_fetch -o run -- python3 bench.py --calibrate-copy
+timeout -s KILL 240 rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/prof/pmc_cal_write -o run -- python3 bench.py --calibrate-copy
+python3 bench.py --summarize-pmc gpurun_out/prof/pmc_cal_fetch gpurun_out/prof/pmc_cal_write gpurun_out/prof/pmc_{fetch,write,tcc,sq1,sq2} --bench-json gpurun_out/prof/bench.log --out profiles/pmc_production_1080p64spp.md"""
+
+
 def summarize_pmc(cal_fetch, cal_write, dirs, bench_json, command):
     """The PMC summary of a workload: per kernel HBM bytes (FETCH_SIZE and
     WRITE_SIZE scaled by the same counters of a 1 GiB device-to-device copy),
     L2 hit rate and VALU use, from one rocprofv3 --pmc run per counter group;
     returns (markdown, {"any": traffic, "closest": traffic}) where traffic is
     the per-launch record bench.py reads into roofline.traffic."""
+    command = command or PMC_COMMANDS
     copy_kib = (1 << 30) / 1024.0
 
     def cal(d, counter):
