@@ -25,6 +25,10 @@ BF16_MFMA_PEAK_TFLOPS = 2500.0     # dense bf16 MFMA of one MI355X
 # the forward pass (4 products), the weight gradients (4) and the input deltas
 # of layers 3..1 (3), 2 * 64 * 64 flops each
 GUIDE_TRAIN_FLOPS_PER_RECORD = 11 * 2 * 64 * 64
+# flops of the inference fused into the guided shade per diffuse/GGX entry:
+# three 64 x 64 layers and the 48 output rows it evaluates (8 lobes x 5 outputs
+# in three 16-row tiles)
+GUIDE_INFER_FLOPS_PER_ENTRY = 2 * 64 * (3 * 64 + 48)
 GUIDE_PARAMS = 16640               # fp32 partial gradient of one 64-record block
 # algorithmic bytes of one closest-hit ray: ray record (origin|tMin, direction|tMax)
 # 32 B in, hit record + queue id 17 B out, 128 B per BVH4 node, 48 B per triangle
@@ -649,8 +653,23 @@ def run_guide(a):
                       "avg_launch_us": round(us, 2), "hbm_gbs": round(hbm, 1)}
         log(f"guide: k_guide_train {us:.1f} us per {batch}-record launch = {tflops:.1f} TFLOP/s bf16 "
             f"({100 * tflops / BF16_MFMA_PEAK_TFLOPS:.2f} % of peak), {hbm:.0f} GB/s of partial gradients and records")
+        # the inference fused into the guided shade: its MLP flops over all the
+        # time guiding adds to the frame (features, inference, guided sampling
+        # and density, records), so a lower bound of the inference's MFMA rate
+        t_guide = sg["render_ms"] - gs["train_ms"] - sp["render_ms"]
+        infer_flops = GUIDE_INFER_FLOPS_PER_ENTRY * gs["shaded_entries"]
+        infer_tflops = infer_flops / (t_guide * 1e-3) / 1e12 if t_guide > 0 else 0.0
+        infer_roof = {"bound": "mfma", "kernel": "k_shade<false, true> (fused MLP inference)",
+                      "achieved": round(infer_tflops, 1), "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                      "frac": round(infer_tflops / BF16_MFMA_PEAK_TFLOPS, 4),
+                      "flops_per_entry": GUIDE_INFER_FLOPS_PER_ENTRY, "entries": gs["shaded_entries"],
+                      "time_ms": round(t_guide, 2),
+                      "time_basis": "guided frame - training - plain frame: all the time guiding adds"}
+        log(f"guide: fused inference {gs['shaded_entries']} entries x {GUIDE_INFER_FLOPS_PER_ENTRY} flops in the "
+            f"{t_guide:.1f} ms guiding adds = {infer_tflops:.0f} TFLOP/s bf16 "
+            f"({100 * infer_tflops / BF16_MFMA_PEAK_TFLOPS:.1f} % of peak, a lower bound)")
         out_cfgs.append({"use_nee": nee, "spp": spp, "plain_ms": round(sp["render_ms"], 2), "plain_nmse": e_p,
-                         "roofline_train": train_roof,
+                         "roofline_train": train_roof, "roofline_infer": infer_roof,
                          "guided_ms": round(sg["render_ms"], 2), "guided_train_ms": round(gs["train_ms"], 2),
                          "guided_nmse": e_g, "error_ratio_equal_samples": round(e_g / e_p, 4),
                          "time_to_equal_error_ratio": round(ratio, 4),

@@ -402,6 +402,7 @@ pt_status Renderer::RenderGuided(const pt_render_settings& s, const pt_camera& c
             total.samples += ps.samples;
             total.iterations += ps.iterations;
             gst.render_ms += ps.render_ms;
+            if (opt.guide_fraction > 0.0f) gst.shaded_entries += h_ctr_ring_[0].dg_entries;
         }
         if (!learn || pass + 1 == gs.passes || gs.train_steps == 0) continue;
 

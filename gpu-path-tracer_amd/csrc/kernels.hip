@@ -1200,6 +1200,7 @@ __global__ void k_frame_begin(PathState p) {
         c->shadow_rays = 0;
         c->closest_nodes = c->closest_tris = 0;
         c->shadow_nodes = c->shadow_tris = 0;
+        c->dg_entries = 0;
     }
 }
 
@@ -1295,6 +1296,7 @@ __global__ void k_prepare(PathState p, int cur) {
         c->ext_rays += traced;
         c->shadow_rays += c->shadow[cur];
         if (traced) c->iterations++;
+        c->dg_entries += (unsigned long long)c->queue[0] + c->queue[1];
         c->ray[nxt] += nr;
         c->launched += nr;
         c->ray[cur] = 0;

@@ -430,6 +430,7 @@ struct Counters {
     unsigned long long closest_tris;
     unsigned long long shadow_nodes;
     unsigned long long shadow_tris;
+    unsigned long long dg_entries;   // diffuse and GGX queue entries shaded by this run
 };
 
 }  // namespace pt

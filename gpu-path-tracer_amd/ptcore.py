@@ -81,7 +81,7 @@ class GuideSettings(C.Structure):
 class GuideStats(C.Structure):
     _fields_ = [("render_ms", C.c_double), ("train_ms", C.c_double), ("records", C.c_uint64),
                 ("loss_first", C.c_float), ("loss_last", C.c_float), ("steps", C.c_uint32), ("passes", C.c_uint32),
-                ("train_kernel_us", C.c_double)]
+                ("train_kernel_us", C.c_double), ("shaded_entries", C.c_uint64)]
 
 
 GUIDE_PARAMS = 16640

@@ -434,6 +434,8 @@ typedef struct pt_guide_stats {
     uint32_t passes;
     double train_kernel_us;   /* average duration of one training forward+backward launch (the first 8 of each
                                  pass, each between its own events; the Adam step not included) */
+    uint64_t shaded_entries;  /* diffuse and GGX entries shaded in the guided passes: the entries the guide
+                                 network is evaluated for (a wave with any of them infers all 64 lanes) */
 } pt_guide_stats;
 
 /* Renders one frame in `passes` passes of growing size; the network is trained

@@ -153,13 +153,17 @@ def _render_set(r, room, w, h, nee):
     r.RenderGuided(dict(base, samples_per_pixel=16384, seed=99), room.view, passes=4, guide_fraction=0.0)
     ref = r.GetFrameHdr(w, h)[..., :3].copy()
     st = dict(base, samples_per_pixel=256, seed=7)
-    sp, _ = r.RenderGuided(st, room.view, passes=4, guide_fraction=0.0)
+    sp, gp = r.RenderGuided(st, room.view, passes=4, guide_fraction=0.0)
     plain = r.GetFrameHdr(w, h)[..., :3].copy()
+    assert gp["shaded_entries"] == 0        # no guided pass: nothing inferred
     sg, gs = r.RenderGuided(st, room.view, passes=4, train_steps=512, batch=16384, learning_rate=3e-3,
                             guide_fraction=0.5, reset=True, seed=1)
     online = r.GetFrameHdr(w, h)[..., :3].copy()
     assert sp["samples"] == sg["samples"] == w * h * 256
     assert gs["steps"] == 3 * 512 and gs["records"] > 0 and math.isfinite(gs["loss_last"])
+    # the guided passes (all but the untrained first) shade diffuse or GGX entries,
+    # at most one per extension ray traced
+    assert 0 < gs["shaded_entries"] <= sg["extension_rays"]
     r.RenderGuided(st, room.view, passes=1, train_steps=0, guide_fraction=0.8, reset=False)
     trained = r.GetFrameHdr(w, h)[..., :3].copy()
     for img in (online, trained):
