@@ -711,9 +711,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--trace-block", type=int, default=0)
     ap.add_argument("--pool", type=int, default=0)
-    ap.add_argument("--quantized", type=int, choices=range(8), default=None,
+    ap.add_argument("--quantized", type=int, choices=range(16), default=None,
                     help="pt_tuning.bvh_quantized: bit 0 closest hit, bit 1 any hit on 64-byte quantized nodes, "
-                         "bit 2 any hit on 128-byte 8-wide quantized nodes (default: the library's)")
+                         "bit 2 any hit on the compressed 8-wide nodes, bit 3 (with bit 2) closest hit on them "
+                         "too (default: the library's)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scale reference for FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--summarize-kernel-trace", metavar="DIR", default=None,
@@ -788,7 +789,8 @@ def main():
     t0 = time.time()
     r.InitializeSceneBuffers(scene)
     bvh = r.bvh_info()
-    node_b = QNODE_B if quant & 1 else NODE_B           # closest-hit node format
+    # closest-hit node format: the 8-wide tree with bit 3, else quantized or fp32 BVH4
+    node_b = Q8NODE_B if (quant & 8 and bvh["bvh8_node_count"]) else QNODE_B if quant & 1 else NODE_B
     # any-hit node format: the 8-wide tree where it was built (4-wide host build)
     node_b_any = Q8NODE_B if bvh["bvh8_node_count"] else QNODE_B if quant & 2 else NODE_B
     log(f"upload + BVH4 build {time.time() - t0:.1f}s: {bvh}")

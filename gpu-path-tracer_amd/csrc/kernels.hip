@@ -590,7 +590,7 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
     if (COUNT) src.flush_counts(acc_nodes, acc_tris);
 }
 
-// Any hit over the compressed 8-wide nodes (Bvh8QNode), after the node-group
+// Traversal of the compressed 8-wide nodes (Bvh8QNode), after the node-group
 // traversal of Ylitie, Karras and Laine (HPG 2017).  A ray of octant oct (bit a
 // set where rd_a < 0) takes a node's entered children in the visit order
 // k ^ oct of their slots, which the builder's octant slots make close to
@@ -598,13 +598,15 @@ __device__ __forceinline__ void trace_persistent(const Src& src, const SceneDev&
 // inner children, whose pending ones stay on the stack as one group entry
 // (first inner child, inner mask | pending visit mask << 8), so a node step
 // pushes at most one entry and a pop reads the next child from the entry
-// alone: no sort, no per-child push.  Persistent lanes, chunked ray claims and
-// the LDS stack with global spill as in trace_persistent; single-scene
-// launches keep the first nodes (the top levels: breadth-first numbering) in
-// LDS.
-template <int BLOCK, int LDS, bool COUNT, bool MULTI, class Src>
-__device__ __forceinline__ void trace_any8(const Src& src, const SceneDev& sc, const SceneDev* scenes,
-                                           const TraceConfig& cfg) {
+// alone: no sort, no per-child push.  ANY: the first hit ends the ray;
+// otherwise the closest hit, the (t, prim) minimum, whatever the order: every
+// box is tested against the current closest t, inclusively, so a tie at that
+// t is still reached.  Persistent lanes, chunked ray claims and the LDS stack
+// with global spill as in trace_persistent; single-scene launches keep the
+// first nodes (the top levels: breadth-first numbering) in LDS.
+template <int BLOCK, int LDS, bool ANY, bool COUNT, bool MULTI, class Src>
+__device__ __forceinline__ void trace_wide8(const Src& src, const SceneDev& sc, const SceneDev* scenes,
+                                            const TraceConfig& cfg) {
     using Node = Bvh8QNode;
     __shared__ uint2 stack[LDS][BLOCK];
     constexpr uint32_t kTopCap =
@@ -631,6 +633,8 @@ __device__ __forceinline__ void trace_any8(const Src& src, const SceneDev& sc, c
     bool active = false;      // lane holds a ray
     bool fin = false;         // its traversal has ended
     bool occluded = false;
+    float best_t = 0.0f, best_u = 0.0f, best_v = 0.0f;      // closest hit so far (not ANY)
+    uint32_t best_prim = kInvalid, best_queue = 0;
     const uint32_t wave_id = gtid >> 6;
     const uint32_t static_end = waves * chunk;
     uint32_t chunk_next = min(wave_id * chunk, n), chunk_end = min(wave_id * chunk + chunk, n);
@@ -689,6 +693,10 @@ __device__ __forceinline__ void trace_any8(const Src& src, const SceneDev& sc, c
                         setup_ray(r, o, d, tmin, tmax);
                         oct = (r.rd.x >= 0.0f ? 0u : 1u) | (r.rd.y >= 0.0f ? 0u : 2u) | (r.rd.z >= 0.0f ? 0u : 4u);
                         occluded = false;
+                        best_t = tmax;
+                        best_u = best_v = 0.0f;
+                        best_prim = kInvalid;
+                        best_queue = 0;
                         nodes = tris = 0;
                         node = 0u;
                         lmask = 0u;
@@ -743,16 +751,33 @@ __device__ __forceinline__ void trace_any8(const Src& src, const SceneDev& sc, c
                 const uint32_t prev = slot == 0u ? 0u : slot == 4u ? (lend0 >> 24) : (ends >> (8u * ((slot - 1u) & 3u))) & 0xFFu;
                 const uint32_t first = lbase + prev, cnt = end - prev;
                 if (COUNT) tris += cnt;
+                // the closest hit keeps the lower prim at equal t (Intersect.cu tie rule)
+                auto closer = [&](const float4& a, const float4& b, float t, float u, float v) {
+                    const uint32_t prim = __float_as_uint(a.w);
+                    if (t < r.tmax || (t == r.tmax && best_prim != kInvalid && prim < best_prim)) {
+                        r.tmax = t;
+                        best_t = t;
+                        best_u = u;
+                        best_v = v;
+                        best_prim = prim;
+                        best_queue = __float_as_uint(b.w);
+                    }
+                };
                 for (uint32_t k = 0; k < cnt; k += 2) {
                     float4 a0, b0, c0, a1, b1, c1;
                     load_tri(tris_base + first + k, a0, b0, c0);
                     const bool two = k + 1 < cnt;
                     if (two) load_tri(tris_base + first + k + 1, a1, b1, c1);
                     float t, u, v;
-                    if (intersect_triangle(a0, b0, c0, r.o, r.d, r.tmin, t, u, v) && t < r.tmax) occluded = true;
-                    if (two && intersect_triangle(a1, b1, c1, r.o, r.d, r.tmin, t, u, v) && t < r.tmax) occluded = true;
+                    if (ANY) {
+                        if (intersect_triangle(a0, b0, c0, r.o, r.d, r.tmin, t, u, v) && t < r.tmax) occluded = true;
+                        if (two && intersect_triangle(a1, b1, c1, r.o, r.d, r.tmin, t, u, v) && t < r.tmax) occluded = true;
+                    } else {
+                        if (intersect_triangle(a0, b0, c0, r.o, r.d, r.tmin, t, u, v)) closer(a0, b0, t, u, v);
+                        if (two && intersect_triangle(a1, b1, c1, r.o, r.d, r.tmin, t, u, v)) closer(a1, b1, t, u, v);
+                    }
                 }
-                if (occluded) fin = true;
+                if (ANY && occluded) fin = true;
             }
         }
 
@@ -772,7 +797,8 @@ __device__ __forceinline__ void trace_any8(const Src& src, const SceneDev& sc, c
 
         // ---- finished rays write their result
         if (active && fin) {
-            src.finish_any(item, occluded, nodes, tris);
+            if (ANY) src.finish_any(item, occluded, nodes, tris);
+            else src.finish_closest(item, best_prim, best_t, best_u, best_v, best_queue, nodes, tris);
             acc_nodes += nodes;
             acc_tris += tris;
             active = false;
@@ -909,8 +935,7 @@ template <int BLOCK, int LDS, int W, bool ANY, bool COUNT, bool MULTI, class Src
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(COUNT ? 1 : PT_TRACE_WAVES)))
 void k_trace(Src src, SceneDev sc, const SceneDev* scenes, TraceConfig cfg) {
     if constexpr (W == kNodeBvh8Q) {
-        static_assert(ANY, "the 8-wide compressed nodes serve the any-hit traversal only");
-        trace_any8<BLOCK, LDS, COUNT, MULTI, Src>(src, sc, scenes, cfg);
+        trace_wide8<BLOCK, LDS, ANY, COUNT, MULTI, Src>(src, sc, scenes, cfg);
     } else {
         trace_persistent<BLOCK, LDS, W, ANY, COUNT, MULTI, Src>(src, sc, scenes, cfg);
     }
@@ -1896,9 +1921,8 @@ hipError_t dispatch_block(hipStream_t s, const Src& src, const SceneDev& sc, con
 template <bool ANY, bool COUNT, bool MULTI, class Src>
 hipError_t dispatch_trace(hipStream_t s, const Src& src, const SceneDev& sc, const SceneDev* scenes,
                           const TraceConfig& cfg) {
-    if constexpr (ANY) {
-        if (sc.node_width == 4 && sc.qnodes8) return dispatch_block<kNodeBvh8Q, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
-    }
+    if (sc.node_width == 4 && sc.qnodes8 && (ANY || (sc.node_quantized & 8u)))
+        return dispatch_block<kNodeBvh8Q, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
     if (sc.node_width == 4 && (sc.node_quantized & (ANY ? 2u : 1u)))
         return dispatch_block<kNodeBvh4Q, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
     if (sc.node_width == 4) return dispatch_block<kNodeBvh4, ANY, COUNT, MULTI>(s, src, sc, scenes, cfg);
@@ -1930,11 +1954,10 @@ int occ_kind(const TraceConfig& cfg, bool any, bool count) {
     return count ? occ<NODE, false, true, MULTI, WavefrontClosestSrc>(cfg) : occ<NODE, false, false, MULTI, WavefrontClosestSrc>(cfg);
 }
 
-// the 8-wide quantized nodes serve the any-hit traversal only
+// the 8-wide quantized nodes: any hit, and closest hit where bvh_quantized has bit 3
 template <bool MULTI>
-int occ_any8(const TraceConfig& cfg, bool count) {
-    return count ? occ<kNodeBvh8Q, true, true, MULTI, WavefrontShadowSrc>(cfg)
-                 : occ<kNodeBvh8Q, true, false, MULTI, WavefrontShadowSrc>(cfg);
+int occ_wide8(const TraceConfig& cfg, bool any, bool count) {
+    return occ_kind<kNodeBvh8Q, MULTI>(cfg, any, count);
 }
 
 }  // namespace
@@ -1945,7 +1968,7 @@ bool trace_config_supported(uint32_t block, uint32_t lds_stack) {
 }
 
 int trace_occupancy(const TraceConfig& cfg, bool any, bool count, int node_kind, bool multi) {
-    if (node_kind == 2 && any) return multi ? occ_any8<true>(cfg, count) : occ_any8<false>(cfg, count);
+    if (node_kind == 2) return multi ? occ_wide8<true>(cfg, any, count) : occ_wide8<false>(cfg, any, count);
     if (node_kind >= 1) return multi ? occ_kind<kNodeBvh4Q, true>(cfg, any, count) : occ_kind<kNodeBvh4Q, false>(cfg, any, count);
     return multi ? occ_kind<kNodeBvh4, true>(cfg, any, count) : occ_kind<kNodeBvh4, false>(cfg, any, count);
 }

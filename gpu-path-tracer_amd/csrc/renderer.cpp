@@ -54,8 +54,9 @@ Renderer::Renderer() {
     tuning_.trace_refill = 16;
     tuning_.bvh_width = 4;
     tuning_.bvh_builder = PT_BVH_BUILDER_HOST;
-    tuning_.bvh_quantized = 6;          // shadow rays on 8-wide quantized nodes (64-byte 4-wide ones
-                                        // where the tree is built on the device)
+    tuning_.bvh_quantized = 14;         // extension and shadow rays on the compressed 8-wide tree (where the
+                                        // tree is built on the device: shadow rays on 64-byte 4-wide quantized
+                                        // nodes, extension rays on the fp32 ones)
 }
 
 Renderer::~Renderer() {
@@ -293,7 +294,7 @@ pt_status Renderer::configure_trace() {
             for (const auto& sc : scenes_) {
                 if (!sc) continue;
                 any_scene = true;
-                const int kind = (a && sc->depth8) ? 2 : (sc->quantized & (a ? 2u : 1u)) ? 1 : 0;
+                const int kind = (sc->depth8 && (a || (sc->quantized & 8u))) ? 2 : (sc->quantized & (a ? 2u : 1u)) ? 1 : 0;
                 occ[m][a] = std::min(occ[m][a], trace_occupancy(cfg, a != 0, false, kind, m != 0));
             }
             if (!any_scene) occ[m][a] = trace_occupancy(cfg, a != 0, false, 0, m != 0);
@@ -1332,7 +1333,9 @@ pt_status Renderer::SetTuning(const pt_tuning& t) {
         return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_builder must be PT_BVH_BUILDER_HOST or PT_BVH_BUILDER_GPU");
     if (t.bvh_builder == PT_BVH_BUILDER_GPU && t.bvh_width != 4)
         return fail(PT_ERROR_INVALID_ARGUMENT, "the GPU BVH builder emits 4-wide nodes");
-    if (t.bvh_quantized > 7) return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_quantized must be in [0, 7]");
+    if (t.bvh_quantized > 15) return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_quantized must be in [0, 15]");
+    if ((t.bvh_quantized & 8u) && !(t.bvh_quantized & 4u))
+        return fail(PT_ERROR_INVALID_ARGUMENT, "bvh_quantized bit 3 (closest hit on the 8-wide tree) needs bit 2");
     if (t.shade_block != 256) return fail(PT_ERROR_INVALID_ARGUMENT, "shade_block must be 256");
     if (t.trace_blocks_per_cu > 64 || t.shadow_blocks_per_cu > 64)
         return fail(PT_ERROR_INVALID_ARGUMENT, "traversal blocks per CU must be at most 64");

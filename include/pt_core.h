@@ -232,8 +232,9 @@ typedef struct pt_tuning {
     uint32_t bvh_quantized;      /* traversals on 64-byte nodes whose child boxes are 8-bit offsets
                                     (conservative, results unchanged): bit 0 closest hit, bit 1 any
                                     hit; bit 2: any hit on 128-byte 8-wide quantized nodes instead,
-                                    a collapse of the host builder's tree; 4-wide trees only; next
-                                    upload; default 6 */
+                                    a collapse of the host builder's tree; bit 3 (with bit 2):
+                                    closest hit on them as well; 4-wide trees only; next upload;
+                                    default 14 */
     uint32_t shadow_blocks_per_cu;/* persistent grid of the any-hit traversal, blocks per CU; 0 =
                                     trace_blocks_per_cu if that is set, else as many as fit, less one
                                     wave per SIMD where the launch overlaps a closest-hit launch: in

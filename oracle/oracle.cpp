@@ -967,21 +967,27 @@ bool bvh4q_any(const Scene& s, const std::vector<uint32_t>& by_prim, const uint3
     }
 }
 
-// Any hit over the core's 8-wide compressed quantized nodes, in the kernel's
+// Walks over the core's 8-wide compressed quantized nodes, in the kernel's
 // order: a ray of octant oct (bit a set where its reciprocal direction
 // component is negative) takes a node's entered children in visit order
 // k ^ oct of their slots -- first every leaf child, then the inner children,
 // the pending ones kept as one stack entry per node -- counting node steps and
-// triangle tests.  32 words per node: origin x y z | scale exponents, inner
-// mask in byte 3 | lo rows x0 x1 y0 y1 z0 z1 | hi rows | first inner child |
-// first triangle | triangle ends of the slots (bytes) x2 | pad.  `order`
-// maps the tree's triangle positions to prims.
-bool bvh8q_any(const Scene& s, const std::vector<uint32_t>& by_prim, const uint32_t* qnodes, const uint32_t* order,
-               const Ray& ray, Bvh4Walk& w) {
+// triangle tests.  ANY: the first hit ends the walk; otherwise the closest hit
+// with the (t, prim) tie rule, every box tested against the current closest t.
+// 32 words per node: origin x y z | scale exponents, inner mask in byte 3 | lo
+// rows x0 x1 y0 y1 z0 z1 | hi rows | first inner child | first triangle |
+// triangle ends of the slots (bytes) x2 | pad.  `order` maps the tree's
+// triangle positions to prims.
+template <bool ANY>
+HitRec bvh8q_walk(const Scene& s, const std::vector<uint32_t>& by_prim, const uint32_t* qnodes, const uint32_t* order,
+                  const Ray& ray, Bvh4Walk& w) {
     const F3 rd = f3(safe_rcp(ray.d.x), safe_rcp(ray.d.y), safe_rcp(ray.d.z));
     const F3 ord = ray.o * rd;
     const bool px = rd.x >= 0.0f, py = rd.y >= 0.0f, pz = rd.z >= 0.0f;
     const uint32_t oct = (px ? 0u : 1u) | (py ? 0u : 2u) | (pz ? 0u : 4u);
+    float tmax = ray.tmax;
+    HitRec best{kInvalid, 0, 0, 0};
+    uint32_t best_prim = kInvalid;
     struct Group {
         uint32_t base, imask, pending;
     };
@@ -1015,7 +1021,7 @@ bool bvh8q_any(const Scene& s, const std::vector<uint32_t>& by_prim, const uint3
             const float tny = std::fma(qbyte(ny, b), ay, by), tfy = std::fma(qbyte(fy, b), ay, by);
             const float tnz = std::fma(qbyte(nz, b), az, bz), tfz = std::fma(qbyte(fz, b), az, bz);
             const float tn = std::fmax(std::fmax(tnx, tny), std::fmax(tnz, ray.tmin));
-            const float tf = std::fmin(std::fmin(tfx, tfy), std::fmin(tfz, ray.tmax));
+            const float tf = std::fmin(std::fmin(tfx, tfy), std::fmin(tfz, tmax));
             if (!(tn <= tf)) continue;
             const uint32_t pos = (uint32_t)k ^ oct;
             if (imask >> k & 1u) inner_v |= 1u << pos;
@@ -1034,22 +1040,38 @@ bool bvh8q_any(const Scene& s, const std::vector<uint32_t>& by_prim, const uint3
             w.tris += cnt;
             bool hit = false;
             for (uint32_t k = 0; k < cnt; k++) {
+                const uint32_t prim = order[tri_base + first + k];
+                const uint32_t ti = by_prim[prim];
                 float t, u, v;
-                if (mt(s.tris[by_prim[order[tri_base + first + k]]], ray.o, ray.d, ray.tmin, t, u, v) && t < ray.tmax)
-                    hit = true;
+                if (!mt(s.tris[ti], ray.o, ray.d, ray.tmin, t, u, v)) continue;
+                if (ANY) {
+                    if (t < tmax) {
+                        hit = true;
+                        best = HitRec{ti, t, u, v};
+                    }
+                } else if (t < tmax || (t == tmax && best_prim != kInvalid && prim < best_prim)) {
+                    tmax = t;
+                    best = HitRec{ti, t, u, v};
+                    best_prim = prim;
+                }
             }
-            if (hit) return true;
+            if (ANY && hit) return best;
         }
         if (next != kInvalid) {
             node = next;
             continue;
         }
-        if (stack.empty()) return false;
+        if (stack.empty()) return best;
         Group& g = stack.back();
         node = child(g.base, g.imask, g.pending);
         g.pending &= g.pending - 1u;
         if (!g.pending) stack.pop_back();
     }
+}
+
+bool bvh8q_any(const Scene& s, const std::vector<uint32_t>& by_prim, const uint32_t* qnodes, const uint32_t* order,
+               const Ray& ray, Bvh4Walk& w) {
+    return bvh8q_walk<true>(s, by_prim, qnodes, order, ray, w).tri != kInvalid;
 }
 
 HitRec bvh4_closest(const Scene& s, const std::vector<uint32_t>& by_prim, const uint32_t* nodes, const uint32_t* order,
@@ -1312,6 +1334,27 @@ void orc_bvh8q_any(void* h, const uint32_t* qnodes, const uint32_t* order, const
         const Ray ray{f3(r[0], r[1], r[2]), f3(r[4], r[5], r[6]), r[3], r[7]};
         Bvh4Walk w;
         occluded[i] = bvh8q_any(s, s.by_prim, qnodes, order, ray, w) ? 1u : 0u;
+        if (nodes_visited) nodes_visited[i] = w.nodes;
+        if (tris_tested) tris_tested[i] = w.tris;
+        if (max_stack) max_stack[i] = w.max_stack;
+    });
+}
+
+void orc_bvh8q_closest(void* h, const uint32_t* qnodes, const uint32_t* order, const float* rays, uint32_t n,
+                       void* hits, uint32_t* nodes_visited, uint32_t* tris_tested, uint32_t* max_stack,
+                       uint32_t threads) {
+    const Scene& s = *static_cast<Scene*>(h);
+    parallel_for(n, threads, [&](uint64_t i) {
+        const float* r = rays + 8 * i;
+        const Ray ray{f3(r[0], r[1], r[2]), f3(r[4], r[5], r[6]), r[3], r[7]};
+        Bvh4Walk w;
+        const HitRec hr = bvh8q_walk<false>(s, s.by_prim, qnodes, order, ray, w);
+        uint32_t* o = static_cast<uint32_t*>(hits) + 4 * i;
+        o[0] = hr.tri == kInvalid ? kInvalid : s.tris[hr.tri].prim;
+        const float z = 0.0f;
+        std::memcpy(o + 1, hr.tri == kInvalid ? &z : &hr.t, 4);
+        std::memcpy(o + 2, hr.tri == kInvalid ? &z : &hr.u, 4);
+        std::memcpy(o + 3, hr.tri == kInvalid ? &z : &hr.v, 4);
         if (nodes_visited) nodes_visited[i] = w.nodes;
         if (tris_tested) tris_tested[i] = w.tris;
         if (max_stack) max_stack[i] = w.max_stack;

@@ -31,6 +31,7 @@ def lib():
             "orc_bvh4_closest": (None, [_P, _P, _P, _P, _U32, _P, _P, _P, _P, _U32]),
             "orc_bvh4q_any": (None, [_P, _P, _P, _P, _U32, _P, _P, _P, _U32]),
             "orc_bvh8q_any": (None, [_P, _P, _P, _P, _U32, _P, _P, _P, _P, _U32]),
+            "orc_bvh8q_closest": (None, [_P, _P, _P, _P, _U32, _P, _P, _P, _P, _U32]),
             "orc_alias_build": (None, [_P, _U32, _P]),
             "orc_alias_sample": (None, [_P, _U32, _P, _U32, _P, _P]),
             "orc_scene_alias": (None, [_P, _P]),
@@ -124,6 +125,20 @@ class OracleScene:
         nv, nt, ms = (np.zeros(n, np.uint32) for _ in range(3))
         lib().orc_bvh4_closest(self.h, _p(nodes), _p(order), _p(rays), n, _p(hits), _p(nv), _p(nt), _p(ms),
                                threads or default_threads())
+        return hits, nv, nt, ms
+
+    def bvh8q_closest(self, qnodes, order, rays, threads=None):
+        """Closest hit over the core's 8-wide compressed nodes in the GPU
+        kernel's order: hits (prim, t, u, v bits) and per-ray node / triangle
+        visit counts and peak stack depth."""
+        qnodes = np.ascontiguousarray(qnodes, np.uint32)
+        order = np.ascontiguousarray(order, np.uint32)
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = len(rays)
+        hits = np.zeros((n, 4), np.uint32)
+        nv, nt, ms = (np.zeros(n, np.uint32) for _ in range(3))
+        lib().orc_bvh8q_closest(self.h, _p(qnodes), _p(order), _p(rays), n, _p(hits), _p(nv), _p(nt), _p(ms),
+                                threads or default_threads())
         return hits, nv, nt, ms
 
     def bvh4q_any(self, qnodes, order, rays, threads=None):

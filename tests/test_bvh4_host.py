@@ -276,6 +276,14 @@ def test_bvh8_quantized_structure_and_walk(make, oracle_lib):
         assert ms.max() <= depth8 and nv.min() >= 1
         _, nv4, _ = o.bvh4q_any(q4, order, r)
         assert nv.mean() < nv4.mean()
+    # closest hit over the same nodes (pt_tuning.bvh_quantized bit 3): the (t, prim)
+    # minimum of brute force bit for bit, whatever the visit order
+    full = rays.copy()
+    full[:, 7] = 1e34
+    for r in (rays, full):
+        hits, nv, _, ms = o.bvh8q_closest(q8, order8, r)
+        assert np.array_equal(hits, o.closest(r, brute=True))
+        assert ms.max() <= depth8 and nv.min() >= 1
 
 
 def test_bvh8_quantized_tiny_scenes():

@@ -189,20 +189,22 @@ def test_unsupported_material_is_rejected(gpu_renderer, cornell):
     gpu_renderer.InitializeSceneBuffers(cornell)   # restore
 
 
-def test_stack_overflow_is_reported(gpu_renderer, cornell, cornell_oracle, oracle_lib):
+def test_stack_overflow_is_reported(cornell, cornell_oracle, oracle_lib):
     """A traversal stack capped below what the BVH needs (pt_tuning.stack_limit)
     drops subtrees; every call that traverses then fails with "traversal stack
     overflow" instead of returning wrong hits, and the default cap serves again.
-    The any-hit walk of the 8-wide tree keeps at most one stack entry per level
-    (a node's pending inner children), and rarely more than one in all: its
-    overflow is shown on twelve spheres (245,762 triangles), where a few of
-    16,384 rays need up to four entries in the oracle's walk."""
-    r = gpu_renderer
-    r.InitializeSceneBuffers(cornell)
+    The traversals of the fp32 BVH4 (the room, closest hit on 4-wide nodes) need
+    more than one entry everywhere; those of the compressed 8-wide tree keep at
+    most one entry per level (a node's pending inner children) and rarely more
+    than one in all: theirs is shown on twelve spheres (245,762 triangles),
+    where a few of 16,384 rays need up to four entries in the oracle's walks."""
     rays = scenes.random_rays(1 << 16, (-1, 0, -1), (1, 2, 1), 301)
-    tun = r.get_tuning()
     st = dict(width=32, height=32, samples_per_pixel=4, path_depth=8, use_nee=1, seed=2)
-    try:
+    with ptcore.Renderer() as r:
+        r.set_tuning(bvh_quantized=6)         # closest hit on the fp32 BVH4
+        r.InitializeRenderingBuffers(32, 32)
+        r.InitializeSceneBuffers(cornell)
+        tun = r.get_tuning()
         r.set_tuning(stack_limit=1)
         for call in (lambda: r.trace_closest(rays), lambda: r.classify(rays),
                      lambda: r.RenderFrame(st, cornell.view),
@@ -210,10 +212,9 @@ def test_stack_overflow_is_reported(gpu_renderer, cornell, cornell_oracle, oracl
             with pytest.raises(ptcore.PtError) as e:
                 call()
             assert e.value.status == 4 and "traversal stack overflow" in str(e.value)
-    finally:
         r.set_tuning(**tun)
-    assert np.array_equal(r.trace_closest(rays), cornell_oracle.closest(rays, tprim=True))
-    r.RenderFrame(st, cornell.view)
+        assert np.array_equal(r.trace_closest(rays), cornell_oracle.closest(rays, tprim=True))
+        r.RenderFrame(st, cornell.view)
 
     deep = scenes.spheres_scene(count=12, subdivisions=5, lights=64)
     with ptcore.Renderer() as r2:
@@ -222,15 +223,18 @@ def test_stack_overflow_is_reported(gpu_renderer, cornell, cornell_oracle, oracl
         segs = scenes.random_rays(1 << 14, lo - 1, hi + 1, 302)
         q8, order8 = r2.bvh8_quantized()
         o = oracle_lib.OracleScene(deep)
-        want, _, _, ms = o.bvh8q_any(q8, order8, segs)
-        assert ms.max() >= 2
+        occ_want, _, _, ms_any = o.bvh8q_any(q8, order8, segs)
+        hits_want, _, _, ms_closest = o.bvh8q_closest(q8, order8, segs)
+        assert ms_any.max() >= 2 and ms_closest.max() >= 2
         tun2 = r2.get_tuning()
         r2.set_tuning(stack_limit=1)
-        with pytest.raises(ptcore.PtError) as e:
-            r2.trace_any(segs)
-        assert e.value.status == 4 and "traversal stack overflow" in str(e.value)
+        for call in (lambda: r2.trace_any(segs), lambda: r2.trace_closest(segs)):
+            with pytest.raises(ptcore.PtError) as e:
+                call()
+            assert e.value.status == 4 and "traversal stack overflow" in str(e.value)
         r2.set_tuning(**tun2)
-        assert np.array_equal(r2.trace_any(segs), want)
+        assert np.array_equal(r2.trace_any(segs), occ_want)
+        assert np.array_equal(r2.trace_closest(segs), hits_want)
 
 
 @pytest.mark.parametrize("make", [scenes.cornell_box, scenes.specular_room])
@@ -339,6 +343,16 @@ def test_c2_any_hit_matches_oracle(c2_pair):
     assert np.array_equal(r.trace_any(rays), o.any(rays))
 
 
+def _closest_hit_walk(r, o, rays):
+    """The oracle's walk of the closest-hit tree a renderer traverses: the 8-wide
+    compressed nodes where pt_tuning.bvh_quantized has bit 3 (the default) and
+    they were built, else the 128-byte BVH4 nodes."""
+    if r.get_tuning()["bvh_quantized"] & 8 and r.bvh_info()["bvh8_node_count"]:
+        return o.bvh8q_closest(*r.bvh8_quantized(), rays)
+    nodes, order = r.bvh4()
+    return o.bvh4_closest(nodes, order, rays)
+
+
 def _any_hit_walk(r, o, rays):
     """The oracle's walk of the any-hit tree a renderer traverses: the 8-wide
     quantized nodes where they were built, else the 4-wide quantized ones."""
@@ -355,10 +369,9 @@ def test_c2_visit_counts_equal_the_oracle_walk(c2_pair):
     per ray, the kernels' COUNT counters equal the oracle's walk of the same
     uploaded trees in the same order, for any hit over the default quantized
     nodes (262,144 shadow segments to the emitters) and closest hit over the
-    128-byte nodes (262,144 random rays); the any-hit results equal brute force
+    default closest-hit tree (262,144 random rays); the results equal brute force
     on a subset."""
     s, r, o = c2_pair
-    nodes, order = r.bvh4()
     sh = scenes.shadow_rays(s, 1 << 18, 205)
     occ, nv, nt = r.trace_any_counts(sh)
     oocc, onv, ont = _any_hit_walk(r, o, sh)
@@ -366,10 +379,35 @@ def test_c2_visit_counts_equal_the_oracle_walk(c2_pair):
     assert np.array_equal(occ[:512], o.any(sh[:512], brute=True))
     rays = scenes.random_rays(1 << 18, (-10, -10, -10), (110, 110, 110), 206)
     hits, cnv, cnt = r.trace_closest_counts(rays)
-    ohits, ocnv, ocnt, _ = o.bvh4_closest(nodes, order, rays)
+    ohits, ocnv, ocnt, _ = _closest_hit_walk(r, o, rays)
     assert np.array_equal(hits, ohits) and np.array_equal(cnv, ocnv) and np.array_equal(cnt, ocnt)
+    assert np.array_equal(hits[:512], o.closest(rays[:512], brute=True))
     print(f"any hit: {nv.mean():.3f} nodes, {nt.mean():.3f} triangles per shadow segment; "
           f"closest hit: {cnv.mean():.3f} nodes, {cnt.mean():.3f} triangles per random ray")
+
+
+def test_c2_closest_hit_on_the_8wide_tree_equals_the_oracle_walk(c2_pair):
+    """Closest hit over the compressed 8-wide nodes (pt_tuning.bvh_quantized bit
+    3, the default): per ray, (prim, t, u, v) and node / triangle visits equal
+    the oracle's walk of the uploaded tree on 262,144 random rays (the walk itself
+    equals brute force), and the hits equal the fp32 BVH4 kernel's on 2^20 rays."""
+    s, _, o = c2_pair
+    with ptcore.Renderer() as r, ptcore.Renderer() as r4:
+        r4.set_tuning(bvh_quantized=6)
+        r4.InitializeRenderingBuffers(64, 64)
+        r4.InitializeSceneBuffers(s)
+        r.set_tuning(bvh_quantized=14)
+        r.InitializeRenderingBuffers(64, 64)
+        r.InitializeSceneBuffers(s)
+        q8, order8 = r.bvh8_quantized()
+        rays = scenes.random_rays(1 << 18, (-10, -10, -10), (110, 110, 110), 207)
+        hits, nv, nt = r.trace_closest_counts(rays)
+        ohits, onv, ont, ms = o.bvh8q_closest(q8, order8, rays)
+        assert np.array_equal(hits, ohits) and np.array_equal(nv, onv) and np.array_equal(nt, ont)
+        assert np.array_equal(ohits[:512], o.closest(rays[:512], brute=True))
+        big = scenes.random_rays(1 << 20, (-10, -10, -10), (110, 110, 110), 208)
+        assert np.array_equal(r.trace_closest(big), r4.trace_closest(big))
+        print(f"closest hit on the 8-wide tree: {nv.mean():.3f} nodes, {nt.mean():.3f} triangles per random ray")
 
 
 @pytest.mark.parametrize("quantized", [2, 6])

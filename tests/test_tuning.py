@@ -13,10 +13,10 @@ import scenes
 pytestmark = pytest.mark.gpu
 
 # (node width, block, bvh_quantized: bit 0 closest hit, bit 1 any hit on 64-byte quantized nodes,
-#  bit 2 any hit on 128-byte 8-wide quantized nodes)
+#  bit 2 any hit on 128-byte 8-wide quantized nodes, bit 3 closest hit on them as well)
 POINTS = [(4, 256, 0), (4, 512, 0), (2, 64, 2), (2, 256, 0), (2, 512, 2), (4, 64, 0), (4, 128, 2),
           (4, 256, 3), (4, 64, 1), (4, 512, 3), (4, 256, 2), (4, 256, 6), (4, 512, 7), (4, 64, 4),
-          (2, 256, 6)]
+          (2, 256, 6), (4, 256, 14), (4, 128, 12), (4, 64, 15)]
 
 
 @pytest.fixture(scope="module")
@@ -124,7 +124,7 @@ def test_width2_production_scene_hits(oracle_lib):
 
 
 def test_invalid_tuning_is_rejected(gpu_renderer):
-    for bad in (dict(bvh_width=3), dict(trace_block=1024), dict(trace_block=96), dict(bvh_quantized=8),
+    for bad in (dict(bvh_width=3), dict(trace_block=1024), dict(trace_block=96), dict(bvh_quantized=8), dict(bvh_quantized=16),
                 dict(trace_blocks_per_cu=65), dict(shadow_blocks_per_cu=65)):
         with pytest.raises(ptcore.PtError) as e:
             gpu_renderer.set_tuning(**bad)
