@@ -270,10 +270,13 @@ pt_status Renderer::publish_frame() {
 
 namespace {
 #ifndef PT_CLOSEST_CHUNK
-#define PT_CLOSEST_CHUNK 512
+#define PT_CLOSEST_CHUNK 1024
+#endif
+#ifndef PT_ANY_CHUNK
+#define PT_ANY_CHUNK 256
 #endif
 constexpr uint32_t kClosestHitChunk = PT_CLOSEST_CHUNK;
-constexpr uint32_t kAnyHitChunk = 256;
+constexpr uint32_t kAnyHitChunk = PT_ANY_CHUNK;
 }  // namespace
 
 pt_status Renderer::configure_trace() {
