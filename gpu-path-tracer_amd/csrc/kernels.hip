@@ -763,7 +763,8 @@ __device__ __forceinline__ void trace_wide8(const Src& src, const SceneDev& sc, 
                         best_queue = __float_as_uint(b.w);
                     }
                 };
-                for (uint32_t k = 0; k < cnt; k += 2) {
+                // ANY: an occluded ray stops testing the leaf (its count stays the leaf's)
+                for (uint32_t k = 0; k < cnt && !(ANY && occluded); k += 2) {
                     float4 a0, b0, c0, a1, b1, c1;
                     load_tri(tris_base + first + k, a0, b0, c0);
                     const bool two = k + 1 < cnt;
