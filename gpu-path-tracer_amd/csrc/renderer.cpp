@@ -272,10 +272,14 @@ namespace {
 #ifndef PT_CLOSEST_CHUNK
 #define PT_CLOSEST_CHUNK 1024
 #endif
+#ifndef PT_CLOSEST_CHUNK_MULTI
+#define PT_CLOSEST_CHUNK_MULTI 512
+#endif
 #ifndef PT_ANY_CHUNK
 #define PT_ANY_CHUNK 256
 #endif
 constexpr uint32_t kClosestHitChunk = PT_CLOSEST_CHUNK;
+constexpr uint32_t kClosestHitChunkMulti = PT_CLOSEST_CHUNK_MULTI;
 constexpr uint32_t kAnyHitChunk = PT_ANY_CHUNK;
 }  // namespace
 
@@ -351,12 +355,15 @@ pt_status Renderer::configure_trace() {
             out[m][a]->blocks = blocks[m][a];
             out[m][a]->spill = d_spill_.as<uint2>() + (a ? spill_closest : 0);
             // largest ray chunk a wave claims with one atomic on the launch's
-            // work counter, a single address all eight XCDs contend for: the
-            // closest-hit launch (16.7M rays of the production frame) spent a
-            // fifth of its time in those claims at 256 rays (12.8 -> 9.9 ms per
-            // frame at 512); the any-hit launch's fewer, longer rays balance
-            // better with smaller chunks
-            out[m][a]->chunk = a ? kAnyHitChunk : kClosestHitChunk;
+            // work counter, a single address all eight XCDs contend for: over
+            // fp32 BVH4 nodes the closest-hit launch (16.7M rays of the
+            // production frame) spent a fifth of its time in those claims at
+            // 256 rays (12.8 -> 9.9 ms per frame at 512); over the compressed
+            // 8-wide tree the production frame takes 27.4 ms at 512 and 26.9 at
+            // 1,024, while the several-scene launches of the serving loop run
+            // 1 % faster at 512; the any-hit launch's fewer, longer rays
+            // balance better with smaller chunks (128 to 512 equal)
+            out[m][a]->chunk = a ? kAnyHitChunk : (m ? kClosestHitChunkMulti : kClosestHitChunk);
         }
     hook_trace_ = trace_;
     hook_trace_.overflow = d_overflow_.as<unsigned int>();
